@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Throughput of the NSW6 microphysics step (mp_driver) on one MI355X.
+
+Workload: 10 regions x 16,900 columns = 169,000 columns x 96 levels, dt = 15 s,
+consecutive steps on a device-resident state (seeded synthetic NICAM columns).
+Prints ONE JSON line with column-steps/s, the roofline of the step measured with
+HIP events on the launch stream, and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+# HBM bytes a step must move at least, per column (96 levels):
+# reads 22 fields + writes 27 fields per level (8 B each) + 15 per-column values.
+ALGO_BYTES_PER_COLUMN = 96 * (22 + 27) * 8 + 15 * 8          # 37,752 B
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", default=None)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=None,
+                    help="JSON file with measured HBM bytes per step (from rocprofv3 --pmc runs)")
+    return ap.parse_args()
+
+
+def cpu_baseline(reps: int):
+    """The CPU oracle (an OpenMP FP64 port of the reference path) on one region of
+    16,900 columns: 1 warm-up, median of `reps` timed steps of the mp_driver call."""
+    import numpy as np  # noqa: F401
+    from nicam_kokkos_microphysics_amd import fields as F
+    from oracle import oracle as O
+    gz, gzh, st = F.synth_state(1, 16900, seed=20240101)
+    P = O.params()
+    O.mp_driver(st, gz, gzh, 15.0, P=P)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.mp_driver(st, gz, gzh, 15.0, P=P)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": 16900 / med, "unit": "column-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 region x 16,900 columns x 96 levels, dt=15 s, median of {reps} steps after 1 warm-up "
+                      f"({med * 1e3:.1f} ms/step)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the step shards by NICAM region with no "
+                                   "communication, and multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+
+    wname = args.workload or ("nsw6_169000x96_mixed" if args.precision == "mixed" else "nsw6_169000x96_fp64")
+    w = F.WORKLOADS[wname]
+    ncol = w.lall * w.ijdim
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    del st
+    prec = N.NSW6_MIXED if args.precision == "mixed" else N.NSW6_FP64
+    drv = NSW6(gz, gzh, max_columns=ncol, precision=prec)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        drv.step(dev, w.dt, stream=stream, check=False)
+    drv.check()
+    torch.cuda.synchronize()
+
+    drv.timing(True)
+    drv.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        drv.step(dev, w.dt, stream=stream, check=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rep = drv.check()
+    step_ms, nev = drv.timing_read(0)
+    pre_ms, _ = drv.timing_read(1)
+    sweep_ms, _ = drv.timing_read(2)
+    drv.timing(False)
+
+    ms_per_step = wall * 1e3 / args.steps
+    algo_bytes = ALGO_BYTES_PER_COLUMN * ncol
+    achieved = algo_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            traffic = json.load(fh).get("bytes_per_step")
+
+    out = {
+        "metric": "column-steps/s at 169k x 96 (FP64 mp_driver step)" if prec == N.NSW6_FP64
+        else "column-steps/s at 169k x 96 (mixed-precision mp_driver step)",
+        "value": ncol * args.steps / wall,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == N.NSW6_FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": wname, "regions": w.lall, "columns_per_region": w.ijdim, "levels": 96,
+                   "columns": ncol, "dt_s": w.dt, "tracers": 6},
+        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_sweep)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": algo_bytes, "event_ms": step_ms, "launches_timed": nev},
+        "kernels_ms": {"step": step_ms, "rates": step_ms - pre_ms - sweep_ms if step_ms else None,
+                       "extremes+bounds": pre_ms, "sweep": sweep_ms},
+        "general_regions": rep.n_general_regions,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
+    drv.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

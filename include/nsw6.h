@@ -1,0 +1,187 @@
+/*
+ * nsw6.h -- C ABI of the MI355X-native NICAM NSW6 single-moment microphysics step.
+ *
+ * This header is the drop-in boundary for NICAM's `mp_driver` call chain
+ * (mp_driver -> MP_NSW6 rates -> PRECIP_transport/vadv1d -> SATURATION_adjustment
+ * -> THRMDYN).  Every array is FP64, in NICAM's (ij,k,l) Fortran layout, i.e. a
+ * row-major C array [l][k][ij] with ij fastest.  Tracer arrays are [l][nq][k][ij]
+ * (NICAM's *_Lswp ordering), per-column arrays [l][ij], two-row precip [l][2][ij]
+ * and one-row diagnostics (rctop, rwtop, tctop) [l][1][ij].
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   nsw6_init            <- MP_DRIVER::mp_init("NSW6") + GRD_Setup + CVW_CPW_Setup +
+ *                           SATURATION_Setup (src/mod_mp_driver.cpp:11-24,
+ *                           src/mod_debug.cpp:76-187, src/mod_satadjust.cpp:24-47)
+ *   nsw6_params_default  <- MP_NSW6::mp_nsw6_init (src/mod_mp_nsw6.cpp:276-434) and the
+ *                           effective-radius constants of mp_nsw6_new::init
+ *                           (src/mod_mp_nsw6.cpp:506-517); struct GlobalParams
+ *                           (include/mod_mp_nsw6.h:16-123)
+ *   nsw6_mp_driver       <- MP_DRIVER::mp_driver (include/mod_mp_driver.h:71-124,
+ *                           src/mod_mp_driver.cpp:335-613), same argument order
+ *   nsw6_mp_driver_fields<- the same call with the 49 arrays gathered in a struct
+ *   nsw6_finalize        <- the end of the reference's process (views released)
+ *
+ * All device pointers handed to nsw6_mp_driver* must be HIP device memory on the
+ * device the context was created on.  Calls are stream-ordered and asynchronous;
+ * errors detected on the device (Newton non-convergence, NaN temperature) are
+ * reported by nsw6_last_status() after the stream has been synchronised.
+ */
+#ifndef NSW6_H
+#define NSW6_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSW6_KALL  96   /* ADM_kall (include/problemsize.h:64)                  */
+#define NSW6_KMIN  1    /* ADM_kmin (include/problemsize.h:66), 0-based          */
+#define NSW6_KMAX  94   /* ADM_kmax (include/problemsize.h:67), 0-based          */
+#define NSW6_NQMAX 6    /* QV,QC,QR,QI,QS,QG (include/problemsize.h:226-231)    */
+
+enum nsw6_status {
+    NSW6_OK             = 0,
+    NSW6_NOT_CONVERGED  = 1,  /* satadjust Newton hit itelim=100 (src/mod_satadjust.cpp:883-891) */
+    NSW6_NAN_DETECTED   = 2,  /* tem*0 != 0 guard fired (src/mod_satadjust.cpp:880)             */
+    NSW6_BAD_ARG        = 3,
+    NSW6_HIP_ERROR      = 4,
+    NSW6_NO_DEVICE      = 5
+};
+
+enum nsw6_precision {
+    NSW6_FP64  = 0,   /* every operation in FP64 (parity form)                            */
+    NSW6_MIXED = 1    /* FP32 process rates / fall speeds / slope+gamma terms, FP64 energy, */
+                      /* Newton inversion and column flux / mass sums                      */
+};
+
+/* Scheme coefficients: GlobalParams after mp_nsw6_init, plus the two literals of the
+ * rate code that are exposed as tunable coefficients (kk_aut_rate, icenuc_coef) and
+ * the effective-radius constants of mp_nsw6_new::init. */
+typedef struct nsw6_params {
+    double N0r, N0s, N0g, rho_s, rho_g, C_d, Cr, Cs;
+    double Ar, As, Ag, Br, Bs, Bg, Cg, Dr, Ds, Dg;
+    double GAM, GAM_2, GAM_3;
+    double GAM_1br, GAM_2br, GAM_3br, GAM_3dr, GAM_6dr, GAM_1brdr, GAM_5dr_h;
+    double GAM_1bs, GAM_2bs, GAM_3bs, GAM_3ds, GAM_1bsds, GAM_5ds_h;
+    double GAM_1bg, GAM_3dg, GAM_1bgdg, GAM_5dg_h;
+    double sw_kk2000, sw_roh2014, ln10;
+    double coef_a[10], coef_b[10];
+    double Eiw, Erw, Esw, Egw, Eri, Esi, Egi, Esr, Egr, Egs;
+    double gamma_sacr, gamma_gacs, mi;
+    double Nc_def;
+    double beta_saut, gamma_saut, beta_gaut, gamma_gaut, qicrt_saut, qscrt_gaut;
+    double f1r, f2r, f1s, f2s, f1g, f2g;
+    double A_frz, B_frz;
+    double mi40, mi50, vti50, Ri50;
+    double sw_expice, Nc_ihtr, Di_max, Di_a;
+    double sw_constVti, CONST_Vti;
+    double kk_aut_rate;   /* 1350.0  in Praut_kk  (src/mod_mp_nsw6.cpp:1154) */
+    double icenuc_coef;   /* 4.92E-11 in Qi0      (src/mod_mp_nsw6.cpp:1083) */
+    double dgamma_a, GAM_dgam, GAM_dgam23, coef_dgam, coef_xf;  /* src/mod_mp_nsw6.cpp:511-516 */
+} nsw6_params;
+
+/* The 8 tunable NSW6 coefficients, in this order, for gradients and perturbations:
+ * 0 kk_aut_rate (KK2000 autoconversion rate), 1 qicrt_saut (autoconversion threshold),
+ * 2 Esi (snow-ice collection efficiency), 3 Esr (snow-rain collection efficiency),
+ * 4 Cr, 5 Cs, 6 Cg (rain/snow/graupel fall-speed prefactors),
+ * 7 icenuc_coef (ice nucleation coefficient). */
+#define NSW6_NCOEF 8
+
+/* Vertical grid (GRD_* of include/problemsize.h:143-163, filled by GRD_Setup). */
+typedef struct nsw6_grid {
+    double gz[NSW6_KALL], gzh[NSW6_KALL];
+    double dgz[NSW6_KALL], dgzh[NSW6_KALL], rdgz[NSW6_KALL], rdgzh[NSW6_KALL];
+    double afact[NSW6_KALL], bfact[NSW6_KALL], cfact[NSW6_KALL], dfact[NSW6_KALL];
+} nsw6_grid;
+
+/* The 49 arrays of mp_driver, in reference order.  RW = read and written in place. */
+typedef struct nsw6_fields {
+    double *rhog, *rhogvx, *rhogvy, *rhogvz, *rhogw, *rhoge;   /* RW [l][k][ij]        */
+    double *rhogq;                                              /* RW [l][nq][k][ij]    */
+    double *vx, *vy, *vz, *w, *unccn;                           /* R  [l][k][ij]        */
+    double *rho, *tem, *pre;                                    /* RW [l][k][ij]        */
+    double *q;                                                  /* RW [l][nq][k][ij]    */
+    double *qd;                                                 /* W  [l][k][ij]        */
+    double *precip, *ISO1_precip, *ISO2_precip;                 /* W  [l][2][ij]        */
+    double *precip_rhoe, *precip_lh_heat, *precip_rhophi, *precip_rhokin; /* W [l][ij] */
+    double *re_liquid, *re_solid, *re_cld;                      /* W  [l][k][ij]        */
+    double *rctop, *rwtop, *tctop;                              /* W  [l][1][ij]        */
+    double *frhoge_af, *frhogqv_af, *frhoge_rad, *rhogqke;      /* unused [l][k][ij]    */
+    double *gsgam2, *gsgam2h, *gam2, *gam2h;                    /* R  [l][k][ij]        */
+    double *ix, *iy, *iz, *jx, *jy, *jz;                        /* R  [l][ij]           */
+    double *z, *zh;                                             /* R  [l][k][ij]        */
+    double *GDCLW, *GDCFRC, *GPREC, *CBMFX;                     /* W (CBMFX unused)     */
+} nsw6_fields;
+
+/* Device-side error report of the last completed step (all regions). */
+typedef struct nsw6_report {
+    int32_t status;          /* enum nsw6_status                                   */
+    int32_t n_not_converged; /* Newton points that hit itelim                      */
+    int32_t n_nan;           /* Newton points that produced a NaN temperature      */
+    int32_t first_l, first_k, first_ij;  /* first failing point (-1 if none)        */
+    int32_t n_general_regions; /* regions that took the exact general sedimentation */
+                               /* path (departure beyond one cell / upward / exit)  */
+} nsw6_report;
+
+typedef struct nsw6_ctx nsw6_ctx;
+
+/* Fill *p with the mp_nsw6_init defaults (Roh & Satoh 2014 + KK2000, 3WATER,
+ * EIN_TYPE=SIMPLE, Nc_def = Nc_ocn = 50). */
+void nsw6_params_default(nsw6_params *p);
+
+/* GRD_Setup (src/mod_debug.cpp:92-187): derive the metric factors from gz and gzh. */
+void nsw6_grid_setup(const double *gz, const double *gzh, nsw6_grid *g);
+
+/* Create a context on the current HIP device.  `grid` holds gz/gzh (the rest is
+ * derived), `params` may be NULL for the defaults.  `max_columns` = the largest
+ * lall*ijdim the context will be called with; workspace is allocated once here. */
+int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_columns,
+              nsw6_ctx **out);
+
+/* Replace the scheme coefficients of an existing context (e.g. perturbed tunables). */
+int nsw6_set_params(nsw6_ctx *ctx, const nsw6_params *params);
+
+/* Choose the arithmetic of subsequent steps (enum nsw6_precision). */
+int nsw6_set_precision(nsw6_ctx *ctx, int precision);
+
+/* One mp_driver step over `lall` regions of `ijdim` columns each.  `fields` holds
+ * device pointers.  `stream` is a hipStream_t (NULL = the default stream). */
+int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
+                          const nsw6_fields *fields, double dt, double ct, void *stream);
+
+/* Same call with the reference's flat argument list (include/mod_mp_driver.h:71-124),
+ * preceded by the context and the batch shape and followed by the stream. */
+int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
+    double *rhog, double *rhogvx, double *rhogvy, double *rhogvz, double *rhogw,
+    double *rhoge, double *rhogq, double *vx, double *vy, double *vz, double *w,
+    double *unccn, double *rho, double *tem, double *pre, double *q, double *qd,
+    double *precip, double *ISO1_precip, double *ISO2_precip, double *precip_rhoe,
+    double *precip_lh_heat, double *precip_rhophi, double *precip_rhokin,
+    double *re_liquid, double *re_solid, double *re_cld, double *rctop, double *rwtop,
+    double *tctop, double *frhoge_af, double *frhogqv_af, double *frhoge_rad,
+    double *rhogqke, double *gsgam2, double *gsgam2h, double *gam2, double *gam2h,
+    double *ix, double *iy, double *iz, double *jx, double *jy, double *jz,
+    double *z, double *zh, double dt, double ct, double *GDCLW, double *GDCFRC,
+    double *GPREC, double *CBMFX, void *stream);
+
+/* Synchronise the context's last stream and return the device report of the last step. */
+int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *report);
+
+/* Average duration (ms) of the named kernel over the launches recorded since the
+ * last reset, measured with HIP events on the stream the kernel ran on.
+ * which: 0 = whole step, 1 = fall-speed/extremes pre-pass, 2 = fused column step.
+ * Recording is off until nsw6_timing_enable(ctx, 1). */
+int nsw6_timing_enable(nsw6_ctx *ctx, int on);
+int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count);
+int nsw6_timing_reset(nsw6_ctx *ctx);
+
+void nsw6_finalize(nsw6_ctx *ctx);
+
+const char *nsw6_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSW6_H */

@@ -1,0 +1,491 @@
+// nsw6_device.h -- device-side physics of the NSW6 step for gfx950.
+//
+// Point-wise pieces of the reference step (paths relative to the reference repo):
+//   negative filter          src/mod_mp_nsw6.cpp:1750-1832
+//   psat_liq / psat_ice      src/mod_satadjust.cpp:80-106, 133-160
+//   Bergeron_param           src/mod_mp_nsw6.cpp:1834-1890
+//   TagBigLoop (49 rates)    src/mod_mp_nsw6.cpp:760-1594
+//   THRMDYN_qd / _cv         src/mod_thrmdyn.cpp:98-135
+//   satadjust_liq Newton     src/mod_satadjust.cpp:828-894
+//
+// The rate code is templated on the arithmetic type R: double for the FP64 form
+// and float for the mixed form.  Operation order follows the reference so that the
+// FP64 form reproduces it to round-off; the deliberate differences are
+//   * integer powers pow(x,n), n = 2..7, are products (<= 2 ulp),
+//   * log(Xs2+zerosw) of the seven Roh & Satoh moments is evaluated once,
+//   * logs/powers of constants (log(mi40), log(mi50), Nc^-1.79, log(Nc/2000)) are
+//     computed once on the host,
+//   * the dead first assignment of Praut_kk (overwritten at :1154) is dropped.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nsw6 {
+
+constexpr int KALL = 96, KMIN = 1, KMAX = 94, NQ = 6;
+constexpr int QV = 0, QC = 1, QR = 2, QI = 3, QS = 4, QG = 5;
+
+constexpr double C_PI = 3.141592653589793;
+constexpr double C_EPS = 2.220446E-16;
+constexpr double C_UNDEF = -9.9999E+30;
+constexpr double C_GRAV = 9.79764;
+constexpr double C_Rdry = 287.04;
+constexpr double C_CVdry = 717.56;
+constexpr double C_Rvap = 461.46;
+constexpr double C_CL = 4218.0;
+constexpr double C_PSAT0 = 610.7;
+constexpr double C_LHV0 = 2501000.0;
+constexpr double C_LHS0 = 2834000.0;
+constexpr double C_LHF0 = 333000.0;
+constexpr double C_Pstd = 101325.0;
+constexpr double C_TEM00 = 273.15;
+constexpr double C_DWATR = 1000.0;
+constexpr double C_CVW = 717.56;      // CVW(:) = CVdry under EIN_TYPE = SIMPLE
+constexpr double TEM_MIN = 10.0;
+constexpr double DENS00 = 1.28;
+constexpr double Ka0 = 2.428E-2, dKa_dT = 7.47E-5;
+constexpr double Kd0 = 2.222E-5, dKd_dT = 1.37E-7;
+constexpr double nu0 = 1.718E-5, dnu_dT = 5.28E-8;
+
+// Scheme coefficients as the kernels see them (nsw6_params + host-side constants).
+struct DevParams {
+    double N0r, N0s, N0g, Ar, As, Ag, Cr, Cs, Cg;
+    double GAM, GAM_2, GAM_3;
+    double GAM_1br, GAM_2br, GAM_3br, GAM_3dr, GAM_6dr, GAM_1brdr, GAM_5dr_h;
+    double GAM_1bs, GAM_2bs, GAM_3bs, GAM_3ds, GAM_1bsds, GAM_5ds_h;
+    double GAM_1bg, GAM_3dg, GAM_1bgdg, GAM_5dg_h;
+    double sw_kk2000, sw_roh2014, ln10;
+    double coef_a[10], coef_b[10];
+    double Eiw, Erw, Esw, Egw, Eri, Esi, Egi, Esr, Egr, Egs;
+    double gamma_sacr, gamma_gacs, mi;
+    double Nc_def, beta_saut, gamma_saut, beta_gaut, gamma_gaut, qicrt_saut, qscrt_gaut;
+    double f1r, f2r, f1s, f2s, f1g, f2g, A_frz, B_frz;
+    double mi40, mi50, vti50, Ri50;
+    double sw_expice, Nc_ihtr, Di_max, Di_a, sw_constVti, CONST_Vti;
+    double kk_aut_rate, icenuc_coef, coef_dgam, coef_xf;
+    // host-computed constants
+    double ln_mi40, ln_mi50;   // log(mi40), log(mi50)
+    double Nc_pow_m179;        // pow(Nc_def, -1.79)
+    double Dc;                 // 0.146 - 5.964e-2 * log(Nc_def / 2000)
+    double LovR_liq, LovR_ice, RTEM00;
+    double dtemp_criteria;     // 10^(-(15+1)/2) = 1e-8
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float dmax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float dmin(float a, float b) { return (b < a) ? b : a; }
+
+// 0.5 + copysign(0.5, x): 1 for x >= +0, 0 for x <= -0 (IEEE sign, as the reference).
+template <typename R> __device__ __forceinline__ R step_pos(R x) { return R(0.5) + copysign(R(0.5), x); }
+template <typename R> __device__ __forceinline__ R step_neg(R x) { return R(0.5) - copysign(R(0.5), x); }
+
+__device__ __forceinline__ double psat_liq(const DevParams &P, double tem)
+{
+    double rtem = 1.0 / dmax(tem, TEM_MIN);
+    // pow(tem*RTEM00, CPovR_liq) with CPovR_liq = 0 under SIMPLE is exactly 1
+    return C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+}
+__device__ __forceinline__ double psat_ice(const DevParams &P, double tem)
+{
+    double rtem = 1.0 / dmax(tem, TEM_MIN);
+    return C_PSAT0 * 1.0 * exp(P.LovR_ice * (P.RTEM00 - rtem));
+}
+
+// Bergeron_param at one point (keeps the reference's off-by-one interpolation).
+__device__ __forceinline__ void bergeron(const double *a1_tab, const double *a2_tab, double tem,
+                                         double &a1, double &a2, double &ma2)
+{
+    double temc = dmin(dmax(tem - C_TEM00, -30.99), 0.0);
+    int itemc = (int)(-temc);
+    double fact = -(temc + (double)(itemc - 1));
+    double v1 = (1.0 - fact) * a1_tab[itemc] + (fact)*a1_tab[itemc + 1];
+    double v2 = (1.0 - fact) * a2_tab[itemc] + (fact)*a2_tab[itemc + 1];
+    ma2 = 1.0 - v2;
+    a2 = v2;
+    a1 = v1 * pow(1.0E-3, ma2);
+}
+
+// Input of one rate evaluation.
+template <typename R> struct RateIn {
+    R dens, temp, q[6], qsatl, qsati, pre, rhog, a1, a2, ma2;
+};
+// Output of one rate evaluation: tendencies * rhog * dt, fall speeds, radii.
+template <typename R> struct RateOut {
+    R drhogq[6];
+    R Vtr, Vti, Vts, Vtg;
+    R rceff, rceff_cld, rctop, tctop;
+};
+
+// TagBigLoop at one point.
+template <typename R>
+__device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
+{
+    const R PI = R(C_PI), EPS = R(C_EPS), UNDEF = R(C_UNDEF);
+    const R Rvap = R(C_Rvap), LHV0 = R(C_LHV0), LHS0 = R(C_LHS0), LHF0 = R(C_LHF0);
+    const R PRE00 = R(C_Pstd), TEM00 = R(C_TEM00), CL = R(C_CL), rho_w = R(C_DWATR);
+    const R Ar = R(P.Ar), As = R(P.As), Ag = R(P.Ag), Cr = R(P.Cr), Cs = R(P.Cs), Cg = R(P.Cg);
+    const R N0r = R(P.N0r), N0s = R(P.N0s), N0g = R(P.N0g);
+    const R GAM = R(P.GAM), GAM_2 = R(P.GAM_2), GAM_3 = R(P.GAM_3);
+    const R dens = in.dens, temp = in.temp;
+
+    const R qv = dmax(in.q[QV], R(0)), qc = dmax(in.q[QC], R(0)), qr = dmax(in.q[QR], R(0));
+    const R qi = dmax(in.q[QI], R(0)), qs = dmax(in.q[QS], R(0)), qg = dmax(in.q[QG], R(0));
+
+    const R Sliq = qv / dmax(in.qsatl, EPS);
+    const R Sice = qv / dmax(in.qsati, EPS);
+    const R Rdens = R(1) / dens;
+    const R rho_fact = sqrt(R(DENS00) * Rdens);
+    const R temc = temp - TEM00;
+
+    const R delta1 = step_pos(qr - R(1.0E-4));
+    const R delta2 = step_pos(R(1.0E-4) - qr) * step_pos(R(1.0E-4) - qs);
+    const R spsati = step_pos(Sice - R(1));
+    const R iceflg = step_neg(temc);
+    const R dqv_dt = qv / dt, dqc_dt = qc / dt, dqr_dt = qr / dt;
+    const R dqi_dt = qi / dt, dqs_dt = qs / dt, dqg_dt = qg / dt;
+    const R sw_expice = R(P.sw_expice);
+    const R sw_bergeron = step_pos(temc + R(30)) * step_pos(R(0) - temc) * (R(1) - sw_expice);
+
+    // slope parameter lambda (rain)
+    R zerosw = step_neg(qr - R(1.0E-12));
+    const R RLMDr = sqrt(sqrt(dens * qr / (Ar * N0r * R(P.GAM_1br)) + zerosw)) * (R(1) - zerosw);
+    const R RLMDr_dr = sqrt(RLMDr);
+    const R RLMDr_2 = RLMDr * RLMDr;
+    const R RLMDr_3 = RLMDr_2 * RLMDr;
+    const R RLMDr_1br = RLMDr_2 * RLMDr_2;
+    const R RLMDr_2br = RLMDr_1br * RLMDr;
+    const R RLMDr_3br = RLMDr_3 * RLMDr_3;
+    const R RLMDr_7 = RLMDr_3br * RLMDr;
+    const R RLMDr_3dr = RLMDr_3 * RLMDr_dr;
+    const R RLMDr_5dr = RLMDr_2br * RLMDr_dr;
+    const R RLMDr_6dr = RLMDr_3br * RLMDr_dr;
+
+    // slope parameter lambda (snow) and its moments
+    zerosw = step_neg(qs - R(1.0E-12));
+    const R RLMDs = sqrt(sqrt(dens * qs / (As * N0s * R(P.GAM_1bs)) + zerosw)) * (R(1) - zerosw);
+    const R RLMDs_ds = sqrt(sqrt(RLMDs));
+    const R RLMDs_2 = RLMDs * RLMDs;
+    const R RLMDs_3 = RLMDs_2 * RLMDs;
+    const R RLMDs_1bs = RLMDs_2 * RLMDs_2;
+    const R RLMDs_2bs = RLMDs_1bs * RLMDs;
+    const R RLMDs_3bs = RLMDs_3 * RLMDs_3;
+    const R RLMDs_3ds = RLMDs_3 * RLMDs_ds;
+    const R RLMDs_5ds = RLMDs_2bs * RLMDs_ds;
+
+    R MOMs_0 = N0s * GAM * RLMDs;
+    R MOMs_1 = N0s * GAM_2 * RLMDs_2;
+    R MOMs_2 = N0s * GAM_3 * RLMDs_3;
+    R MOMs_0bs = N0s * R(P.GAM_1bs) * RLMDs_1bs;
+    R MOMs_1bs = N0s * R(P.GAM_2bs) * RLMDs_2bs;
+    R MOMs_2bs = N0s * R(P.GAM_3bs) * RLMDs_3bs;
+    R MOMs_2ds = N0s * R(P.GAM_3ds) * RLMDs_3ds;
+    R MOMs_5ds_h = N0s * R(P.GAM_5ds_h) * sqrt(RLMDs_5ds);
+    R RMOMs_Vt = R(P.GAM_1bsds) / R(P.GAM_1bs) * RLMDs_ds;
+
+    // Roh and Satoh (2014) bimodal snow moments
+    const R Xs2 = dens * qs / As;
+    zerosw = step_neg(Xs2 - R(1.0E-12));
+    const R tems = dmin(R(-0.1), temc);
+    R at[4], bt[4];
+    at[0] = R(P.coef_a[0]) + tems * (R(P.coef_a[1]) + tems * (R(P.coef_a[4]) + tems * R(P.coef_a[8])));
+    at[1] = R(P.coef_a[2]) + tems * (R(P.coef_a[3]) + tems * R(P.coef_a[6]));
+    at[2] = R(P.coef_a[5]) + tems * R(P.coef_a[7]);
+    at[3] = R(P.coef_a[9]);
+    bt[0] = R(P.coef_b[0]) + tems * (R(P.coef_b[1]) + tems * (R(P.coef_b[4]) + tems * R(P.coef_b[8])));
+    bt[1] = R(P.coef_b[2]) + tems * (R(P.coef_b[3]) + tems * R(P.coef_b[6]));
+    bt[2] = R(P.coef_b[5]) + tems * R(P.coef_b[7]);
+    bt[3] = R(P.coef_b[9]);
+    const R sw_roh = R(P.sw_roh2014), ln10 = R(P.ln10);
+    const R lx = log(Xs2 + zerosw);
+    const R omz = R(1) - zerosw;
+#define NSW6_MOM(NM, OUT)                                                                        \
+    {                                                                                            \
+        const R nm = R(NM);                                                                      \
+        const R loga_ = at[0] + nm * (at[1] + nm * (at[2] + nm * at[3]));                        \
+        const R b_ = bt[0] + nm * (bt[1] + nm * (bt[2] + nm * bt[3]));                           \
+        OUT = exp(ln10 * loga_) * exp(lx * b_);                                                  \
+    }
+    R m;
+    m = exp(ln10 * at[0]) * exp(lx * bt[0]);
+    MOMs_0 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0;
+    NSW6_MOM(1.0, m)
+    MOMs_1 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1;
+    MOMs_2 = sw_roh * Xs2 + (R(1) - sw_roh) * MOMs_2;
+    NSW6_MOM(2.0, m)
+    MOMs_0bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0bs;
+    NSW6_MOM(3.0, m)
+    MOMs_1bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1bs;
+    NSW6_MOM(4.0, m)
+    MOMs_2bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2bs;
+    NSW6_MOM(2.25, m)
+    MOMs_2ds = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2ds;
+    NSW6_MOM(1.625, m)
+    MOMs_5ds_h = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_5ds_h;
+    // RMOMs_Vt uses the same (nm = 2.25) moment as MOMs_2ds
+    {
+        const R nm = R(2.25);
+        const R loga_ = at[0] + nm * (at[1] + nm * (at[2] + nm * at[3]));
+        const R b_ = bt[0] + nm * (bt[1] + nm * (bt[2] + nm * bt[3]));
+        m = exp(ln10 * loga_) * exp(lx * b_);
+    }
+    RMOMs_Vt = sw_roh * m * omz / (MOMs_0bs + zerosw) + (R(1) - sw_roh) * RMOMs_Vt;
+#undef NSW6_MOM
+
+    // slope parameter lambda (graupel)
+    zerosw = step_neg(qg - R(1.0E-12));
+    const R RLMDg = sqrt(sqrt(dens * qg / (Ag * N0g * R(P.GAM_1bg)) + zerosw)) * (R(1) - zerosw);
+    const R RLMDg_dg = sqrt(RLMDg);
+    const R RLMDg_2 = RLMDg * RLMDg;
+    const R RLMDg_3 = RLMDg_2 * RLMDg;
+    const R RLMDg_3dg = RLMDg_3 * RLMDg_dg;
+    const R RLMDg_5dg = (RLMDg_3 * RLMDg_2) * RLMDg_dg;
+
+    // terminal velocity
+    zerosw = step_neg(qi - R(1.0E-8));
+    const R sw_constVti = R(P.sw_constVti);
+    const R Vti = (R(1) - sw_constVti) * R(-3.29) * exp(log(dens * qi + zerosw) * R(0.16)) * (R(1) - zerosw) +
+                  (sw_constVti) * (-R(P.CONST_Vti));
+    const R Vtr = -Cr * rho_fact * R(P.GAM_1brdr) / R(P.GAM_1br) * RLMDr_dr;
+    const R Vts = -Cs * rho_fact * RMOMs_Vt;
+    const R Vtg = -Cg * rho_fact * R(P.GAM_1bgdg) / R(P.GAM_1bg) * RLMDg_dg;
+
+    // nucleation
+    const R Ni0 = dmax(exp(R(-0.1) * temc), R(1)) * R(1000);
+    const R Qi0 = R(P.icenuc_coef) * exp(log(Ni0) * R(1.33)) * Rdens;
+    R Pigen = dmax(dmin(Qi0 - qi, qv - in.qsati), R(0)) / dt;
+
+    // accretion
+    const R Esi = R(P.Esi), Egs = R(P.Egs);
+    const R Esi_mod = dmin(Esi, Esi * exp(R(P.gamma_sacr) * temc));
+    const R Egs_mod = dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
+    const R GAM_3dr = R(P.GAM_3dr), GAM_3dg = R(P.GAM_3dg);
+    const R Pracw_orig = qc * R(0.25) * PI * R(P.Erw) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
+    zerosw = step_neg(qc * qr - R(1.0E-12));
+    const R Pracw_kk = R(67) * exp(log(qc * qr + zerosw) * R(1.15)) * (R(1) - zerosw);
+    const R sw_kk = R(P.sw_kk2000);
+    R Pracw = (R(1) - sw_kk) * Pracw_orig + (sw_kk)*Pracw_kk;
+    R Psacw = qc * R(0.25) * PI * R(P.Esw) * Cs * MOMs_2ds * rho_fact;
+    R Pgacw = qc * R(0.25) * PI * R(P.Egw) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
+    R Praci = qi * R(0.25) * PI * R(P.Eri) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
+    R Psaci = qi * R(0.25) * PI * Esi_mod * Cs * MOMs_2ds * rho_fact;
+    R Pgaci = qi * R(0.25) * PI * R(P.Egi) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
+    R Piacr = qi * Ar / R(P.mi) * R(0.25) * PI * R(P.Eri) * N0r * Cr * R(P.GAM_6dr) * RLMDr_6dr * rho_fact;
+    const R GAM_1br = R(P.GAM_1br), GAM_2br = R(P.GAM_2br), GAM_3br = R(P.GAM_3br);
+    const R Esr = R(P.Esr);
+    R Psacr = Ar * R(0.25) * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
+              (GAM_1br * RLMDr_1br * MOMs_2 + R(2) * GAM_2br * RLMDr_2br * MOMs_1 + GAM_3br * RLMDr_3br * MOMs_0);
+    R Pgacr = Ar * R(0.25) * PI * Rdens * R(P.Egr) * N0g * N0r * fabs(Vtg - Vtr) *
+              (GAM_1br * RLMDr_1br * GAM_3 * RLMDg_3 + R(2) * GAM_2br * RLMDr_2br * GAM_2 * RLMDg_2 +
+               GAM_3br * RLMDr_3br * GAM * RLMDg);
+    R Pracs = As * R(0.25) * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
+              (MOMs_0bs * GAM_3 * RLMDr_3 + R(2) * MOMs_1bs * GAM_2 * RLMDr_2 + MOMs_2bs * GAM * RLMDr);
+    R Pgacs = As * R(0.25) * PI * Rdens * Egs_mod * N0g * fabs(Vtg - Vts) *
+              (MOMs_0bs * GAM_3 * RLMDg_3 + R(2) * MOMs_1bs * GAM_2 * RLMDg_2 + MOMs_2bs * GAM * RLMDg);
+
+    // autoconversion
+    const R Nc = R(P.Nc_def);
+    const R rhoqc = dens * qc * R(1000);
+    const R Dc = R(P.Dc);
+    const R Praut_berry = Rdens * R(1.67E-5) * rhoqc * rhoqc / (R(5) + R(3.66E-2) * Nc / (Dc * rhoqc + EPS));
+    const R Praut_kk = R(P.kk_aut_rate) * pow(qc, R(2.47)) * R(P.Nc_pow_m179);
+    R Praut = (R(1) - sw_kk) * Praut_berry + sw_kk * Praut_kk;
+    const R beta_saut = R(P.beta_saut), beta_gaut = R(P.beta_gaut);
+    const R betai = dmin(beta_saut, beta_saut * exp(R(P.gamma_saut) * temc));
+    R Psaut = dmax(betai * (qi - R(P.qicrt_saut)), R(0));
+    const R betas = dmin(beta_gaut, beta_gaut * exp(R(P.gamma_gaut) * temc));
+    R Pgaut = dmax(betas * (qs - R(P.qscrt_gaut)), R(0));
+
+    // evaporation, sublimation, melting, freezing
+    const R Ka = (R(Ka0) + R(dKa_dT) * temc);
+    const R Kd = (R(Kd0) + R(dKd_dT) * temc) * PRE00 / in.pre;
+    const R Nu = (R(nu0) + R(dnu_dT) * temc) * Rdens;
+    const R Glv = R(1) / (LHV0 / (Ka * temp) * (LHV0 / (Rvap * temp) - R(1)) + R(1) / (Kd * dens * in.qsatl));
+    const R Giv = R(1) / (LHS0 / (Ka * temp) * (LHS0 / (Rvap * temp) - R(1)) + R(1) / (Kd * dens * in.qsati));
+    const R Gil = R(1) / (LHF0 / (Ka * temc));
+    const R ventr = R(P.f1r) * GAM_2 * RLMDr_2 + R(P.f2r) * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * R(P.GAM_5dr_h);
+    R Prevp = R(2) * PI * Rdens * N0r * (R(1) - dmin(Sliq, R(1))) * Glv * ventr;
+
+    const R rhoqi = dmax(dens * qi, EPS);
+    const R XNi = dmin(dmax(R(5.38E+7) * exp(log(rhoqi) * R(0.75)), R(1.0E+3)), R(1.0E+6));
+    const R XMi = rhoqi / XNi;
+    const R Di = dmin(R(P.Di_a) * sqrt(XMi), R(P.Di_max));
+    R tmp = R(4) * Di * XNi * Rdens * (Sice - R(1)) * Giv;
+    R Pidep = (spsati) * (tmp);
+    R Pisub = (R(1) - spsati) * (-tmp);
+
+    R sw = step_neg(temc + R(40));
+    R Pihom = sw * qc / dt;
+    sw = step_pos(temc + R(40)) * step_neg(temc);
+    const R A_frz = R(P.A_frz), B_frz = R(P.B_frz);
+    R Pihtr = sw * (dens / rho_w * (qc * qc) / (R(P.Nc_ihtr) * R(1.0E+6))) * B_frz * (exp(-A_frz * temc) - R(1));
+    sw = step_pos(temc);
+    R Pimlt = sw * qi / dt;
+
+    const R vents = R(P.f1s) * MOMs_1 + R(P.f2s) * sqrt(Cs * rho_fact / Nu) * MOMs_5ds_h;
+    tmp = R(2) * PI * Rdens * (Sice - R(1)) * Giv * vents;
+    R Psdep = (spsati) * (tmp);
+    R Pssub = (R(1) - spsati) * (-tmp);
+    R Psmlt = R(2) * PI * Rdens * Gil * vents + CL * temc / LHF0 * (Psacw + Psacr);
+    Psmlt = dmax(Psmlt, R(0));
+
+    const R ventg = R(P.f1g) * GAM_2 * RLMDg_2 + R(P.f2g) * sqrt(Cg * rho_fact / Nu * RLMDg_5dg) * R(P.GAM_5dg_h);
+    tmp = R(2) * PI * Rdens * N0g * (Sice - R(1)) * Giv * ventg;
+    R Pgdep = (spsati) * (tmp);
+    R Pgsub = (R(1) - spsati) * (-tmp);
+    R Pgmlt = R(2) * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * (Pgacw + Pgacr);
+    Pgmlt = dmax(Pgmlt, R(0));
+    R Pgfrz = R(2) * PI * Rdens * N0r * R(60) * B_frz * Ar * (exp(-A_frz * temc) - R(1)) * RLMDr_7;
+
+    // Bergeron process
+    const R mi50 = R(P.mi50);
+    const R dt1 = (exp(R(P.ln_mi50) * in.ma2) - exp(R(P.ln_mi40) * in.ma2)) / (in.a1 * in.ma2);
+    const R Ni50 = qi * dt / (mi50 * dt1);
+    R Psfw = Ni50 * (in.a1 * pow(mi50, in.a2) + PI * R(P.Eiw) * dens * qc * R(P.Ri50) * R(P.Ri50) * R(P.vti50));
+    R Psfi = qi / dt1;
+
+    // limiters
+    Pigen = dmin(Pigen, dqv_dt) * (iceflg)*sw_expice;
+    Pidep = dmin(Pidep, dqv_dt) * (iceflg)*sw_expice;
+    Psdep = dmin(Psdep, dqv_dt) * (iceflg);
+    Pgdep = dmin(Pgdep, dqv_dt) * (iceflg);
+
+    Pracw = Pracw + Psacw * (R(1) - iceflg) + Pgacw * (R(1) - iceflg);
+
+    Praut = dmin(Praut, dqc_dt);
+    Pracw = dmin(Pracw, dqc_dt);
+    Pihom = dmin(Pihom, dqc_dt) * (iceflg)*sw_expice;
+    Pihtr = dmin(Pihtr, dqc_dt) * (iceflg)*sw_expice;
+    Psacw = dmin(Psacw, dqc_dt) * (iceflg);
+    Psfw = dmin(Psfw, dqc_dt) * (iceflg)*sw_bergeron;
+    Pgacw = dmin(Pgacw, dqc_dt) * (iceflg);
+
+    Prevp = dmin(Prevp, dqr_dt);
+    Piacr = dmin(Piacr, dqr_dt) * (iceflg);
+    Psacr = dmin(Psacr, dqr_dt) * (iceflg);
+    Pgacr = dmin(Pgacr, dqr_dt) * (iceflg);
+    Pgfrz = dmin(Pgfrz, dqr_dt) * (iceflg);
+
+    Pisub = dmin(Pisub, dqi_dt) * (iceflg)*sw_expice;
+    Pimlt = dmin(Pimlt, dqi_dt) * (R(1) - iceflg) * sw_expice;
+    Psaut = dmin(Psaut, dqi_dt) * (iceflg);
+    Praci = dmin(Praci, dqi_dt) * (iceflg);
+    Psaci = dmin(Psaci, dqi_dt) * (iceflg);
+    Psfi = dmin(Psfi, dqi_dt) * (iceflg)*sw_bergeron;
+    Pgaci = dmin(Pgaci, dqi_dt) * (iceflg);
+
+    Pssub = dmin(Pssub, dqs_dt) * (iceflg);
+    Psmlt = dmin(Psmlt, dqs_dt) * (R(1) - iceflg);
+    Pgaut = dmin(Pgaut, dqs_dt) * (iceflg);
+    Pracs = dmin(Pracs, dqs_dt) * (iceflg);
+    Pgacs = dmin(Pgacs, dqs_dt);
+
+    Pgsub = dmin(Pgsub, dqg_dt) * (iceflg);
+    Pgmlt = dmin(Pgmlt, dqg_dt) * (R(1) - iceflg);
+
+    R Piacr_s = (R(1) - delta1) * Piacr;
+    R Piacr_g = (delta1)*Piacr;
+    R Praci_s = (R(1) - delta1) * Praci;
+    R Praci_g = (delta1)*Praci;
+    R Psacr_s = (delta2)*Psacr;
+    R Psacr_g = (R(1) - delta2) * Psacr;
+    Pracs = (R(1) - delta2) * Pracs;
+
+    R net, fac_sw, fac;
+    // [QC]
+    net = +Pimlt - Praut - Pracw - Pihom - Pihtr - Psacw - Psfw - Pgacw;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqc_dt / (net - fac_sw), R(1));
+    Pimlt *= fac; Praut *= fac; Pracw *= fac; Pihom *= fac;
+    Pihtr *= fac; Psacw *= fac; Psfw *= fac; Pgacw *= fac;
+    // [QI]
+    net = +Pigen + Pidep + Pihom + Pihtr - Pisub - Pimlt - Psaut - Praci_s - Psaci - Psfi - Praci_g - Pgaci;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqi_dt / (net - fac_sw), R(1));
+    Pigen *= fac; Pidep *= fac; Pihom *= fac; Pihtr *= fac; Pisub *= fac; Pimlt *= fac;
+    Psaut *= fac; Praci_s *= fac; Psaci *= fac; Psfi *= fac; Praci_g *= fac; Pgaci *= fac;
+    // [QR]
+    net = +Praut + Pracw + Psmlt + Pgmlt - Prevp - Piacr_s - Psacr_s - Piacr_g - Psacr_g - Pgacr - Pgfrz;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqr_dt / (net - fac_sw), R(1));
+    Praut *= fac; Pracw *= fac; Psmlt *= fac; Pgmlt *= fac; Prevp *= fac; Piacr_s *= fac;
+    Psacr_s *= fac; Piacr_g *= fac; Psacr_g *= fac; Pgacr *= fac; Pgfrz *= fac;
+    // [QV]
+    net = +Prevp + Pisub + Pssub + Pgsub - Pigen - Pidep - Psdep - Pgdep;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqv_dt / (net - fac_sw), R(1));
+    Prevp *= fac; Pisub *= fac; Pssub *= fac; Pgsub *= fac;
+    Pigen *= fac; Pidep *= fac; Psdep *= fac; Pgdep *= fac;
+    // [QS]
+    net = +Psdep + Psacw + Psfw + Piacr_s + Psacr_s + Psaut + Praci_s + Psaci + Psfi - Pssub - Psmlt - Pgaut -
+          Pracs - Pgacs;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqs_dt / (net - fac_sw), R(1));
+    Psdep *= fac; Psacw *= fac; Psfw *= fac; Piacr_s *= fac; Psacr_s *= fac; Psaut *= fac; Praci_s *= fac;
+    Psaci *= fac; Psfi *= fac; Pssub *= fac; Psmlt *= fac; Pgaut *= fac; Pracs *= fac; Pgacs *= fac;
+    // [QG]
+    net = +Pgdep + Pgacw + Piacr_g + Psacr_g + Pgacr + Pgfrz + Praci_g + Pgaci + Pgaut + Pracs + Pgacs - Pgsub -
+          Pgmlt;
+    fac_sw = step_pos(net + EPS);
+    fac = fac_sw + (R(1) - fac_sw) * dmin(-dqg_dt / (net - fac_sw), R(1));
+    Pgdep *= fac; Pgacw *= fac; Piacr_g *= fac; Psacr_g *= fac; Pgacr *= fac; Pgfrz *= fac; Praci_g *= fac;
+    Pgaci *= fac; Pgaut *= fac; Pracs *= fac; Pgacs *= fac; Pgsub *= fac; Pgmlt *= fac;
+
+    R qc_t = +Pimlt - Praut - Pracw - Pihom - Pihtr - Psacw - Psfw - Pgacw;
+    R qr_t = +Praut + Pracw + Psmlt + Pgmlt - Prevp - Piacr_s - Psacr_s - Piacr_g - Psacr_g - Pgacr - Pgfrz;
+    R qi_t = +Pigen + Pidep + Pihom + Pihtr - Pisub - Pimlt - Psaut - Praci_s - Psaci - Psfi - Praci_g - Pgaci;
+    R qs_t = +Psdep + Psacw + Psfw + Piacr_s + Psacr_s + Psaut + Praci_s + Psaci + Psfi - Pssub - Psmlt - Pgaut -
+             Pracs - Pgacs;
+    R qg_t = +Pgdep + Pgacw + Piacr_g + Psacr_g + Pgacr + Pgfrz + Praci_g + Pgaci + Pgaut + Pracs + Pgacs - Pgsub -
+             Pgmlt;
+    qc_t = dmax(qc_t, -dqc_dt);
+    qr_t = dmax(qr_t, -dqr_dt);
+    qi_t = dmax(qi_t, -dqi_dt);
+    qs_t = dmax(qs_t, -dqs_dt);
+    qg_t = dmax(qg_t, -dqg_dt);
+    const R qv_t = -(qc_t + qr_t + qi_t + qs_t + qg_t);
+
+    o.drhogq[QV] = in.rhog * qv_t * dt;
+    o.drhogq[QC] = in.rhog * qc_t * dt;
+    o.drhogq[QR] = in.rhog * qr_t * dt;
+    o.drhogq[QI] = in.rhog * qi_t * dt;
+    o.drhogq[QS] = in.rhog * qs_t * dt;
+    o.drhogq[QG] = in.rhog * qg_t * dt;
+    o.Vtr = Vtr;
+    o.Vti = Vti;
+    o.Vts = Vts;
+    o.Vtg = Vtg;
+
+    // effective radius of liquid water
+    const R coef_xf = R(P.coef_xf);
+    const R xf_qc = rhoqc / Nc * R(1.0E-9);
+    const R rf_qc = pow((coef_xf * xf_qc + EPS), R(0.33333333));
+    const R r2_qc = R(P.coef_dgam) * (rf_qc * rf_qc) * (Nc * R(1.0E+6));
+    const R r2_qr = R(0.25) * N0r * GAM_3 * RLMDr_3;
+    const R r3_qc = coef_xf * dens * qc;
+    const R r3_qr = coef_xf * dens * qr;
+    zerosw = step_neg((r2_qc + r2_qr) - R(1.0E-10));
+    o.rceff = (r3_qc + r3_qr) / (r2_qc + r2_qr + zerosw) * (R(1) - zerosw);
+    sw = step_pos((qc + qr) - R(1.0E-5)) * zerosw;
+    o.rctop = (sw)*o.rceff + (R(1) - sw) * UNDEF;
+    o.tctop = (sw)*temp + (R(1) - sw) * UNDEF;
+    zerosw = step_neg(r2_qc - R(1.0E-10));
+    o.rceff_cld = r3_qc / (r2_qc + zerosw) * (R(1) - zerosw);
+}
+
+// Order-preserving map of doubles to unsigned 64-bit integers (for atomic max/min).
+__device__ __forceinline__ unsigned long long d2key(double d)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+}
+__host__ __device__ __forceinline__ double key2d(unsigned long long k)
+{
+    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffULL) : ~k;
+    double d;
+    __builtin_memcpy(&d, &u, sizeof d);
+    return d;
+}
+
+}  // namespace nsw6

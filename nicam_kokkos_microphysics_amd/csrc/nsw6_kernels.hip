@@ -1,0 +1,1036 @@
+// nsw6_kernels.hip -- HIP kernels of the NSW6 microphysics step for gfx950 (MI355X).
+//
+// One step = four launches on one stream:
+//   k_rates    point-wise over (l,k,ij): negative filter, psat/qsat, Bergeron, the
+//              49 process rates and the state update (src/mod_mp_nsw6.cpp:439-589);
+//              writes the pre-sedimentation state and the fall speeds to workspace.
+//   k_extremes one lane per column: departure distances zdis of the full and half
+//              levels for QR, QS, QG (src/mod_vadv1d.cpp:270-302) and their
+//              per-region, per-level max/min (:317-327) through atomics.
+//   k_bounds   one thread per (region, tracer, level type, level): kcell_min/max
+//              search (:329-351) and the per-region choice of sweep.
+//   k_sweep    one lane per column, bottom to top: per-column departure cells,
+//              semi-Lagrangian fluxes (:393-450), flux divergences, surface fluxes,
+//              KIN2EIN, rediagnosis (src/mod_precip_transport.cpp:573-1087),
+//              saturation adjustment (src/mod_satadjust.cpp:274-362,771-896) and the
+//              final negative filter; every output is written once.
+// All column loops keep ij fastest, so every per-level access of a wavefront is one
+// contiguous 512-byte segment.  No MFMA: nothing here is a dense contraction.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "nsw6_device.h"
+#include "nsw6_internal.h"
+
+using namespace nsw6;
+
+__constant__ double c_a1_tab[32] = {
+    0.0001E-7, 0.7939E-7, 0.7841E-6, 0.3369E-5, 0.4336E-5, 0.5285E-5, 0.3728E-5, 0.1852E-5,
+    0.2991E-6, 0.4248E-6, 0.7434E-6, 0.1812E-5, 0.4394E-5, 0.9145E-5, 0.1725E-4, 0.3348E-4,
+    0.1725E-4, 0.9175E-5, 0.4412E-5, 0.2252E-5, 0.9115E-6, 0.4876E-6, 0.3473E-6, 0.4758E-6,
+    0.6306E-6, 0.8573E-6, 0.7868E-6, 0.7192E-6, 0.6513E-6, 0.5956E-6, 0.5333E-6, 0.4834E-6};
+__constant__ double c_a2_tab[32] = {
+    0.0100, 0.4006, 0.4831, 0.5320, 0.5307, 0.5319, 0.5249, 0.4888, 0.3849, 0.4047, 0.4318,
+    0.4771, 0.5183, 0.5463, 0.5651, 0.5813, 0.5655, 0.5478, 0.5203, 0.4906, 0.4447, 0.4126,
+    0.3960, 0.4149, 0.4320, 0.4506, 0.4483, 0.4460, 0.4433, 0.4413, 0.4382, 0.4361};
+
+// ---------------------------------------------------------------------------------
+// k_rates
+// ---------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_rates(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const int l = blockIdx.z, k = blockIdx.y;
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+
+    // Region-wide reductions of this step start from their neutral values.
+    if (blockIdx.x == 0 && k == 0) {
+        unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
+        for (int i = threadIdx.x; i < EXT_PER_REGION; i += blockDim.x)
+            ext[i] = (i & 1) ? d2key(__builtin_inf()) : d2key(-__builtin_inf());
+        if (threadIdx.x == 0) a.ws.region_flag[l] = 0;
+        if (l == 0 && threadIdx.x < REPORT_INTS) a.ws.report[threadIdx.x] = (threadIdx.x >= 3) ? -1 : 0;
+    }
+    if (ij >= ijdim) return;
+
+    const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;   // + n*K2
+    const int64_t pv = (int64_t)l * 3 * K2 + (int64_t)k * ijdim + ij;    // + t*K2
+    const Fields &f = a.f;
+
+    if (k == 0 || k == KALL - 1) {
+        a.ws.rhog[p] = f.rhog[p];
+        a.ws.rhoge[p] = f.rhoge[p];
+#pragma unroll
+        for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
+#pragma unroll
+        for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = 0.0;
+        f.re_liquid[p] = 0.0;
+        f.re_cld[p] = 0.0;
+        return;
+    }
+
+    const DevParams &P = a.P;
+    // ---- negative_filter (src/mod_mp_nsw6.cpp:1770-1831), interior point
+    double rq[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) rq[n] = f.rhogq[pq + n * K2];
+    double diffq = 0.0;
+#pragma unroll
+    for (int n = 2; n <= 5; n++) {
+        diffq += rq[n];
+        rq[n] = dmax(rq[n], 0.0);
+    }
+#pragma unroll
+    for (int n = 2; n <= 5; n++) diffq -= rq[n];
+    rq[QV] += diffq;
+    diffq = rq[QV];
+    rq[QV] = dmax(rq[QV], 0.0);
+    diffq -= rq[QV];
+    const double rhog = f.rhog[p] * (1.0 - diffq);
+    const double gsgam2 = f.gsgam2[p];
+    const double rho = rhog / gsgam2;
+    double q[NQ];
+    q[QV] = f.q[pq];                                  // stale input QV (reference keeps it)
+#pragma unroll
+    for (int n = 1; n <= 5; n++) q[n] = rq[n] / rhog;
+    double qd = 1.0;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qd = qd - q[n];
+    double cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
+    const double tem = f.tem[p];
+    double rhoge = tem * rhog * cva;
+    const double pre = rho * (qd * C_Rdry + q[QV] * C_Rvap) * tem;
+
+    // ---- psat / qsat (TagSATWater), Bergeron
+    const double qsatl = psat_liq(P, tem) / (rho * C_Rvap * tem);
+    const double qsati = psat_ice(P, tem) / (rho * C_Rvap * tem);
+    double a1, a2, ma2;
+    bergeron(c_a1_tab, c_a2_tab, tem, a1, a2, ma2);
+
+    // ---- TagBigLoop
+    RateIn<R> in;
+    in.dens = (R)rho;
+    in.temp = (R)tem;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) in.q[n] = (R)q[n];
+    in.qsatl = (R)qsatl;
+    in.qsati = (R)qsati;
+    in.pre = (R)pre;
+    in.rhog = (R)rhog;
+    in.a1 = (R)a1;
+    in.a2 = (R)a2;
+    in.ma2 = (R)ma2;
+    RateOut<R> o;
+    rates<R>(P, (R)a.dt, in, o);
+
+    // ---- TagUpdateRHOGQ / TagUpdateRHOGE
+#pragma unroll
+    for (int n = 0; n < NQ; n++) rq[n] += (double)o.drhogq[n];
+    rhoge = rhoge - C_LHV0 * (double)o.drhogq[QV] + C_LHF0 * (double)o.drhogq[QI] +
+            C_LHF0 * (double)o.drhogq[QS] + C_LHF0 * (double)o.drhogq[QG];
+
+    a.ws.rhog[p] = rhog;
+    a.ws.rhoge[p] = rhoge;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = rq[n];
+    a.ws.vt[pv + 0 * K2] = (double)o.Vtr;
+    a.ws.vt[pv + 1 * K2] = (double)o.Vts;
+    a.ws.vt[pv + 2 * K2] = (double)o.Vtg;
+    f.re_liquid[p] = (double)o.rceff;
+    f.re_cld[p] = (double)o.rceff_cld;
+    if (k == KMAX) {
+        // every level writes the cloud-top row in the reference; the serial order leaves kmax
+        const int64_t pc = (int64_t)l * ijdim + ij;
+        const double rct = (double)o.rctop, tct = (double)o.tctop;
+        f.rctop[pc] = rct;
+        f.tctop[pc] = tct;
+        const double sw = step_pos(tct - C_TEM00);
+        f.rwtop[pc] = (sw)*rct + (1.0 - sw) * C_UNDEF;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// departure distance of vadv1d_prep (src/mod_vadv1d.cpp:284-302)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double zdis_interior(double dt, double dt2, double dt3, double whm, double whk,
+                                                double whp, double dzm, double dzk)
+{
+    const double s = dzm + dzk;
+    const double t1 = dt * whk;
+    const double t2 = dt2 * whk * (whp - whm) / s / 2.0;
+    const double g = (whp - whm) / s;
+    const double inner = ((whp - whk) / dzk - (whk - whm) / dzm) / s * 2.0;
+    const double t3 = dt3 * whk * (g * g + whk * inner) / 6.0;
+    return t1 - t2 + t3;
+}
+__device__ __forceinline__ double zdis_bottom(double dt, double dt2, double whk, double whp, double dzk)
+{
+    return dt * whk - dt2 * whk * (whp - whk) / dzk / 2.0;
+}
+
+// Sliding window over the fall speed Vt(k) of one tracer that produces, level by
+// level, wh/zdis of the full level (mkmin=1, mkmax=94) and of the half level
+// (Vtermh, mkmin=2, mkmax=94) exactly as vadv1d_prep builds them.
+struct WindWindow {
+    double vt_k;          // Vt(k)
+    double wh_km1, wh_k;  // wh(k-1), wh(k)          (full level)
+    double vh_k;          // Vtermh(k)
+    double whh_km1, whh_k;// wh_h(k-1), wh_h(k)      (half level)
+};
+// Initialise with cell 0 and cell 1 known (Vt(0) = 0): state for k = 1.
+__device__ __forceinline__ void wind_init(WindWindow &w, double vt1)
+{
+    w.vt_k = vt1;
+    w.wh_km1 = 0.0;   // wh(0) = Vt(0) = 0
+    w.wh_k = vt1;     // wh(1) = Vt(1)
+    w.vh_k = 0.0;     // Vtermh(1) = 0
+    w.whh_km1 = 0.0;  // wh_h(0) = 0
+    w.whh_k = 0.0;    // wh_h(1) = Vtermh(1) = 0
+}
+// Given Vt(k+1) (0 for k+1 = 95), produce wh(k+1) and wh_h(k+1) for level k.
+__device__ __forceinline__ void wind_next(const WindWindow &w, int k, double vt_kp1, double &wh_kp1,
+                                          double &vh_kp1, double &whh_kp1)
+{
+    const int kp = k + 1;
+    wh_kp1 = (kp <= KMAX) ? 0.5 * (w.vt_k + vt_kp1) : w.vt_k;            // wh(95) = Vt(94)
+    vh_kp1 = (kp >= 2 && kp <= KMAX - 1) ? 0.5 * (vt_kp1 + w.vt_k) : 0.0; // Vtermh
+    whh_kp1 = (kp >= 3 && kp <= KMAX) ? 0.5 * (w.vh_k + vh_kp1) : ((kp == 2) ? vh_kp1 : 0.0);
+}
+__device__ __forceinline__ void wind_shift(WindWindow &w, double vt_kp1, double wh_kp1, double vh_kp1,
+                                           double whh_kp1)
+{
+    w.vt_k = vt_kp1;
+    w.wh_km1 = w.wh_k;
+    w.wh_k = wh_kp1;
+    w.vh_k = vh_kp1;
+    w.whh_km1 = w.whh_k;
+    w.whh_k = whh_kp1;
+}
+__device__ __forceinline__ double zdis_full(const GridDev &g, double dt, double dt2, double dt3,
+                                            const WindWindow &w, int k, double wh_kp1)
+{
+    return (k == KMIN) ? zdis_bottom(dt, dt2, w.wh_k, wh_kp1, g.dgz[KMIN])
+                       : zdis_interior(dt, dt2, dt3, w.wh_km1, w.wh_k, wh_kp1, g.dgz[k - 1], g.dgz[k]);
+}
+__device__ __forceinline__ double zdis_half(const GridDev &g, double dt, double dt2, double dt3,
+                                            const WindWindow &w, int k, double whh_kp1)
+{
+    return (k == KMIN + 1) ? zdis_bottom(dt, dt2, w.whh_k, whh_kp1, g.dgzh[KMIN + 1])
+                           : zdis_interior(dt, dt2, dt3, w.whh_km1, w.whh_k, whh_kp1, g.dgzh[k - 1], g.dgzh[k]);
+}
+
+template <int N>
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = N / 2; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, N));
+    return v;
+}
+template <int N>
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = N / 2; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, N));
+    return v;
+}
+
+// ---------------------------------------------------------------------------------
+// k_extremes: one lane per column, one wavefront per block
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_extremes(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const int l = blockIdx.y;
+    const int ij = blockIdx.x * 64 + threadIdx.x;
+    const bool live = ij < ijdim;
+    const int ijc = live ? ij : (ijdim - 1);
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const double *vt = a.ws.vt + (int64_t)l * 3 * K2 + ijc;
+    const GridDev &g = *a.grid;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3;
+    unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
+
+    WindWindow w[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) wind_init(w[t], vt[t * K2 + 1 * ijdim]);
+    for (int k = KMIN; k <= KMAX; k++) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const double vt_kp1 = (k + 1 <= KMAX) ? vt[t * K2 + (int64_t)(k + 1) * ijdim] : 0.0;
+            double wh_kp1, vh_kp1, whh_kp1;
+            wind_next(w[t], k, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+            const double zf = zdis_full(g, dt, dt2, dt3, w[t], k, wh_kp1);
+            const bool okf = live && (zf == zf);
+            double vmax = wave_max<64>(okf ? zf : -__builtin_inf());
+            double vmin = wave_min<64>(okf ? zf : __builtin_inf());
+            if (threadIdx.x == 0) {
+                atomicMax(&ext[ext_index(t, 0, k, 0)], d2key(vmax));
+                atomicMin(&ext[ext_index(t, 0, k, 1)], d2key(vmin));
+            }
+            if (k >= KMIN + 1) {
+                const double zh = zdis_half(g, dt, dt2, dt3, w[t], k, whh_kp1);
+                const bool okh = live && (zh == zh);
+                vmax = wave_max<64>(okh ? zh : -__builtin_inf());
+                vmin = wave_min<64>(okh ? zh : __builtin_inf());
+                if (threadIdx.x == 0) {
+                    atomicMax(&ext[ext_index(t, 1, k, 0)], d2key(vmax));
+                    atomicMin(&ext[ext_index(t, 1, k, 1)], d2key(vmin));
+                }
+            }
+            wind_shift(w[t], vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_bounds: kcell_min/kcell_max of vadv1d_prep (src/mod_vadv1d.cpp:317-352)
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(192) k_bounds(StepArgs a)
+{
+    const int l = blockIdx.y;
+    const int t = blockIdx.x / 2, type = blockIdx.x % 2;   // grid.x = 6
+    const int k = threadIdx.x;                              // 0..95 used
+    if (k >= KALL) return;
+    const GridDev &g = *a.grid;
+    const double *zh = (type == 0) ? g.gzh : g.gz_shift;
+    const int mkmin = (type == 0) ? KMIN : KMIN + 1, mkmax = KMAX;
+    int kcmin = k, kcmax = k;
+    int general = 0;
+    if (k >= mkmin && k <= mkmax) {
+        const unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
+        const double amax = key2d(ext[ext_index(t, type, k, 0)]);
+        const double amin = key2d(ext[ext_index(t, type, k, 1)]);
+        const double zzmax = (amax > DBL_MIN) ? amax : DBL_MIN;   // starts at numeric_limits::min()
+        const double zzmin = (amin < DBL_MIN) ? amin : DBL_MIN;
+        if (zzmax > 0.0) {
+            for (int k2 = k; k2 >= mkmin; k2--) {
+                if ((zh[k2] <= zh[k] - zzmax) && (zh[k2 + 1] > zh[k] - zzmax)) {
+                    kcmin = k2;
+                    break;
+                }
+            }
+        }
+        if (zzmin < 0.0) {
+            for (int k2 = k; k2 <= mkmax; k2++) {
+                if ((zh[k2] <= zh[k] - zzmin) && (zh[k2 + 1] > zh[k] - zzmin)) {
+                    kcmax = k2;
+                    break;
+                }
+            }
+        }
+        // The register-window sweep needs every departure cell in [k, k+1].
+        const int ktop = (k + 2 <= mkmax + 1) ? k + 2 : mkmax + 1;
+        if (kcmin != k || kcmax > k + 1) general = 1;
+        if (amax > 0.0 && zh[k] - zzmax < zh[k]) general = 1;
+        if (zh[k] - zzmin >= zh[ktop]) general = 1;
+    }
+    int *b = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    b[bounds_index(t, type, k, 0)] = kcmin;
+    b[bounds_index(t, type, k, 1)] = kcmax;
+    if (general) atomicOr(&a.ws.region_flag[l], 1);
+}
+
+// ---------------------------------------------------------------------------------
+// k_sweep
+// ---------------------------------------------------------------------------------
+// Compact transport data of one cell j (what the fluxes need).
+struct TCell {
+    double rhoq[3];     // rhogq'(t) * rgs
+    double qrgs[3];     // q'(t) * rgs
+    double tem, z, kinh, up, vp;
+    double hv[3][2];    // half-level values at wall j: qh*rhogkin_v, qh*rhogw
+};
+// Values of cell j kept until the cell is finalised.
+struct FCell {
+    double rhog, rhogw, kinv, c2w0, c2w1;   // pre-transport; kinv/c2w* belong to wall j
+    double drhoq[3], drhoe, drhophi, drhokinh, drhogu, drhogv;  // full-level tendencies
+    double drhokinv, drhogw;                                   // half-level tendencies (wall j)
+};
+
+__device__ __forceinline__ double rhof_of(const TCell &c, int t, int i)
+{
+    switch (i) {
+    case 0: return c.rhoq[t];
+    case 1: return c.rhoq[t] * C_CVW * c.tem;
+    case 2: return c.rhoq[t] * C_GRAV * c.z;
+    case 3: return c.qrgs[t] * c.kinh;
+    case 4: return c.rhoq[t] * c.up;
+    default: return c.rhoq[t] * c.vp;
+    }
+}
+__device__ __forceinline__ double cs1(double x) { return copysign(1.0, x); }
+__device__ __forceinline__ double fact_of(int k2, int kc, int k, double dz)
+{
+    const double f1 = (double)(k2 - (kc + 1)), f2 = (double)((k - 1) - k2);
+    const double f3 = (double)(k2 - k), f4 = (double)((kc - 1) - k2);
+    return dz * 0.25 * ((cs1(f1) + 1.0) * (cs1(f2) + 1.0) - (cs1(f3) + 1.0) * (cs1(f4) + 1.0));
+}
+__device__ __forceinline__ double eps_filter(double f)
+{
+    return f * (0.5 + copysign(0.5, fabs(f) - C_EPS));
+}
+// per-column departure cell (src/mod_vadv1d.cpp:355-384)
+__device__ __forceinline__ int departure_cell(const double *zh, int k, int kcmin, int kcmax, double zdis, int mkmin,
+                                              int mkmax)
+{
+    int kc;
+    if (kcmin == k && kcmax == k) {
+        kc = k;
+    } else {
+        kc = 0;
+        const double x = zh[k] - zdis;
+        for (int k2 = kcmin; k2 <= kcmax; k2++) {
+            const int tmp = (int)((double)k2 * copysign(1.0, x - zh[k2]) * copysign(1.0, zh[k2 + 1] - x));
+            kc = (kc > tmp) ? kc : tmp;
+        }
+    }
+    if (kc == 0) kc = mkmin;
+    if (kc == mkmax + 1) kc = mkmax;
+    return kc;
+}
+
+struct SweepCtx {
+    const StepArgs *a;
+    int l, ij, ijdim;
+    int64_t p2, p3, pv, K2;     // base offsets of column (l, ij)
+    double ix, iy, iz, jx, jy, jz;
+};
+
+__device__ __forceinline__ void load_tcell(const SweepCtx &s, int j, TCell &c, double &rhog_j, double q_j[3],
+                                           double &gsgam2_j)
+{
+    const StepArgs &a = *s.a;
+    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
+    const double rg = a.ws.rhog[o];
+    double rq[6];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) rq[n] = a.ws.rhogq[s.p3 + (int64_t)(n * KALL + j) * s.ijdim];
+    const double re = a.ws.rhoge[o];
+    const double gs = a.f.gsgam2[o];
+    const bool interior = (j >= KMIN && j <= KMAX);
+    const double rgs = interior ? a.f.gam2[o] / gs : 0.0;
+    double qq[6];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qq[n] = rq[n] / rg;
+    double qd = 1.0;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qd = qd - qq[n];
+    double cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) cva = cva + qq[n] * C_CVW;
+    c.tem = interior ? re / (rg * cva) : 0.0;
+    const double rvx = a.f.rhogvx[o], rvy = a.f.rhogvy[o], rvz = a.f.rhogvz[o];
+    c.kinh = interior ? 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / rg : 0.0;
+    const double vx = a.f.vx[o], vy = a.f.vy[o], vz = a.f.vz[o];
+    c.up = vx * s.ix + vy * s.iy + vz * s.iz;
+    c.vp = vx * s.jx + vy * s.jy + vz * s.jz;
+    c.z = a.f.z[o];
+    c.rhoq[0] = rq[QR] * rgs;
+    c.rhoq[1] = rq[QS] * rgs;
+    c.rhoq[2] = rq[QG] * rgs;
+    c.qrgs[0] = qq[QR] * rgs;
+    c.qrgs[1] = qq[QS] * rgs;
+    c.qrgs[2] = qq[QG] * rgs;
+    rhog_j = rg;
+    q_j[0] = qq[QR];
+    q_j[1] = qq[QS];
+    q_j[2] = qq[QG];
+    gsgam2_j = gs;
+}
+
+// Half-level values at wall j and the pre-transport wall quantities of FCell j.
+__device__ __forceinline__ void load_wall(const SweepCtx &s, const GridDev &g, int j, double rhog_j,
+                                          double rhog_jm1, const double q_j[3], const double q_jm1[3],
+                                          double gsgam2_j, double gsgam2_jm1, TCell &c, FCell &fc)
+{
+    const StepArgs &a = *s.a;
+    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
+    const double rw = a.f.rhogw[o];
+    const double gsh = a.f.gsgam2h[o];
+    fc.rhog = rhog_j;
+    fc.rhogw = rw;
+    if (j >= KMIN) {
+        fc.c2w0 = g.afact[j] / gsgam2_j * gsh;
+        fc.c2w1 = g.bfact[j] / gsgam2_jm1 * gsh;
+    } else {
+        fc.c2w0 = 0.0;
+        fc.c2w1 = 0.0;
+    }
+    if (j >= KMIN + 1 && j <= KMAX) {
+        fc.kinv = 0.5 * (rw * rw) / (fc.c2w0 * rhog_j + fc.c2w1 * rhog_jm1);
+        const double rgsh = a.f.gam2h[o] / gsh;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const double qh = 0.5 * (q_j[t] + q_jm1[t]) * rgsh;
+            c.hv[t][0] = qh * fc.kinv;
+            c.hv[t][1] = qh * rw;
+        }
+    } else {
+        fc.kinv = 0.0;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            c.hv[t][0] = 0.0;
+            c.hv[t][1] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) fc.drhoq[t] = 0.0;
+    fc.drhoe = fc.drhophi = fc.drhokinh = fc.drhogu = fc.drhogv = 0.0;
+    fc.drhokinv = fc.drhogw = 0.0;
+}
+
+// Pre-transport data of an arbitrary cell/wall, from memory (general sweep only).
+__device__ double general_rhof(const SweepCtx &s, const GridDev &g, int j, int t, int i)
+{
+    TCell c;
+    double rg, qj[3], gs;
+    load_tcell(s, j, c, rg, qj, gs);
+    return rhof_of(c, t, i);
+}
+__device__ double general_hv(const SweepCtx &s, const GridDev &g, int j, int t, int i)
+{
+    if (j < KMIN + 1 || j > KMAX) return 0.0;
+    TCell c, cm;
+    FCell fc;
+    double rg, qj[3], gs, rgm, qjm[3], gsm;
+    load_tcell(s, j, c, rg, qj, gs);
+    load_tcell(s, j - 1, cm, rgm, qjm, gsm);
+    load_wall(s, g, j, rg, rgm, qj, qjm, gs, gsm, c, fc);
+    return c.hv[t][i];
+}
+
+// Saturation adjustment of one point after sedimentation (wrapper of
+// src/mod_satadjust.cpp:312-361 with satadjust_liq :826-894), followed by the final
+// negative filter (interior only) and the output writes.
+template <bool INTERIOR>
+__device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double rhog, double rhogq[6], double rhoge,
+                                               double gsgam2, int *fail_flag)
+{
+    const StepArgs &a = *s.a;
+    const Fields &f = a.f;
+    const DevParams &P = a.P;
+    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
+    // rediagnosis at the end of precip_transport_new (:1074-1086)
+    double q[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) q[n] = rhogq[n] / rhog;
+    double rho = rhog / gsgam2;
+    const double ein = rhoge / rhog;
+    double qd = 1.0;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qd = qd - q[n];
+    double cv = 0.0, qd2 = 1.0;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) {
+        cv = cv + q[n] * C_CVW;
+        qd2 = qd2 - q[n];
+    }
+    cv = cv + qd2 * C_CVdry;
+    double tem = ein / cv;
+    double pre = rho * tem * (qd2 * C_Rdry + q[QV] * C_Rvap);
+
+    // SATURATION_adjustment, ice_adjust = false
+    const double ein_moist = rhoge / rhog + q[QV] * C_LHV0;
+    const double qsum = q[QV] + q[QC];
+    q[QV] = qsum;
+    q[QC] = 0.0;
+    double CVtot = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
+    const double rho_sa = rhog / gsgam2;
+    tem = (ein_moist - q[QV] * C_LHV0) / CVtot;
+    if (INTERIOR) {
+        double qdl = 1.0;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) qdl = qdl - q[n];
+        double rtem = 1.0 / dmax(tem, TEM_MIN);
+        double psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+        double qsat = psat / (rho_sa * C_Rvap * tem);
+        if (qsum - qsat > C_EPS) {
+            bool converged = false, nan_break = false;
+            for (int ite = 0; ite < 100; ite++) {
+                rtem = 1.0 / dmax(tem, TEM_MIN);
+                psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+                qsat = psat / (rho_sa * C_Rvap * tem);
+                q[QV] = qsat;
+                q[QC] = qsum - qsat;
+                double CVt = qdl * C_CVdry;
+#pragma unroll
+                for (int n = 1; n <= 5; n++) CVt = CVt + q[n] * C_CVW;
+                const double Emoist_new = tem * CVt + q[QV] * C_LHV0;
+                const double dqsat_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsat;
+                const double dCVtot_dT = dqsat_dT * (C_CVW - C_CVW);
+                const double dEmoist_dT = tem * dCVtot_dT + CVt + dqsat_dT * C_LHV0;
+                const double dtemp = (Emoist_new - ein_moist) / dEmoist_dT;
+                tem = tem - dtemp;
+                if (fabs(dtemp) < P.dtemp_criteria) {
+                    converged = true;
+                    break;
+                }
+                if (tem * 0.0 != 0.0) {
+                    nan_break = true;
+                    break;
+                }
+            }
+            if (!converged) {
+                const int code = nan_break ? 2 : 1;
+                if (atomicAdd(&a.ws.report[code], 1) == 0) {
+                    // first failing point (lowest l, k, ij is not guaranteed; any one is)
+                    a.ws.report[3] = s.l;
+                    a.ws.report[4] = c;
+                    a.ws.report[5] = s.ij;
+                }
+                *fail_flag = 1;
+            }
+        }
+    }
+    rhogq[QV] = rhog * q[QV];
+    rhogq[QC] = rhog * q[QC];
+    double CVtot2 = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;
+    rhoge = rhog * tem * CVtot2;
+
+    if (INTERIOR) {
+        // final negative_filter (src/mod_mp_nsw6.cpp:1770-1831)
+        double diffq = 0.0;
+#pragma unroll
+        for (int n = 2; n <= 5; n++) {
+            diffq += rhogq[n];
+            rhogq[n] = dmax(rhogq[n], 0.0);
+        }
+#pragma unroll
+        for (int n = 2; n <= 5; n++) diffq -= rhogq[n];
+        rhogq[QV] += diffq;
+        diffq = rhogq[QV];
+        rhogq[QV] = dmax(rhogq[QV], 0.0);
+        diffq -= rhogq[QV];
+        rhog = rhog * (1.0 - diffq);
+        rho = rhog / gsgam2;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) q[n] = rhogq[n] / rhog;
+        double qdn = 1.0;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) qdn = qdn - q[n];
+        double cva = qdn * C_CVdry;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
+        rhoge = tem * rhog * cva;
+        pre = rho * (qdn * C_Rdry + q[QV] * C_Rvap) * tem;
+    }
+
+    f.rhog[o] = rhog;
+    f.rhoge[o] = rhoge;
+    f.rho[o] = rho;
+    f.tem[o] = tem;
+    f.pre[o] = pre;
+    f.qd[o] = qd;
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        const int64_t oq = s.p3 + (int64_t)(n * KALL + c) * s.ijdim;
+        f.rhogq[oq] = rhogq[n];
+        f.q[oq] = q[n];
+    }
+    f.re_solid[o] = 20.0E-6;
+    f.GDCLW[o] = 0.0 * 1.0;
+    f.GDCFRC[o] = 0.0 * 1.0;
+}
+
+template <bool GENERAL>
+__global__ void __launch_bounds__(64) k_sweep(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const int l = blockIdx.y;
+    if ((a.ws.region_flag[l] != 0) != GENERAL) return;
+    const int ij = blockIdx.x * 64 + threadIdx.x;
+    if (ij >= ijdim) return;
+    const GridDev &g = *a.grid;
+    const Fields &f = a.f;
+    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3;
+
+    SweepCtx s;
+    s.a = &a;
+    s.l = l;
+    s.ij = ij;
+    s.ijdim = ijdim;
+    s.K2 = (int64_t)KALL * ijdim;
+    s.p2 = (int64_t)l * s.K2 + ij;
+    s.p3 = (int64_t)l * NQ * s.K2 + ij;
+    s.pv = (int64_t)l * 3 * s.K2 + ij;
+    const int64_t pc = (int64_t)l * ijdim + ij;
+    s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
+    s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
+    const double *vt = a.ws.vt + s.pv;
+    double *mom = GENERAL ? (a.ws.mom + (int64_t)l * 4 * s.K2 + ij) : nullptr;
+
+    // window: cells w (cur) and w+1 (nxt); finalisation records for w-2 .. w+1
+    TCell cur, nxt;
+    FCell fm2, fm1, f0, fp1;
+    double rhog_cur, q_cur[3], gs_cur;
+    double rhog_nxt, q_nxt[3], gs_nxt;
+    WindWindow wind[3];
+    double fprev[3][6], fprevh[3][2];
+    double rhog_new_prev = 0.0;       // rhog after sedimentation of the last finalised cell
+    double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
+    int fail = 0;
+
+    // cell 0 (halo) and cell 1
+    {
+        TCell c0;
+        double rg0, q0[3], gs0;
+        load_tcell(s, 0, c0, rg0, q0, gs0);
+        FCell fdummy;
+        load_wall(s, g, 0, rg0, rg0, q0, q0, gs0, gs0, c0, fdummy);
+        fm1 = fdummy;     // cell 0 record (w-1 for w = 1)
+        load_tcell(s, 1, cur, rhog_cur, q_cur, gs_cur);
+        load_wall(s, g, 1, rhog_cur, rg0, q_cur, q0, gs_cur, gs0, cur, f0);
+        rhog_new_prev = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        wind_init(wind[t], vt[t * s.K2 + 1 * (int64_t)ijdim]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) fprev[t][i] = 0.0;    // f(0) = 0
+        fprevh[t][0] = fprevh[t][1] = 0.0;                // f_h(1) = 0
+    }
+
+    for (int w = KMIN; w <= KMAX + 1; w++) {
+        double fcur[3][6], fcurh[3][2];
+        if (w <= KMAX) {
+            // ---- load cell w+1
+            load_tcell(s, w + 1, nxt, rhog_nxt, q_nxt, gs_nxt);
+            load_wall(s, g, w + 1, rhog_nxt, rhog_cur, q_nxt, q_cur, gs_nxt, gs_cur, nxt, fp1);
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const double vt_kp1 = (w + 1 <= KMAX) ? vt[t * s.K2 + (int64_t)(w + 1) * ijdim] : 0.0;
+                double wh_kp1, vh_kp1, whh_kp1;
+                wind_next(wind[t], w, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+                // ---- full-level flux at wall w
+                {
+                    const double z0 = zdis_full(g, dt, dt2, dt3, wind[t], w, wh_kp1);
+                    const int kcmin = bnd[bounds_index(t, 0, w, 0)], kcmax = bnd[bounds_index(t, 0, w, 1)];
+                    const int kc = departure_cell(g.gzh, w, kcmin, kcmax, z0, KMIN, KMAX);
+#pragma unroll
+                    for (int i = 0; i < 6; i++) {
+                        double acc = 0.0, zd = 0.0;
+                        if (kcmin == w && kcmax == w) {
+                            zd = z0;
+                        } else {
+                            for (int k2 = kcmin; k2 <= kcmax; k2++) {
+                                const double fa = fact_of(k2, kc, w, g.dgz[k2]);
+                                double rf;
+                                if (k2 == w) rf = rhof_of(cur, t, i);
+                                else if (k2 == w + 1) rf = rhof_of(nxt, t, i);
+                                else rf = GENERAL ? general_rhof(s, g, k2, t, i) : 0.0;
+                                acc = acc + rf * fa;
+                                zd = z0 - fa;
+                            }
+                        }
+                        double rfc;
+                        if (kc == w) rfc = rhof_of(cur, t, i);
+                        else if (kc == w + 1) rfc = rhof_of(nxt, t, i);
+                        else rfc = GENERAL ? general_rhof(s, g, kc, t, i) : 0.0;
+                        acc = acc + rfc * zd;
+                        fcur[t][i] = eps_filter(acc);
+                    }
+                }
+                // ---- half-level flux at wall w (walls 2..94)
+                if (w >= KMIN + 1) {
+                    const double z0 = zdis_half(g, dt, dt2, dt3, wind[t], w, whh_kp1);
+                    const int khmin = bnd[bounds_index(t, 1, w, 0)], khmax = bnd[bounds_index(t, 1, w, 1)];
+                    const int kfmax = bnd[bounds_index(t, 0, w, 1)];   // full-level kcell_max
+                    const int kc = departure_cell(g.gz_shift, w, khmin, khmax, z0, KMIN + 1, KMAX);
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        double acc = 0.0, zd = 0.0;
+                        if (khmin == w && kfmax == w) {
+                            zd = z0;
+                        } else {
+                            for (int k2 = khmin; k2 <= kfmax; k2++) {
+                                const double fa = fact_of(k2, kc, w, g.dgzh[k2]);
+                                double hvv;
+                                if (k2 == w) hvv = cur.hv[t][i];
+                                else if (k2 == w + 1) hvv = nxt.hv[t][i];
+                                else hvv = GENERAL ? general_hv(s, g, k2, t, i) : 0.0;
+                                acc = acc + hvv * fa;
+                                zd = z0 - fa;
+                            }
+                        }
+                        double hvc;
+                        if (kc == w) hvc = cur.hv[t][i];
+                        else if (kc == w + 1) hvc = nxt.hv[t][i];
+                        else hvc = GENERAL ? general_hv(s, g, kc, t, i) : 0.0;
+                        acc = acc + hvc * zd;
+                        fcurh[t][i] = eps_filter(acc);
+                    }
+                } else {
+                    fcurh[t][0] = fcurh[t][1] = 0.0;
+                }
+                wind_shift(wind[t], vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+            }
+            // GPREC = frain = -fprec_q(QR)/dt on k = 1..94, then * fraction_mp
+            f.GPREC[s.p2 + (int64_t)w * ijdim] = (0.0 + (-fcur[0][0] / dt)) * 1.0;
+            if (w == KMIN) {
+                precip0 = precip0 - fcur[0][0] / dt;
+                precip1 = precip1 - fcur[1][0] / dt;
+                lh = lh + fcur[1][0] * C_LHF0 / dt;
+                precip1 = precip1 - fcur[2][0] / dt;
+                lh = lh + fcur[2][0] * C_LHF0 / dt;
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    prhoe = prhoe - fcur[t][1] / dt;
+                    prhophi = prhophi - fcur[t][2] / dt;
+                    prhokin = prhokin - fcur[t][3] / dt - 0.0 / dt;   // f_h(kmin) = 0
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) fcur[t][i] = 0.0;
+                fcurh[t][0] = fcurh[t][1] = 0.0;
+            }
+        }
+
+        // ---- full-level divergence of cell w-1 (k = 1..94); its record is fm1
+        if (w - 1 >= KMIN) {
+            const double r = g.rdgz[w - 1];
+#pragma unroll
+            for (int t = 0; t < 3; t++) fm1.drhoq[t] = -(fcur[t][0] - fprev[t][0]) * r;
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                fm1.drhoe = fm1.drhoe - (fcur[t][1] - fprev[t][1]) * r;
+                fm1.drhophi = fm1.drhophi - (fcur[t][2] - fprev[t][2]) * r;
+                fm1.drhokinh = fm1.drhokinh - (fcur[t][3] - fprev[t][3]) * r;
+                fm1.drhogu = fm1.drhogu - (fcur[t][4] - fprev[t][4]) * r;
+                fm1.drhogv = fm1.drhogv - (fcur[t][5] - fprev[t][5]) * r;
+            }
+        }
+        // ---- half-level divergence at wall w (2..94)
+        if (w >= KMIN + 1 && w <= KMAX) {
+            const double r = g.rdgzh[w];
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                f0.drhokinv = f0.drhokinv - (fcurh[t][0] - fprevh[t][0]) * r;
+                f0.drhogw = f0.drhogw - (fcurh[t][1] - fprevh[t][1]) * r;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+#pragma unroll
+            for (int i = 0; i < 6; i++) fprev[t][i] = fcur[t][i];
+            fprevh[t][0] = fcurh[t][0];
+            fprevh[t][1] = fcurh[t][1];
+        }
+
+        // ---- finalise cell c = w-2 (needs cell w-1's tendencies, computed above)
+        const int c = w - 2;
+        if (c >= KMIN) {
+            // fm2 = cell c, fm1 = cell c+1
+            const int64_t o = s.p2 + (int64_t)c * ijdim;
+            double rq[6];
+#pragma unroll
+            for (int n = 0; n < 6; n++) rq[n] = a.ws.rhogq[s.p3 + (int64_t)(n * KALL + c) * ijdim];
+            const double re0 = a.ws.rhoge[o];
+            double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
+            const double z = f.z[o], gs = f.gsgam2[o];
+            const double gshp = f.gsgam2h[o + ijdim];
+            const double w2c0 = g.cfact[c] * gs / gshp, w2c1 = g.dfact[c] * gs / gshp;
+            const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / fm2.rhog;
+            // sedimentation apply (src/mod_precip_transport.cpp:960-989)
+            double e = re0 + fm2.drhoe + fm2.drhophi;
+            const double kinh_u = kinh + fm2.drhokinh;
+            const double kinv_c = fm2.kinv + fm2.drhokinv;
+            const double kinv_cp = fm1.kinv + fm1.drhokinv;
+            const double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
+            const double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
+            const double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
+            const double rw_n = fm2.rhogw + fm2.drhogw;
+            double rg = fm2.rhog;
+            // nq = 1..5: QC(0), QR, QI(0), QS, QG
+            rq[QC] = rq[QC] + 0.0;
+            rq[QR] = rq[QR] + fm2.drhoq[0];
+            rg = rg + fm2.drhoq[0];
+            e = e - fm2.drhoq[0] * C_GRAV * z;
+            rq[QI] = rq[QI] + 0.0;
+            rq[QS] = rq[QS] + fm2.drhoq[1];
+            rg = rg + fm2.drhoq[1];
+            e = e - fm2.drhoq[1] * C_GRAV * z;
+            rq[QG] = rq[QG] + fm2.drhoq[2];
+            rg = rg + fm2.drhoq[2];
+            e = e - fm2.drhoq[2] * C_GRAV * z;
+            rq[QV] = rq[QV] + 0.0;
+            // KIN2EIN (:991-1010)
+            const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
+            const double rg_cp = ((fm1.rhog + fm1.drhoq[0]) + fm1.drhoq[1]) + fm1.drhoq[2];
+            const double rw_cp = fm1.rhogw + fm1.drhogw;
+            const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
+            const double kinv_new_c =
+                (c >= KMIN + 1) ? 0.5 * (rw_n * rw_n) / (fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev) : 0.0;
+            const double kinv_new_cp =
+                (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (fm1.c2w0 * rg_cp + fm1.c2w1 * rg) : 0.0;
+            const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
+            e = e + (tmp2 - tmpk);
+            rhog_new_prev = rg;
+            if (GENERAL) {
+                mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
+                mom[1 * s.K2 + (int64_t)c * ijdim] = rvy_n;
+                mom[2 * s.K2 + (int64_t)c * ijdim] = rvz_n;
+                mom[3 * s.K2 + (int64_t)c * ijdim] = rw_n;
+            } else {
+                f.rhogvx[o] = rvx_n;
+                f.rhogvy[o] = rvy_n;
+                f.rhogvz[o] = rvz_n;
+                f.rhogw[o] = rw_n;
+            }
+            finalize_point<true>(s, c, rg, rq, e, gs, &fail);
+        }
+
+        // ---- shift the window
+        fm2 = fm1;
+        fm1 = f0;
+        f0 = fp1;
+        cur = nxt;
+        rhog_cur = rhog_nxt;
+        gs_cur = gs_nxt;
+#pragma unroll
+        for (int t = 0; t < 3; t++) q_cur[t] = q_nxt[t];
+    }
+    // After the loop (w = 95 handled): cells up to 93 finalised. Cell 94 remains,
+    // with cell 95 (halo) as its upper neighbour: fm2 = cell 94, fm1 = cell 95.
+    {
+        const int c = KMAX;
+        const int64_t o = s.p2 + (int64_t)c * ijdim;
+        double rq[6];
+#pragma unroll
+        for (int n = 0; n < 6; n++) rq[n] = a.ws.rhogq[s.p3 + (int64_t)(n * KALL + c) * ijdim];
+        const double re0 = a.ws.rhoge[o];
+        double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
+        const double z = f.z[o], gs = f.gsgam2[o];
+        const double gshp = f.gsgam2h[o + ijdim];
+        const double w2c0 = g.cfact[c] * gs / gshp, w2c1 = g.dfact[c] * gs / gshp;
+        const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / fm2.rhog;
+        double e = re0 + fm2.drhoe + fm2.drhophi;
+        const double kinh_u = kinh + fm2.drhokinh;
+        const double kinv_c = fm2.kinv + fm2.drhokinv;
+        const double kinv_cp = 0.0 + 0.0;    // wall 95
+        const double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
+        const double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
+        const double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
+        const double rw_n = fm2.rhogw + fm2.drhogw;
+        double rg = fm2.rhog;
+        rq[QC] = rq[QC] + 0.0;
+        rq[QR] = rq[QR] + fm2.drhoq[0];
+        rg = rg + fm2.drhoq[0];
+        e = e - fm2.drhoq[0] * C_GRAV * z;
+        rq[QI] = rq[QI] + 0.0;
+        rq[QS] = rq[QS] + fm2.drhoq[1];
+        rg = rg + fm2.drhoq[1];
+        e = e - fm2.drhoq[1] * C_GRAV * z;
+        rq[QG] = rq[QG] + fm2.drhoq[2];
+        rg = rg + fm2.drhoq[2];
+        e = e - fm2.drhoq[2] * C_GRAV * z;
+        rq[QV] = rq[QV] + 0.0;
+        const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
+        const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
+        const double kinv_new_c = 0.5 * (rw_n * rw_n) / (fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev);
+        const double tmpk = kinh_new + (w2c0 * 0.0 + w2c1 * kinv_new_c);
+        e = e + (tmp2 - tmpk);
+        if (GENERAL) {
+            mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
+            mom[1 * s.K2 + (int64_t)c * ijdim] = rvy_n;
+            mom[2 * s.K2 + (int64_t)c * ijdim] = rvz_n;
+            mom[3 * s.K2 + (int64_t)c * ijdim] = rw_n;
+        } else {
+            f.rhogvx[o] = rvx_n;
+            f.rhogvy[o] = rvy_n;
+            f.rhogvz[o] = rvz_n;
+            f.rhogw[o] = rw_n;
+        }
+        finalize_point<true>(s, c, rg, rq, e, gs, &fail);
+    }
+    // halo cells 0 and 95: zero tendencies, no KIN2EIN, no Newton, no filter
+#pragma unroll
+    for (int hc = 0; hc < 2; hc++) {
+        const int c = hc ? (KALL - 1) : 0;
+        const int64_t o = s.p2 + (int64_t)c * ijdim;
+        double rq[6];
+#pragma unroll
+        for (int n = 0; n < 6; n++) rq[n] = a.ws.rhogq[s.p3 + (int64_t)(n * KALL + c) * ijdim] + 0.0;
+        const double z = f.z[o];
+        double e = a.ws.rhoge[o] + 0.0 + 0.0;
+        e = e - 0.0 * C_GRAV * z;
+        double rg = a.ws.rhog[o] + 0.0;
+        const double rvx_n = f.rhogvx[o] + 0.0 * s.ix + 0.0 * s.jx;
+        const double rvy_n = f.rhogvy[o] + 0.0 * s.iy + 0.0 * s.jy;
+        const double rvz_n = f.rhogvz[o] + 0.0 * s.iz + 0.0 * s.jz;
+        const double rw_n = f.rhogw[o] + 0.0;
+        if (GENERAL) {
+            mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
+            mom[1 * s.K2 + (int64_t)c * ijdim] = rvy_n;
+            mom[2 * s.K2 + (int64_t)c * ijdim] = rvz_n;
+            mom[3 * s.K2 + (int64_t)c * ijdim] = rw_n;
+        } else {
+            f.rhogvx[o] = rvx_n;
+            f.rhogvy[o] = rvy_n;
+            f.rhogvz[o] = rvz_n;
+            f.rhogw[o] = rw_n;
+        }
+        f.GPREC[o] = (0.0 + 0.0) * 1.0;
+        finalize_point<false>(s, c, rg, rq, e, f.gsgam2[o], &fail);
+    }
+    if (GENERAL) {
+        for (int c = 0; c < KALL; c++) {
+            const int64_t o = s.p2 + (int64_t)c * ijdim;
+            f.rhogvx[o] = mom[0 * s.K2 + (int64_t)c * ijdim];
+            f.rhogvy[o] = mom[1 * s.K2 + (int64_t)c * ijdim];
+            f.rhogvz[o] = mom[2 * s.K2 + (int64_t)c * ijdim];
+            f.rhogw[o] = mom[3 * s.K2 + (int64_t)c * ijdim];
+        }
+    }
+    // surface fluxes (precip_transport) then mp_driver's sum * fraction_mp
+    const int64_t pp = (int64_t)l * 2 * ijdim + ij;
+    f.precip[pp] = (0.0 + precip0) * 1.0;
+    f.precip[pp + ijdim] = (0.0 + precip1) * 1.0;
+    f.ISO1_precip[pp] = 0.0;
+    f.ISO1_precip[pp + ijdim] = 0.0;
+    f.ISO2_precip[pp] = 0.0;
+    f.ISO2_precip[pp + ijdim] = 0.0;
+    f.precip_rhoe[pc] = (0.0 + prhoe) * 1.0;
+    f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
+    f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
+    f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
+    if (fail) atomicOr(&a.ws.report[0], 1);
+}
+
+// ---------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------
+extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipStream_t stream,
+                                       hipEvent_t *ev /* 5 events or NULL */)
+{
+    const StepArgs &a = *args;
+    const dim3 grid_r((a.ijdim + 255) / 256, KALL, a.lall);
+    const dim3 grid_c((a.ijdim + 63) / 64, a.lall);
+    if (ev) (void)hipEventRecord(ev[0], stream);
+    if (precision == 1)
+        hipLaunchKernelGGL(k_rates<float>, grid_r, dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_rates<double>, grid_r, dim3(256), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[1], stream);
+    hipLaunchKernelGGL(k_extremes, grid_c, dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[2], stream);
+    hipLaunchKernelGGL(k_sweep<false>, grid_c, dim3(64), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL(k_sweep<true>, grid_c, dim3(64), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[4], stream);
+    return hipGetLastError();
+}

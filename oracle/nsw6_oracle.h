@@ -1,0 +1,70 @@
+/*
+ * nsw6_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU FP64 restatement of the reference NSW6 microphysics step, used as the
+ * correctness oracle by tests/, by __graft_entry__.smoke() and as the CPU baseline
+ * of bench.py.  The product library never links, loads or calls anything here.
+ *
+ * Every function follows the reference source phase by phase (file:line cited at
+ * each definition in nsw6_oracle.c).  Arrays use the public layout of nsw6.h.
+ */
+#ifndef NSW6_ORACLE_H
+#define NSW6_ORACLE_H
+
+#include "../include/nsw6.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Outputs of one TagBigLoop point (src/mod_mp_nsw6.cpp:760-1594). */
+typedef struct ora_bigloop_out {
+    double drhogq[6];     /* drhogqv..drhogqg, tracer order QV,QC,QR,QI,QS,QG */
+    double Vt[6];         /* Vt(QR), Vt(QI), Vt(QS), Vt(QG); QV, QC left 0     */
+    double rceff, rceff_cld;
+    double rctop, tctop, rctop_cld, tctop_cld;
+    double wk[49];        /* final process-rate work array                    */
+    double q_t[6];        /* final tendencies qv_t..qg_t                       */
+} ora_bigloop_out;
+
+double ora_gammafunc(double x);
+void   ora_params_init(nsw6_params *p);
+void   ora_grid_setup(const double *gz, const double *gzh, nsw6_grid *g);
+double ora_psat_liq(double tem);
+double ora_psat_ice(double tem);
+void   ora_bergeron_point(double tem, double *a1, double *a2, double *ma2);
+void   ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp,
+                         const double q6[6], double qsatl, double qsati, double pre,
+                         double rhog, double Nc, double a1, double a2, double ma2,
+                         ora_bigloop_out *o);
+
+/* vadv1d on one region, arrays [k][ij] (src/mod_vadv1d.cpp:231-450). */
+void ora_vadv1d_prep(int ijdim, int mkmin, int mkmax, const double *dz, const double *zh,
+                     const double *wp, double *zdis, int *kcell, int *kcell_max,
+                     int *kcell_min, double dt);
+void ora_vadv1d_getflux(int ijdim, int mkmin, int mkmax, const double *dz,
+                        const double *rhof, const double *zdis0, const int *kcell,
+                        const int *kcell_max, const int *kcell_min, double *frhof);
+
+/* satadjust_liq Newton on a single point: returns iterations used (100 = not
+ * converged), updates *tem, *qv, *qc (src/mod_satadjust.cpp:828-894). */
+int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_loc,
+                            const double q6[6], double *tem, double *qv, double *qc);
+
+/* mp_driver over lall regions of host arrays (src/mod_mp_driver.cpp:335-613).
+ * `ice_adjust` < 0 keeps the reference choice (false under Roh & Satoh);
+ * 0/1 force satadjust_liq/satadjust_all.  Returns NSW6_OK or a failure status
+ * and fills *rep (may be NULL). */
+int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
+                  const nsw6_fields *f, double dt, double ct, int ice_adjust,
+                  nsw6_report *rep);
+
+/* Seconds of the last ora_mp_driver call (wall clock, like the reference's
+ * "NSW6 Elapsed Time", src/mod_mp_driver.cpp:500-552). */
+double ora_last_elapsed(void);
+int    ora_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
