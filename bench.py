@@ -94,8 +94,8 @@ def main():
     wall = time.perf_counter() - t0
     rep = drv.check()
     step_ms, nev = drv.timing_read(0)
-    pre_ms, _ = drv.timing_read(1)
-    sweep_ms, _ = drv.timing_read(2)
+    kernel_ms = {name: drv.timing_read(i)[0] for i, name in
+                 ((1, "rates"), (2, "extremes+bounds"), (3, "sweep"), (4, "adjust"))}
     drv.timing(False)
 
     ms_per_step = wall * 1e3 / args.steps
@@ -122,12 +122,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": wname, "regions": w.lall, "columns_per_region": w.ijdim, "levels": 96,
                    "columns": ncol, "dt_s": w.dt, "tracers": 6},
-        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_sweep)",
+        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_sweep + k_adjust)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "event_ms": step_ms, "launches_timed": nev},
-        "kernels_ms": {"step": step_ms, "rates": step_ms - pre_ms - sweep_ms if step_ms else None,
-                       "extremes+bounds": pre_ms, "sweep": sweep_ms},
+        "kernels_ms": kernel_ms,
         "general_regions": rep.n_general_regions,
     }
     if not args.no_cpu:

@@ -15,7 +15,8 @@ using namespace nsw6;
 
 namespace {
 
-constexpr int N_EVENTS = 5;   // rates | extremes+bounds | sweep fast | sweep general | end
+constexpr int N_EVENTS = NSW6_STEP_EVENTS;
+constexpr int N_TIMERS = 5;   // step, rates, extremes+bounds, sweep, adjust
 
 struct PendingTiming {
     hipEvent_t ev[N_EVENTS];
@@ -35,7 +36,7 @@ struct nsw6_ctx {
     int last_lall = 0;
     bool timing = false;
     std::vector<PendingTiming> pending;
-    double t_sum[3] = {0, 0, 0};   // whole step, extremes pre-pass, fused sweep
+    double t_sum[N_TIMERS] = {0, 0, 0, 0, 0};
     int64_t t_count = 0;
 };
 
@@ -81,23 +82,26 @@ static void to_dev_params(const nsw6_params &p, DevParams &d)
 static void free_ws(nsw6_ctx *c)
 {
     Workspace &w = c->ws;
-    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.mom, w.ext, w.bounds, w.region_flag, w.report};
+    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.mom, w.part, w.bounds, w.region_flag, w.report};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     memset(&w, 0, sizeof w);
 }
 
+// Per-region buffers: kcell bounds, region flags and the per-wavefront partial
+// extremes (sized for any lall * ijdim <= max_columns split into `cap` regions).
 static int ensure_regions(nsw6_ctx *c, int lall)
 {
     if (lall <= c->region_capacity) return NSW6_OK;
     int cap = c->region_capacity ? c->region_capacity : 64;
     while (cap < lall) cap *= 2;
     Workspace &w = c->ws;
-    if (w.ext) (void)hipFree(w.ext);
+    if (w.part) (void)hipFree(w.part);
     if (w.bounds) (void)hipFree(w.bounds);
     if (w.region_flag) (void)hipFree(w.region_flag);
-    w.ext = nullptr; w.bounds = nullptr; w.region_flag = nullptr;
-    if (hipMalloc(&w.ext, sizeof(unsigned long long) * EXT_PER_REGION * (size_t)cap) != hipSuccess ||
+    w.part = nullptr; w.bounds = nullptr; w.region_flag = nullptr;
+    const size_t waves = (size_t)(c->max_columns / EXT_COLS_PER_WAVE) + (size_t)cap;
+    if (hipMalloc(&w.part, sizeof(double) * (size_t)KALL * 12 * waves) != hipSuccess ||
         hipMalloc(&w.bounds, sizeof(int) * BOUNDS_PER_REGION * (size_t)cap) != hipSuccess ||
         hipMalloc(&w.region_flag, sizeof(int) * (size_t)cap) != hipSuccess) {
         c->region_capacity = 0;
@@ -138,6 +142,10 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
         gh.gz_shift[k] = 0.0;
     }
     for (int k = KMIN; k <= KMAX + 1; k++) gh.gz_shift[k] = g.gz[k - 1];   // precip_transport :705-708
+    for (int k = 0; k < KALL; k++) {
+        gh.rsum_gz[k] = (k >= 1) ? 1.0 / (g.dgz[k - 1] + g.dgz[k]) : 0.0;
+        gh.rsum_gzh[k] = (k >= 1) ? 1.0 / (g.dgzh[k - 1] + g.dgzh[k]) : 0.0;
+    }
 
     const size_t K2 = (size_t)KALL * (size_t)max_columns;
     Workspace &w = c->ws;
@@ -226,6 +234,7 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     a.dt = dt;
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
+    a.dt3_6 = a.dt3 / 6.0;
 
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t *ev = nullptr;
@@ -301,13 +310,12 @@ static int drain_timing(nsw6_ctx *ctx)
 {
     for (PendingTiming &pt : ctx->pending) {
         if (hipEventSynchronize(pt.ev[N_EVENTS - 1]) != hipSuccess) return NSW6_HIP_ERROR;
-        float ms_all = 0, ms_ext = 0, ms_sw = 0;
-        (void)hipEventElapsedTime(&ms_all, pt.ev[0], pt.ev[N_EVENTS - 1]);
-        (void)hipEventElapsedTime(&ms_ext, pt.ev[1], pt.ev[2]);
-        (void)hipEventElapsedTime(&ms_sw, pt.ev[2], pt.ev[3]);
-        ctx->t_sum[0] += ms_all;
-        ctx->t_sum[1] += ms_ext;
-        ctx->t_sum[2] += ms_sw;
+        const int from[N_TIMERS] = {0, 0, 1, 2, 3}, to[N_TIMERS] = {4, 1, 2, 3, 4};
+        for (int i = 0; i < N_TIMERS; i++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, pt.ev[from[i]], pt.ev[to[i]]);
+            ctx->t_sum[i] += ms;
+        }
         ctx->t_count++;
         for (int i = 0; i < N_EVENTS; i++) (void)hipEventDestroy(pt.ev[i]);
     }
@@ -324,7 +332,7 @@ int nsw6_timing_enable(nsw6_ctx *ctx, int on)
 
 int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count)
 {
-    if (!ctx || which < 0 || which > 2 || !avg_ms) return NSW6_BAD_ARG;
+    if (!ctx || which < 0 || which >= N_TIMERS || !avg_ms) return NSW6_BAD_ARG;
     int st = drain_timing(ctx);
     if (st != NSW6_OK) return st;
     *avg_ms = ctx->t_count ? ctx->t_sum[which] / (double)ctx->t_count : 0.0;
@@ -336,7 +344,7 @@ int nsw6_timing_reset(nsw6_ctx *ctx)
 {
     if (!ctx) return NSW6_BAD_ARG;
     int st = drain_timing(ctx);
-    ctx->t_sum[0] = ctx->t_sum[1] = ctx->t_sum[2] = 0.0;
+    for (int i = 0; i < N_TIMERS; i++) ctx->t_sum[i] = 0.0;
     ctx->t_count = 0;
     return st;
 }

@@ -15,7 +15,14 @@
 //   * log(Xs2+zerosw) of the seven Roh & Satoh moments is evaluated once,
 //   * logs/powers of constants (log(mi40), log(mi50), Nc^-1.79, log(Nc/2000)) are
 //     computed once on the host,
-//   * the dead first assignment of Praut_kk (overwritten at :1154) is dropped.
+//   * the dead first assignment of Praut_kk (overwritten at :1154) is dropped,
+//   * exp(ln10*a)*exp(lx*b) of the Roh moments is exp(ln10*a + lx*b) and pow(x,y)
+//     with real y is exp(y*log(x)) (a few ulp),
+//   * terms multiplied by a switch that is exactly zero for the whole step are not
+//     evaluated when their value is finite by construction: the Bergeron growth
+//     (x sw_bergeron = 1 - sw_expice), Berry autoconversion and the original
+//     Pracw (x (1 - sw_kk2000)), the non-Roh snow moments (x (1 - sw_roh2014)),
+//     and the QI fall speed (QI does not sediment under "3WATER").
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -162,27 +169,31 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R RLMDr_5dr = RLMDr_2br * RLMDr_dr;
     const R RLMDr_6dr = RLMDr_3br * RLMDr_dr;
 
-    // slope parameter lambda (snow) and its moments
-    zerosw = step_neg(qs - R(1.0E-12));
-    const R RLMDs = sqrt(sqrt(dens * qs / (As * N0s * R(P.GAM_1bs)) + zerosw)) * (R(1) - zerosw);
-    const R RLMDs_ds = sqrt(sqrt(RLMDs));
-    const R RLMDs_2 = RLMDs * RLMDs;
-    const R RLMDs_3 = RLMDs_2 * RLMDs;
-    const R RLMDs_1bs = RLMDs_2 * RLMDs_2;
-    const R RLMDs_2bs = RLMDs_1bs * RLMDs;
-    const R RLMDs_3bs = RLMDs_3 * RLMDs_3;
-    const R RLMDs_3ds = RLMDs_3 * RLMDs_ds;
-    const R RLMDs_5ds = RLMDs_2bs * RLMDs_ds;
-
-    R MOMs_0 = N0s * GAM * RLMDs;
-    R MOMs_1 = N0s * GAM_2 * RLMDs_2;
-    R MOMs_2 = N0s * GAM_3 * RLMDs_3;
-    R MOMs_0bs = N0s * R(P.GAM_1bs) * RLMDs_1bs;
-    R MOMs_1bs = N0s * R(P.GAM_2bs) * RLMDs_2bs;
-    R MOMs_2bs = N0s * R(P.GAM_3bs) * RLMDs_3bs;
-    R MOMs_2ds = N0s * R(P.GAM_3ds) * RLMDs_3ds;
-    R MOMs_5ds_h = N0s * R(P.GAM_5ds_h) * sqrt(RLMDs_5ds);
-    R RMOMs_Vt = R(P.GAM_1bsds) / R(P.GAM_1bs) * RLMDs_ds;
+    // slope parameter lambda (snow) and its moments (only the non-Roh form needs them)
+    const bool roh = (P.sw_roh2014 == 1.0);
+    R MOMs_0 = R(0), MOMs_1 = R(0), MOMs_2 = R(0), MOMs_0bs = R(0), MOMs_1bs = R(0), MOMs_2bs = R(0);
+    R MOMs_2ds = R(0), MOMs_5ds_h = R(0), RMOMs_Vt = R(0);
+    if (!roh) {
+        zerosw = step_neg(qs - R(1.0E-12));
+        const R RLMDs = sqrt(sqrt(dens * qs / (As * N0s * R(P.GAM_1bs)) + zerosw)) * (R(1) - zerosw);
+        const R RLMDs_ds = sqrt(sqrt(RLMDs));
+        const R RLMDs_2 = RLMDs * RLMDs;
+        const R RLMDs_3 = RLMDs_2 * RLMDs;
+        const R RLMDs_1bs = RLMDs_2 * RLMDs_2;
+        const R RLMDs_2bs = RLMDs_1bs * RLMDs;
+        const R RLMDs_3bs = RLMDs_3 * RLMDs_3;
+        const R RLMDs_3ds = RLMDs_3 * RLMDs_ds;
+        const R RLMDs_5ds = RLMDs_2bs * RLMDs_ds;
+        MOMs_0 = N0s * GAM * RLMDs;
+        MOMs_1 = N0s * GAM_2 * RLMDs_2;
+        MOMs_2 = N0s * GAM_3 * RLMDs_3;
+        MOMs_0bs = N0s * R(P.GAM_1bs) * RLMDs_1bs;
+        MOMs_1bs = N0s * R(P.GAM_2bs) * RLMDs_2bs;
+        MOMs_2bs = N0s * R(P.GAM_3bs) * RLMDs_3bs;
+        MOMs_2ds = N0s * R(P.GAM_3ds) * RLMDs_3ds;
+        MOMs_5ds_h = N0s * R(P.GAM_5ds_h) * sqrt(RLMDs_5ds);
+        RMOMs_Vt = R(P.GAM_1bsds) / R(P.GAM_1bs) * RLMDs_ds;
+    }
 
     // Roh and Satoh (2014) bimodal snow moments
     const R Xs2 = dens * qs / As;
@@ -200,37 +211,52 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R sw_roh = R(P.sw_roh2014), ln10 = R(P.ln10);
     const R lx = log(Xs2 + zerosw);
     const R omz = R(1) - zerosw;
+    // moment of order nm: 10^loga(nm) * Xs2^b(nm)
 #define NSW6_MOM(NM, OUT)                                                                        \
     {                                                                                            \
         const R nm = R(NM);                                                                      \
         const R loga_ = at[0] + nm * (at[1] + nm * (at[2] + nm * at[3]));                        \
         const R b_ = bt[0] + nm * (bt[1] + nm * (bt[2] + nm * bt[3]));                           \
-        OUT = exp(ln10 * loga_) * exp(lx * b_);                                                  \
+        OUT = exp(ln10 * loga_ + lx * b_);                                                       \
     }
     R m;
-    m = exp(ln10 * at[0]) * exp(lx * bt[0]);
-    MOMs_0 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0;
-    NSW6_MOM(1.0, m)
-    MOMs_1 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1;
-    MOMs_2 = sw_roh * Xs2 + (R(1) - sw_roh) * MOMs_2;
-    NSW6_MOM(2.0, m)
-    MOMs_0bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0bs;
-    NSW6_MOM(3.0, m)
-    MOMs_1bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1bs;
-    NSW6_MOM(4.0, m)
-    MOMs_2bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2bs;
-    NSW6_MOM(2.25, m)
-    MOMs_2ds = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2ds;
-    NSW6_MOM(1.625, m)
-    MOMs_5ds_h = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_5ds_h;
-    // RMOMs_Vt uses the same (nm = 2.25) moment as MOMs_2ds
-    {
-        const R nm = R(2.25);
-        const R loga_ = at[0] + nm * (at[1] + nm * (at[2] + nm * at[3]));
-        const R b_ = bt[0] + nm * (bt[1] + nm * (bt[2] + nm * bt[3]));
-        m = exp(ln10 * loga_) * exp(lx * b_);
+    if (roh) {
+        m = exp(ln10 * at[0] + lx * bt[0]);
+        MOMs_0 = m * omz;
+        NSW6_MOM(1.0, m)
+        MOMs_1 = m * omz;
+        MOMs_2 = Xs2;
+        NSW6_MOM(2.0, m)
+        MOMs_0bs = m * omz;
+        NSW6_MOM(3.0, m)
+        MOMs_1bs = m * omz;
+        NSW6_MOM(4.0, m)
+        MOMs_2bs = m * omz;
+        NSW6_MOM(2.25, m)
+        MOMs_2ds = m * omz;
+        NSW6_MOM(1.625, m)
+        MOMs_5ds_h = m * omz;
+        // RMOMs_Vt uses the (nm = 2.25) moment of MOMs_2ds
+        RMOMs_Vt = MOMs_2ds / (MOMs_0bs + zerosw);
+    } else {
+        m = exp(ln10 * at[0] + lx * bt[0]);
+        MOMs_0 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0;
+        NSW6_MOM(1.0, m)
+        MOMs_1 = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1;
+        MOMs_2 = sw_roh * Xs2 + (R(1) - sw_roh) * MOMs_2;
+        NSW6_MOM(2.0, m)
+        MOMs_0bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_0bs;
+        NSW6_MOM(3.0, m)
+        MOMs_1bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_1bs;
+        NSW6_MOM(4.0, m)
+        MOMs_2bs = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2bs;
+        NSW6_MOM(2.25, m)
+        const R m225 = m;
+        MOMs_2ds = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_2ds;
+        NSW6_MOM(1.625, m)
+        MOMs_5ds_h = sw_roh * m * omz + (R(1) - sw_roh) * MOMs_5ds_h;
+        RMOMs_Vt = sw_roh * m225 * omz / (MOMs_0bs + zerosw) + (R(1) - sw_roh) * RMOMs_Vt;
     }
-    RMOMs_Vt = sw_roh * m * omz / (MOMs_0bs + zerosw) + (R(1) - sw_roh) * RMOMs_Vt;
 #undef NSW6_MOM
 
     // slope parameter lambda (graupel)
@@ -242,11 +268,8 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R RLMDg_3dg = RLMDg_3 * RLMDg_dg;
     const R RLMDg_5dg = (RLMDg_3 * RLMDg_2) * RLMDg_dg;
 
-    // terminal velocity
-    zerosw = step_neg(qi - R(1.0E-8));
-    const R sw_constVti = R(P.sw_constVti);
-    const R Vti = (R(1) - sw_constVti) * R(-3.29) * exp(log(dens * qi + zerosw) * R(0.16)) * (R(1) - zerosw) +
-                  (sw_constVti) * (-R(P.CONST_Vti));
+    // terminal velocity (QI does not sediment under "3WATER": its fall speed is unused)
+    const R Vti = R(0);
     const R Vtr = -Cr * rho_fact * R(P.GAM_1brdr) / R(P.GAM_1br) * RLMDr_dr;
     const R Vts = -Cs * rho_fact * RMOMs_Vt;
     const R Vtg = -Cg * rho_fact * R(P.GAM_1bgdg) / R(P.GAM_1bg) * RLMDg_dg;
@@ -261,11 +284,17 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R Esi_mod = dmin(Esi, Esi * exp(R(P.gamma_sacr) * temc));
     const R Egs_mod = dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
     const R GAM_3dr = R(P.GAM_3dr), GAM_3dg = R(P.GAM_3dg);
-    const R Pracw_orig = qc * R(0.25) * PI * R(P.Erw) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
+    const bool kk = (P.sw_kk2000 == 1.0);
     zerosw = step_neg(qc * qr - R(1.0E-12));
     const R Pracw_kk = R(67) * exp(log(qc * qr + zerosw) * R(1.15)) * (R(1) - zerosw);
     const R sw_kk = R(P.sw_kk2000);
-    R Pracw = (R(1) - sw_kk) * Pracw_orig + (sw_kk)*Pracw_kk;
+    R Pracw;
+    if (kk) {
+        Pracw = Pracw_kk;
+    } else {
+        const R Pracw_orig = qc * R(0.25) * PI * R(P.Erw) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
+        Pracw = (R(1) - sw_kk) * Pracw_orig + (sw_kk)*Pracw_kk;
+    }
     R Psacw = qc * R(0.25) * PI * R(P.Esw) * Cs * MOMs_2ds * rho_fact;
     R Pgacw = qc * R(0.25) * PI * R(P.Egw) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
     R Praci = qi * R(0.25) * PI * R(P.Eri) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
@@ -287,10 +316,15 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     // autoconversion
     const R Nc = R(P.Nc_def);
     const R rhoqc = dens * qc * R(1000);
-    const R Dc = R(P.Dc);
-    const R Praut_berry = Rdens * R(1.67E-5) * rhoqc * rhoqc / (R(5) + R(3.66E-2) * Nc / (Dc * rhoqc + EPS));
-    const R Praut_kk = R(P.kk_aut_rate) * pow(qc, R(2.47)) * R(P.Nc_pow_m179);
-    R Praut = (R(1) - sw_kk) * Praut_berry + sw_kk * Praut_kk;
+    const R Praut_kk = R(P.kk_aut_rate) * exp(R(2.47) * log(qc)) * R(P.Nc_pow_m179);
+    R Praut;
+    if (kk) {
+        Praut = Praut_kk;
+    } else {
+        const R Dc = R(P.Dc);
+        const R Praut_berry = Rdens * R(1.67E-5) * rhoqc * rhoqc / (R(5) + R(3.66E-2) * Nc / (Dc * rhoqc + EPS));
+        Praut = (R(1) - sw_kk) * Praut_berry + sw_kk * Praut_kk;
+    }
     const R beta_saut = R(P.beta_saut), beta_gaut = R(P.beta_gaut);
     const R betai = dmin(beta_saut, beta_saut * exp(R(P.gamma_saut) * temc));
     R Psaut = dmax(betai * (qi - R(P.qicrt_saut)), R(0));
@@ -338,12 +372,15 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     Pgmlt = dmax(Pgmlt, R(0));
     R Pgfrz = R(2) * PI * Rdens * N0r * R(60) * B_frz * Ar * (exp(-A_frz * temc) - R(1)) * RLMDr_7;
 
-    // Bergeron process
-    const R mi50 = R(P.mi50);
-    const R dt1 = (exp(R(P.ln_mi50) * in.ma2) - exp(R(P.ln_mi40) * in.ma2)) / (in.a1 * in.ma2);
-    const R Ni50 = qi * dt / (mi50 * dt1);
-    R Psfw = Ni50 * (in.a1 * pow(mi50, in.a2) + PI * R(P.Eiw) * dens * qc * R(P.Ri50) * R(P.Ri50) * R(P.vti50));
-    R Psfi = qi / dt1;
+    // Bergeron process (zeroed by sw_bergeron = 0 when explicit ice generation is on)
+    R Psfw = R(0), Psfi = R(0);
+    if (P.sw_expice != 1.0) {
+        const R mi50 = R(P.mi50);
+        const R dt1 = (exp(R(P.ln_mi50) * in.ma2) - exp(R(P.ln_mi40) * in.ma2)) / (in.a1 * in.ma2);
+        const R Ni50 = qi * dt / (mi50 * dt1);
+        Psfw = Ni50 * (in.a1 * exp(in.a2 * R(P.ln_mi50)) + PI * R(P.Eiw) * dens * qc * R(P.Ri50) * R(P.Ri50) * R(P.vti50));
+        Psfi = qi / dt1;
+    }
 
     // limiters
     Pigen = dmin(Pigen, dqv_dt) * (iceflg)*sw_expice;
@@ -460,7 +497,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     // effective radius of liquid water
     const R coef_xf = R(P.coef_xf);
     const R xf_qc = rhoqc / Nc * R(1.0E-9);
-    const R rf_qc = pow((coef_xf * xf_qc + EPS), R(0.33333333));
+    const R rf_qc = exp(R(0.33333333) * log(coef_xf * xf_qc + EPS));
     const R r2_qc = R(P.coef_dgam) * (rf_qc * rf_qc) * (Nc * R(1.0E+6));
     const R r2_qr = R(0.25) * N0r * GAM_3 * RLMDr_3;
     const R r3_qc = coef_xf * dens * qc;

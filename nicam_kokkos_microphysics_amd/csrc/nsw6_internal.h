@@ -27,10 +27,14 @@ struct GridDev {
     double gz[KALL], gzh[KALL], gz_shift[KALL];
     double dgz[KALL], dgzh[KALL], rdgz[KALL], rdgzh[KALL];
     double afact[KALL], bfact[KALL], cfact[KALL], dfact[KALL];
+    double rsum_gz[KALL], rsum_gzh[KALL];   // 1/(dgz(k-1)+dgz(k)), 1/(dgzh(k-1)+dgzh(k))
 };
 
 // Per-region reductions: [tracer 3][level type 2: full, half][k 96][max, min]
 constexpr int EXT_PER_REGION = 3 * 2 * KALL * 2;
+// k_extremes: one column per lane and one tracer per wavefront; per-wave partial
+// extremes are stored as [l][k][12][wave]
+constexpr int EXT_COLS_PER_WAVE = 64;
 constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
 __host__ __device__ constexpr int ext_index(int t, int type, int k, int mm)
 {
@@ -45,7 +49,7 @@ struct Workspace {
     double *rhog, *rhoge, *rhogq;   // state after the rate update (pre-sedimentation)
     double *vt;                     // fall speeds of QR, QS, QG  [l][3][k][ij]
     double *mom;                    // deferred momentum of the general sweep [l][4][k][ij]
-    unsigned long long *ext;        // [l][EXT_PER_REGION] order-preserving keys
+    double *part;                   // [l][KALL][12][waves per region] partial extremes
     int *bounds;                    // [l][BOUNDS_PER_REGION]
     int *region_flag;               // [l] 1 = region needs the general sweep
     int *report;                    // [REPORT_INTS]
@@ -57,10 +61,14 @@ struct StepArgs {
     DevParams P;
     const GridDev *grid;
     int lall, ijdim;
-    double dt, dt2, dt3;
+    double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 
 }  // namespace nsw6
+
+// events recorded by nsw6_launch_step: before k_rates, after k_rates, after
+// k_extremes+k_bounds, after the sweeps, after k_adjust
+constexpr int NSW6_STEP_EVENTS = 5;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev);

@@ -1,6 +1,6 @@
 // nsw6_kernels.hip -- HIP kernels of the NSW6 microphysics step for gfx950 (MI355X).
 //
-// One step = four launches on one stream:
+// One step = six launches on one stream:
 //   k_rates    point-wise over (l,k,ij): negative filter, psat/qsat, Bergeron, the
 //              49 process rates and the state update (src/mod_mp_nsw6.cpp:439-589);
 //              writes the pre-sedimentation state and the fall speeds to workspace.
@@ -10,10 +10,10 @@
 //   k_bounds   one thread per (region, tracer, level type, level): kcell_min/max
 //              search (:329-351) and the per-region choice of sweep.
 //   k_sweep    one lane per column, bottom to top: per-column departure cells,
-//              semi-Lagrangian fluxes (:393-450), flux divergences, surface fluxes,
-//              KIN2EIN, rediagnosis (src/mod_precip_transport.cpp:573-1087),
-//              saturation adjustment (src/mod_satadjust.cpp:274-362,771-896) and the
-//              final negative filter; every output is written once.
+//              semi-Lagrangian fluxes (:393-450), flux divergences, surface fluxes
+//              and KIN2EIN (src/mod_precip_transport.cpp:573-1011).
+//   k_adjust   point-wise: rediagnosis (:1074-1086), saturation adjustment
+//              (src/mod_satadjust.cpp:274-362,771-896) and the final negative filter.
 // All column loops keep ij fastest, so every per-level access of a wavefront is one
 // contiguous 512-byte segment.  No MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
@@ -47,9 +47,6 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 
     // Region-wide reductions of this step start from their neutral values.
     if (blockIdx.x == 0 && k == 0) {
-        unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
-        for (int i = threadIdx.x; i < EXT_PER_REGION; i += blockDim.x)
-            ext[i] = (i & 1) ? d2key(__builtin_inf()) : d2key(-__builtin_inf());
         if (threadIdx.x == 0) a.ws.region_flag[l] = 0;
         if (l == 0 && threadIdx.x < REPORT_INTS) a.ws.report[threadIdx.x] = (threadIdx.x >= 3) ? -1 : 0;
     }
@@ -110,8 +107,8 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     // ---- psat / qsat (TagSATWater), Bergeron
     const double qsatl = psat_liq(P, tem) / (rho * C_Rvap * tem);
     const double qsati = psat_ice(P, tem) / (rho * C_Rvap * tem);
-    double a1, a2, ma2;
-    bergeron(c_a1_tab, c_a2_tab, tem, a1, a2, ma2);
+    double a1 = 1.0, a2 = 0.0, ma2 = 1.0;   // only the Bergeron growth (off with explicit ice) uses them
+    if (P.sw_expice != 1.0) bergeron(c_a1_tab, c_a2_tab, tem, a1, a2, ma2);
 
     // ---- TagBigLoop
     RateIn<R> in;
@@ -158,20 +155,22 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 // ---------------------------------------------------------------------------------
 // departure distance of vadv1d_prep (src/mod_vadv1d.cpp:284-302)
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ double zdis_interior(double dt, double dt2, double dt3, double whm, double whk,
-                                                double whp, double dzm, double dzk)
+// zdis = dt*wh - dt^2*wh*dw/dz/2 + dt^3*wh*((dw/dz)^2 + wh*d2w/dz2)/6 with the grid
+// divisions taken as products with reciprocals computed once on the host
+__device__ __forceinline__ double zdis_interior(double dt, double dt2, double dt3_6, double whm, double whk,
+                                                double whp, double rdzm, double rdzk, double rs)
 {
-    const double s = dzm + dzk;
+    const double d = whp - whm;
     const double t1 = dt * whk;
-    const double t2 = dt2 * whk * (whp - whm) / s / 2.0;
-    const double g = (whp - whm) / s;
-    const double inner = ((whp - whk) / dzk - (whk - whm) / dzm) / s * 2.0;
-    const double t3 = dt3 * whk * (g * g + whk * inner) / 6.0;
+    const double t2 = dt2 * whk * d * rs * 0.5;
+    const double g = d * rs;
+    const double inner = ((whp - whk) * rdzk - (whk - whm) * rdzm) * rs * 2.0;
+    const double t3 = dt3_6 * whk * (g * g + whk * inner);
     return t1 - t2 + t3;
 }
-__device__ __forceinline__ double zdis_bottom(double dt, double dt2, double whk, double whp, double dzk)
+__device__ __forceinline__ double zdis_bottom(double dt, double dt2, double whk, double whp, double rdzk)
 {
-    return dt * whk - dt2 * whk * (whp - whk) / dzk / 2.0;
+    return dt * whk - dt2 * whk * (whp - whk) * rdzk * 0.5;
 }
 
 // Sliding window over the fall speed Vt(k) of one tracer that produces, level by
@@ -212,19 +211,20 @@ __device__ __forceinline__ void wind_shift(WindWindow &w, double vt_kp1, double 
     w.whh_km1 = w.whh_k;
     w.whh_k = whh_kp1;
 }
-__device__ __forceinline__ double zdis_full(const GridDev &g, double dt, double dt2, double dt3,
+__device__ __forceinline__ double zdis_full(const GridDev &g, double dt, double dt2, double dt3_6,
                                             const WindWindow &w, int k, double wh_kp1)
 {
-    return (k == KMIN) ? zdis_bottom(dt, dt2, w.wh_k, wh_kp1, g.dgz[KMIN])
-                       : zdis_interior(dt, dt2, dt3, w.wh_km1, w.wh_k, wh_kp1, g.dgz[k - 1], g.dgz[k]);
+    return (k == KMIN) ? zdis_bottom(dt, dt2, w.wh_k, wh_kp1, g.rdgz[KMIN])
+                       : zdis_interior(dt, dt2, dt3_6, w.wh_km1, w.wh_k, wh_kp1, g.rdgz[k - 1], g.rdgz[k],
+                                       g.rsum_gz[k]);
 }
-__device__ __forceinline__ double zdis_half(const GridDev &g, double dt, double dt2, double dt3,
+__device__ __forceinline__ double zdis_half(const GridDev &g, double dt, double dt2, double dt3_6,
                                             const WindWindow &w, int k, double whh_kp1)
 {
-    return (k == KMIN + 1) ? zdis_bottom(dt, dt2, w.whh_k, whh_kp1, g.dgzh[KMIN + 1])
-                           : zdis_interior(dt, dt2, dt3, w.whh_km1, w.whh_k, whh_kp1, g.dgzh[k - 1], g.dgzh[k]);
+    return (k == KMIN + 1) ? zdis_bottom(dt, dt2, w.whh_k, whh_kp1, g.rdgzh[KMIN + 1])
+                           : zdis_interior(dt, dt2, dt3_6, w.whh_km1, w.whh_k, whh_kp1, g.rdgzh[k - 1],
+                                           g.rdgzh[k], g.rsum_gzh[k]);
 }
-
 template <int N>
 __device__ __forceinline__ double wave_max(double v)
 {
@@ -241,50 +241,49 @@ __device__ __forceinline__ double wave_min(double v)
 }
 
 // ---------------------------------------------------------------------------------
-// k_extremes: one lane per column, one wavefront per block
+// k_extremes: one wavefront per block, one column per lane, one tracer per
+// wavefront (grid.y).  For every level the wavefront reduces its 64 columns and
+// stores the 4 extremes of its tracer ({full, half} x {max, min}) to its slot of a
+// partials array [l][k][12][wave]; k_bounds reduces the partials of a region.
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_extremes(StepArgs a)
 {
     const int ijdim = a.ijdim;
-    const int l = blockIdx.y;
-    const int ij = blockIdx.x * 64 + threadIdx.x;
+    const int t = blockIdx.y, l = blockIdx.z, wv = blockIdx.x;
+    const int nw = (ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE;
+    const int ij = wv * EXT_COLS_PER_WAVE + threadIdx.x;
     const bool live = ij < ijdim;
-    const int ijc = live ? ij : (ijdim - 1);
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const double *vt = a.ws.vt + (int64_t)l * 3 * K2 + ijc;
+    const double *vt = a.ws.vt + ((int64_t)l * 3 + t) * K2 + (live ? ij : ijdim - 1);
     const GridDev &g = *a.grid;
-    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3;
-    unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
+    double *part = a.ws.part + (int64_t)l * KALL * 12 * nw + wv;
 
-    WindWindow w[3];
-#pragma unroll
-    for (int t = 0; t < 3; t++) wind_init(w[t], vt[t * K2 + 1 * ijdim]);
+    WindWindow w;
+    wind_init(w, vt[1 * ijdim]);
+    double vn = vt[2 * (int64_t)ijdim];
     for (int k = KMIN; k <= KMAX; k++) {
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const double vt_kp1 = (k + 1 <= KMAX) ? vt[t * K2 + (int64_t)(k + 1) * ijdim] : 0.0;
-            double wh_kp1, vh_kp1, whh_kp1;
-            wind_next(w[t], k, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
-            const double zf = zdis_full(g, dt, dt2, dt3, w[t], k, wh_kp1);
-            const bool okf = live && (zf == zf);
-            double vmax = wave_max<64>(okf ? zf : -__builtin_inf());
-            double vmin = wave_min<64>(okf ? zf : __builtin_inf());
-            if (threadIdx.x == 0) {
-                atomicMax(&ext[ext_index(t, 0, k, 0)], d2key(vmax));
-                atomicMin(&ext[ext_index(t, 0, k, 1)], d2key(vmin));
-            }
-            if (k >= KMIN + 1) {
-                const double zh = zdis_half(g, dt, dt2, dt3, w[t], k, whh_kp1);
-                const bool okh = live && (zh == zh);
-                vmax = wave_max<64>(okh ? zh : -__builtin_inf());
-                vmin = wave_min<64>(okh ? zh : __builtin_inf());
-                if (threadIdx.x == 0) {
-                    atomicMax(&ext[ext_index(t, 1, k, 0)], d2key(vmax));
-                    atomicMin(&ext[ext_index(t, 1, k, 1)], d2key(vmin));
-                }
-            }
-            wind_shift(w[t], vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+        const double vc = vn;   // Vt(k+1); Vt(95) = 0 is stored by k_rates
+        if (k + 2 <= KALL - 1) vn = vt[(int64_t)(k + 2) * ijdim];
+        const double v = (k + 1 <= KMAX) ? vc : 0.0;
+        double wh, vh, whh;
+        wind_next(w, k, v, wh, vh, whh);
+        const double zf = zdis_full(g, dt, dt2, dt3, w, k, wh);
+        const bool okf = live && (zf == zf);
+        double e0 = wave_max<64>(okf ? zf : -__builtin_inf());
+        double e1 = wave_min<64>(okf ? zf : __builtin_inf());
+        double e2 = -__builtin_inf(), e3 = __builtin_inf();
+        if (k >= KMIN + 1) {
+            const double zh = zdis_half(g, dt, dt2, dt3, w, k, whh);
+            const bool okh = live && (zh == zh);
+            e2 = wave_max<64>(okh ? zh : -__builtin_inf());
+            e3 = wave_min<64>(okh ? zh : __builtin_inf());
         }
+        if (threadIdx.x < 4) {
+            const double v4 = (threadIdx.x == 0) ? e0 : (threadIdx.x == 1) ? e1 : (threadIdx.x == 2) ? e2 : e3;
+            part[((int64_t)k * 12 + t * 4 + threadIdx.x) * nw] = v4;
+        }
+        wind_shift(w, v, wh, vh, whh);
     }
 }
 
@@ -303,9 +302,14 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
     int kcmin = k, kcmax = k;
     int general = 0;
     if (k >= mkmin && k <= mkmax) {
-        const unsigned long long *ext = a.ws.ext + (size_t)l * EXT_PER_REGION;
-        const double amax = key2d(ext[ext_index(t, type, k, 0)]);
-        const double amin = key2d(ext[ext_index(t, type, k, 1)]);
+        const int nw = (a.ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE;
+        const double *pmax = a.ws.part + (((int64_t)l * KALL + k) * 12 + (t * 2 + type) * 2 + 0) * nw;
+        const double *pmin = pmax + nw;
+        double amax = -__builtin_inf(), amin = __builtin_inf();
+        for (int i = 0; i < nw; i++) {
+            amax = dmax(amax, pmax[i]);
+            amin = dmin(amin, pmin[i]);
+        }
         const double zzmax = (amax > DBL_MIN) ? amax : DBL_MIN;   // starts at numeric_limits::min()
         const double zzmin = (amin < DBL_MIN) ? amin : DBL_MIN;
         if (zzmax > 0.0) {
@@ -505,18 +509,47 @@ __device__ double general_hv(const SweepCtx &s, const GridDev &g, int j, int t, 
     return c.hv[t][i];
 }
 
-// Saturation adjustment of one point after sedimentation (wrapper of
-// src/mod_satadjust.cpp:312-361 with satadjust_liq :826-894), followed by the final
-// negative filter (interior only) and the output writes.
-template <bool INTERIOR>
-__device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double rhog, double rhogq[6], double rhoge,
-                                               double gsgam2, int *fail_flag)
+// ---------------------------------------------------------------------------------
+// k_adjust: the point-wise tail of the step after sedimentation -- rediagnosis
+// (src/mod_precip_transport.cpp:1074-1086), SATURATION_adjustment with
+// satadjust_liq (src/mod_satadjust.cpp:274-362, 771-896) and the final
+// negative_filter (src/mod_mp_nsw6.cpp:671, 1770-1831).  One thread per point.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adjust(StepArgs a)
 {
-    const StepArgs &a = *s.a;
+    const int ijdim = a.ijdim;
+    const int l = blockIdx.z, k = blockIdx.y;
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= ijdim) return;
     const Fields &f = a.f;
     const DevParams &P = a.P;
-    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
-    // rediagnosis at the end of precip_transport_new (:1074-1086)
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
+    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
+    const bool interior = (k >= KMIN && k <= KMAX);
+
+    // State after sedimentation.  The sweep wrote rhog, rhoge, QR, QS, QG of the
+    // interior cells; QV, QC, QI and the halo levels only receive zero tendencies.
+    double rhog, rhoge, rhogq[6];
+    if (interior) {
+        rhog = f.rhog[p];
+        rhoge = f.rhoge[p];
+        rhogq[QR] = f.rhogq[pq + QR * K2];
+        rhogq[QS] = f.rhogq[pq + QS * K2];
+        rhogq[QG] = f.rhogq[pq + QG * K2];
+    } else {
+        rhog = a.ws.rhog[p] + 0.0;
+        rhoge = a.ws.rhoge[p] + 0.0;
+        rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
+        rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
+        rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+    }
+    rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
+    rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
+    rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+    const double gsgam2 = f.gsgam2[p];
+
+    // rediagnosis at the end of precip_transport_new
     double q[6];
 #pragma unroll
     for (int n = 0; n < 6; n++) q[n] = rhogq[n] / rhog;
@@ -545,7 +578,7 @@ __device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double 
     for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
     const double rho_sa = rhog / gsgam2;
     tem = (ein_moist - q[QV] * C_LHV0) / CVtot;
-    if (INTERIOR) {
+    if (interior) {
         double qdl = 1.0;
 #pragma unroll
         for (int n = 1; n <= 5; n++) qdl = qdl - q[n];
@@ -581,12 +614,11 @@ __device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double 
             if (!converged) {
                 const int code = nan_break ? 2 : 1;
                 if (atomicAdd(&a.ws.report[code], 1) == 0) {
-                    // first failing point (lowest l, k, ij is not guaranteed; any one is)
-                    a.ws.report[3] = s.l;
-                    a.ws.report[4] = c;
-                    a.ws.report[5] = s.ij;
+                    a.ws.report[3] = l;
+                    a.ws.report[4] = k;
+                    a.ws.report[5] = ij;
                 }
-                *fail_flag = 1;
+                atomicOr(&a.ws.report[0], 1);
             }
         }
     }
@@ -597,8 +629,8 @@ __device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double 
     for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;
     rhoge = rhog * tem * CVtot2;
 
-    if (INTERIOR) {
-        // final negative_filter (src/mod_mp_nsw6.cpp:1770-1831)
+    if (interior) {
+        // final negative_filter
         double diffq = 0.0;
 #pragma unroll
         for (int n = 2; n <= 5; n++) {
@@ -625,21 +657,409 @@ __device__ __forceinline__ void finalize_point(const SweepCtx &s, int c, double 
         pre = rho * (qdn * C_Rdry + q[QV] * C_Rvap) * tem;
     }
 
-    f.rhog[o] = rhog;
-    f.rhoge[o] = rhoge;
-    f.rho[o] = rho;
-    f.tem[o] = tem;
-    f.pre[o] = pre;
-    f.qd[o] = qd;
+    f.rhog[p] = rhog;
+    f.rhoge[p] = rhoge;
+    f.rho[p] = rho;
+    f.tem[p] = tem;
+    f.pre[p] = pre;
+    f.qd[p] = qd;
 #pragma unroll
     for (int n = 0; n < 6; n++) {
-        const int64_t oq = s.p3 + (int64_t)(n * KALL + c) * s.ijdim;
-        f.rhogq[oq] = rhogq[n];
-        f.q[oq] = q[n];
+        f.rhogq[pq + n * K2] = rhogq[n];
+        f.q[pq + n * K2] = q[n];
     }
-    f.re_solid[o] = 20.0E-6;
-    f.GDCLW[o] = 0.0 * 1.0;
-    f.GDCFRC[o] = 0.0 * 1.0;
+    f.re_solid[p] = 20.0E-6;
+    f.GDCLW[p] = 0.0 * 1.0;
+    f.GDCFRC[p] = 0.0 * 1.0;
+}
+
+// Post-sedimentation writes of one interior cell: rhog, rhoge and the sedimenting
+// tracers (the rest of the state is finished by k_adjust).
+__device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, double rg, const double rq[6], double e)
+{
+    const Fields &f = s.a->f;
+    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
+    f.rhog[o] = rg;
+    f.rhoge[o] = e;
+    f.rhogq[s.p3 + (int64_t)(QR * KALL + c) * s.ijdim] = rq[QR];
+    f.rhogq[s.p3 + (int64_t)(QS * KALL + c) * s.ijdim] = rq[QS];
+    f.rhogq[s.p3 + (int64_t)(QG * KALL + c) * s.ijdim] = rq[QG];
+}
+
+// ---------------------------------------------------------------------------------
+// k_sweep_fast: the sweep for regions whose departure cells all lie in {k, k+1}
+// (checked by k_bounds).  Each semi-Lagrangian flux then has the closed form of
+// vadv1d_getflux_new for a range [k, k+1]:
+//     kc = k   : f = rhof(k) * zdis0
+//     kc = k+1 : f = -rhof(k) * dz(k) + rhof(k+1) * zdis0   (zdis0 kept, not refreshed)
+// and the half level uses the full level's kcell_max as the reference does.  Flux divergences of the non-mass quantities
+// are taken on tracer-summed fluxes (a reordering of the reference's sum).
+// ---------------------------------------------------------------------------------
+struct Raw {
+    double rhog, rq[5], rhoge, vt[3];
+    double gs, gam2, gsh, gam2h;
+    double rvx, rvy, rvz, rw, vx, vy, vz, z;
+};
+struct Cell {
+    double rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
+    double tem, z, kinh, up, vp;
+    double hv[3][2];                 // half-level values at the cell's lower wall
+    double rhog, rhogw, kinv, c2w0, c2w1, gs;
+};
+struct WallRec {
+    double rhog, rhogw, kinv, c2w0, c2w1, gs;
+};
+struct Tend {
+    double dq[3], de, dphi, dkinh, du, dv;
+};
+struct HTend {
+    double dkinv, dw;
+};
+
+__device__ __forceinline__ void load_raw(const SweepCtx &s, int j, Raw &r)
+{
+    const StepArgs &a = *s.a;
+    const Fields &f = a.f;
+    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
+    const int64_t oq = s.p3 + (int64_t)j * s.ijdim;
+    r.rhog = a.ws.rhog[o];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * s.K2];
+    r.rhoge = a.ws.rhoge[o];
+#pragma unroll
+    for (int t = 0; t < 3; t++) r.vt[t] = a.ws.vt[s.pv + t * s.K2 + (int64_t)j * s.ijdim];
+    r.gs = f.gsgam2[o];
+    r.gam2 = f.gam2[o];
+    r.gsh = f.gsgam2h[o];
+    r.gam2h = f.gam2h[o];
+    r.rvx = f.rhogvx[o];
+    r.rvy = f.rhogvy[o];
+    r.rvz = f.rhogvz[o];
+    r.rw = f.rhogw[o];
+    r.vx = f.vx[o];
+    r.vy = f.vy[o];
+    r.vz = f.vz[o];
+    r.z = f.z[o];
+}
+
+__device__ __forceinline__ void derive_cell(const SweepCtx &s, const GridDev &g, int j, const Raw &r,
+                                            const Cell &below, Cell &c)
+{
+    const bool interior = (j >= KMIN && j <= KMAX);
+    const double rg = r.rhog;
+    const double rgs = interior ? r.gam2 / r.gs : 0.0;
+    double qq[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
+    double qd = 1.0;
+#pragma unroll
+    for (int n = 0; n < 5; n++) qd = qd - qq[n];
+    double cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
+    c.tem = interior ? r.rhoge / (rg * cva) : 0.0;
+    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : 0.0;
+    c.up = r.vx * s.ix + r.vy * s.iy + r.vz * s.iz;
+    c.vp = r.vx * s.jx + r.vy * s.jy + r.vz * s.jz;
+    c.z = r.z;
+    // tracer order inside rq/qq: QC, QR, QI, QS, QG
+    c.q[0] = qq[1];
+    c.q[1] = qq[3];
+    c.q[2] = qq[4];
+    c.rhoq[0] = r.rq[1] * rgs;
+    c.rhoq[1] = r.rq[3] * rgs;
+    c.rhoq[2] = r.rq[4] * rgs;
+#pragma unroll
+    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
+    c.rhog = rg;
+    c.rhogw = r.rw;
+    c.gs = r.gs;
+    if (j >= KMIN) {
+        c.c2w0 = g.afact[j] / r.gs * r.gsh;
+        c.c2w1 = g.bfact[j] / below.gs * r.gsh;
+    } else {
+        c.c2w0 = 0.0;
+        c.c2w1 = 0.0;
+    }
+    if (j >= KMIN + 1 && j <= KMAX) {
+        c.kinv = 0.5 * (r.rw * r.rw) / (c.c2w0 * rg + c.c2w1 * below.rhog);
+        const double rgsh = r.gam2h / r.gsh;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const double qh = 0.5 * (c.q[t] + below.q[t]) * rgsh;
+            c.hv[t][0] = qh * c.kinv;
+            c.hv[t][1] = qh * r.rw;
+        }
+    } else {
+        c.kinv = 0.0;
+#pragma unroll
+        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double rhof_cell(const Cell &c, int t, int i)
+{
+    switch (i) {
+    case 0: return c.rhoq[t];
+    case 1: return c.rhoq[t] * C_CVW * c.tem;
+    case 2: return c.rhoq[t] * C_GRAV * c.z;
+    case 3: return c.qrgs[t] * c.kinh;
+    case 4: return c.rhoq[t] * c.up;
+    default: return c.rhoq[t] * c.vp;
+    }
+}
+__device__ __forceinline__ WallRec wall_of(const Cell &c)
+{
+    WallRec w;
+    w.rhog = c.rhog;
+    w.rhogw = c.rhogw;
+    w.kinv = c.kinv;
+    w.c2w0 = c.c2w0;
+    w.c2w1 = c.c2w1;
+    w.gs = c.gs;
+    return w;
+}
+
+// Apply the sedimentation tendencies and KIN2EIN to interior cell c
+// (src/mod_precip_transport.cpp:960-1011) and store the result.
+__device__ __forceinline__ void finish_cell(const SweepCtx &s, const GridDev &g, int c, const WallRec &wc,
+                                            const WallRec &wcp, const Tend &tc, const Tend &tcp,
+                                            const HTend &hc, const HTend &hcp, double &rhog_new_prev)
+{
+    const StepArgs &a = *s.a;
+    const Fields &f = a.f;
+    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
+    const int64_t oq = s.p3 + (int64_t)c * s.ijdim;
+    double rq[6];
+    rq[QR] = a.ws.rhogq[oq + QR * s.K2];
+    rq[QS] = a.ws.rhogq[oq + QS * s.K2];
+    rq[QG] = a.ws.rhogq[oq + QG * s.K2];
+    const double re0 = a.ws.rhoge[o];
+    const double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
+    const double z = f.z[o];
+    const double gshp = f.gsgam2h[o + s.ijdim];
+    const double w2c0 = g.cfact[c] * wc.gs / gshp, w2c1 = g.dfact[c] * wc.gs / gshp;
+    const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / wc.rhog;
+    double e = re0 + tc.de + tc.dphi;
+    const double kinh_u = kinh + tc.dkinh;
+    const double kinv_c = wc.kinv + hc.dkinv;
+    const double kinv_cp = wcp.kinv + hcp.dkinv;
+    const double rvx_n = rvx + tc.du * s.ix + tc.dv * s.jx;
+    const double rvy_n = rvy + tc.du * s.iy + tc.dv * s.jy;
+    const double rvz_n = rvz + tc.du * s.iz + tc.dv * s.jz;
+    const double rw_n = wc.rhogw + hc.dw;
+    double rg = wc.rhog;
+    rq[QR] = rq[QR] + tc.dq[0];
+    rg = rg + tc.dq[0];
+    e = e - tc.dq[0] * C_GRAV * z;
+    rq[QS] = rq[QS] + tc.dq[1];
+    rg = rg + tc.dq[1];
+    e = e - tc.dq[1] * C_GRAV * z;
+    rq[QG] = rq[QG] + tc.dq[2];
+    rg = rg + tc.dq[2];
+    e = e - tc.dq[2] * C_GRAV * z;
+    const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
+    const double rg_cp = ((wcp.rhog + tcp.dq[0]) + tcp.dq[1]) + tcp.dq[2];
+    const double rw_cp = wcp.rhogw + hcp.dw;
+    const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
+    const double kinv_new_c = (c >= KMIN + 1) ? 0.5 * (rw_n * rw_n) / (wc.c2w0 * rg + wc.c2w1 * rhog_new_prev) : 0.0;
+    const double kinv_new_cp = (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (wcp.c2w0 * rg_cp + wcp.c2w1 * rg) : 0.0;
+    const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
+    e = e + (tmp2 - tmpk);
+    rhog_new_prev = rg;
+    f.rhogvx[o] = rvx_n;
+    f.rhogvy[o] = rvy_n;
+    f.rhogvz[o] = rvz_n;
+    f.rhogw[o] = rw_n;
+    f.rhog[o] = rg;
+    f.rhoge[o] = e;
+    f.rhogq[oq + QR * s.K2] = rq[QR];
+    f.rhogq[oq + QS * s.K2] = rq[QS];
+    f.rhogq[oq + QG * s.K2] = rq[QG];
+}
+
+__global__ void __launch_bounds__(64) k_sweep_fast(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const int l = blockIdx.y;
+    if (a.ws.region_flag[l] != 0) return;
+    const int ij = blockIdx.x * 64 + threadIdx.x;
+    if (ij >= ijdim) return;
+    const GridDev &g = *a.grid;
+    const Fields &f = a.f;
+    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
+
+    SweepCtx s;
+    s.a = &a;
+    s.l = l;
+    s.ij = ij;
+    s.ijdim = ijdim;
+    s.K2 = (int64_t)KALL * ijdim;
+    s.p2 = (int64_t)l * s.K2 + ij;
+    s.p3 = (int64_t)l * NQ * s.K2 + ij;
+    s.pv = (int64_t)l * 3 * s.K2 + ij;
+    const int64_t pc = (int64_t)l * ijdim + ij;
+    s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
+    s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
+
+    Raw rn1;
+    Cell cur, nxt;
+    WallRec wr_m2, wr_m1;
+    Tend t_m2, t_m1;
+    HTend h_m2, h_m1, h_0;
+    WindWindow wind[3];
+    double fq_prev[3], fs_prev[5], fh_prev[2];
+    double rhog_new_prev = 0.0;
+    double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
+
+    {
+        Raw r0;
+        Cell c0;
+        load_raw(s, 0, r0);
+        c0.gs = r0.gs;
+        c0.rhog = r0.rhog;
+        derive_cell(s, g, 0, r0, c0, c0);
+        load_raw(s, 1, r0);
+        derive_cell(s, g, 1, r0, c0, cur);
+        wr_m1 = wall_of(c0);
+#pragma unroll
+        for (int t = 0; t < 3; t++) wind_init(wind[t], r0.vt[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) fq_prev[t] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) fs_prev[i] = 0.0;
+    fh_prev[0] = fh_prev[1] = 0.0;
+    h_m1.dkinv = h_m1.dw = 0.0;   // wall 0
+    h_0.dkinv = h_0.dw = 0.0;     // wall 1
+    h_m2 = h_m1;
+    t_m2 = {};
+    wr_m2 = wr_m1;
+
+    for (int w = KMIN; w <= KMAX + 1; w++) {
+        double fq[3] = {0.0, 0.0, 0.0}, fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fh[2] = {0.0, 0.0};
+        h_0.dkinv = h_0.dw = 0.0;
+        if (w <= KMAX) {
+            load_raw(s, w + 1, rn1);
+            derive_cell(s, g, w + 1, rn1, cur, nxt);
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const double vt_kp1 = (w + 1 <= KMAX) ? rn1.vt[t] : 0.0;
+                double wh_kp1, vh_kp1, whh_kp1;
+                wind_next(wind[t], w, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+                // full level
+                const double z0 = zdis_full(g, dt, dt2, dt3, wind[t], w, wh_kp1);
+                const int kcmax = bnd[bounds_index(t, 0, w, 1)];
+                const int kc = departure_cell(g.gzh, w, w, kcmax, z0, KMIN, KMAX);
+                const bool up1 = (kcmax != w) && (kc == w + 1);
+                const double dzw = g.dgz[w];
+                double ft[6];
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    const double r0 = rhof_cell(cur, t, i);
+                    const double v = up1 ? (-(r0 * dzw) + rhof_cell(nxt, t, i) * z0) : r0 * z0;
+                    ft[i] = eps_filter(v);
+                }
+                fq[t] = ft[0];
+#pragma unroll
+                for (int i = 1; i < 6; i++) fs[i - 1] += ft[i];
+                if (w == KMIN) {
+                    if (t == 0) {
+                        precip0 = precip0 - ft[0] / dt;
+                    } else {
+                        precip1 = precip1 - ft[0] / dt;
+                        lh = lh + ft[0] * C_LHF0 / dt;
+                    }
+                    prhoe = prhoe - ft[1] / dt;
+                    prhophi = prhophi - ft[2] / dt;
+                    prhokin = prhokin - ft[3] / dt - 0.0 / dt;
+                }
+                if (t == 0) f.GPREC[s.p2 + (int64_t)w * ijdim] = (0.0 + (-ft[0] / dt)) * 1.0;
+                // half level (walls 2..94)
+                if (w >= KMIN + 1) {
+                    const double zh0 = zdis_half(g, dt, dt2, dt3, wind[t], w, whh_kp1);
+                    const int khmax = bnd[bounds_index(t, 1, w, 1)];
+                    const int kch = departure_cell(g.gz_shift, w, w, khmax, zh0, KMIN + 1, KMAX);
+                    const double dzh = g.dgzh[w];
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const double hc = cur.hv[t][i], hn = nxt.hv[t][i];
+                        double v;
+                        if (kcmax == w)  // region-wide fast path of getflux: rhof(kcell) * zdis0
+                            v = ((kch == w + 1) ? hn : hc) * zh0;
+                        else
+                            v = (kch == w + 1) ? (-(hc * dzh) + hn * zh0) : hc * zh0;
+                        fh[i] += eps_filter(v);
+                    }
+                }
+                wind_shift(wind[t], vt_kp1, wh_kp1, vh_kp1, whh_kp1);
+            }
+        }
+        // divergence of cell w-1 (cells 1..94) and of wall w (walls 2..94)
+        t_m1 = {};
+        if (w - 1 >= KMIN) {
+            const double r = g.rdgz[w - 1];
+#pragma unroll
+            for (int t = 0; t < 3; t++) t_m1.dq[t] = -(fq[t] - fq_prev[t]) * r;
+            t_m1.de = -(fs[0] - fs_prev[0]) * r;
+            t_m1.dphi = -(fs[1] - fs_prev[1]) * r;
+            t_m1.dkinh = -(fs[2] - fs_prev[2]) * r;
+            t_m1.du = -(fs[3] - fs_prev[3]) * r;
+            t_m1.dv = -(fs[4] - fs_prev[4]) * r;
+        }
+        if (w >= KMIN + 1 && w <= KMAX) {
+            const double r = g.rdgzh[w];
+            h_0.dkinv = -(fh[0] - fh_prev[0]) * r;
+            h_0.dw = -(fh[1] - fh_prev[1]) * r;
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) fq_prev[t] = fq[t];
+#pragma unroll
+        for (int i = 0; i < 5; i++) fs_prev[i] = fs[i];
+        fh_prev[0] = fh[0];
+        fh_prev[1] = fh[1];
+
+        // finish cell w-2 (needs the tendencies of cell w-1)
+        if (w - 2 >= KMIN) finish_cell(s, g, w - 2, wr_m2, wr_m1, t_m2, t_m1, h_m2, h_m1, rhog_new_prev);
+
+        // shift
+        t_m2 = t_m1;
+        h_m2 = h_m1;
+        h_m1 = h_0;
+        wr_m2 = wr_m1;
+        wr_m1 = wall_of(cur);
+        cur = nxt;
+    }
+    // cell 94: its upper neighbour is the halo cell 95 (wall 95 carries no flux)
+    {
+        HTend hz;
+        hz.dkinv = hz.dw = 0.0;
+        Tend tz = {};
+        finish_cell(s, g, KMAX, wr_m2, wr_m1, t_m2, tz, h_m2, hz, rhog_new_prev);
+    }
+    // halo cells: zero tendencies on the momentum (scalars are finished by k_adjust)
+#pragma unroll
+    for (int hc = 0; hc < 2; hc++) {
+        const int c = hc ? (KALL - 1) : 0;
+        const int64_t o = s.p2 + (int64_t)c * ijdim;
+        f.rhogvx[o] = f.rhogvx[o] + 0.0 * s.ix + 0.0 * s.jx;
+        f.rhogvy[o] = f.rhogvy[o] + 0.0 * s.iy + 0.0 * s.jy;
+        f.rhogvz[o] = f.rhogvz[o] + 0.0 * s.iz + 0.0 * s.jz;
+        f.rhogw[o] = f.rhogw[o] + 0.0;
+        f.GPREC[o] = (0.0 + 0.0) * 1.0;
+    }
+    const int64_t pp = (int64_t)l * 2 * ijdim + ij;
+    f.precip[pp] = (0.0 + precip0) * 1.0;
+    f.precip[pp + ijdim] = (0.0 + precip1) * 1.0;
+    f.ISO1_precip[pp] = 0.0;
+    f.ISO1_precip[pp + ijdim] = 0.0;
+    f.ISO2_precip[pp] = 0.0;
+    f.ISO2_precip[pp + ijdim] = 0.0;
+    f.precip_rhoe[pc] = (0.0 + prhoe) * 1.0;
+    f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
+    f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
+    f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
 }
 
 template <bool GENERAL>
@@ -653,7 +1073,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     const GridDev &g = *a.grid;
     const Fields &f = a.f;
     const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
-    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
 
     SweepCtx s;
     s.a = &a;
@@ -679,7 +1099,6 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     double fprev[3][6], fprevh[3][2];
     double rhog_new_prev = 0.0;       // rhog after sedimentation of the last finalised cell
     double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
-    int fail = 0;
 
     // cell 0 (halo) and cell 1
     {
@@ -890,7 +1309,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                 f.rhogvz[o] = rvz_n;
                 f.rhogw[o] = rw_n;
             }
-            finalize_point<true>(s, c, rg, rq, e, gs, &fail);
+            store_sedimented(s, c, rg, rq, e);
         }
 
         // ---- shift the window
@@ -954,20 +1373,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             f.rhogvz[o] = rvz_n;
             f.rhogw[o] = rw_n;
         }
-        finalize_point<true>(s, c, rg, rq, e, gs, &fail);
+        store_sedimented(s, c, rg, rq, e);
     }
-    // halo cells 0 and 95: zero tendencies, no KIN2EIN, no Newton, no filter
+    // halo cells 0 and 95: zero tendencies (their scalars are finished by k_adjust)
 #pragma unroll
     for (int hc = 0; hc < 2; hc++) {
         const int c = hc ? (KALL - 1) : 0;
         const int64_t o = s.p2 + (int64_t)c * ijdim;
-        double rq[6];
-#pragma unroll
-        for (int n = 0; n < 6; n++) rq[n] = a.ws.rhogq[s.p3 + (int64_t)(n * KALL + c) * ijdim] + 0.0;
-        const double z = f.z[o];
-        double e = a.ws.rhoge[o] + 0.0 + 0.0;
-        e = e - 0.0 * C_GRAV * z;
-        double rg = a.ws.rhog[o] + 0.0;
         const double rvx_n = f.rhogvx[o] + 0.0 * s.ix + 0.0 * s.jx;
         const double rvy_n = f.rhogvy[o] + 0.0 * s.iy + 0.0 * s.jy;
         const double rvz_n = f.rhogvz[o] + 0.0 * s.iz + 0.0 * s.jz;
@@ -984,7 +1396,6 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             f.rhogw[o] = rw_n;
         }
         f.GPREC[o] = (0.0 + 0.0) * 1.0;
-        finalize_point<false>(s, c, rg, rq, e, f.gsgam2[o], &fail);
     }
     if (GENERAL) {
         for (int c = 0; c < KALL; c++) {
@@ -1007,30 +1418,31 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
     f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
     f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
-    if (fail) atomicOr(&a.ws.report[0], 1);
 }
 
 // ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
 extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipStream_t stream,
-                                       hipEvent_t *ev /* 5 events or NULL */)
+                                       hipEvent_t *ev /* NSW6_STEP_EVENTS events or NULL */)
 {
     const StepArgs &a = *args;
-    const dim3 grid_r((a.ijdim + 255) / 256, KALL, a.lall);
+    const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
     const dim3 grid_c((a.ijdim + 63) / 64, a.lall);
+    const dim3 grid_e((a.ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE, 3, a.lall);
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (precision == 1)
-        hipLaunchKernelGGL(k_rates<float>, grid_r, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_rates<float>, grid_p, dim3(256), 0, stream, a);
     else
-        hipLaunchKernelGGL(k_rates<double>, grid_r, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_rates<double>, grid_p, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
-    hipLaunchKernelGGL(k_extremes, grid_c, dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_extremes, grid_e, dim3(64), 0, stream, a);
     hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[2], stream);
-    hipLaunchKernelGGL(k_sweep<false>, grid_c, dim3(64), 0, stream, a);
-    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL(k_sweep_fast, grid_c, dim3(64), 0, stream, a);
     hipLaunchKernelGGL(k_sweep<true>, grid_c, dim3(64), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL(k_adjust, grid_p, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
     return hipGetLastError();
 }
