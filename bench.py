@@ -34,8 +34,23 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None,
-                    help="JSON file with measured HBM bytes per step (from rocprofv3 --pmc runs)")
+                    help="text file with a 'bytes_per_step: N' line: HBM bytes per step measured with "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (default profiles/traffic_<workload>.txt)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the measurement, run 3 plain device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profiled run against known bytes)")
     return ap.parse_args()
+
+
+def read_traffic(path):
+    """HBM bytes per step from a profile summary line 'bytes_per_step: N' (None if absent)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("bytes_per_step:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(reps: int):
@@ -95,16 +110,13 @@ def main():
     rep = drv.check()
     step_ms, nev = drv.timing_read(0)
     kernel_ms = {name: drv.timing_read(i)[0] for i, name in
-                 ((1, "rates"), (2, "extremes+bounds"), (3, "sweep"), (4, "adjust"))}
+                 ((1, "rates"), (2, "extremes+bounds"), (3, "flux"), (4, "finish"))}
     drv.timing(False)
 
     ms_per_step = wall * 1e3 / args.steps
     algo_bytes = ALGO_BYTES_PER_COLUMN * ncol
     achieved = algo_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
-    traffic = None
-    if args.traffic and os.path.exists(args.traffic):
-        with open(args.traffic) as fh:
-            traffic = json.load(fh).get("bytes_per_step")
+    traffic = read_traffic(args.traffic or os.path.join(REPO, "profiles", f"traffic_{wname}.txt"))
 
     out = {
         "metric": "column-steps/s at 169k x 96 (FP64 mp_driver step)" if prec == N.NSW6_FP64
@@ -122,7 +134,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wname, "regions": w.lall, "columns_per_region": w.ijdim, "levels": 96,
                    "columns": ncol, "dt_s": w.dt, "tracers": 6},
-        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_sweep + k_adjust)",
+        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_flux + k_sweep + k_finish)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": algo_bytes, "event_ms": step_ms, "launches_timed": nev},
@@ -132,6 +144,15 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     drv.close()
+    del dev
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * n * 8
     print(json.dumps(out))
 
 

@@ -172,7 +172,7 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *report);
 /* Average duration (ms) of the named kernel over the launches recorded since the
  * last reset, measured with HIP events on the stream the kernel ran on.
  * which: 0 = whole step, 1 = process rates, 2 = fall-speed extremes + kcell bounds,
- * 3 = sedimentation column sweep, 4 = point-wise saturation adjustment tail.
+ * 3 = sedimentation fluxes, 4 = flux divergence + KIN2EIN + saturation adjustment tail.
  * Recording is off until nsw6_timing_enable(ctx, 1). */
 int nsw6_timing_enable(nsw6_ctx *ctx, int on);
 int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count);

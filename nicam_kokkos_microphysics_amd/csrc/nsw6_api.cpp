@@ -30,6 +30,7 @@ struct nsw6_ctx {
     int region_capacity = 0;
     int precision = NSW6_FP64;
     DevParams P{};
+    DevParams *P_dev = nullptr;
     GridDev *grid_dev = nullptr;
     Workspace ws{};
     hipStream_t last_stream = nullptr;
@@ -82,14 +83,14 @@ static void to_dev_params(const nsw6_params &p, DevParams &d)
 static void free_ws(nsw6_ctx *c)
 {
     Workspace &w = c->ws;
-    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.mom, w.part, w.bounds, w.region_flag, w.report};
+    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.flux, w.part, w.bounds, w.region_flag, w.report};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     memset(&w, 0, sizeof w);
 }
 
-// Per-region buffers: kcell bounds, region flags and the per-wavefront partial
-// extremes (sized for any lall * ijdim <= max_columns split into `cap` regions).
+// Per-region buffers: kcell bounds, region flags and the per-block partial extremes
+// (sized for any lall * ijdim <= max_columns split into `cap` regions).
 static int ensure_regions(nsw6_ctx *c, int lall)
 {
     if (lall <= c->region_capacity) return NSW6_OK;
@@ -100,8 +101,8 @@ static int ensure_regions(nsw6_ctx *c, int lall)
     if (w.bounds) (void)hipFree(w.bounds);
     if (w.region_flag) (void)hipFree(w.region_flag);
     w.part = nullptr; w.bounds = nullptr; w.region_flag = nullptr;
-    const size_t waves = (size_t)(c->max_columns / EXT_COLS_PER_WAVE) + (size_t)cap;
-    if (hipMalloc(&w.part, sizeof(double) * (size_t)KALL * 12 * waves) != hipSuccess ||
+    const size_t blocks = (size_t)(c->max_columns / EXT_COLS_PER_BLOCK) + (size_t)cap;
+    if (hipMalloc(&w.part, sizeof(double) * (size_t)KALL * 12 * blocks) != hipSuccess ||
         hipMalloc(&w.bounds, sizeof(int) * BOUNDS_PER_REGION * (size_t)cap) != hipSuccess ||
         hipMalloc(&w.region_flag, sizeof(int) * (size_t)cap) != hipSuccess) {
         c->region_capacity = 0;
@@ -152,11 +153,13 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
     int st = NSW6_OK;
     if (hipMalloc(&c->grid_dev, sizeof(GridDev)) != hipSuccess ||
         hipMemcpy(c->grid_dev, &gh, sizeof(GridDev), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&c->P_dev, sizeof(DevParams)) != hipSuccess ||
+        hipMemcpy(c->P_dev, &c->P, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&w.rhog, sizeof(double) * K2) != hipSuccess ||
         hipMalloc(&w.rhoge, sizeof(double) * K2) != hipSuccess ||
         hipMalloc(&w.rhogq, sizeof(double) * K2 * NQ) != hipSuccess ||
         hipMalloc(&w.vt, sizeof(double) * K2 * 3) != hipSuccess ||
-        hipMalloc(&w.mom, sizeof(double) * K2 * 4) != hipSuccess ||
+        hipMalloc(&w.flux, sizeof(double) * K2 * FLUX_SLOTS) != hipSuccess ||
         hipMalloc(&w.report, sizeof(int) * REPORT_INTS) != hipSuccess ||
         hipMemset(w.report, 0, sizeof(int) * REPORT_INTS) != hipSuccess)
         st = NSW6_HIP_ERROR;
@@ -164,6 +167,7 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
     if (st != NSW6_OK) {
         free_ws(c);
         if (c->grid_dev) (void)hipFree(c->grid_dev);
+        if (c->P_dev) (void)hipFree(c->P_dev);
         delete c;
         return st;
     }
@@ -175,7 +179,10 @@ int nsw6_set_params(nsw6_ctx *ctx, const nsw6_params *params)
 {
     if (!ctx || !params) return NSW6_BAD_ARG;
     to_dev_params(*params, ctx->P);
-    return NSW6_OK;
+    // steps already queued read the old coefficients; let them finish first
+    if (ctx->last_stream && hipStreamSynchronize(ctx->last_stream) != hipSuccess) return NSW6_HIP_ERROR;
+    if (hipDeviceSynchronize() != hipSuccess) return NSW6_HIP_ERROR;
+    return hip_ok(hipMemcpy(ctx->P_dev, &ctx->P, sizeof(DevParams), hipMemcpyHostToDevice));
 }
 
 int nsw6_set_precision(nsw6_ctx *ctx, int precision)
@@ -227,7 +234,7 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     f.z = fl->z; f.zh = fl->zh;
     f.GDCLW = fl->GDCLW; f.GDCFRC = fl->GDCFRC; f.GPREC = fl->GPREC; f.CBMFX = fl->CBMFX;
     a.ws = ctx->ws;
-    a.P = ctx->P;
+    a.P = ctx->P_dev;
     a.grid = ctx->grid_dev;
     a.lall = lall;
     a.ijdim = ijdim;
@@ -356,6 +363,7 @@ void nsw6_finalize(nsw6_ctx *ctx)
     (void)hipDeviceSynchronize();
     free_ws(ctx);
     if (ctx->grid_dev) (void)hipFree(ctx->grid_dev);
+    if (ctx->P_dev) (void)hipFree(ctx->P_dev);
     delete ctx;
 }
 

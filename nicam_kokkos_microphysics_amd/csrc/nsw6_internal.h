@@ -32,10 +32,24 @@ struct GridDev {
 
 // Per-region reductions: [tracer 3][level type 2: full, half][k 96][max, min]
 constexpr int EXT_PER_REGION = 3 * 2 * KALL * 2;
-// k_extremes: one column per lane and one tracer per wavefront; per-wave partial
-// extremes are stored as [l][k][12][wave]
-constexpr int EXT_COLS_PER_WAVE = 64;
+// k_extremes: one block of EXT_THREADS threads reduces EXT_COLS_PER_BLOCK columns of
+// one level; per-block partial extremes are stored as [l][k][12][block], slot
+// t*4 + {0 full max, 1 full min, 2 half max, 3 half min}
+constexpr int EXT_THREADS = 256;
+constexpr int EXT_COLS_PER_THREAD = 4;
+constexpr int EXT_COLS_PER_BLOCK = EXT_THREADS * EXT_COLS_PER_THREAD;
 constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
+// Flux workspace [l][FLUX_SLOTS][k][ij].  For the single-cell regions, wall w holds:
+//   0..2  mass fluxes of QR, QS, QG
+//   3..7  tracer-summed fluxes of internal energy, potential energy, horizontal
+//         kinetic energy, and the two horizontal momentum components
+//   8..9  tracer-summed half-level fluxes of vertical kinetic energy and momentum
+//   10    the pre-transport rhogw of cell w
+// The regions that take the general sweep use slots 0..3 of their own block for the
+// deferred momentum (rhogvx, rhogvy, rhogvz, rhogw).
+constexpr int FLUX_SLOTS = 11;
+constexpr int FX_Q = 0, FX_E = 3, FX_PHI = 4, FX_KINH = 5, FX_U = 6, FX_V = 7, FX_KINV = 8, FX_W = 9,
+              FX_RW = 10;
 __host__ __device__ constexpr int ext_index(int t, int type, int k, int mm)
 {
     return ((t * 2 + type) * KALL + k) * 2 + mm;
@@ -44,12 +58,42 @@ __host__ __device__ constexpr int bounds_index(int t, int type, int k, int mm) {
 // report ints: [0] any failure, [1] not converged, [2] NaN, [3] l, [4] k, [5] ij
 constexpr int REPORT_INTS = 6;
 
+// XCD-aware block order for the stencil kernels.  MI355X deals consecutive
+// workgroups round-robin to its 8 XCDs, each with its own L2.  A 1-D grid of
+// NUM_XCD * per workgroups is mapped so that workgroup b takes logical block
+// (b % NUM_XCD) * per + b / NUM_XCD: every XCD walks one contiguous range of logical
+// blocks, ordered level-fastest within a column strip, so the neighbouring levels
+// a block re-reads were just loaded into the same L2.
+constexpr int NUM_XCD = 8;
+struct LevelBlock {
+    int l, bx, k;
+    bool valid;
+};
+__host__ __device__ inline int level_grid(int nbx, int nk, int lall)
+{
+    const int total = nbx * nk * lall;
+    return ((total + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
+}
+__device__ __forceinline__ LevelBlock level_block(int b, int nbx, int nk, int k0, int lall)
+{
+    const int total = nbx * nk * lall;
+    const int per = (total + NUM_XCD - 1) / NUM_XCD;
+    const int L = (b % NUM_XCD) * per + b / NUM_XCD;
+    LevelBlock r;
+    r.valid = L < total;
+    r.k = k0 + L % nk;
+    const int t = L / nk;
+    r.bx = t % nbx;
+    r.l = t / nbx;
+    return r;
+}
+
 // Context-owned device workspace (allocated once for the maximum column count).
 struct Workspace {
     double *rhog, *rhoge, *rhogq;   // state after the rate update (pre-sedimentation)
     double *vt;                     // fall speeds of QR, QS, QG  [l][3][k][ij]
-    double *mom;                    // deferred momentum of the general sweep [l][4][k][ij]
-    double *part;                   // [l][KALL][12][waves per region] partial extremes
+    double *flux;                   // [l][FLUX_SLOTS][k][ij] (see FLUX_SLOTS)
+    double *part;                   // [l][KALL][12][blocks per region] partial extremes
     int *bounds;                    // [l][BOUNDS_PER_REGION]
     int *region_flag;               // [l] 1 = region needs the general sweep
     int *report;                    // [REPORT_INTS]
@@ -58,7 +102,8 @@ struct Workspace {
 struct StepArgs {
     Fields f;
     Workspace ws;
-    DevParams P;
+    const DevParams *P;   // device copy owned by the context (kept out of the kernel
+                          // arguments: ~130 doubles there overflow the scalar registers)
     const GridDev *grid;
     int lall, ijdim;
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
@@ -67,7 +112,7 @@ struct StepArgs {
 }  // namespace nsw6
 
 // events recorded by nsw6_launch_step: before k_rates, after k_rates, after
-// k_extremes+k_bounds, after the sweeps, after k_adjust
+// k_extremes+k_bounds, after the flux kernels (k_flux, general sweep), after k_finish
 constexpr int NSW6_STEP_EVENTS = 5;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,

@@ -4,18 +4,21 @@
 //   k_rates    point-wise over (l,k,ij): negative filter, psat/qsat, Bergeron, the
 //              49 process rates and the state update (src/mod_mp_nsw6.cpp:439-589);
 //              writes the pre-sedimentation state and the fall speeds to workspace.
-//   k_extremes one lane per column: departure distances zdis of the full and half
-//              levels for QR, QS, QG (src/mod_vadv1d.cpp:270-302) and their
-//              per-region, per-level max/min (:317-327) through atomics.
+//   k_extremes point-wise departure distances zdis of the full and half levels for
+//              QR, QS, QG (src/mod_vadv1d.cpp:270-302) reduced per block to partial
+//              per-level max/min (:317-327).
 //   k_bounds   one thread per (region, tracer, level type, level): kcell_min/max
-//              search (:329-351) and the per-region choice of sweep.
-//   k_sweep    one lane per column, bottom to top: per-column departure cells,
-//              semi-Lagrangian fluxes (:393-450), flux divergences, surface fluxes
-//              and KIN2EIN (src/mod_precip_transport.cpp:573-1011).
-//   k_adjust   point-wise: rediagnosis (:1074-1086), saturation adjustment
+//              search (:329-351) and the per-region choice of transport path.
+//   k_flux     point-wise over walls of the single-cell regions (every departure
+//              cell in {k, k+1}): closed-form semi-Lagrangian fluxes (:393-450),
+//              frain and the surface fluxes (src/mod_precip_transport.cpp:573-958).
+//   k_sweep    one lane per column, bottom to top, for the general regions only
+//              (departures beyond one cell, upward, or out of the domain).
+//   k_finish   point-wise: flux divergences and KIN2EIN (:960-1011) of the
+//              single-cell regions, rediagnosis (:1074-1086), saturation adjustment
 //              (src/mod_satadjust.cpp:274-362,771-896) and the final negative filter.
-// All column loops keep ij fastest, so every per-level access of a wavefront is one
-// contiguous 512-byte segment.  No MFMA: nothing here is a dense contraction.
+// Every access of a wavefront is one contiguous 512-byte segment of a level (ij
+// fastest).  No MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -70,7 +73,7 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
         return;
     }
 
-    const DevParams &P = a.P;
+    const DevParams &P = *a.P;
     // ---- negative_filter (src/mod_mp_nsw6.cpp:1770-1831), interior point
     double rq[NQ];
 #pragma unroll
@@ -240,50 +243,114 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
+// Point-wise form of the same wall speeds: with V[d + 3] = Vt(k + d), d = -3..+1,
+// wh_full(V, k, d) is the full-level wh at wall k + d and wh_half(V, k, d) the
+// half-level one (Vtermh averaged again), identical to the sliding window above.
+__device__ __forceinline__ double wh_full(const double V[5], int k, int d)
+{
+    const int j = k + d;
+    if (j == 0) return 0.0;
+    if (j == 1) return V[d + 3];                        // wh(1) = Vt(1)
+    if (j <= KMAX) return 0.5 * (V[d + 2] + V[d + 3]);  // 0.5 (Vt(j-1) + Vt(j))
+    return V[d + 2];                                    // wh(95) = Vt(94)
+}
+__device__ __forceinline__ double vh_half(const double V[5], int k, int d)
+{
+    const int j = k + d;
+    return (j >= 2 && j <= KMAX - 1) ? 0.5 * (V[d + 3] + V[d + 2]) : 0.0;
+}
+__device__ __forceinline__ double wh_half(const double V[5], int k, int d)
+{
+    const int j = k + d;
+    if (j >= 3 && j <= KMAX) return 0.5 * (vh_half(V, k, d - 1) + vh_half(V, k, d));
+    return (j == 2) ? vh_half(V, k, d) : 0.0;
+}
+// zdis of the full level at wall k (1..94) and of the half level at wall k (2..94)
+__device__ __forceinline__ double zdis_full_at(const GridDev &g, double dt, double dt2, double dt3_6,
+                                               const double V[5], int k)
+{
+    const double whp = wh_full(V, k, 1);
+    if (k == KMIN) return zdis_bottom(dt, dt2, wh_full(V, k, 0), whp, g.rdgz[KMIN]);
+    return zdis_interior(dt, dt2, dt3_6, wh_full(V, k, -1), wh_full(V, k, 0), whp, g.rdgz[k - 1], g.rdgz[k],
+                         g.rsum_gz[k]);
+}
+__device__ __forceinline__ double zdis_half_at(const GridDev &g, double dt, double dt2, double dt3_6,
+                                               const double V[5], int k)
+{
+    const double whp = wh_half(V, k, 1);
+    if (k == KMIN + 1) return zdis_bottom(dt, dt2, wh_half(V, k, 0), whp, g.rdgzh[KMIN + 1]);
+    return zdis_interior(dt, dt2, dt3_6, wh_half(V, k, -1), wh_half(V, k, 0), whp, g.rdgzh[k - 1], g.rdgzh[k],
+                         g.rsum_gzh[k]);
+}
+// Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used).
+__device__ __forceinline__ void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
+{
+#pragma unroll
+    for (int d = -3; d <= 1; d++) V[d + 3] = (k + d >= 0) ? vt_col[(int64_t)(k + d) * ijdim] : 0.0;
+}
+
 // ---------------------------------------------------------------------------------
-// k_extremes: one wavefront per block, one column per lane, one tracer per
-// wavefront (grid.y).  For every level the wavefront reduces its 64 columns and
-// stores the 4 extremes of its tracer ({full, half} x {max, min}) to its slot of a
-// partials array [l][k][12][wave]; k_bounds reduces the partials of a region.
+// k_extremes: per-level max/min over the columns of a region of the departure
+// distances (src/mod_vadv1d.cpp:317-327).  One block per (column chunk, level k in
+// 1..94, region); each thread takes EXT_COLS_PER_THREAD columns strided by the block
+// size.  The block's 12 extremes (3 tracers x {full, half} x {max, min}) go to the
+// partials array [l][k][12][block] that k_bounds reduces.
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_extremes(StepArgs a)
+__global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
 {
     const int ijdim = a.ijdim;
-    const int t = blockIdx.y, l = blockIdx.z, wv = blockIdx.x;
-    const int nw = (ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE;
-    const int ij = wv * EXT_COLS_PER_WAVE + threadIdx.x;
-    const bool live = ij < ijdim;
-    const int64_t K2 = (int64_t)KALL * ijdim;
-    const double *vt = a.ws.vt + ((int64_t)l * 3 + t) * K2 + (live ? ij : ijdim - 1);
+    const int nb = (ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK;
+    const LevelBlock lb = level_block(blockIdx.x, nb, KMAX - KMIN + 1, KMIN, a.lall);
+    if (!lb.valid) return;
+    const int l = lb.l, k = lb.k;
     const GridDev &g = *a.grid;
     const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
-    double *part = a.ws.part + (int64_t)l * KALL * 12 * nw + wv;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const double *vt = a.ws.vt + (int64_t)l * 3 * K2;
 
-    WindWindow w;
-    wind_init(w, vt[1 * ijdim]);
-    double vn = vt[2 * (int64_t)ijdim];
-    for (int k = KMIN; k <= KMAX; k++) {
-        const double vc = vn;   // Vt(k+1); Vt(95) = 0 is stored by k_rates
-        if (k + 2 <= KALL - 1) vn = vt[(int64_t)(k + 2) * ijdim];
-        const double v = (k + 1 <= KMAX) ? vc : 0.0;
-        double wh, vh, whh;
-        wind_next(w, k, v, wh, vh, whh);
-        const double zf = zdis_full(g, dt, dt2, dt3, w, k, wh);
-        const bool okf = live && (zf == zf);
-        double e0 = wave_max<64>(okf ? zf : -__builtin_inf());
-        double e1 = wave_min<64>(okf ? zf : __builtin_inf());
-        double e2 = -__builtin_inf(), e3 = __builtin_inf();
-        if (k >= KMIN + 1) {
-            const double zh = zdis_half(g, dt, dt2, dt3, w, k, whh);
-            const bool okh = live && (zh == zh);
-            e2 = wave_max<64>(okh ? zh : -__builtin_inf());
-            e3 = wave_min<64>(okh ? zh : __builtin_inf());
+    double e[12];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        e[t * 4 + 0] = -__builtin_inf();
+        e[t * 4 + 1] = __builtin_inf();
+        e[t * 4 + 2] = -__builtin_inf();
+        e[t * 4 + 3] = __builtin_inf();
+    }
+    for (int i = 0; i < EXT_COLS_PER_THREAD; i++) {
+        const int ij = lb.bx * EXT_COLS_PER_BLOCK + i * EXT_THREADS + threadIdx.x;
+        if (ij >= ijdim) break;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            double V[5];
+            load_vt_window(vt + t * K2 + ij, ijdim, k, V);
+            const double zf = zdis_full_at(g, dt, dt2, dt3, V, k);
+            if (zf == zf) {
+                e[t * 4 + 0] = dmax(e[t * 4 + 0], zf);
+                e[t * 4 + 1] = dmin(e[t * 4 + 1], zf);
+            }
+            if (k >= KMIN + 1) {
+                const double zh = zdis_half_at(g, dt, dt2, dt3, V, k);
+                if (zh == zh) {
+                    e[t * 4 + 2] = dmax(e[t * 4 + 2], zh);
+                    e[t * 4 + 3] = dmin(e[t * 4 + 3], zh);
+                }
+            }
         }
-        if (threadIdx.x < 4) {
-            const double v4 = (threadIdx.x == 0) ? e0 : (threadIdx.x == 1) ? e1 : (threadIdx.x == 2) ? e2 : e3;
-            part[((int64_t)k * 12 + t * 4 + threadIdx.x) * nw] = v4;
-        }
-        wind_shift(w, v, wh, vh, whh);
+    }
+    __shared__ double red[EXT_THREADS / 64][12];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+        const double v = (j & 1) ? wave_min<64>(e[j]) : wave_max<64>(e[j]);
+        if (lane == 0) red[wv][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int j = threadIdx.x;
+        double v = red[0][j];
+#pragma unroll
+        for (int w = 1; w < EXT_THREADS / 64; w++) v = (j & 1) ? dmin(v, red[w][j]) : dmax(v, red[w][j]);
+        a.ws.part[(((int64_t)l * KALL + k) * 12 + j) * nb + lb.bx] = v;
     }
 }
 
@@ -302,7 +369,7 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
     int kcmin = k, kcmax = k;
     int general = 0;
     if (k >= mkmin && k <= mkmax) {
-        const int nw = (a.ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE;
+        const int nw = (a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK;
         const double *pmax = a.ws.part + (((int64_t)l * KALL + k) * 12 + (t * 2 + type) * 2 + 0) * nw;
         const double *pmin = pmax + nw;
         double amax = -__builtin_inf(), amin = __builtin_inf();
@@ -509,40 +576,432 @@ __device__ double general_hv(const SweepCtx &s, const GridDev &g, int j, int t, 
     return c.hv[t][i];
 }
 
+// Post-sedimentation writes of one interior cell of a general region: rhog, rhoge and
+// the sedimenting tracers (the rest of the state is finished by k_finish).
+__device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, double rg, const double rq[6], double e)
+{
+    const Fields &f = s.a->f;
+    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
+    f.rhog[o] = rg;
+    f.rhoge[o] = e;
+    f.rhogq[s.p3 + (int64_t)(QR * KALL + c) * s.ijdim] = rq[QR];
+    f.rhogq[s.p3 + (int64_t)(QS * KALL + c) * s.ijdim] = rq[QS];
+    f.rhogq[s.p3 + (int64_t)(QG * KALL + c) * s.ijdim] = rq[QG];
+}
+
 // ---------------------------------------------------------------------------------
-// k_adjust: the point-wise tail of the step after sedimentation -- rediagnosis
-// (src/mod_precip_transport.cpp:1074-1086), SATURATION_adjustment with
-// satadjust_liq (src/mod_satadjust.cpp:274-362, 771-896) and the final
-// negative_filter (src/mod_mp_nsw6.cpp:671, 1770-1831).  One thread per point.
+// Single-cell regions: every departure cell lies in {k, k+1} (checked by k_bounds),
+// so each semi-Lagrangian flux of vadv1d_getflux_new (src/mod_vadv1d.cpp:393-450)
+// has the closed form
+//     kc = k   : f = rhof(k) * zdis0
+//     kc = k+1 : f = -rhof(k) * dz(k) + rhof(k+1) * zdis0   (zdis0 kept, not refreshed)
+// and depends only on cells k, k+1 and the fall speeds Vt(k-3..k+1).  The half level
+// uses the full level's kcell_max as the reference does
+// (src/mod_precip_transport.cpp:890-892).  The transport then splits into two
+// point-wise kernels: k_flux (one thread per wall) and k_finish (one thread per cell).
+// Flux divergences of the non-mass quantities are taken on tracer-summed fluxes (a
+// reordering of the reference's per-tracer sum).
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adjust(StepArgs a)
+struct Raw {
+    double rhog, rq[5], rhoge;
+    double gs, gam2, gsh, gam2h;
+    double rvx, rvy, rvz, rw, vx, vy, vz, z;
+};
+struct Cell {
+    double rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
+    double tem, z, kinh, up, vp;
+    double hv[3][2];                 // half-level values at the cell's lower wall
+    double rhog, rhogw, kinv, c2w0, c2w1, gs;
+};
+
+__device__ __forceinline__ void load_raw(const SweepCtx &s, int j, Raw &r)
+{
+    const StepArgs &a = *s.a;
+    const Fields &f = a.f;
+    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
+    const int64_t oq = s.p3 + (int64_t)j * s.ijdim;
+    r.rhog = a.ws.rhog[o];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * s.K2];
+    r.rhoge = a.ws.rhoge[o];
+    r.gs = f.gsgam2[o];
+    r.gam2 = f.gam2[o];
+    r.gsh = f.gsgam2h[o];
+    r.gam2h = f.gam2h[o];
+    r.rvx = f.rhogvx[o];
+    r.rvy = f.rhogvy[o];
+    r.rvz = f.rhogvz[o];
+    r.rw = f.rhogw[o];
+    r.vx = f.vx[o];
+    r.vy = f.vy[o];
+    r.vz = f.vz[o];
+    r.z = f.z[o];
+}
+
+// Transported quantities of cell j (src/mod_precip_transport.cpp:740-780, 846-888)
+// given the wall-relevant part of cell j-1 (rhog, q, gsgam2).
+__device__ __forceinline__ void derive_cell(const SweepCtx &s, const GridDev &g, int j, const Raw &r,
+                                            const Cell &below, Cell &c)
+{
+    const bool interior = (j >= KMIN && j <= KMAX);
+    const double rg = r.rhog;
+    const double rgs = interior ? r.gam2 / r.gs : 0.0;
+    double qq[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
+    double qd = 1.0;
+#pragma unroll
+    for (int n = 0; n < 5; n++) qd = qd - qq[n];
+    double cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
+    c.tem = interior ? r.rhoge / (rg * cva) : 0.0;
+    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : 0.0;
+    c.up = r.vx * s.ix + r.vy * s.iy + r.vz * s.iz;
+    c.vp = r.vx * s.jx + r.vy * s.jy + r.vz * s.jz;
+    c.z = r.z;
+    // tracer order inside rq/qq: QC, QR, QI, QS, QG
+    c.q[0] = qq[1];
+    c.q[1] = qq[3];
+    c.q[2] = qq[4];
+    c.rhoq[0] = r.rq[1] * rgs;
+    c.rhoq[1] = r.rq[3] * rgs;
+    c.rhoq[2] = r.rq[4] * rgs;
+#pragma unroll
+    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
+    c.rhog = rg;
+    c.rhogw = r.rw;
+    c.gs = r.gs;
+    if (j >= KMIN) {
+        c.c2w0 = g.afact[j] / r.gs * r.gsh;
+        c.c2w1 = g.bfact[j] / below.gs * r.gsh;
+    } else {
+        c.c2w0 = 0.0;
+        c.c2w1 = 0.0;
+    }
+    if (j >= KMIN + 1 && j <= KMAX) {
+        c.kinv = 0.5 * (r.rw * r.rw) / (c.c2w0 * rg + c.c2w1 * below.rhog);
+        const double rgsh = r.gam2h / r.gsh;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const double qh = 0.5 * (c.q[t] + below.q[t]) * rgsh;
+            c.hv[t][0] = qh * c.kinv;
+            c.hv[t][1] = qh * r.rw;
+        }
+    } else {
+        c.kinv = 0.0;
+#pragma unroll
+        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double rhof_cell(const Cell &c, int t, int i)
+{
+    switch (i) {
+    case 0: return c.rhoq[t];
+    case 1: return c.rhoq[t] * C_CVW * c.tem;
+    case 2: return c.rhoq[t] * C_GRAV * c.z;
+    case 3: return c.qrgs[t] * c.kinh;
+    case 4: return c.rhoq[t] * c.up;
+    default: return c.rhoq[t] * c.vp;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_flux: one thread per (wall w, column) of the single-cell regions.  Writes the flux
+// workspace of wall w (FLUX_SLOTS), frain = GPREC (src/mod_precip_transport.cpp:
+// 916-927) and, at wall kmin, the surface fluxes precip, precip_lh_heat,
+// precip_rhoe/rhophi/rhokin (:900-958) scaled by fraction_mp = 1 (src/mod_mp_driver.cpp:
+// 560-612).  Walls 0 and 95 carry no flux.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
-    const int l = blockIdx.z, k = blockIdx.y;
-    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
+    if (!lb.valid) return;
+    const int l = lb.l, w = lb.k;
+    if (a.ws.region_flag[l] != 0) return;
+    const int ij = lb.bx * 256 + threadIdx.x;
+    if (ij >= ijdim) return;
+    const GridDev &g = *a.grid;
+    const Fields &f = a.f;
+
+    SweepCtx s;
+    s.a = &a;
+    s.l = l;
+    s.ij = ij;
+    s.ijdim = ijdim;
+    s.K2 = (int64_t)KALL * ijdim;
+    s.p2 = (int64_t)l * s.K2 + ij;
+    s.p3 = (int64_t)l * NQ * s.K2 + ij;
+    s.pv = (int64_t)l * 3 * s.K2 + ij;
+    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * s.K2 + (int64_t)w * ijdim + ij;   // + slot*K2
+    const int64_t ow = s.p2 + (int64_t)w * ijdim;
+
+    fx[FX_RW * s.K2] = f.rhogw[ow];
+    if (w == 0 || w == KALL - 1) {
+#pragma unroll
+        for (int i = 0; i < FX_RW; i++) fx[i * s.K2] = 0.0;
+        f.GPREC[ow] = (0.0 + 0.0) * 1.0;
+        return;
+    }
+    const int64_t pc = (int64_t)l * ijdim + ij;
+    s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
+    s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
+    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
+
+    // cell w-1 enters the walls of cell w only through rhog, q and gsgam2
+    Cell below, cur, nxt;
+    {
+        const int64_t o = s.p2 + (int64_t)(w - 1) * ijdim;
+        const int64_t oq = s.p3 + (int64_t)(w - 1) * ijdim;
+        below.rhog = a.ws.rhog[o];
+        below.gs = f.gsgam2[o];
+        below.q[0] = a.ws.rhogq[oq + QR * s.K2] / below.rhog;
+        below.q[1] = a.ws.rhogq[oq + QS * s.K2] / below.rhog;
+        below.q[2] = a.ws.rhogq[oq + QG * s.K2] / below.rhog;
+    }
+    {
+        Raw r;
+        load_raw(s, w, r);
+        derive_cell(s, g, w, r, below, cur);
+        load_raw(s, w + 1, r);
+        derive_cell(s, g, w + 1, r, cur, nxt);
+    }
+
+    double fq[3], fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fh[2] = {0.0, 0.0};
+    double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        double V[5];
+        load_vt_window(a.ws.vt + s.pv + t * s.K2, ijdim, w, V);
+        // full level
+        const double z0 = zdis_full_at(g, dt, dt2, dt3, V, w);
+        const int kcmax = bnd[bounds_index(t, 0, w, 1)];
+        const int kc = departure_cell(g.gzh, w, w, kcmax, z0, KMIN, KMAX);
+        const bool up1 = (kcmax != w) && (kc == w + 1);
+        const double dzw = g.dgz[w];
+        double ft[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const double r0 = rhof_cell(cur, t, i);
+            const double v = up1 ? (-(r0 * dzw) + rhof_cell(nxt, t, i) * z0) : r0 * z0;
+            ft[i] = eps_filter(v);
+        }
+        fq[t] = ft[0];
+#pragma unroll
+        for (int i = 1; i < 6; i++) fs[i - 1] += ft[i];
+        if (w == KMIN) {
+            if (t == 0) {
+                precip0 = precip0 - ft[0] / dt;
+            } else {
+                precip1 = precip1 - ft[0] / dt;
+                lh = lh + ft[0] * C_LHF0 / dt;
+            }
+            prhoe = prhoe - ft[1] / dt;
+            prhophi = prhophi - ft[2] / dt;
+            prhokin = prhokin - ft[3] / dt - 0.0 / dt;   // the half-level flux at kmin is 0
+        }
+        // half level (walls 2..94)
+        if (w >= KMIN + 1) {
+            const double zh0 = zdis_half_at(g, dt, dt2, dt3, V, w);
+            const int khmax = bnd[bounds_index(t, 1, w, 1)];
+            const int kch = departure_cell(g.gz_shift, w, w, khmax, zh0, KMIN + 1, KMAX);
+            const double dzh = g.dgzh[w];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const double hc = cur.hv[t][i], hn = nxt.hv[t][i];
+                double v;
+                if (kcmax == w)  // region-wide fast path of getflux: rhof(kcell) * zdis0
+                    v = ((kch == w + 1) ? hn : hc) * zh0;
+                else
+                    v = (kch == w + 1) ? (-(hc * dzh) + hn * zh0) : hc * zh0;
+                fh[i] += eps_filter(v);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * s.K2] = fq[t];
+#pragma unroll
+    for (int i = 0; i < 5; i++) fx[(FX_E + i) * s.K2] = fs[i];
+    fx[FX_KINV * s.K2] = fh[0];
+    fx[FX_W * s.K2] = fh[1];
+    f.GPREC[ow] = (0.0 + (-fq[0] / dt)) * 1.0;
+    if (w == KMIN) {
+        const int64_t pp = (int64_t)l * 2 * ijdim + ij;
+        f.precip[pp] = (0.0 + precip0) * 1.0;
+        f.precip[pp + ijdim] = (0.0 + precip1) * 1.0;
+        f.ISO1_precip[pp] = 0.0;
+        f.ISO1_precip[pp + ijdim] = 0.0;
+        f.ISO2_precip[pp] = 0.0;
+        f.ISO2_precip[pp + ijdim] = 0.0;
+        f.precip_rhoe[pc] = (0.0 + prhoe) * 1.0;
+        f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
+        f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
+        f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
+    }
+}
+
+// Sedimentation apply and KIN2EIN of interior cell c of a single-cell region
+// (src/mod_precip_transport.cpp:960-1011) from the k_flux workspace: returns rhog,
+// rhoge and QR, QS, QG after transport and stores the transported momentum.
+__device__ __forceinline__ void sediment_cell(const StepArgs &a, int l, int ij, int c, double &rg_out,
+                                              double &e_out, double rq[6])
+{
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const GridDev &g = *a.grid;
+    const Fields &f = a.f;
+    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    const int64_t p3 = (int64_t)l * NQ * K2 + ij;
+#define FXV(slot, wall) fx[(int64_t)(slot) * K2 + (int64_t)(wall) * ijdim]
+
+    // tendencies of cell c (walls c and c+1; wall 95 carries zero fluxes)
+    const double rc = g.rdgz[c];
+    double dq[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) dq[t] = -(FXV(FX_Q + t, c + 1) - FXV(FX_Q + t, c)) * rc;
+    const double de = -(FXV(FX_E, c + 1) - FXV(FX_E, c)) * rc;
+    const double dphi = -(FXV(FX_PHI, c + 1) - FXV(FX_PHI, c)) * rc;
+    const double dkinh = -(FXV(FX_KINH, c + 1) - FXV(FX_KINH, c)) * rc;
+    const double du = -(FXV(FX_U, c + 1) - FXV(FX_U, c)) * rc;
+    const double dv = -(FXV(FX_V, c + 1) - FXV(FX_V, c)) * rc;
+    // mass tendencies of cells c+1 and c-1
+    double dqp[3] = {0.0, 0.0, 0.0}, dqm[3] = {0.0, 0.0, 0.0};
+    if (c + 1 <= KMAX) {
+        const double r = g.rdgz[c + 1];
+#pragma unroll
+        for (int t = 0; t < 3; t++) dqp[t] = -(FXV(FX_Q + t, c + 2) - FXV(FX_Q + t, c + 1)) * r;
+    }
+    if (c - 1 >= KMIN) {
+        const double r = g.rdgz[c - 1];
+#pragma unroll
+        for (int t = 0; t < 3; t++) dqm[t] = -(FXV(FX_Q + t, c) - FXV(FX_Q + t, c - 1)) * r;
+    }
+    // half-level tendencies at walls c and c+1 (walls 2..94; the flux of wall 1 is 0)
+    double hk_c = 0.0, hw_c = 0.0, hk_p = 0.0, hw_p = 0.0;
+    if (c >= KMIN + 1) {
+        const double r = g.rdgzh[c];
+        hk_c = -(FXV(FX_KINV, c) - FXV(FX_KINV, c - 1)) * r;
+        hw_c = -(FXV(FX_W, c) - FXV(FX_W, c - 1)) * r;
+    }
+    if (c + 1 <= KMAX) {
+        const double r = g.rdgzh[c + 1];
+        hk_p = -(FXV(FX_KINV, c + 1) - FXV(FX_KINV, c)) * r;
+        hw_p = -(FXV(FX_W, c + 1) - FXV(FX_W, c)) * r;
+    }
+    const double rw_c = FXV(FX_RW, c), rw_p = FXV(FX_RW, c + 1);
+#undef FXV
+
+    // pre-transport wall quantities of cells c and c+1 (C2W factors, rhogkin_v)
+    const int64_t o = p2 + (int64_t)c * ijdim;
+    const double rg_m = a.ws.rhog[o - ijdim], rg_c = a.ws.rhog[o], rg_p = a.ws.rhog[o + ijdim];
+    const double gs_m = f.gsgam2[o - ijdim], gs_c = f.gsgam2[o], gs_p = f.gsgam2[o + ijdim];
+    const double gsh_c = f.gsgam2h[o], gsh_p = f.gsgam2h[o + ijdim];
+    const double c2w0_c = g.afact[c] / gs_c * gsh_c, c2w1_c = g.bfact[c] / gs_m * gsh_c;
+    const double c2w0_p = g.afact[c + 1] / gs_p * gsh_p, c2w1_p = g.bfact[c + 1] / gs_c * gsh_p;
+    const double kinv0_c = (c >= KMIN + 1) ? 0.5 * (rw_c * rw_c) / (c2w0_c * rg_c + c2w1_c * rg_m) : 0.0;
+    const double kinv0_p = (c + 1 <= KMAX) ? 0.5 * (rw_p * rw_p) / (c2w0_p * rg_p + c2w1_p * rg_c) : 0.0;
+
+    const int64_t pc = (int64_t)l * ijdim + ij;
+    const double ix = f.ix[pc], iy = f.iy[pc], iz = f.iz[pc];
+    const double jx = f.jx[pc], jy = f.jy[pc], jz = f.jz[pc];
+    const int64_t oq = p3 + (int64_t)c * ijdim;
+    rq[QR] = a.ws.rhogq[oq + QR * K2];
+    rq[QS] = a.ws.rhogq[oq + QS * K2];
+    rq[QG] = a.ws.rhogq[oq + QG * K2];
+    const double re0 = a.ws.rhoge[o];
+    const double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
+    const double z = f.z[o];
+    const double w2c0 = g.cfact[c] * gs_c / gsh_p, w2c1 = g.dfact[c] * gs_c / gsh_p;
+    const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / rg_c;
+    double e = re0 + de + dphi;
+    const double kinh_u = kinh + dkinh;
+    const double kinv_c = kinv0_c + hk_c;
+    const double kinv_cp = kinv0_p + hk_p;
+    const double rvx_n = rvx + du * ix + dv * jx;
+    const double rvy_n = rvy + du * iy + dv * jy;
+    const double rvz_n = rvz + du * iz + dv * jz;
+    const double rw_n = rw_c + hw_c;
+    double rg = rg_c;
+    rq[QR] = rq[QR] + dq[0];
+    rg = rg + dq[0];
+    e = e - dq[0] * C_GRAV * z;
+    rq[QS] = rq[QS] + dq[1];
+    rg = rg + dq[1];
+    e = e - dq[1] * C_GRAV * z;
+    rq[QG] = rq[QG] + dq[2];
+    rg = rg + dq[2];
+    e = e - dq[2] * C_GRAV * z;
+    // KIN2EIN: kinetic energy before minus after, with the neighbours' new densities
+    const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
+    const double rg_cp = ((rg_p + dqp[0]) + dqp[1]) + dqp[2];
+    const double rg_cm = ((rg_m + dqm[0]) + dqm[1]) + dqm[2];
+    const double rw_cp = rw_p + hw_p;
+    const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
+    const double kinv_new_c = (c >= KMIN + 1) ? 0.5 * (rw_n * rw_n) / (c2w0_c * rg + c2w1_c * rg_cm) : 0.0;
+    const double kinv_new_cp = (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (c2w0_p * rg_cp + c2w1_p * rg) : 0.0;
+    const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
+    e = e + (tmp2 - tmpk);
+    f.rhogvx[o] = rvx_n;
+    f.rhogvy[o] = rvy_n;
+    f.rhogvz[o] = rvz_n;
+    f.rhogw[o] = rw_n;
+    rg_out = rg;
+    e_out = e;
+}
+
+// ---------------------------------------------------------------------------------
+// k_finish: one thread per point.  The sedimented state comes from sediment_cell in
+// the single-cell regions and from the general sweep elsewhere; then the point-wise
+// tail of the step: rediagnosis (src/mod_precip_transport.cpp:1074-1086),
+// SATURATION_adjustment with satadjust_liq (src/mod_satadjust.cpp:274-362, 771-896)
+// and the final negative_filter (src/mod_mp_nsw6.cpp:671, 1770-1831).
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_finish(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
+    if (!lb.valid) return;
+    const int l = lb.l, k = lb.k;
+    const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
     const Fields &f = a.f;
-    const DevParams &P = a.P;
+    const DevParams &P = *a.P;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
     const bool interior = (k >= KMIN && k <= KMAX);
+    const bool general = a.ws.region_flag[l] != 0;
 
-    // State after sedimentation.  The sweep wrote rhog, rhoge, QR, QS, QG of the
-    // interior cells; QV, QC, QI and the halo levels only receive zero tendencies.
+    // State after sedimentation.  QV, QC, QI and the halo levels only receive zero
+    // tendencies.
     double rhog, rhoge, rhogq[6];
     if (interior) {
-        rhog = f.rhog[p];
-        rhoge = f.rhoge[p];
-        rhogq[QR] = f.rhogq[pq + QR * K2];
-        rhogq[QS] = f.rhogq[pq + QS * K2];
-        rhogq[QG] = f.rhogq[pq + QG * K2];
+        if (general) {
+            rhog = f.rhog[p];
+            rhoge = f.rhoge[p];
+            rhogq[QR] = f.rhogq[pq + QR * K2];
+            rhogq[QS] = f.rhogq[pq + QS * K2];
+            rhogq[QG] = f.rhogq[pq + QG * K2];
+        } else {
+            sediment_cell(a, l, ij, k, rhog, rhoge, rhogq);
+        }
     } else {
         rhog = a.ws.rhog[p] + 0.0;
         rhoge = a.ws.rhoge[p] + 0.0;
         rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
         rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
         rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+        if (!general) {
+            const int64_t pc = (int64_t)l * ijdim + ij;
+            f.rhogvx[p] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
+            f.rhogvy[p] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
+            f.rhogvz[p] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
+            f.rhogw[p] = f.rhogw[p] + 0.0;
+        }
     }
     rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
     rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
@@ -673,395 +1132,6 @@ __global__ void __launch_bounds__(256) k_adjust(StepArgs a)
     f.GDCFRC[p] = 0.0 * 1.0;
 }
 
-// Post-sedimentation writes of one interior cell: rhog, rhoge and the sedimenting
-// tracers (the rest of the state is finished by k_adjust).
-__device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, double rg, const double rq[6], double e)
-{
-    const Fields &f = s.a->f;
-    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
-    f.rhog[o] = rg;
-    f.rhoge[o] = e;
-    f.rhogq[s.p3 + (int64_t)(QR * KALL + c) * s.ijdim] = rq[QR];
-    f.rhogq[s.p3 + (int64_t)(QS * KALL + c) * s.ijdim] = rq[QS];
-    f.rhogq[s.p3 + (int64_t)(QG * KALL + c) * s.ijdim] = rq[QG];
-}
-
-// ---------------------------------------------------------------------------------
-// k_sweep_fast: the sweep for regions whose departure cells all lie in {k, k+1}
-// (checked by k_bounds).  Each semi-Lagrangian flux then has the closed form of
-// vadv1d_getflux_new for a range [k, k+1]:
-//     kc = k   : f = rhof(k) * zdis0
-//     kc = k+1 : f = -rhof(k) * dz(k) + rhof(k+1) * zdis0   (zdis0 kept, not refreshed)
-// and the half level uses the full level's kcell_max as the reference does.  Flux divergences of the non-mass quantities
-// are taken on tracer-summed fluxes (a reordering of the reference's sum).
-// ---------------------------------------------------------------------------------
-struct Raw {
-    double rhog, rq[5], rhoge, vt[3];
-    double gs, gam2, gsh, gam2h;
-    double rvx, rvy, rvz, rw, vx, vy, vz, z;
-};
-struct Cell {
-    double rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
-    double tem, z, kinh, up, vp;
-    double hv[3][2];                 // half-level values at the cell's lower wall
-    double rhog, rhogw, kinv, c2w0, c2w1, gs;
-};
-struct WallRec {
-    double rhog, rhogw, kinv, c2w0, c2w1, gs;
-};
-struct Tend {
-    double dq[3], de, dphi, dkinh, du, dv;
-};
-struct HTend {
-    double dkinv, dw;
-};
-
-__device__ __forceinline__ void load_raw(const SweepCtx &s, int j, Raw &r)
-{
-    const StepArgs &a = *s.a;
-    const Fields &f = a.f;
-    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
-    const int64_t oq = s.p3 + (int64_t)j * s.ijdim;
-    r.rhog = a.ws.rhog[o];
-#pragma unroll
-    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * s.K2];
-    r.rhoge = a.ws.rhoge[o];
-#pragma unroll
-    for (int t = 0; t < 3; t++) r.vt[t] = a.ws.vt[s.pv + t * s.K2 + (int64_t)j * s.ijdim];
-    r.gs = f.gsgam2[o];
-    r.gam2 = f.gam2[o];
-    r.gsh = f.gsgam2h[o];
-    r.gam2h = f.gam2h[o];
-    r.rvx = f.rhogvx[o];
-    r.rvy = f.rhogvy[o];
-    r.rvz = f.rhogvz[o];
-    r.rw = f.rhogw[o];
-    r.vx = f.vx[o];
-    r.vy = f.vy[o];
-    r.vz = f.vz[o];
-    r.z = f.z[o];
-}
-
-__device__ __forceinline__ void derive_cell(const SweepCtx &s, const GridDev &g, int j, const Raw &r,
-                                            const Cell &below, Cell &c)
-{
-    const bool interior = (j >= KMIN && j <= KMAX);
-    const double rg = r.rhog;
-    const double rgs = interior ? r.gam2 / r.gs : 0.0;
-    double qq[5];
-#pragma unroll
-    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
-    double qd = 1.0;
-#pragma unroll
-    for (int n = 0; n < 5; n++) qd = qd - qq[n];
-    double cva = qd * C_CVdry;
-#pragma unroll
-    for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
-    c.tem = interior ? r.rhoge / (rg * cva) : 0.0;
-    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : 0.0;
-    c.up = r.vx * s.ix + r.vy * s.iy + r.vz * s.iz;
-    c.vp = r.vx * s.jx + r.vy * s.jy + r.vz * s.jz;
-    c.z = r.z;
-    // tracer order inside rq/qq: QC, QR, QI, QS, QG
-    c.q[0] = qq[1];
-    c.q[1] = qq[3];
-    c.q[2] = qq[4];
-    c.rhoq[0] = r.rq[1] * rgs;
-    c.rhoq[1] = r.rq[3] * rgs;
-    c.rhoq[2] = r.rq[4] * rgs;
-#pragma unroll
-    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
-    c.rhog = rg;
-    c.rhogw = r.rw;
-    c.gs = r.gs;
-    if (j >= KMIN) {
-        c.c2w0 = g.afact[j] / r.gs * r.gsh;
-        c.c2w1 = g.bfact[j] / below.gs * r.gsh;
-    } else {
-        c.c2w0 = 0.0;
-        c.c2w1 = 0.0;
-    }
-    if (j >= KMIN + 1 && j <= KMAX) {
-        c.kinv = 0.5 * (r.rw * r.rw) / (c.c2w0 * rg + c.c2w1 * below.rhog);
-        const double rgsh = r.gam2h / r.gsh;
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const double qh = 0.5 * (c.q[t] + below.q[t]) * rgsh;
-            c.hv[t][0] = qh * c.kinv;
-            c.hv[t][1] = qh * r.rw;
-        }
-    } else {
-        c.kinv = 0.0;
-#pragma unroll
-        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = 0.0;
-    }
-}
-
-__device__ __forceinline__ double rhof_cell(const Cell &c, int t, int i)
-{
-    switch (i) {
-    case 0: return c.rhoq[t];
-    case 1: return c.rhoq[t] * C_CVW * c.tem;
-    case 2: return c.rhoq[t] * C_GRAV * c.z;
-    case 3: return c.qrgs[t] * c.kinh;
-    case 4: return c.rhoq[t] * c.up;
-    default: return c.rhoq[t] * c.vp;
-    }
-}
-__device__ __forceinline__ WallRec wall_of(const Cell &c)
-{
-    WallRec w;
-    w.rhog = c.rhog;
-    w.rhogw = c.rhogw;
-    w.kinv = c.kinv;
-    w.c2w0 = c.c2w0;
-    w.c2w1 = c.c2w1;
-    w.gs = c.gs;
-    return w;
-}
-
-// Apply the sedimentation tendencies and KIN2EIN to interior cell c
-// (src/mod_precip_transport.cpp:960-1011) and store the result.
-__device__ __forceinline__ void finish_cell(const SweepCtx &s, const GridDev &g, int c, const WallRec &wc,
-                                            const WallRec &wcp, const Tend &tc, const Tend &tcp,
-                                            const HTend &hc, const HTend &hcp, double &rhog_new_prev)
-{
-    const StepArgs &a = *s.a;
-    const Fields &f = a.f;
-    const int64_t o = s.p2 + (int64_t)c * s.ijdim;
-    const int64_t oq = s.p3 + (int64_t)c * s.ijdim;
-    double rq[6];
-    rq[QR] = a.ws.rhogq[oq + QR * s.K2];
-    rq[QS] = a.ws.rhogq[oq + QS * s.K2];
-    rq[QG] = a.ws.rhogq[oq + QG * s.K2];
-    const double re0 = a.ws.rhoge[o];
-    const double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
-    const double z = f.z[o];
-    const double gshp = f.gsgam2h[o + s.ijdim];
-    const double w2c0 = g.cfact[c] * wc.gs / gshp, w2c1 = g.dfact[c] * wc.gs / gshp;
-    const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / wc.rhog;
-    double e = re0 + tc.de + tc.dphi;
-    const double kinh_u = kinh + tc.dkinh;
-    const double kinv_c = wc.kinv + hc.dkinv;
-    const double kinv_cp = wcp.kinv + hcp.dkinv;
-    const double rvx_n = rvx + tc.du * s.ix + tc.dv * s.jx;
-    const double rvy_n = rvy + tc.du * s.iy + tc.dv * s.jy;
-    const double rvz_n = rvz + tc.du * s.iz + tc.dv * s.jz;
-    const double rw_n = wc.rhogw + hc.dw;
-    double rg = wc.rhog;
-    rq[QR] = rq[QR] + tc.dq[0];
-    rg = rg + tc.dq[0];
-    e = e - tc.dq[0] * C_GRAV * z;
-    rq[QS] = rq[QS] + tc.dq[1];
-    rg = rg + tc.dq[1];
-    e = e - tc.dq[1] * C_GRAV * z;
-    rq[QG] = rq[QG] + tc.dq[2];
-    rg = rg + tc.dq[2];
-    e = e - tc.dq[2] * C_GRAV * z;
-    const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
-    const double rg_cp = ((wcp.rhog + tcp.dq[0]) + tcp.dq[1]) + tcp.dq[2];
-    const double rw_cp = wcp.rhogw + hcp.dw;
-    const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
-    const double kinv_new_c = (c >= KMIN + 1) ? 0.5 * (rw_n * rw_n) / (wc.c2w0 * rg + wc.c2w1 * rhog_new_prev) : 0.0;
-    const double kinv_new_cp = (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (wcp.c2w0 * rg_cp + wcp.c2w1 * rg) : 0.0;
-    const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
-    e = e + (tmp2 - tmpk);
-    rhog_new_prev = rg;
-    f.rhogvx[o] = rvx_n;
-    f.rhogvy[o] = rvy_n;
-    f.rhogvz[o] = rvz_n;
-    f.rhogw[o] = rw_n;
-    f.rhog[o] = rg;
-    f.rhoge[o] = e;
-    f.rhogq[oq + QR * s.K2] = rq[QR];
-    f.rhogq[oq + QS * s.K2] = rq[QS];
-    f.rhogq[oq + QG * s.K2] = rq[QG];
-}
-
-__global__ void __launch_bounds__(64) k_sweep_fast(StepArgs a)
-{
-    const int ijdim = a.ijdim;
-    const int l = blockIdx.y;
-    if (a.ws.region_flag[l] != 0) return;
-    const int ij = blockIdx.x * 64 + threadIdx.x;
-    if (ij >= ijdim) return;
-    const GridDev &g = *a.grid;
-    const Fields &f = a.f;
-    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
-    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
-
-    SweepCtx s;
-    s.a = &a;
-    s.l = l;
-    s.ij = ij;
-    s.ijdim = ijdim;
-    s.K2 = (int64_t)KALL * ijdim;
-    s.p2 = (int64_t)l * s.K2 + ij;
-    s.p3 = (int64_t)l * NQ * s.K2 + ij;
-    s.pv = (int64_t)l * 3 * s.K2 + ij;
-    const int64_t pc = (int64_t)l * ijdim + ij;
-    s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
-    s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
-
-    Raw rn1;
-    Cell cur, nxt;
-    WallRec wr_m2, wr_m1;
-    Tend t_m2, t_m1;
-    HTend h_m2, h_m1, h_0;
-    WindWindow wind[3];
-    double fq_prev[3], fs_prev[5], fh_prev[2];
-    double rhog_new_prev = 0.0;
-    double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
-
-    {
-        Raw r0;
-        Cell c0;
-        load_raw(s, 0, r0);
-        c0.gs = r0.gs;
-        c0.rhog = r0.rhog;
-        derive_cell(s, g, 0, r0, c0, c0);
-        load_raw(s, 1, r0);
-        derive_cell(s, g, 1, r0, c0, cur);
-        wr_m1 = wall_of(c0);
-#pragma unroll
-        for (int t = 0; t < 3; t++) wind_init(wind[t], r0.vt[t]);
-    }
-#pragma unroll
-    for (int t = 0; t < 3; t++) fq_prev[t] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) fs_prev[i] = 0.0;
-    fh_prev[0] = fh_prev[1] = 0.0;
-    h_m1.dkinv = h_m1.dw = 0.0;   // wall 0
-    h_0.dkinv = h_0.dw = 0.0;     // wall 1
-    h_m2 = h_m1;
-    t_m2 = {};
-    wr_m2 = wr_m1;
-
-    for (int w = KMIN; w <= KMAX + 1; w++) {
-        double fq[3] = {0.0, 0.0, 0.0}, fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fh[2] = {0.0, 0.0};
-        h_0.dkinv = h_0.dw = 0.0;
-        if (w <= KMAX) {
-            load_raw(s, w + 1, rn1);
-            derive_cell(s, g, w + 1, rn1, cur, nxt);
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const double vt_kp1 = (w + 1 <= KMAX) ? rn1.vt[t] : 0.0;
-                double wh_kp1, vh_kp1, whh_kp1;
-                wind_next(wind[t], w, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
-                // full level
-                const double z0 = zdis_full(g, dt, dt2, dt3, wind[t], w, wh_kp1);
-                const int kcmax = bnd[bounds_index(t, 0, w, 1)];
-                const int kc = departure_cell(g.gzh, w, w, kcmax, z0, KMIN, KMAX);
-                const bool up1 = (kcmax != w) && (kc == w + 1);
-                const double dzw = g.dgz[w];
-                double ft[6];
-#pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    const double r0 = rhof_cell(cur, t, i);
-                    const double v = up1 ? (-(r0 * dzw) + rhof_cell(nxt, t, i) * z0) : r0 * z0;
-                    ft[i] = eps_filter(v);
-                }
-                fq[t] = ft[0];
-#pragma unroll
-                for (int i = 1; i < 6; i++) fs[i - 1] += ft[i];
-                if (w == KMIN) {
-                    if (t == 0) {
-                        precip0 = precip0 - ft[0] / dt;
-                    } else {
-                        precip1 = precip1 - ft[0] / dt;
-                        lh = lh + ft[0] * C_LHF0 / dt;
-                    }
-                    prhoe = prhoe - ft[1] / dt;
-                    prhophi = prhophi - ft[2] / dt;
-                    prhokin = prhokin - ft[3] / dt - 0.0 / dt;
-                }
-                if (t == 0) f.GPREC[s.p2 + (int64_t)w * ijdim] = (0.0 + (-ft[0] / dt)) * 1.0;
-                // half level (walls 2..94)
-                if (w >= KMIN + 1) {
-                    const double zh0 = zdis_half(g, dt, dt2, dt3, wind[t], w, whh_kp1);
-                    const int khmax = bnd[bounds_index(t, 1, w, 1)];
-                    const int kch = departure_cell(g.gz_shift, w, w, khmax, zh0, KMIN + 1, KMAX);
-                    const double dzh = g.dgzh[w];
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-                        const double hc = cur.hv[t][i], hn = nxt.hv[t][i];
-                        double v;
-                        if (kcmax == w)  // region-wide fast path of getflux: rhof(kcell) * zdis0
-                            v = ((kch == w + 1) ? hn : hc) * zh0;
-                        else
-                            v = (kch == w + 1) ? (-(hc * dzh) + hn * zh0) : hc * zh0;
-                        fh[i] += eps_filter(v);
-                    }
-                }
-                wind_shift(wind[t], vt_kp1, wh_kp1, vh_kp1, whh_kp1);
-            }
-        }
-        // divergence of cell w-1 (cells 1..94) and of wall w (walls 2..94)
-        t_m1 = {};
-        if (w - 1 >= KMIN) {
-            const double r = g.rdgz[w - 1];
-#pragma unroll
-            for (int t = 0; t < 3; t++) t_m1.dq[t] = -(fq[t] - fq_prev[t]) * r;
-            t_m1.de = -(fs[0] - fs_prev[0]) * r;
-            t_m1.dphi = -(fs[1] - fs_prev[1]) * r;
-            t_m1.dkinh = -(fs[2] - fs_prev[2]) * r;
-            t_m1.du = -(fs[3] - fs_prev[3]) * r;
-            t_m1.dv = -(fs[4] - fs_prev[4]) * r;
-        }
-        if (w >= KMIN + 1 && w <= KMAX) {
-            const double r = g.rdgzh[w];
-            h_0.dkinv = -(fh[0] - fh_prev[0]) * r;
-            h_0.dw = -(fh[1] - fh_prev[1]) * r;
-        }
-#pragma unroll
-        for (int t = 0; t < 3; t++) fq_prev[t] = fq[t];
-#pragma unroll
-        for (int i = 0; i < 5; i++) fs_prev[i] = fs[i];
-        fh_prev[0] = fh[0];
-        fh_prev[1] = fh[1];
-
-        // finish cell w-2 (needs the tendencies of cell w-1)
-        if (w - 2 >= KMIN) finish_cell(s, g, w - 2, wr_m2, wr_m1, t_m2, t_m1, h_m2, h_m1, rhog_new_prev);
-
-        // shift
-        t_m2 = t_m1;
-        h_m2 = h_m1;
-        h_m1 = h_0;
-        wr_m2 = wr_m1;
-        wr_m1 = wall_of(cur);
-        cur = nxt;
-    }
-    // cell 94: its upper neighbour is the halo cell 95 (wall 95 carries no flux)
-    {
-        HTend hz;
-        hz.dkinv = hz.dw = 0.0;
-        Tend tz = {};
-        finish_cell(s, g, KMAX, wr_m2, wr_m1, t_m2, tz, h_m2, hz, rhog_new_prev);
-    }
-    // halo cells: zero tendencies on the momentum (scalars are finished by k_adjust)
-#pragma unroll
-    for (int hc = 0; hc < 2; hc++) {
-        const int c = hc ? (KALL - 1) : 0;
-        const int64_t o = s.p2 + (int64_t)c * ijdim;
-        f.rhogvx[o] = f.rhogvx[o] + 0.0 * s.ix + 0.0 * s.jx;
-        f.rhogvy[o] = f.rhogvy[o] + 0.0 * s.iy + 0.0 * s.jy;
-        f.rhogvz[o] = f.rhogvz[o] + 0.0 * s.iz + 0.0 * s.jz;
-        f.rhogw[o] = f.rhogw[o] + 0.0;
-        f.GPREC[o] = (0.0 + 0.0) * 1.0;
-    }
-    const int64_t pp = (int64_t)l * 2 * ijdim + ij;
-    f.precip[pp] = (0.0 + precip0) * 1.0;
-    f.precip[pp + ijdim] = (0.0 + precip1) * 1.0;
-    f.ISO1_precip[pp] = 0.0;
-    f.ISO1_precip[pp + ijdim] = 0.0;
-    f.ISO2_precip[pp] = 0.0;
-    f.ISO2_precip[pp + ijdim] = 0.0;
-    f.precip_rhoe[pc] = (0.0 + prhoe) * 1.0;
-    f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
-    f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
-    f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
-}
-
 template <bool GENERAL>
 __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
 {
@@ -1088,7 +1158,8 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
     s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
     const double *vt = a.ws.vt + s.pv;
-    double *mom = GENERAL ? (a.ws.mom + (int64_t)l * 4 * s.K2 + ij) : nullptr;
+    // deferred momentum: slots 0..3 of the region's flux block (k_flux skips general regions)
+    double *mom = GENERAL ? (a.ws.flux + (int64_t)l * FLUX_SLOTS * s.K2 + ij) : nullptr;
 
     // window: cells w (cur) and w+1 (nxt); finalisation records for w-2 .. w+1
     TCell cur, nxt;
@@ -1375,7 +1446,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         }
         store_sedimented(s, c, rg, rq, e);
     }
-    // halo cells 0 and 95: zero tendencies (their scalars are finished by k_adjust)
+    // halo cells 0 and 95: zero tendencies (their scalars are finished by k_finish)
 #pragma unroll
     for (int hc = 0; hc < 2; hc++) {
         const int c = hc ? (KALL - 1) : 0;
@@ -1429,20 +1500,21 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     const StepArgs &a = *args;
     const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
     const dim3 grid_c((a.ijdim + 63) / 64, a.lall);
-    const dim3 grid_e((a.ijdim + EXT_COLS_PER_WAVE - 1) / EXT_COLS_PER_WAVE, 3, a.lall);
+    const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
+    const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (precision == 1)
         hipLaunchKernelGGL(k_rates<float>, grid_p, dim3(256), 0, stream, a);
     else
         hipLaunchKernelGGL(k_rates<double>, grid_p, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
-    hipLaunchKernelGGL(k_extremes, grid_e, dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[2], stream);
-    hipLaunchKernelGGL(k_sweep_fast, grid_c, dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_flux, grid_l, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(k_sweep<true>, grid_c, dim3(64), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    hipLaunchKernelGGL(k_adjust, grid_p, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_finish, grid_l, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
     return hipGetLastError();
 }
