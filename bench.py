@@ -53,6 +53,55 @@ def read_traffic(path):
     return None
 
 
+def mixed_error(w, steps):
+    """Per-field error and column budget drift of the mixed form against the FP64 form
+    (itself held to the oracle within 1e-10 by the parity tests), on region 0 of the
+    workload: the error of one mixed step from each state of a `steps`-step FP64
+    trajectory (worst over the steps), and the divergence of a free-running mixed
+    trajectory after `steps` steps."""
+    import numpy as np
+    import torch
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.check import budget_drift, field_errors
+    from nicam_kokkos_microphysics_amd.driver import NSW6, grid_setup
+    gz, gzh, st = F.workload_state(w, lall=1)
+    dgz = np.ctypeslib.as_array(grid_setup(gz, gzh).dgz)
+    keys = [k for k in F.OUTPUTS if k not in ("rhogq", "q")] + \
+        [f"rhogq:{t}" for t in F.TRACERS] + [f"q:{t}" for t in F.TRACERS]
+    a, b = F.to_device(st), F.to_device(st)
+    d64 = NSW6(gz, gzh, max_columns=w.ijdim, precision=N.NSW6_FP64)
+    dmx = NSW6(gz, gzh, max_columns=w.ijdim, precision=N.NSW6_MIXED)
+    one_step = {k: 0.0 for k in keys}
+    one_drift = {}
+    for _ in range(steps):
+        c = F.copy_state(a)
+        d64.step(a, w.dt)
+        dmx.step(c, w.dt)
+        dmx.step(b, w.dt)
+        torch.cuda.synchronize()
+        ha, hc = F.to_host(a), F.to_host(c)
+        for k, v in field_errors(hc, ha, keys).items():
+            one_step[k] = max(one_step[k], v[2])
+        for k, v in budget_drift(hc, ha, dgz).items():
+            one_drift[k] = max(one_drift.get(k, 0.0), v)
+        del c
+    torch.cuda.synchronize()
+    d64.close()
+    dmx.close()
+    ha, hb = F.to_host(a), F.to_host(b)
+    free = {k: v[2] for k, v in field_errors(hb, ha, keys).items()}
+    worst1 = max(one_step.items(), key=lambda kv: kv[1])
+    worstf = max(free.items(), key=lambda kv: kv[1])
+    return {"max_field_error": worst1[1], "worst_field": worst1[0], "per_field": one_step,
+            "budget_drift": one_drift,
+            "free_running": {"max_field_error": worstf[1], "worst_field": worstf[0],
+                             "budget_drift": budget_drift(hb, ha, dgz)},
+            "sample": f"region 0 ({w.ijdim} columns): worst one-step error over a {steps}-step FP64 "
+                      f"trajectory, and a {steps}-step free-running mixed trajectory; error / max-abs "
+                      f"of the FP64 field"}
+
+
 def cpu_baseline(reps: int):
     """The CPU oracle (an OpenMP FP64 port of the reference path) on one region of
     16,900 columns: 1 warm-up, median of `reps` timed steps of the mp_driver call."""
@@ -145,6 +194,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     drv.close()
     del dev
+    if prec == N.NSW6_MIXED:
+        out["error_vs_fp64"] = mixed_error(w, w.steps)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

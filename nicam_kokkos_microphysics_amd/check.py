@@ -41,3 +41,35 @@ def compare_states(state, reference, keys, rel_tol: float) -> dict:
         if not (ratio <= rel_tol):
             bad[key] = (err, scale, ratio)
     return bad
+
+
+def field_errors(state, reference, keys) -> dict:
+    """Per-field (max |a-b|, max |b|, ratio) for every key."""
+    return {key: maxabs_error(field_view(state, key), field_view(reference, key)) for key in keys}
+
+
+def column_budgets(state, dgz) -> dict:
+    """Column integrals over the interior levels k = 1..94 of dry+moist mass (rhog),
+    total water (sum of the 6 tracers of rhogq) and internal energy (rhoge), each
+    weighted by the layer thickness dgz; shape [l][ij] per budget.  The surface
+    precipitation of the step is the water that left the column."""
+    dz = np.asarray(dgz, dtype=np.float64)[1:95]
+    rhog = np.asarray(state["rhog"], dtype=np.float64)[:, 1:95, :]
+    rhogq = np.asarray(state["rhogq"], dtype=np.float64)[:, :, 1:95, :]
+    rhoge = np.asarray(state["rhoge"], dtype=np.float64)[:, 1:95, :]
+    return {
+        "mass": np.einsum("lki,k->li", rhog, dz),
+        "water": np.einsum("lnki,k->li", rhogq, dz),
+        "energy": np.einsum("lki,k->li", rhoge, dz),
+    }
+
+
+def budget_drift(state, reference, dgz) -> dict:
+    """max over columns of |budget(state) - budget(reference)| / max |budget(reference)|."""
+    a = column_budgets(state, dgz)
+    b = column_budgets(reference, dgz)
+    out = {}
+    for k in a:
+        scale = float(np.max(np.abs(b[k])))
+        out[k] = float(np.max(np.abs(a[k] - b[k]))) / scale if scale > 0 else 0.0
+    return out

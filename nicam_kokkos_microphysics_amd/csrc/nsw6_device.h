@@ -114,15 +114,96 @@ __device__ __forceinline__ void bergeron(const double *a1_tab, const double *a2_
     a1 = v1 * pow(1.0E-3, ma2);
 }
 
+// Discrete switches of TagBigLoop that depend only on the point's input state
+// (src/mod_mp_nsw6.cpp:935-951, 956, 971, 997, 1060, 1094, 1195-1207).  Both
+// forms evaluate them in FP64: on FP32-rounded inputs a switch flips near its
+// threshold (0 degC, q = 1e-4, ...) and changes the point's tendencies by O(1).
+struct RateSwitches {
+    double delta1, delta2, spsati, iceflg, sw_bergeron;
+    double zr, zs, zx, zg, zcr;   // zerosw of rain, snow (non-Roh), Roh Xs2, graupel, qc*qr
+    double sw_hom, sw_htr, sw_mlt;
+};
+__device__ __forceinline__ RateSwitches rate_switches(const DevParams &P, double dens, double temp,
+                                                      const double q[6], double qsati)
+{
+    RateSwitches s;
+    const double qv = dmax(q[QV], 0.0), qc = dmax(q[QC], 0.0), qr = dmax(q[QR], 0.0);
+    const double qs = dmax(q[QS], 0.0), qg = dmax(q[QG], 0.0);
+    const double Sice = qv / dmax(qsati, C_EPS);
+    const double temc = temp - C_TEM00;
+    s.delta1 = step_pos(qr - 1.0E-4);
+    s.delta2 = step_pos(1.0E-4 - qr) * step_pos(1.0E-4 - qs);
+    s.spsati = step_pos(Sice - 1.0);
+    s.iceflg = step_neg(temc);
+    s.sw_bergeron = step_pos(temc + 30.0) * step_pos(0.0 - temc) * (1.0 - P.sw_expice);
+    s.zr = step_neg(qr - 1.0E-12);
+    s.zs = step_neg(qs - 1.0E-12);
+    s.zx = step_neg(dens * qs / P.As - 1.0E-12);
+    s.zg = step_neg(qg - 1.0E-12);
+    s.zcr = step_neg(qc * qr - 1.0E-12);
+    s.sw_hom = step_neg(temc + 40.0);
+    s.sw_htr = step_pos(temc + 40.0) * step_neg(temc);
+    s.sw_mlt = step_pos(temc);
+    return s;
+}
+
+// Effective radius of liquid water and the cloud-top diagnostics of one point
+// (src/mod_mp_nsw6.cpp:1566-1592), in FP64 for both forms.
+__device__ __forceinline__ void effective_radius(const DevParams &P, const RateSwitches &sw, double dens,
+                                                 double temp, const double q[6], double &rceff,
+                                                 double &rceff_cld, double &rctop, double &tctop)
+{
+    const double qc = dmax(q[QC], 0.0), qr = dmax(q[QR], 0.0);
+    const double zr = sw.zr;
+    const double RLMDr = sqrt(sqrt(dens * qr / (P.Ar * P.N0r * P.GAM_1br) + zr)) * (1.0 - zr);
+    const double RLMDr_3 = (RLMDr * RLMDr) * RLMDr;
+    const double Nc = P.Nc_def;
+    const double rhoqc = dens * qc * 1000.0;
+    const double coef_xf = P.coef_xf;
+    const double xf_qc = rhoqc / Nc * 1.0E-9;
+    const double rf_qc = exp(0.33333333 * log(coef_xf * xf_qc + C_EPS));
+    const double r2_qc = P.coef_dgam * (rf_qc * rf_qc) * (Nc * 1.0E+6);
+    const double r2_qr = 0.25 * P.N0r * P.GAM_3 * RLMDr_3;
+    const double r3_qc = coef_xf * dens * qc;
+    const double r3_qr = coef_xf * dens * qr;
+    double zerosw = step_neg((r2_qc + r2_qr) - 1.0E-10);
+    rceff = (r3_qc + r3_qr) / (r2_qc + r2_qr + zerosw) * (1.0 - zerosw);
+    const double s = step_pos((qc + qr) - 1.0E-5) * zerosw;
+    rctop = (s)*rceff + (1.0 - s) * C_UNDEF;
+    tctop = (s)*temp + (1.0 - s) * C_UNDEF;
+    zerosw = step_neg(r2_qc - 1.0E-10);
+    rceff_cld = r3_qc / (r2_qc + zerosw) * (1.0 - zerosw);
+}
+
 // Input of one rate evaluation.
 template <typename R> struct RateIn {
     R dens, temp, q[6], qsatl, qsati, pre, rhog, a1, a2, ma2;
+    RateSwitches sw;   // FP64 switches (exactly 0 or 1)
+    // differences of nearly equal FP64 inputs, formed in FP64 and rounded once (in
+    // FP32 they would lose most digits near 0 degC and near saturation)
+    R temc;            // temp - TEM00
+    R sice_m1;         // Sice - 1
+    R sliq_1m;         // 1 - min(Sliq, 1)
+    R qv_qsati;        // max(qv, 0) - qsati
 };
-// Output of one rate evaluation: tendencies * rhog * dt, fall speeds, radii.
+// Fill the FP64-formed differences of RateIn from the FP64 inputs.
+template <typename R>
+__device__ __forceinline__ void rate_differences(RateIn<R> &in, double temp, const double q[6], double qsatl,
+                                                 double qsati)
+{
+    const double qv = dmax(q[QV], 0.0);
+    const double Sliq = qv / dmax(qsatl, C_EPS);
+    const double Sice = qv / dmax(qsati, C_EPS);
+    in.temc = (R)(temp - C_TEM00);
+    in.sice_m1 = (R)(Sice - 1.0);
+    in.sliq_1m = (R)(1.0 - dmin(Sliq, 1.0));
+    in.qv_qsati = (R)(qv - qsati);
+}
+// Output of one rate evaluation: the limited tendencies q_t (1/s) of QC..QG (the
+// caller closes QV and forms rhog * q_t * dt in FP64) and the fall speeds.
 template <typename R> struct RateOut {
-    R drhogq[6];
+    R qt[6];
     R Vtr, Vti, Vts, Vtg;
-    R rceff, rceff_cld, rctop, tctop;
 };
 
 // TagBigLoop at one point.
@@ -140,23 +221,21 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R qv = dmax(in.q[QV], R(0)), qc = dmax(in.q[QC], R(0)), qr = dmax(in.q[QR], R(0));
     const R qi = dmax(in.q[QI], R(0)), qs = dmax(in.q[QS], R(0)), qg = dmax(in.q[QG], R(0));
 
-    const R Sliq = qv / dmax(in.qsatl, EPS);
-    const R Sice = qv / dmax(in.qsati, EPS);
     const R Rdens = R(1) / dens;
     const R rho_fact = sqrt(R(DENS00) * Rdens);
-    const R temc = temp - TEM00;
+    const R temc = in.temc;
 
-    const R delta1 = step_pos(qr - R(1.0E-4));
-    const R delta2 = step_pos(R(1.0E-4) - qr) * step_pos(R(1.0E-4) - qs);
-    const R spsati = step_pos(Sice - R(1));
-    const R iceflg = step_neg(temc);
+    const R delta1 = R(in.sw.delta1);
+    const R delta2 = R(in.sw.delta2);
+    const R spsati = R(in.sw.spsati);
+    const R iceflg = R(in.sw.iceflg);
     const R dqv_dt = qv / dt, dqc_dt = qc / dt, dqr_dt = qr / dt;
     const R dqi_dt = qi / dt, dqs_dt = qs / dt, dqg_dt = qg / dt;
     const R sw_expice = R(P.sw_expice);
-    const R sw_bergeron = step_pos(temc + R(30)) * step_pos(R(0) - temc) * (R(1) - sw_expice);
+    const R sw_bergeron = R(in.sw.sw_bergeron);
 
     // slope parameter lambda (rain)
-    R zerosw = step_neg(qr - R(1.0E-12));
+    R zerosw = R(in.sw.zr);
     const R RLMDr = sqrt(sqrt(dens * qr / (Ar * N0r * R(P.GAM_1br)) + zerosw)) * (R(1) - zerosw);
     const R RLMDr_dr = sqrt(RLMDr);
     const R RLMDr_2 = RLMDr * RLMDr;
@@ -174,7 +253,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     R MOMs_0 = R(0), MOMs_1 = R(0), MOMs_2 = R(0), MOMs_0bs = R(0), MOMs_1bs = R(0), MOMs_2bs = R(0);
     R MOMs_2ds = R(0), MOMs_5ds_h = R(0), RMOMs_Vt = R(0);
     if (!roh) {
-        zerosw = step_neg(qs - R(1.0E-12));
+        zerosw = R(in.sw.zs);
         const R RLMDs = sqrt(sqrt(dens * qs / (As * N0s * R(P.GAM_1bs)) + zerosw)) * (R(1) - zerosw);
         const R RLMDs_ds = sqrt(sqrt(RLMDs));
         const R RLMDs_2 = RLMDs * RLMDs;
@@ -197,7 +276,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
 
     // Roh and Satoh (2014) bimodal snow moments
     const R Xs2 = dens * qs / As;
-    zerosw = step_neg(Xs2 - R(1.0E-12));
+    zerosw = R(in.sw.zx);
     const R tems = dmin(R(-0.1), temc);
     R at[4], bt[4];
     at[0] = R(P.coef_a[0]) + tems * (R(P.coef_a[1]) + tems * (R(P.coef_a[4]) + tems * R(P.coef_a[8])));
@@ -260,7 +339,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
 #undef NSW6_MOM
 
     // slope parameter lambda (graupel)
-    zerosw = step_neg(qg - R(1.0E-12));
+    zerosw = R(in.sw.zg);
     const R RLMDg = sqrt(sqrt(dens * qg / (Ag * N0g * R(P.GAM_1bg)) + zerosw)) * (R(1) - zerosw);
     const R RLMDg_dg = sqrt(RLMDg);
     const R RLMDg_2 = RLMDg * RLMDg;
@@ -277,7 +356,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     // nucleation
     const R Ni0 = dmax(exp(R(-0.1) * temc), R(1)) * R(1000);
     const R Qi0 = R(P.icenuc_coef) * exp(log(Ni0) * R(1.33)) * Rdens;
-    R Pigen = dmax(dmin(Qi0 - qi, qv - in.qsati), R(0)) / dt;
+    R Pigen = dmax(dmin(Qi0 - qi, in.qv_qsati), R(0)) / dt;
 
     // accretion
     const R Esi = R(P.Esi), Egs = R(P.Egs);
@@ -285,7 +364,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R Egs_mod = dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
     const R GAM_3dr = R(P.GAM_3dr), GAM_3dg = R(P.GAM_3dg);
     const bool kk = (P.sw_kk2000 == 1.0);
-    zerosw = step_neg(qc * qr - R(1.0E-12));
+    zerosw = R(in.sw.zcr);
     const R Pracw_kk = R(67) * exp(log(qc * qr + zerosw) * R(1.15)) * (R(1) - zerosw);
     const R sw_kk = R(P.sw_kk2000);
     R Pracw;
@@ -339,38 +418,41 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     const R Giv = R(1) / (LHS0 / (Ka * temp) * (LHS0 / (Rvap * temp) - R(1)) + R(1) / (Kd * dens * in.qsati));
     const R Gil = R(1) / (LHF0 / (Ka * temc));
     const R ventr = R(P.f1r) * GAM_2 * RLMDr_2 + R(P.f2r) * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * R(P.GAM_5dr_h);
-    R Prevp = R(2) * PI * Rdens * N0r * (R(1) - dmin(Sliq, R(1))) * Glv * ventr;
+    R Prevp = R(2) * PI * Rdens * N0r * in.sliq_1m * Glv * ventr;
 
     const R rhoqi = dmax(dens * qi, EPS);
     const R XNi = dmin(dmax(R(5.38E+7) * exp(log(rhoqi) * R(0.75)), R(1.0E+3)), R(1.0E+6));
     const R XMi = rhoqi / XNi;
     const R Di = dmin(R(P.Di_a) * sqrt(XMi), R(P.Di_max));
-    R tmp = R(4) * Di * XNi * Rdens * (Sice - R(1)) * Giv;
+    R tmp = R(4) * Di * XNi * Rdens * in.sice_m1 * Giv;
     R Pidep = (spsati) * (tmp);
     R Pisub = (R(1) - spsati) * (-tmp);
 
-    R sw = step_neg(temc + R(40));
+    R sw = R(in.sw.sw_hom);
     R Pihom = sw * qc / dt;
-    sw = step_pos(temc + R(40)) * step_neg(temc);
+    sw = R(in.sw.sw_htr);
     const R A_frz = R(P.A_frz), B_frz = R(P.B_frz);
-    R Pihtr = sw * (dens / rho_w * (qc * qc) / (R(P.Nc_ihtr) * R(1.0E+6))) * B_frz * (exp(-A_frz * temc) - R(1));
-    sw = step_pos(temc);
+    // exp(-A_frz temc) - 1 cancels near 0 degC: FP32 takes expm1, FP64 keeps the
+    // reference's form
+    const R frz_em1 = (sizeof(R) == sizeof(float)) ? R(expm1(-A_frz * temc)) : (exp(-A_frz * temc) - R(1));
+    R Pihtr = sw * (dens / rho_w * (qc * qc) / (R(P.Nc_ihtr) * R(1.0E+6))) * B_frz * frz_em1;
+    sw = R(in.sw.sw_mlt);
     R Pimlt = sw * qi / dt;
 
     const R vents = R(P.f1s) * MOMs_1 + R(P.f2s) * sqrt(Cs * rho_fact / Nu) * MOMs_5ds_h;
-    tmp = R(2) * PI * Rdens * (Sice - R(1)) * Giv * vents;
+    tmp = R(2) * PI * Rdens * in.sice_m1 * Giv * vents;
     R Psdep = (spsati) * (tmp);
     R Pssub = (R(1) - spsati) * (-tmp);
     R Psmlt = R(2) * PI * Rdens * Gil * vents + CL * temc / LHF0 * (Psacw + Psacr);
     Psmlt = dmax(Psmlt, R(0));
 
     const R ventg = R(P.f1g) * GAM_2 * RLMDg_2 + R(P.f2g) * sqrt(Cg * rho_fact / Nu * RLMDg_5dg) * R(P.GAM_5dg_h);
-    tmp = R(2) * PI * Rdens * N0g * (Sice - R(1)) * Giv * ventg;
+    tmp = R(2) * PI * Rdens * N0g * in.sice_m1 * Giv * ventg;
     R Pgdep = (spsati) * (tmp);
     R Pgsub = (R(1) - spsati) * (-tmp);
     R Pgmlt = R(2) * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * (Pgacw + Pgacr);
     Pgmlt = dmax(Pgmlt, R(0));
-    R Pgfrz = R(2) * PI * Rdens * N0r * R(60) * B_frz * Ar * (exp(-A_frz * temc) - R(1)) * RLMDr_7;
+    R Pgfrz = R(2) * PI * Rdens * N0r * R(60) * B_frz * Ar * frz_em1 * RLMDr_7;
 
     // Bergeron process (zeroed by sw_bergeron = 0 when explicit ice generation is on)
     R Psfw = R(0), Psfi = R(0);
@@ -481,34 +563,16 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     qi_t = dmax(qi_t, -dqi_dt);
     qs_t = dmax(qs_t, -dqs_dt);
     qg_t = dmax(qg_t, -dqg_dt);
-    const R qv_t = -(qc_t + qr_t + qi_t + qs_t + qg_t);
-
-    o.drhogq[QV] = in.rhog * qv_t * dt;
-    o.drhogq[QC] = in.rhog * qc_t * dt;
-    o.drhogq[QR] = in.rhog * qr_t * dt;
-    o.drhogq[QI] = in.rhog * qi_t * dt;
-    o.drhogq[QS] = in.rhog * qs_t * dt;
-    o.drhogq[QG] = in.rhog * qg_t * dt;
+    o.qt[QV] = R(0);
+    o.qt[QC] = qc_t;
+    o.qt[QR] = qr_t;
+    o.qt[QI] = qi_t;
+    o.qt[QS] = qs_t;
+    o.qt[QG] = qg_t;
     o.Vtr = Vtr;
     o.Vti = Vti;
     o.Vts = Vts;
     o.Vtg = Vtg;
-
-    // effective radius of liquid water
-    const R coef_xf = R(P.coef_xf);
-    const R xf_qc = rhoqc / Nc * R(1.0E-9);
-    const R rf_qc = exp(R(0.33333333) * log(coef_xf * xf_qc + EPS));
-    const R r2_qc = R(P.coef_dgam) * (rf_qc * rf_qc) * (Nc * R(1.0E+6));
-    const R r2_qr = R(0.25) * N0r * GAM_3 * RLMDr_3;
-    const R r3_qc = coef_xf * dens * qc;
-    const R r3_qr = coef_xf * dens * qr;
-    zerosw = step_neg((r2_qc + r2_qr) - R(1.0E-10));
-    o.rceff = (r3_qc + r3_qr) / (r2_qc + r2_qr + zerosw) * (R(1) - zerosw);
-    sw = step_pos((qc + qr) - R(1.0E-5)) * zerosw;
-    o.rctop = (sw)*o.rceff + (R(1) - sw) * UNDEF;
-    o.tctop = (sw)*temp + (R(1) - sw) * UNDEF;
-    zerosw = step_neg(r2_qc - R(1.0E-10));
-    o.rceff_cld = r3_qc / (r2_qc + zerosw) * (R(1) - zerosw);
 }
 
 // Order-preserving map of doubles to unsigned 64-bit integers (for atomic max/min).

@@ -126,14 +126,26 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     in.a1 = (R)a1;
     in.a2 = (R)a2;
     in.ma2 = (R)ma2;
+    in.sw = rate_switches(P, rho, tem, q, qsati);
+    rate_differences(in, tem, q, qsatl, qsati);
     RateOut<R> o;
     rates<R>(P, (R)a.dt, in, o);
+    double rceff, rceff_cld, rct, tct;
+    effective_radius(P, in.sw, rho, tem, q, rceff, rceff_cld, rct, tct);
 
-    // ---- TagUpdateRHOGQ / TagUpdateRHOGE
+    // ---- drhogq = rhog * q_t * dt with qv_t = -(sum of the others) (src/mod_mp_nsw6.cpp:
+    // 1548-1559), in FP64 for both forms so the point's total water is conserved to FP64
+    // round-off; then TagUpdateRHOGQ / TagUpdateRHOGE
+    double qt[NQ];
 #pragma unroll
-    for (int n = 0; n < NQ; n++) rq[n] += (double)o.drhogq[n];
-    rhoge = rhoge - C_LHV0 * (double)o.drhogq[QV] + C_LHF0 * (double)o.drhogq[QI] +
-            C_LHF0 * (double)o.drhogq[QS] + C_LHF0 * (double)o.drhogq[QG];
+    for (int n = 1; n < NQ; n++) qt[n] = (double)o.qt[n];
+    qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
+    double drq[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) drq[n] = rhog * qt[n] * a.dt;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) rq[n] += drq[n];
+    rhoge = rhoge - C_LHV0 * drq[QV] + C_LHF0 * drq[QI] + C_LHF0 * drq[QS] + C_LHF0 * drq[QG];
 
     a.ws.rhog[p] = rhog;
     a.ws.rhoge[p] = rhoge;
@@ -142,12 +154,11 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     a.ws.vt[pv + 0 * K2] = (double)o.Vtr;
     a.ws.vt[pv + 1 * K2] = (double)o.Vts;
     a.ws.vt[pv + 2 * K2] = (double)o.Vtg;
-    f.re_liquid[p] = (double)o.rceff;
-    f.re_cld[p] = (double)o.rceff_cld;
+    f.re_liquid[p] = rceff;
+    f.re_cld[p] = rceff_cld;
     if (k == KMAX) {
         // every level writes the cloud-top row in the reference; the serial order leaves kmax
         const int64_t pc = (int64_t)l * ijdim + ij;
-        const double rct = (double)o.rctop, tct = (double)o.tctop;
         f.rctop[pc] = rct;
         f.tctop[pc] = tct;
         const double sw = step_pos(tct - C_TEM00);

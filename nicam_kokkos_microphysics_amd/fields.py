@@ -160,4 +160,5 @@ def to_host(state: dict) -> dict:
 
 
 def copy_state(state: dict) -> dict:
-    return {k: v.copy() for k, v in state.items()}
+    """Deep copy of a host (numpy) or device (torch) state."""
+    return {k: (v.clone() if hasattr(v, "clone") else v.copy()) for k, v in state.items()}
