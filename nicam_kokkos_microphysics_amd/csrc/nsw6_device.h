@@ -28,9 +28,25 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
+// The point-wise physics is compiled for the device and for the host: the host
+// instantiations serve the dual-number gradient checks of the adjoint.
+#define NSW6_HD __host__ __device__ __forceinline__
+
 namespace nsw6 {
 
+// float arguments take the float overloads on the host as on the device
+using std::copysign;
+using std::exp;
+using std::expm1;
+using std::fabs;
+using std::log;
+using std::pow;
+using std::sqrt;
+
 constexpr int KALL = 96, KMIN = 1, KMAX = 94, NQ = 6;
+constexpr int NCOEF = 8;   // tunable coefficients (order of include/nsw6.h)
 constexpr int QV = 0, QC = 1, QR = 2, QI = 3, QS = 4, QG = 5;
 
 constexpr double C_PI = 3.141592653589793;
@@ -79,29 +95,29 @@ struct DevParams {
     double dtemp_criteria;     // 10^(-(15+1)/2) = 1e-8
 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float dmax(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float dmin(float a, float b) { return (b < a) ? b : a; }
+NSW6_HD double dmax(double a, double b) { return (a < b) ? b : a; }
+NSW6_HD double dmin(double a, double b) { return (b < a) ? b : a; }
+NSW6_HD float dmax(float a, float b) { return (a < b) ? b : a; }
+NSW6_HD float dmin(float a, float b) { return (b < a) ? b : a; }
 
 // 0.5 + copysign(0.5, x): 1 for x >= +0, 0 for x <= -0 (IEEE sign, as the reference).
-template <typename R> __device__ __forceinline__ R step_pos(R x) { return R(0.5) + copysign(R(0.5), x); }
-template <typename R> __device__ __forceinline__ R step_neg(R x) { return R(0.5) - copysign(R(0.5), x); }
+template <typename R> NSW6_HD R step_pos(R x) { return R(0.5) + copysign(R(0.5), x); }
+template <typename R> NSW6_HD R step_neg(R x) { return R(0.5) - copysign(R(0.5), x); }
 
-__device__ __forceinline__ double psat_liq(const DevParams &P, double tem)
+NSW6_HD double psat_liq(const DevParams &P, double tem)
 {
     double rtem = 1.0 / dmax(tem, TEM_MIN);
     // pow(tem*RTEM00, CPovR_liq) with CPovR_liq = 0 under SIMPLE is exactly 1
     return C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
 }
-__device__ __forceinline__ double psat_ice(const DevParams &P, double tem)
+NSW6_HD double psat_ice(const DevParams &P, double tem)
 {
     double rtem = 1.0 / dmax(tem, TEM_MIN);
     return C_PSAT0 * 1.0 * exp(P.LovR_ice * (P.RTEM00 - rtem));
 }
 
 // Bergeron_param at one point (keeps the reference's off-by-one interpolation).
-__device__ __forceinline__ void bergeron(const double *a1_tab, const double *a2_tab, double tem,
+NSW6_HD void bergeron(const double *a1_tab, const double *a2_tab, double tem,
                                          double &a1, double &a2, double &ma2)
 {
     double temc = dmin(dmax(tem - C_TEM00, -30.99), 0.0);
@@ -123,7 +139,7 @@ struct RateSwitches {
     double zr, zs, zx, zg, zcr;   // zerosw of rain, snow (non-Roh), Roh Xs2, graupel, qc*qr
     double sw_hom, sw_htr, sw_mlt;
 };
-__device__ __forceinline__ RateSwitches rate_switches(const DevParams &P, double dens, double temp,
+NSW6_HD RateSwitches rate_switches(const DevParams &P, double dens, double temp,
                                                       const double q[6], double qsati)
 {
     RateSwitches s;
@@ -149,7 +165,7 @@ __device__ __forceinline__ RateSwitches rate_switches(const DevParams &P, double
 
 // Effective radius of liquid water and the cloud-top diagnostics of one point
 // (src/mod_mp_nsw6.cpp:1566-1592), in FP64 for both forms.
-__device__ __forceinline__ void effective_radius(const DevParams &P, const RateSwitches &sw, double dens,
+NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double dens,
                                                  double temp, const double q[6], double &rceff,
                                                  double &rceff_cld, double &rctop, double &tctop)
 {
@@ -185,19 +201,34 @@ template <typename R> struct RateIn {
     R sice_m1;         // Sice - 1
     R sliq_1m;         // 1 - min(Sliq, 1)
     R qv_qsati;        // max(qv, 0) - qsati
+    // the 8 tunable coefficients: kk_aut_rate, qicrt_saut, Esi, Esr, Cr, Cs, Cg,
+    // icenuc_coef (carried here so that a dual-number scalar can hold their tangents)
+    R tun[NCOEF];
 };
-// Fill the FP64-formed differences of RateIn from the FP64 inputs.
-template <typename R>
-__device__ __forceinline__ void rate_differences(RateIn<R> &in, double temp, const double q[6], double qsatl,
-                                                 double qsati)
+// The tunable coefficients of a parameter set, in the order above.
+NSW6_HD void tunables(const DevParams &P, double tun[NCOEF])
 {
-    const double qv = dmax(q[QV], 0.0);
-    const double Sliq = qv / dmax(qsatl, C_EPS);
-    const double Sice = qv / dmax(qsati, C_EPS);
-    in.temc = (R)(temp - C_TEM00);
-    in.sice_m1 = (R)(Sice - 1.0);
-    in.sliq_1m = (R)(1.0 - dmin(Sliq, 1.0));
-    in.qv_qsati = (R)(qv - qsati);
+    tun[0] = P.kk_aut_rate;
+    tun[1] = P.qicrt_saut;
+    tun[2] = P.Esi;
+    tun[3] = P.Esr;
+    tun[4] = P.Cr;
+    tun[5] = P.Cs;
+    tun[6] = P.Cg;
+    tun[7] = P.icenuc_coef;
+}
+// Fill the differences of RateIn from the precise inputs (type D: double for the
+// HIP forms, a dual number in the gradient check).
+template <typename R, typename D>
+NSW6_HD void rate_differences(RateIn<R> &in, D temp, const D q[6], D qsatl, D qsati)
+{
+    const D qv = dmax(q[QV], D(0.0));
+    const D Sliq = qv / dmax(qsatl, D(C_EPS));
+    const D Sice = qv / dmax(qsati, D(C_EPS));
+    in.temc = R(temp - C_TEM00);
+    in.sice_m1 = R(Sice - 1.0);
+    in.sliq_1m = R(1.0 - dmin(Sliq, D(1.0)));
+    in.qv_qsati = R(qv - qsati);
 }
 // Output of one rate evaluation: the limited tendencies q_t (1/s) of QC..QG (the
 // caller closes QV and forms rhog * q_t * dt in FP64) and the fall speeds.
@@ -208,12 +239,12 @@ template <typename R> struct RateOut {
 
 // TagBigLoop at one point.
 template <typename R>
-__device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
+NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
 {
     const R PI = R(C_PI), EPS = R(C_EPS), UNDEF = R(C_UNDEF);
     const R Rvap = R(C_Rvap), LHV0 = R(C_LHV0), LHS0 = R(C_LHS0), LHF0 = R(C_LHF0);
     const R PRE00 = R(C_Pstd), TEM00 = R(C_TEM00), CL = R(C_CL), rho_w = R(C_DWATR);
-    const R Ar = R(P.Ar), As = R(P.As), Ag = R(P.Ag), Cr = R(P.Cr), Cs = R(P.Cs), Cg = R(P.Cg);
+    const R Ar = R(P.Ar), As = R(P.As), Ag = R(P.Ag), Cr = in.tun[4], Cs = in.tun[5], Cg = in.tun[6];
     const R N0r = R(P.N0r), N0s = R(P.N0s), N0g = R(P.N0g);
     const R GAM = R(P.GAM), GAM_2 = R(P.GAM_2), GAM_3 = R(P.GAM_3);
     const R dens = in.dens, temp = in.temp;
@@ -355,11 +386,11 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
 
     // nucleation
     const R Ni0 = dmax(exp(R(-0.1) * temc), R(1)) * R(1000);
-    const R Qi0 = R(P.icenuc_coef) * exp(log(Ni0) * R(1.33)) * Rdens;
+    const R Qi0 = in.tun[7] * exp(log(Ni0) * R(1.33)) * Rdens;
     R Pigen = dmax(dmin(Qi0 - qi, in.qv_qsati), R(0)) / dt;
 
     // accretion
-    const R Esi = R(P.Esi), Egs = R(P.Egs);
+    const R Esi = in.tun[2], Egs = R(P.Egs);
     const R Esi_mod = dmin(Esi, Esi * exp(R(P.gamma_sacr) * temc));
     const R Egs_mod = dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
     const R GAM_3dr = R(P.GAM_3dr), GAM_3dg = R(P.GAM_3dg);
@@ -381,7 +412,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     R Pgaci = qi * R(0.25) * PI * R(P.Egi) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
     R Piacr = qi * Ar / R(P.mi) * R(0.25) * PI * R(P.Eri) * N0r * Cr * R(P.GAM_6dr) * RLMDr_6dr * rho_fact;
     const R GAM_1br = R(P.GAM_1br), GAM_2br = R(P.GAM_2br), GAM_3br = R(P.GAM_3br);
-    const R Esr = R(P.Esr);
+    const R Esr = in.tun[3];
     R Psacr = Ar * R(0.25) * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
               (GAM_1br * RLMDr_1br * MOMs_2 + R(2) * GAM_2br * RLMDr_2br * MOMs_1 + GAM_3br * RLMDr_3br * MOMs_0);
     R Pgacr = Ar * R(0.25) * PI * Rdens * R(P.Egr) * N0g * N0r * fabs(Vtg - Vtr) *
@@ -395,7 +426,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     // autoconversion
     const R Nc = R(P.Nc_def);
     const R rhoqc = dens * qc * R(1000);
-    const R Praut_kk = R(P.kk_aut_rate) * exp(R(2.47) * log(qc)) * R(P.Nc_pow_m179);
+    const R Praut_kk = in.tun[0] * exp(R(2.47) * log(qc)) * R(P.Nc_pow_m179);
     R Praut;
     if (kk) {
         Praut = Praut_kk;
@@ -406,7 +437,7 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     }
     const R beta_saut = R(P.beta_saut), beta_gaut = R(P.beta_gaut);
     const R betai = dmin(beta_saut, beta_saut * exp(R(P.gamma_saut) * temc));
-    R Psaut = dmax(betai * (qi - R(P.qicrt_saut)), R(0));
+    R Psaut = dmax(betai * (qi - in.tun[1]), R(0));
     const R betas = dmin(beta_gaut, beta_gaut * exp(R(P.gamma_gaut) * temc));
     R Pgaut = dmax(betas * (qs - R(P.qscrt_gaut)), R(0));
 
@@ -573,20 +604,6 @@ __device__ __forceinline__ void rates(const DevParams &P, R dt, const RateIn<R> 
     o.Vti = Vti;
     o.Vts = Vts;
     o.Vtg = Vtg;
-}
-
-// Order-preserving map of doubles to unsigned 64-bit integers (for atomic max/min).
-__device__ __forceinline__ unsigned long long d2key(double d)
-{
-    unsigned long long u = (unsigned long long)__double_as_longlong(d);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
-}
-__host__ __device__ __forceinline__ double key2d(unsigned long long k)
-{
-    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffULL) : ~k;
-    double d;
-    __builtin_memcpy(&d, &u, sizeof d);
-    return d;
 }
 
 }  // namespace nsw6

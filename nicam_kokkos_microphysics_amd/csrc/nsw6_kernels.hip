@@ -25,6 +25,7 @@
 
 #include "nsw6_device.h"
 #include "nsw6_internal.h"
+#include "nsw6_step.h"
 
 using namespace nsw6;
 
@@ -74,86 +75,26 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     }
 
     const DevParams &P = *a.P;
-    // ---- negative_filter (src/mod_mp_nsw6.cpp:1770-1831), interior point
-    double rq[NQ];
+    PointIn<double> x;
+    x.rhog = f.rhog[p];
 #pragma unroll
-    for (int n = 0; n < NQ; n++) rq[n] = f.rhogq[pq + n * K2];
-    double diffq = 0.0;
-#pragma unroll
-    for (int n = 2; n <= 5; n++) {
-        diffq += rq[n];
-        rq[n] = dmax(rq[n], 0.0);
-    }
-#pragma unroll
-    for (int n = 2; n <= 5; n++) diffq -= rq[n];
-    rq[QV] += diffq;
-    diffq = rq[QV];
-    rq[QV] = dmax(rq[QV], 0.0);
-    diffq -= rq[QV];
-    const double rhog = f.rhog[p] * (1.0 - diffq);
-    const double gsgam2 = f.gsgam2[p];
-    const double rho = rhog / gsgam2;
-    double q[NQ];
-    q[QV] = f.q[pq];                                  // stale input QV (reference keeps it)
-#pragma unroll
-    for (int n = 1; n <= 5; n++) q[n] = rq[n] / rhog;
-    double qd = 1.0;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) qd = qd - q[n];
-    double cva = qd * C_CVdry;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
-    const double tem = f.tem[p];
-    double rhoge = tem * rhog * cva;
-    const double pre = rho * (qd * C_Rdry + q[QV] * C_Rvap) * tem;
-
-    // ---- psat / qsat (TagSATWater), Bergeron
-    const double qsatl = psat_liq(P, tem) / (rho * C_Rvap * tem);
-    const double qsati = psat_ice(P, tem) / (rho * C_Rvap * tem);
-    double a1 = 1.0, a2 = 0.0, ma2 = 1.0;   // only the Bergeron growth (off with explicit ice) uses them
-    if (P.sw_expice != 1.0) bergeron(c_a1_tab, c_a2_tab, tem, a1, a2, ma2);
-
-    // ---- TagBigLoop
-    RateIn<R> in;
-    in.dens = (R)rho;
-    in.temp = (R)tem;
-#pragma unroll
-    for (int n = 0; n < NQ; n++) in.q[n] = (R)q[n];
-    in.qsatl = (R)qsatl;
-    in.qsati = (R)qsati;
-    in.pre = (R)pre;
-    in.rhog = (R)rhog;
-    in.a1 = (R)a1;
-    in.a2 = (R)a2;
-    in.ma2 = (R)ma2;
-    in.sw = rate_switches(P, rho, tem, q, qsati);
-    rate_differences(in, tem, q, qsatl, qsati);
-    RateOut<R> o;
-    rates<R>(P, (R)a.dt, in, o);
+    for (int n = 0; n < NQ; n++) x.rhogq[n] = f.rhogq[pq + n * K2];
+    x.tem = f.tem[p];
+    x.qv = f.q[pq];
+    x.gsgam2 = f.gsgam2[p];
+    double tun[NCOEF];
+    tunables(P, tun);
+    PointOut<double> y;
+    point_rates<double, R>(P, a.dt, x, tun, c_a1_tab, c_a2_tab, y);
     double rceff, rceff_cld, rct, tct;
-    effective_radius(P, in.sw, rho, tem, q, rceff, rceff_cld, rct, tct);
+    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct);
 
-    // ---- drhogq = rhog * q_t * dt with qv_t = -(sum of the others) (src/mod_mp_nsw6.cpp:
-    // 1548-1559), in FP64 for both forms so the point's total water is conserved to FP64
-    // round-off; then TagUpdateRHOGQ / TagUpdateRHOGE
-    double qt[NQ];
+    a.ws.rhog[p] = y.rhog;
+    a.ws.rhoge[p] = y.rhoge;
 #pragma unroll
-    for (int n = 1; n < NQ; n++) qt[n] = (double)o.qt[n];
-    qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
-    double drq[NQ];
+    for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = y.rhogq[n];
 #pragma unroll
-    for (int n = 0; n < NQ; n++) drq[n] = rhog * qt[n] * a.dt;
-#pragma unroll
-    for (int n = 0; n < NQ; n++) rq[n] += drq[n];
-    rhoge = rhoge - C_LHV0 * drq[QV] + C_LHF0 * drq[QI] + C_LHF0 * drq[QS] + C_LHF0 * drq[QG];
-
-    a.ws.rhog[p] = rhog;
-    a.ws.rhoge[p] = rhoge;
-#pragma unroll
-    for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = rq[n];
-    a.ws.vt[pv + 0 * K2] = (double)o.Vtr;
-    a.ws.vt[pv + 1 * K2] = (double)o.Vts;
-    a.ws.vt[pv + 2 * K2] = (double)o.Vtg;
+    for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
     f.re_liquid[p] = rceff;
     f.re_cld[p] = rceff_cld;
     if (k == KMAX) {
@@ -166,26 +107,7 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     }
 }
 
-// ---------------------------------------------------------------------------------
-// departure distance of vadv1d_prep (src/mod_vadv1d.cpp:284-302)
-// ---------------------------------------------------------------------------------
-// zdis = dt*wh - dt^2*wh*dw/dz/2 + dt^3*wh*((dw/dz)^2 + wh*d2w/dz2)/6 with the grid
-// divisions taken as products with reciprocals computed once on the host
-__device__ __forceinline__ double zdis_interior(double dt, double dt2, double dt3_6, double whm, double whk,
-                                                double whp, double rdzm, double rdzk, double rs)
-{
-    const double d = whp - whm;
-    const double t1 = dt * whk;
-    const double t2 = dt2 * whk * d * rs * 0.5;
-    const double g = d * rs;
-    const double inner = ((whp - whk) * rdzk - (whk - whm) * rdzm) * rs * 2.0;
-    const double t3 = dt3_6 * whk * (g * g + whk * inner);
-    return t1 - t2 + t3;
-}
-__device__ __forceinline__ double zdis_bottom(double dt, double dt2, double whk, double whp, double rdzk)
-{
-    return dt * whk - dt2 * whk * (whp - whk) * rdzk * 0.5;
-}
+// zdis_interior / zdis_bottom and the point-wise wall speeds are in nsw6_step.h.
 
 // Sliding window over the fall speed Vt(k) of one tracer that produces, level by
 // level, wh/zdis of the full level (mkmin=1, mkmax=94) and of the half level
@@ -254,45 +176,6 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
-// Point-wise form of the same wall speeds: with V[d + 3] = Vt(k + d), d = -3..+1,
-// wh_full(V, k, d) is the full-level wh at wall k + d and wh_half(V, k, d) the
-// half-level one (Vtermh averaged again), identical to the sliding window above.
-__device__ __forceinline__ double wh_full(const double V[5], int k, int d)
-{
-    const int j = k + d;
-    if (j == 0) return 0.0;
-    if (j == 1) return V[d + 3];                        // wh(1) = Vt(1)
-    if (j <= KMAX) return 0.5 * (V[d + 2] + V[d + 3]);  // 0.5 (Vt(j-1) + Vt(j))
-    return V[d + 2];                                    // wh(95) = Vt(94)
-}
-__device__ __forceinline__ double vh_half(const double V[5], int k, int d)
-{
-    const int j = k + d;
-    return (j >= 2 && j <= KMAX - 1) ? 0.5 * (V[d + 3] + V[d + 2]) : 0.0;
-}
-__device__ __forceinline__ double wh_half(const double V[5], int k, int d)
-{
-    const int j = k + d;
-    if (j >= 3 && j <= KMAX) return 0.5 * (vh_half(V, k, d - 1) + vh_half(V, k, d));
-    return (j == 2) ? vh_half(V, k, d) : 0.0;
-}
-// zdis of the full level at wall k (1..94) and of the half level at wall k (2..94)
-__device__ __forceinline__ double zdis_full_at(const GridDev &g, double dt, double dt2, double dt3_6,
-                                               const double V[5], int k)
-{
-    const double whp = wh_full(V, k, 1);
-    if (k == KMIN) return zdis_bottom(dt, dt2, wh_full(V, k, 0), whp, g.rdgz[KMIN]);
-    return zdis_interior(dt, dt2, dt3_6, wh_full(V, k, -1), wh_full(V, k, 0), whp, g.rdgz[k - 1], g.rdgz[k],
-                         g.rsum_gz[k]);
-}
-__device__ __forceinline__ double zdis_half_at(const GridDev &g, double dt, double dt2, double dt3_6,
-                                               const double V[5], int k)
-{
-    const double whp = wh_half(V, k, 1);
-    if (k == KMIN + 1) return zdis_bottom(dt, dt2, wh_half(V, k, 0), whp, g.rdgzh[KMIN + 1]);
-    return zdis_interior(dt, dt2, dt3_6, wh_half(V, k, -1), wh_half(V, k, 0), whp, g.rdgzh[k - 1], g.rdgzh[k],
-                         g.rsum_gzh[k]);
-}
 // Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used).
 __device__ __forceinline__ void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
 {
@@ -453,30 +336,6 @@ __device__ __forceinline__ double fact_of(int k2, int kc, int k, double dz)
     const double f3 = (double)(k2 - k), f4 = (double)((kc - 1) - k2);
     return dz * 0.25 * ((cs1(f1) + 1.0) * (cs1(f2) + 1.0) - (cs1(f3) + 1.0) * (cs1(f4) + 1.0));
 }
-__device__ __forceinline__ double eps_filter(double f)
-{
-    return f * (0.5 + copysign(0.5, fabs(f) - C_EPS));
-}
-// per-column departure cell (src/mod_vadv1d.cpp:355-384)
-__device__ __forceinline__ int departure_cell(const double *zh, int k, int kcmin, int kcmax, double zdis, int mkmin,
-                                              int mkmax)
-{
-    int kc;
-    if (kcmin == k && kcmax == k) {
-        kc = k;
-    } else {
-        kc = 0;
-        const double x = zh[k] - zdis;
-        for (int k2 = kcmin; k2 <= kcmax; k2++) {
-            const int tmp = (int)((double)k2 * copysign(1.0, x - zh[k2]) * copysign(1.0, zh[k2 + 1] - x));
-            kc = (kc > tmp) ? kc : tmp;
-        }
-    }
-    if (kc == 0) kc = mkmin;
-    if (kc == mkmax + 1) kc = mkmax;
-    return kc;
-}
-
 struct SweepCtx {
     const StepArgs *a;
     int l, ij, ijdim;
@@ -602,38 +461,19 @@ __device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, doubl
 
 // ---------------------------------------------------------------------------------
 // Single-cell regions: every departure cell lies in {k, k+1} (checked by k_bounds),
-// so each semi-Lagrangian flux of vadv1d_getflux_new (src/mod_vadv1d.cpp:393-450)
-// has the closed form
-//     kc = k   : f = rhof(k) * zdis0
-//     kc = k+1 : f = -rhof(k) * dz(k) + rhof(k+1) * zdis0   (zdis0 kept, not refreshed)
-// and depends only on cells k, k+1 and the fall speeds Vt(k-3..k+1).  The half level
-// uses the full level's kcell_max as the reference does
-// (src/mod_precip_transport.cpp:890-892).  The transport then splits into two
-// point-wise kernels: k_flux (one thread per wall) and k_finish (one thread per cell).
-// Flux divergences of the non-mass quantities are taken on tracer-summed fluxes (a
-// reordering of the reference's per-tracer sum).
+// so each flux depends only on cells w, w+1 and the fall speeds Vt(w-3..w+1), and the
+// transport splits into two point-wise kernels: k_flux (one thread per wall) and
+// k_finish (one thread per cell).  The arithmetic is in nsw6_step.h.
 // ---------------------------------------------------------------------------------
-struct Raw {
-    double rhog, rq[5], rhoge;
-    double gs, gam2, gsh, gam2h;
-    double rvx, rvy, rvz, rw, vx, vy, vz, z;
-};
-struct Cell {
-    double rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
-    double tem, z, kinh, up, vp;
-    double hv[3][2];                 // half-level values at the cell's lower wall
-    double rhog, rhogw, kinv, c2w0, c2w1, gs;
-};
-
-__device__ __forceinline__ void load_raw(const SweepCtx &s, int j, Raw &r)
+__device__ __forceinline__ void load_cell(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int j,
+                                          CellIn<double> &r)
 {
-    const StepArgs &a = *s.a;
     const Fields &f = a.f;
-    const int64_t o = s.p2 + (int64_t)j * s.ijdim;
-    const int64_t oq = s.p3 + (int64_t)j * s.ijdim;
+    const int64_t o = p2 + (int64_t)j * ijdim;
+    const int64_t oq = p3 + (int64_t)j * ijdim;
     r.rhog = a.ws.rhog[o];
 #pragma unroll
-    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * s.K2];
+    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * K2];
     r.rhoge = a.ws.rhoge[o];
     r.gs = f.gsgam2[o];
     r.gam2 = f.gam2[o];
@@ -648,82 +488,22 @@ __device__ __forceinline__ void load_raw(const SweepCtx &s, int j, Raw &r)
     r.vz = f.vz[o];
     r.z = f.z[o];
 }
-
-// Transported quantities of cell j (src/mod_precip_transport.cpp:740-780, 846-888)
-// given the wall-relevant part of cell j-1 (rhog, q, gsgam2).
-__device__ __forceinline__ void derive_cell(const SweepCtx &s, const GridDev &g, int j, const Raw &r,
-                                            const Cell &below, Cell &c)
+__device__ __forceinline__ Unit load_unit(const Fields &f, int64_t pc)
 {
-    const bool interior = (j >= KMIN && j <= KMAX);
-    const double rg = r.rhog;
-    const double rgs = interior ? r.gam2 / r.gs : 0.0;
-    double qq[5];
-#pragma unroll
-    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
-    double qd = 1.0;
-#pragma unroll
-    for (int n = 0; n < 5; n++) qd = qd - qq[n];
-    double cva = qd * C_CVdry;
-#pragma unroll
-    for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
-    c.tem = interior ? r.rhoge / (rg * cva) : 0.0;
-    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : 0.0;
-    c.up = r.vx * s.ix + r.vy * s.iy + r.vz * s.iz;
-    c.vp = r.vx * s.jx + r.vy * s.jy + r.vz * s.jz;
-    c.z = r.z;
-    // tracer order inside rq/qq: QC, QR, QI, QS, QG
-    c.q[0] = qq[1];
-    c.q[1] = qq[3];
-    c.q[2] = qq[4];
-    c.rhoq[0] = r.rq[1] * rgs;
-    c.rhoq[1] = r.rq[3] * rgs;
-    c.rhoq[2] = r.rq[4] * rgs;
-#pragma unroll
-    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
-    c.rhog = rg;
-    c.rhogw = r.rw;
-    c.gs = r.gs;
-    if (j >= KMIN) {
-        c.c2w0 = g.afact[j] / r.gs * r.gsh;
-        c.c2w1 = g.bfact[j] / below.gs * r.gsh;
-    } else {
-        c.c2w0 = 0.0;
-        c.c2w1 = 0.0;
-    }
-    if (j >= KMIN + 1 && j <= KMAX) {
-        c.kinv = 0.5 * (r.rw * r.rw) / (c.c2w0 * rg + c.c2w1 * below.rhog);
-        const double rgsh = r.gam2h / r.gsh;
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const double qh = 0.5 * (c.q[t] + below.q[t]) * rgsh;
-            c.hv[t][0] = qh * c.kinv;
-            c.hv[t][1] = qh * r.rw;
-        }
-    } else {
-        c.kinv = 0.0;
-#pragma unroll
-        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = 0.0;
-    }
-}
-
-__device__ __forceinline__ double rhof_cell(const Cell &c, int t, int i)
-{
-    switch (i) {
-    case 0: return c.rhoq[t];
-    case 1: return c.rhoq[t] * C_CVW * c.tem;
-    case 2: return c.rhoq[t] * C_GRAV * c.z;
-    case 3: return c.qrgs[t] * c.kinh;
-    case 4: return c.rhoq[t] * c.up;
-    default: return c.rhoq[t] * c.vp;
-    }
+    Unit u;
+    u.ix = f.ix[pc];
+    u.iy = f.iy[pc];
+    u.iz = f.iz[pc];
+    u.jx = f.jx[pc];
+    u.jy = f.jy[pc];
+    u.jz = f.jz[pc];
+    return u;
 }
 
 // ---------------------------------------------------------------------------------
 // k_flux: one thread per (wall w, column) of the single-cell regions.  Writes the flux
 // workspace of wall w (FLUX_SLOTS), frain = GPREC (src/mod_precip_transport.cpp:
-// 916-927) and, at wall kmin, the surface fluxes precip, precip_lh_heat,
-// precip_rhoe/rhophi/rhokin (:900-958) scaled by fraction_mp = 1 (src/mod_mp_driver.cpp:
-// 560-612).  Walls 0 and 95 carry no flux.
+// 916-927) and, at wall kmin, the surface fluxes.  Walls 0 and 95 carry no flux.
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 {
@@ -736,240 +516,128 @@ __global__ void __launch_bounds__(256) k_flux(StepArgs a)
     if (ij >= ijdim) return;
     const GridDev &g = *a.grid;
     const Fields &f = a.f;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    const int64_t p3 = (int64_t)l * NQ * K2 + ij;
+    const int64_t pv = (int64_t)l * 3 * K2 + ij;
+    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
+    const int64_t ow = p2 + (int64_t)w * ijdim;
 
-    SweepCtx s;
-    s.a = &a;
-    s.l = l;
-    s.ij = ij;
-    s.ijdim = ijdim;
-    s.K2 = (int64_t)KALL * ijdim;
-    s.p2 = (int64_t)l * s.K2 + ij;
-    s.p3 = (int64_t)l * NQ * s.K2 + ij;
-    s.pv = (int64_t)l * 3 * s.K2 + ij;
-    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * s.K2 + (int64_t)w * ijdim + ij;   // + slot*K2
-    const int64_t ow = s.p2 + (int64_t)w * ijdim;
-
-    fx[FX_RW * s.K2] = f.rhogw[ow];
+    fx[FX_RW * K2] = f.rhogw[ow];
     if (w == 0 || w == KALL - 1) {
 #pragma unroll
-        for (int i = 0; i < FX_RW; i++) fx[i * s.K2] = 0.0;
+        for (int i = 0; i < FX_RW; i++) fx[i * K2] = 0.0;
         f.GPREC[ow] = (0.0 + 0.0) * 1.0;
         return;
     }
     const int64_t pc = (int64_t)l * ijdim + ij;
-    s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
-    s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
+    const Unit u = load_unit(f, pc);
     const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
-    const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
 
     // cell w-1 enters the walls of cell w only through rhog, q and gsgam2
-    Cell below, cur, nxt;
+    const int64_t om = p2 + (int64_t)(w - 1) * ijdim, oqm = p3 + (int64_t)(w - 1) * ijdim;
+    const double rg_m = a.ws.rhog[om], gs_m = f.gsgam2[om];
+    const double q_m[3] = {a.ws.rhogq[oqm + QR * K2] / rg_m, a.ws.rhogq[oqm + QS * K2] / rg_m,
+                           a.ws.rhogq[oqm + QG * K2] / rg_m};
+    Cell<double> cur, nxt;
     {
-        const int64_t o = s.p2 + (int64_t)(w - 1) * ijdim;
-        const int64_t oq = s.p3 + (int64_t)(w - 1) * ijdim;
-        below.rhog = a.ws.rhog[o];
-        below.gs = f.gsgam2[o];
-        below.q[0] = a.ws.rhogq[oq + QR * s.K2] / below.rhog;
-        below.q[1] = a.ws.rhogq[oq + QS * s.K2] / below.rhog;
-        below.q[2] = a.ws.rhogq[oq + QG * s.K2] / below.rhog;
+        CellIn<double> r;
+        load_cell(a, p2, p3, K2, ijdim, w, r);
+        derive_cell(g, w, r, rg_m, q_m, gs_m, u, cur);
+        load_cell(a, p2, p3, K2, ijdim, w + 1, r);
+        derive_cell(g, w + 1, r, cur.rhog, cur.q, cur.gs, u, nxt);
     }
-    {
-        Raw r;
-        load_raw(s, w, r);
-        derive_cell(s, g, w, r, below, cur);
-        load_raw(s, w + 1, r);
-        derive_cell(s, g, w + 1, r, cur, nxt);
-    }
-
-    double fq[3], fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fh[2] = {0.0, 0.0};
-    double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
+    double V[3][5];
+    int kcmax[3], khmax[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-        double V[5];
-        load_vt_window(a.ws.vt + s.pv + t * s.K2, ijdim, w, V);
-        // full level
-        const double z0 = zdis_full_at(g, dt, dt2, dt3, V, w);
-        const int kcmax = bnd[bounds_index(t, 0, w, 1)];
-        const int kc = departure_cell(g.gzh, w, w, kcmax, z0, KMIN, KMAX);
-        const bool up1 = (kcmax != w) && (kc == w + 1);
-        const double dzw = g.dgz[w];
-        double ft[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const double r0 = rhof_cell(cur, t, i);
-            const double v = up1 ? (-(r0 * dzw) + rhof_cell(nxt, t, i) * z0) : r0 * z0;
-            ft[i] = eps_filter(v);
-        }
-        fq[t] = ft[0];
-#pragma unroll
-        for (int i = 1; i < 6; i++) fs[i - 1] += ft[i];
-        if (w == KMIN) {
-            if (t == 0) {
-                precip0 = precip0 - ft[0] / dt;
-            } else {
-                precip1 = precip1 - ft[0] / dt;
-                lh = lh + ft[0] * C_LHF0 / dt;
-            }
-            prhoe = prhoe - ft[1] / dt;
-            prhophi = prhophi - ft[2] / dt;
-            prhokin = prhokin - ft[3] / dt - 0.0 / dt;   // the half-level flux at kmin is 0
-        }
-        // half level (walls 2..94)
-        if (w >= KMIN + 1) {
-            const double zh0 = zdis_half_at(g, dt, dt2, dt3, V, w);
-            const int khmax = bnd[bounds_index(t, 1, w, 1)];
-            const int kch = departure_cell(g.gz_shift, w, w, khmax, zh0, KMIN + 1, KMAX);
-            const double dzh = g.dgzh[w];
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const double hc = cur.hv[t][i], hn = nxt.hv[t][i];
-                double v;
-                if (kcmax == w)  // region-wide fast path of getflux: rhof(kcell) * zdis0
-                    v = ((kch == w + 1) ? hn : hc) * zh0;
-                else
-                    v = (kch == w + 1) ? (-(hc * dzh) + hn * zh0) : hc * zh0;
-                fh[i] += eps_filter(v);
-            }
-        }
+        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
+        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
     }
+    double z0[3], zh0[3];
+    WallGeom geo;
+    wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmax, khmax, z0, zh0, geo);
+    WallFlux<double> F;
+    double ft[3][4];
+    wall_flux(g, w, cur, nxt, z0, zh0, geo, F, ft);
+
 #pragma unroll
-    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * s.K2] = fq[t];
+    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * K2] = F.fq[t];
 #pragma unroll
-    for (int i = 0; i < 5; i++) fx[(FX_E + i) * s.K2] = fs[i];
-    fx[FX_KINV * s.K2] = fh[0];
-    fx[FX_W * s.K2] = fh[1];
-    f.GPREC[ow] = (0.0 + (-fq[0] / dt)) * 1.0;
+    for (int i = 0; i < 5; i++) fx[(FX_E + i) * K2] = F.fs[i];
+    fx[FX_KINV * K2] = F.fh[0];
+    fx[FX_W * K2] = F.fh[1];
+    f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
     if (w == KMIN) {
+        Surface<double> sf;
+        surface_fluxes(ft, a.dt, sf);
         const int64_t pp = (int64_t)l * 2 * ijdim + ij;
-        f.precip[pp] = (0.0 + precip0) * 1.0;
-        f.precip[pp + ijdim] = (0.0 + precip1) * 1.0;
+        f.precip[pp] = sf.precip0;
+        f.precip[pp + ijdim] = sf.precip1;
         f.ISO1_precip[pp] = 0.0;
         f.ISO1_precip[pp + ijdim] = 0.0;
         f.ISO2_precip[pp] = 0.0;
         f.ISO2_precip[pp + ijdim] = 0.0;
-        f.precip_rhoe[pc] = (0.0 + prhoe) * 1.0;
-        f.precip_lh_heat[pc] = (0.0 + lh) * 1.0;
-        f.precip_rhophi[pc] = (0.0 + prhophi) * 1.0;
-        f.precip_rhokin[pc] = (0.0 + prhokin) * 1.0;
+        f.precip_rhoe[pc] = sf.prhoe;
+        f.precip_lh_heat[pc] = sf.lh;
+        f.precip_rhophi[pc] = sf.prhophi;
+        f.precip_rhokin[pc] = sf.prhokin;
     }
 }
 
-// Sedimentation apply and KIN2EIN of interior cell c of a single-cell region
-// (src/mod_precip_transport.cpp:960-1011) from the k_flux workspace: returns rhog,
-// rhoge and QR, QS, QG after transport and stores the transported momentum.
-__device__ __forceinline__ void sediment_cell(const StepArgs &a, int l, int ij, int c, double &rg_out,
-                                              double &e_out, double rq[6])
+// Inputs of sediment() for interior cell c of column (l, ij) from the k_flux workspace.
+__device__ __forceinline__ void load_sed(const StepArgs &a, int l, int ij, int c, SedIn<double> &x)
 {
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const GridDev &g = *a.grid;
     const Fields &f = a.f;
     const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
-    const int64_t p2 = (int64_t)l * K2 + ij;
-    const int64_t p3 = (int64_t)l * NQ * K2 + ij;
 #define FXV(slot, wall) fx[(int64_t)(slot) * K2 + (int64_t)(wall) * ijdim]
-
-    // tendencies of cell c (walls c and c+1; wall 95 carries zero fluxes)
-    const double rc = g.rdgz[c];
-    double dq[3];
 #pragma unroll
-    for (int t = 0; t < 3; t++) dq[t] = -(FXV(FX_Q + t, c + 1) - FXV(FX_Q + t, c)) * rc;
-    const double de = -(FXV(FX_E, c + 1) - FXV(FX_E, c)) * rc;
-    const double dphi = -(FXV(FX_PHI, c + 1) - FXV(FX_PHI, c)) * rc;
-    const double dkinh = -(FXV(FX_KINH, c + 1) - FXV(FX_KINH, c)) * rc;
-    const double du = -(FXV(FX_U, c + 1) - FXV(FX_U, c)) * rc;
-    const double dv = -(FXV(FX_V, c + 1) - FXV(FX_V, c)) * rc;
-    // mass tendencies of cells c+1 and c-1
-    double dqp[3] = {0.0, 0.0, 0.0}, dqm[3] = {0.0, 0.0, 0.0};
-    if (c + 1 <= KMAX) {
-        const double r = g.rdgz[c + 1];
+    for (int t = 0; t < 3; t++) {
+        x.fq[0][t] = FXV(FX_Q + t, c - 1);
+        x.fq[1][t] = FXV(FX_Q + t, c);
+        x.fq[2][t] = FXV(FX_Q + t, c + 1);
+        x.fq[3][t] = (c + 1 <= KMAX) ? FXV(FX_Q + t, c + 2) : 0.0;   // wall 96 does not exist
+    }
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqp[t] = -(FXV(FX_Q + t, c + 2) - FXV(FX_Q + t, c + 1)) * r;
+    for (int i = 0; i < 5; i++) {
+        x.fs[0][i] = FXV(FX_E + i, c);
+        x.fs[1][i] = FXV(FX_E + i, c + 1);
     }
-    if (c - 1 >= KMIN) {
-        const double r = g.rdgz[c - 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqm[t] = -(FXV(FX_Q + t, c) - FXV(FX_Q + t, c - 1)) * r;
+    for (int m = 0; m < 3; m++) {
+        x.fh[m][0] = FXV(FX_KINV, c - 1 + m);
+        x.fh[m][1] = FXV(FX_W, c - 1 + m);
     }
-    // half-level tendencies at walls c and c+1 (walls 2..94; the flux of wall 1 is 0)
-    double hk_c = 0.0, hw_c = 0.0, hk_p = 0.0, hw_p = 0.0;
-    if (c >= KMIN + 1) {
-        const double r = g.rdgzh[c];
-        hk_c = -(FXV(FX_KINV, c) - FXV(FX_KINV, c - 1)) * r;
-        hw_c = -(FXV(FX_W, c) - FXV(FX_W, c - 1)) * r;
-    }
-    if (c + 1 <= KMAX) {
-        const double r = g.rdgzh[c + 1];
-        hk_p = -(FXV(FX_KINV, c + 1) - FXV(FX_KINV, c)) * r;
-        hw_p = -(FXV(FX_W, c + 1) - FXV(FX_W, c)) * r;
-    }
-    const double rw_c = FXV(FX_RW, c), rw_p = FXV(FX_RW, c + 1);
+    x.rw_c = FXV(FX_RW, c);
+    x.rw_p = FXV(FX_RW, c + 1);
 #undef FXV
-
-    // pre-transport wall quantities of cells c and c+1 (C2W factors, rhogkin_v)
-    const int64_t o = p2 + (int64_t)c * ijdim;
-    const double rg_m = a.ws.rhog[o - ijdim], rg_c = a.ws.rhog[o], rg_p = a.ws.rhog[o + ijdim];
-    const double gs_m = f.gsgam2[o - ijdim], gs_c = f.gsgam2[o], gs_p = f.gsgam2[o + ijdim];
-    const double gsh_c = f.gsgam2h[o], gsh_p = f.gsgam2h[o + ijdim];
-    const double c2w0_c = g.afact[c] / gs_c * gsh_c, c2w1_c = g.bfact[c] / gs_m * gsh_c;
-    const double c2w0_p = g.afact[c + 1] / gs_p * gsh_p, c2w1_p = g.bfact[c + 1] / gs_c * gsh_p;
-    const double kinv0_c = (c >= KMIN + 1) ? 0.5 * (rw_c * rw_c) / (c2w0_c * rg_c + c2w1_c * rg_m) : 0.0;
-    const double kinv0_p = (c + 1 <= KMAX) ? 0.5 * (rw_p * rw_p) / (c2w0_p * rg_p + c2w1_p * rg_c) : 0.0;
-
-    const int64_t pc = (int64_t)l * ijdim + ij;
-    const double ix = f.ix[pc], iy = f.iy[pc], iz = f.iz[pc];
-    const double jx = f.jx[pc], jy = f.jy[pc], jz = f.jz[pc];
-    const int64_t oq = p3 + (int64_t)c * ijdim;
-    rq[QR] = a.ws.rhogq[oq + QR * K2];
-    rq[QS] = a.ws.rhogq[oq + QS * K2];
-    rq[QG] = a.ws.rhogq[oq + QG * K2];
-    const double re0 = a.ws.rhoge[o];
-    const double rvx = f.rhogvx[o], rvy = f.rhogvy[o], rvz = f.rhogvz[o];
-    const double z = f.z[o];
-    const double w2c0 = g.cfact[c] * gs_c / gsh_p, w2c1 = g.dfact[c] * gs_c / gsh_p;
-    const double kinh = 0.5 * (rvx * rvx + rvy * rvy + rvz * rvz) / rg_c;
-    double e = re0 + de + dphi;
-    const double kinh_u = kinh + dkinh;
-    const double kinv_c = kinv0_c + hk_c;
-    const double kinv_cp = kinv0_p + hk_p;
-    const double rvx_n = rvx + du * ix + dv * jx;
-    const double rvy_n = rvy + du * iy + dv * jy;
-    const double rvz_n = rvz + du * iz + dv * jz;
-    const double rw_n = rw_c + hw_c;
-    double rg = rg_c;
-    rq[QR] = rq[QR] + dq[0];
-    rg = rg + dq[0];
-    e = e - dq[0] * C_GRAV * z;
-    rq[QS] = rq[QS] + dq[1];
-    rg = rg + dq[1];
-    e = e - dq[1] * C_GRAV * z;
-    rq[QG] = rq[QG] + dq[2];
-    rg = rg + dq[2];
-    e = e - dq[2] * C_GRAV * z;
-    // KIN2EIN: kinetic energy before minus after, with the neighbours' new densities
-    const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
-    const double rg_cp = ((rg_p + dqp[0]) + dqp[1]) + dqp[2];
-    const double rg_cm = ((rg_m + dqm[0]) + dqm[1]) + dqm[2];
-    const double rw_cp = rw_p + hw_p;
-    const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
-    const double kinv_new_c = (c >= KMIN + 1) ? 0.5 * (rw_n * rw_n) / (c2w0_c * rg + c2w1_c * rg_cm) : 0.0;
-    const double kinv_new_cp = (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (c2w0_p * rg_cp + c2w1_p * rg) : 0.0;
-    const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
-    e = e + (tmp2 - tmpk);
-    f.rhogvx[o] = rvx_n;
-    f.rhogvy[o] = rvy_n;
-    f.rhogvz[o] = rvz_n;
-    f.rhogw[o] = rw_n;
-    rg_out = rg;
-    e_out = e;
+    const int64_t o = (int64_t)l * K2 + (int64_t)c * ijdim + ij;
+    const int64_t oq = (int64_t)l * NQ * K2 + (int64_t)c * ijdim + ij;
+    x.rg_m = a.ws.rhog[o - ijdim];
+    x.rg_c = a.ws.rhog[o];
+    x.rg_p = a.ws.rhog[o + ijdim];
+    x.rq[0] = a.ws.rhogq[oq + QR * K2];
+    x.rq[1] = a.ws.rhogq[oq + QS * K2];
+    x.rq[2] = a.ws.rhogq[oq + QG * K2];
+    x.re0 = a.ws.rhoge[o];
+    x.rvx = f.rhogvx[o];
+    x.rvy = f.rhogvy[o];
+    x.rvz = f.rhogvz[o];
+    x.gs_m = f.gsgam2[o - ijdim];
+    x.gs_c = f.gsgam2[o];
+    x.gs_p = f.gsgam2[o + ijdim];
+    x.gsh_c = f.gsgam2h[o];
+    x.gsh_p = f.gsgam2h[o + ijdim];
+    x.z = f.z[o];
 }
 
 // ---------------------------------------------------------------------------------
-// k_finish: one thread per point.  The sedimented state comes from sediment_cell in
-// the single-cell regions and from the general sweep elsewhere; then the point-wise
-// tail of the step: rediagnosis (src/mod_precip_transport.cpp:1074-1086),
-// SATURATION_adjustment with satadjust_liq (src/mod_satadjust.cpp:274-362, 771-896)
-// and the final negative_filter (src/mod_mp_nsw6.cpp:671, 1770-1831).
+// k_finish: one thread per point.  The sedimented state comes from sediment() in the
+// single-cell regions and from the general sweep elsewhere; then the point-wise tail
+// (rediagnosis, saturation adjustment, final negative filter).
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_finish(StepArgs a)
 {
@@ -980,7 +648,6 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
     const Fields &f = a.f;
-    const DevParams &P = *a.P;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
@@ -989,23 +656,35 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
 
     // State after sedimentation.  QV, QC, QI and the halo levels only receive zero
     // tendencies.
-    double rhog, rhoge, rhogq[6];
+    TailIn<double> x;
     if (interior) {
         if (general) {
-            rhog = f.rhog[p];
-            rhoge = f.rhoge[p];
-            rhogq[QR] = f.rhogq[pq + QR * K2];
-            rhogq[QS] = f.rhogq[pq + QS * K2];
-            rhogq[QG] = f.rhogq[pq + QG * K2];
+            x.rhog = f.rhog[p];
+            x.rhoge = f.rhoge[p];
+            x.rhogq[QR] = f.rhogq[pq + QR * K2];
+            x.rhogq[QS] = f.rhogq[pq + QS * K2];
+            x.rhogq[QG] = f.rhogq[pq + QG * K2];
         } else {
-            sediment_cell(a, l, ij, k, rhog, rhoge, rhogq);
+            SedIn<double> si;
+            load_sed(a, l, ij, k, si);
+            SedOut<double> so;
+            sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so);
+            f.rhogvx[p] = so.rvx;
+            f.rhogvy[p] = so.rvy;
+            f.rhogvz[p] = so.rvz;
+            f.rhogw[p] = so.rw;
+            x.rhog = so.rg;
+            x.rhoge = so.e;
+            x.rhogq[QR] = so.rq[0];
+            x.rhogq[QS] = so.rq[1];
+            x.rhogq[QG] = so.rq[2];
         }
     } else {
-        rhog = a.ws.rhog[p] + 0.0;
-        rhoge = a.ws.rhoge[p] + 0.0;
-        rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
-        rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
-        rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+        x.rhog = a.ws.rhog[p] + 0.0;
+        x.rhoge = a.ws.rhoge[p] + 0.0;
+        x.rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
+        x.rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
+        x.rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
         if (!general) {
             const int64_t pc = (int64_t)l * ijdim + ij;
             f.rhogvx[p] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
@@ -1014,129 +693,32 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             f.rhogw[p] = f.rhogw[p] + 0.0;
         }
     }
-    rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
-    rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
-    rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
-    const double gsgam2 = f.gsgam2[p];
+    x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
+    x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
+    x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+    x.gsgam2 = f.gsgam2[p];
 
-    // rediagnosis at the end of precip_transport_new
-    double q[6];
-#pragma unroll
-    for (int n = 0; n < 6; n++) q[n] = rhogq[n] / rhog;
-    double rho = rhog / gsgam2;
-    const double ein = rhoge / rhog;
-    double qd = 1.0;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) qd = qd - q[n];
-    double cv = 0.0, qd2 = 1.0;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) {
-        cv = cv + q[n] * C_CVW;
-        qd2 = qd2 - q[n];
-    }
-    cv = cv + qd2 * C_CVdry;
-    double tem = ein / cv;
-    double pre = rho * tem * (qd2 * C_Rdry + q[QV] * C_Rvap);
-
-    // SATURATION_adjustment, ice_adjust = false
-    const double ein_moist = rhoge / rhog + q[QV] * C_LHV0;
-    const double qsum = q[QV] + q[QC];
-    q[QV] = qsum;
-    q[QC] = 0.0;
-    double CVtot = qd * C_CVdry;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
-    const double rho_sa = rhog / gsgam2;
-    tem = (ein_moist - q[QV] * C_LHV0) / CVtot;
-    if (interior) {
-        double qdl = 1.0;
-#pragma unroll
-        for (int n = 1; n <= 5; n++) qdl = qdl - q[n];
-        double rtem = 1.0 / dmax(tem, TEM_MIN);
-        double psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
-        double qsat = psat / (rho_sa * C_Rvap * tem);
-        if (qsum - qsat > C_EPS) {
-            bool converged = false, nan_break = false;
-            for (int ite = 0; ite < 100; ite++) {
-                rtem = 1.0 / dmax(tem, TEM_MIN);
-                psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
-                qsat = psat / (rho_sa * C_Rvap * tem);
-                q[QV] = qsat;
-                q[QC] = qsum - qsat;
-                double CVt = qdl * C_CVdry;
-#pragma unroll
-                for (int n = 1; n <= 5; n++) CVt = CVt + q[n] * C_CVW;
-                const double Emoist_new = tem * CVt + q[QV] * C_LHV0;
-                const double dqsat_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsat;
-                const double dCVtot_dT = dqsat_dT * (C_CVW - C_CVW);
-                const double dEmoist_dT = tem * dCVtot_dT + CVt + dqsat_dT * C_LHV0;
-                const double dtemp = (Emoist_new - ein_moist) / dEmoist_dT;
-                tem = tem - dtemp;
-                if (fabs(dtemp) < P.dtemp_criteria) {
-                    converged = true;
-                    break;
-                }
-                if (tem * 0.0 != 0.0) {
-                    nan_break = true;
-                    break;
-                }
-            }
-            if (!converged) {
-                const int code = nan_break ? 2 : 1;
-                if (atomicAdd(&a.ws.report[code], 1) == 0) {
-                    a.ws.report[3] = l;
-                    a.ws.report[4] = k;
-                    a.ws.report[5] = ij;
-                }
-                atomicOr(&a.ws.report[0], 1);
-            }
+    TailOut<double> y;
+    const TailStatus st = tail(*a.P, interior, x, y);
+    if (st.newton < 0) {
+        const int code = (st.newton == -1) ? 2 : 1;
+        if (atomicAdd(&a.ws.report[code], 1) == 0) {
+            a.ws.report[3] = l;
+            a.ws.report[4] = k;
+            a.ws.report[5] = ij;
         }
+        atomicOr(&a.ws.report[0], 1);
     }
-    rhogq[QV] = rhog * q[QV];
-    rhogq[QC] = rhog * q[QC];
-    double CVtot2 = qd * C_CVdry;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;
-    rhoge = rhog * tem * CVtot2;
-
-    if (interior) {
-        // final negative_filter
-        double diffq = 0.0;
-#pragma unroll
-        for (int n = 2; n <= 5; n++) {
-            diffq += rhogq[n];
-            rhogq[n] = dmax(rhogq[n], 0.0);
-        }
-#pragma unroll
-        for (int n = 2; n <= 5; n++) diffq -= rhogq[n];
-        rhogq[QV] += diffq;
-        diffq = rhogq[QV];
-        rhogq[QV] = dmax(rhogq[QV], 0.0);
-        diffq -= rhogq[QV];
-        rhog = rhog * (1.0 - diffq);
-        rho = rhog / gsgam2;
-#pragma unroll
-        for (int n = 1; n <= 5; n++) q[n] = rhogq[n] / rhog;
-        double qdn = 1.0;
-#pragma unroll
-        for (int n = 1; n <= 5; n++) qdn = qdn - q[n];
-        double cva = qdn * C_CVdry;
-#pragma unroll
-        for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
-        rhoge = tem * rhog * cva;
-        pre = rho * (qdn * C_Rdry + q[QV] * C_Rvap) * tem;
-    }
-
-    f.rhog[p] = rhog;
-    f.rhoge[p] = rhoge;
-    f.rho[p] = rho;
-    f.tem[p] = tem;
-    f.pre[p] = pre;
-    f.qd[p] = qd;
+    f.rhog[p] = y.rhog;
+    f.rhoge[p] = y.rhoge;
+    f.rho[p] = y.rho;
+    f.tem[p] = y.tem;
+    f.pre[p] = y.pre;
+    f.qd[p] = y.qd;
 #pragma unroll
     for (int n = 0; n < 6; n++) {
-        f.rhogq[pq + n * K2] = rhogq[n];
-        f.q[pq + n * K2] = q[n];
+        f.rhogq[pq + n * K2] = y.rhogq[n];
+        f.q[pq + n * K2] = y.q[n];
     }
     f.re_solid[p] = 20.0E-6;
     f.GDCLW[p] = 0.0 * 1.0;

@@ -1,0 +1,674 @@
+// nsw6_step.h -- the point, wall and cell pieces of one NSW6 step, templated on the
+// scalar type S.  The HIP kernels instantiate them with S = double (the FP64 form;
+// the mixed form adds an FP32 rate type R); the host dual-number gradient check
+// instantiates them with a forward-mode dual number, so the adjoint is checked
+// against derivatives of exactly this arithmetic.  Operation order follows the
+// reference (paths relative to the reference repository) so that S = double
+// reproduces it to round-off.
+//
+// A scalar type S provides the arithmetic operators, sqrt/exp/log/expm1/fabs/
+// copysign, dmax/dmin (select by value) and val(S) -> double.
+#pragma once
+
+#include "nsw6_device.h"
+#include "nsw6_internal.h"
+
+namespace nsw6 {
+
+NSW6_HD double val(double x) { return x; }
+NSW6_HD float val(float x) { return x; }
+
+// ---------------------------------------------------------------------------------
+// Point-wise rate update (k_rates): first negative_filter (src/mod_mp_nsw6.cpp:
+// 1770-1831), THRMDYN (src/mod_thrmdyn.cpp:98-180), psat/qsat (src/mod_satadjust.cpp:
+// 80-106, 133-160), TagBigLoop (src/mod_mp_nsw6.cpp:760-1594) with rate type R, and
+// TagUpdateRHOGQ/RHOGE (:692-758).  Interior points only.
+// ---------------------------------------------------------------------------------
+template <typename S> struct PointIn {
+    S rhog, rhogq[6], tem, qv;   // qv: the input q(QV) (the reference uses it stale)
+    double gsgam2;
+};
+template <typename S> struct PointOut {
+    S rhog, rhoge, rhogq[6], vt[3];   // pre-sedimentation state and fall speeds of QR, QS, QG
+    S rho, q[6];                      // density and q after the filter (q(QV) stale)
+    RateSwitches sw;
+};
+
+template <typename S> NSW6_HD S psat_of(double LovR, double RTEM00, S tem)
+{
+    const S rtem = S(1.0) / dmax(tem, S(TEM_MIN));
+    return S(C_PSAT0) * 1.0 * exp(LovR * (RTEM00 - rtem));
+}
+
+// a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation,
+// sw_expice != 1; the Bergeron parameters then enter as values, not differentiated).
+template <typename S, typename R>
+NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, const S tun[NCOEF],
+                         const double *a1_tab, const double *a2_tab, PointOut<S> &y)
+{
+    S rq[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) rq[n] = x.rhogq[n];
+    S diffq = S(0.0);
+#pragma unroll
+    for (int n = 2; n <= 5; n++) {
+        diffq += rq[n];
+        rq[n] = dmax(rq[n], S(0.0));
+    }
+#pragma unroll
+    for (int n = 2; n <= 5; n++) diffq -= rq[n];
+    rq[QV] += diffq;
+    diffq = rq[QV];
+    rq[QV] = dmax(rq[QV], S(0.0));
+    diffq -= rq[QV];
+    const S rhog = x.rhog * (1.0 - diffq);
+    const S rho = rhog / x.gsgam2;
+    S q[NQ];
+    q[QV] = x.qv;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) q[n] = rq[n] / rhog;
+    S qd = S(1.0);
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qd = qd - q[n];
+    S cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
+    const S tem = x.tem;
+    S rhoge = tem * rhog * cva;
+    const S pre = rho * (qd * C_Rdry + q[QV] * C_Rvap) * tem;
+    const S qsatl = psat_of(P.LovR_liq, P.RTEM00, tem) / (rho * C_Rvap * tem);
+    const S qsati = psat_of(P.LovR_ice, P.RTEM00, tem) / (rho * C_Rvap * tem);
+
+    RateIn<R> in;
+    in.dens = R(rho);
+    in.temp = R(tem);
+#pragma unroll
+    for (int n = 0; n < NQ; n++) in.q[n] = R(q[n]);
+    in.qsatl = R(qsatl);
+    in.qsati = R(qsati);
+    in.pre = R(pre);
+    in.rhog = R(rhog);
+    double a1 = 1.0, a2 = 0.0, ma2 = 1.0;
+    if (P.sw_expice != 1.0) bergeron(a1_tab, a2_tab, val(tem), a1, a2, ma2);
+    in.a1 = R(a1);
+    in.a2 = R(a2);
+    in.ma2 = R(ma2);
+    double qv_[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) qv_[n] = val(q[n]);
+    in.sw = rate_switches(P, val(rho), val(tem), qv_, val(qsati));
+    rate_differences<R, S>(in, tem, q, qsatl, qsati);
+#pragma unroll
+    for (int i = 0; i < NCOEF; i++) in.tun[i] = R(tun[i]);
+    RateOut<R> o;
+    rates<R>(P, R(dt), in, o);
+
+    // drhogq = rhog * q_t * dt with qv_t = -(sum of the others) (src/mod_mp_nsw6.cpp:
+    // 1548-1559) in the precise type, so the point's total water is conserved to its
+    // round-off; then TagUpdateRHOGQ / TagUpdateRHOGE
+    S qt[NQ];
+#pragma unroll
+    for (int n = 1; n < NQ; n++) qt[n] = S(o.qt[n]);
+    qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
+    S drq[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) drq[n] = rhog * qt[n] * dt;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) rq[n] += drq[n];
+    rhoge = rhoge - C_LHV0 * drq[QV] + C_LHF0 * drq[QI] + C_LHF0 * drq[QS] + C_LHF0 * drq[QG];
+
+    y.rhog = rhog;
+    y.rhoge = rhoge;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) y.rhogq[n] = rq[n];
+    y.vt[0] = S(o.Vtr);
+    y.vt[1] = S(o.Vts);
+    y.vt[2] = S(o.Vtg);
+    y.rho = rho;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) y.q[n] = q[n];
+    y.sw = in.sw;
+}
+
+// ---------------------------------------------------------------------------------
+// Departure distance of vadv1d_prep (src/mod_vadv1d.cpp:270-302):
+// zdis = dt*wh - dt^2*wh*dw/dz/2 + dt^3*wh*((dw/dz)^2 + wh*d2w/dz2)/6, with the grid
+// divisions taken as products with reciprocals computed once on the host.
+// ---------------------------------------------------------------------------------
+template <typename S>
+NSW6_HD S zdis_interior(double dt, double dt2, double dt3_6, S whm, S whk, S whp, double rdzm, double rdzk,
+                        double rs)
+{
+    const S d = whp - whm;
+    const S t1 = dt * whk;
+    const S t2 = dt2 * whk * d * rs * 0.5;
+    const S g = d * rs;
+    const S inner = ((whp - whk) * rdzk - (whk - whm) * rdzm) * rs * 2.0;
+    const S t3 = dt3_6 * whk * (g * g + whk * inner);
+    return t1 - t2 + t3;
+}
+template <typename S> NSW6_HD S zdis_bottom(double dt, double dt2, S whk, S whp, double rdzk)
+{
+    return dt * whk - dt2 * whk * (whp - whk) * rdzk * 0.5;
+}
+// With V[d + 3] = Vt(k + d), d = -3..+1: wh_full(V, k, d) is the full-level wh at wall
+// k + d (wh(0) = 0, wh(1) = Vt(1), wh(95) = Vt(94)) and wh_half the half-level one
+// (Vtermh of levels 2..93 averaged again).
+template <typename S> NSW6_HD S wh_full(const S V[5], int k, int d)
+{
+    const int j = k + d;
+    if (j == 0) return S(0.0);
+    if (j == 1) return V[d + 3];
+    if (j <= KMAX) return 0.5 * (V[d + 2] + V[d + 3]);
+    return V[d + 2];
+}
+template <typename S> NSW6_HD S vh_half(const S V[5], int k, int d)
+{
+    const int j = k + d;
+    return (j >= 2 && j <= KMAX - 1) ? 0.5 * (V[d + 3] + V[d + 2]) : S(0.0);
+}
+template <typename S> NSW6_HD S wh_half(const S V[5], int k, int d)
+{
+    const int j = k + d;
+    if (j >= 3 && j <= KMAX) return 0.5 * (vh_half(V, k, d - 1) + vh_half(V, k, d));
+    return (j == 2) ? vh_half(V, k, d) : S(0.0);
+}
+template <typename S>
+NSW6_HD S zdis_full_at(const GridDev &g, double dt, double dt2, double dt3_6, const S V[5], int k)
+{
+    const S whp = wh_full(V, k, 1);
+    if (k == KMIN) return zdis_bottom(dt, dt2, wh_full(V, k, 0), whp, g.rdgz[KMIN]);
+    return zdis_interior(dt, dt2, dt3_6, wh_full(V, k, -1), wh_full(V, k, 0), whp, g.rdgz[k - 1], g.rdgz[k],
+                         g.rsum_gz[k]);
+}
+template <typename S>
+NSW6_HD S zdis_half_at(const GridDev &g, double dt, double dt2, double dt3_6, const S V[5], int k)
+{
+    const S whp = wh_half(V, k, 1);
+    if (k == KMIN + 1) return zdis_bottom(dt, dt2, wh_half(V, k, 0), whp, g.rdgzh[KMIN + 1]);
+    return zdis_interior(dt, dt2, dt3_6, wh_half(V, k, -1), wh_half(V, k, 0), whp, g.rdgzh[k - 1], g.rdgzh[k],
+                         g.rsum_gzh[k]);
+}
+// per-column departure cell (src/mod_vadv1d.cpp:355-384)
+NSW6_HD int departure_cell(const double *zh, int k, int kcmin, int kcmax, double zdis, int mkmin, int mkmax)
+{
+    int kc;
+    if (kcmin == k && kcmax == k) {
+        kc = k;
+    } else {
+        kc = 0;
+        const double x = zh[k] - zdis;
+        for (int k2 = kcmin; k2 <= kcmax; k2++) {
+            const int tmp = (int)((double)k2 * copysign(1.0, x - zh[k2]) * copysign(1.0, zh[k2 + 1] - x));
+            kc = (kc > tmp) ? kc : tmp;
+        }
+    }
+    if (kc == 0) kc = mkmin;
+    if (kc == mkmax + 1) kc = mkmax;
+    return kc;
+}
+// |f| < EPS -> 0 (src/mod_vadv1d.cpp:445-449)
+template <typename S> NSW6_HD S eps_filter(S f) { return f * (0.5 + copysign(0.5, fabs(val(f)) - C_EPS)); }
+NSW6_HD double eps_mask(double f) { return 0.5 + copysign(0.5, fabs(f) - C_EPS); }
+
+// ---------------------------------------------------------------------------------
+// Transported quantities of one cell (src/mod_precip_transport.cpp:740-780, 846-888)
+// ---------------------------------------------------------------------------------
+template <typename S> struct CellIn {
+    S rhog, rq[5], rhoge;           // pre-sedimentation state; rq = QC, QR, QI, QS, QG
+    S rvx, rvy, rvz, rw, vx, vy, vz;
+    double gs, gam2, gsh, gam2h, z;
+};
+template <typename S> struct Cell {
+    S rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
+    S tem, kinh, up, vp;
+    S hv[3][2];                 // half-level values at the cell's lower wall
+    S rhog, kinv;
+    double z, gs;
+};
+struct Unit {   // horizontal unit vectors of a column
+    double ix, iy, iz, jx, jy, jz;
+};
+
+// Cell j from its raw data and the part of cell j-1 its lower wall needs.
+template <typename S>
+NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rhog, const S below_q[3],
+                         double below_gs, const Unit &u, Cell<S> &c)
+{
+    const bool interior = (j >= KMIN && j <= KMAX);
+    const S rg = r.rhog;
+    const double rgs = interior ? r.gam2 / r.gs : 0.0;
+    S qq[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
+    S qd = S(1.0);
+#pragma unroll
+    for (int n = 0; n < 5; n++) qd = qd - qq[n];
+    S cva = qd * C_CVdry;
+#pragma unroll
+    for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
+    c.tem = interior ? r.rhoge / (rg * cva) : S(0.0);
+    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : S(0.0);
+    c.up = r.vx * u.ix + r.vy * u.iy + r.vz * u.iz;
+    c.vp = r.vx * u.jx + r.vy * u.jy + r.vz * u.jz;
+    c.z = r.z;
+    c.q[0] = qq[1];
+    c.q[1] = qq[3];
+    c.q[2] = qq[4];
+    c.rhoq[0] = r.rq[1] * rgs;
+    c.rhoq[1] = r.rq[3] * rgs;
+    c.rhoq[2] = r.rq[4] * rgs;
+#pragma unroll
+    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
+    c.rhog = rg;
+    c.gs = r.gs;
+    double c2w0 = 0.0, c2w1 = 0.0;
+    if (j >= KMIN) {
+        c2w0 = g.afact[j] / r.gs * r.gsh;
+        c2w1 = g.bfact[j] / below_gs * r.gsh;
+    }
+    if (j >= KMIN + 1 && j <= KMAX) {
+        c.kinv = 0.5 * (r.rw * r.rw) / (c2w0 * rg + c2w1 * below_rhog);
+        const double rgsh = r.gam2h / r.gsh;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const S qh = 0.5 * (c.q[t] + below_q[t]) * rgsh;
+            c.hv[t][0] = qh * c.kinv;
+            c.hv[t][1] = qh * r.rw;
+        }
+    } else {
+        c.kinv = S(0.0);
+#pragma unroll
+        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = S(0.0);
+    }
+}
+
+template <typename S> NSW6_HD S rhof_cell(const Cell<S> &c, int t, int i)
+{
+    switch (i) {
+    case 0: return c.rhoq[t];
+    case 1: return c.rhoq[t] * C_CVW * c.tem;
+    case 2: return c.rhoq[t] * C_GRAV * c.z;
+    case 3: return c.qrgs[t] * c.kinh;
+    case 4: return c.rhoq[t] * c.up;
+    default: return c.rhoq[t] * c.vp;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Fluxes of wall w (1..94) of a single-cell region: the closed forms of
+// vadv1d_getflux_new (src/mod_vadv1d.cpp:393-450) for the range [w, w+1]
+//     kc = w   : f = rhof(w) * zdis0
+//     kc = w+1 : f = -rhof(w) * dz(w) + rhof(w+1) * zdis0   (zdis0 kept, not refreshed)
+// with the half level on the full level's kcell_max (src/mod_precip_transport.cpp:
+// 890-892).  Non-mass fluxes are summed over the three tracers.  sflux[t] is the
+// tracer's surface flux -f(kmin)/dt and feeds precip (:900-958).
+// ---------------------------------------------------------------------------------
+template <typename S> struct WallFlux {
+    S fq[3], fs[5], fh[2];
+};
+struct WallGeom {          // departure decisions of one wall and column
+    bool up1[3];           // full level: the departure cell is w+1
+    bool half_next[3];     // half level: the cell taken is w+1
+    bool half_full[3];     // half level: the two-cell form -h(w) dz + h(w+1) zdis0
+};
+
+template <typename S>
+NSW6_HD void wall_geometry(const GridDev &g, int w, double dt, double dt2, double dt3_6, const S V[3][5],
+                           const int kcmax[3], const int khmax[3], S z0[3], S zh0[3], WallGeom &geo)
+{
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        z0[t] = zdis_full_at(g, dt, dt2, dt3_6, V[t], w);
+        const int kc = departure_cell(g.gzh, w, w, kcmax[t], val(z0[t]), KMIN, KMAX);
+        geo.up1[t] = (kcmax[t] != w) && (kc == w + 1);
+        if (w >= KMIN + 1) {
+            zh0[t] = zdis_half_at(g, dt, dt2, dt3_6, V[t], w);
+            const int kch = departure_cell(g.gz_shift, w, w, khmax[t], val(zh0[t]), KMIN + 1, KMAX);
+            geo.half_next[t] = (kch == w + 1);
+            geo.half_full[t] = (kcmax[t] != w) && (kch == w + 1);
+        } else {
+            zh0[t] = S(0.0);
+            geo.half_next[t] = geo.half_full[t] = false;
+        }
+    }
+}
+
+template <typename S>
+NSW6_HD void wall_flux(const GridDev &g, int w, const Cell<S> &cur, const Cell<S> &nxt, const S z0[3],
+                       const S zh0[3], const WallGeom &geo, WallFlux<S> &F, S ft_surf[3][4])
+{
+#pragma unroll
+    for (int i = 0; i < 5; i++) F.fs[i] = S(0.0);
+    F.fh[0] = F.fh[1] = S(0.0);
+    const double dzw = g.dgz[w];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        S ft[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const S r0 = rhof_cell(cur, t, i);
+            const S v = geo.up1[t] ? S(-(r0 * dzw) + rhof_cell(nxt, t, i) * z0[t]) : S(r0 * z0[t]);
+            ft[i] = eps_filter(v);
+        }
+        F.fq[t] = ft[0];
+#pragma unroll
+        for (int i = 1; i < 6; i++) F.fs[i - 1] += ft[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) ft_surf[t][i] = ft[i];
+        if (w >= KMIN + 1) {
+            const double dzh = g.dgzh[w];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const S hc = cur.hv[t][i], hn = nxt.hv[t][i];
+                S v;
+                if (geo.half_full[t])
+                    v = -(hc * dzh) + hn * zh0[t];
+                else
+                    v = (geo.half_next[t] ? hn : hc) * zh0[t];
+                F.fh[i] += eps_filter(v);
+            }
+        }
+    }
+}
+
+// Surface fluxes of wall kmin (src/mod_precip_transport.cpp:900-958) and mp_driver's
+// scaling by fraction_mp = 1 (src/mod_mp_driver.cpp:560-612).
+template <typename S> struct Surface {
+    S precip0, precip1, lh, prhoe, prhophi, prhokin;
+};
+template <typename S> NSW6_HD void surface_fluxes(const S ft[3][4], double dt, Surface<S> &s)
+{
+    S precip0 = S(0.0), precip1 = S(0.0), lh = S(0.0), prhoe = S(0.0), prhophi = S(0.0), prhokin = S(0.0);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        if (t == 0) {
+            precip0 = precip0 - ft[t][0] / dt;
+        } else {
+            precip1 = precip1 - ft[t][0] / dt;
+            lh = lh + ft[t][0] * C_LHF0 / dt;
+        }
+        prhoe = prhoe - ft[t][1] / dt;
+        prhophi = prhophi - ft[t][2] / dt;
+        prhokin = prhokin - ft[t][3] / dt - 0.0 / dt;   // the half-level flux at kmin is 0
+    }
+    s.precip0 = (0.0 + precip0) * 1.0;
+    s.precip1 = (0.0 + precip1) * 1.0;
+    s.lh = (0.0 + lh) * 1.0;
+    s.prhoe = (0.0 + prhoe) * 1.0;
+    s.prhophi = (0.0 + prhophi) * 1.0;
+    s.prhokin = (0.0 + prhokin) * 1.0;
+}
+
+// ---------------------------------------------------------------------------------
+// Sedimentation apply and KIN2EIN of interior cell c (src/mod_precip_transport.cpp:
+// 960-1011) from the fluxes of walls c-1 .. c+2.
+// ---------------------------------------------------------------------------------
+template <typename S> struct SedIn {
+    // fluxes: fq of walls c-1..c+2, fs of walls c..c+1, fh of walls c-1..c+1
+    S fq[4][3], fs[2][5], fh[3][2];
+    S rw_c, rw_p;                     // pre-transport rhogw of cells c, c+1
+    S rg_m, rg_c, rg_p;               // pre-transport rhog of cells c-1, c, c+1
+    S rq[3], re0;                     // QR, QS, QG and rhoge of cell c
+    S rvx, rvy, rvz;                  // momentum of cell c
+    double gs_m, gs_c, gs_p, gsh_c, gsh_p, z;
+};
+template <typename S> struct SedOut {
+    S rg, e, rq[3], rvx, rvy, rvz, rw;
+};
+
+template <typename S>
+NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x, SedOut<S> &y)
+{
+    // tendencies of cell c (walls c and c+1; wall 95 carries zero fluxes)
+    const double rc = g.rdgz[c];
+    S dq[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) dq[t] = -(x.fq[2][t] - x.fq[1][t]) * rc;
+    const S de = -(x.fs[1][0] - x.fs[0][0]) * rc;
+    const S dphi = -(x.fs[1][1] - x.fs[0][1]) * rc;
+    const S dkinh = -(x.fs[1][2] - x.fs[0][2]) * rc;
+    const S du = -(x.fs[1][3] - x.fs[0][3]) * rc;
+    const S dv = -(x.fs[1][4] - x.fs[0][4]) * rc;
+    // mass tendencies of cells c+1 and c-1
+    S dqp[3] = {S(0.0), S(0.0), S(0.0)}, dqm[3] = {S(0.0), S(0.0), S(0.0)};
+    if (c + 1 <= KMAX) {
+        const double r = g.rdgz[c + 1];
+#pragma unroll
+        for (int t = 0; t < 3; t++) dqp[t] = -(x.fq[3][t] - x.fq[2][t]) * r;
+    }
+    if (c - 1 >= KMIN) {
+        const double r = g.rdgz[c - 1];
+#pragma unroll
+        for (int t = 0; t < 3; t++) dqm[t] = -(x.fq[1][t] - x.fq[0][t]) * r;
+    }
+    // half-level tendencies at walls c and c+1 (walls 2..94; the flux of wall 1 is 0)
+    S hk_c = S(0.0), hw_c = S(0.0), hk_p = S(0.0), hw_p = S(0.0);
+    if (c >= KMIN + 1) {
+        const double r = g.rdgzh[c];
+        hk_c = -(x.fh[1][0] - x.fh[0][0]) * r;
+        hw_c = -(x.fh[1][1] - x.fh[0][1]) * r;
+    }
+    if (c + 1 <= KMAX) {
+        const double r = g.rdgzh[c + 1];
+        hk_p = -(x.fh[2][0] - x.fh[1][0]) * r;
+        hw_p = -(x.fh[2][1] - x.fh[1][1]) * r;
+    }
+    // pre-transport wall quantities of cells c and c+1 (C2W factors, rhogkin_v)
+    const double c2w0_c = g.afact[c] / x.gs_c * x.gsh_c, c2w1_c = g.bfact[c] / x.gs_m * x.gsh_c;
+    const double c2w0_p = g.afact[c + 1] / x.gs_p * x.gsh_p, c2w1_p = g.bfact[c + 1] / x.gs_c * x.gsh_p;
+    const S kinv0_c = (c >= KMIN + 1) ? S(0.5 * (x.rw_c * x.rw_c) / (c2w0_c * x.rg_c + c2w1_c * x.rg_m)) : S(0.0);
+    const S kinv0_p = (c + 1 <= KMAX) ? S(0.5 * (x.rw_p * x.rw_p) / (c2w0_p * x.rg_p + c2w1_p * x.rg_c)) : S(0.0);
+
+    const double w2c0 = g.cfact[c] * x.gs_c / x.gsh_p, w2c1 = g.dfact[c] * x.gs_c / x.gsh_p;
+    const S kinh = 0.5 * (x.rvx * x.rvx + x.rvy * x.rvy + x.rvz * x.rvz) / x.rg_c;
+    S e = x.re0 + de + dphi;
+    const S kinh_u = kinh + dkinh;
+    const S kinv_c = kinv0_c + hk_c;
+    const S kinv_cp = kinv0_p + hk_p;
+    y.rvx = x.rvx + du * u.ix + dv * u.jx;
+    y.rvy = x.rvy + du * u.iy + dv * u.jy;
+    y.rvz = x.rvz + du * u.iz + dv * u.jz;
+    y.rw = x.rw_c + hw_c;
+    S rg = x.rg_c;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        y.rq[t] = x.rq[t] + dq[t];
+        rg = rg + dq[t];
+        e = e - dq[t] * C_GRAV * x.z;
+    }
+    // KIN2EIN: kinetic energy before minus after, with the neighbours' new densities
+    const S tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
+    const S rg_cp = ((x.rg_p + dqp[0]) + dqp[1]) + dqp[2];
+    const S rg_cm = ((x.rg_m + dqm[0]) + dqm[1]) + dqm[2];
+    const S rw_cp = x.rw_p + hw_p;
+    const S kinh_new = 0.5 * (y.rvx * y.rvx + y.rvy * y.rvy + y.rvz * y.rvz) / rg;
+    const S kinv_new_c = (c >= KMIN + 1) ? S(0.5 * (y.rw * y.rw) / (c2w0_c * rg + c2w1_c * rg_cm)) : S(0.0);
+    const S kinv_new_cp = (c + 1 <= KMAX) ? S(0.5 * (rw_cp * rw_cp) / (c2w0_p * rg_cp + c2w1_p * rg)) : S(0.0);
+    const S tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
+    e = e + (tmp2 - tmpk);
+    y.rg = rg;
+    y.e = e;
+}
+
+// ---------------------------------------------------------------------------------
+// Point-wise tail: rediagnosis (src/mod_precip_transport.cpp:1074-1086),
+// SATURATION_adjustment with satadjust_liq (src/mod_satadjust.cpp:274-362, 771-896)
+// and the final negative_filter (src/mod_mp_nsw6.cpp:671, 1770-1831).
+// ---------------------------------------------------------------------------------
+template <typename S> struct TailIn {
+    S rhog, rhoge, rhogq[6];   // state after sedimentation
+    double gsgam2;
+};
+template <typename S> struct TailOut {
+    S rhog, rhoge, rho, tem, pre, qd, rhogq[6], q[6];
+};
+
+// Newton of satadjust_liq on values (src/mod_satadjust.cpp:828-894).  q[QV] and q[QC]
+// are updated in place; returns 0 when no iteration ran, n > 0 after convergence in n
+// iterations, -1 on a NaN temperature, -2 when itelim = 100 is reached.
+NSW6_HD int satadjust_liq_newton(const DevParams &P, double &tem, double q[6], double qsum, double qdl,
+                                 double rho_sa, double ein_moist)
+{
+    double rtem = 1.0 / dmax(tem, TEM_MIN);
+    double psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+    double qsat = psat / (rho_sa * C_Rvap * tem);
+    if (!(qsum - qsat > C_EPS)) return 0;
+    for (int ite = 0; ite < 100; ite++) {
+        rtem = 1.0 / dmax(tem, TEM_MIN);
+        psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+        qsat = psat / (rho_sa * C_Rvap * tem);
+        q[QV] = qsat;
+        q[QC] = qsum - qsat;
+        double CVt = qdl * C_CVdry;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) CVt = CVt + q[n] * C_CVW;
+        const double Emoist_new = tem * CVt + q[QV] * C_LHV0;
+        const double dqsat_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsat;
+        const double dCVtot_dT = dqsat_dT * (C_CVW - C_CVW);
+        const double dEmoist_dT = tem * dCVtot_dT + CVt + dqsat_dT * C_LHV0;
+        const double dtemp = (Emoist_new - ein_moist) / dEmoist_dT;
+        tem = tem - dtemp;
+        if (fabs(dtemp) < P.dtemp_criteria) return ite + 1;
+        if (tem * 0.0 != 0.0) return -1;
+    }
+    return -2;
+}
+
+// Residual of the satadjust_liq solve, E(T) - Emoist, for the implicit-function
+// derivative of the converged temperature: with q(QC) = qsum - qsat(T) and the
+// reference's heat capacity sum, E(T) = T (qdl CVdry + (qsum - qsat + qrest) CVW) +
+// qsat LHV0.
+template <typename S>
+NSW6_HD S satadjust_residual(const DevParams &P, S tem, S qsum, S qdl, S qrest, S rho_sa, S ein_moist)
+{
+    const S qsat = psat_of(P.LovR_liq, P.RTEM00, tem) / (rho_sa * C_Rvap * tem);
+    const S CVt = qdl * C_CVdry + ((qsum - qsat) + qrest) * C_CVW;
+    return tem * CVt + qsat * C_LHV0 - ein_moist;
+}
+// dE/dT of that residual at fixed inputs (exact, unlike the Newton's own slope)
+NSW6_HD double satadjust_dRdT(const DevParams &P, double tem, double qsum, double qdl, double qrest, double rho_sa)
+{
+    const double psat = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - 1.0 / dmax(tem, TEM_MIN)));
+    const double qsat = psat / (rho_sa * C_Rvap * tem);
+    const double dpsat = (tem > TEM_MIN) ? psat * P.LovR_liq / (tem * tem) : 0.0;
+    const double dqsat = dpsat / (rho_sa * C_Rvap * tem) - qsat / tem;
+    const double CVt = qdl * C_CVdry + ((qsum - qsat) + qrest) * C_CVW;
+    return CVt - tem * dqsat * C_CVW + dqsat * C_LHV0;
+}
+
+// Status of one point's tail (for the device report).
+struct TailStatus {
+    int newton;   // satadjust_liq_newton's return value
+};
+
+// The converged temperature and q(QV) = qsat of the satadjust_liq solve as values of
+// the scalar type.  For double they are the Newton's own last iterates; a dual-number
+// scalar overloads both (found by argument-dependent lookup) to carry the
+// implicit-function tangent of the solution.
+NSW6_HD double newton_tem(const DevParams &, double tem_final, double, double, double, double, double)
+{
+    return tem_final;
+}
+NSW6_HD double newton_qsat(const DevParams &, double qv_final, double, double) { return qv_final; }
+
+template <typename S>
+NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, TailOut<S> &y)
+{
+    TailStatus st{0};
+    S rhog = x.rhog, rhoge = x.rhoge, rhogq[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) rhogq[n] = x.rhogq[n];
+    const double gsgam2 = x.gsgam2;
+    S q[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) q[n] = rhogq[n] / rhog;
+    S rho = rhog / gsgam2;
+    const S ein = rhoge / rhog;
+    S qd = S(1.0);
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qd = qd - q[n];
+    S cv = S(0.0), qd2 = S(1.0);
+#pragma unroll
+    for (int n = 1; n <= 5; n++) {
+        cv = cv + q[n] * C_CVW;
+        qd2 = qd2 - q[n];
+    }
+    cv = cv + qd2 * C_CVdry;
+    S tem = ein / cv;
+    S pre = rho * tem * (qd2 * C_Rdry + q[QV] * C_Rvap);
+
+    // SATURATION_adjustment, ice_adjust = false
+    const S ein_moist = rhoge / rhog + q[QV] * C_LHV0;
+    const S qsum = q[QV] + q[QC];
+    q[QV] = qsum;
+    q[QC] = S(0.0);
+    S CVtot = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
+    const S rho_sa = rhog / gsgam2;
+    tem = (ein_moist - q[QV] * C_LHV0) / CVtot;
+    if (interior) {
+        S qdl = S(1.0);
+#pragma unroll
+        for (int n = 1; n <= 5; n++) qdl = qdl - q[n];
+        double tv = val(tem), qv6[6];
+#pragma unroll
+        for (int n = 0; n < 6; n++) qv6[n] = val(q[n]);
+        st.newton = satadjust_liq_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));
+        if (st.newton != 0) {
+            const S qrest = ((q[QR] + q[QI]) + q[QS]) + q[QG];
+            tem = newton_tem(P, tv, qsum, qdl, qrest, rho_sa, ein_moist);
+            q[QV] = newton_qsat(P, qv6[QV], tem, rho_sa);
+            q[QC] = qsum - q[QV];
+        }
+    }
+    rhogq[QV] = rhog * q[QV];
+    rhogq[QC] = rhog * q[QC];
+    S CVtot2 = qd * C_CVdry;
+#pragma unroll
+    for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;
+    rhoge = rhog * tem * CVtot2;
+
+    if (interior) {
+        // final negative_filter
+        S diffq = S(0.0);
+#pragma unroll
+        for (int n = 2; n <= 5; n++) {
+            diffq += rhogq[n];
+            rhogq[n] = dmax(rhogq[n], S(0.0));
+        }
+#pragma unroll
+        for (int n = 2; n <= 5; n++) diffq -= rhogq[n];
+        rhogq[QV] += diffq;
+        diffq = rhogq[QV];
+        rhogq[QV] = dmax(rhogq[QV], S(0.0));
+        diffq -= rhogq[QV];
+        rhog = rhog * (1.0 - diffq);
+        rho = rhog / gsgam2;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) q[n] = rhogq[n] / rhog;
+        S qdn = S(1.0);
+#pragma unroll
+        for (int n = 1; n <= 5; n++) qdn = qdn - q[n];
+        S cva = qdn * C_CVdry;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
+        rhoge = tem * rhog * cva;
+        pre = rho * (qdn * C_Rdry + q[QV] * C_Rvap) * tem;
+    }
+    y.rhog = rhog;
+    y.rhoge = rhoge;
+    y.rho = rho;
+    y.tem = tem;
+    y.pre = pre;
+    y.qd = qd;
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        y.rhogq[n] = rhogq[n];
+        y.q[n] = q[n];
+    }
+    return st;
+}
+
+}  // namespace nsw6

@@ -1,4 +1,5 @@
-"""TEST INFRASTRUCTURE ONLY: ctypes loader of the CPU oracle (liboracle_nsw6.so).
+"""TEST INFRASTRUCTURE ONLY: ctypes loaders of the CPU oracle (liboracle_nsw6.so) and
+of the host dual-number gradient check of the adjoint (libnsw6_grad.so).
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
 module, and only as the checker or the timed CPU baseline.
@@ -13,8 +14,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_nsw6.so")
+GRAD_LIB = os.path.join(HERE, "libnsw6_grad.so")
+GRAD_NDIR = 8
 
 _lib = None
+_glib = None
 
 
 def build(verbose: bool = False) -> None:
@@ -116,3 +120,119 @@ def last_elapsed() -> float:
 
 def num_threads() -> int:
     return lib().ora_num_threads()
+
+
+# ---------------------------------------------------------------------------------
+# host gradient check of the adjoint (libnsw6_grad.so)
+# ---------------------------------------------------------------------------------
+def grad_lib():
+    global _glib
+    if _glib is None:
+        if not os.path.exists(GRAD_LIB):
+            build()
+        from nicam_kokkos_microphysics_amd import _native as N
+        L = ctypes.CDLL(GRAD_LIB)
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.grad_step_fp64.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, ctypes.POINTER(N.FieldPtrs), D]
+        L.grad_step_fp64.restype = I
+        L.grad_loss_fp64.argtypes = [P, P, ctypes.POINTER(N.Params), P, I, I, ctypes.POINTER(N.FieldPtrs), D, P]
+        L.grad_loss_fp64.restype = I
+        L.grad_dual_loss.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, ctypes.POINTER(N.FieldPtrs), D, I,
+                                     ctypes.POINTER(N.FieldPtrs), P, P, P]
+        L.grad_dual_loss.restype = I
+        L.grad_check_pieces.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, ctypes.POINTER(N.FieldPtrs), D, P, P]
+        L.grad_check_pieces.restype = I
+        _glib = L
+    return _glib
+
+
+def _shape(state):
+    return state["rhog"].shape[0], state["rhog"].shape[-1]
+
+
+def step_fp64(state: dict, gz, gzh, dt: float, P=None) -> int:
+    """The templated step in FP64 on the host, in place (0 = ok, 1 = Newton failure,
+    -1 = a region needs the general transport)."""
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    fp = field_ptrs(state)
+    return grad_lib().grad_step_fp64(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                     ctypes.byref(fp), float(dt))
+
+
+def loss_fp64(state: dict, gz, gzh, dt: float, coef=None, P=None) -> float:
+    """The adjoint-check loss of one FP64 host step (state not modified); `coef`
+    overrides the 8 tunable coefficients."""
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    fp = field_ptrs(state)
+    out = ctypes.c_double()
+    c = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    st = grad_lib().grad_loss_fp64(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P),
+                                   None if c is None else c.ctypes.data, lall, ijdim, ctypes.byref(fp), float(dt),
+                                   ctypes.addressof(out))
+    if st < 0:
+        raise RuntimeError(f"grad_loss_fp64 status {st}")
+    return out.value
+
+
+def dual_loss(state: dict, gz, gzh, dt: float, dirs, dcoef=None, P=None):
+    """Loss and its directional derivatives along the directions `dirs` (a list of
+    dicts of direction arrays keyed like the state; missing keys are zero) and the
+    coefficient directions `dcoef` [len(dirs)][8], by forward-mode dual numbers."""
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    fp = field_ptrs(state)
+    ndir = len(dirs)
+    out = np.zeros(ndir)
+    loss = 0.0
+    for c0 in range(0, ndir, GRAD_NDIR):
+        chunk = dirs[c0:c0 + GRAD_NDIR]
+        arr = (N.FieldPtrs * len(chunk))()
+        keep = []
+        for i, d in enumerate(chunk):
+            dd = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in d.items()}
+            keep.append(dd)
+            arr[i] = field_ptrs(dd)
+        dc = np.zeros((len(chunk), 8)) if dcoef is None else np.ascontiguousarray(dcoef[c0:c0 + GRAD_NDIR],
+                                                                                dtype=np.float64)
+        lv = ctypes.c_double()
+        dl = np.zeros(len(chunk))
+        st = grad_lib().grad_dual_loss(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                       ctypes.byref(fp), float(dt), len(chunk), arr, dc.ctypes.data,
+                                       ctypes.addressof(lv), dl.ctypes.data)
+        if st < 0:
+            raise RuntimeError(f"grad_dual_loss status {st}")
+        loss = lv.value
+        out[c0:c0 + len(chunk)] = dl
+    return loss, out
+
+
+def check_pieces(state: dict, gz, gzh, dt: float, P=None):
+    """Largest relative mismatch between each hand-written reverse piece of the step
+    (point rates, wall fluxes, cell divergence/KIN2EIN, point-wise tail) and forward-
+    mode dual numbers over every point, wall and cell of `state`; returns (err[4],
+    count[4]).  Raises if a region needs the general transport."""
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    fp = field_ptrs(state)
+    err = np.zeros(4)
+    cnt = np.zeros(4, dtype=np.int64)
+    st = grad_lib().grad_check_pieces(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                      ctypes.byref(fp), float(dt), err.ctypes.data, cnt.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"grad_check_pieces status {st}")
+    return err, cnt
