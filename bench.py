@@ -3,8 +3,10 @@
 
 Workload: 10 regions x 16,900 columns = 169,000 columns x 96 levels, dt = 15 s,
 consecutive steps on a device-resident state (seeded synthetic NICAM columns).
-Prints ONE JSON line with column-steps/s, the roofline of the step measured with
-HIP events on the launch stream, and the CPU oracle timed on this host.
+Prints ONE JSON line: FP64 column-steps/s (the headline value) with the roofline of
+the step measured with HIP events on the launch stream; the mixed-precision form's
+throughput and its per-field error against the FP64 form; the reverse mode's cost
+relative to the forward step at 16,900 x 96; and the CPU oracle timed on this host.
 """
 from __future__ import annotations
 
@@ -29,7 +31,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="form of the headline measurement (the other form is reported beside it "
+                         "unless --only)")
+    ap.add_argument("--only", action="store_true",
+                    help="measure the headline form only (no second form, no adjoint; for profiling)")
+    ap.add_argument("--adjoint-reps", type=int, default=10)
     ap.add_argument("--workload", default=None)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
@@ -122,6 +129,93 @@ def cpu_baseline(reps: int):
                       f"({med * 1e3:.1f} ms/step)"}
 
 
+def time_steps(drv, dev, dt, steps, warmup, stream):
+    """`warmup` untimed steps, then `steps` timed steps bracketed by device
+    synchronisation; also the HIP-event durations of the step and its kernels."""
+    import torch
+    for _ in range(warmup):
+        drv.step(dev, dt, stream=stream, check=False)
+    drv.check()
+    torch.cuda.synchronize()
+    drv.timing(True)
+    drv.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        drv.step(dev, dt, stream=stream, check=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rep = drv.check()
+    step_ms, nev = drv.timing_read(0)
+    kernel_ms = {name: drv.timing_read(i)[0] for i, name in
+                 ((1, "rates"), (2, "extremes+bounds"), (3, "flux"), (4, "finish"))}
+    drv.timing(False)
+    return wall, step_ms, nev, kernel_ms, rep
+
+
+def other_form(prec_other, steps, warmup):
+    """Throughput of the other precision form on its own workload (169,000 x 96)."""
+    import torch
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    wname = "nsw6_169000x96_mixed" if prec_other == N.NSW6_MIXED else "nsw6_169000x96_fp64"
+    w = F.WORKLOADS[wname]
+    ncol = w.lall * w.ijdim
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    del st
+    drv = NSW6(gz, gzh, max_columns=ncol, precision=prec_other)
+    wall, step_ms, _, kernel_ms, rep = time_steps(drv, dev, w.dt, steps, warmup, torch.cuda.current_stream())
+    drv.close()
+    del dev
+    torch.cuda.empty_cache()
+    return {"workload": wname, "value": ncol * steps / wall, "unit": "column-steps/s",
+            "ms_per_step": wall * 1e3 / steps, "event_ms": step_ms, "kernels_ms": kernel_ms,
+            "general_regions": rep.n_general_regions}
+
+
+def adjoint_cost(reps):
+    """Reverse mode (nsw6_mp_driver_adj) against the FP64 forward step on the adjoint
+    workload (1 region x 16,900 columns x 96 levels): HIP-event durations on the launch
+    stream, `reps` calls of each after one warm-up.  The cotangents are those of the
+    adjoint-check loss (2 x surface precipitation, 1/dt on rhoge)."""
+    import torch
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    w = F.WORKLOADS["nsw6_16900x96_adjoint"]
+    ncol = w.lall * w.ijdim
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    drv = NSW6(gz, gzh, max_columns=ncol)
+    stream = torch.cuda.current_stream()
+    fwd = F.copy_state(dev)
+    drv.step(fwd, w.dt, stream=stream)
+    cot = {"precip": 2.0 * fwd["precip"], "rhoge": torch.full_like(fwd["rhoge"], 1.0 / w.dt)}
+    grad, gcoef = drv.adjoint(dev, cot, w.dt, stream=stream)   # warm-up (allocates the records)
+    torch.cuda.synchronize()
+    drv.timing(True)
+    drv.timing_reset()
+    for _ in range(reps):
+        s = F.copy_state(dev)
+        drv.step(s, w.dt, stream=stream, check=False)
+    for _ in range(reps):
+        drv.adjoint(dev, cot, w.dt, grad=grad, grad_coef=gcoef, stream=stream, check=False)
+    torch.cuda.synchronize()
+    rep = drv.check()
+    fwd_ms = drv.timing_read(0)[0]
+    adj_ms, nadj = drv.timing_read(5)
+    adj_fwd_ms = drv.timing_read(6)[0]
+    adj_rev_ms = drv.timing_read(7)[0]
+    drv.timing(False)
+    drv.close()
+    return {"workload": w.name, "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "ratio": adj_ms / fwd_ms,
+            "adjoint_forward_part_ms": adj_fwd_ms, "adjoint_reverse_passes_ms": adj_rev_ms,
+            "calls_timed": nadj, "general_regions": rep.n_general_regions,
+            "sample": f"{reps} forward steps and {reps} reverse-mode calls on the same input state, "
+                      f"HIP events on the launch stream"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -142,34 +236,15 @@ def main():
     prec = N.NSW6_MIXED if args.precision == "mixed" else N.NSW6_FP64
     drv = NSW6(gz, gzh, max_columns=ncol, precision=prec)
     stream = torch.cuda.current_stream()
-
-    for _ in range(args.warmup):
-        drv.step(dev, w.dt, stream=stream, check=False)
-    drv.check()
-    torch.cuda.synchronize()
-
-    drv.timing(True)
-    drv.timing_reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        drv.step(dev, w.dt, stream=stream, check=False)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    rep = drv.check()
-    step_ms, nev = drv.timing_read(0)
-    kernel_ms = {name: drv.timing_read(i)[0] for i, name in
-                 ((1, "rates"), (2, "extremes+bounds"), (3, "flux"), (4, "finish"))}
-    drv.timing(False)
-
+    wall, step_ms, nev, kernel_ms, rep = time_steps(drv, dev, w.dt, args.steps, args.warmup, stream)
     ms_per_step = wall * 1e3 / args.steps
     algo_bytes = ALGO_BYTES_PER_COLUMN * ncol
     achieved = algo_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
     traffic = read_traffic(args.traffic or os.path.join(REPO, "profiles", f"traffic_{wname}.txt"))
 
     out = {
-        "metric": "column-steps/s at 169k x 96 (FP64 mp_driver step)" if prec == N.NSW6_FP64
-        else "column-steps/s at 169k x 96 (mixed-precision mp_driver step)",
+        "metric": "column-steps/s at 169k x 96, FP64 vs mixed at stated error; adjoint/forward ratio",
+        "form": "fp64" if prec == N.NSW6_FP64 else "mixed",
         "value": ncol * args.steps / wall,
         "unit": "column-steps/s",
         "n_gpus": 1,
@@ -194,8 +269,15 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     drv.close()
     del dev
+    if not args.only:
+        other = N.NSW6_FP64 if prec == N.NSW6_MIXED else N.NSW6_MIXED
+        out["fp64" if other == N.NSW6_FP64 else "mixed"] = other_form(other, args.steps, args.warmup)
     if prec == N.NSW6_MIXED:
         out["error_vs_fp64"] = mixed_error(w, w.steps)
+    elif not args.only:
+        out["mixed"]["error_vs_fp64"] = mixed_error(F.WORKLOADS["nsw6_169000x96_mixed"], w.steps)
+    if not args.only:
+        out["adjoint"] = adjoint_cost(args.adjoint_reps)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

@@ -19,6 +19,8 @@
  *   nsw6_mp_driver       <- MP_DRIVER::mp_driver (include/mod_mp_driver.h:71-124,
  *                           src/mod_mp_driver.cpp:335-613), same argument order
  *   nsw6_mp_driver_fields<- the same call with the 49 arrays gathered in a struct
+ *   nsw6_mp_driver_adj   <- no counterpart: the reverse mode (vector-Jacobian product)
+ *                           of one FP64 mp_driver step
  *   nsw6_finalize        <- the end of the reference's process (views released)
  *
  * All device pointers handed to nsw6_mp_driver* must be HIP device memory on the
@@ -47,7 +49,9 @@ enum nsw6_status {
     NSW6_NAN_DETECTED   = 2,  /* tem*0 != 0 guard fired (src/mod_satadjust.cpp:880)             */
     NSW6_BAD_ARG        = 3,
     NSW6_HIP_ERROR      = 4,
-    NSW6_NO_DEVICE      = 5
+    NSW6_NO_DEVICE      = 5,
+    NSW6_UNSUPPORTED    = 6   /* the adjoint met a region that needs the general       */
+                              /* (multi-cell) sedimentation; its gradient is left 0   */
 };
 
 enum nsw6_precision {
@@ -166,13 +170,40 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
     double *z, double *zh, double dt, double ct, double *GDCLW, double *GDCFRC,
     double *GPREC, double *CBMFX, void *stream);
 
-/* Synchronise the context's last stream and return the device report of the last step. */
+/* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
+ * explicit ice nucleation; sw_expice = 1): given the step's INPUT state `state`
+ * (not modified; the diagnostics a step would write are kept in context scratch)
+ * and cotangents of the step's outputs, write the cotangents of its inputs,
+ *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.
+ * Output cotangents `cot` (NULL members = 0): rhog, rhogvx, rhogvy, rhogvz, rhogw,
+ *   rhoge, rhogq, rho, tem, pre, q, qd (after the step), precip, precip_rhoe,
+ *   precip_lh_heat, precip_rhophi, precip_rhokin, GPREC; plus `cot_sflux`
+ *   [l][3][ij] (NULL = 0) for the surface mass fluxes sflux_t = -f_t(kmin)/dt of
+ *   QR, QS, QG.  The other diagnostics carry no cotangent.
+ * Input cotangents `grad` (NULL members are not written): rhog, rhogvx, rhogvy,
+ *   rhogvz, rhogw, rhoge (nonzero on the two halo levels only: the interior internal
+ *   energy is recomputed from tem), rhogq, vx, vy, vz, tem, q (QV only; other
+ *   species get 0).  `grad_coef` (device, NSW6_NCOEF doubles, may be NULL): the
+ *   cotangents of the tunable coefficients in NSW6_NCOEF order.
+ * Discrete choices (switches, limiters, kcell bounds, departure cells) are taken at
+ * the input state, and the saturation adjustment is differentiated at its converged
+ * solution (implicit-function theorem).  Regions whose departures leave one cell
+ * get gradient 0 and make nsw6_last_status() return NSW6_UNSUPPORTED.
+ * A non-default scheme returns NSW6_UNSUPPORTED at once (nothing is launched). */
+int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
+                       const nsw6_fields *state, const nsw6_fields *cot,
+                       const double *cot_sflux, const nsw6_fields *grad,
+                       double *grad_coef, double dt, double ct, void *stream);
+
+/* Synchronise the context's last stream and return the device report of the last
+ * step (or of the forward part recomputed by the last nsw6_mp_driver_adj). */
 int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *report);
 
 /* Average duration (ms) of the named kernel over the launches recorded since the
  * last reset, measured with HIP events on the stream the kernel ran on.
  * which: 0 = whole step, 1 = process rates, 2 = fall-speed extremes + kcell bounds,
- * 3 = sedimentation fluxes, 4 = flux divergence + KIN2EIN + saturation adjustment tail.
+ * 3 = sedimentation fluxes, 4 = flux divergence + KIN2EIN + saturation adjustment tail;
+ * for nsw6_mp_driver_adj: 5 = whole call, 6 = its forward part, 7 = the reverse passes.
  * Recording is off until nsw6_timing_enable(ctx, 1). */
 int nsw6_timing_enable(nsw6_ctx *ctx, int on);
 int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count);

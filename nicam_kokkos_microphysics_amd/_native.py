@@ -18,11 +18,12 @@ LIB_SYNTH = os.path.join(HERE, "libnsw6_synth.so")
 
 KALL, KMIN, KMAX, NQMAX = 96, 1, 94, 6
 NSW6_OK, NSW6_NOT_CONVERGED, NSW6_NAN_DETECTED = 0, 1, 2
-NSW6_BAD_ARG, NSW6_HIP_ERROR, NSW6_NO_DEVICE = 3, 4, 5
+NSW6_BAD_ARG, NSW6_HIP_ERROR, NSW6_NO_DEVICE, NSW6_UNSUPPORTED = 3, 4, 5, 6
 NSW6_FP64, NSW6_MIXED = 0, 1
 
 # The 8 tunable coefficients (order of NSW6_NCOEF in nsw6.h).
 TUNABLE_COEFFS = ("kk_aut_rate", "qicrt_saut", "Esi", "Esr", "Cr", "Cs", "Cg", "icenuc_coef")
+NCOEF = len(TUNABLE_COEFFS)
 
 
 def _struct_body(text: str, name: str) -> str:
@@ -88,6 +89,8 @@ SIGNATURES = {
     "nsw6_set_precision": (_I, [_P, _I]),
     "nsw6_mp_driver_fields": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), _D, _D, _P]),
     "nsw6_mp_driver": (_I, [_P, _I, _I, _I] + [_P] * 46 + [_D, _D] + [_P] * 4 + [_P]),
+    "nsw6_mp_driver_adj": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), ctypes.POINTER(FieldPtrs), _P,
+                                ctypes.POINTER(FieldPtrs), _P, _D, _D, _P]),
     "nsw6_last_status": (_I, [_P, ctypes.POINTER(Report)]),
     "nsw6_timing_enable": (_I, [_P, _I]),
     "nsw6_timing_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)]),
@@ -108,6 +111,10 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(
                 f"{LIB_NSW6} is missing: build it with `python -m nicam_kokkos_microphysics_amd.build_lib` "
                 "(there is no CPU fallback for the NSW6 step)")
+        # torch brings its own copy of the HIP runtime (same soname): load it first so
+        # that libnsw6.so binds to the runtime instance that owns torch's streams and
+        # allocations
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_NSW6)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

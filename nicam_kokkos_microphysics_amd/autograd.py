@@ -1,0 +1,57 @@
+"""``torch.autograd`` binding of the FP64 NSW6 step and its hand-written reverse mode.
+
+:class:`MPStep` is a ``torch.autograd.Function``: its forward runs one mp_driver step
+(nsw6_mp_driver_fields) on copies of the differentiable inputs, its backward calls
+nsw6_mp_driver_adj with the saved input state.  Differentiable inputs are the
+tensors of ``fields.ADJ_INPUTS`` and the 8 tunable coefficients; differentiable
+outputs are the tensors of ``fields.ADJ_OUTPUTS`` (freshly allocated per call).
+The per-tracer surface mass fluxes have a cotangent argument in
+``NSW6.adjoint(cot_sflux=...)`` but are not outputs of this Function.
+
+The coefficients a step runs with are those of the context (``NSW6.set_params``);
+their tensor in the call carries the gradient only, and must equal the context's.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _native as N
+from .fields import ADJ_INPUTS, ADJ_OUTPUTS
+
+
+class MPStep(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, nsw6, metrics: dict, dt: float, coef: torch.Tensor, *inputs):
+        state_in = dict(zip(ADJ_INPUTS, inputs))
+        saved = {k: v.detach().clone() for k, v in state_in.items()}
+        work = dict(metrics)
+        work.update({k: torch.empty_like(metrics[k]) for k in ADJ_OUTPUTS if k not in saved})
+        work.update({k: v.clone() for k, v in saved.items()})
+        nsw6.step(work, dt)
+        outs = tuple(work[k] for k in ADJ_OUTPUTS)
+        ctx.nsw6, ctx.metrics, ctx.dt = nsw6, metrics, dt
+        ctx.save_for_backward(*saved.values())
+        return outs
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        saved = dict(zip(ADJ_INPUTS, ctx.saved_tensors))
+        state = dict(ctx.metrics)
+        state.update(saved)
+        cot = {k: g.contiguous() for k, g in zip(ADJ_OUTPUTS, gouts) if g is not None}
+        grad, gcoef = ctx.nsw6.adjoint(state, cot, ctx.dt)
+        return (None, None, None, gcoef) + tuple(grad[k] for k in ADJ_INPUTS)
+
+
+def mp_step(nsw6, state: dict, dt: float, coef: torch.Tensor | None = None) -> dict:
+    """One differentiable FP64 step: `state` holds every mp_driver array as device
+    tensors (the differentiable ones may require grad); returns a dict of the
+    ADJ_OUTPUTS tensors after the step.  `coef` (8 doubles, default: the context's
+    coefficients) receives the coefficient gradient."""
+    if coef is None:
+        p = nsw6.params
+        coef = torch.tensor([getattr(p, n) for n in N.TUNABLE_COEFFS], dtype=torch.float64,
+                            device=state["rhog"].device)
+    metrics = {k: v for k, v in state.items() if k not in ADJ_INPUTS}
+    outs = MPStep.apply(nsw6, metrics, dt, coef, *[state[k] for k in ADJ_INPUTS])
+    return dict(zip(ADJ_OUTPUTS, outs))

@@ -1,7 +1,8 @@
 """Build the native libraries of the package in-tree.
 
-* ``libnsw6.so``       -- HIP kernels for gfx950 + the C ABI of ``include/nsw6.h``
-                          (hipcc), plus the host set-up code (gcc).
+* ``libnsw6.so``       -- HIP kernels for gfx950 (the step and its reverse mode) +
+                          the C ABI of ``include/nsw6.h`` (hipcc), plus the host
+                          set-up code (gcc).
 * ``libnsw6_synth.so`` -- the seeded synthetic-input generator (gcc + OpenMP).
 
 The FP64 kernels are compiled without FMA contraction (``-ffp-contract=off``) so
@@ -14,6 +15,7 @@ import os
 import shlex
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -48,13 +50,20 @@ def build(verbose: bool = True) -> None:
     """Compile both libraries (always recompiles)."""
     os.makedirs(BUILD, exist_ok=True)
     obj_k = os.path.join(BUILD, "nsw6_kernels.o")
+    obj_r = os.path.join(BUILD, "nsw6_adjoint.o")
     obj_a = os.path.join(BUILD, "nsw6_api.o")
     obj_h = os.path.join(BUILD, "nsw6_host.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nsw6_kernels.hip"), "-o", obj_k], verbose)
-    _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "nsw6_api.cpp"), "-o", obj_a], verbose)
-    _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_host.c"), "-o", obj_h], verbose)
+    steps = [
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nsw6_kernels.hip"), "-o", obj_k],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nsw6_adjoint.hip"), "-o", obj_r],
+        [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "nsw6_api.cpp"), "-o", obj_a],
+        ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_host.c"), "-o", obj_h],
+    ]
+    with ThreadPoolExecutor(max_workers=len(steps)) as pool:
+        for fut in [pool.submit(_run, cmd, verbose) for cmd in steps]:
+            fut.result()
     tmp = LIB_NSW6 + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_a, obj_h], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_r, obj_a, obj_h], verbose)
     os.replace(tmp, LIB_NSW6)
     tmp = LIB_SYNTH + ".tmp"
     _run(["gcc", *C_FLAGS, "-fopenmp", "-shared", "-o", tmp, os.path.join(CSRC, "nsw6_synth.c"),

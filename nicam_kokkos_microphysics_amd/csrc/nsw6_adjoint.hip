@@ -1,0 +1,114 @@
+// nsw6_adjoint.hip -- reverse mode of one FP64 NSW6 step on gfx950.
+//
+// Launch sequence (one stream): the forward step up to the wall fluxes
+// (nsw6_launch_forward_to_flux: k_rates<double>, k_extremes, k_bounds, k_flux), then
+// three point-wise reverse passes over (l, k, ij) -- k_adj_cell, k_adj_wall,
+// k_adj_point, whose bodies are in nsw6_adjoint_sweep.h -- and k_adj_coef, which sums
+// the per-block coefficient cotangents in a fixed order.  The passes communicate only
+// through gather-only record arrays, so the result is deterministic and needs no
+// atomics.  Block order is the XCD-aware level order of the forward kernels: a block
+// re-reads the records of the neighbouring levels just written by the same XCD.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nsw6_adjoint_sweep.h"
+
+using namespace nsw6;
+
+extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStream_t stream);
+
+namespace {
+
+struct AdjLaunch {
+    AdjArgs A;
+    double *coef_part;   // [blocks][NCOEF]
+    double *grad_coef;   // [NCOEF]
+};
+
+__global__ void __launch_bounds__(256) k_adj_cell(AdjLaunch L)
+{
+    const int ijdim = L.A.a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    const int ij = lb.bx * 256 + threadIdx.x;
+    if (!lb.valid || ij >= ijdim) return;
+    adj_cell(L.A, lb.l, lb.k, ij);
+}
+
+__global__ void __launch_bounds__(256) k_adj_wall(AdjLaunch L)
+{
+    const int ijdim = L.A.a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    const int ij = lb.bx * 256 + threadIdx.x;
+    if (!lb.valid || ij >= ijdim) return;
+    adj_wall(L.A, lb.l, lb.k, ij);
+}
+
+__global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
+{
+    __shared__ double red[4][NCOEF];
+    const int ijdim = L.A.a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    const int ij = lb.bx * 256 + threadIdx.x;
+    double tun_b[NCOEF];
+#pragma unroll
+    for (int c = 0; c < NCOEF; c++) tun_b[c] = 0.0;
+    if (lb.valid && ij < ijdim) adj_point(L.A, lb.l, lb.k, ij, tun_b);
+    // per-block sums of the coefficient cotangents (every thread reaches this point)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NCOEF; c++) {
+        double v = tun_b[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wv][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NCOEF) {
+        const int c = threadIdx.x;
+        L.coef_part[(int64_t)blockIdx.x * NCOEF + c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    }
+}
+
+// One block: sum of the per-block coefficient cotangents in a fixed order.
+__global__ void __launch_bounds__(256) k_adj_coef(AdjLaunch L, int nblocks)
+{
+    __shared__ double red[256];
+    for (int c = 0; c < NCOEF; c++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) s += L.coef_part[(int64_t)b * NCOEF + c];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) L.grad_coef[c] = red[0];
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// Number of blocks of the reverse passes (the coefficient partials hold NCOEF each).
+extern "C" int nsw6_adjoint_blocks(int lall, int ijdim) { return level_grid((ijdim + 255) / 256, KALL, lall); }
+
+// Events (NULL or 3): before the forward part, after it, after the reverse passes.
+extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part, double *grad_coef,
+                                          hipStream_t stream, hipEvent_t *ev)
+{
+    AdjLaunch L;
+    L.A = *adj;
+    L.coef_part = coef_part;
+    L.grad_coef = grad_coef;
+    if (ev) (void)hipEventRecord(ev[0], stream);
+    hipError_t e = nsw6_launch_forward_to_flux(&L.A.a, stream);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], stream);
+    const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
+    hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_point, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_coef, dim3(1), dim3(256), 0, stream, L, nb);
+    if (ev) (void)hipEventRecord(ev[2], stream);
+    return hipGetLastError();
+}

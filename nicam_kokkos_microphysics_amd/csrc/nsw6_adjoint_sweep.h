@@ -1,0 +1,380 @@
+// nsw6_adjoint_sweep.h -- reverse mode of one FP64 NSW6 step over the single-cell
+// regions, as three point-wise passes shared by the HIP kernels and the host check.
+//
+// The forward step is rebuilt up to the wall fluxes first (rates, kcell bounds,
+// flux_wall).  Then
+//   adj_cell   per cell c: sediment() and the tail recomputed, tail_b, sediment_b.
+//              Writes the cell record: the cotangents cell c owes to the fluxes of
+//              walls c-1..c+2, to rhog of cells c-1..c+1 and to rhogw of cells c, c+1,
+//              and its own pre-sedimentation / momentum cotangents.
+//   adj_wall   per wall w: the flux cotangent is gathered from the four cell records
+//              that read the wall, plus the frain and surface-flux terms; then
+//              wall_flux_b, the departure-distance reverse and derive_cell_b.  Writes
+//              the wall record: cotangents of cell w-1 (rhog, QR, QS, QG), cells w
+//              and w+1 (raw cell inputs) and of Vt(w-3..w+1).
+//   adj_point  per point: gathers every record naming the point, then point_rates_b;
+//              writes the input cotangents and returns the coefficient cotangents.
+// Every record slot has one writer and every gather only reads: no atomics, and the
+// result does not depend on the order the points are visited in.
+//
+// Cotangents of the outputs are those of the mp_driver arrays after the step
+// (state, rho, tem, pre, q, qd, surface precipitation and its energy fluxes, GPREC)
+// plus the tracers' surface mass fluxes sflux_t = -f_t(kmin)/dt.  The diagnostics
+// re_*, rctop, rwtop, tctop carry no cotangent.  Inputs that receive a cotangent:
+// rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge (halo levels only: the interior rhoge
+// is recomputed from tem), rhogq, vx, vy, vz, tem, q(QV), and the 8 tunable rate
+// coefficients.
+#pragma once
+
+#include <stdint.h>
+
+#include "nsw6_adjoint.h"
+#include "nsw6_columns.h"
+
+namespace nsw6 {
+
+NSW6_HD double cot_at(const double *p, int64_t i) { return p ? p[i] : 0.0; }
+
+NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
+{
+    const StepArgs &a = A.a;
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
+    const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
+    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
+    const bool interior = (k >= KMIN && k <= KMAX);
+    double cr[NCR], dws[NDW], din[NDI];
+#pragma unroll
+    for (int i = 0; i < NCR; i++) cr[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NDW; i++) dws[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NDI; i++) din[i] = 0.0;
+
+    if (a.ws.region_flag[l] == 0) {
+        const Fields &C = A.cot;
+        TailOut<double> yb;
+        yb.rhog = cot_at(C.rhog, p);
+        yb.rhoge = cot_at(C.rhoge, p);
+        yb.rho = cot_at(C.rho, p);
+        yb.tem = cot_at(C.tem, p);
+        yb.pre = cot_at(C.pre, p);
+        yb.qd = cot_at(C.qd, p);
+#pragma unroll
+        for (int n = 0; n < NQ; n++) {
+            yb.rhogq[n] = cot_at(C.rhogq, pq + n * K2);
+            yb.q[n] = cot_at(C.q, pq + n * K2);
+        }
+        TailIn<double> x;
+        x.gsgam2 = a.f.gsgam2[p];
+        x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
+        x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
+        x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+        TailInB xb;
+        xb.rhog = xb.rhoge = 0.0;
+#pragma unroll
+        for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
+        if (interior) {
+            SedIn<double> si;
+            load_sed(a, l, ij, k, si);
+            const Unit u = load_unit(a.f, (int64_t)l * ijdim + ij);
+            SedOut<double> so;
+            sediment(*a.grid, k, u, si, so);
+            x.rhog = so.rg;
+            x.rhoge = so.e;
+            x.rhogq[QR] = so.rq[0];
+            x.rhogq[QS] = so.rq[1];
+            x.rhogq[QG] = so.rq[2];
+            tail_b(*a.P, true, x, yb, xb);
+            SedOut<double> sb;
+            sb.rg = xb.rhog;
+            sb.e = xb.rhoge;
+            sb.rq[0] = xb.rhogq[QR];
+            sb.rq[1] = xb.rhogq[QS];
+            sb.rq[2] = xb.rhogq[QG];
+            sb.rvx = cot_at(C.rhogvx, p);
+            sb.rvy = cot_at(C.rhogvy, p);
+            sb.rvz = cot_at(C.rhogvz, p);
+            sb.rw = cot_at(C.rhogw, p);
+            SedInB s;
+            zero(s);
+            sediment_b(*a.grid, k, u, si, sb, s);
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int t = 0; t < 3; t++) cr[CR_FQ + m * 3 + t] = s.fq[m][t];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int i = 0; i < 5; i++) cr[CR_FS + m * 5 + i] = s.fs[m][i];
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+#pragma unroll
+                for (int i = 0; i < 2; i++) cr[CR_FH + m * 2 + i] = s.fh[m][i];
+            cr[CR_RG + 0] = s.rg_m;
+            cr[CR_RG + 1] = s.rg_c;
+            cr[CR_RG + 2] = s.rg_p;
+            cr[CR_RW + 0] = s.rw_c;
+            cr[CR_RW + 1] = s.rw_p;
+            dws[1] = s.re0;
+            dws[2 + QV] = xb.rhogq[QV];
+            dws[2 + QC] = xb.rhogq[QC];
+            dws[2 + QR] = s.rq[0];
+            dws[2 + QI] = xb.rhogq[QI];
+            dws[2 + QS] = s.rq[1];
+            dws[2 + QG] = s.rq[2];
+            din[0] = s.rvx;
+            din[1] = s.rvy;
+            din[2] = s.rvz;
+        } else {
+            x.rhog = a.ws.rhog[p] + 0.0;
+            x.rhoge = a.ws.rhoge[p] + 0.0;
+            x.rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
+            x.rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
+            x.rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+            tail_b(*a.P, false, x, yb, xb);
+            dws[0] = xb.rhog;
+            dws[1] = xb.rhoge;
+#pragma unroll
+            for (int n = 0; n < NQ; n++) dws[2 + n] = xb.rhogq[n];
+            // halo momentum passes through unchanged
+            din[0] = cot_at(C.rhogvx, p);
+            din[1] = cot_at(C.rhogvy, p);
+            din[2] = cot_at(C.rhogvz, p);
+            din[3] = cot_at(C.rhogw, p);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NCR; i++) A.cr[i * N2 + p] = cr[i];
+#pragma unroll
+    for (int i = 0; i < NDW; i++) A.dws[i * N2 + p] = dws[i];
+#pragma unroll
+    for (int i = 0; i < NDI; i++) A.din[i * N2 + p] = din[i];
+}
+
+NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
+{
+    const StepArgs &a = A.a;
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    const int64_t pw = p2 + (int64_t)w * ijdim;
+    double wr[NWR];
+#pragma unroll
+    for (int i = 0; i < NWR; i++) wr[i] = 0.0;
+
+    if (a.ws.region_flag[l] == 0 && w >= KMIN && w <= KMAX) {
+        const GridDev &g = *a.grid;
+        const double dt = a.dt;
+        // the wall's flux cotangent, gathered from the interior cells that read it:
+        // cell w+1-m reads fq of wall w as its wall m (m = 0..3), cell w-m reads fs as
+        // its wall m (m = 0..1), cell w+1-m reads fh as its wall m (m = 0..2)
+        auto CRV = [&](int slot, int c) {
+            return (c >= KMIN && c <= KMAX) ? A.cr[slot * N2 + p2 + (int64_t)c * ijdim] : 0.0;
+        };
+        WallFlux<double> Fb;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; m++) s += CRV(CR_FQ + m * 3 + t, w + 1 - m);
+            Fb.fq[t] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 5; i++) Fb.fs[i] = CRV(CR_FS + i, w) + CRV(CR_FS + 5 + i, w - 1);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; m++) s += CRV(CR_FH + m * 2 + i, w + 1 - m);
+            Fb.fh[i] = s;
+        }
+        // GPREC(w) = frain = -fq_QR(w) / dt
+        Fb.fq[0] += -cot_at(A.cot.GPREC, pw) / dt;
+        double ftb[3][4];
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) ftb[t][i] = 0.0;
+        if (w == KMIN) {
+            const int64_t pc = (int64_t)l * ijdim + ij, pp = (int64_t)l * 2 * ijdim + ij;
+            const double c_p0 = cot_at(A.cot.precip, pp), c_p1 = cot_at(A.cot.precip, pp + ijdim);
+            const double c_lh = cot_at(A.cot.precip_lh_heat, pc), c_e = cot_at(A.cot.precip_rhoe, pc);
+            const double c_phi = cot_at(A.cot.precip_rhophi, pc), c_kin = cot_at(A.cot.precip_rhokin, pc);
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                ftb[t][0] = (t == 0) ? -c_p0 / dt : (-c_p1 / dt + c_lh * C_LHF0 / dt);
+                ftb[t][1] = -c_e / dt;
+                ftb[t][2] = -c_phi / dt;
+                ftb[t][3] = -c_kin / dt;
+                if (A.cot_sflux) Fb.fq[t] += -A.cot_sflux[((int64_t)l * 3 + t) * ijdim + ij] / dt;
+            }
+        }
+        WallState s;
+        wall_state(a, l, w, ij, s);
+        CellB cb0, cb1;
+        zero(cb0);
+        zero(cb1);
+        double z0b[3] = {0.0, 0.0, 0.0}, zh0b[3] = {0.0, 0.0, 0.0};
+        wall_flux_b(g, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, Fb, ftb, cb0, cb1, z0b, zh0b);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            double Vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            zdis_full_b(g, a.dt, a.dt2, a.dt3_6, s.V[t], w, z0b[t], Vb);
+            if (w >= KMIN + 1) zdis_half_b(g, a.dt, a.dt2, a.dt3_6, s.V[t], w, zh0b[t], Vb);
+#pragma unroll
+            for (int d = 0; d < 5; d++) wr[WR_V + t * 5 + d] = Vb[d];
+        }
+        CellInB rb0, rb1;
+        zero(rb0);
+        zero(rb1);
+        double brg_b = 0.0, bq_b[3] = {0.0, 0.0, 0.0};
+        derive_cell_b(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt, cb1, rb1, cb0.rhog, cb0.q);
+        derive_cell_b(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur, cb0, rb0, brg_b, bq_b);
+        // q_m[t] = rhogq_t(w-1) / rhog(w-1)
+        double rgm_b = brg_b;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            wr[WR_M + 1 + t] = bq_b[t] / s.m.rhog;
+            rgm_b -= bq_b[t] * s.m.q[t] / s.m.rhog;
+        }
+        wr[WR_M + 0] = rgm_b;
+        const CellInB *rbs[2] = {&rb0, &rb1};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const CellInB &b = *rbs[c];
+            double *o = wr + (c ? WR_C1 : WR_C0);
+            o[0] = b.rhog;
+#pragma unroll
+            for (int n = 0; n < 5; n++) o[1 + n] = b.rq[n];
+            o[6] = b.rhoge;
+            o[7] = b.rvx;
+            o[8] = b.rvy;
+            o[9] = b.rvz;
+            o[10] = b.rw;
+            o[11] = b.vx;
+            o[12] = b.vy;
+            o[13] = b.vz;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NWR; i++) A.wr[i * N2 + pw] = wr[i];
+}
+
+// Input cotangents of point (l, k, ij); the coefficient cotangents of the point are
+// added to tun_b.
+NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2, double g_rhog, double g_rhoge,
+                              const double g_rhogq[NQ], double g_tem, double g_qv, double gvx, double gvy, double gvz,
+                              double gw, double gux, double guy, double guz)
+{
+    if (G.rhog) G.rhog[p] = g_rhog;
+    if (G.rhoge) G.rhoge[p] = g_rhoge;
+    if (G.tem) G.tem[p] = g_tem;
+    if (G.rhogvx) G.rhogvx[p] = gvx;
+    if (G.rhogvy) G.rhogvy[p] = gvy;
+    if (G.rhogvz) G.rhogvz[p] = gvz;
+    if (G.rhogw) G.rhogw[p] = gw;
+    if (G.vx) G.vx[p] = gux;
+    if (G.vy) G.vy[p] = guy;
+    if (G.vz) G.vz[p] = guz;
+#pragma unroll
+    for (int n = 0; n < NQ; n++) {
+        if (G.rhogq) G.rhogq[pq + n * K2] = g_rhogq[n];
+        if (G.q) G.q[pq + n * K2] = (n == QV) ? g_qv : 0.0;
+    }
+}
+
+NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
+{
+    const StepArgs &a = A.a;
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    const int64_t p = p2 + (int64_t)k * ijdim;
+    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
+    if (a.ws.region_flag[l] != 0) {
+        // regions that need the general sweep get no gradient (the caller is told)
+        const double z[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const Fields &f = a.f;
+    auto CRV = [&](int slot, int c) {
+        return (c >= KMIN && c <= KMAX) ? A.cr[slot * N2 + p2 + (int64_t)c * ijdim] : 0.0;
+    };
+    auto WRV = [&](int slot, int w) {
+        return (w >= KMIN && w <= KMAX) ? A.wr[slot * N2 + p2 + (int64_t)w * ijdim] : 0.0;
+    };
+    // cell k is cell c (own record), c-1 of cell k+1, c+1 of cell k-1; cell w-1 of
+    // wall k+1, cell w of wall k, cell w+1 of wall k-1
+    const double rg_b = A.dws[0 * N2 + p] + CRV(CR_RG + 1, k) + CRV(CR_RG + 0, k + 1) + CRV(CR_RG + 2, k - 1) +
+                        WRV(WR_M + 0, k + 1) + WRV(WR_C0 + 0, k) + WRV(WR_C1 + 0, k - 1);
+    const double re_b = A.dws[1 * N2 + p] + WRV(WR_C0 + 6, k) + WRV(WR_C1 + 6, k - 1);
+    double rq_b[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) {
+        rq_b[n] = A.dws[(2 + n) * N2 + p];
+        if (n >= 1) rq_b[n] += WRV(WR_C0 + n, k) + WRV(WR_C1 + n, k - 1);
+    }
+    rq_b[QR] += WRV(WR_M + 1, k + 1);
+    rq_b[QS] += WRV(WR_M + 2, k + 1);
+    rq_b[QG] += WRV(WR_M + 3, k + 1);
+    // Vt(k) is Vt(w + d - 3) of wall w = k + 3 - d, d = 0..4
+    double vt_b[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 5; d++) s += WRV(WR_V + t * 5 + d, k + 3 - d);
+        vt_b[t] = s;
+    }
+    const double gvx = A.din[0 * N2 + p] + WRV(WR_C0 + 7, k) + WRV(WR_C1 + 7, k - 1);
+    const double gvy = A.din[1 * N2 + p] + WRV(WR_C0 + 8, k) + WRV(WR_C1 + 8, k - 1);
+    const double gvz = A.din[2 * N2 + p] + WRV(WR_C0 + 9, k) + WRV(WR_C1 + 9, k - 1);
+    const double gw = A.din[3 * N2 + p] + CRV(CR_RW + 0, k) + CRV(CR_RW + 1, k - 1) + WRV(WR_C0 + 10, k) +
+                      WRV(WR_C1 + 10, k - 1);
+    const double gux = WRV(WR_C0 + 11, k) + WRV(WR_C1 + 11, k - 1);
+    const double guy = WRV(WR_C0 + 12, k) + WRV(WR_C1 + 12, k - 1);
+    const double guz = WRV(WR_C0 + 13, k) + WRV(WR_C1 + 13, k - 1);
+
+    double g_rhog, g_rhoge, g_rhogq[NQ], g_tem = 0.0, g_qv = 0.0;
+    if (k >= KMIN && k <= KMAX) {
+        const DevParams &P = *a.P;
+        PointIn<double> x;
+        x.rhog = f.rhog[p];
+#pragma unroll
+        for (int n = 0; n < NQ; n++) x.rhogq[n] = f.rhogq[pq + n * K2];
+        x.tem = f.tem[p];
+        x.qv = f.q[pq];
+        x.gsgam2 = f.gsgam2[p];
+        double tun[NCOEF];
+        tunables(P, tun);
+        PointOutB yb;
+        yb.rhog = rg_b;
+        yb.rhoge = re_b;
+#pragma unroll
+        for (int n = 0; n < NQ; n++) yb.rhogq[n] = rq_b[n];
+#pragma unroll
+        for (int t = 0; t < 3; t++) yb.vt[t] = vt_b[t];
+        PointInB xb;
+        xb.rhog = xb.tem = xb.qv = 0.0;
+#pragma unroll
+        for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
+        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b);
+        g_rhog = xb.rhog;
+        g_rhoge = 0.0;   // the interior rhoge is recomputed from tem
+#pragma unroll
+        for (int n = 0; n < NQ; n++) g_rhogq[n] = xb.rhogq[n];
+        g_tem = xb.tem;
+        g_qv = xb.qv;
+    } else {
+        // halo levels: the pre-sedimentation state is the input itself
+        g_rhog = rg_b;
+        g_rhoge = re_b;
+#pragma unroll
+        for (int n = 0; n < NQ; n++) g_rhogq[n] = rq_b[n];
+    }
+    store_point_grad(A.grad, p, pq, K2, g_rhog, g_rhoge, g_rhogq, g_tem, g_qv, gvx, gvy, gvz, gw, gux, guy, guz);
+}
+
+}  // namespace nsw6

@@ -17,10 +17,26 @@ using namespace nsw6;
 namespace {
 
 constexpr int N_EVENTS = NSW6_STEP_EVENTS;
-constexpr int N_TIMERS = 5;   // step, rates, extremes+bounds, sweep, adjust
+// step, rates, extremes+bounds, fluxes, finish; adjoint call, its forward part, its
+// reverse passes
+constexpr int N_TIMERS = 8;
 
 struct PendingTiming {
     hipEvent_t ev[N_EVENTS];
+};
+struct PendingAdjTiming {
+    hipEvent_t ev[NSW6_ADJ_EVENTS];
+};
+
+// Scratch of the reverse mode: the records of the reverse passes, the per-block
+// coefficient partials and the diagnostics the recomputed forward part writes (kept
+// away from the caller's arrays).  Allocated on the first nsw6_mp_driver_adj call.
+struct AdjScratch {
+    double *rec = nullptr;    // [ADJ_RECORD_SLOTS][max_columns * KALL]
+    double *diag = nullptr;   // re_liquid, re_cld, GPREC [K2]; 13 rows of [columns]
+    double *part = nullptr;   // [blocks][NSW6_NCOEF]
+    int part_blocks = 0;
+    double *coef = nullptr;   // [NSW6_NCOEF] sink when the caller passes none
 };
 
 }  // namespace
@@ -38,20 +54,25 @@ struct nsw6_ctx {
     int last_lall = 0;
     bool timing = false;
     std::vector<PendingTiming> pending;
-    double t_sum[N_TIMERS] = {0, 0, 0, 0, 0};
+    double t_sum[N_TIMERS] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t t_count = 0;
+    AdjScratch adj;
+    bool last_adjoint = false;
+    std::vector<PendingAdjTiming> pending_adj;
+    int64_t t_adj_count = 0;
 };
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? NSW6_OK : NSW6_HIP_ERROR; }
 
-
 static void free_ws(nsw6_ctx *c)
 {
     Workspace &w = c->ws;
-    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.flux, w.part, w.bounds, w.region_flag, w.report};
+    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.flux, w.part, w.bounds, w.region_flag, w.report,
+                    c->adj.rec, c->adj.diag, c->adj.part, c->adj.coef};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     memset(&w, 0, sizeof w);
+    c->adj = AdjScratch();
 }
 
 // Per-region buffers: kcell bounds, region flags and the per-block partial extremes
@@ -74,6 +95,48 @@ static int ensure_regions(nsw6_ctx *c, int lall)
         return NSW6_HIP_ERROR;
     }
     c->region_capacity = cap;
+    return NSW6_OK;
+}
+
+static Fields to_fields(const nsw6_fields &s)
+{
+    Fields f;
+    f.rhog = s.rhog; f.rhogvx = s.rhogvx; f.rhogvy = s.rhogvy; f.rhogvz = s.rhogvz;
+    f.rhogw = s.rhogw; f.rhoge = s.rhoge; f.rhogq = s.rhogq;
+    f.vx = s.vx; f.vy = s.vy; f.vz = s.vz; f.w = s.w; f.unccn = s.unccn;
+    f.rho = s.rho; f.tem = s.tem; f.pre = s.pre; f.q = s.q; f.qd = s.qd;
+    f.precip = s.precip; f.ISO1_precip = s.ISO1_precip; f.ISO2_precip = s.ISO2_precip;
+    f.precip_rhoe = s.precip_rhoe; f.precip_lh_heat = s.precip_lh_heat;
+    f.precip_rhophi = s.precip_rhophi; f.precip_rhokin = s.precip_rhokin;
+    f.re_liquid = s.re_liquid; f.re_solid = s.re_solid; f.re_cld = s.re_cld;
+    f.rctop = s.rctop; f.rwtop = s.rwtop; f.tctop = s.tctop;
+    f.frhoge_af = s.frhoge_af; f.frhogqv_af = s.frhogqv_af; f.frhoge_rad = s.frhoge_rad;
+    f.rhogqke = s.rhogqke;
+    f.gsgam2 = s.gsgam2; f.gsgam2h = s.gsgam2h; f.gam2 = s.gam2; f.gam2h = s.gam2h;
+    f.ix = s.ix; f.iy = s.iy; f.iz = s.iz; f.jx = s.jx; f.jy = s.jy; f.jz = s.jz;
+    f.z = s.z; f.zh = s.zh;
+    f.GDCLW = s.GDCLW; f.GDCFRC = s.GDCFRC; f.GPREC = s.GPREC; f.CBMFX = s.CBMFX;
+    return f;
+}
+
+static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, StepArgs &a)
+{
+    a.ws = ctx->ws;
+    a.P = ctx->P_dev;
+    a.grid = ctx->grid_dev;
+    a.lall = lall;
+    a.ijdim = ijdim;
+    a.dt = dt;
+    a.dt2 = pow(dt, 2);
+    a.dt3 = pow(dt, 3);
+    a.dt3_6 = a.dt3 / 6.0;
+}
+
+static int use_device(const nsw6_ctx *ctx)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return NSW6_HIP_ERROR;
+    if (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess) return NSW6_HIP_ERROR;
     return NSW6_OK;
 }
 
@@ -160,39 +223,14 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
                            fl->GPREC};
     for (const double *p : req)
         if (!p) return NSW6_BAD_ARG;
-    int cur = -1;
-    if (hipGetDevice(&cur) != hipSuccess) return NSW6_HIP_ERROR;
-    if (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess) return NSW6_HIP_ERROR;
-    int st = ensure_regions(ctx, lall);
+    int st = use_device(ctx);
+    if (st == NSW6_OK) st = ensure_regions(ctx, lall);
     if (st != NSW6_OK) return st;
 
     StepArgs a;
     memset(&a, 0, sizeof a);
-    Fields &f = a.f;
-    f.rhog = fl->rhog; f.rhogvx = fl->rhogvx; f.rhogvy = fl->rhogvy; f.rhogvz = fl->rhogvz;
-    f.rhogw = fl->rhogw; f.rhoge = fl->rhoge; f.rhogq = fl->rhogq;
-    f.vx = fl->vx; f.vy = fl->vy; f.vz = fl->vz; f.w = fl->w; f.unccn = fl->unccn;
-    f.rho = fl->rho; f.tem = fl->tem; f.pre = fl->pre; f.q = fl->q; f.qd = fl->qd;
-    f.precip = fl->precip; f.ISO1_precip = fl->ISO1_precip; f.ISO2_precip = fl->ISO2_precip;
-    f.precip_rhoe = fl->precip_rhoe; f.precip_lh_heat = fl->precip_lh_heat;
-    f.precip_rhophi = fl->precip_rhophi; f.precip_rhokin = fl->precip_rhokin;
-    f.re_liquid = fl->re_liquid; f.re_solid = fl->re_solid; f.re_cld = fl->re_cld;
-    f.rctop = fl->rctop; f.rwtop = fl->rwtop; f.tctop = fl->tctop;
-    f.frhoge_af = fl->frhoge_af; f.frhogqv_af = fl->frhogqv_af; f.frhoge_rad = fl->frhoge_rad;
-    f.rhogqke = fl->rhogqke;
-    f.gsgam2 = fl->gsgam2; f.gsgam2h = fl->gsgam2h; f.gam2 = fl->gam2; f.gam2h = fl->gam2h;
-    f.ix = fl->ix; f.iy = fl->iy; f.iz = fl->iz; f.jx = fl->jx; f.jy = fl->jy; f.jz = fl->jz;
-    f.z = fl->z; f.zh = fl->zh;
-    f.GDCLW = fl->GDCLW; f.GDCFRC = fl->GDCFRC; f.GPREC = fl->GPREC; f.CBMFX = fl->CBMFX;
-    a.ws = ctx->ws;
-    a.P = ctx->P_dev;
-    a.grid = ctx->grid_dev;
-    a.lall = lall;
-    a.ijdim = ijdim;
-    a.dt = dt;
-    a.dt2 = pow(dt, 2);
-    a.dt3 = pow(dt, 3);
-    a.dt3_6 = a.dt3 / 6.0;
+    a.f = to_fields(*fl);
+    fill_step_args(ctx, lall, ijdim, dt, a);
 
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t *ev = nullptr;
@@ -206,6 +244,109 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     if (ctx->timing) ctx->pending.push_back(pt);
     ctx->last_stream = s;
     ctx->last_lall = lall;
+    ctx->last_adjoint = false;
+    return hip_ok(e);
+}
+
+// The reverse mode covers the default scheme only (see include/nsw6.h).
+static bool adjoint_supported(const DevParams &P)
+{
+    return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0;
+}
+
+static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)
+{
+    AdjScratch &s = ctx->adj;
+    const size_t K2 = (size_t)KALL * (size_t)ctx->max_columns;
+    if (!s.rec) {
+        if (hipMalloc(&s.rec, sizeof(double) * K2 * ADJ_RECORD_SLOTS) != hipSuccess ||
+            hipMalloc(&s.diag, sizeof(double) * (3 * K2 + 13 * (size_t)ctx->max_columns)) != hipSuccess ||
+            hipMalloc(&s.coef, sizeof(double) * NSW6_NCOEF) != hipSuccess) {
+            (void)hipFree(s.rec);
+            (void)hipFree(s.diag);
+            (void)hipFree(s.coef);
+            s = AdjScratch();
+            return NSW6_HIP_ERROR;
+        }
+    }
+    const int nb = nsw6_adjoint_blocks(lall, ijdim);
+    if (nb > s.part_blocks) {
+        // a queued adjoint may still read the old partials
+        if (s.part && hipDeviceSynchronize() != hipSuccess) return NSW6_HIP_ERROR;
+        (void)hipFree(s.part);
+        s.part = nullptr;
+        s.part_blocks = 0;
+        if (hipMalloc(&s.part, sizeof(double) * NSW6_NCOEF * (size_t)nb) != hipSuccess) return NSW6_HIP_ERROR;
+        s.part_blocks = nb;
+    }
+    return NSW6_OK;
+}
+
+int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *state,
+                       const nsw6_fields *cot, const double *cot_sflux, const nsw6_fields *grad,
+                       double *grad_coef, double dt, double ct, void *stream)
+{
+    (void)l_region;
+    (void)ct;
+    if (!ctx || !state || !cot || !grad || lall <= 0 || ijdim <= 0) return NSW6_BAD_ARG;
+    if ((int64_t)lall * ijdim > ctx->max_columns) return NSW6_BAD_ARG;
+    if (!(dt > 0.0)) return NSW6_BAD_ARG;
+    const nsw6_fields *fl = state;
+    const double *req[] = {fl->rhog, fl->rhogvx, fl->rhogvy, fl->rhogvz, fl->rhogw, fl->rhoge, fl->rhogq,
+                           fl->vx, fl->vy, fl->vz, fl->tem, fl->q, fl->gsgam2, fl->gsgam2h, fl->gam2,
+                           fl->gam2h, fl->ix, fl->iy, fl->iz, fl->jx, fl->jy, fl->jz, fl->z};
+    for (const double *p : req)
+        if (!p) return NSW6_BAD_ARG;
+    if (!adjoint_supported(ctx->P)) return NSW6_UNSUPPORTED;
+    int st = use_device(ctx);
+    if (st == NSW6_OK) st = ensure_regions(ctx, lall);
+    if (st == NSW6_OK) st = ensure_adj_scratch(ctx, lall, ijdim);
+    if (st != NSW6_OK) return st;
+
+    AdjArgs A;
+    memset(&A, 0, sizeof A);
+    StepArgs &a = A.a;
+    a.f = to_fields(*state);
+    // diagnostics of the recomputed forward part go to scratch
+    const size_t K2 = (size_t)KALL * (size_t)lall * (size_t)ijdim, C = (size_t)lall * (size_t)ijdim;
+    double *d = ctx->adj.diag;
+    a.f.re_liquid = d;
+    a.f.re_cld = d + K2;
+    a.f.GPREC = d + 2 * K2;
+    double *dc = d + 3 * K2;
+    a.f.rctop = dc;
+    a.f.tctop = dc + C;
+    a.f.rwtop = dc + 2 * C;
+    a.f.precip = dc + 3 * C;
+    a.f.ISO1_precip = dc + 5 * C;
+    a.f.ISO2_precip = dc + 7 * C;
+    a.f.precip_rhoe = dc + 9 * C;
+    a.f.precip_lh_heat = dc + 10 * C;
+    a.f.precip_rhophi = dc + 11 * C;
+    a.f.precip_rhokin = dc + 12 * C;
+    fill_step_args(ctx, lall, ijdim, dt, a);
+    A.cot = to_fields(*cot);
+    A.cot_sflux = cot_sflux;
+    A.grad = to_fields(*grad);
+    // records [slot][l][k][ij] of the current batch shape
+    A.cr = ctx->adj.rec;
+    A.wr = A.cr + (size_t)NCR * K2;
+    A.dws = A.wr + (size_t)NWR * K2;
+    A.din = A.dws + (size_t)NDW * K2;
+
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t *ev = nullptr;
+    PendingAdjTiming pt;
+    if (ctx->timing) {
+        for (int i = 0; i < NSW6_ADJ_EVENTS; i++)
+            if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
+        ev = pt.ev;
+    }
+    hipError_t e = nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev);
+    if (ctx->timing) ctx->pending_adj.push_back(pt);
+    ctx->last_stream = s;
+    ctx->last_lall = lall;
+    ctx->last_adjoint = true;
     return hip_ok(e);
 }
 
@@ -260,7 +401,10 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *r)
         }
         for (int v : flags) r->n_general_regions += (v != 0);
     }
-    r->status = r->n_nan ? NSW6_NAN_DETECTED : (r->n_not_converged ? NSW6_NOT_CONVERGED : NSW6_OK);
+    if (ctx->last_adjoint)
+        r->status = r->n_general_regions ? NSW6_UNSUPPORTED : NSW6_OK;
+    else
+        r->status = r->n_nan ? NSW6_NAN_DETECTED : (r->n_not_converged ? NSW6_NOT_CONVERGED : NSW6_OK);
     return r->status;
 }
 
@@ -278,6 +422,18 @@ static int drain_timing(nsw6_ctx *ctx)
         for (int i = 0; i < N_EVENTS; i++) (void)hipEventDestroy(pt.ev[i]);
     }
     ctx->pending.clear();
+    for (PendingAdjTiming &pt : ctx->pending_adj) {
+        if (hipEventSynchronize(pt.ev[NSW6_ADJ_EVENTS - 1]) != hipSuccess) return NSW6_HIP_ERROR;
+        const int from[3] = {0, 0, 1}, to[3] = {2, 1, 2};
+        for (int i = 0; i < 3; i++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, pt.ev[from[i]], pt.ev[to[i]]);
+            ctx->t_sum[5 + i] += ms;
+        }
+        ctx->t_adj_count++;
+        for (int i = 0; i < NSW6_ADJ_EVENTS; i++) (void)hipEventDestroy(pt.ev[i]);
+    }
+    ctx->pending_adj.clear();
     return NSW6_OK;
 }
 
@@ -293,8 +449,9 @@ int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count)
     if (!ctx || which < 0 || which >= N_TIMERS || !avg_ms) return NSW6_BAD_ARG;
     int st = drain_timing(ctx);
     if (st != NSW6_OK) return st;
-    *avg_ms = ctx->t_count ? ctx->t_sum[which] / (double)ctx->t_count : 0.0;
-    if (count) *count = ctx->t_count;
+    const int64_t n = (which >= 5) ? ctx->t_adj_count : ctx->t_count;
+    *avg_ms = n ? ctx->t_sum[which] / (double)n : 0.0;
+    if (count) *count = n;
     return NSW6_OK;
 }
 
@@ -304,6 +461,7 @@ int nsw6_timing_reset(nsw6_ctx *ctx)
     int st = drain_timing(ctx);
     for (int i = 0; i < N_TIMERS; i++) ctx->t_sum[i] = 0.0;
     ctx->t_count = 0;
+    ctx->t_adj_count = 0;
     return st;
 }
 

@@ -1,0 +1,207 @@
+// nsw6_columns.h -- column access of the single-cell transport, shared by the HIP
+// kernels (forward k_flux / k_finish and the reverse sweep) and the host check of the
+// reverse sweep: gathers of one cell, one wall window or one cell's divergence inputs
+// from the step workspace, and the per-wall flux computation itself.
+//
+// In a single-cell region (every departure cell in {k, k+1}) each flux of wall w
+// depends only on cells w-1 (rhog, q), w, w+1 and the fall speeds Vt(w-3..w+1)
+// (src/mod_vadv1d.cpp:393-450 with the kcell bounds of :317-352), so one thread per
+// wall computes it without a column sweep.
+#pragma once
+
+#include <stdint.h>
+
+#include "nsw6_internal.h"
+#include "nsw6_step.h"
+
+namespace nsw6 {
+
+// Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used).
+NSW6_HD void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
+{
+#pragma unroll
+    for (int d = -3; d <= 1; d++) V[d + 3] = (k + d >= 0) ? vt_col[(int64_t)(k + d) * ijdim] : 0.0;
+}
+
+// Pre-sedimentation cell j of the column at p2 = l*K2 + ij, p3 = l*NQ*K2 + ij.
+NSW6_HD void load_cell(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int j, CellIn<double> &r)
+{
+    const Fields &f = a.f;
+    const int64_t o = p2 + (int64_t)j * ijdim;
+    const int64_t oq = p3 + (int64_t)j * ijdim;
+    r.rhog = a.ws.rhog[o];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * K2];
+    r.rhoge = a.ws.rhoge[o];
+    r.gs = f.gsgam2[o];
+    r.gam2 = f.gam2[o];
+    r.gsh = f.gsgam2h[o];
+    r.gam2h = f.gam2h[o];
+    r.rvx = f.rhogvx[o];
+    r.rvy = f.rhogvy[o];
+    r.rvz = f.rhogvz[o];
+    r.rw = f.rhogw[o];
+    r.vx = f.vx[o];
+    r.vy = f.vy[o];
+    r.vz = f.vz[o];
+    r.z = f.z[o];
+}
+
+NSW6_HD Unit load_unit(const Fields &f, int64_t pc)
+{
+    Unit u;
+    u.ix = f.ix[pc];
+    u.iy = f.iy[pc];
+    u.iz = f.iz[pc];
+    u.jx = f.jx[pc];
+    u.jy = f.jy[pc];
+    u.jz = f.jz[pc];
+    return u;
+}
+
+// Cell w-1 enters the walls of cell w only through rhog, q (QR, QS, QG) and gsgam2.
+struct BelowCell {
+    double rhog, q[3], gs;
+};
+NSW6_HD BelowCell load_below(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int w)
+{
+    const int64_t om = p2 + (int64_t)(w - 1) * ijdim, oqm = p3 + (int64_t)(w - 1) * ijdim;
+    BelowCell b;
+    b.rhog = a.ws.rhog[om];
+    b.gs = a.f.gsgam2[om];
+    b.q[0] = a.ws.rhogq[oqm + QR * K2] / b.rhog;
+    b.q[1] = a.ws.rhogq[oqm + QS * K2] / b.rhog;
+    b.q[2] = a.ws.rhogq[oqm + QG * K2] / b.rhog;
+    return b;
+}
+
+// Everything the flux of wall w (1..94) of column (l, ij) is computed from.
+struct WallState {
+    Unit u;
+    BelowCell m;
+    CellIn<double> r0, r1;
+    Cell<double> cur, nxt;
+    double V[3][5];
+    double z0[3], zh0[3];
+    WallGeom geo;
+};
+NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallState &s)
+{
+    const int ijdim = a.ijdim;
+    const GridDev &g = *a.grid;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij, pv = (int64_t)l * 3 * K2 + ij;
+    s.u = load_unit(a.f, (int64_t)l * ijdim + ij);
+    s.m = load_below(a, p2, p3, K2, ijdim, w);
+    load_cell(a, p2, p3, K2, ijdim, w, s.r0);
+    derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
+    load_cell(a, p2, p3, K2, ijdim, w + 1, s.r1);
+    derive_cell(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt);
+    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    int kcmax[3], khmax[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
+        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
+    }
+    wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
+}
+
+// Flux workspace of wall w (FLUX_SLOTS), frain = GPREC (src/mod_precip_transport.cpp:
+// 916-927) and, at wall kmin, the surface fluxes.  Walls 0 and 95 carry no flux.
+NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
+{
+    const int ijdim = a.ijdim;
+    const Fields &f = a.f;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
+    const int64_t ow = p2 + (int64_t)w * ijdim;
+
+    fx[FX_RW * K2] = f.rhogw[ow];
+    if (w == 0 || w == KALL - 1) {
+#pragma unroll
+        for (int i = 0; i < FX_RW; i++) fx[i * K2] = 0.0;
+        f.GPREC[ow] = (0.0 + 0.0) * 1.0;
+        return;
+    }
+    WallState s;
+    wall_state(a, l, w, ij, s);
+    WallFlux<double> F;
+    double ft[3][4];
+    wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);
+
+#pragma unroll
+    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * K2] = F.fq[t];
+#pragma unroll
+    for (int i = 0; i < 5; i++) fx[(FX_E + i) * K2] = F.fs[i];
+    fx[FX_KINV * K2] = F.fh[0];
+    fx[FX_W * K2] = F.fh[1];
+    f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
+    if (w == KMIN) {
+        Surface<double> sf;
+        surface_fluxes(ft, a.dt, sf);
+        const int64_t pc = (int64_t)l * ijdim + ij;
+        const int64_t pp = (int64_t)l * 2 * ijdim + ij;
+        f.precip[pp] = sf.precip0;
+        f.precip[pp + ijdim] = sf.precip1;
+        f.ISO1_precip[pp] = 0.0;
+        f.ISO1_precip[pp + ijdim] = 0.0;
+        f.ISO2_precip[pp] = 0.0;
+        f.ISO2_precip[pp + ijdim] = 0.0;
+        f.precip_rhoe[pc] = sf.prhoe;
+        f.precip_lh_heat[pc] = sf.lh;
+        f.precip_rhophi[pc] = sf.prhophi;
+        f.precip_rhokin[pc] = sf.prhokin;
+    }
+}
+
+// Inputs of sediment() for interior cell c of column (l, ij) from the flux workspace.
+NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedIn<double> &x)
+{
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const Fields &f = a.f;
+    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
+    auto FXV = [&](int slot, int wall) { return fx[(int64_t)slot * K2 + (int64_t)wall * ijdim]; };
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        x.fq[0][t] = FXV(FX_Q + t, c - 1);
+        x.fq[1][t] = FXV(FX_Q + t, c);
+        x.fq[2][t] = FXV(FX_Q + t, c + 1);
+        x.fq[3][t] = (c + 1 <= KMAX) ? FXV(FX_Q + t, c + 2) : 0.0;   // wall 96 does not exist
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        x.fs[0][i] = FXV(FX_E + i, c);
+        x.fs[1][i] = FXV(FX_E + i, c + 1);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        x.fh[m][0] = FXV(FX_KINV, c - 1 + m);
+        x.fh[m][1] = FXV(FX_W, c - 1 + m);
+    }
+    x.rw_c = FXV(FX_RW, c);
+    x.rw_p = FXV(FX_RW, c + 1);
+    const int64_t o = (int64_t)l * K2 + (int64_t)c * ijdim + ij;
+    const int64_t oq = (int64_t)l * NQ * K2 + (int64_t)c * ijdim + ij;
+    x.rg_m = a.ws.rhog[o - ijdim];
+    x.rg_c = a.ws.rhog[o];
+    x.rg_p = a.ws.rhog[o + ijdim];
+    x.rq[0] = a.ws.rhogq[oq + QR * K2];
+    x.rq[1] = a.ws.rhogq[oq + QS * K2];
+    x.rq[2] = a.ws.rhogq[oq + QG * K2];
+    x.re0 = a.ws.rhoge[o];
+    x.rvx = f.rhogvx[o];
+    x.rvy = f.rhogvy[o];
+    x.rvz = f.rhogvz[o];
+    x.gs_m = f.gsgam2[o - ijdim];
+    x.gs_c = f.gsgam2[o];
+    x.gs_p = f.gsgam2[o + ijdim];
+    x.gsh_c = f.gsgam2h[o];
+    x.gsh_p = f.gsgam2h[o + ijdim];
+    x.z = f.z[o];
+}
+
+}  // namespace nsw6

@@ -152,6 +152,7 @@ const char *nsw6_status_string(int status)
     case NSW6_BAD_ARG: return "bad argument";
     case NSW6_HIP_ERROR: return "HIP runtime error";
     case NSW6_NO_DEVICE: return "no HIP device";
+    case NSW6_UNSUPPORTED: return "not supported by the reverse mode (non-default scheme or a region needing the general sedimentation)";
     default: return "unknown status";
     }
 }

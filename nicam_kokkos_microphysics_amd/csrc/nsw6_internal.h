@@ -109,6 +109,26 @@ struct StepArgs {
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 
+// Reverse sweep (nsw6_adjoint_sweep.h): record arrays [slot][l][k][ij] written by one
+// pass and gathered by the next.
+// cell record slots
+constexpr int CR_FQ = 0, CR_FS = 12, CR_FH = 22, CR_RG = 28, CR_RW = 31, NCR = 33;
+// wall record slots: cell w-1 (rhog, QR, QS, QG); cells w and w+1 in CellInB order
+// (rhog, rq[5], rhoge, rvx, rvy, rvz, rw, vx, vy, vz); Vt(w-3..w+1) of QR, QS, QG
+constexpr int WR_M = 0, WR_C0 = 4, WR_C1 = 18, WR_V = 32, NWR = 47;
+// own-cell cotangents: pre-sedimentation state (rhog, rhoge, rhogq[6]) and the input
+// momentum (rhogvx, rhogvy, rhogvz, rhogw)
+constexpr int NDW = 8, NDI = 4;
+constexpr int ADJ_RECORD_SLOTS = NCR + NWR + NDW + NDI;
+
+struct AdjArgs {
+    StepArgs a;                 // forward step state (workspace filled up to the fluxes)
+    Fields cot;                 // output cotangents (NULL pointer = zero)
+    const double *cot_sflux;    // [l][3][ij] or NULL
+    Fields grad;                // input cotangents written (NULL pointer = not wanted)
+    double *cr, *wr, *dws, *din;   // records [slot][l][k][ij]
+};
+
 }  // namespace nsw6
 
 // events recorded by nsw6_launch_step: before k_rates, after k_rates, after
@@ -117,3 +137,10 @@ constexpr int NSW6_STEP_EVENTS = 5;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev);
+
+// events recorded by nsw6_launch_adjoint: before the forward part, after it, after the
+// reverse passes
+constexpr int NSW6_ADJ_EVENTS = 3;
+extern "C" int nsw6_adjoint_blocks(int lall, int ijdim);
+extern "C" hipError_t nsw6_launch_adjoint(const nsw6::AdjArgs *adj, double *coef_part, double *grad_coef,
+                                          hipStream_t stream, hipEvent_t *ev);

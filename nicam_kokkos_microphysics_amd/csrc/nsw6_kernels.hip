@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include "nsw6_device.h"
+#include "nsw6_columns.h"
 #include "nsw6_internal.h"
 #include "nsw6_step.h"
 
@@ -174,13 +175,6 @@ __device__ __forceinline__ double wave_min(double v)
 #pragma unroll
     for (int o = N / 2; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, N));
     return v;
-}
-
-// Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used).
-__device__ __forceinline__ void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
-{
-#pragma unroll
-    for (int d = -3; d <= 1; d++) V[d + 3] = (k + d >= 0) ? vt_col[(int64_t)(k + d) * ijdim] : 0.0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -463,175 +457,18 @@ __device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, doubl
 // Single-cell regions: every departure cell lies in {k, k+1} (checked by k_bounds),
 // so each flux depends only on cells w, w+1 and the fall speeds Vt(w-3..w+1), and the
 // transport splits into two point-wise kernels: k_flux (one thread per wall) and
-// k_finish (one thread per cell).  The arithmetic is in nsw6_step.h.
-// ---------------------------------------------------------------------------------
-__device__ __forceinline__ void load_cell(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int j,
-                                          CellIn<double> &r)
-{
-    const Fields &f = a.f;
-    const int64_t o = p2 + (int64_t)j * ijdim;
-    const int64_t oq = p3 + (int64_t)j * ijdim;
-    r.rhog = a.ws.rhog[o];
-#pragma unroll
-    for (int n = 1; n <= 5; n++) r.rq[n - 1] = a.ws.rhogq[oq + n * K2];
-    r.rhoge = a.ws.rhoge[o];
-    r.gs = f.gsgam2[o];
-    r.gam2 = f.gam2[o];
-    r.gsh = f.gsgam2h[o];
-    r.gam2h = f.gam2h[o];
-    r.rvx = f.rhogvx[o];
-    r.rvy = f.rhogvy[o];
-    r.rvz = f.rhogvz[o];
-    r.rw = f.rhogw[o];
-    r.vx = f.vx[o];
-    r.vy = f.vy[o];
-    r.vz = f.vz[o];
-    r.z = f.z[o];
-}
-__device__ __forceinline__ Unit load_unit(const Fields &f, int64_t pc)
-{
-    Unit u;
-    u.ix = f.ix[pc];
-    u.iy = f.iy[pc];
-    u.iz = f.iz[pc];
-    u.jx = f.jx[pc];
-    u.jy = f.jy[pc];
-    u.jz = f.jz[pc];
-    return u;
-}
-
-// ---------------------------------------------------------------------------------
-// k_flux: one thread per (wall w, column) of the single-cell regions.  Writes the flux
-// workspace of wall w (FLUX_SLOTS), frain = GPREC (src/mod_precip_transport.cpp:
-// 916-927) and, at wall kmin, the surface fluxes.  Walls 0 and 95 carry no flux.
+// k_finish (one thread per cell).  The arithmetic is in nsw6_step.h, the column
+// gathers and the per-wall flux (flux_wall) in nsw6_columns.h.
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
     if (!lb.valid) return;
-    const int l = lb.l, w = lb.k;
-    if (a.ws.region_flag[l] != 0) return;
+    if (a.ws.region_flag[lb.l] != 0) return;
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
-    const GridDev &g = *a.grid;
-    const Fields &f = a.f;
-    const int64_t K2 = (int64_t)KALL * ijdim;
-    const int64_t p2 = (int64_t)l * K2 + ij;
-    const int64_t p3 = (int64_t)l * NQ * K2 + ij;
-    const int64_t pv = (int64_t)l * 3 * K2 + ij;
-    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
-    const int64_t ow = p2 + (int64_t)w * ijdim;
-
-    fx[FX_RW * K2] = f.rhogw[ow];
-    if (w == 0 || w == KALL - 1) {
-#pragma unroll
-        for (int i = 0; i < FX_RW; i++) fx[i * K2] = 0.0;
-        f.GPREC[ow] = (0.0 + 0.0) * 1.0;
-        return;
-    }
-    const int64_t pc = (int64_t)l * ijdim + ij;
-    const Unit u = load_unit(f, pc);
-    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
-
-    // cell w-1 enters the walls of cell w only through rhog, q and gsgam2
-    const int64_t om = p2 + (int64_t)(w - 1) * ijdim, oqm = p3 + (int64_t)(w - 1) * ijdim;
-    const double rg_m = a.ws.rhog[om], gs_m = f.gsgam2[om];
-    const double q_m[3] = {a.ws.rhogq[oqm + QR * K2] / rg_m, a.ws.rhogq[oqm + QS * K2] / rg_m,
-                           a.ws.rhogq[oqm + QG * K2] / rg_m};
-    Cell<double> cur, nxt;
-    {
-        CellIn<double> r;
-        load_cell(a, p2, p3, K2, ijdim, w, r);
-        derive_cell(g, w, r, rg_m, q_m, gs_m, u, cur);
-        load_cell(a, p2, p3, K2, ijdim, w + 1, r);
-        derive_cell(g, w + 1, r, cur.rhog, cur.q, cur.gs, u, nxt);
-    }
-    double V[3][5];
-    int kcmax[3], khmax[3];
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
-        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
-        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
-    }
-    double z0[3], zh0[3];
-    WallGeom geo;
-    wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmax, khmax, z0, zh0, geo);
-    WallFlux<double> F;
-    double ft[3][4];
-    wall_flux(g, w, cur, nxt, z0, zh0, geo, F, ft);
-
-#pragma unroll
-    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * K2] = F.fq[t];
-#pragma unroll
-    for (int i = 0; i < 5; i++) fx[(FX_E + i) * K2] = F.fs[i];
-    fx[FX_KINV * K2] = F.fh[0];
-    fx[FX_W * K2] = F.fh[1];
-    f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
-    if (w == KMIN) {
-        Surface<double> sf;
-        surface_fluxes(ft, a.dt, sf);
-        const int64_t pp = (int64_t)l * 2 * ijdim + ij;
-        f.precip[pp] = sf.precip0;
-        f.precip[pp + ijdim] = sf.precip1;
-        f.ISO1_precip[pp] = 0.0;
-        f.ISO1_precip[pp + ijdim] = 0.0;
-        f.ISO2_precip[pp] = 0.0;
-        f.ISO2_precip[pp + ijdim] = 0.0;
-        f.precip_rhoe[pc] = sf.prhoe;
-        f.precip_lh_heat[pc] = sf.lh;
-        f.precip_rhophi[pc] = sf.prhophi;
-        f.precip_rhokin[pc] = sf.prhokin;
-    }
-}
-
-// Inputs of sediment() for interior cell c of column (l, ij) from the k_flux workspace.
-__device__ __forceinline__ void load_sed(const StepArgs &a, int l, int ij, int c, SedIn<double> &x)
-{
-    const int ijdim = a.ijdim;
-    const int64_t K2 = (int64_t)KALL * ijdim;
-    const Fields &f = a.f;
-    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
-#define FXV(slot, wall) fx[(int64_t)(slot) * K2 + (int64_t)(wall) * ijdim]
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        x.fq[0][t] = FXV(FX_Q + t, c - 1);
-        x.fq[1][t] = FXV(FX_Q + t, c);
-        x.fq[2][t] = FXV(FX_Q + t, c + 1);
-        x.fq[3][t] = (c + 1 <= KMAX) ? FXV(FX_Q + t, c + 2) : 0.0;   // wall 96 does not exist
-    }
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        x.fs[0][i] = FXV(FX_E + i, c);
-        x.fs[1][i] = FXV(FX_E + i, c + 1);
-    }
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        x.fh[m][0] = FXV(FX_KINV, c - 1 + m);
-        x.fh[m][1] = FXV(FX_W, c - 1 + m);
-    }
-    x.rw_c = FXV(FX_RW, c);
-    x.rw_p = FXV(FX_RW, c + 1);
-#undef FXV
-    const int64_t o = (int64_t)l * K2 + (int64_t)c * ijdim + ij;
-    const int64_t oq = (int64_t)l * NQ * K2 + (int64_t)c * ijdim + ij;
-    x.rg_m = a.ws.rhog[o - ijdim];
-    x.rg_c = a.ws.rhog[o];
-    x.rg_p = a.ws.rhog[o + ijdim];
-    x.rq[0] = a.ws.rhogq[oq + QR * K2];
-    x.rq[1] = a.ws.rhogq[oq + QS * K2];
-    x.rq[2] = a.ws.rhogq[oq + QG * K2];
-    x.re0 = a.ws.rhoge[o];
-    x.rvx = f.rhogvx[o];
-    x.rvy = f.rhogvy[o];
-    x.rvz = f.rhogvz[o];
-    x.gs_m = f.gsgam2[o - ijdim];
-    x.gs_c = f.gsgam2[o];
-    x.gs_p = f.gsgam2[o + ijdim];
-    x.gsh_c = f.gsgam2h[o];
-    x.gsh_p = f.gsgam2h[o + ijdim];
-    x.z = f.z[o];
+    flux_wall(a, lb.l, lb.k, ij);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1109,5 +946,20 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     if (ev) (void)hipEventRecord(ev[3], stream);
     hipLaunchKernelGGL(k_finish, grid_l, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
+    return hipGetLastError();
+}
+
+// The FP64 step up to the wall fluxes of the single-cell regions (the forward part
+// that the reverse sweep recomputes); general regions keep region_flag = 1.
+extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStream_t stream)
+{
+    const StepArgs &a = *args;
+    const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
+    const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
+    const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
+    hipLaunchKernelGGL(k_rates<double>, grid_p, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
+    hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
+    hipLaunchKernelGGL(k_flux, grid_l, dim3(256), 0, stream, a);
     return hipGetLastError();
 }

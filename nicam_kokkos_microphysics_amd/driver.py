@@ -19,7 +19,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
-from .fields import field_ptrs, shape_of
+from .fields import ADJ_INPUTS, field_ptrs, shape_of
 
 
 class NSW6Error(RuntimeError):
@@ -112,6 +112,37 @@ class NSW6:
         args = dict(locals())
         state = {n: args[n] for n in N.FIELD_NAMES}
         self.step(state, dt, ct, l_region=l_region, stream=stream, check=check)
+
+    def adjoint(self, state: dict, cot: dict, dt: float, *, cot_sflux=None, grad: dict | None = None,
+                grad_coef=None, ct: float = 0.0, l_region: int = 0, stream=None, check: bool = True):
+        """Reverse mode of one FP64 step (nsw6_mp_driver_adj): `state` holds the step's
+        input (device tensors, not modified), `cot` the cotangents of its outputs (keys
+        of fields.ADJ_OUTPUTS; missing = 0), `cot_sflux` [l][3][ij] those of the
+        surface mass fluxes of QR, QS, QG.  Returns (grad, grad_coef): the cotangents of
+        the inputs (keys of fields.ADJ_INPUTS) and of the 8 tunable coefficients."""
+        import torch
+        rhog = state["rhog"]
+        lall, ijdim = int(rhog.shape[0]), int(rhog.shape[-1])
+        for d in (state, cot):
+            for k, v in d.items():
+                if v is not None and tuple(v.shape) != shape_of(k, lall, ijdim):
+                    raise ValueError(f"{k}: shape {tuple(v.shape)} != {shape_of(k, lall, ijdim)}")
+        if cot_sflux is not None and tuple(cot_sflux.shape) != (lall, 3, ijdim):
+            raise ValueError(f"cot_sflux: shape {tuple(cot_sflux.shape)} != {(lall, 3, ijdim)}")
+        if grad is None:
+            grad = {k: torch.empty_like(state[k]) for k in ADJ_INPUTS}
+        if grad_coef is None:
+            grad_coef = torch.empty(N.NCOEF, dtype=torch.float64, device=rhog.device)
+        fs, fc, fg = field_ptrs(state), field_ptrs(cot), field_ptrs(grad)
+        cs = None if cot_sflux is None else ctypes.c_void_p(cot_sflux.data_ptr())
+        rc = self._L.nsw6_mp_driver_adj(self._ctx, l_region, lall, ijdim, ctypes.byref(fs), ctypes.byref(fc), cs,
+                                        ctypes.byref(fg), ctypes.c_void_p(grad_coef.data_ptr()), float(dt),
+                                        float(ct), _stream_handle(stream))
+        if rc != 0:
+            raise NSW6Error(rc)
+        if check:
+            self.check()
+        return grad, grad_coef
 
     def check(self) -> N.Report:
         rep = N.Report()

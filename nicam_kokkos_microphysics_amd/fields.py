@@ -32,6 +32,10 @@ OUTPUTS = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "rho
            "precip", "ISO1_precip", "ISO2_precip", "precip_rhoe", "precip_lh_heat", "precip_rhophi",
            "precip_rhokin", "re_liquid", "re_solid", "re_cld", "rctop", "rwtop", "tctop", "GDCLW", "GDCFRC",
            "GPREC")
+# Reverse mode: outputs that take a cotangent, inputs that receive one.
+ADJ_OUTPUTS = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "rho", "tem", "pre", "q", "qd",
+               "precip", "precip_rhoe", "precip_lh_heat", "precip_rhophi", "precip_rhokin", "GPREC")
+ADJ_INPUTS = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "vx", "vy", "vz", "tem", "q")
 # The reference's check table (src/main.cpp:339-350): rhogq per tracer, not q.
 CHECK_FIELDS = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge") + tuple(f"rhogq:{t}" for t in TRACERS)
 

@@ -414,6 +414,43 @@ int grad_step_fp64(const double *gz, const double *gzh, const nsw6_params *param
     return fails ? 1 : 0;
 }
 
+// Surface mass fluxes sflux_t = -f_t(kmin)/dt of QR, QS, QG ([l][3][ij]) of one FP64
+// host step from the state `f` (not modified).
+int grad_sflux_fp64(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
+                    const nsw6_fields *f, double dt, double *sflux)
+{
+    DevParams P;
+    to_dev_params(*params, P);
+    GridDev g;
+    make_grid_dev(gz, gzh, g);
+    const int64_t N2 = (int64_t)lall * KALL * ijdim;
+    HostState<double> h;
+    h.lall = lall;
+    h.ijdim = ijdim;
+    auto cp = [](std::vector<double> &v, const double *src, int64_t n) { v.assign(src, src + n); };
+    cp(h.rhog, f->rhog, N2);
+    cp(h.rhogvx, f->rhogvx, N2);
+    cp(h.rhogvy, f->rhogvy, N2);
+    cp(h.rhogvz, f->rhogvz, N2);
+    cp(h.rhogw, f->rhogw, N2);
+    cp(h.rhoge, f->rhoge, N2);
+    cp(h.rhogq, f->rhogq, N2 * NQ);
+    cp(h.vx, f->vx, N2);
+    cp(h.vy, f->vy, N2);
+    cp(h.vz, f->vz, N2);
+    cp(h.tem, f->tem, N2);
+    cp(h.q, f->q, N2 * NQ);
+    bind_metrics(h, f);
+    alloc_outputs(h);
+    double tun[NCOEF];
+    tunables(P, tun);
+    int fails = 0;
+    const int st = host_step(P, g, dt, tun, h, &fails);
+    if (st != 0) return st;
+    memcpy(sflux, h.sflux.data(), h.sflux.size() * sizeof(double));
+    return fails ? 1 : 0;
+}
+
 // The check loss L (header comment) of the FP64 host step, for finite differences.
 int grad_loss_fp64(const double *gz, const double *gzh, const nsw6_params *params, const double *coef, int lall,
                    int ijdim, const nsw6_fields *f, double dt, double *loss)
@@ -950,3 +987,263 @@ extern "C" int grad_check_pieces(const double *gz, const double *gzh, const nsw6
             }
     return 0;
 }
+
+// ---------------------------------------------------------------------------------
+// Whole-step reverse mode on the host: the passes of nsw6_adjoint_sweep.h (the same
+// bodies the HIP kernels run) over every point, after the forward part rebuilt in the
+// device workspace layout.  Checked against grad_dual_vjp; the GPU reverse mode is
+// checked against this.
+// ---------------------------------------------------------------------------------
+#include "../nicam_kokkos_microphysics_amd/csrc/nsw6_adjoint_sweep.h"
+
+namespace {
+
+Fields host_fields(const nsw6_fields &s)
+{
+    Fields f;
+    memset(&f, 0, sizeof f);
+    static_assert(sizeof(Fields) == sizeof(nsw6_fields), "field lists differ");
+    memcpy(&f, &s, sizeof f);   // same 50 pointers in the same order
+    return f;
+}
+
+// Dual-number state seeded with the value of `f` and `ndir` tangent directions.
+template <int N>
+void seed_state(HostState<Dual<N>> &h, const nsw6_fields *f, int ndir, const nsw6_fields *dirs)
+{
+    const int64_t N2 = (int64_t)h.lall * KALL * h.ijdim;
+    auto seed = [&](std::vector<Dual<N>> &v, const double *src, size_t off, int64_t n) {
+        v.resize(n);
+        for (int64_t e = 0; e < n; e++) {
+            v[e] = Dual<N>(src[e]);
+            for (int i = 0; i < ndir; i++) {
+                const double *dp = *(const double *const *)((const char *)&dirs[i] + off);
+                v[e].d[i] = dp ? dp[e] : 0.0;
+            }
+        }
+    };
+#define SEED(field, n) seed(h.field, f->field, offsetof(nsw6_fields, field), n)
+    SEED(rhog, N2);
+    SEED(rhogvx, N2);
+    SEED(rhogvy, N2);
+    SEED(rhogvz, N2);
+    SEED(rhogw, N2);
+    SEED(rhoge, N2);
+    SEED(rhogq, N2 * NQ);
+    SEED(vx, N2);
+    SEED(vy, N2);
+    SEED(vz, N2);
+    SEED(tem, N2);
+    SEED(q, N2 * NQ);
+#undef SEED
+}
+
+}  // namespace
+
+extern "C" {
+
+// Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
+// NSW6_OK, NSW6_UNSUPPORTED if a region needs the general transport (its gradient is
+// 0) or for a non-default scheme.
+int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
+                      const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
+                      const nsw6_fields *grad, double *grad_coef, double dt)
+{
+    DevParams P;
+    to_dev_params(*params, P);
+    if (!(P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0)) return NSW6_UNSUPPORTED;
+    GridDev g;
+    make_grid_dev(gz, gzh, g);
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)lall * K2, C = (int64_t)lall * ijdim;
+    std::vector<double> w_rhog(N2), w_rhoge(N2), w_rhogq(N2 * NQ), w_vt(N2 * 3), w_flux(N2 * FLUX_SLOTS, 0.0);
+    std::vector<int> w_bounds((size_t)lall * BOUNDS_PER_REGION, 0), w_flag(lall, 0);
+    std::vector<double> diag(3 * N2 + 13 * C, 0.0), rec((size_t)ADJ_RECORD_SLOTS * N2, 0.0);
+
+    AdjArgs A;
+    memset(&A, 0, sizeof A);
+    StepArgs &a = A.a;
+    a.f = host_fields(*state);
+    a.f.re_liquid = diag.data();
+    a.f.re_cld = diag.data() + N2;
+    a.f.GPREC = diag.data() + 2 * N2;
+    double *dc = diag.data() + 3 * N2;
+    a.f.rctop = dc;
+    a.f.tctop = dc + C;
+    a.f.rwtop = dc + 2 * C;
+    a.f.precip = dc + 3 * C;
+    a.f.ISO1_precip = dc + 5 * C;
+    a.f.ISO2_precip = dc + 7 * C;
+    a.f.precip_rhoe = dc + 9 * C;
+    a.f.precip_lh_heat = dc + 10 * C;
+    a.f.precip_rhophi = dc + 11 * C;
+    a.f.precip_rhokin = dc + 12 * C;
+    a.ws.rhog = w_rhog.data();
+    a.ws.rhoge = w_rhoge.data();
+    a.ws.rhogq = w_rhogq.data();
+    a.ws.vt = w_vt.data();
+    a.ws.flux = w_flux.data();
+    a.ws.bounds = w_bounds.data();
+    a.ws.region_flag = w_flag.data();
+    a.P = &P;
+    a.grid = &g;
+    a.lall = lall;
+    a.ijdim = ijdim;
+    a.dt = dt;
+    a.dt2 = pow(dt, 2);
+    a.dt3 = pow(dt, 3);
+    a.dt3_6 = a.dt3 / 6.0;
+    A.cot = host_fields(*cot);
+    A.cot_sflux = cot_sflux;
+    A.grad = host_fields(*grad);
+    A.cr = rec.data();
+    A.wr = A.cr + (size_t)NCR * N2;
+    A.dws = A.wr + (size_t)NWR * N2;
+    A.din = A.dws + (size_t)NDW * N2;
+
+    double tun[NCOEF];
+    tunables(P, tun);
+    const Fields &f = a.f;
+    // ---- forward part: rates (k_rates without the diagnostics)
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int k = 0; k < KALL; k++)
+            for (int ij = 0; ij < ijdim; ij++) {
+                const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
+                const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
+                const int64_t pv = (int64_t)l * 3 * K2 + (int64_t)k * ijdim + ij;
+                if (k == 0 || k == KALL - 1) {
+                    w_rhog[p] = f.rhog[p];
+                    w_rhoge[p] = f.rhoge[p];
+                    for (int n = 0; n < NQ; n++) w_rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
+                    for (int t = 0; t < 3; t++) w_vt[pv + t * K2] = 0.0;
+                    continue;
+                }
+                PointIn<double> x;
+                x.rhog = f.rhog[p];
+                for (int n = 0; n < NQ; n++) x.rhogq[n] = f.rhogq[pq + n * K2];
+                x.tem = f.tem[p];
+                x.qv = f.q[pq];
+                x.gsgam2 = f.gsgam2[p];
+                PointOut<double> y;
+                point_rates<double, double>(P, dt, x, tun, A1_TAB, A2_TAB, y);
+                w_rhog[p] = y.rhog;
+                w_rhoge[p] = y.rhoge;
+                for (int n = 0; n < NQ; n++) w_rhogq[pq + n * K2] = y.rhogq[n];
+                for (int t = 0; t < 3; t++) w_vt[pv + t * K2] = y.vt[t];
+            }
+    // ---- kcell bounds in the device layout (kcell_max is what the fluxes read)
+    Bounds b;
+    b.kcmax.assign((size_t)lall * 3 * KALL, 0);
+    b.khmax.assign((size_t)lall * 3 * KALL, 0);
+    b.general.assign(lall, 0);
+    for (int l = 0; l < lall; l++) region_bounds(g, dt, a.dt2, a.dt3_6, l, ijdim, w_vt, b);
+    for (int l = 0; l < lall; l++) {
+        w_flag[l] = b.general[l];
+        int *bl = &w_bounds[(size_t)l * BOUNDS_PER_REGION];
+        for (int t = 0; t < 3; t++)
+            for (int k = 0; k < KALL; k++) {
+                bl[bounds_index(t, 0, k, 0)] = k;
+                bl[bounds_index(t, 1, k, 0)] = k;
+                bl[bounds_index(t, 0, k, 1)] = b.kcmax[((size_t)l * 3 + t) * KALL + k];
+                bl[bounds_index(t, 1, k, 1)] = b.khmax[((size_t)l * 3 + t) * KALL + k];
+            }
+    }
+    // ---- wall fluxes
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int w = 0; w < KALL; w++)
+            if (!w_flag[l])
+                for (int ij = 0; ij < ijdim; ij++) flux_wall(a, l, w, ij);
+    // ---- reverse passes
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int k = 0; k < KALL; k++)
+            for (int ij = 0; ij < ijdim; ij++) adj_cell(A, l, k, ij);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int w = 0; w < KALL; w++)
+            for (int ij = 0; ij < ijdim; ij++) adj_wall(A, l, w, ij);
+    std::vector<double> tb((size_t)lall * KALL * NCOEF, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int k = 0; k < KALL; k++) {
+            double *t = &tb[((size_t)l * KALL + k) * NCOEF];
+            for (int ij = 0; ij < ijdim; ij++) adj_point(A, l, k, ij, t);
+        }
+    if (grad_coef)
+        for (int c = 0; c < NCOEF; c++) {
+            double s = 0.0;
+            for (size_t e = 0; e < (size_t)lall * KALL; e++) s += tb[e * NCOEF + c];
+            grad_coef[c] = s;
+        }
+    int ngen = 0;
+    for (int l = 0; l < lall; l++) ngen += w_flag[l];
+    return ngen ? NSW6_UNSUPPORTED : NSW6_OK;
+}
+
+// J = sum <cot, y(x)> + sum cot_sflux sflux over the step's outputs (the output list
+// of nsw6_mp_driver_adj; NULL cotangents = 0) and its directional derivatives along
+// `ndir` <= 8 input directions `dirs` (NULL members = 0) and coefficient directions
+// dcoef [ndir][8], by forward-mode dual numbers.  Returns 0, 1 (Newton failure) or
+// -1 (a region needs the general transport).
+int grad_dual_vjp(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
+                  const nsw6_fields *f, double dt, const nsw6_fields *cot, const double *cot_sflux, int ndir,
+                  const nsw6_fields *dirs, const double *dcoef, double *J, double *dJ)
+{
+    if (ndir < 1 || ndir > GRAD_NDIR) return -2;
+    typedef Dual<GRAD_NDIR> D;
+    DevParams P;
+    to_dev_params(*params, P);
+    GridDev g;
+    make_grid_dev(gz, gzh, g);
+    const int64_t N2 = (int64_t)lall * KALL * ijdim, NC = (int64_t)lall * ijdim;
+    HostState<D> h;
+    h.lall = lall;
+    h.ijdim = ijdim;
+    seed_state(h, f, ndir, dirs);
+    bind_metrics(h, f);
+    alloc_outputs(h);
+    double t0[NCOEF];
+    tunables(P, t0);
+    D tun[NCOEF];
+    for (int c = 0; c < NCOEF; c++) {
+        tun[c] = D(t0[c]);
+        for (int i = 0; i < ndir; i++) tun[c].d[i] = dcoef ? dcoef[i * NCOEF + c] : 0.0;
+    }
+    int fails = 0;
+    const int st = host_step(P, g, dt, tun, h, &fails);
+    if (st != 0) return st;
+    D S(0.0);
+    auto dot = [&](const double *c, const std::vector<D> &y) {
+        if (!c) return;
+        D s(0.0);
+        for (size_t e = 0; e < y.size(); e++) s += c[e] * y[e];
+        S += s;
+    };
+    dot(cot->rhog, h.rhog);
+    dot(cot->rhogvx, h.rhogvx);
+    dot(cot->rhogvy, h.rhogvy);
+    dot(cot->rhogvz, h.rhogvz);
+    dot(cot->rhogw, h.rhogw);
+    dot(cot->rhoge, h.rhoge);
+    dot(cot->rhogq, h.rhogq);
+    dot(cot->rho, h.rho);
+    dot(cot->tem, h.tem);
+    dot(cot->pre, h.pre);
+    dot(cot->q, h.q);
+    dot(cot->qd, h.qd);
+    dot(cot->precip, h.precip);
+    dot(cot->precip_rhoe, h.prhoe);
+    dot(cot->precip_lh_heat, h.lh);
+    dot(cot->precip_rhophi, h.prhophi);
+    dot(cot->precip_rhokin, h.prhokin);
+    dot(cot->GPREC, h.GPREC);
+    dot(cot_sflux, h.sflux);
+    (void)N2;
+    (void)NC;
+    *J = S.v;
+    for (int i = 0; i < ndir; i++) dJ[i] = S.d[i];
+    return fails ? 1 : 0;
+}
+
+}  // extern "C"

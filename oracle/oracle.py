@@ -142,6 +142,13 @@ def grad_lib():
         L.grad_dual_loss.restype = I
         L.grad_check_pieces.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, ctypes.POINTER(N.FieldPtrs), D, P, P]
         L.grad_check_pieces.restype = I
+        FP = ctypes.POINTER(N.FieldPtrs)
+        L.grad_sflux_fp64.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, FP, D, P]
+        L.grad_sflux_fp64.restype = I
+        L.grad_adjoint_host.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, FP, FP, P, FP, P, D]
+        L.grad_adjoint_host.restype = I
+        L.grad_dual_vjp.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, FP, D, FP, P, I, FP, P, P, P]
+        L.grad_dual_vjp.restype = I
         _glib = L
     return _glib
 
@@ -236,3 +243,78 @@ def check_pieces(state: dict, gz, gzh, dt: float, P=None):
     if st != 0:
         raise RuntimeError(f"grad_check_pieces status {st}")
     return err, cnt
+
+
+def sflux_fp64(state: dict, gz, gzh, dt: float, P=None):
+    """Surface mass fluxes -f_t(kmin)/dt of QR, QS, QG, [l][3][ij], of one FP64 host
+    step (state not modified)."""
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    out = np.zeros((lall, 3, ijdim))
+    fp = field_ptrs(state)
+    st = grad_lib().grad_sflux_fp64(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                    ctypes.byref(fp), float(dt), out.ctypes.data)
+    if st < 0:
+        raise RuntimeError(f"grad_sflux_fp64 status {st}")
+    return out
+
+
+def adjoint_host(state: dict, gz, gzh, dt: float, cot: dict, cot_sflux=None, P=None):
+    """Host run of the reverse-mode passes (the HIP kernels' bodies) with the
+    arguments of nsw6_mp_driver_adj: returns (grad dict keyed by fields.ADJ_INPUTS,
+    grad_coef[8], status)."""
+    from nicam_kokkos_microphysics_amd.fields import ADJ_INPUTS, field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    cot = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in cot.items()}
+    grad = {k: np.zeros_like(state[k], dtype=np.float64) for k in ADJ_INPUTS}
+    gc = np.zeros(8)
+    cs = None if cot_sflux is None else np.ascontiguousarray(cot_sflux, dtype=np.float64)
+    fs, fc, fg = field_ptrs(state), field_ptrs(cot), field_ptrs(grad)
+    st = grad_lib().grad_adjoint_host(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                      ctypes.byref(fs), ctypes.byref(fc), None if cs is None else cs.ctypes.data,
+                                      ctypes.byref(fg), gc.ctypes.data, float(dt))
+    return grad, gc, st
+
+
+def dual_vjp(state: dict, gz, gzh, dt: float, cot: dict, cot_sflux, dirs, dcoef=None, P=None):
+    """J = sum <cot, outputs> + <cot_sflux, sflux> of one FP64 host step and its
+    directional derivatives along `dirs` (list of dicts of input directions) and the
+    coefficient directions `dcoef` [len(dirs)][8], by forward-mode dual numbers."""
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    gz = np.ascontiguousarray(gz, dtype=np.float64)
+    gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+    lall, ijdim = _shape(state)
+    cot = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in cot.items()}
+    cs = None if cot_sflux is None else np.ascontiguousarray(cot_sflux, dtype=np.float64)
+    fs, fc = field_ptrs(state), field_ptrs(cot)
+    out = np.zeros(len(dirs))
+    J = 0.0
+    for c0 in range(0, len(dirs), GRAD_NDIR):
+        chunk = dirs[c0:c0 + GRAD_NDIR]
+        arr = (N.FieldPtrs * len(chunk))()
+        keep = []
+        for i, d in enumerate(chunk):
+            dd = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in d.items()}
+            keep.append(dd)
+            arr[i] = field_ptrs(dd)
+        dc = np.zeros((len(chunk), 8)) if dcoef is None else np.ascontiguousarray(dcoef[c0:c0 + GRAD_NDIR],
+                                                                                dtype=np.float64)
+        jv = ctypes.c_double()
+        dj = np.zeros(len(chunk))
+        st = grad_lib().grad_dual_vjp(gz.ctypes.data, gzh.ctypes.data, ctypes.byref(P), lall, ijdim,
+                                      ctypes.byref(fs), float(dt), ctypes.byref(fc),
+                                      None if cs is None else cs.ctypes.data, len(chunk), arr, dc.ctypes.data,
+                                      ctypes.addressof(jv), dj.ctypes.data)
+        if st < 0:
+            raise RuntimeError(f"grad_dual_vjp status {st}")
+        J = jv.value
+        out[c0:c0 + len(chunk)] = dj
+    return J, out

@@ -3,6 +3,9 @@ import sys
 
 import pytest
 
+# torch first: its HIP runtime is the one every library of the package must bind to
+import torch  # noqa: F401,E402
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if REPO not in sys.path:
     sys.path.insert(0, REPO)

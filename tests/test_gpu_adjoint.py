@@ -1,0 +1,214 @@
+"""Reverse mode of the FP64 step on the GPU (nsw6_mp_driver_adj through the C ABI).
+
+* Elementwise agreement with the host run of the same reverse passes
+  (oracle.adjoint_host): every input cotangent within 1e-9 of its field's max-abs,
+  the coefficient cotangents within 1e-9 relative.
+* Directional agreement with forward-mode dual numbers on the host (oracle.dual_vjp):
+  |<grad, v> - dJ/dv| <= 1e-6 |dJ/dv| along random directions through every input
+  field and the 8 coefficients.
+* Central finite differences (h = 1e-6 relative) of the GPU forward step itself on
+  the adjoint-check loss (sum of squared surface precipitation plus the rhoge
+  tendency) agree with the GPU gradient to 1e-6 relative.
+* Through two chained steps, the torch.autograd.Function gives the gradient of the
+  chained direct calls to round-off and matches central differences of the
+  two-step loss.
+* Determinism (two calls are bitwise equal) and the unsupported-region status.
+"""
+import numpy as np
+import pytest
+
+from nicam_kokkos_microphysics_amd import _native as N
+from nicam_kokkos_microphysics_amd import fields as F
+
+from adjoint_helpers import DT, STATE_KEYS, coefs, directional, random_cotangents, random_directions
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(case, lall=2, ijdim=300):
+    if case == "default":
+        return F.synth_state(lall, ijdim, seed=11)
+    if case == "supersaturated":
+        return F.synth_state(lall, ijdim, seed=5, frac_supersat=0.2)
+    if case == "towers":
+        return F.synth_state(lall, ijdim, seed=9, frac_towers=0.3)
+    raise ValueError(case)
+
+
+def _gpu_adjoint(st, gz, gzh, cot, cs, max_columns=None):
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    lall, ijdim = st["rhog"].shape[0], st["rhog"].shape[-1]
+    drv = NSW6(gz, gzh, max_columns=max_columns or lall * ijdim)
+    dev = F.to_device(st)
+    dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+    dcs = None if cs is None else torch.from_numpy(np.ascontiguousarray(cs)).cuda()
+    print(f"adjoint lall={lall} ijdim={ijdim}", flush=True)
+    grad, gc = drv.adjoint(dev, dcot, DT, cot_sflux=dcs)
+    torch.cuda.synchronize()
+    rep = drv.report()
+    out = {k: v.cpu().numpy() for k, v in grad.items()}
+    gcoef = gc.cpu().numpy()
+    drv.close()
+    return out, gcoef, rep
+
+
+@pytest.mark.parametrize("case", ["default", "supersaturated", "towers"])
+def test_gpu_reverse_matches_host_passes(case):
+    from oracle import oracle as O
+    gz, gzh, st = _state(case)
+    cot, cs = random_cotangents(st, gz, gzh, 7)
+    g_host, gc_host, status = O.adjoint_host(st, gz, gzh, DT, cot, cs)
+    assert status == 0
+    g_gpu, gc_gpu, rep = _gpu_adjoint(st, gz, gzh, cot, cs)
+    assert rep.status == 0 and rep.n_general_regions == 0
+    for k in F.ADJ_INPUTS:
+        scale = np.abs(g_host[k]).max()
+        err = np.abs(g_gpu[k] - g_host[k]).max()
+        assert err <= 1e-9 * scale + 1e-300, (k, err, scale)
+    np.testing.assert_allclose(gc_gpu, gc_host, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("case", ["default", "supersaturated"])
+def test_gpu_reverse_matches_dual_numbers(case):
+    from oracle import oracle as O
+    gz, gzh, st = _state(case, lall=1, ijdim=200)
+    cot, cs = random_cotangents(st, gz, gzh, 3)
+    g_gpu, gc_gpu, rep = _gpu_adjoint(st, gz, gzh, cot, cs)
+    assert rep.status == 0
+    dirs, dcoef = random_directions(st, 17, 8)
+    _, dj = O.dual_vjp(st, gz, gzh, DT, cot, cs, dirs, dcoef)
+    for i in range(len(dirs)):
+        s, _ = directional(g_gpu, gc_gpu, dirs[i], dcoef[i])
+        assert abs(s - dj[i]) <= 1e-6 * abs(dj[i]), (i, s, dj[i])
+
+
+def _check_loss_gpu(drv, st_dev, metrics_keys=None):
+    """Loss of the adjoint-check form on GPU outputs: sum of squared surface
+    precipitation (rain row and solid row) plus the rhoge tendency."""
+    import torch
+    s = F.copy_state(st_dev)
+    rhoge_in = s["rhoge"].clone()
+    drv.step(s, DT)
+    torch.cuda.synchronize()
+    p = s["precip"]
+    return float((p * p).sum() + ((s["rhoge"] - rhoge_in) / DT).sum()), s
+
+
+def test_gpu_reverse_matches_central_differences():
+    """Gradient of L = sum precip^2 + sum (rhoge_out - rhoge_in)/dt with respect to
+    every input field and coefficient, against central differences of the GPU step
+    (h = 1e-6 relative) along combined random directions."""
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state("supersaturated", lall=1, ijdim=256)
+    drv = NSW6(gz, gzh, max_columns=256)
+    dev = F.to_device(st)
+    L0, out = _check_loss_gpu(drv, dev)
+    cot = {"precip": 2.0 * out["precip"], "rhoge": torch.full_like(out["rhoge"], 1.0 / DT)}
+    grad, gc = drv.adjoint(dev, cot, DT)
+    grad["rhoge"] = grad["rhoge"] - 1.0 / DT   # explicit -rhoge_in/dt term of the loss
+    g = {k: v.cpu().numpy() for k, v in grad.items()}
+    gcoef = gc.cpu().numpy()
+    dirs, dcoef = random_directions(st, 29, 3)
+    c0 = coefs()
+    h = 1e-6
+    for i in range(3):
+        vals = []
+        for sgn in (1.0, -1.0):
+            P = default_params()
+            for c, name in enumerate(N.TUNABLE_COEFFS):
+                setattr(P, name, c0[c] + sgn * h * dcoef[i][c])
+            drv.set_params(P)
+            sp = F.copy_state(st)
+            for k, d in dirs[i].items():
+                sp[k] = sp[k] + sgn * h * d
+            vals.append(_check_loss_gpu(drv, F.to_device(sp))[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        s, _ = directional(g, gcoef, dirs[i], dcoef[i])
+        assert abs(s - fd) <= 1e-6 * abs(fd), (i, s, fd)
+    drv.close()
+
+
+def test_gpu_reverse_is_deterministic_and_reports_general_regions():
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6, NSW6Error
+    gz, gzh, st = _state("default", lall=2, ijdim=300)
+    cot, cs = random_cotangents(st, gz, gzh, 5)
+    a, ga, _ = _gpu_adjoint(st, gz, gzh, cot, cs)
+    b, gb, _ = _gpu_adjoint(st, gz, gzh, cot, cs)
+    for k in F.ADJ_INPUTS:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(ga, gb)
+    # a 600 s step carries rain beyond one cell near the surface: the general path
+    drv = NSW6(gz, gzh, max_columns=600)
+    dev = F.to_device(st)
+    dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+    with pytest.raises(NSW6Error) as ei:
+        drv.adjoint(dev, dcot, 600.0)
+    assert ei.value.code == N.NSW6_UNSUPPORTED
+    rep = drv.report()
+    assert rep.n_general_regions > 0
+    drv.close()
+
+
+def test_autograd_function_and_two_step_chain():
+    """Two chained differentiable steps through MPStep: the autograd gradient equals
+    the chain of direct reverse-mode calls (step 2's input cotangents as step 1's
+    output cotangents) to round-off, and central differences of the two-step loss.
+    The differences use h = 1e-7: at h = 1e-6 the stencil of this state crosses
+    limiter switches at some of its 24,576 points after the first step (the
+    difference quotient then moves by ~1e-3 between h = 1e-5 and 1e-6 and settles
+    only below), which is a kink of the step, not a gradient error."""
+    import torch
+    from nicam_kokkos_microphysics_amd.autograd import mp_step
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    gz, gzh, st = _state("supersaturated", lall=1, ijdim=256)
+    drv = NSW6(gz, gzh, max_columns=256)
+
+    def two_step_loss(host_state, track):
+        dev = F.to_device(host_state)
+        leaves = {}
+        for k in F.ADJ_INPUTS:
+            if track:
+                dev[k].requires_grad_(True)
+                leaves[k] = dev[k]
+        o1 = mp_step(drv, dev, DT)
+        s2 = dict(dev)
+        s2.update(o1)
+        o2 = mp_step(drv, s2, DT)
+        L = (o1["precip"] ** 2).sum() + (o2["precip"] ** 2).sum() + ((o2["rhoge"] - dev["rhoge"]) / DT).sum()
+        return L, leaves, o1, o2
+
+    L, leaves, o1, o2 = two_step_loss(st, True)
+    L.backward()
+    g = {k: v.grad.cpu().numpy() for k, v in leaves.items()}
+
+    # the same chain by hand through the C ABI
+    with torch.no_grad():
+        x = F.to_device(st)
+        s1 = dict(x)
+        s1.update({k: v.detach() for k, v in o1.items()})
+        g2, _ = drv.adjoint(s1, {"precip": 2.0 * o2["precip"].detach(),
+                                 "rhoge": torch.full_like(x["rhoge"], 1.0 / DT)}, DT)
+        cot1 = {k: g2[k] for k in F.ADJ_INPUTS if k in F.ADJ_OUTPUTS}
+        cot1["precip"] = 2.0 * o1["precip"].detach()
+        g1, _ = drv.adjoint(x, cot1, DT)
+        ref = {k: v.cpu().numpy() for k, v in g1.items()}
+        for k in ("vx", "vy", "vz"):
+            ref[k] = ref[k] + g2[k].cpu().numpy()
+        ref["rhoge"] = ref["rhoge"] - 1.0 / DT
+    for k in F.ADJ_INPUTS:
+        scale = np.abs(ref[k]).max()
+        assert np.abs(g[k] - ref[k]).max() <= 1e-12 * scale + 1e-300, k
+
+    dirs, _ = random_directions(st, 31, 2)
+    h = 1e-7
+    with torch.no_grad():
+        for i in range(2):
+            sp = {k: (st[k] + h * dirs[i][k] if k in dirs[i] else st[k]) for k in st}
+            sm = {k: (st[k] - h * dirs[i][k] if k in dirs[i] else st[k]) for k in st}
+            fd = (float(two_step_loss(sp, False)[0]) - float(two_step_loss(sm, False)[0])) / (2 * h)
+            s = sum(float(np.sum(g[k] * dirs[i][k])) for k in dirs[i] if k in g)
+            assert abs(s - fd) <= 1e-5 * abs(fd), (i, s, fd)
+    drv.close()
