@@ -122,8 +122,16 @@ struct CellB {   // cotangent of a Cell
 struct CellInB {   // cotangent of a CellIn (differentiable members)
     double rhog, rq[5], rhoge, rvx, rvy, rvz, rw, vx, vy, vz;
 };
-NSW6_HD void zero(CellB &c) { memset(&c, 0, sizeof c); }
-NSW6_HD void zero(CellInB &c) { memset(&c, 0, sizeof c); }
+// Zeroing by elements (a memset of the whole struct would keep it in scratch memory).
+template <typename T> NSW6_HD void zero_words(T &v)
+{
+    static_assert(sizeof(T) % sizeof(double) == 0, "struct of doubles");
+    double *w = reinterpret_cast<double *>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / sizeof(double)); i++) w[i] = 0.0;
+}
+NSW6_HD void zero(CellB &c) { zero_words(c); }
+NSW6_HD void zero(CellInB &c) { zero_words(c); }
 
 NSW6_HD void rhof_b(const Cell<double> &c, int t, int i, double b, CellB &cb)
 {
@@ -149,56 +157,57 @@ NSW6_HD void rhof_b(const Cell<double> &c, int t, int i, double b, CellB &cb)
     }
 }
 
+// Reverse of wall_flux for tracer T (a compile-time index, so that the Cell
+// cotangents stay in registers).
+template <int T>
+NSW6_HD void wall_flux_b_tracer(const GridDev &g, int w, const Cell<double> &cur, const Cell<double> &nxt,
+                                const double z0[3], const double zh0[3], const WallGeom &geo,
+                                const WallFlux<double> &Fb, const double ft_surf_b[3][4], CellB &cur_b,
+                                CellB &nxt_b, double z0_b[3], double zh0_b[3])
+{
+    const double dzw = g.dgz[w];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const double r0 = rhof_cell(cur, T, i);
+        const double rn = rhof_cell(nxt, T, i);
+        const double v = geo.up1[T] ? (-(r0 * dzw) + rn * z0[T]) : r0 * z0[T];
+        double ft_b = (i == 0) ? Fb.fq[T] : Fb.fs[i - 1];
+        if (i < 4) ft_b += ft_surf_b[T][i];
+        const double v_b = ft_b * eps_mask(v);
+        // both cells take a (possibly zero) share: one call per cell keeps each
+        // cotangent struct in its own registers
+        const bool up = geo.up1[T];
+        rhof_b(cur, T, i, up ? -v_b * dzw : v_b * z0[T], cur_b);
+        rhof_b(nxt, T, i, up ? v_b * z0[T] : 0.0, nxt_b);
+        z0_b[T] += v_b * (up ? rn : r0);
+    }
+    if (w >= KMIN + 1) {
+        const double dzh = g.dgzh[w];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const double hc = cur.hv[T][i], hn = nxt.hv[T][i];
+            double v;
+            if (geo.half_full[T])
+                v = -(hc * dzh) + hn * zh0[T];
+            else
+                v = (geo.half_next[T] ? hn : hc) * zh0[T];
+            const double v_b = Fb.fh[i] * eps_mask(v);
+            const bool full = geo.half_full[T], next = geo.half_next[T];
+            cur_b.hv[T][i] += full ? -v_b * dzh : (next ? 0.0 : v_b * zh0[T]);
+            nxt_b.hv[T][i] += (full || next) ? v_b * zh0[T] : 0.0;
+            zh0_b[T] += v_b * ((full || next) ? hn : hc);
+        }
+    }
+}
+
 // Reverse of wall_flux (the departure geometry is given).
 NSW6_HD void wall_flux_b(const GridDev &g, int w, const Cell<double> &cur, const Cell<double> &nxt,
                          const double z0[3], const double zh0[3], const WallGeom &geo, const WallFlux<double> &Fb,
                          const double ft_surf_b[3][4], CellB &cur_b, CellB &nxt_b, double z0_b[3], double zh0_b[3])
 {
-    const double dzw = g.dgz[w];
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const double r0 = rhof_cell(cur, t, i);
-            const double rn = rhof_cell(nxt, t, i);
-            const double v = geo.up1[t] ? (-(r0 * dzw) + rn * z0[t]) : r0 * z0[t];
-            double ft_b = (i == 0) ? Fb.fq[t] : Fb.fs[i - 1];
-            if (i < 4) ft_b += ft_surf_b[t][i];
-            const double v_b = ft_b * eps_mask(v);
-            if (geo.up1[t]) {
-                rhof_b(cur, t, i, -v_b * dzw, cur_b);
-                rhof_b(nxt, t, i, v_b * z0[t], nxt_b);
-                z0_b[t] += v_b * rn;
-            } else {
-                rhof_b(cur, t, i, v_b * z0[t], cur_b);
-                z0_b[t] += v_b * r0;
-            }
-        }
-        if (w >= KMIN + 1) {
-            const double dzh = g.dgzh[w];
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const double hc = cur.hv[t][i], hn = nxt.hv[t][i];
-                double v;
-                if (geo.half_full[t])
-                    v = -(hc * dzh) + hn * zh0[t];
-                else
-                    v = (geo.half_next[t] ? hn : hc) * zh0[t];
-                const double v_b = Fb.fh[i] * eps_mask(v);
-                if (geo.half_full[t]) {
-                    cur_b.hv[t][i] += -v_b * dzh;
-                    nxt_b.hv[t][i] += v_b * zh0[t];
-                    zh0_b[t] += v_b * hn;
-                } else if (geo.half_next[t]) {
-                    nxt_b.hv[t][i] += v_b * zh0[t];
-                    zh0_b[t] += v_b * hn;
-                } else {
-                    cur_b.hv[t][i] += v_b * zh0[t];
-                    zh0_b[t] += v_b * hc;
-                }
-            }
-        }
-    }
+    wall_flux_b_tracer<0>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    wall_flux_b_tracer<1>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    wall_flux_b_tracer<2>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
 }
 
 // Reverse of derive_cell: Cell cotangent -> raw cell cotangent and the below-cell
@@ -274,7 +283,7 @@ struct SedInB {
     double fq[4][3], fs[2][5], fh[3][2];
     double rw_c, rw_p, rg_m, rg_c, rg_p, rq[3], re0, rvx, rvy, rvz;
 };
-NSW6_HD void zero(SedInB &s) { memset(&s, 0, sizeof s); }
+NSW6_HD void zero(SedInB &s) { zero_words(s); }
 
 NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<double> &x, const SedOut<double> &yb,
                         SedInB &xb)
@@ -731,382 +740,240 @@ constexpr int tend_blk[6] = {-1, 0, 2, 1, 4, 5};
 
 NSW6_HD double sel(bool c, double a, double b) { return c ? a : b; }
 
-NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in, const double qt_b[6],
-                     const double vt_b[3], RateInB &b)
+// A copy of the rate inputs that the device compiler treats as unknown values, so
+// that expressions of them evaluated before and after are computed twice rather
+// than kept in registers in between.
+NSW6_HD double opaque(double x)
 {
-    // =============================== forward ===============================
-    const double PI = C_PI, EPS = C_EPS;
-    const double Ar = P.Ar, As = P.As, Ag = P.Ag;
-    const double Cr = in.tun[4], Cs = in.tun[5], Cg = in.tun[6];
-    const double N0r = P.N0r, N0g = P.N0g, GAM = P.GAM, GAM_2 = P.GAM_2, GAM_3 = P.GAM_3;
-    const double dens = in.dens, temp = in.temp, temc = in.temc;
-    double qx[6];
-    bool qpos[6];
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(x));
+#endif
+    return x;
+}
+NSW6_HD RateIn<double> opaque_copy(const RateIn<double> &a)
+{
+    RateIn<double> r = a;
+    double *w = reinterpret_cast<double *>(&r);
+    static_assert(sizeof(RateIn<double>) % sizeof(double) == 0, "RateIn<double> holds doubles only");
 #pragma unroll
-    for (int n = 0; n < 6; n++) {
-        qpos[n] = !(in.q[n] < 0.0);
-        qx[n] = qpos[n] ? in.q[n] : 0.0;
-    }
-    const double qv = qx[QV], qc = qx[QC], qr = qx[QR], qi = qx[QI], qs = qx[QS], qg = qx[QG];
-    const RateSwitches &sw = in.sw;
-    const double Rdens = 1.0 / dens;
-    const double rho_fact = sqrt(DENS00 * Rdens);
-    const double delta1 = sw.delta1, delta2 = sw.delta2, spsati = sw.spsati, iceflg = sw.iceflg;
-    double dqdt[6];
+    for (int i = 0; i < (int)(sizeof(RateIn<double>) / sizeof(double)); i++) w[i] = opaque(w[i]);
+    return r;
+}
+NSW6_HD PointIn<double> opaque_point(const PointIn<double> &a)
+{
+    PointIn<double> r;
+    r.rhog = opaque(a.rhog);
 #pragma unroll
-    for (int n = 0; n < 6; n++) dqdt[n] = qx[n] / dt;
-    // rain slope and powers
-    const double zr = sw.zr, Kr = Ar * N0r * P.GAM_1br;
-    const double Xr = dens * qr / Kr + zr;
-    const double s1r = sqrt(Xr), s2r = sqrt(s1r);
-    const double L1 = s2r * (1.0 - zr);
-    const double Ldr = sqrt(L1);
-    const double L2 = L1 * L1, L3 = L2 * L1, L4 = L2 * L2, L5 = L4 * L1, L6 = L3 * L3, L7 = L6 * L1;
-    const double L3dr = L3 * Ldr, L5dr = L5 * Ldr, L6dr = L6 * Ldr;
-    // Roh and Satoh snow moments
-    const double Xs2 = dens * qs / As;
-    const double zx = sw.zx;
-    const bool tcold = (temc < -0.1);
-    const double tems = tcold ? temc : -0.1;
-    const double *ca = P.coef_a, *cb = P.coef_b;
-    double at[4], bt[4];
-    at[0] = ca[0] + tems * (ca[1] + tems * (ca[4] + tems * ca[8]));
-    at[1] = ca[2] + tems * (ca[3] + tems * ca[6]);
-    at[2] = ca[5] + tems * ca[7];
-    at[3] = ca[9];
-    bt[0] = cb[0] + tems * (cb[1] + tems * (cb[4] + tems * cb[8]));
-    bt[1] = cb[2] + tems * (cb[3] + tems * cb[6]);
-    bt[2] = cb[5] + tems * cb[7];
-    bt[3] = cb[9];
-    const double ln10 = P.ln10;
-    const double lx = log(Xs2 + zx), omz = 1.0 - zx;
-    const double nmv[7] = {0.0, 1.0, 2.0, 3.0, 4.0, 2.25, 1.625};
-    double mom[7], lga[7], bb[7];
-#pragma unroll
-    for (int m = 0; m < 7; m++) {
-        const double nm = nmv[m];
-        lga[m] = (m == 0) ? at[0] : at[0] + nm * (at[1] + nm * (at[2] + nm * at[3]));
-        bb[m] = (m == 0) ? bt[0] : bt[0] + nm * (bt[1] + nm * (bt[2] + nm * bt[3]));
-        mom[m] = exp(ln10 * lga[m] + lx * bb[m]);
-    }
-    const double M0 = mom[0] * omz, M1 = mom[1] * omz, M2 = Xs2, M0bs = mom[2] * omz, M1bs = mom[3] * omz;
-    const double M2bs = mom[4] * omz, M2ds = mom[5] * omz, M5dsh = mom[6] * omz;
-    const double Vden = M0bs + zx;
-    const double RMV = M2ds / Vden;
-    // graupel slope
-    const double zg = sw.zg, Kg = Ag * N0g * P.GAM_1bg;
-    const double Xg = dens * qg / Kg + zg;
-    const double s1g = sqrt(Xg), s2g = sqrt(s1g);
-    const double G1 = s2g * (1.0 - zg);
-    const double Gdg = sqrt(G1);
-    const double G2 = G1 * G1, G3 = G2 * G1, G3dg = G3 * Gdg, G5dg = (G3 * G2) * Gdg;
-    // fall speeds
-    const double kVr = P.GAM_1brdr / P.GAM_1br, kVg = P.GAM_1bgdg / P.GAM_1bg;
-    const double Vtr = -Cr * rho_fact * P.GAM_1brdr / P.GAM_1br * Ldr;
-    const double Vts = -Cs * rho_fact * RMV;
-    const double Vtg = -Cg * rho_fact * P.GAM_1bgdg / P.GAM_1bg * Gdg;
-    // nucleation
-    const double eN = exp(-0.1 * temc);
-    const bool Nlow = (eN < 1.0);
-    const double Ni0 = (Nlow ? 1.0 : eN) * 1000.0;
-    const double E133 = exp(log(Ni0) * 1.33);
-    const double icen = in.tun[7];
-    const double Qi0 = icen * E133 * Rdens;
-    const double gA = Qi0 - qi, gB = in.qv_qsati;
-    const bool gselB = (gB < gA);
-    const double gmin = gselB ? gB : gA;
-    const bool gpos = !(gmin < 0.0);
-    const double Pigen0 = (gpos ? gmin : 0.0) / dt;
-    // accretion
-    const double Esi = in.tun[2], Esr = in.tun[3];
-    const double eEsi = exp(P.gamma_sacr * temc);
-    const bool EsiE = (Esi * eEsi < Esi);
-    const double Esi_mod = EsiE ? Esi * eEsi : Esi;
-    const double eEgs = exp(P.gamma_gacs * temc);
-    const bool EgsE = (P.Egs * eEgs < P.Egs);
-    const double Egs_mod = EgsE ? P.Egs * eEgs : P.Egs;
-    const double zcr = sw.zcr;
-    const double Xcr = qc * qr + zcr;
-    const double Ecr = exp(log(Xcr) * 1.15);
-    const double Pracw0 = 67.0 * Ecr * (1.0 - zcr);
-    const double kac = 0.25 * PI;
-    const double Psacw0 = qc * 0.25 * PI * P.Esw * Cs * M2ds * rho_fact;
-    const double Pgacw0 = qc * 0.25 * PI * P.Egw * N0g * Cg * P.GAM_3dg * G3dg * rho_fact;
-    const double Praci0 = qi * 0.25 * PI * P.Eri * N0r * Cr * P.GAM_3dr * L3dr * rho_fact;
-    const double Psaci0 = qi * 0.25 * PI * Esi_mod * Cs * M2ds * rho_fact;
-    const double Pgaci0 = qi * 0.25 * PI * P.Egi * N0g * Cg * P.GAM_3dg * G3dg * rho_fact;
-    const double Piacr0 = qi * Ar / P.mi * 0.25 * PI * P.Eri * N0r * Cr * P.GAM_6dr * L6dr * rho_fact;
-    const double SR = P.GAM_1br * L4 * M2 + 2.0 * P.GAM_2br * L5 * M1 + P.GAM_3br * L6 * M0;
-    const double dVrs = Vtr - Vts, aVrs = fabs(dVrs), sVrs = (dVrs < 0.0) ? -1.0 : 1.0;
-    const double Psacr0 = Ar * 0.25 * PI * Rdens * Esr * N0r * aVrs * SR;
-    const double SG = P.GAM_1br * L4 * GAM_3 * G3 + 2.0 * P.GAM_2br * L5 * GAM_2 * G2 + P.GAM_3br * L6 * GAM * G1;
-    const double dVgr = Vtg - Vtr, aVgr = fabs(dVgr), sVgr = (dVgr < 0.0) ? -1.0 : 1.0;
-    const double Pgacr0 = Ar * 0.25 * PI * Rdens * P.Egr * N0g * N0r * aVgr * SG;
-    const double SS = M0bs * GAM_3 * L3 + 2.0 * M1bs * GAM_2 * L2 + M2bs * GAM * L1;
-    const double Pracs0 = As * 0.25 * PI * Rdens * Esr * N0r * aVrs * SS;
-    const double SGS = M0bs * GAM_3 * G3 + 2.0 * M1bs * GAM_2 * G2 + M2bs * GAM * G1;
-    const double dVgs = Vtg - Vts, aVgs = fabs(dVgs), sVgs = (dVgs < 0.0) ? -1.0 : 1.0;
-    const double Pgacs0 = As * 0.25 * PI * Rdens * Egs_mod * N0g * aVgs * SGS;
-    // autoconversion
-    const double kk = in.tun[0];
-    const double Eqc = exp(2.47 * log(qc));
-    const double Praut0 = kk * Eqc * P.Nc_pow_m179;
-    const double ebi = exp(P.gamma_saut * temc);
-    const bool biE = (P.beta_saut * ebi < P.beta_saut);
-    const double betai = biE ? P.beta_saut * ebi : P.beta_saut;
-    const double qicrt = in.tun[1];
-    const double Psaut00 = betai * (qi - qicrt);
-    const bool saut_pos = !(Psaut00 < 0.0);
-    const double Psaut0 = saut_pos ? Psaut00 : 0.0;
-    const double ebs = exp(P.gamma_gaut * temc);
-    const bool bsE = (P.beta_gaut * ebs < P.beta_gaut);
-    const double betas = bsE ? P.beta_gaut * ebs : P.beta_gaut;
-    const double Pgaut00 = betas * (qs - P.qscrt_gaut);
-    const bool gaut_pos = !(Pgaut00 < 0.0);
-    const double Pgaut0 = gaut_pos ? Pgaut00 : 0.0;
-    // evaporation, sublimation, melting, freezing
-    const double Ka = Ka0 + dKa_dT * temc;
-    const double Kdt = Kd0 + dKd_dT * temc;
-    const double Kd = Kdt * C_Pstd / in.pre;
-    const double Nut = nu0 + dnu_dT * temc;
-    const double Nu = Nut * Rdens;
-    const double av1 = C_LHV0 / (Ka * temp), av2 = C_LHV0 / (C_Rvap * temp) - 1.0, Dv = Kd * dens * in.qsatl;
-    const double denv = av1 * av2 + 1.0 / Dv, Glv = 1.0 / denv;
-    const double ai1 = C_LHS0 / (Ka * temp), ai2 = C_LHS0 / (C_Rvap * temp) - 1.0, Di_ = Kd * dens * in.qsati;
-    const double deni = ai1 * ai2 + 1.0 / Di_, Giv = 1.0 / deni;
-    const double Gil = 1.0 / (C_LHF0 / (Ka * temc));
-    const double srr = sqrt(Cr * rho_fact / Nu * L5dr);
-    const double ventr = P.f1r * GAM_2 * L2 + P.f2r * srr * P.GAM_5dr_h;
-    const double kev = 2.0 * PI;
-    const double Prevp0 = kev * Rdens * N0r * in.sliq_1m * Glv * ventr;
-    const double rqi0 = dens * qi;
-    const bool rqi_pos = !(rqi0 < EPS);
-    const double rhoqi = rqi_pos ? rqi0 : EPS;
-    const double XN0 = 5.38E+7 * exp(log(rhoqi) * 0.75);
-    const bool XNlo = (XN0 < 1.0E+3);
-    const double XN1 = XNlo ? 1.0E+3 : XN0;
-    const bool XNhi = (1.0E+6 < XN1);
-    const double XNi = XNhi ? 1.0E+6 : XN1;
-    const double XMi = rhoqi / XNi;
-    const double sXM = sqrt(XMi);
-    const double Di0 = P.Di_a * sXM;
-    const bool Dihi = (P.Di_max < Di0);
-    const double Di = Dihi ? P.Di_max : Di0;
-    const double tmpi = 4.0 * Di * XNi * Rdens * in.sice_m1 * Giv;
-    const double Pidep0 = spsati * tmpi, Pisub0 = (1.0 - spsati) * (-tmpi);
-    const double Pihom0 = sw.sw_hom * qc / dt;
-    const double efr = exp(-P.A_frz * temc);
-    const double frz = efr - 1.0;
-    const double Pihtr0 = sw.sw_htr * (dens / C_DWATR * (qc * qc) / (P.Nc_ihtr * 1.0E+6)) * P.B_frz * frz;
-    const double Pimlt0 = sw.sw_mlt * qi / dt;
-    const double srs = sqrt(Cs * rho_fact / Nu);
-    const double vents = P.f1s * M1 + P.f2s * srs * M5dsh;
-    const double tmps = kev * Rdens * in.sice_m1 * Giv * vents;
-    const double Psdep0 = spsati * tmps, Pssub0 = (1.0 - spsati) * (-tmps);
-    const double cmlt = C_CL * temc / C_LHF0;
-    const double Psmlt00 = kev * Rdens * Gil * vents + cmlt * (Psacw0 + Psacr0);
-    const bool smlt_pos = !(Psmlt00 < 0.0);
-    const double Psmlt0 = smlt_pos ? Psmlt00 : 0.0;
-    const double srg = sqrt(Cg * rho_fact / Nu * G5dg);
-    const double ventg = P.f1g * GAM_2 * G2 + P.f2g * srg * P.GAM_5dg_h;
-    const double tmpg = kev * Rdens * N0g * in.sice_m1 * Giv * ventg;
-    const double Pgdep0 = spsati * tmpg, Pgsub0 = (1.0 - spsati) * (-tmpg);
-    const double Pgmlt00 = kev * Rdens * N0g * Gil * ventg + cmlt * (Pgacw0 + Pgacr0);
-    const bool gmlt_pos = !(Pgmlt00 < 0.0);
-    const double Pgmlt0 = gmlt_pos ? Pgmlt00 : 0.0;
-    const double kfrz = kev * N0r * 60.0 * P.B_frz * Ar;   // Pgfrz0 = kfrz Rdens frz L7
-    const double Pgfrz0 = 2.0 * PI * Rdens * N0r * 60.0 * P.B_frz * Ar * frz * L7;
+    for (int n = 0; n < NQ; n++) r.rhogq[n] = opaque(a.rhogq[n]);
+    r.tem = opaque(a.tem);
+    r.qv = opaque(a.qv);
+    r.gsgam2 = opaque(a.gsgam2);
+    return r;
+}
 
-    // limiters: R = min(P0, dq_dt) * c; lim = true when dq_dt is taken
-    double R[NRATE];
-    bool lim[NRATE];
-    double cst[NRATE];
-    double raw[NRATE];   // the limited quantity before min
-    int limq[NRATE];     // species whose dq_dt limits it (-1: none)
-#pragma unroll
-    for (int j = 0; j < NRATE; j++) {
-        lim[j] = false;
-        cst[j] = 1.0;
-        limq[j] = -1;
-        raw[j] = 0.0;
-    }
-    const double ie = iceflg, we = 1.0 - iceflg;
-    auto setlim = [&](int j, double p0, int sp, double c) {
-        raw[j] = p0;
-        limq[j] = sp;
-        cst[j] = c;
-        lim[j] = (dqdt[sp] < p0);
-        R[j] = (lim[j] ? dqdt[sp] : p0) * c;
-    };
-    setlim(rPigen, Pigen0, QV, ie);
-    setlim(rPidep, Pidep0, QV, ie);
-    setlim(rPsdep, Psdep0, QV, ie);
-    setlim(rPgdep, Pgdep0, QV, ie);
-    const double Pracw1 = Pracw0 + Psacw0 * we + Pgacw0 * we;
-    setlim(rPraut, Praut0, QC, 1.0);
-    setlim(rPracw, Pracw1, QC, 1.0);
-    setlim(rPihom, Pihom0, QC, ie);
-    setlim(rPihtr, Pihtr0, QC, ie);
-    setlim(rPsacw, Psacw0, QC, ie);
-    setlim(rPsfw, 0.0, QC, 0.0);
-    setlim(rPgacw, Pgacw0, QC, ie);
-    setlim(rPrevp, Prevp0, QR, 1.0);
-    double Piacr1, Psacr1;
-    {
-        const bool l1 = (dqdt[QR] < Piacr0);
-        Piacr1 = (l1 ? dqdt[QR] : Piacr0) * ie;
-        lim[rPiacr_s] = lim[rPiacr_g] = l1;
-        raw[rPiacr_s] = raw[rPiacr_g] = Piacr0;
-        limq[rPiacr_s] = limq[rPiacr_g] = QR;
-        const bool l2 = (dqdt[QR] < Psacr0);
-        Psacr1 = (l2 ? dqdt[QR] : Psacr0) * ie;
-        lim[rPsacr_s] = lim[rPsacr_g] = l2;
-        raw[rPsacr_s] = raw[rPsacr_g] = Psacr0;
-        limq[rPsacr_s] = limq[rPsacr_g] = QR;
-    }
-    setlim(rPgacr, Pgacr0, QR, ie);
-    setlim(rPgfrz, Pgfrz0, QR, ie);
-    setlim(rPisub, Pisub0, QI, ie);
-    setlim(rPimlt, Pimlt0, QI, we);
-    setlim(rPsaut, Psaut0, QI, ie);
-    double Praci1;
-    {
-        const bool l1 = (dqdt[QI] < Praci0);
-        Praci1 = (l1 ? dqdt[QI] : Praci0) * ie;
-        lim[rPraci_s] = lim[rPraci_g] = l1;
-        raw[rPraci_s] = raw[rPraci_g] = Praci0;
-        limq[rPraci_s] = limq[rPraci_g] = QI;
-    }
-    setlim(rPsaci, Psaci0, QI, ie);
-    setlim(rPsfi, 0.0, QI, 0.0);
-    setlim(rPgaci, Pgaci0, QI, ie);
-    setlim(rPssub, Pssub0, QS, ie);
-    setlim(rPsmlt, Psmlt0, QS, we);
-    setlim(rPgaut, Pgaut0, QS, ie);
-    double Pracs1;
-    {
-        const bool l1 = (dqdt[QS] < Pracs0);
-        Pracs1 = (l1 ? dqdt[QS] : Pracs0) * ie;
-        lim[rPracs] = l1;
-        raw[rPracs] = Pracs0;
-        limq[rPracs] = QS;
-    }
-    setlim(rPgacs, Pgacs0, QS, 1.0);
-    setlim(rPgsub, Pgsub0, QG, ie);
-    setlim(rPgmlt, Pgmlt0, QG, we);
-    // splits
-    R[rPiacr_s] = (1.0 - delta1) * Piacr1;
-    R[rPiacr_g] = delta1 * Piacr1;
-    R[rPraci_s] = (1.0 - delta1) * Praci1;
-    R[rPraci_g] = delta1 * Praci1;
-    R[rPsacr_s] = delta2 * Psacr1;
-    R[rPsacr_g] = (1.0 - delta2) * Psacr1;
-    R[rPracs] = (1.0 - delta2) * Pracs1;
-
-    // limiter blocks: R0[b] = block entry values of the members, fac, ratio branch
-    double Rin[NBLK][BLKMAX], net[NBLK], fac[NBLK];
-    bool ratio[NBLK];
-#pragma unroll
-    for (int k = 0; k < NBLK; k++) {
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < BLKMAX; m++) {
-            if (m >= blk_n[k]) break;
-            Rin[k][m] = R[blk_r[k][m]];
-            s = (m == 0) ? blk_s[k][m] * Rin[k][m] : s + blk_s[k][m] * Rin[k][m];
-        }
-        net[k] = s;
-        const double fs = 0.5 + copysign(0.5, s + EPS);
-        const double rt = -dqdt[blk_q[k]] / (s - fs);
-        ratio[k] = (fs == 0.0) && !(1.0 < rt);
-        fac[k] = fs + (1.0 - fs) * dmin(rt, 1.0);
-#pragma unroll
-        for (int m = 0; m < BLKMAX; m++) {
-            if (m >= blk_n[k]) break;
-            R[blk_r[k][m]] *= fac[k];
-        }
-    }
-
-    // =============================== reverse ===============================
-    double R_b[NRATE];
-#pragma unroll
-    for (int j = 0; j < NRATE; j++) R_b[j] = 0.0;
-    double dqdt_b[6] = {0, 0, 0, 0, 0, 0};
-    // tendencies q_t = max(sum, -dq_dt)
-#pragma unroll
-    for (int n = 1; n < 6; n++) {
-        const int k = tend_blk[n];
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < BLKMAX; m++) {
-            if (m >= blk_n[k]) break;
-            s = (m == 0) ? blk_s[k][m] * R[blk_r[k][m]] : s + blk_s[k][m] * R[blk_r[k][m]];
-        }
-        if (s < -dqdt[n]) {
-            dqdt_b[n] -= qt_b[n];
-        } else {
-#pragma unroll
-            for (int m = 0; m < BLKMAX; m++) {
-                if (m >= blk_n[k]) break;
-                R_b[blk_r[k][m]] += blk_s[k][m] * qt_b[n];
-            }
-        }
-    }
-    // limiter blocks in reverse
-#pragma unroll
-    for (int k = NBLK - 1; k >= 0; k--) {
-        double fac_b = 0.0;
-#pragma unroll
-        for (int m = 0; m < BLKMAX; m++) {
-            if (m >= blk_n[k]) break;
-            const int j = blk_r[k][m];
-            fac_b += R_b[j] * Rin[k][m];
-            R_b[j] *= fac[k];
-        }
-        if (ratio[k]) {
-            // fac = -dq_dt / net
-            const double n_ = net[k];
-            dqdt_b[blk_q[k]] -= fac_b / n_;
-            const double net_b = fac_b * dqdt[blk_q[k]] / (n_ * n_);
-#pragma unroll
-            for (int m = 0; m < BLKMAX; m++) {
-                if (m >= blk_n[k]) break;
-                R_b[blk_r[k][m]] += blk_s[k][m] * net_b;
-            }
-        }
-    }
-    // splits
-    const double Piacr1_b = (1.0 - delta1) * R_b[rPiacr_s] + delta1 * R_b[rPiacr_g];
-    const double Praci1_b = (1.0 - delta1) * R_b[rPraci_s] + delta1 * R_b[rPraci_g];
-    const double Psacr1_b = delta2 * R_b[rPsacr_s] + (1.0 - delta2) * R_b[rPsacr_g];
-    const double Pracs1_b = (1.0 - delta2) * R_b[rPracs];
-    // limiters: cotangent of the quantity before min (raw) or of dq_dt
-    double raw_b[NRATE];
+// Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
+NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const double qt_b[6],
+                     const double vt_b[3], RateInB &b, double qt[6])
+{
+    double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < NRATE; j++) raw_b[j] = 0.0;
-    auto unlim = [&](int j, double rb) {
-        const double v = rb * cst[j];
-        if (lim[j])
-            dqdt_b[limq[j]] += v;
-        else
-            raw_b[j] += v;
-    };
-    const int plain[] = {rPigen, rPidep, rPsdep, rPgdep, rPraut, rPracw, rPihom, rPihtr, rPsacw, rPsfw, rPgacw,
-                         rPrevp, rPgacr, rPgfrz, rPisub, rPimlt, rPsaut, rPsaci, rPsfi, rPgaci, rPssub, rPsmlt,
-                         rPgaut, rPgacs, rPgsub, rPgmlt};
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(plain) / sizeof(plain[0])); i++) unlim(plain[i], R_b[plain[i]]);
-    // split rates share one limited quantity: constant ie, limiter of the pair
+    // ---- phase 1: forward, limiter decisions, reverse of the limiter algebra -> raw_b
     {
-        const double v1 = Piacr1_b * ie;
-        if (lim[rPiacr_s]) dqdt_b[QR] += v1; else raw_b[rPiacr_s] += v1;
-        const double v2 = Psacr1_b * ie;
-        if (lim[rPsacr_s]) dqdt_b[QR] += v2; else raw_b[rPsacr_s] += v2;
-        const double v3 = Praci1_b * ie;
-        if (lim[rPraci_s]) dqdt_b[QI] += v3; else raw_b[rPraci_s] += v3;
-        const double v4 = Pracs1_b * ie;
-        if (lim[rPracs]) dqdt_b[QS] += v4; else raw_b[rPracs] += v4;
+        const RateIn<double> &in = in0;
+#include "nsw6_rates_fwd.inc"
+        // limiters: R = min(P0, dq_dt) * c; lim = true when dq_dt is taken
+        double R[NRATE];
+        bool lim[NRATE];
+        double cst[NRATE];
+        double raw[NRATE];   // the limited quantity before min
+        int limq[NRATE];     // species whose dq_dt limits it (-1: none)
+#pragma unroll
+        for (int j = 0; j < NRATE; j++) {
+            lim[j] = false;
+            cst[j] = 1.0;
+            limq[j] = -1;
+            raw[j] = 0.0;
+        }
+        auto setlim = [&](int j, double p0, int sp, double c) {
+            raw[j] = p0;
+            limq[j] = sp;
+            cst[j] = c;
+            lim[j] = (dqdt[sp] < p0);
+            R[j] = (lim[j] ? dqdt[sp] : p0) * c;
+        };
+        setlim(rPigen, Pigen0, QV, ie);
+        setlim(rPidep, Pidep0, QV, ie);
+        setlim(rPsdep, Psdep0, QV, ie);
+        setlim(rPgdep, Pgdep0, QV, ie);
+        const double Pracw1 = Pracw0 + Psacw0 * we + Pgacw0 * we;
+        setlim(rPraut, Praut0, QC, 1.0);
+        setlim(rPracw, Pracw1, QC, 1.0);
+        setlim(rPihom, Pihom0, QC, ie);
+        setlim(rPihtr, Pihtr0, QC, ie);
+        setlim(rPsacw, Psacw0, QC, ie);
+        setlim(rPsfw, 0.0, QC, 0.0);
+        setlim(rPgacw, Pgacw0, QC, ie);
+        setlim(rPrevp, Prevp0, QR, 1.0);
+        double Piacr1, Psacr1;
+        {
+            const bool l1 = (dqdt[QR] < Piacr0);
+            Piacr1 = (l1 ? dqdt[QR] : Piacr0) * ie;
+            lim[rPiacr_s] = lim[rPiacr_g] = l1;
+            raw[rPiacr_s] = raw[rPiacr_g] = Piacr0;
+            limq[rPiacr_s] = limq[rPiacr_g] = QR;
+            const bool l2 = (dqdt[QR] < Psacr0);
+            Psacr1 = (l2 ? dqdt[QR] : Psacr0) * ie;
+            lim[rPsacr_s] = lim[rPsacr_g] = l2;
+            raw[rPsacr_s] = raw[rPsacr_g] = Psacr0;
+            limq[rPsacr_s] = limq[rPsacr_g] = QR;
+        }
+        setlim(rPgacr, Pgacr0, QR, ie);
+        setlim(rPgfrz, Pgfrz0, QR, ie);
+        setlim(rPisub, Pisub0, QI, ie);
+        setlim(rPimlt, Pimlt0, QI, we);
+        setlim(rPsaut, Psaut0, QI, ie);
+        double Praci1;
+        {
+            const bool l1 = (dqdt[QI] < Praci0);
+            Praci1 = (l1 ? dqdt[QI] : Praci0) * ie;
+            lim[rPraci_s] = lim[rPraci_g] = l1;
+            raw[rPraci_s] = raw[rPraci_g] = Praci0;
+            limq[rPraci_s] = limq[rPraci_g] = QI;
+        }
+        setlim(rPsaci, Psaci0, QI, ie);
+        setlim(rPsfi, 0.0, QI, 0.0);
+        setlim(rPgaci, Pgaci0, QI, ie);
+        setlim(rPssub, Pssub0, QS, ie);
+        setlim(rPsmlt, Psmlt0, QS, we);
+        setlim(rPgaut, Pgaut0, QS, ie);
+        double Pracs1;
+        {
+            const bool l1 = (dqdt[QS] < Pracs0);
+            Pracs1 = (l1 ? dqdt[QS] : Pracs0) * ie;
+            lim[rPracs] = l1;
+            raw[rPracs] = Pracs0;
+            limq[rPracs] = QS;
+        }
+        setlim(rPgacs, Pgacs0, QS, 1.0);
+        setlim(rPgsub, Pgsub0, QG, ie);
+        setlim(rPgmlt, Pgmlt0, QG, we);
+        // splits
+        R[rPiacr_s] = (1.0 - delta1) * Piacr1;
+        R[rPiacr_g] = delta1 * Piacr1;
+        R[rPraci_s] = (1.0 - delta1) * Praci1;
+        R[rPraci_g] = delta1 * Praci1;
+        R[rPsacr_s] = delta2 * Psacr1;
+        R[rPsacr_g] = (1.0 - delta2) * Psacr1;
+        R[rPracs] = (1.0 - delta2) * Pracs1;
+
+        // limiter blocks: R0[b] = block entry values of the members, fac, ratio branch
+        double Rin[NBLK][BLKMAX], net[NBLK], fac[NBLK];
+        bool ratio[NBLK];
+#pragma unroll
+        for (int k = 0; k < NBLK; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < BLKMAX; m++) {
+                if (m >= blk_n[k]) break;
+                Rin[k][m] = R[blk_r[k][m]];
+                s = (m == 0) ? blk_s[k][m] * Rin[k][m] : s + blk_s[k][m] * Rin[k][m];
+            }
+            net[k] = s;
+            const double fs = 0.5 + copysign(0.5, s + EPS);
+            const double rt = -dqdt[blk_q[k]] / (s - fs);
+            ratio[k] = (fs == 0.0) && !(1.0 < rt);
+            fac[k] = fs + (1.0 - fs) * dmin(rt, 1.0);
+#pragma unroll
+            for (int m = 0; m < BLKMAX; m++) {
+                if (m >= blk_n[k]) break;
+                R[blk_r[k][m]] *= fac[k];
+            }
+        }
+
+        // =============================== reverse ===============================
+        double R_b[NRATE];
+#pragma unroll
+        for (int j = 0; j < NRATE; j++) R_b[j] = 0.0;
+        // tendencies q_t = max(sum, -dq_dt)
+#pragma unroll
+        for (int n = 1; n < 6; n++) {
+            const int k = tend_blk[n];
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < BLKMAX; m++) {
+                if (m >= blk_n[k]) break;
+                s = (m == 0) ? blk_s[k][m] * R[blk_r[k][m]] : s + blk_s[k][m] * R[blk_r[k][m]];
+            }
+            qt[n] = (s < -dqdt[n]) ? -dqdt[n] : s;
+            if (s < -dqdt[n]) {
+                dqdt_b[n] -= qt_b[n];
+            } else {
+#pragma unroll
+                for (int m = 0; m < BLKMAX; m++) {
+                    if (m >= blk_n[k]) break;
+                    R_b[blk_r[k][m]] += blk_s[k][m] * qt_b[n];
+                }
+            }
+        }
+        // limiter blocks in reverse
+#pragma unroll
+        for (int k = NBLK - 1; k >= 0; k--) {
+            double fac_b = 0.0;
+#pragma unroll
+            for (int m = 0; m < BLKMAX; m++) {
+                if (m >= blk_n[k]) break;
+                const int j = blk_r[k][m];
+                fac_b += R_b[j] * Rin[k][m];
+                R_b[j] *= fac[k];
+            }
+            if (ratio[k]) {
+                // fac = -dq_dt / net
+                const double n_ = net[k];
+                dqdt_b[blk_q[k]] -= fac_b / n_;
+                const double net_b = fac_b * dqdt[blk_q[k]] / (n_ * n_);
+#pragma unroll
+                for (int m = 0; m < BLKMAX; m++) {
+                    if (m >= blk_n[k]) break;
+                    R_b[blk_r[k][m]] += blk_s[k][m] * net_b;
+                }
+            }
+        }
+        // splits
+        const double Piacr1_b = (1.0 - delta1) * R_b[rPiacr_s] + delta1 * R_b[rPiacr_g];
+        const double Praci1_b = (1.0 - delta1) * R_b[rPraci_s] + delta1 * R_b[rPraci_g];
+        const double Psacr1_b = delta2 * R_b[rPsacr_s] + (1.0 - delta2) * R_b[rPsacr_g];
+        const double Pracs1_b = (1.0 - delta2) * R_b[rPracs];
+        // limiters: cotangent of the quantity before min (raw) or of dq_dt
+        auto unlim = [&](int j, double rb) {
+            const double v = rb * cst[j];
+            if (lim[j])
+                dqdt_b[limq[j]] += v;
+            else
+                raw_b[j] += v;
+        };
+        const int plain[] = {rPigen, rPidep, rPsdep, rPgdep, rPraut, rPracw, rPihom, rPihtr, rPsacw, rPsfw, rPgacw,
+                             rPrevp, rPgacr, rPgfrz, rPisub, rPimlt, rPsaut, rPsaci, rPsfi, rPgaci, rPssub, rPsmlt,
+                             rPgaut, rPgacs, rPgsub, rPgmlt};
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(plain) / sizeof(plain[0])); i++) unlim(plain[i], R_b[plain[i]]);
+        // split rates share one limited quantity: constant ie, limiter of the pair
+        {
+            const double v1 = Piacr1_b * ie;
+            if (lim[rPiacr_s]) dqdt_b[QR] += v1; else raw_b[rPiacr_s] += v1;
+            const double v2 = Psacr1_b * ie;
+            if (lim[rPsacr_s]) dqdt_b[QR] += v2; else raw_b[rPsacr_s] += v2;
+            const double v3 = Praci1_b * ie;
+            if (lim[rPraci_s]) dqdt_b[QI] += v3; else raw_b[rPraci_s] += v3;
+            const double v4 = Pracs1_b * ie;
+            if (lim[rPracs]) dqdt_b[QS] += v4; else raw_b[rPracs] += v4;
+        }
     }
+    // ---- phase 2: the forward values again, from inputs the compiler cannot see
+    // through (so it recomputes them rather than keeping phase 1 values live), then
+    // the reverse of the rate formulas
+    const RateIn<double> in = opaque_copy(in0);
+#include "nsw6_rates_fwd.inc"
     // raw rates -> their pre-limiter forms
     double Pigen0_b = raw_b[rPigen], Pidep0_b = raw_b[rPidep], Psdep0_b = raw_b[rPsdep], Pgdep0_b = raw_b[rPgdep];
     double Praut0_b = raw_b[rPraut], Pihom0_b = raw_b[rPihom], Pihtr0_b = raw_b[rPihtr];
@@ -1555,163 +1422,110 @@ struct PointOutB {
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
 {
-    // ---- forward recomputation
-    double rq0[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) rq0[n] = x.rhogq[n];
-    bool pos[NQ];
-    double rq[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) rq[n] = rq0[n];
-    double diffq = 0.0;
-#pragma unroll
-    for (int n = 2; n <= 5; n++) {
-        diffq += rq[n];
-        pos[n] = !(rq[n] < 0.0);
-        rq[n] = pos[n] ? rq[n] : 0.0;
-    }
-#pragma unroll
-    for (int n = 2; n <= 5; n++) diffq -= rq[n];
-    const double a_qv = rq[QV] + diffq;
-    pos[QV] = !(a_qv < 0.0);
-    rq[QV] = pos[QV] ? a_qv : 0.0;
-    const double diffq2 = a_qv - rq[QV];
-    const double rhog = x.rhog * (1.0 - diffq2);
-    const double rho = rhog / x.gsgam2;
-    double q[NQ];
-    q[QV] = x.qv;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) q[n] = rq[n] / rhog;
-    double qd = 1.0;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) qd = qd - q[n];
-    double cva = qd * C_CVdry;
-#pragma unroll
-    for (int n = 1; n <= 5; n++) cva = cva + q[n] * C_CVW;
-    const double tem = x.tem;
-    const double rhoge0 = tem * rhog * cva;
-    const double Rg = qd * C_Rdry + q[QV] * C_Rvap;
-    const double pre = rho * Rg * tem;
-    const double psl = psat_of(P.LovR_liq, P.RTEM00, tem), psi = psat_of(P.LovR_ice, P.RTEM00, tem);
-    const double qden = rho * C_Rvap * tem;
-    const double qsatl = psl / qden, qsati = psi / qden;
-    RateIn<double> in;
-    in.dens = rho;
-    in.temp = tem;
-#pragma unroll
-    for (int n = 0; n < NQ; n++) in.q[n] = q[n];
-    in.qsatl = qsatl;
-    in.qsati = qsati;
-    in.pre = pre;
-    in.rhog = rhog;
-    in.a1 = 1.0;
-    in.a2 = 0.0;
-    in.ma2 = 1.0;
-    in.sw = rate_switches(P, rho, tem, q, qsati);
-    rate_differences<double, double>(in, tem, q, qsatl, qsati);
-#pragma unroll
-    for (int i = 0; i < NCOEF; i++) in.tun[i] = tun[i];
-    RateOut<double> o;
-    rates<double>(P, dt, in, o);
-    double qt[NQ];
-#pragma unroll
-    for (int n = 1; n < NQ; n++) qt[n] = o.qt[n];
-    qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
-
-    // ---- reverse
-    // rhoge = rhoge0 - LHV0 drq[QV] + LHF0 (drq[QI] + drq[QS] + drq[QG]); rq_out = rq + drq
-    double drq_b[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) drq_b[n] = yb.rhogq[n];
-    drq_b[QV] -= yb.rhoge * C_LHV0;
-    drq_b[QI] += yb.rhoge * C_LHF0;
-    drq_b[QS] += yb.rhoge * C_LHF0;
-    drq_b[QG] += yb.rhoge * C_LHF0;
-    double rq_b[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) rq_b[n] = yb.rhogq[n];
-    double rhoge0_b = yb.rhoge;
-    double rhog_b = yb.rhog;
-    // drq = rhog qt dt
-    double qt_b[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) {
-        rhog_b += drq_b[n] * qt[n] * dt;
-        qt_b[n] = drq_b[n] * rhog * dt;
-    }
-#pragma unroll
-    for (int n = 1; n < NQ; n++) qt_b[n] -= qt_b[QV];
-    // the rate kernel
+    // ---- forward preprocessing, reverse of the state update, the rate kernel
+    double drq_b[NQ], rq_b[NQ], rhoge0_b, rhog_b;
     RateInB ib;
-    memset(&ib, 0, sizeof ib);
-    rates_b(P, dt, in, qt_b, yb.vt, ib);
-#pragma unroll
-    for (int i = 0; i < NCOEF; i++) tun_b[i] += ib.tun[i];
-    // differences formed from the inputs
-    double q_b[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; n++) q_b[n] = ib.q[n];
-    double tem_b = ib.temp + ib.temc;
-    double rho_b = ib.dens, pre_b = ib.pre, qsatl_b = ib.qsatl, qsati_b = ib.qsati;
+    zero_words(ib);
     {
-        const double qv = (q[QV] < 0.0) ? 0.0 : q[QV];
-        const bool qvpos = !(q[QV] < 0.0);
-        double qv_b = 0.0;
-        // qv_qsati = qv - qsati
-        qv_b += ib.qv_qsati;
-        qsati_b -= ib.qv_qsati;
-        // sice_m1 = qv / max(qsati, EPS) - 1
-        const double dqi = dmax(qsati, C_EPS);
-        qv_b += ib.sice_m1 / dqi;
-        if (!(qsati < C_EPS)) qsati_b -= ib.sice_m1 * (qv / dqi) / dqi;
-        // sliq_1m = 1 - min(qv / max(qsatl, EPS), 1)
-        const double dql = dmax(qsatl, C_EPS);
-        const double Sliq = qv / dql;
-        if (!(1.0 < Sliq)) {
-            qv_b -= ib.sliq_1m / dql;
-            if (!(qsatl < C_EPS)) qsatl_b += ib.sliq_1m * Sliq / dql;
+#include "nsw6_point_fwd.inc"
+        // ---- reverse (the forward tendencies qt come out of rates_b's recomputation)
+        // rhoge = rhoge0 - LHV0 drq[QV] + LHF0 (drq[QI] + drq[QS] + drq[QG]); rq_out = rq + drq
+#pragma unroll
+        for (int n = 0; n < NQ; n++) drq_b[n] = yb.rhogq[n];
+        drq_b[QV] -= yb.rhoge * C_LHV0;
+        drq_b[QI] += yb.rhoge * C_LHF0;
+        drq_b[QS] += yb.rhoge * C_LHF0;
+        drq_b[QG] += yb.rhoge * C_LHF0;
+#pragma unroll
+        for (int n = 0; n < NQ; n++) rq_b[n] = yb.rhogq[n];
+        rhoge0_b = yb.rhoge;
+        rhog_b = yb.rhog;
+        // drq = rhog qt dt, qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG])
+        double qt_b[NQ];
+#pragma unroll
+        for (int n = 0; n < NQ; n++) qt_b[n] = drq_b[n] * rhog * dt;
+#pragma unroll
+        for (int n = 1; n < NQ; n++) qt_b[n] -= qt_b[QV];
+        // the rate kernel
+        double qt[NQ];
+        rates_b(P, dt, in, qt_b, yb.vt, ib, qt);
+        qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
+#pragma unroll
+        for (int n = 0; n < NQ; n++) rhog_b += drq_b[n] * qt[n] * dt;
+#pragma unroll
+        for (int i = 0; i < NCOEF; i++) tun_b[i] += ib.tun[i];
+    }
+    // ---- the preprocessing again, from inputs the compiler cannot see through
+    const PointIn<double> xo = opaque_point(x);
+    {
+        const PointIn<double> &x = xo;
+#include "nsw6_point_fwd.inc"
+        // differences formed from the inputs
+        double q_b[NQ];
+#pragma unroll
+        for (int n = 0; n < NQ; n++) q_b[n] = ib.q[n];
+        double tem_b = ib.temp + ib.temc;
+        double rho_b = ib.dens, pre_b = ib.pre, qsatl_b = ib.qsatl, qsati_b = ib.qsati;
+        {
+            const double qv = (q[QV] < 0.0) ? 0.0 : q[QV];
+            const bool qvpos = !(q[QV] < 0.0);
+            double qv_b = 0.0;
+            // qv_qsati = qv - qsati
+            qv_b += ib.qv_qsati;
+            qsati_b -= ib.qv_qsati;
+            // sice_m1 = qv / max(qsati, EPS) - 1
+            const double dqi = dmax(qsati, C_EPS);
+            qv_b += ib.sice_m1 / dqi;
+            if (!(qsati < C_EPS)) qsati_b -= ib.sice_m1 * (qv / dqi) / dqi;
+            // sliq_1m = 1 - min(qv / max(qsatl, EPS), 1)
+            const double dql = dmax(qsatl, C_EPS);
+            const double Sliq = qv / dql;
+            if (!(1.0 < Sliq)) {
+                qv_b -= ib.sliq_1m / dql;
+                if (!(qsatl < C_EPS)) qsatl_b += ib.sliq_1m * Sliq / dql;
+            }
+            if (qvpos) q_b[QV] += qv_b;
         }
-        if (qvpos) q_b[QV] += qv_b;
-    }
-    // qsat = psat(tem) / (rho Rvap tem)
-    {
-        const double dpsl = (tem > TEM_MIN) ? psl * P.LovR_liq / (tem * tem) : 0.0;
-        const double dpsi = (tem > TEM_MIN) ? psi * P.LovR_ice / (tem * tem) : 0.0;
-        tem_b += qsatl_b * (dpsl / qden - qsatl / tem) + qsati_b * (dpsi / qden - qsati / tem);
-        rho_b -= (qsatl_b * qsatl + qsati_b * qsati) / rho;
-    }
-    // pre = rho Rg tem, Rg = qd Rdry + q[QV] Rvap
-    rho_b += pre_b * Rg * tem;
-    tem_b += pre_b * rho * Rg;
-    const double Rg_b = pre_b * rho * tem;
-    double qd_b = Rg_b * C_Rdry;
-    q_b[QV] += Rg_b * C_Rvap;
-    // rhoge0 = tem rhog cva (cva does not depend on q: CVW = CVdry)
-    tem_b += rhoge0_b * rhog * cva;
-    rhog_b += rhoge0_b * tem * cva;
+        // qsat = psat(tem) / (rho Rvap tem)
+        {
+            const double dpsl = (tem > TEM_MIN) ? psl * P.LovR_liq / (tem * tem) : 0.0;
+            const double dpsi = (tem > TEM_MIN) ? psi * P.LovR_ice / (tem * tem) : 0.0;
+            tem_b += qsatl_b * (dpsl / qden - qsatl / tem) + qsati_b * (dpsi / qden - qsati / tem);
+            rho_b -= (qsatl_b * qsatl + qsati_b * qsati) / rho;
+        }
+        // pre = rho Rg tem, Rg = qd Rdry + q[QV] Rvap
+        rho_b += pre_b * Rg * tem;
+        tem_b += pre_b * rho * Rg;
+        const double Rg_b = pre_b * rho * tem;
+        double qd_b = Rg_b * C_Rdry;
+        q_b[QV] += Rg_b * C_Rvap;
+        // rhoge0 = tem rhog cva (cva does not depend on q: CVW = CVdry)
+        tem_b += rhoge0_b * rhog * cva;
+        rhog_b += rhoge0_b * tem * cva;
 #pragma unroll
-    for (int n = 1; n <= 5; n++) q_b[n] -= qd_b;
-    // q[n] = rq[n] / rhog
+        for (int n = 1; n <= 5; n++) q_b[n] -= qd_b;
+        // q[n] = rq[n] / rhog
 #pragma unroll
-    for (int n = 1; n <= 5; n++) {
-        rq_b[n] += q_b[n] / rhog;
-        rhog_b -= q_b[n] * q[n] / rhog;
-    }
-    xb.qv += q_b[QV];
-    xb.tem += tem_b;
-    rhog_b += rho_b / x.gsgam2;
-    // rhog = rhog_in (1 - diffq2)
-    xb.rhog += rhog_b * (1.0 - diffq2);
-    const double diffq2_b = -rhog_b * x.rhog;
-    // negative filter: a = rq0[QV] + sum (rq0[n] - clip(rq0[n])); rq[QV] = clip(a); diffq2 = a - clip(a)
-    const double ma = pos[QV] ? 1.0 : 0.0;
-    const double a_b = rq_b[QV] * ma + diffq2_b * (1.0 - ma);
-    xb.rhogq[QV] += a_b;
-    xb.rhogq[QC] += rq_b[QC];
+        for (int n = 1; n <= 5; n++) {
+            rq_b[n] += q_b[n] / rhog;
+            rhog_b -= q_b[n] * q[n] / rhog;
+        }
+        xb.qv += q_b[QV];
+        xb.tem += tem_b;
+        rhog_b += rho_b / x.gsgam2;
+        // rhog = rhog_in (1 - diffq2)
+        xb.rhog += rhog_b * (1.0 - diffq2);
+        const double diffq2_b = -rhog_b * x.rhog;
+        // negative filter: a = rq0[QV] + sum (rq0[n] - clip(rq0[n])); rq[QV] = clip(a); diffq2 = a - clip(a)
+        const double ma = pos[QV] ? 1.0 : 0.0;
+        const double a_b = rq_b[QV] * ma + diffq2_b * (1.0 - ma);
+        xb.rhogq[QV] += a_b;
+        xb.rhogq[QC] += rq_b[QC];
 #pragma unroll
-    for (int n = 2; n <= 5; n++) {
-        const double m = pos[n] ? 1.0 : 0.0;
-        xb.rhogq[n] += rq_b[n] * m + a_b * (1.0 - m);
+        for (int n = 2; n <= 5; n++) {
+            const double m = pos[n] ? 1.0 : 0.0;
+            xb.rhogq[n] += rq_b[n] * m + a_b * (1.0 - m);
+        }
     }
 }
 

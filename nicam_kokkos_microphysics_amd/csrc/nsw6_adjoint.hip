@@ -34,7 +34,9 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjLaunch L)
     adj_cell(L.A, lb.l, lb.k, ij);
 }
 
-__global__ void __launch_bounds__(256) k_adj_wall(AdjLaunch L)
+// Two waves per SIMD: ~32 doubles spill to scratch, which costs less than the
+// latency exposed at one wave per SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_adj_wall(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
@@ -69,22 +71,20 @@ __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
     }
 }
 
-// One block: sum of the per-block coefficient cotangents in a fixed order.
+// One block per coefficient: sum of the per-block partials in a fixed order.
 __global__ void __launch_bounds__(256) k_adj_coef(AdjLaunch L, int nblocks)
 {
     __shared__ double red[256];
-    for (int c = 0; c < NCOEF; c++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) s += L.coef_part[(int64_t)b * NCOEF + c];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) L.grad_coef[c] = red[0];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += L.coef_part[(int64_t)b * NCOEF + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
+    if (threadIdx.x == 0) L.grad_coef[c] = red[0];
 }
 
 }  // namespace
@@ -108,7 +108,7 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_point, dim3(nb), dim3(256), 0, stream, L);
-    hipLaunchKernelGGL(k_adj_coef, dim3(1), dim3(256), 0, stream, L, nb);
+    hipLaunchKernelGGL(k_adj_coef, dim3(NCOEF), dim3(256), 0, stream, L, nb);
     if (ev) (void)hipEventRecord(ev[2], stream);
     return hipGetLastError();
 }

@@ -152,6 +152,22 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
     for (int i = 0; i < NDI; i++) A.din[i * N2 + p] = din[i];
 }
 
+// CellInB in wall-record order
+NSW6_HD void put_cell_b(const CellInB &b, double *o)
+{
+    o[0] = b.rhog;
+#pragma unroll
+    for (int n = 0; n < 5; n++) o[1 + n] = b.rq[n];
+    o[6] = b.rhoge;
+    o[7] = b.rvx;
+    o[8] = b.rvy;
+    o[9] = b.rvz;
+    o[10] = b.rw;
+    o[11] = b.vx;
+    o[12] = b.vy;
+    o[13] = b.vz;
+}
+
 NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 {
     const StepArgs &a = A.a;
@@ -239,23 +255,8 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
             rgm_b -= bq_b[t] * s.m.q[t] / s.m.rhog;
         }
         wr[WR_M + 0] = rgm_b;
-        const CellInB *rbs[2] = {&rb0, &rb1};
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const CellInB &b = *rbs[c];
-            double *o = wr + (c ? WR_C1 : WR_C0);
-            o[0] = b.rhog;
-#pragma unroll
-            for (int n = 0; n < 5; n++) o[1 + n] = b.rq[n];
-            o[6] = b.rhoge;
-            o[7] = b.rvx;
-            o[8] = b.rvy;
-            o[9] = b.rvz;
-            o[10] = b.rw;
-            o[11] = b.vx;
-            o[12] = b.vy;
-            o[13] = b.vz;
-        }
+        put_cell_b(rb0, wr + WR_C0);
+        put_cell_b(rb1, wr + WR_C1);
     }
 #pragma unroll
     for (int i = 0; i < NWR; i++) A.wr[i * N2 + pw] = wr[i];

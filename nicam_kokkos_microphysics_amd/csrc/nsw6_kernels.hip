@@ -43,7 +43,9 @@ __constant__ double c_a2_tab[32] = {
 // ---------------------------------------------------------------------------------
 // k_rates
 // ---------------------------------------------------------------------------------
-template <typename R>
+// DIAG = false (the forward part of the reverse mode) skips the effective radii and
+// cloud-top diagnostics, which carry no cotangent.
+template <typename R, bool DIAG>
 __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -70,8 +72,10 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
         for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
 #pragma unroll
         for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = 0.0;
-        f.re_liquid[p] = 0.0;
-        f.re_cld[p] = 0.0;
+        if (DIAG) {
+            f.re_liquid[p] = 0.0;
+            f.re_cld[p] = 0.0;
+        }
         return;
     }
 
@@ -87,8 +91,6 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     tunables(P, tun);
     PointOut<double> y;
     point_rates<double, R>(P, a.dt, x, tun, c_a1_tab, c_a2_tab, y);
-    double rceff, rceff_cld, rct, tct;
-    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct);
 
     a.ws.rhog[p] = y.rhog;
     a.ws.rhoge[p] = y.rhoge;
@@ -96,6 +98,9 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = y.rhogq[n];
 #pragma unroll
     for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
+    if (!DIAG) return;
+    double rceff, rceff_cld, rct, tct;
+    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct);
     f.re_liquid[p] = rceff;
     f.re_cld[p] = rceff_cld;
     if (k == KMAX) {
@@ -934,9 +939,9 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (precision == 1)
-        hipLaunchKernelGGL(k_rates<float>, grid_p, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_rates<float, true>), grid_p, dim3(256), 0, stream, a);
     else
-        hipLaunchKernelGGL(k_rates<double>, grid_p, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_rates<double, true>), grid_p, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
     hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
@@ -957,7 +962,7 @@ extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStrea
     const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
     const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
-    hipLaunchKernelGGL(k_rates<double>, grid_p, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((k_rates<double, false>), grid_p, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
     hipLaunchKernelGGL(k_flux, grid_l, dim3(256), 0, stream, a);
