@@ -93,6 +93,10 @@ struct DevParams {
     double Dc;                 // 0.146 - 5.964e-2 * log(Nc_def / 2000)
     double LovR_liq, LovR_ice, RTEM00;
     double dtemp_criteria;     // 10^(-(15+1)/2) = 1e-8
+    // SATURATION_adjustment path (src/mod_mp_nsw6.cpp:647-660): satadjust_all with
+    // SATURATION_Setrange(273.16, 233.16) when explicit ice generation is off
+    // (sw_expice != 1), else satadjust_liq (range 100, 90)
+    double ice_adjust, sat_ulimit, sat_llimit;
 };
 
 NSW6_HD double dmax(double a, double b) { return (a < b) ? b : a; }

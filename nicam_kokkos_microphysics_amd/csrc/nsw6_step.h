@@ -535,6 +535,56 @@ NSW6_HD int satadjust_liq_newton(const DevParams &P, double &tem, double q[6], d
     return -2;
 }
 
+// satadjust_all (src/mod_satadjust.cpp:620-769): the mixed-phase Newton.  The liquid
+// fraction alpha = clamp((T - LL) / (UL - LL), 0, 1) splits the condensate between
+// QC and QI and blends the liquid and ice saturation pressures; E(T) carries the
+// fusion heat of the ice part.  Same return codes as satadjust_liq_newton; q holds
+// the last iterate's QV, QC, QI.
+NSW6_HD int satadjust_all_newton(const DevParams &P, double &tem, double q[6], double qsum, double qdl,
+                                 double rho_sa, double ein_moist)
+{
+    const double UL = P.sat_ulimit, LL = P.sat_llimit;
+    double alpha = dmin(dmax((tem - LL) / (UL - LL), 0.0), 1.0);
+    double rtem = 1.0 / dmax(tem, TEM_MIN);
+    double psatl = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+    double psati = C_PSAT0 * 1.0 * exp(P.LovR_ice * (P.RTEM00 - rtem));
+    double psat = psatl * alpha + psati * (1.0 - alpha);
+    double qsat = psat / (rho_sa * C_Rvap * tem);
+    if (!(qsum - qsat > C_EPS)) return 0;
+    for (int ite = 0; ite < 100; ite++) {
+        alpha = dmin(dmax((tem - LL) / (UL - LL), 0.0), 1.0);
+        rtem = 1.0 / dmax(tem, TEM_MIN);
+        psatl = C_PSAT0 * 1.0 * exp(P.LovR_liq * (P.RTEM00 - rtem));
+        psati = C_PSAT0 * 1.0 * exp(P.LovR_ice * (P.RTEM00 - rtem));
+        psat = psatl * alpha + psati * (1.0 - alpha);
+        const double qsatl = psatl / (rho_sa * C_Rvap * tem);
+        const double qsati = psati / (rho_sa * C_Rvap * tem);
+        qsat = psat / (rho_sa * C_Rvap * tem);
+        q[QV] = qsat;
+        q[QC] = (qsum - qsat) * alpha;
+        q[QI] = (qsum - qsat) * (1.0 - alpha);
+        double CVt = qdl * C_CVdry;
+#pragma unroll
+        for (int n = 1; n <= 5; n++) CVt = CVt + q[n] * C_CVW;
+        const double Emoist_new = tem * CVt + q[QV] * C_LHV0 - q[QI] * C_LHF0;
+        const double lim1 = 0.5 + copysign(0.5, UL - tem);
+        const double lim2 = 0.5 + copysign(0.5, tem - LL);
+        const double dalpha_dT = lim1 * lim2 / (UL - LL);
+        const double dqsatl_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsatl;
+        const double dqsati_dT = (P.LovR_ice / (tem * tem) + 0.0 / tem) * qsati;
+        const double dqsat_dT = qsatl * dalpha_dT + dqsatl_dT * alpha - qsati * dalpha_dT + dqsati_dT * (1.0 - alpha);
+        const double dqc_dT = (qsum - qsat) * dalpha_dT - dqsat_dT * alpha;
+        const double dqi_dT = -(qsum - qsat) * dalpha_dT - dqsat_dT * (1.0 - alpha);
+        const double dCVtot_dT = dqsat_dT * C_CVW + dqc_dT * C_CVW + dqi_dT * C_CVW;
+        const double dEmoist_dT = tem * dCVtot_dT + CVt + dqsat_dT * C_LHV0 - dqi_dT * C_LHF0;
+        const double dtemp = (Emoist_new - ein_moist) / dEmoist_dT;
+        tem = tem - dtemp;
+        if (fabs(dtemp) < P.dtemp_criteria) return ite + 1;
+        if (tem * 0.0 != 0.0) return -1;
+    }
+    return -2;
+}
+
 // Residual of the satadjust_liq solve, E(T) - Emoist, for the implicit-function
 // derivative of the converged temperature: with q(QC) = qsum - qsat(T) and the
 // reference's heat capacity sum, E(T) = T (qdl CVdry + (qsum - qsat + qrest) CVW) +
@@ -598,11 +648,14 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
     S tem = ein / cv;
     S pre = rho * tem * (qd2 * C_Rdry + q[QV] * C_Rvap);
 
-    // SATURATION_adjustment, ice_adjust = false
-    const S ein_moist = rhoge / rhog + q[QV] * C_LHV0;
-    const S qsum = q[QV] + q[QC];
+    // SATURATION_adjustment (src/mod_satadjust.cpp:274-362): satadjust_liq, or
+    // satadjust_all when explicit ice generation is off
+    const bool ice = P.ice_adjust != 0.0;
+    const S ein_moist = ice ? S(rhoge / rhog + q[QV] * C_LHV0 - q[QI] * C_LHF0) : S(rhoge / rhog + q[QV] * C_LHV0);
+    const S qsum = ice ? S(q[QV] + q[QC] + q[QI]) : S(q[QV] + q[QC]);
     q[QV] = qsum;
     q[QC] = S(0.0);
+    if (ice) q[QI] = S(0.0);
     S CVtot = qd * C_CVdry;
 #pragma unroll
     for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
@@ -615,16 +668,28 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
         double tv = val(tem), qv6[6];
 #pragma unroll
         for (int n = 0; n < 6; n++) qv6[n] = val(q[n]);
-        st.newton = satadjust_liq_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));
-        if (st.newton != 0) {
-            const S qrest = ((q[QR] + q[QI]) + q[QS]) + q[QG];
-            tem = newton_tem(P, tv, qsum, qdl, qrest, rho_sa, ein_moist);
-            q[QV] = newton_qsat(P, qv6[QV], tem, rho_sa);
-            q[QC] = qsum - q[QV];
+        if (ice) {
+            // (the dual-number check covers the default scheme: values only here)
+            st.newton = satadjust_all_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));
+            if (st.newton != 0) {
+                tem = S(tv);
+                q[QV] = S(qv6[QV]);
+                q[QC] = S(qv6[QC]);
+                q[QI] = S(qv6[QI]);
+            }
+        } else {
+            st.newton = satadjust_liq_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));
+            if (st.newton != 0) {
+                const S qrest = ((q[QR] + q[QI]) + q[QS]) + q[QG];
+                tem = newton_tem(P, tv, qsum, qdl, qrest, rho_sa, ein_moist);
+                q[QV] = newton_qsat(P, qv6[QV], tem, rho_sa);
+                q[QC] = qsum - q[QV];
+            }
         }
     }
     rhogq[QV] = rhog * q[QV];
     rhogq[QC] = rhog * q[QC];
+    if (ice) rhogq[QI] = rhog * q[QI];
     S CVtot2 = qd * C_CVdry;
 #pragma unroll
     for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;

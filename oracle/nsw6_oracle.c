@@ -1217,6 +1217,69 @@ int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_lo
     return code;
 }
 
+/* One point of satadjust_all (src/mod_satadjust.cpp:620-769): the mixed-phase Newton
+ * with the liquid fraction alpha = clamp((T - LL) / (UL - LL), 0, 1).  Returns 0 (not
+ * supersaturated), the iteration count, -1 (NaN) or -2 (itelim); writes the final
+ * tem and the last iterate's q(QV), q(QC), q(QI). */
+int ora_satadjust_all_point(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                            double UL, double LL, double *tem, double *qv, double *qc, double *qi)
+{
+    const double RTEM00 = 1.0 / CONST_TEM00;
+    const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));
+    double qq[6];
+    for (int i = 0; i < 6; i++) qq[i] = q6[i];
+    double t = *tem;
+    double alpha = (t - LL) / (UL - LL);
+    alpha = dmin(dmax(alpha, 0.0), 1.0);
+    double rtem = 1.0 / (dmax(t, TEM_MIN));
+    double psatl = CONST_PSAT0 * pow((t * RTEM00), CPovR_liq) * exp(LovR_liq * (RTEM00 - rtem));
+    double psati = CONST_PSAT0 * pow((t * RTEM00), CPovR_ice) * exp(LovR_ice * (RTEM00 - rtem));
+    double psat = psatl * alpha + psati * (1.0 - alpha);
+    double qsat = psat / (rho * CONST_Rvap * t);
+    int code = 0;
+    if (qsum - qsat > CONST_EPS) {
+        int converged = 0, nan_break = 0, ite;
+        for (ite = 0; ite < 100; ite++) {
+            alpha = (t - LL) / (UL - LL);
+            alpha = dmin(dmax(alpha, 0.0), 1.0);
+            rtem = 1.0 / (dmax(t, TEM_MIN));
+            psatl = CONST_PSAT0 * pow((t * RTEM00), CPovR_liq) * exp(LovR_liq * (RTEM00 - rtem));
+            psati = CONST_PSAT0 * pow((t * RTEM00), CPovR_ice) * exp(LovR_ice * (RTEM00 - rtem));
+            psat = psatl * alpha + psati * (1.0 - alpha);
+            double qsatl = psatl / (rho * CONST_Rvap * t);
+            double qsati = psati / (rho * CONST_Rvap * t);
+            qsat = psat / (rho * CONST_Rvap * t);
+            qq[I_QV] = qsat;
+            qq[I_QC] = (qsum - qsat) * alpha;
+            qq[I_QI] = (qsum - qsat) * (1.0 - alpha);
+            double CVt = qd_loc * CONST_CVdry;
+            for (int nq = NQW_STR; nq <= NQW_END; nq++) CVt = CVt + qq[nq] * CVW[nq];
+            double Emoist_new = t * CVt + qq[I_QV] * LHV - qq[I_QI] * LHF;
+            double lim1 = 0.5 + copysign(0.5, UL - t);
+            double lim2 = 0.5 + copysign(0.5, t - LL);
+            double dalpha_dT = lim1 * lim2 / (UL - LL);
+            double dqsatl_dT = (LovR_liq / (pow(t, 2)) + CVovR_liq / t) * qsatl;
+            double dqsati_dT = (LovR_ice / (pow(t, 2)) + CVovR_ice / t) * qsati;
+            double dqsat_dT = qsatl * dalpha_dT + dqsatl_dT * alpha - qsati * dalpha_dT +
+                              dqsati_dT * (1.0 - alpha);
+            double dqc_dT = (qsum - qsat) * dalpha_dT - dqsat_dT * alpha;
+            double dqi_dT = -(qsum - qsat) * dalpha_dT - dqsat_dT * (1.0 - alpha);
+            double dCVtot_dT = dqsat_dT * CVW[I_QV] + dqc_dT * CVW[I_QC] + dqi_dT * CVW[I_QI];
+            double dEmoist_dT = t * dCVtot_dT + CVt + dqsat_dT * LHV - dqi_dT * LHF;
+            double dtemp = (Emoist_new - Emoist) / dEmoist_dT;
+            t = t - dtemp;
+            if (fabs(dtemp) < dtemp_criteria) { converged = 1; break; }
+            if (t * 0.0 != 0.0) { nan_break = 1; break; }
+        }
+        code = converged ? ite + 1 : (nan_break ? -1 : -2);
+    }
+    *tem = t;
+    *qv = qq[I_QV];
+    *qc = qq[I_QC];
+    *qi = qq[I_QI];
+    return code;
+}
+
 /* ---- SATURATION_adjustment: src/mod_satadjust.cpp:274-362, satadjust_liq :771-896,
  *      satadjust_all :620-769 ---------------------------------------------------- */
 static double g_sat_ulimit = 273.15, g_sat_llimit = 233.15;  /* SATURATION_Setrange state */
@@ -1283,62 +1346,27 @@ static void saturation_adjustment(int ijdim, double *rhog, double *rhoge, double
                 }
             }
     } else {
-        const double RTEM00 = 1.0 / CONST_TEM00;
-        const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));
         const double UL = g_sat_ulimit, LL = g_sat_llimit;
 #pragma omp parallel for collapse(2) reduction(+ : n_nc, n_nan)
         for (int k = KMIN; k <= KMAX; k++)
             for (int ij = 0; ij < ijdim; ij++) {
                 size_t p = IDX(k, ij);
-                double t = tem[p];
-                double alpha = (t - LL) / (UL - LL);
-                alpha = dmin(dmax(alpha, 0.0), 1.0);
-                double rtem = 1.0 / (dmax(t, TEM_MIN));
-                double psatl = CONST_PSAT0 * pow((t * RTEM00), CPovR_liq) * exp(LovR_liq * (RTEM00 - rtem));
-                double psati = CONST_PSAT0 * pow((t * RTEM00), CPovR_ice) * exp(LovR_ice * (RTEM00 - rtem));
-                double psat = psatl * alpha + psati * (1.0 - alpha);
-                double qsat = psat / (rho[p] * CONST_Rvap * t);
-                if (qsum[p] - qsat > CONST_EPS) {
-                    int converged = 0, nan_break = 0, ite;
-                    for (ite = 0; ite < 100; ite++) {
-                        alpha = (t - LL) / (UL - LL);
-                        alpha = dmin(dmax(alpha, 0.0), 1.0);
-                        rtem = 1.0 / (dmax(t, TEM_MIN));
-                        psatl = CONST_PSAT0 * pow((t * RTEM00), CPovR_liq) * exp(LovR_liq * (RTEM00 - rtem));
-                        psati = CONST_PSAT0 * pow((t * RTEM00), CPovR_ice) * exp(LovR_ice * (RTEM00 - rtem));
-                        psat = psatl * alpha + psati * (1.0 - alpha);
-                        double qsatl = psatl / (rho[p] * CONST_Rvap * t);
-                        double qsati = psati / (rho[p] * CONST_Rvap * t);
-                        qsat = psat / (rho[p] * CONST_Rvap * t);
-                        q[QIDX(I_QV, k, ij)] = qsat;
-                        q[QIDX(I_QC, k, ij)] = (qsum[p] - qsat) * alpha;
-                        q[QIDX(I_QI, k, ij)] = (qsum[p] - qsat) * (1.0 - alpha);
-                        double CVt = qd_loc[p] * CONST_CVdry;
-                        for (int nq = NQW_STR; nq <= NQW_END; nq++) CVt = CVt + q[QIDX(nq, k, ij)] * CVW[nq];
-                        double Emoist_new = t * CVt + q[QIDX(I_QV, k, ij)] * LHV - q[QIDX(I_QI, k, ij)] * LHF;
-                        double lim1 = 0.5 + copysign(0.5, UL - t);
-                        double lim2 = 0.5 + copysign(0.5, t - LL);
-                        double dalpha_dT = lim1 * lim2 / (UL - LL);
-                        double dqsatl_dT = (LovR_liq / (pow(t, 2)) + CVovR_liq / t) * qsatl;
-                        double dqsati_dT = (LovR_ice / (pow(t, 2)) + CVovR_ice / t) * qsati;
-                        double dqsat_dT = qsatl * dalpha_dT + dqsatl_dT * alpha - qsati * dalpha_dT +
-                                          dqsati_dT * (1.0 - alpha);
-                        double dqc_dT = (qsum[p] - qsat) * dalpha_dT - dqsat_dT * alpha;
-                        double dqi_dT = -(qsum[p] - qsat) * dalpha_dT - dqsat_dT * (1.0 - alpha);
-                        double dCVtot_dT = dqsat_dT * CVW[I_QV] + dqc_dT * CVW[I_QC] + dqi_dT * CVW[I_QI];
-                        double dEmoist_dT = t * dCVtot_dT + CVt + dqsat_dT * LHV - dqi_dT * LHF;
-                        double dtemp = (Emoist_new - ein_moist[p]) / dEmoist_dT;
-                        t = t - dtemp;
-                        if (fabs(dtemp) < dtemp_criteria) { converged = 1; break; }
-                        if (t * 0.0 != 0.0) { nan_break = 1; break; }
-                    }
-                    tem[p] = t;
-                    if (!converged) {
-                        if (nan_break) n_nan++; else n_nc++;
+                double q6[6], t = tem[p], qv, qc, qi;
+                for (int nq = 0; nq < 6; nq++) q6[nq] = q[QIDX(nq, k, ij)];
+                qv = q6[I_QV];
+                qc = q6[I_QC];
+                qi = q6[I_QI];
+                int ite = ora_satadjust_all_point(rho[p], ein_moist[p], qsum[p], qd_loc[p], q6, UL, LL, &t, &qv,
+                                                  &qc, &qi);
+                tem[p] = t;
+                q[QIDX(I_QV, k, ij)] = qv;
+                q[QIDX(I_QC, k, ij)] = qc;
+                q[QIDX(I_QI, k, ij)] = qi;
+                if (ite < 0) {
+                    if (ite == -1) n_nan++; else n_nc++;
 #pragma omp critical
-                        {
-                            if (fk < 0 || k < fk || (k == fk && ij < fij)) { fk = k; fij = ij; }
-                        }
+                    {
+                        if (fk < 0 || k < fk || (k == fk && ij < fij)) { fk = k; fij = ij; }
                     }
                 }
             }
@@ -1471,15 +1499,14 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
     int flags[NQMAX] = {0, 0, 1, 0, 1, 1};   /* 3WATER: QR, QS, QG */
     precip_transport_new(G, ijdim, R, Vt, flags, rgs, rgsh, dt);
 
-    /* OPT_EXPLICIT_ICEGEN: SATURATION_Setrange(100, 90), ice_adjust = false */
-    int ice_adjust = 0;
-    g_sat_ulimit = 100.0;
-    g_sat_llimit = 90.0;
-    if (ice_adjust_override == 1) {
-        ice_adjust = 1;
-        g_sat_ulimit = 273.16;
-        g_sat_llimit = 233.16;
-    }
+    /* src/mod_mp_nsw6.cpp:647-660: OPT_EXPLICIT_ICEGEN (sw_expice = 1) takes
+     * satadjust_liq with SATURATION_Setrange(100, 90); otherwise satadjust_all with
+     * SATURATION_Setrange(273.16, 233.16).  ice_adjust_override 0 / 1 forces a path. */
+    int ice_adjust = (P->sw_expice != 1.0);
+    if (ice_adjust_override == 0) ice_adjust = 0;
+    if (ice_adjust_override == 1) ice_adjust = 1;
+    g_sat_ulimit = ice_adjust ? 273.16 : 100.0;
+    g_sat_llimit = ice_adjust ? 233.16 : 90.0;
     saturation_adjustment(ijdim, R->rhog, R->rhoge, R->rhogq, R->tem, R->q, R->qd, R->gsgam2,
                           ice_adjust, rep, l);
     negative_filter(ijdim, R->rhog, R->rhoge, R->rhogq, R->rho, R->tem, R->pre, R->q, R->gsgam2);

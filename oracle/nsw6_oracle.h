@@ -50,10 +50,15 @@ void ora_vadv1d_getflux(int ijdim, int mkmin, int mkmax, const double *dz,
  * converged), updates *tem, *qv, *qc (src/mod_satadjust.cpp:828-894). */
 int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_loc,
                             const double q6[6], double *tem, double *qv, double *qc);
+/* One point of satadjust_all (mixed-phase Newton, src/mod_satadjust.cpp:620-769);
+ * returns 0 / iterations / -1 NaN / -2 itelim. */
+int ora_satadjust_all_point(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                            double UL, double LL, double *tem, double *qv, double *qc, double *qi);
 
 /* mp_driver over lall regions of host arrays (src/mod_mp_driver.cpp:335-613).
- * `ice_adjust` < 0 keeps the reference choice (false under Roh & Satoh);
- * 0/1 force satadjust_liq/satadjust_all.  Returns NSW6_OK or a failure status
+ * `ice_adjust` < 0 keeps the reference choice (satadjust_all exactly when
+ * sw_expice != 1, i.e. without explicit ice generation); 0/1 force
+ * satadjust_liq/satadjust_all.  Returns NSW6_OK or a failure status
  * and fills *rep (may be NULL). */
 int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
                   const nsw6_fields *f, double dt, double ct, int ice_adjust,

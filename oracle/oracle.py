@@ -57,6 +57,8 @@ def lib():
         L.ora_vadv1d_getflux.restype = None
         L.ora_satadjust_liq_point.argtypes = [D, D, D, D, P, P, P, P]
         L.ora_satadjust_liq_point.restype = I
+        L.ora_satadjust_all_point.argtypes = [D, D, D, D, P, D, D, P, P, P, P]
+        L.ora_satadjust_all_point.restype = I
         L.ora_mp_driver.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
                                     ctypes.POINTER(N.FieldPtrs), D, D, I, ctypes.POINTER(N.Report)]
         L.ora_mp_driver.restype = I

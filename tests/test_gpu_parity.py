@@ -85,3 +85,16 @@ def test_general_sweep_fast_rain():
     out, ref, reps = _run_both(2, 800, 1, seed=3, params_edit={"Cr": 130.0 * 25})
     assert reps[-1].n_general_regions == 2
     _assert_parity(out, ref)
+
+
+def test_mixed_phase_scheme_without_explicit_ice_generation():
+    # sw_expice = 0: Bergeron growth in the rates and the mixed-phase saturation
+    # adjustment (satadjust_all with SATURATION_Setrange(273.16, 233.16))
+    out, ref, reps = _run_both(1, 1500, 3, seed=9, params_edit={"sw_expice": 0.0}, frac_towers=0.3)
+    assert reps[-1].n_not_converged == 0 and reps[-1].n_nan == 0
+    _assert_parity(out, ref)
+
+
+def test_mixed_phase_scheme_supersaturated():
+    out, ref, _ = _run_both(2, 700, 2, seed=5, params_edit={"sw_expice": 0.0}, frac_supersat=0.3)
+    _assert_parity(out, ref)

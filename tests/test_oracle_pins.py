@@ -159,6 +159,83 @@ def test_satadjust_unsaturated_point_is_unchanged():
     assert ite == 0 and t.value == 290.0 and qv.value == 1e-3 and qc.value == 0.0
 
 
+def _all_point(rho, Em, qsum, q6, T0, UL=273.16, LL=233.16):
+    L = O.lib()
+    q6 = np.ascontiguousarray(q6, dtype=np.float64)
+    t = ctypes.c_double(T0)
+    qv, qc, qi = ctypes.c_double(q6[0]), ctypes.c_double(q6[1]), ctypes.c_double(q6[3])
+    ite = L.ora_satadjust_all_point(rho, Em, qsum, 1.0, q6.ctypes.data, UL, LL, ctypes.byref(t), ctypes.byref(qv),
+                                    ctypes.byref(qc), ctypes.byref(qi))
+    return ite, t.value, qv.value, qc.value, qi.value
+
+
+def test_satadjust_all_reduces_to_liquid_adjustment_above_the_range():
+    """alpha = 1 above the upper limit: the mixed-phase Newton is the liquid one,
+    term for term (dalpha/dT = 0, the ice share and its fusion heat vanish)."""
+    L = O.lib()
+    rho, T0, Rv = 1.0, 290.0, 461.46
+    qsum = 1.1 * L.ora_psat_liq(T0) / (rho * Rv * T0)
+    Em = T0 * 717.56 + qsum * 2501000.0
+    q6 = np.zeros(6)
+    q6[0] = qsum
+    t = ctypes.c_double(T0)
+    qv, qc = ctypes.c_double(qsum), ctypes.c_double(0.0)
+    ite_l = L.ora_satadjust_liq_point(rho, Em, qsum, 1.0, q6.ctypes.data, ctypes.byref(t), ctypes.byref(qv),
+                                      ctypes.byref(qc))
+    ite_a, T, qv_a, qc_a, qi_a = _all_point(rho, Em, qsum, q6, T0)
+    assert ite_l > 0 and ite_a == ite_l
+    assert T == t.value and qv_a == qv.value and qc_a == qc.value and qi_a == 0.0
+
+
+def test_satadjust_all_cold_point_deposits_ice():
+    """alpha = 0 below the lower limit: all condensate is ice, q(QV) = qsat_ice(T),
+    and the moist energy with the fusion term is conserved to the last update."""
+    L = O.lib()
+    rho, T0, Rv, Lv, Lf, cv = 0.5, 220.0, 461.46, 2501000.0, 333000.0, 717.56
+    qsat0 = L.ora_psat_ice(T0) / (rho * Rv * T0)
+    qsum = 1.5 * qsat0
+    Em = T0 * cv + qsum * Lv            # all water as vapour, no ice yet
+    q6 = np.zeros(6)
+    q6[0] = qsum
+    ite, T, qv, qc, qi = _all_point(rho, Em, qsum, q6, T0)
+    assert ite > 0 and T > T0
+    assert qc == 0.0
+    qs = L.ora_psat_ice(T) / (rho * Rv * T)
+    assert abs(qv - qs) <= 1e-7 * qs
+    assert abs(qv + qi - qsum) <= 1e-18
+    E = T * cv * (1.0 + qi) + qv * Lv - qi * Lf
+    dEdT = cv * (1.0 + qi) + (Lv + Lf) * qs * 2834000.0 / Rv / T ** 2
+    assert abs(E - Em) <= 2e-8 * dEdT
+
+
+def test_satadjust_all_mixed_phase_split():
+    """Inside the range the condensate splits as alpha : (1 - alpha) with
+    alpha = (T - LL)/(UL - LL), and q(QV) is the alpha-blend of the two saturation
+    mixing ratios (checked at the final temperature, within the last update)."""
+    L = O.lib()
+    rho, T0, Rv, UL, LL = 0.8, 255.0, 461.46, 273.16, 233.16
+    a0 = (T0 - LL) / (UL - LL)
+    qsat0 = (a0 * L.ora_psat_liq(T0) + (1 - a0) * L.ora_psat_ice(T0)) / (rho * Rv * T0)
+    qsum = 1.3 * qsat0
+    Em = T0 * 717.56 + qsum * 2501000.0
+    q6 = np.zeros(6)
+    q6[0] = qsum
+    ite, T, qv, qc, qi = _all_point(rho, Em, qsum, q6, T0)
+    assert ite > 0
+    a = (T - LL) / (UL - LL)
+    assert abs(qc / (qc + qi) - a) <= 1e-6
+    qs = (a * L.ora_psat_liq(T) + (1 - a) * L.ora_psat_ice(T)) / (rho * Rv * T)
+    assert abs(qv - qs) <= 1e-6 * qs
+    assert abs(qv + qc + qi - qsum) <= 1e-18
+
+
+def test_satadjust_all_unsaturated_point_is_unchanged():
+    q6 = np.zeros(6)
+    q6[0] = 1e-6
+    ite, T, qv, qc, qi = _all_point(1.0, 0.0, 1e-6, q6, 250.0)
+    assert ite == 0 and T == 250.0 and qv == 1e-6 and qc == 0.0 and qi == 0.0
+
+
 # ---------------------------------------------------------------- Bergeron
 def test_bergeron_uses_reference_tables_with_offset_interpolation():
     L = O.lib()
