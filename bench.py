@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--only", action="store_true",
                     help="measure the headline form only (no second form, no adjoint; for profiling)")
     ap.add_argument("--adjoint-reps", type=int, default=10)
+    ap.add_argument("--rollout", action="store_true",
+                    help="instead of the step benchmark: the long rollout (40 regions x 16,900 columns x 96 "
+                         "levels, 20 mixed-precision steps, checkpoints every 5 steps, FP64 reverse mode over "
+                         "all steps)")
     ap.add_argument("--workload", default=None)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
@@ -216,12 +220,56 @@ def adjoint_cost(reps):
                       f"HIP events on the launch stream"}
 
 
+def rollout_bench():
+    """The long rollout: forward (mixed precision) and backward (recompute + FP64 reverse
+    mode, 10 regions at a time) wall times, and the peak device memory."""
+    import torch
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.rollout import Rollout
+    w = F.WORKLOADS["nsw6_676000x96_rollout"]
+    every = 5
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    del st
+    rhoge0 = dev["rhoge"].clone()
+    ro = Rollout(gz, gzh, w.lall, w.ijdim, precision=N.NSW6_MIXED, chunk_regions=10)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    ckpt, precip = ro.forward(dev, w.dt, w.steps, every)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L = ro.loss(rhoge0, dev, precip, w.dt)
+    grad, gcoef = ro.backward(dev, ckpt, precip, w.dt, w.steps, every)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ncol = w.lall * w.ijdim
+    gnorm = float(sum(float((g * g).sum()) for g in grad.values())) ** 0.5
+    out = {"metric": "long rollout: column-steps/s forward and backward", "workload": w.name,
+           "columns": ncol, "levels": 96, "steps": w.steps, "checkpoint_every": every,
+           "forward_s": t1 - t0, "backward_s": t2 - t1,
+           "forward_column_steps_per_s": ncol * w.steps / (t1 - t0),
+           "backward_column_steps_per_s": ncol * w.steps / (t2 - t1),
+           "backward_over_forward": (t2 - t1) / (t1 - t0),
+           "loss": L, "grad_norm": gnorm, "grad_coef": [float(x) for x in gcoef.cpu()],
+           "peak_device_GiB": torch.cuda.max_memory_allocated() / 2 ** 30,
+           "forward_precision": "mixed", "reverse_precision": "f64", "data": "synthetic",
+           "note": "backward = per-segment recomputation from the checkpoint + FP64 reverse mode per step, "
+                   "10 regions per reverse-mode call; includes the per-call status synchronisation"}
+    ro.close()
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the step shards by NICAM region with no "
                                    "communication, and multi-GPU runs are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.rollout:
+        rollout_bench()
+        return
     import torch
     from nicam_kokkos_microphysics_amd import _native as N
     from nicam_kokkos_microphysics_amd import fields as F
