@@ -93,6 +93,11 @@ struct DevParams {
     double Dc;                 // 0.146 - 5.964e-2 * log(Nc_def / 2000)
     double LovR_liq, LovR_ice, RTEM00;
     double dtemp_criteria;     // 10^(-(15+1)/2) = 1e-8
+    // reciprocals and ratios of parameters, formed once on the host so that the
+    // point-wise code multiplies instead of dividing
+    double rK_r, rK_s, rK_g;    // 1/(Ar N0r GAM_1br), 1/(As N0s GAM_1bs), 1/(Ag N0g GAM_1bg)
+    double rAs, rmi, rNc_ihtr;  // 1/As, 1/mi, 1/(Nc_ihtr 1e6)
+    double Vtr_fac, Vtg_fac;    // GAM_1brdr/GAM_1br, GAM_1bgdg/GAM_1bg
     // SATURATION_adjustment path (src/mod_mp_nsw6.cpp:647-660): satadjust_all with
     // SATURATION_Setrange(273.16, 233.16) when explicit ice generation is off
     // (sw_expice != 1), else satadjust_liq (range 100, 90)
@@ -158,7 +163,7 @@ NSW6_HD RateSwitches rate_switches(const DevParams &P, double dens, double temp,
     s.sw_bergeron = step_pos(temc + 30.0) * step_pos(0.0 - temc) * (1.0 - P.sw_expice);
     s.zr = step_neg(qr - 1.0E-12);
     s.zs = step_neg(qs - 1.0E-12);
-    s.zx = step_neg(dens * qs / P.As - 1.0E-12);
+    s.zx = step_neg(dens * qs * P.rAs - 1.0E-12);
     s.zg = step_neg(qg - 1.0E-12);
     s.zcr = step_neg(qc * qr - 1.0E-12);
     s.sw_hom = step_neg(temc + 40.0);
@@ -175,7 +180,7 @@ NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double
 {
     const double qc = dmax(q[QC], 0.0), qr = dmax(q[QR], 0.0);
     const double zr = sw.zr;
-    const double RLMDr = sqrt(sqrt(dens * qr / (P.Ar * P.N0r * P.GAM_1br) + zr)) * (1.0 - zr);
+    const double RLMDr = sqrt(sqrt(dens * qr * P.rK_r + zr)) * (1.0 - zr);
     const double RLMDr_3 = (RLMDr * RLMDr) * RLMDr;
     const double Nc = P.Nc_def;
     const double rhoqc = dens * qc * 1000.0;
@@ -264,14 +269,15 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     const R delta2 = R(in.sw.delta2);
     const R spsati = R(in.sw.spsati);
     const R iceflg = R(in.sw.iceflg);
-    const R dqv_dt = qv / dt, dqc_dt = qc / dt, dqr_dt = qr / dt;
-    const R dqi_dt = qi / dt, dqs_dt = qs / dt, dqg_dt = qg / dt;
+    const R rdt = R(1) / dt;
+    const R dqv_dt = qv * rdt, dqc_dt = qc * rdt, dqr_dt = qr * rdt;
+    const R dqi_dt = qi * rdt, dqs_dt = qs * rdt, dqg_dt = qg * rdt;
     const R sw_expice = R(P.sw_expice);
     const R sw_bergeron = R(in.sw.sw_bergeron);
 
     // slope parameter lambda (rain)
     R zerosw = R(in.sw.zr);
-    const R RLMDr = sqrt(sqrt(dens * qr / (Ar * N0r * R(P.GAM_1br)) + zerosw)) * (R(1) - zerosw);
+    const R RLMDr = sqrt(sqrt(dens * qr * R(P.rK_r) + zerosw)) * (R(1) - zerosw);
     const R RLMDr_dr = sqrt(RLMDr);
     const R RLMDr_2 = RLMDr * RLMDr;
     const R RLMDr_3 = RLMDr_2 * RLMDr;
@@ -289,7 +295,7 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     R MOMs_2ds = R(0), MOMs_5ds_h = R(0), RMOMs_Vt = R(0);
     if (!roh) {
         zerosw = R(in.sw.zs);
-        const R RLMDs = sqrt(sqrt(dens * qs / (As * N0s * R(P.GAM_1bs)) + zerosw)) * (R(1) - zerosw);
+        const R RLMDs = sqrt(sqrt(dens * qs * R(P.rK_s) + zerosw)) * (R(1) - zerosw);
         const R RLMDs_ds = sqrt(sqrt(RLMDs));
         const R RLMDs_2 = RLMDs * RLMDs;
         const R RLMDs_3 = RLMDs_2 * RLMDs;
@@ -310,7 +316,7 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     }
 
     // Roh and Satoh (2014) bimodal snow moments
-    const R Xs2 = dens * qs / As;
+    const R Xs2 = dens * qs * R(P.rAs);
     zerosw = R(in.sw.zx);
     const R tems = dmin(R(-0.1), temc);
     R at[4], bt[4];
@@ -375,7 +381,7 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
 
     // slope parameter lambda (graupel)
     zerosw = R(in.sw.zg);
-    const R RLMDg = sqrt(sqrt(dens * qg / (Ag * N0g * R(P.GAM_1bg)) + zerosw)) * (R(1) - zerosw);
+    const R RLMDg = sqrt(sqrt(dens * qg * R(P.rK_g) + zerosw)) * (R(1) - zerosw);
     const R RLMDg_dg = sqrt(RLMDg);
     const R RLMDg_2 = RLMDg * RLMDg;
     const R RLMDg_3 = RLMDg_2 * RLMDg;
@@ -384,14 +390,14 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
 
     // terminal velocity (QI does not sediment under "3WATER": its fall speed is unused)
     const R Vti = R(0);
-    const R Vtr = -Cr * rho_fact * R(P.GAM_1brdr) / R(P.GAM_1br) * RLMDr_dr;
+    const R Vtr = -Cr * rho_fact * R(P.Vtr_fac) * RLMDr_dr;
     const R Vts = -Cs * rho_fact * RMOMs_Vt;
-    const R Vtg = -Cg * rho_fact * R(P.GAM_1bgdg) / R(P.GAM_1bg) * RLMDg_dg;
+    const R Vtg = -Cg * rho_fact * R(P.Vtg_fac) * RLMDg_dg;
 
     // nucleation
     const R Ni0 = dmax(exp(R(-0.1) * temc), R(1)) * R(1000);
     const R Qi0 = in.tun[7] * exp(log(Ni0) * R(1.33)) * Rdens;
-    R Pigen = dmax(dmin(Qi0 - qi, in.qv_qsati), R(0)) / dt;
+    R Pigen = dmax(dmin(Qi0 - qi, in.qv_qsati), R(0)) * rdt;
 
     // accretion
     const R Esi = in.tun[2], Egs = R(P.Egs);
@@ -414,7 +420,7 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     R Praci = qi * R(0.25) * PI * R(P.Eri) * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
     R Psaci = qi * R(0.25) * PI * Esi_mod * Cs * MOMs_2ds * rho_fact;
     R Pgaci = qi * R(0.25) * PI * R(P.Egi) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
-    R Piacr = qi * Ar / R(P.mi) * R(0.25) * PI * R(P.Eri) * N0r * Cr * R(P.GAM_6dr) * RLMDr_6dr * rho_fact;
+    R Piacr = qi * Ar * R(P.rmi) * R(0.25) * PI * R(P.Eri) * N0r * Cr * R(P.GAM_6dr) * RLMDr_6dr * rho_fact;
     const R GAM_1br = R(P.GAM_1br), GAM_2br = R(P.GAM_2br), GAM_3br = R(P.GAM_3br);
     const R Esr = in.tun[3];
     R Psacr = Ar * R(0.25) * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
@@ -449,10 +455,16 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     const R Ka = (R(Ka0) + R(dKa_dT) * temc);
     const R Kd = (R(Kd0) + R(dKd_dT) * temc) * PRE00 / in.pre;
     const R Nu = (R(nu0) + R(dnu_dT) * temc) * Rdens;
-    const R Glv = R(1) / (LHV0 / (Ka * temp) * (LHV0 / (Rvap * temp) - R(1)) + R(1) / (Kd * dens * in.qsatl));
-    const R Giv = R(1) / (LHS0 / (Ka * temp) * (LHS0 / (Rvap * temp) - R(1)) + R(1) / (Kd * dens * in.qsati));
-    const R Gil = R(1) / (LHF0 / (Ka * temc));
-    const R ventr = R(P.f1r) * GAM_2 * RLMDr_2 + R(P.f2r) * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * R(P.GAM_5dr_h);
+    const R rNu = R(1) / Nu;
+    // G = 1 / (A + 1 / (Kd dens qsat)) = D / (A D + 1) with D = Kd dens qsat, and
+    // 1 / (LHF0 / (Ka temc)) = Ka temc / LHF0 (0 at temc = 0, as 1 / inf)
+    const R rT = R(1) / temp;
+    const R rKaT = rT / Ka;
+    const R Dl = Kd * dens * in.qsatl, Di_ = Kd * dens * in.qsati;
+    const R Glv = Dl / ((LHV0 * rKaT) * (LHV0 * R(1.0 / C_Rvap) * rT - R(1)) * Dl + R(1));
+    const R Giv = Di_ / ((LHS0 * rKaT) * (LHS0 * R(1.0 / C_Rvap) * rT - R(1)) * Di_ + R(1));
+    const R Gil = Ka * temc * R(1.0 / C_LHF0);
+    const R ventr = R(P.f1r) * GAM_2 * RLMDr_2 + R(P.f2r) * sqrt(Cr * rho_fact * rNu * RLMDr_5dr) * R(P.GAM_5dr_h);
     R Prevp = R(2) * PI * Rdens * N0r * in.sliq_1m * Glv * ventr;
 
     const R rhoqi = dmax(dens * qi, EPS);
@@ -464,28 +476,28 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     R Pisub = (R(1) - spsati) * (-tmp);
 
     R sw = R(in.sw.sw_hom);
-    R Pihom = sw * qc / dt;
+    R Pihom = sw * qc * rdt;
     sw = R(in.sw.sw_htr);
     const R A_frz = R(P.A_frz), B_frz = R(P.B_frz);
     // exp(-A_frz temc) - 1 cancels near 0 degC: FP32 takes expm1, FP64 keeps the
     // reference's form
     const R frz_em1 = (sizeof(R) == sizeof(float)) ? R(expm1(-A_frz * temc)) : (exp(-A_frz * temc) - R(1));
-    R Pihtr = sw * (dens / rho_w * (qc * qc) / (R(P.Nc_ihtr) * R(1.0E+6))) * B_frz * frz_em1;
+    R Pihtr = sw * (dens * R(1.0 / C_DWATR) * (qc * qc) * R(P.rNc_ihtr)) * B_frz * frz_em1;
     sw = R(in.sw.sw_mlt);
-    R Pimlt = sw * qi / dt;
+    R Pimlt = sw * qi * rdt;
 
-    const R vents = R(P.f1s) * MOMs_1 + R(P.f2s) * sqrt(Cs * rho_fact / Nu) * MOMs_5ds_h;
+    const R vents = R(P.f1s) * MOMs_1 + R(P.f2s) * sqrt(Cs * rho_fact * rNu) * MOMs_5ds_h;
     tmp = R(2) * PI * Rdens * in.sice_m1 * Giv * vents;
     R Psdep = (spsati) * (tmp);
     R Pssub = (R(1) - spsati) * (-tmp);
-    R Psmlt = R(2) * PI * Rdens * Gil * vents + CL * temc / LHF0 * (Psacw + Psacr);
+    R Psmlt = R(2) * PI * Rdens * Gil * vents + CL * temc * R(1.0 / C_LHF0) * (Psacw + Psacr);
     Psmlt = dmax(Psmlt, R(0));
 
-    const R ventg = R(P.f1g) * GAM_2 * RLMDg_2 + R(P.f2g) * sqrt(Cg * rho_fact / Nu * RLMDg_5dg) * R(P.GAM_5dg_h);
+    const R ventg = R(P.f1g) * GAM_2 * RLMDg_2 + R(P.f2g) * sqrt(Cg * rho_fact * rNu * RLMDg_5dg) * R(P.GAM_5dg_h);
     tmp = R(2) * PI * Rdens * N0g * in.sice_m1 * Giv * ventg;
     R Pgdep = (spsati) * (tmp);
     R Pgsub = (R(1) - spsati) * (-tmp);
-    R Pgmlt = R(2) * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * (Pgacw + Pgacr);
+    R Pgmlt = R(2) * PI * Rdens * N0g * Gil * ventg + CL * temc * R(1.0 / C_LHF0) * (Pgacw + Pgacr);
     Pgmlt = dmax(Pgmlt, R(0));
     R Pgfrz = R(2) * PI * Rdens * N0r * R(60) * B_frz * Ar * frz_em1 * RLMDr_7;
 

@@ -47,6 +47,14 @@ inline void to_dev_params(const nsw6_params &p, DevParams &d)
     d.LovR_ice = C_LHS0 / C_Rvap;
     d.RTEM00 = 1.0 / C_TEM00;
     d.dtemp_criteria = pow(10.0, (double)(-(15 + 1) / 2));
+    d.rK_r = 1.0 / (p.Ar * p.N0r * p.GAM_1br);
+    d.rK_s = 1.0 / (p.As * p.N0s * p.GAM_1bs);
+    d.rK_g = 1.0 / (p.Ag * p.N0g * p.GAM_1bg);
+    d.rAs = 1.0 / p.As;
+    d.rmi = 1.0 / p.mi;
+    d.rNc_ihtr = 1.0 / (p.Nc_ihtr * 1.0E+6);
+    d.Vtr_fac = p.GAM_1brdr / p.GAM_1br;
+    d.Vtg_fac = p.GAM_1bgdg / p.GAM_1bg;
     d.ice_adjust = (p.sw_expice != 1.0) ? 1.0 : 0.0;
     d.sat_ulimit = d.ice_adjust != 0.0 ? 273.16 : 100.0;
     d.sat_llimit = d.ice_adjust != 0.0 ? 233.16 : 90.0;

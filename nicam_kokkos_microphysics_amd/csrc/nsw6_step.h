@@ -65,8 +65,9 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
     const S rho = rhog / x.gsgam2;
     S q[NQ];
     q[QV] = x.qv;
+    const S rrhog = S(1.0) / rhog;
 #pragma unroll
-    for (int n = 1; n <= 5; n++) q[n] = rq[n] / rhog;
+    for (int n = 1; n <= 5; n++) q[n] = rq[n] * rrhog;
     S qd = S(1.0);
 #pragma unroll
     for (int n = 1; n <= 5; n++) qd = qd - q[n];
@@ -76,8 +77,9 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
     const S tem = x.tem;
     S rhoge = tem * rhog * cva;
     const S pre = rho * (qd * C_Rdry + q[QV] * C_Rvap) * tem;
-    const S qsatl = psat_of(P.LovR_liq, P.RTEM00, tem) / (rho * C_Rvap * tem);
-    const S qsati = psat_of(P.LovR_ice, P.RTEM00, tem) / (rho * C_Rvap * tem);
+    const S rqden = S(1.0) / (rho * C_Rvap * tem);
+    const S qsatl = psat_of(P.LovR_liq, P.RTEM00, tem) * rqden;
+    const S qsati = psat_of(P.LovR_ice, P.RTEM00, tem) * rqden;
 
     RateIn<R> in;
     in.dens = R(rho);
@@ -237,10 +239,12 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
 {
     const bool interior = (j >= KMIN && j <= KMAX);
     const S rg = r.rhog;
-    const double rgs = interior ? r.gam2 / r.gs : 0.0;
+    const double rgs_inv = 1.0 / r.gs;
+    const double rgs = interior ? r.gam2 * rgs_inv : 0.0;
+    const S rrg = S(1.0) / rg;
     S qq[5];
 #pragma unroll
-    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] / rg;
+    for (int n = 0; n < 5; n++) qq[n] = r.rq[n] * rrg;
     S qd = S(1.0);
 #pragma unroll
     for (int n = 0; n < 5; n++) qd = qd - qq[n];
@@ -248,7 +252,7 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
 #pragma unroll
     for (int n = 0; n < 5; n++) cva = cva + qq[n] * C_CVW;
     c.tem = interior ? r.rhoge / (rg * cva) : S(0.0);
-    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) / rg : S(0.0);
+    c.kinh = interior ? 0.5 * (r.rvx * r.rvx + r.rvy * r.rvy + r.rvz * r.rvz) * rrg : S(0.0);
     c.up = r.vx * u.ix + r.vy * u.iy + r.vz * u.iz;
     c.vp = r.vx * u.jx + r.vy * u.jy + r.vz * u.jz;
     c.z = r.z;
@@ -264,7 +268,7 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
     c.gs = r.gs;
     double c2w0 = 0.0, c2w1 = 0.0;
     if (j >= KMIN) {
-        c2w0 = g.afact[j] / r.gs * r.gsh;
+        c2w0 = g.afact[j] * rgs_inv * r.gsh;
         c2w1 = g.bfact[j] / below_gs * r.gsh;
     }
     if (j >= KMIN + 1 && j <= KMAX) {
@@ -455,12 +459,14 @@ NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x,
         hw_p = -(x.fh[2][1] - x.fh[1][1]) * r;
     }
     // pre-transport wall quantities of cells c and c+1 (C2W factors, rhogkin_v)
-    const double c2w0_c = g.afact[c] / x.gs_c * x.gsh_c, c2w1_c = g.bfact[c] / x.gs_m * x.gsh_c;
-    const double c2w0_p = g.afact[c + 1] / x.gs_p * x.gsh_p, c2w1_p = g.bfact[c + 1] / x.gs_c * x.gsh_p;
+    const double rgs_c = 1.0 / x.gs_c;
+    const double c2w0_c = g.afact[c] * rgs_c * x.gsh_c, c2w1_c = g.bfact[c] / x.gs_m * x.gsh_c;
+    const double c2w0_p = g.afact[c + 1] / x.gs_p * x.gsh_p, c2w1_p = g.bfact[c + 1] * rgs_c * x.gsh_p;
     const S kinv0_c = (c >= KMIN + 1) ? S(0.5 * (x.rw_c * x.rw_c) / (c2w0_c * x.rg_c + c2w1_c * x.rg_m)) : S(0.0);
     const S kinv0_p = (c + 1 <= KMAX) ? S(0.5 * (x.rw_p * x.rw_p) / (c2w0_p * x.rg_p + c2w1_p * x.rg_c)) : S(0.0);
 
-    const double w2c0 = g.cfact[c] * x.gs_c / x.gsh_p, w2c1 = g.dfact[c] * x.gs_c / x.gsh_p;
+    const double rgsh_p = x.gs_c / x.gsh_p;
+    const double w2c0 = g.cfact[c] * rgsh_p, w2c1 = g.dfact[c] * rgsh_p;
     const S kinh = 0.5 * (x.rvx * x.rvx + x.rvy * x.rvy + x.rvz * x.rvz) / x.rg_c;
     S e = x.re0 + de + dphi;
     const S kinh_u = kinh + dkinh;
@@ -524,7 +530,7 @@ NSW6_HD int satadjust_liq_newton(const DevParams &P, double &tem, double q[6], d
 #pragma unroll
         for (int n = 1; n <= 5; n++) CVt = CVt + q[n] * C_CVW;
         const double Emoist_new = tem * CVt + q[QV] * C_LHV0;
-        const double dqsat_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsat;
+        const double dqsat_dT = (P.LovR_liq / (tem * tem)) * qsat;   // CVovR_liq = 0
         const double dCVtot_dT = dqsat_dT * (C_CVW - C_CVW);
         const double dEmoist_dT = tem * dCVtot_dT + CVt + dqsat_dT * C_LHV0;
         const double dtemp = (Emoist_new - ein_moist) / dEmoist_dT;
@@ -570,8 +576,9 @@ NSW6_HD int satadjust_all_newton(const DevParams &P, double &tem, double q[6], d
         const double lim1 = 0.5 + copysign(0.5, UL - tem);
         const double lim2 = 0.5 + copysign(0.5, tem - LL);
         const double dalpha_dT = lim1 * lim2 / (UL - LL);
-        const double dqsatl_dT = (P.LovR_liq / (tem * tem) + 0.0 / tem) * qsatl;
-        const double dqsati_dT = (P.LovR_ice / (tem * tem) + 0.0 / tem) * qsati;
+        const double rtt = 1.0 / (tem * tem);   // CVovR_liq = CVovR_ice = 0
+        const double dqsatl_dT = (P.LovR_liq * rtt) * qsatl;
+        const double dqsati_dT = (P.LovR_ice * rtt) * qsati;
         const double dqsat_dT = qsatl * dalpha_dT + dqsatl_dT * alpha - qsati * dalpha_dT + dqsati_dT * (1.0 - alpha);
         const double dqc_dT = (qsum - qsat) * dalpha_dT - dqsat_dT * alpha;
         const double dqi_dT = -(qsum - qsat) * dalpha_dT - dqsat_dT * (1.0 - alpha);
@@ -631,10 +638,12 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
     for (int n = 0; n < 6; n++) rhogq[n] = x.rhogq[n];
     const double gsgam2 = x.gsgam2;
     S q[6];
+    const S rrhog = S(1.0) / rhog;
+    const double rgsgam2 = 1.0 / gsgam2;
 #pragma unroll
-    for (int n = 0; n < 6; n++) q[n] = rhogq[n] / rhog;
-    S rho = rhog / gsgam2;
-    const S ein = rhoge / rhog;
+    for (int n = 0; n < 6; n++) q[n] = rhogq[n] * rrhog;
+    S rho = rhog * rgsgam2;
+    const S ein = rhoge * rrhog;
     S qd = S(1.0);
 #pragma unroll
     for (int n = 1; n <= 5; n++) qd = qd - q[n];
@@ -651,7 +660,7 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
     // SATURATION_adjustment (src/mod_satadjust.cpp:274-362): satadjust_liq, or
     // satadjust_all when explicit ice generation is off
     const bool ice = P.ice_adjust != 0.0;
-    const S ein_moist = ice ? S(rhoge / rhog + q[QV] * C_LHV0 - q[QI] * C_LHF0) : S(rhoge / rhog + q[QV] * C_LHV0);
+    const S ein_moist = ice ? S(ein + q[QV] * C_LHV0 - q[QI] * C_LHF0) : S(ein + q[QV] * C_LHV0);
     const S qsum = ice ? S(q[QV] + q[QC] + q[QI]) : S(q[QV] + q[QC]);
     q[QV] = qsum;
     q[QC] = S(0.0);
@@ -659,7 +668,7 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
     S CVtot = qd * C_CVdry;
 #pragma unroll
     for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
-    const S rho_sa = rhog / gsgam2;
+    const S rho_sa = rhog * rgsgam2;
     tem = (ein_moist - q[QV] * C_LHV0) / CVtot;
     if (interior) {
         S qdl = S(1.0);
@@ -710,9 +719,10 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
         rhogq[QV] = dmax(rhogq[QV], S(0.0));
         diffq -= rhogq[QV];
         rhog = rhog * (1.0 - diffq);
-        rho = rhog / gsgam2;
+        rho = rhog * rgsgam2;
+        const S rrhog2 = S(1.0) / rhog;
 #pragma unroll
-        for (int n = 1; n <= 5; n++) q[n] = rhogq[n] / rhog;
+        for (int n = 1; n <= 5; n++) q[n] = rhogq[n] * rrhog2;
         S qdn = S(1.0);
 #pragma unroll
         for (int n = 1; n <= 5; n++) qdn = qdn - q[n];
