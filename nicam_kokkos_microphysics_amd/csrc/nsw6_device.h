@@ -401,8 +401,10 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
 
     // accretion
     const R Esi = in.tun[2], Egs = R(P.Egs);
-    const R Esi_mod = dmin(Esi, Esi * exp(R(P.gamma_sacr) * temc));
-    const R Egs_mod = dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
+    const R e_sacr = exp(R(P.gamma_sacr) * temc);
+    const R Esi_mod = dmin(Esi, Esi * e_sacr);
+    // min(Egs, Egs exp(.)) is exactly Egs = 0 under Roh & Satoh (Egs = 0): no exp needed
+    const R Egs_mod = (P.Egs == 0.0) ? Egs : dmin(Egs, Egs * exp(R(P.gamma_gacs) * temc));
     const R GAM_3dr = R(P.GAM_3dr), GAM_3dg = R(P.GAM_3dg);
     const bool kk = (P.sw_kk2000 == 1.0);
     zerosw = R(in.sw.zcr);
@@ -446,9 +448,13 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
         Praut = (R(1) - sw_kk) * Praut_berry + sw_kk * Praut_kk;
     }
     const R beta_saut = R(P.beta_saut), beta_gaut = R(P.beta_gaut);
-    const R betai = dmin(beta_saut, beta_saut * exp(R(P.gamma_saut) * temc));
+    // exp(gamma_saut temc) is exp(gamma_sacr temc) when the two exponents are equal
+    // (6e-2 both by default); min(beta_gaut, beta_gaut exp(.)) is exactly 0 when Pgaut is
+    // off (beta_gaut = 0)
+    const R e_saut = (P.gamma_saut == P.gamma_sacr) ? e_sacr : exp(R(P.gamma_saut) * temc);
+    const R betai = dmin(beta_saut, beta_saut * e_saut);
     R Psaut = dmax(betai * (qi - in.tun[1]), R(0));
-    const R betas = dmin(beta_gaut, beta_gaut * exp(R(P.gamma_gaut) * temc));
+    const R betas = (P.beta_gaut == 0.0) ? beta_gaut : dmin(beta_gaut, beta_gaut * exp(R(P.gamma_gaut) * temc));
     R Pgaut = dmax(betas * (qs - R(P.qscrt_gaut)), R(0));
 
     // evaporation, sublimation, melting, freezing
