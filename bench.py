@@ -220,6 +220,35 @@ def adjoint_cost(reps):
                       f"HIP events on the launch stream"}
 
 
+def subcycling_cost(steps, warmup, div_num=2):
+    """MP_DIV_NUM sub-cycling on the FP64 workload (169,000 x 96): wall time of one
+    mp_driver call of `div_num` sub-steps of dt / div_num, against div_num plain steps."""
+    import torch
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    w = F.WORKLOADS["nsw6_169000x96_fp64"]
+    ncol = w.lall * w.ijdim
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    del st
+    drv = NSW6(gz, gzh, max_columns=ncol, div_num=div_num)
+    for _ in range(warmup):
+        drv.step(dev, w.dt, check=False)
+    drv.check()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        drv.step(dev, w.dt, check=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rep = drv.check()
+    drv.close()
+    del dev
+    torch.cuda.empty_cache()
+    return {"workload": w.name, "div_num": div_num, "calls": steps, "ms_per_call": wall * 1e3 / steps,
+            "column_steps_per_s": ncol * steps / wall, "general_regions": rep.n_general_regions}
+
+
 def rollout_bench():
     """The long rollout: forward (mixed precision) and backward (recompute + FP64 reverse
     mode, 10 regions at a time) wall times, and the peak device memory."""
@@ -326,6 +355,7 @@ def main():
         out["mixed"]["error_vs_fp64"] = mixed_error(F.WORKLOADS["nsw6_169000x96_mixed"], w.steps)
     if not args.only:
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
+        out["subcycling"] = subcycling_cost(max(2, args.steps // 2), 1)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

@@ -150,6 +150,17 @@ int nsw6_set_params(nsw6_ctx *ctx, const nsw6_params *params);
 /* Choose the arithmetic of subsequent steps (enum nsw6_precision). */
 int nsw6_set_precision(nsw6_ctx *ctx, int precision);
 
+/* MP_DIV_NUM (include/problemsize.h:175, a compile-time 1 in the reference): each
+ * later mp_driver call runs `mp_div_num` NSW6 sub-steps of dt_mp = dt * (1/mp_div_num)
+ * and returns precip, precip_rhoe/lh_heat/rhophi/rhokin, GPREC, GDCLW, GDCFRC and
+ * the ISO precipitations as the sub-step sums times 1/mp_div_num
+ * (src/mod_mp_driver.cpp:420-612); the other diagnostics are the last sub-step's.
+ * 1 <= mp_div_num <= NSW6_MAX_DIV_NUM.  The first call with mp_div_num > 1 allocates
+ * the sums ((96 + 6) doubles per column).  nsw6_mp_driver_adj returns
+ * NSW6_UNSUPPORTED while mp_div_num > 1. */
+#define NSW6_MAX_DIV_NUM 1000
+int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num);
+
 /* One mp_driver step over `lall` regions of `ijdim` columns each.  `fields` holds
  * device pointers.  `stream` is a hipStream_t (NULL = the default stream). */
 int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
@@ -196,7 +207,8 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                        double *grad_coef, double dt, double ct, void *stream);
 
 /* Synchronise the context's last stream and return the device report of the last
- * step (or of the forward part recomputed by the last nsw6_mp_driver_adj). */
+ * step (or of the forward part recomputed by the last nsw6_mp_driver_adj).  With
+ * sub-steps the Newton counts cover all of them; n_general_regions counts the last. */
 int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *report);
 
 /* Average duration (ms) of the named kernel over the launches recorded since the

@@ -19,6 +19,7 @@ LIB_SYNTH = os.path.join(HERE, "libnsw6_synth.so")
 KALL, KMIN, KMAX, NQMAX = 96, 1, 94, 6
 NSW6_OK, NSW6_NOT_CONVERGED, NSW6_NAN_DETECTED = 0, 1, 2
 NSW6_BAD_ARG, NSW6_HIP_ERROR, NSW6_NO_DEVICE, NSW6_UNSUPPORTED = 3, 4, 5, 6
+NSW6_MAX_DIV_NUM = 1000
 NSW6_FP64, NSW6_MIXED = 0, 1
 
 # The 8 tunable coefficients (order of NSW6_NCOEF in nsw6.h).
@@ -87,6 +88,7 @@ SIGNATURES = {
     "nsw6_init": (_I, [ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.c_int64, ctypes.POINTER(_P)]),
     "nsw6_set_params": (_I, [_P, ctypes.POINTER(Params)]),
     "nsw6_set_precision": (_I, [_P, _I]),
+    "nsw6_set_div_num": (_I, [_P, _I]),
     "nsw6_mp_driver_fields": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), _D, _D, _P]),
     "nsw6_mp_driver": (_I, [_P, _I, _I, _I] + [_P] * 46 + [_D, _D] + [_P] * 4 + [_P]),
     "nsw6_mp_driver_adj": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), ctypes.POINTER(FieldPtrs), _P,

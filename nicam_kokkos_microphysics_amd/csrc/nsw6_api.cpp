@@ -60,6 +60,8 @@ struct nsw6_ctx {
     bool last_adjoint = false;
     std::vector<PendingAdjTiming> pending_adj;
     int64_t t_adj_count = 0;
+    int div_num = 1;                 // MP_DIV_NUM sub-steps per call
+    double *div_sum = nullptr;       // their output sums (allocated at the first use)
 };
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? NSW6_OK : NSW6_HIP_ERROR; }
@@ -227,25 +229,49 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     if (st == NSW6_OK) st = ensure_regions(ctx, lall);
     if (st != NSW6_OK) return st;
 
+    const int ndiv = ctx->div_num;
+    if (ndiv > 1 && !ctx->div_sum) {
+        const size_t n = sizeof(double) * (size_t)ctx->max_columns * (KALL + DIV_COLUMN_SUMS);
+        if (hipMalloc(&ctx->div_sum, n) != hipSuccess) {
+            ctx->div_sum = nullptr;
+            return NSW6_HIP_ERROR;
+        }
+    }
+
+    // fraction_mp and dt_mp as src/mod_mp_driver.cpp:420-421
+    const double fraction_mp = 1.0 / double(ndiv);
     StepArgs a;
     memset(&a, 0, sizeof a);
     a.f = to_fields(*fl);
-    fill_step_args(ctx, lall, ijdim, dt, a);
+    fill_step_args(ctx, lall, ijdim, dt * fraction_mp, a);
 
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t *ev = nullptr;
-    PendingTiming pt;
-    if (ctx->timing) {
-        for (int i = 0; i < N_EVENTS; i++)
-            if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
-        ev = pt.ev;
-    }
-    hipError_t e = nsw6_launch_step(&a, ctx->precision, s, ev);
-    if (ctx->timing) ctx->pending.push_back(pt);
     ctx->last_stream = s;
     ctx->last_lall = lall;
     ctx->last_adjoint = false;
-    return hip_ok(e);
+    for (int m = 0; m < ndiv; m++) {
+        a.keep_report = (m > 0);
+        hipEvent_t *ev = nullptr;
+        PendingTiming pt;
+        if (ctx->timing) {
+            for (int i = 0; i < N_EVENTS; i++)
+                if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
+            ev = pt.ev;
+        }
+        hipError_t e = nsw6_launch_step(&a, ctx->precision, s, ev);
+        if (ctx->timing) ctx->pending.push_back(pt);
+        if (e == hipSuccess && ndiv > 1) e = nsw6_launch_div_sum(&a, ctx->div_sum, m == 0, s);
+        if (e != hipSuccess) return NSW6_HIP_ERROR;
+    }
+    if (ndiv > 1) return hip_ok(nsw6_launch_div_scale(&a, ctx->div_sum, fraction_mp, s));
+    return NSW6_OK;
+}
+
+int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
+{
+    if (!ctx || mp_div_num < 1 || mp_div_num > NSW6_MAX_DIV_NUM) return NSW6_BAD_ARG;
+    ctx->div_num = mp_div_num;
+    return NSW6_OK;
 }
 
 // The reverse mode covers the default scheme only (see include/nsw6.h).
@@ -297,7 +323,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
                            fl->gam2h, fl->ix, fl->iy, fl->iz, fl->jx, fl->jy, fl->jz, fl->z};
     for (const double *p : req)
         if (!p) return NSW6_BAD_ARG;
-    if (!adjoint_supported(ctx->P)) return NSW6_UNSUPPORTED;
+    if (!adjoint_supported(ctx->P) || ctx->div_num != 1) return NSW6_UNSUPPORTED;
     int st = use_device(ctx);
     if (st == NSW6_OK) st = ensure_regions(ctx, lall);
     if (st == NSW6_OK) st = ensure_adj_scratch(ctx, lall, ijdim);
@@ -471,6 +497,7 @@ void nsw6_finalize(nsw6_ctx *ctx)
     (void)drain_timing(ctx);
     (void)hipDeviceSynchronize();
     free_ws(ctx);
+    if (ctx->div_sum) (void)hipFree(ctx->div_sum);
     if (ctx->grid_dev) (void)hipFree(ctx->grid_dev);
     if (ctx->P_dev) (void)hipFree(ctx->P_dev);
     delete ctx;

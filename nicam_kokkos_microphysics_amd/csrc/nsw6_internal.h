@@ -106,6 +106,7 @@ struct StepArgs {
                           // arguments: ~130 doubles there overflow the scalar registers)
     const GridDev *grid;
     int lall, ijdim;
+    int keep_report;              // 1: keep the report of earlier sub-steps of one call
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 
@@ -137,6 +138,13 @@ constexpr int NSW6_STEP_EVENTS = 5;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev);
+// MP_DIV_NUM sub-cycling: sums [l][KALL][ij] of GPREC followed by [l][DIV_COLUMN_SUMS][ij]
+// of precip (2 rows), precip_rhoe, precip_lh_heat, precip_rhophi, precip_rhokin.
+constexpr int DIV_COLUMN_SUMS = 6;
+extern "C" hipError_t nsw6_launch_div_sum(const nsw6::StepArgs *args, double *sum, int first,
+                                          hipStream_t stream);
+extern "C" hipError_t nsw6_launch_div_scale(const nsw6::StepArgs *args, const double *sum, double fraction,
+                                            hipStream_t stream);
 
 // events recorded by nsw6_launch_adjoint: before the forward part, after it, after the
 // reverse passes

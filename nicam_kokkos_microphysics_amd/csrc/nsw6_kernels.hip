@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     // Region-wide reductions of this step start from their neutral values.
     if (blockIdx.x == 0 && k == 0) {
         if (threadIdx.x == 0) a.ws.region_flag[l] = 0;
-        if (l == 0 && threadIdx.x < REPORT_INTS) a.ws.report[threadIdx.x] = (threadIdx.x >= 3) ? -1 : 0;
+        if (l == 0 && !a.keep_report && threadIdx.x < REPORT_INTS)
+            a.ws.report[threadIdx.x] = (threadIdx.x >= 3) ? -1 : 0;
     }
     if (ij >= ijdim) return;
 
@@ -951,6 +952,68 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     if (ev) (void)hipEventRecord(ev[3], stream);
     hipLaunchKernelGGL(k_finish, grid_l, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// MP_DIV_NUM sub-cycling (src/mod_mp_driver.cpp:420-612).  After each sub-step the
+// step outputs that mp_driver accumulates are added to the sums (sum = 0 + x at the
+// first sub-step, sum + x after); at the end the outputs are sum * fraction_mp.
+// GDCLW, GDCFRC and the ISO precipitations are zero in every sub-step, and so are
+// their scaled sums.  Grid (ij/256, KALL + DIV_COLUMN_SUMS, l): rows below KALL are
+// the GPREC levels, the rest the per-column outputs.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double *div_output(const StepArgs &a, int l, int r)
+{
+    const int64_t n = a.ijdim;
+    const Fields &f = a.f;
+    if (r < KALL) return f.GPREC + ((int64_t)l * KALL + r) * n;
+    switch (r - KALL) {
+    case 0: return f.precip + (int64_t)l * 2 * n;
+    case 1: return f.precip + (int64_t)l * 2 * n + n;
+    case 2: return f.precip_rhoe + (int64_t)l * n;
+    case 3: return f.precip_lh_heat + (int64_t)l * n;
+    case 4: return f.precip_rhophi + (int64_t)l * n;
+    default: return f.precip_rhokin + (int64_t)l * n;
+    }
+}
+__device__ __forceinline__ int64_t div_sum_row(const StepArgs &a, int l, int r)
+{
+    const int64_t n = a.ijdim;
+    if (r < KALL) return ((int64_t)l * KALL + r) * n;
+    return (int64_t)a.lall * KALL * n + ((int64_t)l * DIV_COLUMN_SUMS + (r - KALL)) * n;
+}
+
+__global__ void __launch_bounds__(256) k_div_sum(StepArgs a, double *sum, int first)
+{
+    const int l = blockIdx.z, r = blockIdx.y;
+    const int ij = blockIdx.x * 256 + threadIdx.x;
+    if (ij >= a.ijdim) return;
+    const double x = div_output(a, l, r)[ij];
+    double *s = sum + div_sum_row(a, l, r) + ij;
+    *s = first ? 0.0 + x : *s + x;
+}
+
+__global__ void __launch_bounds__(256) k_div_scale(StepArgs a, const double *sum, double fraction)
+{
+    const int l = blockIdx.z, r = blockIdx.y;
+    const int ij = blockIdx.x * 256 + threadIdx.x;
+    if (ij >= a.ijdim) return;
+    div_output(a, l, r)[ij] = sum[div_sum_row(a, l, r) + ij] * fraction;
+}
+
+extern "C" hipError_t nsw6_launch_div_sum(const StepArgs *args, double *sum, int first, hipStream_t stream)
+{
+    const dim3 grid((args->ijdim + 255) / 256, KALL + DIV_COLUMN_SUMS, args->lall);
+    hipLaunchKernelGGL(k_div_sum, grid, dim3(256), 0, stream, *args, sum, first);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nsw6_launch_div_scale(const StepArgs *args, const double *sum, double fraction,
+                                            hipStream_t stream)
+{
+    const dim3 grid((args->ijdim + 255) / 256, KALL + DIV_COLUMN_SUMS, args->lall);
+    hipLaunchKernelGGL(k_div_scale, grid, dim3(256), 0, stream, *args, sum, fraction);
     return hipGetLastError();
 }
 

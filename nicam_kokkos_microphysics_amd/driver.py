@@ -60,7 +60,7 @@ class NSW6:
     """One HIP context of the NSW6 step (workspace sized for ``max_columns``)."""
 
     def __init__(self, gz, gzh, params: N.Params | None = None, max_columns: int = 169000,
-                 precision: int = N.NSW6_FP64):
+                 precision: int = N.NSW6_FP64, div_num: int = 1):
         self._L = N.lib()
         self.grid = grid_setup(gz, gzh)
         self.params = params if params is not None else default_params()
@@ -72,6 +72,7 @@ class NSW6:
             raise NSW6Error(rc)
         self._ctx = ctx
         self.set_precision(precision)
+        self.set_div_num(div_num)
 
     # -- configuration ------------------------------------------------------------
     def set_params(self, params: N.Params):
@@ -85,6 +86,13 @@ class NSW6:
         if rc != 0:
             raise NSW6Error(rc)
         self.precision = precision
+
+    def set_div_num(self, div_num: int):
+        """MP_DIV_NUM sub-steps per call (include/problemsize.h:175; the reference fixes 1)."""
+        rc = self._L.nsw6_set_div_num(self._ctx, int(div_num))
+        if rc != 0:
+            raise NSW6Error(rc)
+        self.div_num = int(div_num)
 
     # -- the step -----------------------------------------------------------------
     def step(self, state: dict, dt: float, ct: float = 0.0, *, l_region: int = 0, stream=None,

@@ -124,7 +124,7 @@ def synth_grid():
 
 def synth_state(lall: int, ijdim: int, *, seed: int = 20240101, frac_supersat: float = 0.0,
                 frac_towers: float = 0.0, rh_lo: float = 0.5, rh_hi: float = 1.05, l_offset: int = 0,
-                ij_offset: int = 0, gz=None, gzh=None) -> tuple:
+                ij_offset: int = 0, gz=None, gzh=None, tsfc: float | None = None) -> tuple:
     """Seeded, self-consistent host state: (gz, gzh, state).  Columns are a pure
     function of (seed, l + l_offset, ij + ij_offset), so sub-blocks of a large run can
     be regenerated exactly."""
@@ -141,6 +141,8 @@ def synth_state(lall: int, ijdim: int, *, seed: int = 20240101, frac_supersat: f
     cfg.frac_towers = frac_towers
     cfg.rh_lo = rh_lo
     cfg.rh_hi = rh_hi
+    if tsfc is not None:
+        cfg.tsfc = tsfc
     fp = field_ptrs(st)
     rc = S.nsw6_synth_fill(ctypes.byref(cfg), gz.ctypes.data, gzh.ctypes.data, lall, ijdim, l_offset,
                            ij_offset, ctypes.byref(fp))

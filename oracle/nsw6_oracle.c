@@ -1519,6 +1519,16 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
 int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
                   const nsw6_fields *f, double dt, double ct, int ice_adjust, nsw6_report *rep)
 {
+    return ora_mp_driver_div(P, G, lall, ijdim, f, dt, ct, ice_adjust, 1, rep);
+}
+
+/* The MP_DIV_NUM loop of src/mod_mp_driver.cpp:420-612: div_num NSW6 steps of
+ * dt / div_num, each after zeroing the step outputs; the outputs are summed over the
+ * sub-steps and scaled by fraction_mp = 1 / div_num at the end. */
+int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
+                      const nsw6_fields *f, double dt, double ct, int ice_adjust, int div_num,
+                      nsw6_report *rep)
+{
     (void)ct;
     struct timespec t0, t1;
     nsw6_report local;
@@ -1527,9 +1537,9 @@ int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
     if (!rep) rep = &local;
     memset(rep, 0, sizeof *rep);
     rep->first_l = rep->first_k = rep->first_ij = -1;
-    if (lall <= 0 || ijdim <= 0) { rep->status = NSW6_BAD_ARG; return NSW6_BAD_ARG; }
+    if (lall <= 0 || ijdim <= 0 || div_num < 1) { rep->status = NSW6_BAD_ARG; return NSW6_BAD_ARG; }
 
-    const double fraction_mp = 1.0 / (double)1;
+    const double fraction_mp = 1.0 / (double)div_num;
     const double dt_mp = dt * fraction_mp;
     const size_t K2 = (size_t)KALL * ijdim;
     double elapsed = 0.0;
@@ -1555,33 +1565,34 @@ int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
         double *gdclw_sum = zalloc(K2), *gdcfrc_sum = zalloc(K2), *gprec_sum = zalloc(K2);
         double *iso1 = f->ISO1_precip + (size_t)l * 2 * ijdim, *iso2 = f->ISO2_precip + (size_t)l * 2 * ijdim;
 
-        /* m = 0 (MP_DIV_NUM = 1) */
-        for (size_t i = 0; i < 2 * (size_t)ijdim; i++) { R.precip[i] = 0.0; iso1[i] = 0.0; iso2[i] = 0.0; }
-        for (size_t i = 0; i < K2; i++) {
-            f->GDCLW[o2 + i] = 0.0;
-            f->GDCFRC[o2 + i] = 0.0;
-            f->GPREC[o2 + i] = 0.0;
-        }
-        for (int ij = 0; ij < ijdim; ij++) {
-            R.precip_rhoe[ij] = 0.0; R.precip_lh_heat[ij] = 0.0;
-            R.precip_rhophi[ij] = 0.0; R.precip_rhokin[ij] = 0.0;
-        }
-        clock_gettime(CLOCK_MONOTONIC, &t0);
-        mp_nsw6(P, G, ijdim, &R, f->re_liquid + o2, f->re_cld + o2, f->rctop + o1, f->rwtop + o1,
-                f->tctop + o1, dt_mp, ice_adjust, rep, l);
-        clock_gettime(CLOCK_MONOTONIC, &t1);
-        elapsed += (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
-        for (size_t i = 0; i < K2; i++) f->re_solid[o2 + i] = 20.0E-6;
+        for (int m = 0; m < div_num; m++) {
+            for (size_t i = 0; i < 2 * (size_t)ijdim; i++) { R.precip[i] = 0.0; iso1[i] = 0.0; iso2[i] = 0.0; }
+            for (size_t i = 0; i < K2; i++) {
+                f->GDCLW[o2 + i] = 0.0;
+                f->GDCFRC[o2 + i] = 0.0;
+                f->GPREC[o2 + i] = 0.0;
+            }
+            for (int ij = 0; ij < ijdim; ij++) {
+                R.precip_rhoe[ij] = 0.0; R.precip_lh_heat[ij] = 0.0;
+                R.precip_rhophi[ij] = 0.0; R.precip_rhokin[ij] = 0.0;
+            }
+            clock_gettime(CLOCK_MONOTONIC, &t0);
+            mp_nsw6(P, G, ijdim, &R, f->re_liquid + o2, f->re_cld + o2, f->rctop + o1, f->rwtop + o1,
+                    f->tctop + o1, dt_mp, ice_adjust, rep, l);
+            clock_gettime(CLOCK_MONOTONIC, &t1);
+            elapsed += (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+            for (size_t i = 0; i < K2; i++) f->re_solid[o2 + i] = 20.0E-6;
 
-        for (size_t i = 0; i < 2 * (size_t)ijdim; i++) precip_sum[i] += R.precip[i];
-        for (int ij = 0; ij < ijdim; ij++) {
-            prh_sum[ij] += R.precip_rhoe[ij]; plh_sum[ij] += R.precip_lh_heat[ij];
-            pphi_sum[ij] += R.precip_rhophi[ij]; pkin_sum[ij] += R.precip_rhokin[ij];
-        }
-        for (size_t i = 0; i < K2; i++) {
-            gdclw_sum[i] += f->GDCLW[o2 + i];
-            gdcfrc_sum[i] += f->GDCFRC[o2 + i];
-            gprec_sum[i] += f->GPREC[o2 + i];
+            for (size_t i = 0; i < 2 * (size_t)ijdim; i++) precip_sum[i] += R.precip[i];
+            for (int ij = 0; ij < ijdim; ij++) {
+                prh_sum[ij] += R.precip_rhoe[ij]; plh_sum[ij] += R.precip_lh_heat[ij];
+                pphi_sum[ij] += R.precip_rhophi[ij]; pkin_sum[ij] += R.precip_rhokin[ij];
+            }
+            for (size_t i = 0; i < K2; i++) {
+                gdclw_sum[i] += f->GDCLW[o2 + i];
+                gdcfrc_sum[i] += f->GDCFRC[o2 + i];
+                gprec_sum[i] += f->GPREC[o2 + i];
+            }
         }
         for (size_t i = 0; i < 2 * (size_t)ijdim; i++) {
             R.precip[i] = precip_sum[i] * fraction_mp;

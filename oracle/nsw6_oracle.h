@@ -64,6 +64,13 @@ int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
                   const nsw6_fields *f, double dt, double ct, int ice_adjust,
                   nsw6_report *rep);
 
+/* ora_mp_driver with the reference's MP_DIV_NUM sub-cycling (include/problemsize.h:175,
+ * src/mod_mp_driver.cpp:420-612): div_num >= 1 steps of dt / div_num; the surface
+ * outputs, GPREC, GDCLW and GDCFRC are the sub-step sums times 1 / div_num. */
+int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
+                      const nsw6_fields *f, double dt, double ct, int ice_adjust, int div_num,
+                      nsw6_report *rep);
+
 /* Seconds of the last ora_mp_driver call (wall clock, like the reference's
  * "NSW6 Elapsed Time", src/mod_mp_driver.cpp:500-552). */
 double ora_last_elapsed(void);

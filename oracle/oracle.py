@@ -62,6 +62,9 @@ def lib():
         L.ora_mp_driver.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
                                     ctypes.POINTER(N.FieldPtrs), D, D, I, ctypes.POINTER(N.Report)]
         L.ora_mp_driver.restype = I
+        L.ora_mp_driver_div.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
+                                        ctypes.POINTER(N.FieldPtrs), D, D, I, I, ctypes.POINTER(N.Report)]
+        L.ora_mp_driver_div.restype = I
         L.ora_last_elapsed.restype = D
         L.ora_last_elapsed.argtypes = []
         L.ora_num_threads.restype = I
@@ -93,8 +96,10 @@ def grid(gz, gzh):
     return g
 
 
-def mp_driver(state: dict, gz, gzh, dt: float, ct: float = 0.0, P=None, ice_adjust: int = -1):
-    """Run the oracle step in place on a host state dict; returns the report."""
+def mp_driver(state: dict, gz, gzh, dt: float, ct: float = 0.0, P=None, ice_adjust: int = -1,
+              div_num: int = 1):
+    """Run the oracle step in place on a host state dict; returns the report.
+    div_num > 1 is the MP_DIV_NUM sub-cycling (src/mod_mp_driver.cpp:420-612)."""
     from nicam_kokkos_microphysics_amd import _native as N
     from nicam_kokkos_microphysics_amd.fields import field_ptrs
     if P is None:
@@ -103,8 +108,12 @@ def mp_driver(state: dict, gz, gzh, dt: float, ct: float = 0.0, P=None, ice_adju
     lall, ijdim = state["rhog"].shape[0], state["rhog"].shape[-1]
     fp = field_ptrs(state)
     rep = N.Report()
-    lib().ora_mp_driver(ctypes.byref(P), ctypes.byref(G), lall, ijdim, ctypes.byref(fp), float(dt), float(ct),
-                        int(ice_adjust), ctypes.byref(rep))
+    if div_num == 1:
+        lib().ora_mp_driver(ctypes.byref(P), ctypes.byref(G), lall, ijdim, ctypes.byref(fp), float(dt), float(ct),
+                            int(ice_adjust), ctypes.byref(rep))
+    else:
+        lib().ora_mp_driver_div(ctypes.byref(P), ctypes.byref(G), lall, ijdim, ctypes.byref(fp), float(dt),
+                                float(ct), int(ice_adjust), int(div_num), ctypes.byref(rep))
     return rep
 
 
