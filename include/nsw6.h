@@ -50,8 +50,10 @@ enum nsw6_status {
     NSW6_BAD_ARG        = 3,
     NSW6_HIP_ERROR      = 4,
     NSW6_NO_DEVICE      = 5,
-    NSW6_UNSUPPORTED    = 6   /* the adjoint met a region that needs the general       */
+    NSW6_UNSUPPORTED    = 6,  /* the adjoint met a region that needs the general       */
                               /* (multi-cell) sedimentation; its gradient is left 0   */
+    NSW6_IO_ERROR       = 7   /* a snapshot file is missing, unreadable or of the     */
+                              /* wrong size                                           */
 };
 
 enum nsw6_precision {
@@ -222,6 +224,29 @@ int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count);
 int nsw6_timing_reset(nsw6_ctx *ctx);
 
 void nsw6_finalize(nsw6_ctx *ctx);
+
+/* NICAM snapshot files of the reference's stand-alone run, on HOST arrays in the
+ * nsw6_fields layout.  `dir` is a run directory holding data/<name>.dat (the inputs
+ * src/main.cpp:128-174 reads), data/vgrid/GRD_gz.dat and GRD_gzh.dat
+ * (src/mod_debug.cpp:189-195) and ref_verify/calculated_<name>_DP.dat
+ * (src/main.cpp:326-337).  Raw native doubles, row-major in the reader's shapes
+ * (src/data_io.cpp:207-402): [l][96][ij] fields, [l][6][96][ij] rhogq_Lswp/q_Lswp,
+ * [2][l][1][ij] precip_mp/precip1_mp/precip2_mp (precip, ISO1_precip, ISO2_precip),
+ * [l][1][ij] rhoein/lh/rhophi/rhokin_precip_mp, [l][ij] ix..jz; qke is rhogqke.
+ * Every file must hold exactly its shape's bytes.  NULL members of `fields` (and
+ * NULL gz/gzh) are skipped.  Returns NSW6_OK, NSW6_BAD_ARG or NSW6_IO_ERROR. */
+int nsw6_snapshot_read(const char *dir, int lall, int ijdim, const nsw6_fields *fields,
+                       double *gz, double *gzh);
+/* Write the same files (creating data/, data/vgrid/ under an existing `dir`). */
+int nsw6_snapshot_write(const char *dir, int lall, int ijdim, const nsw6_fields *fields,
+                        const double *gz, const double *gzh);
+/* The verification set: rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge and rhogq of
+ * QV..QG (calculated_QV1_DP.dat .. calculated_QG6_DP.dat), each [l][96][ij].
+ * `check` receives NSW6_VERIFY_FIELDS such arrays back to back. */
+#define NSW6_VERIFY_FIELDS 12
+int nsw6_verify_read(const char *dir, int lall, int ijdim, double *check);
+/* Write the verification set of a state (creating ref_verify/ under `dir`). */
+int nsw6_verify_write(const char *dir, int lall, int ijdim, const nsw6_fields *fields);
 
 const char *nsw6_status_string(int status);
 

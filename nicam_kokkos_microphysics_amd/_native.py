@@ -19,7 +19,9 @@ LIB_SYNTH = os.path.join(HERE, "libnsw6_synth.so")
 KALL, KMIN, KMAX, NQMAX = 96, 1, 94, 6
 NSW6_OK, NSW6_NOT_CONVERGED, NSW6_NAN_DETECTED = 0, 1, 2
 NSW6_BAD_ARG, NSW6_HIP_ERROR, NSW6_NO_DEVICE, NSW6_UNSUPPORTED = 3, 4, 5, 6
+NSW6_IO_ERROR = 7
 NSW6_MAX_DIV_NUM = 1000
+NSW6_VERIFY_FIELDS = 12
 NSW6_FP64, NSW6_MIXED = 0, 1
 
 # The 8 tunable coefficients (order of NSW6_NCOEF in nsw6.h).
@@ -99,6 +101,10 @@ SIGNATURES = {
     "nsw6_timing_reset": (_I, [_P]),
     "nsw6_finalize": (None, [_P]),
     "nsw6_status_string": (ctypes.c_char_p, [_I]),
+    "nsw6_snapshot_read": (_I, [ctypes.c_char_p, _I, _I, ctypes.POINTER(FieldPtrs), _P, _P]),
+    "nsw6_snapshot_write": (_I, [ctypes.c_char_p, _I, _I, ctypes.POINTER(FieldPtrs), _P, _P]),
+    "nsw6_verify_read": (_I, [ctypes.c_char_p, _I, _I, _P]),
+    "nsw6_verify_write": (_I, [ctypes.c_char_p, _I, _I, ctypes.POINTER(FieldPtrs)]),
 }
 
 _lib = None

@@ -58,17 +58,20 @@ def build(verbose: bool = True) -> None:
     obj_r = os.path.join(BUILD, "nsw6_adjoint.o")
     obj_a = os.path.join(BUILD, "nsw6_api.o")
     obj_h = os.path.join(BUILD, "nsw6_host.o")
+    obj_s = os.path.join(BUILD, "nsw6_snapshot.o")
     steps = [
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nsw6_kernels.hip"), "-o", obj_k],
         [HIPCC, *HIP_FLAGS, *ADJ_FLAGS, "-c", os.path.join(CSRC, "nsw6_adjoint.hip"), "-o", obj_r],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "nsw6_api.cpp"), "-o", obj_a],
         ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_host.c"), "-o", obj_h],
+        ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_snapshot.c"), "-o", obj_s],
     ]
     with ThreadPoolExecutor(max_workers=len(steps)) as pool:
         for fut in [pool.submit(_run, cmd, verbose) for cmd in steps]:
             fut.result()
     tmp = LIB_NSW6 + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_r, obj_a, obj_h], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_r, obj_a, obj_h, obj_s],
+         verbose)
     os.replace(tmp, LIB_NSW6)
     tmp = LIB_SYNTH + ".tmp"
     _run(["gcc", *C_FLAGS, "-fopenmp", "-shared", "-o", tmp, os.path.join(CSRC, "nsw6_synth.c"),

@@ -152,7 +152,8 @@ const char *nsw6_status_string(int status)
     case NSW6_BAD_ARG: return "bad argument";
     case NSW6_HIP_ERROR: return "HIP runtime error";
     case NSW6_NO_DEVICE: return "no HIP device";
-    case NSW6_UNSUPPORTED: return "not supported by the reverse mode (non-default scheme or a region needing the general sedimentation)";
+    case NSW6_UNSUPPORTED: return "not supported by the reverse mode (non-default scheme, sub-cycling, or a region needing the general sedimentation)";
+    case NSW6_IO_ERROR: return "snapshot file missing, unreadable or of the wrong size";
     default: return "unknown status";
     }
 }

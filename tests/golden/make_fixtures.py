@@ -9,6 +9,8 @@ reference at run time).
                                reference sources: problemsize.h constants, the
                                GlobalParams defaults, the Roh & Satoh coefficient
                                arrays and the Bergeron tables.
+* snapshot_files.json      -- names and reader ranks of the snapshot files the
+                               stand-alone run reads (data/, ref_verify/).
 """
 import json
 import os
@@ -60,6 +62,14 @@ def main():
     with open(os.path.join(OUT, "reference_constants.json"), "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
     print(f"wrote {len(consts)} constants, {len(defaults)} defaults, tables {list(tabs)}")
+
+    # the snapshot files of the stand-alone run: reader rank and name, in read order
+    main_src = strip_comments(read("src/main.cpp"))
+    files = [{"rank": int(m.group(1)), "dir": m.group(2), "name": m.group(3)}
+             for m in re.finditer(r'read_data_(\d)d\("(data|ref_verify)/(\w+)\.dat"', main_src)]
+    with open(os.path.join(OUT, "snapshot_files.json"), "w") as fh:
+        json.dump({"files": files, "source": "parsed from src/main.cpp"}, fh, indent=1)
+    print(f"wrote {len(files)} snapshot file names")
 
 
 if __name__ == "__main__":
