@@ -86,6 +86,14 @@ typedef struct nsw6_params {
     double kk_aut_rate;   /* 1350.0  in Praut_kk  (src/mod_mp_nsw6.cpp:1154) */
     double icenuc_coef;   /* 4.92E-11 in Qi0      (src/mod_mp_nsw6.cpp:1083) */
     double dgamma_a, GAM_dgam, GAM_dgam23, coef_dgam, coef_xf;  /* src/mod_mp_nsw6.cpp:511-516 */
+    /* Cloud-top diagnostics rctop, tctop, rwtop.  0 (default): the reference as
+     * written (src/mod_mp_nsw6.cpp:1575-1582): every level writes sw*rceff +
+     * (1-sw)*UNDEF with sw = [qc+qr > q_min]*zerosw, and the serial order leaves level
+     * kmax.  1: the cloud top its comments describe: sw = [qc+qr > q_min]*(1-zerosw)
+     * ("if qc+qr > 0.01[g/kg], sw=1"), and the highest level with sw = 1 gives
+     * rctop = rceff and tctop = T there (UNDEF when there is none); rwtop as
+     * TagUpdateCLD (src/mod_mp_nsw6.cpp:717-720).  Deterministic in both modes. */
+    double cloudtop_intended;
 } nsw6_params;
 
 /* The 8 tunable NSW6 coefficients, in this order, for gradients and perturbations:

@@ -69,7 +69,7 @@ static int hip_ok(hipError_t e) { return e == hipSuccess ? NSW6_OK : NSW6_HIP_ER
 static void free_ws(nsw6_ctx *c)
 {
     Workspace &w = c->ws;
-    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.flux, w.part, w.bounds, w.region_flag, w.report,
+    void *ptrs[] = {w.rhog, w.rhoge, w.rhogq, w.vt, w.flux, w.part, w.bounds, w.region_flag, w.report, w.cloud_top,
                     c->adj.rec, c->adj.diag, c->adj.part, c->adj.coef};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -128,6 +128,7 @@ static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, 
     a.grid = ctx->grid_dev;
     a.lall = lall;
     a.ijdim = ijdim;
+    a.cloudtop_intended = ctx->P.cloudtop_intended != 0.0;
     a.dt = dt;
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
@@ -177,7 +178,9 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
         hipMalloc(&w.vt, sizeof(double) * K2 * 3) != hipSuccess ||
         hipMalloc(&w.flux, sizeof(double) * K2 * FLUX_SLOTS) != hipSuccess ||
         hipMalloc(&w.report, sizeof(int) * REPORT_INTS) != hipSuccess ||
-        hipMemset(w.report, 0, sizeof(int) * REPORT_INTS) != hipSuccess)
+        hipMemset(w.report, 0, sizeof(int) * REPORT_INTS) != hipSuccess ||
+        hipMalloc(&w.cloud_top, sizeof(int) * (size_t)max_columns) != hipSuccess ||
+        hipMemset(w.cloud_top, 0xff, sizeof(int) * (size_t)max_columns) != hipSuccess)
         st = NSW6_HIP_ERROR;
     if (st == NSW6_OK) st = ensure_regions(c, 64);
     if (st != NSW6_OK) {

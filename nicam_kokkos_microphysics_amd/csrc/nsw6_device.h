@@ -102,6 +102,7 @@ struct DevParams {
     // SATURATION_Setrange(273.16, 233.16) when explicit ice generation is off
     // (sw_expice != 1), else satadjust_liq (range 100, 90)
     double ice_adjust, sat_ulimit, sat_llimit;
+    double cloudtop_intended;   // 1: highest cloudy level (see nsw6_params)
 };
 
 NSW6_HD double dmax(double a, double b) { return (a < b) ? b : a; }
@@ -176,7 +177,8 @@ NSW6_HD RateSwitches rate_switches(const DevParams &P, double dens, double temp,
 // (src/mod_mp_nsw6.cpp:1566-1592), in FP64 for both forms.
 NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double dens,
                                                  double temp, const double q[6], double &rceff,
-                                                 double &rceff_cld, double &rctop, double &tctop)
+                                                 double &rceff_cld, double &rctop, double &tctop,
+                                                 double &cloud_sw)
 {
     const double qc = dmax(q[QC], 0.0), qr = dmax(q[QR], 0.0);
     const double zr = sw.zr;
@@ -196,6 +198,8 @@ NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double
     const double s = step_pos((qc + qr) - 1.0E-5) * zerosw;
     rctop = (s)*rceff + (1.0 - s) * C_UNDEF;
     tctop = (s)*temp + (1.0 - s) * C_UNDEF;
+    // the switch the comment of src/mod_mp_nsw6.cpp:1577 describes (cloud present)
+    cloud_sw = step_pos((qc + qr) - 1.0E-5) * (1.0 - zerosw);
     zerosw = step_neg(r2_qc - 1.0E-10);
     rceff_cld = r3_qc / (r2_qc + zerosw) * (1.0 - zerosw);
 }

@@ -97,6 +97,7 @@ struct Workspace {
     int *bounds;                    // [l][BOUNDS_PER_REGION]
     int *region_flag;               // [l] 1 = region needs the general sweep
     int *report;                    // [REPORT_INTS]
+    int *cloud_top;                 // [l][ij] highest cloudy level of the step, -1 = none
 };
 
 struct StepArgs {
@@ -107,6 +108,7 @@ struct StepArgs {
     const GridDev *grid;
     int lall, ijdim;
     int keep_report;              // 1: keep the report of earlier sub-steps of one call
+    int cloudtop_intended;        // 1: cloud top = highest cloudy level (k_cloudtop)
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 

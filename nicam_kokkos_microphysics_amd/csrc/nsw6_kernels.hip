@@ -100,11 +100,14 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 #pragma unroll
     for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
     if (!DIAG) return;
-    double rceff, rceff_cld, rct, tct;
-    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct);
+    double rceff, rceff_cld, rct, tct, csw;
+    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct, csw);
     f.re_liquid[p] = rceff;
     f.re_cld[p] = rceff_cld;
-    if (k == KMAX) {
+    if (a.cloudtop_intended) {
+        // integer max: the same level whatever the order of the levels' threads
+        if (csw != 0.0) atomicMax(&a.ws.cloud_top[(int64_t)l * ijdim + ij], k);
+    } else if (k == KMAX) {
         // every level writes the cloud-top row in the reference; the serial order leaves kmax
         const int64_t pc = (int64_t)l * ijdim + ij;
         f.rctop[pc] = rct;
@@ -112,6 +115,30 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
         const double sw = step_pos(tct - C_TEM00);
         f.rwtop[pc] = (sw)*rct + (1.0 - sw) * C_UNDEF;
     }
+}
+
+// Cloud top with the semantics of src/mod_mp_nsw6.cpp:1575-1582's comments: the
+// highest level whose cloud switch fired in k_rates gives rctop = rceff and tctop = T
+// there (UNDEF without one); rwtop as TagUpdateCLD (:717-720).  Runs after k_rates
+// and before k_finish rewrites tem, and clears the level for the next step.
+__global__ void __launch_bounds__(256) k_cloudtop(StepArgs a)
+{
+    const int l = blockIdx.y;
+    const int ij = blockIdx.x * 256 + threadIdx.x;
+    if (ij >= a.ijdim) return;
+    const int64_t pc = (int64_t)l * a.ijdim + ij;
+    const int top = a.ws.cloud_top[pc];
+    double rct = C_UNDEF, tct = C_UNDEF;
+    if (top >= KMIN && top <= KMAX) {
+        const int64_t p = ((int64_t)l * KALL + top) * a.ijdim + ij;
+        rct = a.f.re_liquid[p];
+        tct = a.f.tem[p];
+    }
+    a.f.rctop[pc] = rct;
+    a.f.tctop[pc] = tct;
+    const double sw = step_pos(tct - C_TEM00);
+    a.f.rwtop[pc] = (sw)*rct + (1.0 - sw) * C_UNDEF;
+    a.ws.cloud_top[pc] = -1;
 }
 
 // zdis_interior / zdis_bottom and the point-wise wall speeds are in nsw6_step.h.
@@ -943,6 +970,8 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
         hipLaunchKernelGGL((k_rates<float, true>), grid_p, dim3(256), 0, stream, a);
     else
         hipLaunchKernelGGL((k_rates<double, true>), grid_p, dim3(256), 0, stream, a);
+    if (a.cloudtop_intended)
+        hipLaunchKernelGGL(k_cloudtop, dim3((a.ijdim + 255) / 256, a.lall), dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
     hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);

@@ -750,6 +750,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     sw = (0.5 + copysign(0.5, (qc + qr) - q_min)) * zerosw;
     o->rctop = (sw)*o->rceff + (1.0 - sw) * UNDEF;
     o->tctop = (sw)*temp + (1.0 - sw) * UNDEF;
+    o->cloud_sw = (0.5 + copysign(0.5, (qc + qr) - q_min)) * (1.0 - zerosw);
     zerosw = 0.5 - copysign(0.5, r2_qc - r2_min);
     o->rceff_cld = r3_qc / (r2_qc + zerosw) * (1.0 - zerosw);
     sw = (0.5 + copysign(0.5, qc - q_min)) * zerosw;
@@ -1409,6 +1410,8 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
     double *Nc = zalloc(K2), *a1 = zalloc(K2), *a2 = zalloc(K2), *ma2 = zalloc(K2);
     double *Vt = zalloc((size_t)NQMAX * K2), *cva = zalloc(K2), *rgs = zalloc(K2), *rgsh = zalloc(K2);
     double *rctop_cld = zalloc((size_t)ijdim), *tctop_cld = zalloc((size_t)ijdim);
+    const int intended = P->cloudtop_intended != 0.0;
+    double *csw = intended ? zalloc(K2) : NULL;
 
     /* init() */
     negative_filter(ijdim, R->rhog, R->rhoge, R->rhogq, R->rho, R->tem, R->pre, R->q, R->gsgam2);
@@ -1446,8 +1449,9 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
             Vt[(size_t)I_QG * K2 + p] = o.Vt[I_QG];
             re_liquid[p] = o.rceff;
             re_cld[p] = o.rceff_cld;
+            if (intended) csw[p] = o.cloud_sw;
             /* QUIRK A-10: every k writes the cloud-top row; serial semantics (k = kmax wins) */
-            if (k == KMAX) {
+            else if (k == KMAX) {
                 rctop[ij] = o.rctop;
                 tctop[ij] = o.tctop;
                 rctop_cld[ij] = o.rctop_cld;
@@ -1464,6 +1468,21 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
                           LHF * drhogq[(size_t)I_QI * K2 + p] + LHF * drhogq[(size_t)I_QS * K2 + p] +
                           LHF * drhogq[(size_t)I_QG * K2 + p];
         }
+    if (intended) {
+        /* cloud top as a serial update over the levels, bottom to top:
+         * x = sw*value + (1-sw)*x from x = UNDEF, so the highest cloudy level wins */
+        for (int ij = 0; ij < ijdim; ij++) {
+            double rct = CONST_UNDEF, tct = CONST_UNDEF;
+            for (int k = KMIN; k <= KMAX; k++) {
+                const double s = csw[IDX(k, ij)];
+                rct = (s)*re_liquid[IDX(k, ij)] + (1.0 - s) * rct;
+                tct = (s)*R->tem[IDX(k, ij)] + (1.0 - s) * tct;
+            }
+            rctop[ij] = rct;
+            tctop[ij] = tct;
+        }
+        free(csw);
+    }
     /* TagUpdateCLD */
     for (int ij = 0; ij < ijdim; ij++) {
         re_liquid[IDX(KMIN - 1, ij)] = 0.0;

@@ -23,6 +23,8 @@ typedef struct ora_bigloop_out {
     double Vt[6];         /* Vt(QR), Vt(QI), Vt(QS), Vt(QG); QV, QC left 0     */
     double rceff, rceff_cld;
     double rctop, tctop, rctop_cld, tctop_cld;
+    double cloud_sw;      /* [qc+qr > q_min]*(1-zerosw): the cloud switch of the
+                             comment at src/mod_mp_nsw6.cpp:1577 (cloudtop_intended) */
     double wk[49];        /* final process-rate work array                    */
     double q_t[6];        /* final tendencies qv_t..qg_t                       */
 } ora_bigloop_out;

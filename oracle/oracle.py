@@ -76,7 +76,8 @@ def lib():
 class BigLoopOut(ctypes.Structure):
     _fields_ = [("drhogq", ctypes.c_double * 6), ("Vt", ctypes.c_double * 6), ("rceff", ctypes.c_double),
                 ("rceff_cld", ctypes.c_double), ("rctop", ctypes.c_double), ("tctop", ctypes.c_double),
-                ("rctop_cld", ctypes.c_double), ("tctop_cld", ctypes.c_double), ("wk", ctypes.c_double * 49),
+                ("rctop_cld", ctypes.c_double), ("tctop_cld", ctypes.c_double), ("cloud_sw", ctypes.c_double),
+                ("wk", ctypes.c_double * 49),
                 ("q_t", ctypes.c_double * 6)]
 
 
