@@ -94,6 +94,15 @@ typedef struct nsw6_params {
      * rctop = rceff and tctop = T there (UNDEF when there is none); rwtop as
      * TagUpdateCLD (src/mod_mp_nsw6.cpp:717-720).  Deterministic in both modes. */
     double cloudtop_intended;
+    /* OPT_INDIR (src/mod_mp_nsw6.cpp:131,475-484): 1 = aerosol indirect effect, the
+     * droplet number is Nc = max(unccn * 1e-6, Nc_def) per point (TagAerosolT,
+     * :676-679) instead of Nc_def; the reference default is 0. */
+    double opt_indir;
+    /* PRCIP_TRN_ECORRECT (src/problemsize.cpp:39): 0 = "KIN2EIN" (the reference
+     * default: the kinetic energy lost by the sedimentation flux goes to internal
+     * energy), 1 = "KIN2KIN" (src/mod_precip_transport.cpp:1012-1068: the velocities
+     * are rescaled to keep the transported kinetic energy; internal energy untouched). */
+    double kin2kin;
 } nsw6_params;
 
 /* The 8 tunable NSW6 coefficients, in this order, for gradients and perturbations:

@@ -129,6 +129,8 @@ static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, 
     a.lall = lall;
     a.ijdim = ijdim;
     a.cloudtop_intended = ctx->P.cloudtop_intended != 0.0;
+    a.opt_indir = ctx->P.opt_indir != 0.0;
+    a.kin2kin = ctx->P.kin2kin != 0.0;
     a.dt = dt;
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
@@ -228,6 +230,7 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
                            fl->GPREC};
     for (const double *p : req)
         if (!p) return NSW6_BAD_ARG;
+    if (ctx->P.opt_indir != 0.0 && !fl->unccn) return NSW6_BAD_ARG;   // read under OPT_INDIR
     int st = use_device(ctx);
     if (st == NSW6_OK) st = ensure_regions(ctx, lall);
     if (st != NSW6_OK) return st;
@@ -280,7 +283,8 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
 // The reverse mode covers the default scheme only (see include/nsw6.h).
 static bool adjoint_supported(const DevParams &P)
 {
-    return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0;
+    return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0 && P.opt_indir == 0.0 &&
+           P.kin2kin == 0.0;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)

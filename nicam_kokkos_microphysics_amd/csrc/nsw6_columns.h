@@ -157,6 +157,28 @@ NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
     }
 }
 
+// KIN2KIN at wall kmax+1 of a single-cell region, for the thread of halo cell
+// kmax+1: rhogkin_v is 0 at that wall, and the density of cell kmax after
+// sedimentation (its lower neighbour) is formed from the fluxes of walls kmax and
+// kmax+1 as sediment() forms it.  rhog_top is the halo cell's density.
+NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top)
+{
+    const int ijdim = a.ijdim;
+    const GridDev &g = *a.grid;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t o = (int64_t)l * K2 + (int64_t)(KALL - 1) * ijdim + ij;
+    const int64_t om = o - ijdim;
+    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
+    const double rc = g.rdgz[KMAX];
+    double rg = a.ws.rhog[om];
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+        rg = rg + -(fx[(FX_Q + t) * K2 + (int64_t)(KALL - 1) * ijdim] - fx[(FX_Q + t) * K2 + (int64_t)KMAX * ijdim]) * rc;
+    const double gs = a.f.gsgam2[o], gsh = a.f.gsgam2h[o], gs_m = a.f.gsgam2[om];
+    const double c2w0 = g.afact[KALL - 1] / gs * gsh, c2w1 = g.bfact[KALL - 1] / gs_m * gsh;
+    return kin2kin_v(c2w0 * rhog_top + c2w1 * rg, 0.0 + 0.0, a.f.rhogw[o] + 0.0);
+}
+
 // Inputs of sediment() for interior cell c of column (l, ij) from the flux workspace.
 NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedIn<double> &x)
 {

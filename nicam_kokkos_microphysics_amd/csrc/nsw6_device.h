@@ -103,12 +103,39 @@ struct DevParams {
     // (sw_expice != 1), else satadjust_liq (range 100, 90)
     double ice_adjust, sat_ulimit, sat_llimit;
     double cloudtop_intended;   // 1: highest cloudy level (see nsw6_params)
+    double opt_indir;           // 1: Nc = max(unccn * 1e-6, Nc_def) per point
+    double kin2kin;             // 1: PRCIP_TRN_ECORRECT = "KIN2KIN"
 };
+
 
 NSW6_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 NSW6_HD double dmin(double a, double b) { return (b < a) ? b : a; }
 NSW6_HD float dmax(float a, float b) { return (a < b) ? b : a; }
 NSW6_HD float dmin(float a, float b) { return (b < a) ? b : a; }
+
+// Cloud droplet number of a point and the two functions of it the rates use:
+// Nc_def everywhere (the constants formed on the host), or under the aerosol
+// indirect effect max(unccn * 1e-6, Nc_def) (src/mod_mp_nsw6.cpp:676-679) with
+// Nc^-1.79 (:1154) and Dc (:1146) evaluated at the point.
+struct NcPoint {
+    double Nc, pow_m179, Dc;
+};
+NSW6_HD NcPoint nc_default(const DevParams &P)
+{
+    NcPoint n;
+    n.Nc = P.Nc_def;
+    n.pow_m179 = P.Nc_pow_m179;
+    n.Dc = P.Dc;
+    return n;
+}
+NSW6_HD NcPoint nc_indirect(const DevParams &P, double unccn)
+{
+    NcPoint n;
+    n.Nc = dmax(unccn * 1.0E-6, P.Nc_def);
+    n.pow_m179 = pow(n.Nc, -1.79);
+    n.Dc = 0.146 - 5.964E-2 * log(n.Nc / 2000.0);
+    return n;
+}
 
 // 0.5 + copysign(0.5, x): 1 for x >= +0, 0 for x <= -0 (IEEE sign, as the reference).
 template <typename R> NSW6_HD R step_pos(R x) { return R(0.5) + copysign(R(0.5), x); }
@@ -176,7 +203,7 @@ NSW6_HD RateSwitches rate_switches(const DevParams &P, double dens, double temp,
 // Effective radius of liquid water and the cloud-top diagnostics of one point
 // (src/mod_mp_nsw6.cpp:1566-1592), in FP64 for both forms.
 NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double dens,
-                                                 double temp, const double q[6], double &rceff,
+                                                 double temp, const double q[6], double Nc, double &rceff,
                                                  double &rceff_cld, double &rctop, double &tctop,
                                                  double &cloud_sw)
 {
@@ -184,7 +211,6 @@ NSW6_HD void effective_radius(const DevParams &P, const RateSwitches &sw, double
     const double zr = sw.zr;
     const double RLMDr = sqrt(sqrt(dens * qr * P.rK_r + zr)) * (1.0 - zr);
     const double RLMDr_3 = (RLMDr * RLMDr) * RLMDr;
-    const double Nc = P.Nc_def;
     const double rhoqc = dens * qc * 1000.0;
     const double coef_xf = P.coef_xf;
     const double xf_qc = rhoqc / Nc * 1.0E-9;
@@ -250,9 +276,10 @@ template <typename R> struct RateOut {
     R Vtr, Vti, Vts, Vtg;
 };
 
-// TagBigLoop at one point.
+// TagBigLoop at one point; `nc` is the point's droplet number (nc_default or
+// nc_indirect).
 template <typename R>
-NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
+NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o, const NcPoint &nc)
 {
     const R PI = R(C_PI), EPS = R(C_EPS), UNDEF = R(C_UNDEF);
     const R Rvap = R(C_Rvap), LHV0 = R(C_LHV0), LHS0 = R(C_LHS0), LHF0 = R(C_LHF0);
@@ -440,14 +467,14 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
               (MOMs_0bs * GAM_3 * RLMDg_3 + R(2) * MOMs_1bs * GAM_2 * RLMDg_2 + MOMs_2bs * GAM * RLMDg);
 
     // autoconversion
-    const R Nc = R(P.Nc_def);
+    const R Nc = R(nc.Nc);
     const R rhoqc = dens * qc * R(1000);
-    const R Praut_kk = in.tun[0] * exp(R(2.47) * log(qc)) * R(P.Nc_pow_m179);
+    const R Praut_kk = in.tun[0] * exp(R(2.47) * log(qc)) * R(nc.pow_m179);
     R Praut;
     if (kk) {
         Praut = Praut_kk;
     } else {
-        const R Dc = R(P.Dc);
+        const R Dc = R(nc.Dc);
         const R Praut_berry = Rdens * R(1.67E-5) * rhoqc * rhoqc / (R(5) + R(3.66E-2) * Nc / (Dc * rhoqc + EPS));
         Praut = (R(1) - sw_kk) * Praut_berry + sw_kk * Praut_kk;
     }
@@ -630,6 +657,11 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
     o.Vti = Vti;
     o.Vts = Vts;
     o.Vtg = Vtg;
+}
+template <typename R>
+NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o)
+{
+    rates<R>(P, dt, in, o, nc_default(P));
 }
 
 }  // namespace nsw6

@@ -109,6 +109,8 @@ struct StepArgs {
     int lall, ijdim;
     int keep_report;              // 1: keep the report of earlier sub-steps of one call
     int cloudtop_intended;        // 1: cloud top = highest cloudy level (k_cloudtop)
+    int opt_indir;                // 1: aerosol indirect effect (k_rates<R, DIAG, true>)
+    int kin2kin;                  // 1: KIN2KIN energy correction of the sedimentation
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 

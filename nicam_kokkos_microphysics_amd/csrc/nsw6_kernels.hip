@@ -45,7 +45,7 @@ __constant__ double c_a2_tab[32] = {
 // ---------------------------------------------------------------------------------
 // DIAG = false (the forward part of the reverse mode) skips the effective radii and
 // cloud-top diagnostics, which carry no cotangent.
-template <typename R, bool DIAG>
+template <typename R, bool DIAG, bool INDIR = false>
 __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -90,8 +90,9 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     x.gsgam2 = f.gsgam2[p];
     double tun[NCOEF];
     tunables(P, tun);
+    const NcPoint nc = INDIR ? nc_indirect(P, f.unccn[p]) : nc_default(P);
     PointOut<double> y;
-    point_rates<double, R>(P, a.dt, x, tun, c_a1_tab, c_a2_tab, y);
+    point_rates<double, R>(P, a.dt, x, tun, c_a1_tab, c_a2_tab, y, nc);
 
     a.ws.rhog[p] = y.rhog;
     a.ws.rhoge[p] = y.rhoge;
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
     if (!DIAG) return;
     double rceff, rceff_cld, rct, tct, csw;
-    effective_radius(P, y.sw, y.rho, x.tem, y.q, rceff, rceff_cld, rct, tct, csw);
+    effective_radius(P, y.sw, y.rho, x.tem, y.q, nc.Nc, rceff, rceff_cld, rct, tct, csw);
     f.re_liquid[p] = rceff;
     f.re_cld[p] = rceff_cld;
     if (a.cloudtop_intended) {
@@ -538,7 +539,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             SedIn<double> si;
             load_sed(a, l, ij, k, si);
             SedOut<double> so;
-            sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so);
+            sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so, a.kin2kin != 0);
             f.rhogvx[p] = so.rvx;
             f.rhogvy[p] = so.rvy;
             f.rhogvz[p] = so.rvz;
@@ -560,7 +561,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             f.rhogvx[p] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
             f.rhogvy[p] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
             f.rhogvz[p] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
-            f.rhogw[p] = f.rhogw[p] + 0.0;
+            f.rhogw[p] = (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall(a, l, ij, x.rhog) : f.rhogw[p] + 0.0;
         }
     }
     x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
@@ -632,6 +633,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     WindWindow wind[3];
     double fprev[3][6], fprevh[3][2];
     double rhog_new_prev = 0.0;       // rhog after sedimentation of the last finalised cell
+    double rg_kmax = 0.0;             // rhog of cell kmax after sedimentation
     double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
 
     // cell 0 (halo) and cell 1
@@ -644,7 +646,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         fm1 = fdummy;     // cell 0 record (w-1 for w = 1)
         load_tcell(s, 1, cur, rhog_cur, q_cur, gs_cur);
         load_wall(s, g, 1, rhog_cur, rg0, q_cur, q0, gs_cur, gs0, cur, f0);
-        rhog_new_prev = 0.0;
+        rhog_new_prev = rg0;          // halo cell 0 keeps its density
     }
 #pragma unroll
     for (int t = 0; t < 3; t++) {
@@ -802,10 +804,10 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             const double kinh_u = kinh + fm2.drhokinh;
             const double kinv_c = fm2.kinv + fm2.drhokinv;
             const double kinv_cp = fm1.kinv + fm1.drhokinv;
-            const double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
-            const double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
-            const double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
-            const double rw_n = fm2.rhogw + fm2.drhogw;
+            double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
+            double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
+            double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
+            double rw_n = fm2.rhogw + fm2.drhogw;
             double rg = fm2.rhog;
             // nq = 1..5: QC(0), QR, QI(0), QS, QG
             rq[QC] = rq[QC] + 0.0;
@@ -830,7 +832,12 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             const double kinv_new_cp =
                 (c + 1 <= KMAX) ? 0.5 * (rw_cp * rw_cp) / (fm1.c2w0 * rg_cp + fm1.c2w1 * rg) : 0.0;
             const double tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
-            e = e + (tmp2 - tmpk);
+            if (a.kin2kin) {
+                kin2kin_h(rg, kinh_u, rvx_n, rvy_n, rvz_n);
+                rw_n = kin2kin_v(fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev, kinv_c, rw_n);
+            } else {
+                e = e + (tmp2 - tmpk);
+            }
             rhog_new_prev = rg;
             if (GENERAL) {
                 mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
@@ -874,10 +881,10 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         const double kinh_u = kinh + fm2.drhokinh;
         const double kinv_c = fm2.kinv + fm2.drhokinv;
         const double kinv_cp = 0.0 + 0.0;    // wall 95
-        const double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
-        const double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
-        const double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
-        const double rw_n = fm2.rhogw + fm2.drhogw;
+        double rvx_n = rvx + fm2.drhogu * s.ix + fm2.drhogv * s.jx;
+        double rvy_n = rvy + fm2.drhogu * s.iy + fm2.drhogv * s.jy;
+        double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
+        double rw_n = fm2.rhogw + fm2.drhogw;
         double rg = fm2.rhog;
         rq[QC] = rq[QC] + 0.0;
         rq[QR] = rq[QR] + fm2.drhoq[0];
@@ -895,7 +902,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
         const double kinv_new_c = 0.5 * (rw_n * rw_n) / (fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev);
         const double tmpk = kinh_new + (w2c0 * 0.0 + w2c1 * kinv_new_c);
-        e = e + (tmp2 - tmpk);
+        if (a.kin2kin) {
+            kin2kin_h(rg, kinh_u, rvx_n, rvy_n, rvz_n);
+            rw_n = kin2kin_v(fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev, kinv_c, rw_n);
+        } else {
+            e = e + (tmp2 - tmpk);
+        }
+        rg_kmax = rg;
         if (GENERAL) {
             mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
             mom[1 * s.K2 + (int64_t)c * ijdim] = rvy_n;
@@ -917,7 +930,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         const double rvx_n = f.rhogvx[o] + 0.0 * s.ix + 0.0 * s.jx;
         const double rvy_n = f.rhogvy[o] + 0.0 * s.iy + 0.0 * s.jy;
         const double rvz_n = f.rhogvz[o] + 0.0 * s.iz + 0.0 * s.jz;
-        const double rw_n = f.rhogw[o] + 0.0;
+        double rw_n = f.rhogw[o] + 0.0;
+        if (a.kin2kin && hc) {
+            // KIN2KIN at wall kmax+1: rhogkin_v there is 0, the lower cell is kmax
+            const double gs = f.gsgam2[o], gsh = f.gsgam2h[o], gs_m = f.gsgam2[o - ijdim];
+            const double c2w0 = g.afact[c] / gs * gsh, c2w1 = g.bfact[c] / gs_m * gsh;
+            rw_n = kin2kin_v(c2w0 * (a.ws.rhog[o] + 0.0) + c2w1 * rg_kmax, 0.0 + 0.0, rw_n);
+        }
         if (GENERAL) {
             mom[0 * s.K2 + (int64_t)c * ijdim] = rvx_n;
             mom[1 * s.K2 + (int64_t)c * ijdim] = rvy_n;
@@ -966,8 +985,12 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
     if (ev) (void)hipEventRecord(ev[0], stream);
-    if (precision == 1)
+    if (precision == 1 && a.opt_indir)
+        hipLaunchKernelGGL((k_rates<float, true, true>), grid_p, dim3(256), 0, stream, a);
+    else if (precision == 1)
         hipLaunchKernelGGL((k_rates<float, true>), grid_p, dim3(256), 0, stream, a);
+    else if (a.opt_indir)
+        hipLaunchKernelGGL((k_rates<double, true, true>), grid_p, dim3(256), 0, stream, a);
     else
         hipLaunchKernelGGL((k_rates<double, true>), grid_p, dim3(256), 0, stream, a);
     if (a.cloudtop_intended)

@@ -59,6 +59,8 @@ inline void to_dev_params(const nsw6_params &p, DevParams &d)
     d.sat_ulimit = d.ice_adjust != 0.0 ? 273.16 : 100.0;
     d.sat_llimit = d.ice_adjust != 0.0 ? 233.16 : 90.0;
     d.cloudtop_intended = (p.cloudtop_intended != 0.0) ? 1.0 : 0.0;
+    d.opt_indir = (p.opt_indir != 0.0) ? 1.0 : 0.0;
+    d.kin2kin = (p.kin2kin != 0.0) ? 1.0 : 0.0;
 }
 
 // GRD_Setup (src/mod_debug.cpp:92-187) plus GRD_gz_shift of precip_transport_new

@@ -44,7 +44,7 @@ template <typename S> NSW6_HD S psat_of(double LovR, double RTEM00, S tem)
 // sw_expice != 1; the Bergeron parameters then enter as values, not differentiated).
 template <typename S, typename R>
 NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, const S tun[NCOEF],
-                         const double *a1_tab, const double *a2_tab, PointOut<S> &y)
+                         const double *a1_tab, const double *a2_tab, PointOut<S> &y, const NcPoint &nc)
 {
     S rq[NQ];
 #pragma unroll
@@ -103,7 +103,7 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
 #pragma unroll
     for (int i = 0; i < NCOEF; i++) in.tun[i] = R(tun[i]);
     RateOut<R> o;
-    rates<R>(P, R(dt), in, o);
+    rates<R>(P, R(dt), in, o, nc);
 
     // drhogq = rhog * q_t * dt with qv_t = -(sum of the others) (src/mod_mp_nsw6.cpp:
     // 1548-1559) in the precise type, so the point's total water is conserved to its
@@ -130,6 +130,12 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
 #pragma unroll
     for (int n = 0; n < NQ; n++) y.q[n] = q[n];
     y.sw = in.sw;
+}
+template <typename S, typename R>
+NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, const S tun[NCOEF],
+                         const double *a1_tab, const double *a2_tab, PointOut<S> &y)
+{
+    point_rates<S, R>(P, dt, x, tun, a1_tab, a2_tab, y, nc_default(P));
 }
 
 // ---------------------------------------------------------------------------------
@@ -421,8 +427,40 @@ template <typename S> struct SedOut {
     S rg, e, rq[3], rvx, rvy, rvz, rw;
 };
 
+// KIN2KIN (src/mod_precip_transport.cpp:1012-1068), the alternative to KIN2EIN: the
+// velocities after sedimentation are rescaled so that each point keeps the kinetic
+// energy the transport carried into it.  Horizontal, at cell c in [kmin, kmax]:
+// v = rhogv / rhog scaled by sqrt(|kin_h0 / kin_h|) with kin_h0 = rhogkin_h / rhog.
+template <typename S> NSW6_HD void kin2kin_h(S rg, S kinh, S &rvx, S &rvy, S &rvz)
+{
+    const S kin_h0 = kinh / rg;
+    S vx = rvx / rg, vy = rvy / rg, vz = rvz / rg;
+    const S kin_h = 0.5 * (vx * vx + vy * vy + vz * vz);
+    if (kin_h > 1.0E-20) {
+        const S f = sqrt(fabs(kin_h0 / kin_h));
+        vx = vx * f;
+        vy = vy * f;
+        vz = vz * f;
+    }
+    rvx = vx * rg;
+    rvy = vy * rg;
+    rvz = vz * rg;
+}
+// Vertical, at wall k in [kmin, kmax+1]: rhogh = C2W0 rhog(k) + C2W1 rhog(k-1) (after
+// sedimentation), w = rhogw / rhogh scaled by sqrt(|kin_v0 / kin_v|), kin_v0 =
+// rhogkin_v / rhogh, kin_v = w^2 / 2; returns the new rhogw.
+template <typename S> NSW6_HD S kin2kin_v(S rhogh, S kinv, S rw)
+{
+    const S kin_v0 = kinv / rhogh;
+    S w = rw / rhogh;
+    const S kin_v = 0.5 * (w * w);
+    if (kin_v > 1.0E-20) w = w * sqrt(fabs(kin_v0 / kin_v));
+    return w * rhogh;
+}
+
 template <typename S>
-NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x, SedOut<S> &y)
+NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x, SedOut<S> &y,
+                      bool kin2kin = false)
 {
     // tendencies of cell c (walls c and c+1; wall 95 carries zero fluxes)
     const double rc = g.rdgz[c];
@@ -492,7 +530,12 @@ NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x,
     const S kinv_new_c = (c >= KMIN + 1) ? S(0.5 * (y.rw * y.rw) / (c2w0_c * rg + c2w1_c * rg_cm)) : S(0.0);
     const S kinv_new_cp = (c + 1 <= KMAX) ? S(0.5 * (rw_cp * rw_cp) / (c2w0_p * rg_cp + c2w1_p * rg)) : S(0.0);
     const S tmpk = kinh_new + (w2c0 * kinv_new_cp + w2c1 * kinv_new_c);
-    e = e + (tmp2 - tmpk);
+    if (kin2kin) {
+        kin2kin_h(rg, kinh_u, y.rvx, y.rvy, y.rvz);
+        y.rw = kin2kin_v(S(c2w0_c * rg + c2w1_c * rg_cm), kinv_c, y.rw);
+    } else {
+        e = e + (tmp2 - tmpk);
+    }
     y.rg = rg;
     y.e = e;
 }

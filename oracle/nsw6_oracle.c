@@ -926,14 +926,14 @@ static void cnvvar_rhogkin_in(int ijdim, const double *rhog, const double *rhogv
 /* ---- precip_transport_new: src/mod_precip_transport.cpp:573-1087 -------------- */
 typedef struct region_ptrs {
     double *rhog, *rhogvx, *rhogvy, *rhogvz, *rhogw, *rhoge, *rhogq, *rho, *tem, *pre;
-    double *vx, *vy, *vz, *w, *q, *qd, *z, *precip, *precip_rhoe, *precip_lh_heat;
+    double *vx, *vy, *vz, *w, *unccn, *q, *qd, *z, *precip, *precip_rhoe, *precip_lh_heat;
     double *precip_rhophi, *precip_rhokin, *gprec, *gsgam2, *gsgam2h, *gam2, *gam2h;
     double *ix, *iy, *iz, *jx, *jy, *jz;
 } region_ptrs;
 
 static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, const double *Vterm,
                                  const int precipitating_flag[NQMAX], const double *rgs,
-                                 const double *rgsh, double dt)
+                                 const double *rgsh, double dt, int kin2kin)
 {
     const size_t K2 = (size_t)KALL * ijdim;
     double *rhogkin = zalloc(K2), *rhogkin_h = zalloc(K2), *rhogkin_v = zalloc(K2);
@@ -1131,6 +1131,7 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
         }
     }
 
+    if (!kin2kin) {
     /* PRCIP_TRN_ECORRECT = "KIN2EIN" (src/problemsize.cpp:39) */
 #pragma omp parallel for collapse(2)
     for (int k = KMIN; k <= KMAX; k++)
@@ -1147,6 +1148,40 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
     for (int k = KMIN; k <= KMAX; k++)
         for (int ij = 0; ij < ijdim; ij++)
             rhoge[IDX(k, ij)] = rhoge[IDX(k, ij)] + (tmp2[IDX(k, ij)] - tmp[IDX(k, ij)]);
+    } else {
+    /* "KIN2KIN" (src/mod_precip_transport.cpp:1012-1068): velocities rescaled to the
+     * transported kinetic energy, horizontal on cells [kmin,kmax], vertical on walls
+     * [kmin,kmax+1] (the wall density from the densities after sedimentation) */
+#pragma omp parallel for collapse(2)
+    for (int k = KMIN; k <= KMAX; k++)
+        for (int ij = 0; ij < ijdim; ij++) {
+            const size_t p = IDX(k, ij);
+            const double kin_h0 = rhogkin_h[p] / rhog[p];
+            double vx_t = R->rhogvx[p] / rhog[p];
+            double vy_t = R->rhogvy[p] / rhog[p];
+            double vz_t = R->rhogvz[p] / rhog[p];
+            const double kin_h = 0.5 * (vx_t * vx_t + vy_t * vy_t + vz_t * vz_t);
+            if (kin_h > 1.0E-20) {
+                vx_t = vx_t * sqrt(fabs(kin_h0 / kin_h));
+                vy_t = vy_t * sqrt(fabs(kin_h0 / kin_h));
+                vz_t = vz_t * sqrt(fabs(kin_h0 / kin_h));
+            }
+            R->rhogvx[p] = vx_t * rhog[p];
+            R->rhogvy[p] = vy_t * rhog[p];
+            R->rhogvz[p] = vz_t * rhog[p];
+        }
+#pragma omp parallel for collapse(2)
+    for (int k = KMIN; k <= KMAX + 1; k++)
+        for (int ij = 0; ij < ijdim; ij++) {
+            const size_t p = IDX(k, ij);
+            const double rhogh = C2Wfact[p] * rhog[p] + C2Wfact[K2 + p] * rhog[IDX(k - 1, ij)];
+            const double kin_v0 = rhogkin_v[p] / rhogh;
+            double w_t = R->rhogw[p] / rhogh;
+            const double kin_v = 0.5 * pow(w_t, 2);
+            if (kin_v > 1.0E-20) w_t = w_t * sqrt(fabs(kin_v0 / kin_v));
+            R->rhogw[p] = w_t * rhogh;
+        }
+    }
 
 #pragma omp parallel for collapse(2)
     for (int nq = 0; nq < NQMAX; nq++)
@@ -1415,8 +1450,17 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
 
     /* init() */
     negative_filter(ijdim, R->rhog, R->rhoge, R->rhogq, R->rho, R->tem, R->pre, R->q, R->gsgam2);
+    if (P->opt_indir != 0.0) {
+        /* TagAerosolT (src/mod_mp_nsw6.cpp:676-679): Kokkos::max(UNCCN*1e-6, Nc_def) */
 #pragma omp parallel for
-    for (size_t i = 0; i < K2; i++) Nc[i] = P->Nc_def;           /* TagAerosolF */
+        for (size_t i = 0; i < K2; i++) {
+            const double a = R->unccn[i] * 1.0E-6;
+            Nc[i] = (a < P->Nc_def) ? P->Nc_def : a;
+        }
+    } else {
+#pragma omp parallel for
+        for (size_t i = 0; i < K2; i++) Nc[i] = P->Nc_def;       /* TagAerosolF */
+    }
 #pragma omp parallel for
     for (size_t i = 0; i < K2; i++) {
         psatl[i] = ora_psat_liq(R->tem[i]);
@@ -1516,7 +1560,7 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
             rgsh[p] = R->gam2h[p] / R->gsgam2h[p];
         }
     int flags[NQMAX] = {0, 0, 1, 0, 1, 1};   /* 3WATER: QR, QS, QG */
-    precip_transport_new(G, ijdim, R, Vt, flags, rgs, rgsh, dt);
+    precip_transport_new(G, ijdim, R, Vt, flags, rgs, rgsh, dt, P->kin2kin != 0.0);
 
     /* src/mod_mp_nsw6.cpp:647-660: OPT_EXPLICIT_ICEGEN (sw_expice = 1) takes
      * satadjust_liq with SATURATION_Setrange(100, 90); otherwise satadjust_all with
@@ -1556,7 +1600,10 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
     if (!rep) rep = &local;
     memset(rep, 0, sizeof *rep);
     rep->first_l = rep->first_k = rep->first_ij = -1;
-    if (lall <= 0 || ijdim <= 0 || div_num < 1) { rep->status = NSW6_BAD_ARG; return NSW6_BAD_ARG; }
+    if (lall <= 0 || ijdim <= 0 || div_num < 1 || (P->opt_indir != 0.0 && !f->unccn)) {
+        rep->status = NSW6_BAD_ARG;
+        return NSW6_BAD_ARG;
+    }
 
     const double fraction_mp = 1.0 / (double)div_num;
     const double dt_mp = dt * fraction_mp;
@@ -1570,6 +1617,7 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
         R.rhogvz = f->rhogvz + o2; R.rhogw = f->rhogw + o2; R.rhoge = f->rhoge + o2;
         R.rhogq = f->rhogq + o3; R.rho = f->rho + o2; R.tem = f->tem + o2; R.pre = f->pre + o2;
         R.vx = f->vx + o2; R.vy = f->vy + o2; R.vz = f->vz + o2; R.w = f->w + o2; R.q = f->q + o3;
+        R.unccn = f->unccn ? f->unccn + o2 : NULL;
         R.qd = f->qd + o2; R.z = f->z + o2; R.precip = f->precip + (size_t)l * 2 * ijdim;
         R.precip_rhoe = f->precip_rhoe + o1; R.precip_lh_heat = f->precip_lh_heat + o1;
         R.precip_rhophi = f->precip_rhophi + o1; R.precip_rhokin = f->precip_rhokin + o1;

@@ -1,0 +1,118 @@
+"""OPT_INDIR, the aerosol indirect effect (src/mod_mp_nsw6.cpp:131, 475-484, 676-679):
+the droplet number is Nc = max(unccn * 1e-6, Nc_def) per point instead of Nc_def.
+It enters KK2000 (Nc^-1.79), Berry (Nc, Dc(Nc)) and the effective radius.
+
+* CPU: with unccn * 1e-6 below Nc_def everywhere the step is the default step, bit
+  for bit; with a uniform unccn above it, the step equals the default step whose
+  Nc_def is that value (the init-time effective-radius constants kept); a missing
+  unccn is an argument error.
+* GPU: a point-wise varying unccn matches the oracle within 1e-10 of max-abs, under
+  KK2000 and under Berry; the reverse mode reports the scheme as unsupported.
+"""
+import numpy as np
+import pytest
+
+from nicam_kokkos_microphysics_amd import _native as N
+from nicam_kokkos_microphysics_amd import fields as F
+from nicam_kokkos_microphysics_amd.check import compare_states
+
+DT = 15.0
+COMPARE = [k for k in F.OUTPUTS if k not in ("rhogq", "q")] + \
+          [f"rhogq:{t}" for t in F.TRACERS] + [f"q:{t}" for t in F.TRACERS]
+
+
+def _state(lall=2, ijdim=64, seed=21):
+    return F.synth_state(lall, ijdim, seed=seed, frac_supersat=0.1, frac_towers=0.3)
+
+
+def _run(P, st, gz, gzh):
+    from oracle import oracle as O
+    s = F.copy_state(st)
+    rep = O.mp_driver(s, gz, gzh, DT, P=P)
+    return s, rep.status
+
+
+def test_small_unccn_is_the_default_step():
+    from oracle import oracle as O
+    gz, gzh, st = _state()
+    st["unccn"][...] = 10.0e6          # 10 cm-3 < Nc_def = 50 cm-3
+    a, ra = _run(O.params(), st, gz, gzh)
+    P = O.params()
+    P.opt_indir = 1.0
+    b, rb = _run(P, st, gz, gzh)
+    assert ra == rb == 0
+    for k in F.OUTPUTS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("kk", [1.0, 0.0])
+def test_uniform_unccn_is_nc_def_of_that_value(kk):
+    from oracle import oracle as O
+    gz, gzh, st = _state()
+    st["unccn"][...] = 180.0e6
+    P = O.params()
+    P.sw_kk2000 = kk
+    P.opt_indir = 1.0
+    a, ra = _run(P, st, gz, gzh)
+    Q = O.params()
+    Q.sw_kk2000 = kk
+    Q.Nc_def = 180.0
+    b, rb = _run(Q, st, gz, gzh)
+    assert ra == rb == 0
+    for k in F.OUTPUTS:
+        assert np.array_equal(a[k], b[k]), k
+    # and the droplet number changes the step
+    d, _ = _run(_with(O.params(), sw_kk2000=kk), st, gz, gzh)
+    assert not np.array_equal(a["re_liquid"], d["re_liquid"])
+
+
+def _with(P, **kw):
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+def test_missing_unccn_is_an_argument_error():
+    from oracle import oracle as O
+    gz, gzh, st = _state(lall=1, ijdim=8)
+    st["unccn"] = None
+    P = O.params()
+    P.opt_indir = 1.0
+    assert O.mp_driver(st, gz, gzh, DT, P=P).status == N.NSW6_BAD_ARG
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kk", [1.0, 0.0])
+def test_gpu_aerosol_indirect_matches_oracle(kk):
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state(lall=2, ijdim=900)
+    rng = np.random.default_rng(4)
+    st["unccn"][...] = rng.uniform(0.0, 400.0e6, st["unccn"].shape)
+    P_ora = _with(O.params(), opt_indir=1.0, sw_kk2000=kk)
+    P_gpu = _with(default_params(), opt_indir=1.0, sw_kk2000=kk)
+    ref = F.copy_state(st)
+    dev = F.to_device(st)
+    drv = NSW6(gz, gzh, params=P_gpu, max_columns=2 * 900)
+    for _ in range(2):
+        assert O.mp_driver(ref, gz, gzh, DT, P=P_ora).status == 0
+        drv.step(dev, DT)
+    torch.cuda.synchronize()
+    out = F.to_host(dev)
+    bad = compare_states(out, ref, COMPARE, 1e-10)
+    assert not bad, bad
+    # the droplet number matters: the default scheme gives other effective radii
+    one = F.to_device(st)
+    drv.step(one, DT)
+    base = F.to_device(st)
+    drv.set_params(_with(default_params(), sw_kk2000=kk))
+    drv.step(base, DT)
+    assert not torch.equal(base["re_liquid"], one["re_liquid"])
+    drv.set_params(P_gpu)
+    # the reverse mode covers the default scheme only
+    from nicam_kokkos_microphysics_amd.driver import NSW6Error
+    with pytest.raises(NSW6Error) as e:
+        drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
+    assert e.value.code == N.NSW6_UNSUPPORTED
+    drv.close()

@@ -98,3 +98,15 @@ def test_mixed_phase_scheme_without_explicit_ice_generation():
 def test_mixed_phase_scheme_supersaturated():
     out, ref, _ = _run_both(2, 700, 2, seed=5, params_edit={"sw_expice": 0.0}, frac_supersat=0.3)
     _assert_parity(out, ref)
+
+
+def test_berry_autoconversion_scheme():
+    # sw_kk2000 = 0: Berry autoconversion and the original rain accretion
+    out, ref, _ = _run_both(2, 900, 2, seed=13, params_edit={"sw_kk2000": 0.0}, frac_towers=0.3)
+    _assert_parity(out, ref)
+
+
+def test_non_roh_snow_moments():
+    # sw_roh2014 = 0: the generic snow moments of the exponential size distribution
+    out, ref, _ = _run_both(2, 900, 2, seed=17, params_edit={"sw_roh2014": 0.0}, frac_towers=0.3)
+    _assert_parity(out, ref)

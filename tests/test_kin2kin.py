@@ -1,0 +1,115 @@
+"""PRCIP_TRN_ECORRECT = "KIN2KIN" (src/mod_precip_transport.cpp:1012-1068), the
+alternative to the default KIN2EIN: after sedimentation the velocities are rescaled
+to the transported kinetic energy instead of moving the kinetic-energy change into
+internal energy.
+
+* CPU (oracle, against the KIN2EIN step on the same input, whose momentum is the
+  momentum right after transport):
+  - the horizontal velocity keeps its direction: rhogv(KIN2KIN) = f * rhogv(KIN2EIN)
+    with one factor f > 0 per cell for x, y and z, and f differs from 1 where rain
+    falls;
+  - in columns without sedimenting hydrometeors the transported kinetic energy is the
+    kinetic energy of the momentum, so f = 1 and rhogw is kept on the interior walls
+    (to round-off), while walls kmin and kmax+1, where rhogkin_v is 0, get rhogw = 0
+    wherever w is not negligible;
+  - the internal energy is not corrected: rhoge differs from the KIN2EIN step.
+* GPU: parity with the oracle within 1e-10 of max-abs, single-cell and general
+  regions; the reverse mode reports the scheme as unsupported.
+"""
+import numpy as np
+import pytest
+
+from nicam_kokkos_microphysics_amd import _native as N
+from nicam_kokkos_microphysics_amd import fields as F
+from nicam_kokkos_microphysics_amd.check import compare_states
+
+DT = 15.0
+KMIN, KMAX = 1, 94
+COMPARE = [k for k in F.OUTPUTS if k not in ("rhogq", "q")] + \
+          [f"rhogq:{t}" for t in F.TRACERS] + [f"q:{t}" for t in F.TRACERS]
+
+
+def _state(lall=2, ijdim=64, seed=31):
+    gz, gzh, st = F.synth_state(lall, ijdim, seed=seed, frac_towers=0.3)
+    # the bottom and top walls carry vertical momentum, so their KIN2KIN reset shows
+    st["rhogw"][:, KMIN, :] = 0.05 * st["rhog"][:, KMIN, :]
+    st["rhogw"][:, KMAX + 1, :] = -0.02 * st["rhog"][:, KMAX + 1, :]
+    # every fourth column without sedimenting hydrometeors (QR, QS, QG)
+    for t in (2, 4, 5):
+        st["rhogq"][:, t, :, ::4] = 0.0
+        st["q"][:, t, :, ::4] = 0.0
+    return gz, gzh, st
+
+
+def _pair():
+    from oracle import oracle as O
+    gz, gzh, st = _state()
+    e, k = F.copy_state(st), F.copy_state(st)
+    assert O.mp_driver(e, gz, gzh, DT).status == 0
+    P = O.params()
+    P.kin2kin = 1.0
+    assert O.mp_driver(k, gz, gzh, DT, P=P).status == 0
+    return st, e, k
+
+
+def test_horizontal_velocity_keeps_its_direction():
+    st, e, k = _pair()
+    ex, ey, ez = (e[n][:, KMIN:KMAX + 1] for n in ("rhogvx", "rhogvy", "rhogvz"))
+    kx, ky, kz = (k[n][:, KMIN:KMAX + 1] for n in ("rhogvx", "rhogvy", "rhogvz"))
+    fx, fy, fz = kx / ex, ky / ey, kz / ez
+    assert np.all(fx > 0)
+    np.testing.assert_allclose(fy, fx, rtol=1e-12)
+    np.testing.assert_allclose(fz, fx, rtol=1e-12)
+    wet = np.ones(st["rhog"].shape[-1], bool)
+    wet[::4] = False
+    assert np.abs(fx[..., wet] - 1).max() > 1e-8          # rescaled where rain falls
+
+
+def test_dry_columns_keep_momentum_and_reset_the_boundary_walls():
+    st, e, k = _pair()
+    for n in ("rhogvx", "rhogvy", "rhogvz"):
+        np.testing.assert_allclose(k[n][:, KMIN:KMAX + 1, ::4], e[n][:, KMIN:KMAX + 1, ::4], rtol=1e-13)
+    np.testing.assert_allclose(k["rhogw"][:, KMIN + 1:KMAX + 1, ::4], e["rhogw"][:, KMIN + 1:KMAX + 1, ::4],
+                               rtol=1e-12, atol=1e-300)
+    # walls kmin and kmax+1: rhogkin_v = 0 there, so w is rescaled to 0
+    assert np.all(k["rhogw"][:, KMIN] == 0.0) and np.all(k["rhogw"][:, KMAX + 1] == 0.0)
+    assert np.all(e["rhogw"][:, KMIN] != 0.0) and np.all(e["rhogw"][:, KMAX + 1] != 0.0)
+
+
+def test_internal_energy_is_not_corrected():
+    st, e, k = _pair()
+    assert not np.array_equal(k["rhoge"], e["rhoge"])
+    np.testing.assert_array_equal(k["rhog"], e["rhog"])     # mass untouched
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vscale", [1.0, 25.0])
+def test_gpu_kin2kin_matches_oracle(vscale):
+    """vscale = 25 makes rain fall beyond one cell: the general sweep runs (one step:
+    the second step of this state is beyond the Newton iteration limit in both
+    energy corrections)."""
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6, NSW6Error, default_params
+    gz, gzh, st = _state(lall=2, ijdim=900)
+    P_ora, P_gpu = O.params(), default_params()
+    for P in (P_ora, P_gpu):
+        P.kin2kin = 1.0
+        P.Cr = P.Cr * vscale
+    ref = F.copy_state(st)
+    dev = F.to_device(st)
+    drv = NSW6(gz, gzh, params=P_gpu, max_columns=2 * 900)
+    reps = []
+    for _ in range(1 if vscale > 1 else 2):
+        assert O.mp_driver(ref, gz, gzh, DT, P=P_ora).status == 0
+        drv.step(dev, DT)
+        reps.append(drv.report())
+    torch.cuda.synchronize()
+    bad = compare_states(F.to_host(dev), ref, COMPARE, 1e-10)
+    assert not bad, bad
+    if vscale > 1:
+        assert reps[0].n_general_regions > 0
+    with pytest.raises(NSW6Error) as e:
+        drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
+    assert e.value.code == N.NSW6_UNSUPPORTED
+    drv.close()

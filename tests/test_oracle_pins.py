@@ -289,6 +289,27 @@ def test_kk2000_autoconversion_closed_form():
     assert abs(o.drhogq[1] + o.drhogq[2]) <= 1e-20
 
 
+def test_berry_autoconversion_closed_form():
+    # sw_kk2000 = 0: Berry (1968) autoconversion in the Berry & Reinhardt (1974) form
+    # P = (1/rho) 1.67e-5 (rho qc)^2 / (5 + 3.66e-2 Nc / (Dc rho qc)), rho qc in g m-3,
+    # Dc = 0.146 - 5.964e-2 ln(Nc/2000); no rain, so the original Pracw is 0
+    P = O.params()
+    P.sw_kk2000 = 0.0
+    rho, T, dt = 0.9, 290.0, 15.0
+    qsl, _ = _qsat(T, rho)
+    for qc in (2e-4, 5e-4, 1.5e-3):
+        o = _point(P, T, rho, [qsl, qc, 0, 0, 0, 0], dt)
+        rhoqc = rho * qc * 1000.0
+        Dc = 0.146 - 5.964e-2 * np.log(50.0 / 2000.0)
+        praut = min(1.67e-5 * rhoqc ** 2 / rho / (5.0 + 3.66e-2 * 50.0 / (Dc * rhoqc)), qc / dt)
+        assert abs(o.drhogq[2] / (rho * praut * dt) - 1) < 1e-12
+        assert abs(o.drhogq[1] + o.drhogq[2]) <= 1e-20
+        # and it differs from KK2000 at the same point
+        P.sw_kk2000 = 1.0
+        assert _point(P, T, rho, [qsl, qc, 0, 0, 0, 0], dt).drhogq[2] != o.drhogq[2]
+        P.sw_kk2000 = 0.0
+
+
 def test_rates_conserve_water_and_keep_species_nonnegative():
     P = O.params()
     rng = np.random.default_rng(0)
