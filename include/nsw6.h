@@ -103,6 +103,11 @@ typedef struct nsw6_params {
      * energy), 1 = "KIN2KIN" (src/mod_precip_transport.cpp:1012-1068: the velocities
      * are rescaled to keep the transported kinetic energy; internal energy untouched). */
     double kin2kin;
+    /* precip_transport_type (src/mod_mp_nsw6.cpp:127, 593-600): 0 = "3WATER" (the
+     * reference default: QR, QS, QG sediment), 1 = "4WATER" (QI sediments too, with
+     * Vt(QI) = Vti of src/mod_mp_nsw6.cpp:1073-1075; sw_constVti = 1 with CONST_Vti > 0
+     * forces the constant fall speed -CONST_Vti, as mp_nsw6_init sets it, :399-407). */
+    double four_water;
 } nsw6_params;
 
 /* The 8 tunable NSW6 coefficients, in this order, for gradients and perturbations:

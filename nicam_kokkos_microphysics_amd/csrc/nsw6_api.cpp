@@ -90,8 +90,8 @@ static int ensure_regions(nsw6_ctx *c, int lall)
     if (w.region_flag) (void)hipFree(w.region_flag);
     w.part = nullptr; w.bounds = nullptr; w.region_flag = nullptr;
     const size_t blocks = (size_t)(c->max_columns / EXT_COLS_PER_BLOCK) + (size_t)cap;
-    if (hipMalloc(&w.part, sizeof(double) * (size_t)KALL * 12 * blocks) != hipSuccess ||
-        hipMalloc(&w.bounds, sizeof(int) * BOUNDS_PER_REGION * (size_t)cap) != hipSuccess ||
+    if (hipMalloc(&w.part, sizeof(double) * (size_t)KALL * 16 * blocks) != hipSuccess ||
+        hipMalloc(&w.bounds, sizeof(int) * EXT_PER_REGION_MAX * (size_t)cap) != hipSuccess ||
         hipMalloc(&w.region_flag, sizeof(int) * (size_t)cap) != hipSuccess) {
         c->region_capacity = 0;
         return NSW6_HIP_ERROR;
@@ -131,6 +131,7 @@ static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, 
     a.cloudtop_intended = ctx->P.cloudtop_intended != 0.0;
     a.opt_indir = ctx->P.opt_indir != 0.0;
     a.kin2kin = ctx->P.kin2kin != 0.0;
+    a.four_water = ctx->P.four_water != 0.0;
     a.dt = dt;
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
@@ -177,8 +178,8 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
         hipMalloc(&w.rhog, sizeof(double) * K2) != hipSuccess ||
         hipMalloc(&w.rhoge, sizeof(double) * K2) != hipSuccess ||
         hipMalloc(&w.rhogq, sizeof(double) * K2 * NQ) != hipSuccess ||
-        hipMalloc(&w.vt, sizeof(double) * K2 * 3) != hipSuccess ||
-        hipMalloc(&w.flux, sizeof(double) * K2 * FLUX_SLOTS) != hipSuccess ||
+        hipMalloc(&w.vt, sizeof(double) * K2 * 4) != hipSuccess ||
+        hipMalloc(&w.flux, sizeof(double) * K2 * FLUX_SLOTS_MAX) != hipSuccess ||
         hipMalloc(&w.report, sizeof(int) * REPORT_INTS) != hipSuccess ||
         hipMemset(w.report, 0, sizeof(int) * REPORT_INTS) != hipSuccess ||
         hipMalloc(&w.cloud_top, sizeof(int) * (size_t)max_columns) != hipSuccess ||
@@ -284,7 +285,7 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
 static bool adjoint_supported(const DevParams &P)
 {
     return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0 && P.opt_indir == 0.0 &&
-           P.kin2kin == 0.0;
+           P.kin2kin == 0.0 && P.four_water == 0.0;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)

@@ -59,48 +59,52 @@ NSW6_HD Unit load_unit(const Fields &f, int64_t pc)
     return u;
 }
 
-// Cell w-1 enters the walls of cell w only through rhog, q (QR, QS, QG) and gsgam2.
-struct BelowCell {
-    double rhog, q[3], gs;
+// Cell w-1 enters the walls of cell w only through rhog, q of the sedimenting tracers
+// and gsgam2.
+template <int NT> struct BelowCellN {
+    double rhog, q[NT], gs;
 };
-NSW6_HD BelowCell load_below(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int w)
+using BelowCell = BelowCellN<3>;
+template <int NT = 3>
+NSW6_HD BelowCellN<NT> load_below(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int w)
 {
     const int64_t om = p2 + (int64_t)(w - 1) * ijdim, oqm = p3 + (int64_t)(w - 1) * ijdim;
-    BelowCell b;
+    BelowCellN<NT> b;
     b.rhog = a.ws.rhog[om];
     b.gs = a.f.gsgam2[om];
-    b.q[0] = a.ws.rhogq[oqm + QR * K2] / b.rhog;
-    b.q[1] = a.ws.rhogq[oqm + QS * K2] / b.rhog;
-    b.q[2] = a.ws.rhogq[oqm + QG * K2] / b.rhog;
+#pragma unroll
+    for (int t = 0; t < NT; t++) b.q[t] = a.ws.rhogq[oqm + sed_nq<NT>(t) * K2] / b.rhog;
     return b;
 }
 
 // Everything the flux of wall w (1..94) of column (l, ij) is computed from.
-struct WallState {
+template <int NT> struct WallStateN {
     Unit u;
-    BelowCell m;
+    BelowCellN<NT> m;
     CellIn<double> r0, r1;
-    Cell<double> cur, nxt;
-    double V[3][5];
-    double z0[3], zh0[3];
-    WallGeom geo;
+    CellN<double, NT> cur, nxt;
+    double V[NT][5];
+    double z0[NT], zh0[NT];
+    WallGeomN<NT> geo;
 };
-NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallState &s)
+using WallState = WallStateN<3>;
+template <int NT>
+NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> &s)
 {
     const int ijdim = a.ijdim;
     const GridDev &g = *a.grid;
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij, pv = (int64_t)l * 3 * K2 + ij;
+    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij, pv = (int64_t)l * NT * K2 + ij;
     s.u = load_unit(a.f, (int64_t)l * ijdim + ij);
-    s.m = load_below(a, p2, p3, K2, ijdim, w);
+    s.m = load_below<NT>(a, p2, p3, K2, ijdim, w);
     load_cell(a, p2, p3, K2, ijdim, w, s.r0);
     derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
     load_cell(a, p2, p3, K2, ijdim, w + 1, s.r1);
     derive_cell(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt);
-    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
-    int kcmax[3], khmax[3];
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
+    int kcmax[NT], khmax[NT];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
         kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
         khmax[t] = bnd[bounds_index(t, 1, w, 1)];
@@ -108,40 +112,42 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallState &s)
     wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
 }
 
-// Flux workspace of wall w (FLUX_SLOTS), frain = GPREC (src/mod_precip_transport.cpp:
+// Flux workspace of wall w (FxSlots<NT>), frain = GPREC (src/mod_precip_transport.cpp:
 // 916-927) and, at wall kmin, the surface fluxes.  Walls 0 and 95 carry no flux.
+template <int NT = 3>
 NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
 {
+    using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
     const Fields &f = a.f;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t p2 = (int64_t)l * K2 + ij;
-    double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
+    double *fx = a.ws.flux + (int64_t)l * X::N * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
     const int64_t ow = p2 + (int64_t)w * ijdim;
 
-    fx[FX_RW * K2] = f.rhogw[ow];
+    fx[X::RW * K2] = f.rhogw[ow];
     if (w == 0 || w == KALL - 1) {
 #pragma unroll
-        for (int i = 0; i < FX_RW; i++) fx[i * K2] = 0.0;
+        for (int i = 0; i < X::RW; i++) fx[i * K2] = 0.0;
         f.GPREC[ow] = (0.0 + 0.0) * 1.0;
         return;
     }
-    WallState s;
+    WallStateN<NT> s;
     wall_state(a, l, w, ij, s);
-    WallFlux<double> F;
-    double ft[3][4];
+    WallFluxN<double, NT> F;
+    double ft[NT][4];
     wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);
 
 #pragma unroll
-    for (int t = 0; t < 3; t++) fx[(FX_Q + t) * K2] = F.fq[t];
+    for (int t = 0; t < NT; t++) fx[(X::Q + t) * K2] = F.fq[t];
 #pragma unroll
-    for (int i = 0; i < 5; i++) fx[(FX_E + i) * K2] = F.fs[i];
-    fx[FX_KINV * K2] = F.fh[0];
-    fx[FX_W * K2] = F.fh[1];
+    for (int i = 0; i < 5; i++) fx[(X::E + i) * K2] = F.fs[i];
+    fx[X::KINV * K2] = F.fh[0];
+    fx[X::W * K2] = F.fh[1];
     f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
     if (w == KMIN) {
         Surface<double> sf;
-        surface_fluxes(ft, a.dt, sf);
+        surface_fluxes<NT>(ft, a.dt, sf);
         const int64_t pc = (int64_t)l * ijdim + ij;
         const int64_t pp = (int64_t)l * 2 * ijdim + ij;
         f.precip[pp] = sf.precip0;
@@ -161,59 +167,62 @@ NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
 // kmax+1: rhogkin_v is 0 at that wall, and the density of cell kmax after
 // sedimentation (its lower neighbour) is formed from the fluxes of walls kmax and
 // kmax+1 as sediment() forms it.  rhog_top is the halo cell's density.
+template <int NT>
 NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top)
 {
+    using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
     const GridDev &g = *a.grid;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t o = (int64_t)l * K2 + (int64_t)(KALL - 1) * ijdim + ij;
     const int64_t om = o - ijdim;
-    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
+    const double *fx = a.ws.flux + (int64_t)l * X::N * K2 + ij;
     const double rc = g.rdgz[KMAX];
     double rg = a.ws.rhog[om];
 #pragma unroll
-    for (int t = 0; t < 3; t++)
-        rg = rg + -(fx[(FX_Q + t) * K2 + (int64_t)(KALL - 1) * ijdim] - fx[(FX_Q + t) * K2 + (int64_t)KMAX * ijdim]) * rc;
+    for (int t = 0; t < NT; t++)
+        rg = rg + -(fx[(X::Q + t) * K2 + (int64_t)(KALL - 1) * ijdim] - fx[(X::Q + t) * K2 + (int64_t)KMAX * ijdim]) * rc;
     const double gs = a.f.gsgam2[o], gsh = a.f.gsgam2h[o], gs_m = a.f.gsgam2[om];
     const double c2w0 = g.afact[KALL - 1] / gs * gsh, c2w1 = g.bfact[KALL - 1] / gs_m * gsh;
     return kin2kin_v(c2w0 * rhog_top + c2w1 * rg, 0.0 + 0.0, a.f.rhogw[o] + 0.0);
 }
 
 // Inputs of sediment() for interior cell c of column (l, ij) from the flux workspace.
-NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedIn<double> &x)
+template <int NT>
+NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedInN<double, NT> &x)
 {
+    using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const Fields &f = a.f;
-    const double *fx = a.ws.flux + (int64_t)l * FLUX_SLOTS * K2 + ij;
+    const double *fx = a.ws.flux + (int64_t)l * X::N * K2 + ij;
     auto FXV = [&](int slot, int wall) { return fx[(int64_t)slot * K2 + (int64_t)wall * ijdim]; };
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
-        x.fq[0][t] = FXV(FX_Q + t, c - 1);
-        x.fq[1][t] = FXV(FX_Q + t, c);
-        x.fq[2][t] = FXV(FX_Q + t, c + 1);
-        x.fq[3][t] = (c + 1 <= KMAX) ? FXV(FX_Q + t, c + 2) : 0.0;   // wall 96 does not exist
+    for (int t = 0; t < NT; t++) {
+        x.fq[0][t] = FXV(X::Q + t, c - 1);
+        x.fq[1][t] = FXV(X::Q + t, c);
+        x.fq[2][t] = FXV(X::Q + t, c + 1);
+        x.fq[3][t] = (c + 1 <= KMAX) ? FXV(X::Q + t, c + 2) : 0.0;   // wall 96 does not exist
     }
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-        x.fs[0][i] = FXV(FX_E + i, c);
-        x.fs[1][i] = FXV(FX_E + i, c + 1);
+        x.fs[0][i] = FXV(X::E + i, c);
+        x.fs[1][i] = FXV(X::E + i, c + 1);
     }
 #pragma unroll
     for (int m = 0; m < 3; m++) {
-        x.fh[m][0] = FXV(FX_KINV, c - 1 + m);
-        x.fh[m][1] = FXV(FX_W, c - 1 + m);
+        x.fh[m][0] = FXV(X::KINV, c - 1 + m);
+        x.fh[m][1] = FXV(X::W, c - 1 + m);
     }
-    x.rw_c = FXV(FX_RW, c);
-    x.rw_p = FXV(FX_RW, c + 1);
+    x.rw_c = FXV(X::RW, c);
+    x.rw_p = FXV(X::RW, c + 1);
     const int64_t o = (int64_t)l * K2 + (int64_t)c * ijdim + ij;
     const int64_t oq = (int64_t)l * NQ * K2 + (int64_t)c * ijdim + ij;
     x.rg_m = a.ws.rhog[o - ijdim];
     x.rg_c = a.ws.rhog[o];
     x.rg_p = a.ws.rhog[o + ijdim];
-    x.rq[0] = a.ws.rhogq[oq + QR * K2];
-    x.rq[1] = a.ws.rhogq[oq + QS * K2];
-    x.rq[2] = a.ws.rhogq[oq + QG * K2];
+#pragma unroll
+    for (int t = 0; t < NT; t++) x.rq[t] = a.ws.rhogq[oq + sed_nq<NT>(t) * K2];
     x.re0 = a.ws.rhoge[o];
     x.rvx = f.rhogvx[o];
     x.rvy = f.rhogvy[o];

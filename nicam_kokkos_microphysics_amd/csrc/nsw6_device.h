@@ -105,6 +105,7 @@ struct DevParams {
     double cloudtop_intended;   // 1: highest cloudy level (see nsw6_params)
     double opt_indir;           // 1: Nc = max(unccn * 1e-6, Nc_def) per point
     double kin2kin;             // 1: PRCIP_TRN_ECORRECT = "KIN2KIN"
+    double four_water;          // 1: precip_transport_type = "4WATER" (QI sediments)
 };
 
 
@@ -419,8 +420,13 @@ NSW6_HD void rates(const DevParams &P, R dt, const RateIn<R> &in, RateOut<R> &o,
     const R RLMDg_3dg = RLMDg_3 * RLMDg_dg;
     const R RLMDg_5dg = (RLMDg_3 * RLMDg_2) * RLMDg_dg;
 
-    // terminal velocity (QI does not sediment under "3WATER": its fall speed is unused)
-    const R Vti = R(0);
+    // terminal velocity; QI sediments only under "4WATER" (otherwise Vti is unused)
+    R Vti = R(0);
+    if (P.four_water != 0.0) {
+        const R zi = step_neg(qi - R(1.0E-8));
+        Vti = (R(1) - R(P.sw_constVti)) * R(-3.29) * exp(log(dens * qi + zi) * R(0.16)) * (R(1) - zi) +
+              R(P.sw_constVti) * R(-P.CONST_Vti);
+    }
     const R Vtr = -Cr * rho_fact * R(P.Vtr_fac) * RLMDr_dr;
     const R Vts = -Cs * rho_fact * RMOMs_Vt;
     const R Vtg = -Cg * rho_fact * R(P.Vtg_fac) * RLMDg_dg;

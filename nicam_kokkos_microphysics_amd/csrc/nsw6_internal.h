@@ -30,26 +30,41 @@ struct GridDev {
     double rsum_gz[KALL], rsum_gzh[KALL];   // 1/(dgz(k-1)+dgz(k)), 1/(dgzh(k-1)+dgzh(k))
 };
 
-// Per-region reductions: [tracer 3][level type 2: full, half][k 96][max, min]
-constexpr int EXT_PER_REGION = 3 * 2 * KALL * 2;
+// Per-region reductions: [tracer NT][level type 2: full, half][k 96][max, min]
+// (NT = 3 sedimenting tracers under "3WATER", 4 under "4WATER")
+template <int NT> constexpr int ext_per_region() { return NT * 2 * KALL * 2; }
+constexpr int EXT_PER_REGION = ext_per_region<3>();
+constexpr int EXT_PER_REGION_MAX = ext_per_region<4>();
 // k_extremes: one block of EXT_THREADS threads reduces EXT_COLS_PER_BLOCK columns of
-// one level; per-block partial extremes are stored as [l][k][12][block], slot
+// one level; per-block partial extremes are stored as [l][k][4 NT][block], slot
 // t*4 + {0 full max, 1 full min, 2 half max, 3 half min}
 constexpr int EXT_THREADS = 256;
 constexpr int EXT_COLS_PER_THREAD = 4;
 constexpr int EXT_COLS_PER_BLOCK = EXT_THREADS * EXT_COLS_PER_THREAD;
 constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
 // Flux workspace [l][FLUX_SLOTS][k][ij].  For the single-cell regions, wall w holds:
-//   0..2  mass fluxes of QR, QS, QG
-//   3..7  tracer-summed fluxes of internal energy, potential energy, horizontal
-//         kinetic energy, and the two horizontal momentum components
-//   8..9  tracer-summed half-level fluxes of vertical kinetic energy and momentum
-//   10    the pre-transport rhogw of cell w
+//   0..NT-1      mass fluxes of the sedimenting tracers (QR, [QI,] QS, QG)
+//   NT..NT+4     tracer-summed fluxes of internal energy, potential energy, horizontal
+//                kinetic energy, and the two horizontal momentum components
+//   NT+5..NT+6   tracer-summed half-level fluxes of vertical kinetic energy and momentum
+//   NT+7         the pre-transport rhogw of cell w
 // The regions that take the general sweep use slots 0..3 of their own block for the
 // deferred momentum (rhogvx, rhogvy, rhogvz, rhogw).
-constexpr int FLUX_SLOTS = 11;
-constexpr int FX_Q = 0, FX_E = 3, FX_PHI = 4, FX_KINH = 5, FX_U = 6, FX_V = 7, FX_KINV = 8, FX_W = 9,
-              FX_RW = 10;
+template <int NT> struct FxSlots {
+    static constexpr int Q = 0, E = NT, PHI = NT + 1, KINH = NT + 2, U = NT + 3, V = NT + 4, KINV = NT + 5,
+                         W = NT + 6, RW = NT + 7, N = NT + 8;
+};
+constexpr int FLUX_SLOTS = FxSlots<3>::N;
+constexpr int FLUX_SLOTS_MAX = FxSlots<4>::N;
+constexpr int FX_Q = FxSlots<3>::Q, FX_E = FxSlots<3>::E, FX_PHI = FxSlots<3>::PHI, FX_KINH = FxSlots<3>::KINH,
+              FX_U = FxSlots<3>::U, FX_V = FxSlots<3>::V, FX_KINV = FxSlots<3>::KINV, FX_W = FxSlots<3>::W,
+              FX_RW = FxSlots<3>::RW;
+// Sedimenting tracer t of NT in nq order (the order of the reference's tracer loop,
+// src/mod_precip_transport.cpp:789): 3WATER QR, QS, QG; 4WATER QR, QI, QS, QG.
+template <int NT> __host__ __device__ constexpr int sed_nq(int t)
+{
+    return NT == 4 ? QR + t : (t == 0 ? QR : (t == 1 ? QS : QG));
+}
 __host__ __device__ constexpr int ext_index(int t, int type, int k, int mm)
 {
     return ((t * 2 + type) * KALL + k) * 2 + mm;
@@ -91,7 +106,7 @@ __device__ __forceinline__ LevelBlock level_block(int b, int nbx, int nk, int k0
 // Context-owned device workspace (allocated once for the maximum column count).
 struct Workspace {
     double *rhog, *rhoge, *rhogq;   // state after the rate update (pre-sedimentation)
-    double *vt;                     // fall speeds of QR, QS, QG  [l][3][k][ij]
+    double *vt;                     // fall speeds of the NT sedimenting tracers [l][NT][k][ij]
     double *flux;                   // [l][FLUX_SLOTS][k][ij] (see FLUX_SLOTS)
     double *part;                   // [l][KALL][12][blocks per region] partial extremes
     int *bounds;                    // [l][BOUNDS_PER_REGION]
@@ -111,6 +126,7 @@ struct StepArgs {
     int cloudtop_intended;        // 1: cloud top = highest cloudy level (k_cloudtop)
     int opt_indir;                // 1: aerosol indirect effect (k_rates<R, DIAG, true>)
     int kin2kin;                  // 1: KIN2KIN energy correction of the sedimentation
+    int four_water;               // 1: "4WATER", QI sediments (transport kernels with NT = 4)
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 

@@ -63,7 +63,9 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;   // + n*K2
-    const int64_t pv = (int64_t)l * 3 * K2 + (int64_t)k * ijdim + ij;    // + t*K2
+    // fall speeds [l][NT][k][ij] of QR, QS, QG ("3WATER") or QR, QI, QS, QG ("4WATER")
+    const int nt = a.four_water ? 4 : 3;
+    const int64_t pv = (int64_t)l * nt * K2 + (int64_t)k * ijdim + ij;   // + t*K2
     const Fields &f = a.f;
 
     if (k == 0 || k == KALL - 1) {
@@ -71,8 +73,7 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
         a.ws.rhoge[p] = f.rhoge[p];
 #pragma unroll
         for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
-#pragma unroll
-        for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = 0.0;
+        for (int t = 0; t < nt; t++) a.ws.vt[pv + t * K2] = 0.0;
         if (DIAG) {
             f.re_liquid[p] = 0.0;
             f.re_cld[p] = 0.0;
@@ -98,8 +99,15 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     a.ws.rhoge[p] = y.rhoge;
 #pragma unroll
     for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = y.rhogq[n];
+    if (a.four_water) {
+        a.ws.vt[pv] = y.vt[0];
+        a.ws.vt[pv + K2] = y.vti;
+        a.ws.vt[pv + 2 * K2] = y.vt[1];
+        a.ws.vt[pv + 3 * K2] = y.vt[2];
+    } else {
 #pragma unroll
-    for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
+        for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
+    }
     if (!DIAG) return;
     double rceff, rceff_cld, rct, tct, csw;
     effective_radius(P, y.sw, y.rho, x.tem, y.q, nc.Nc, rceff, rceff_cld, rct, tct, csw);
@@ -216,8 +224,9 @@ __device__ __forceinline__ double wave_min(double v)
 // distances (src/mod_vadv1d.cpp:317-327).  One block per (column chunk, level k in
 // 1..94, region); each thread takes EXT_COLS_PER_THREAD columns strided by the block
 // size.  The block's 12 extremes (3 tracers x {full, half} x {max, min}) go to the
-// partials array [l][k][12][block] that k_bounds reduces.
+// partials array [l][k][4 NT][block] that k_bounds reduces (NT = 3 or 4 tracers).
 // ---------------------------------------------------------------------------------
+template <int NT>
 __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -228,11 +237,11 @@ __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
     const GridDev &g = *a.grid;
     const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const double *vt = a.ws.vt + (int64_t)l * 3 * K2;
+    const double *vt = a.ws.vt + (int64_t)l * NT * K2;
 
-    double e[12];
+    double e[4 * NT];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         e[t * 4 + 0] = -__builtin_inf();
         e[t * 4 + 1] = __builtin_inf();
         e[t * 4 + 2] = -__builtin_inf();
@@ -242,7 +251,7 @@ __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
         const int ij = lb.bx * EXT_COLS_PER_BLOCK + i * EXT_THREADS + threadIdx.x;
         if (ij >= ijdim) break;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             double V[5];
             load_vt_window(vt + t * K2 + ij, ijdim, k, V);
             const double zf = zdis_full_at(g, dt, dt2, dt3, V, k);
@@ -259,30 +268,31 @@ __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
             }
         }
     }
-    __shared__ double red[EXT_THREADS / 64][12];
+    __shared__ double red[EXT_THREADS / 64][4 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 12; j++) {
+    for (int j = 0; j < 4 * NT; j++) {
         const double v = (j & 1) ? wave_min<64>(e[j]) : wave_max<64>(e[j]);
         if (lane == 0) red[wv][j] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 12) {
+    if (threadIdx.x < 4 * NT) {
         const int j = threadIdx.x;
         double v = red[0][j];
 #pragma unroll
         for (int w = 1; w < EXT_THREADS / 64; w++) v = (j & 1) ? dmin(v, red[w][j]) : dmax(v, red[w][j]);
-        a.ws.part[(((int64_t)l * KALL + k) * 12 + j) * nb + lb.bx] = v;
+        a.ws.part[(((int64_t)l * KALL + k) * (4 * NT) + j) * nb + lb.bx] = v;
     }
 }
 
 // ---------------------------------------------------------------------------------
 // k_bounds: kcell_min/kcell_max of vadv1d_prep (src/mod_vadv1d.cpp:317-352)
 // ---------------------------------------------------------------------------------
+template <int NT>
 __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
 {
     const int l = blockIdx.y;
-    const int t = blockIdx.x / 2, type = blockIdx.x % 2;   // grid.x = 6
+    const int t = blockIdx.x / 2, type = blockIdx.x % 2;   // grid.x = 2 NT
     const int k = threadIdx.x;                              // 0..95 used
     if (k >= KALL) return;
     const GridDev &g = *a.grid;
@@ -292,7 +302,7 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
     int general = 0;
     if (k >= mkmin && k <= mkmax) {
         const int nw = (a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK;
-        const double *pmax = a.ws.part + (((int64_t)l * KALL + k) * 12 + (t * 2 + type) * 2 + 0) * nw;
+        const double *pmax = a.ws.part + (((int64_t)l * KALL + k) * (4 * NT) + (t * 2 + type) * 2 + 0) * nw;
         const double *pmin = pmax + nw;
         double amax = -__builtin_inf(), amin = __builtin_inf();
         for (int i = 0; i < nw; i++) {
@@ -323,7 +333,7 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
         if (amax > 0.0 && zh[k] - zzmax < zh[k]) general = 1;
         if (zh[k] - zzmin >= zh[ktop]) general = 1;
     }
-    int *b = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    int *b = a.ws.bounds + (size_t)l * ext_per_region<NT>();
     b[bounds_index(t, type, k, 0)] = kcmin;
     b[bounds_index(t, type, k, 1)] = kcmax;
     if (general) atomicOr(&a.ws.region_flag[l], 1);
@@ -332,21 +342,46 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
 // ---------------------------------------------------------------------------------
 // k_sweep
 // ---------------------------------------------------------------------------------
-// Compact transport data of one cell j (what the fluxes need).
-struct TCell {
-    double rhoq[3];     // rhogq'(t) * rgs
-    double qrgs[3];     // q'(t) * rgs
+// Compact transport data of one cell j (what the fluxes need), NT sedimenting tracers.
+template <int NT> struct TCell {
+    double rhoq[NT];     // rhogq'(t) * rgs
+    double qrgs[NT];     // q'(t) * rgs
     double tem, z, kinh, up, vp;
-    double hv[3][2];    // half-level values at wall j: qh*rhogkin_v, qh*rhogw
+    double hv[NT][2];    // half-level values at wall j: qh*rhogkin_v, qh*rhogw
 };
 // Values of cell j kept until the cell is finalised.
-struct FCell {
+template <int NT> struct FCell {
     double rhog, rhogw, kinv, c2w0, c2w1;   // pre-transport; kinv/c2w* belong to wall j
-    double drhoq[3], drhoe, drhophi, drhokinh, drhogu, drhogv;  // full-level tendencies
-    double drhokinv, drhogw;                                   // half-level tendencies (wall j)
+    double drhoq[NT], drhoe, drhophi, drhokinh, drhogu, drhogv;  // full-level tendencies
+    double drhokinv, drhogw;                                    // half-level tendencies (wall j)
 };
+// Sedimenting tracer index of tracer nq (-1 if it does not sediment).
+template <int NT> __device__ __forceinline__ int sed_index(int nq)
+{
+    if (NT == 4) return (nq >= QR) ? nq - QR : -1;
+    return (nq == QR) ? 0 : (nq == QS) ? 1 : (nq == QG) ? 2 : -1;
+}
+// The tracer update of src/mod_precip_transport.cpp:970-989 for one cell: nq = 1..5
+// (QC..QG) in order, the sedimenting tracers adding their tendency to rhogq, rhog and
+// (as -d g z) to rhoge, the others adding 0; then QV.
+template <int NT>
+__device__ __forceinline__ void apply_sedimented(const FCell<NT> &fc, double z, double rq[6], double &rg, double &e)
+{
+#pragma unroll
+    for (int n = 1; n <= 5; n++) {
+        const int t = sed_index<NT>(n);
+        if (t >= 0) {
+            rq[n] = rq[n] + fc.drhoq[t];
+            rg = rg + fc.drhoq[t];
+            e = e - fc.drhoq[t] * C_GRAV * z;
+        } else {
+            rq[n] = rq[n] + 0.0;
+        }
+    }
+    rq[QV] = rq[QV] + 0.0;
+}
 
-__device__ __forceinline__ double rhof_of(const TCell &c, int t, int i)
+template <int NT> __device__ __forceinline__ double rhof_of(const TCell<NT> &c, int t, int i)
 {
     switch (i) {
     case 0: return c.rhoq[t];
@@ -371,7 +406,8 @@ struct SweepCtx {
     double ix, iy, iz, jx, jy, jz;
 };
 
-__device__ __forceinline__ void load_tcell(const SweepCtx &s, int j, TCell &c, double &rhog_j, double q_j[3],
+template <int NT>
+__device__ __forceinline__ void load_tcell(const SweepCtx &s, int j, TCell<NT> &c, double &rhog_j, double *q_j,
                                            double &gsgam2_j)
 {
     const StepArgs &a = *s.a;
@@ -400,23 +436,21 @@ __device__ __forceinline__ void load_tcell(const SweepCtx &s, int j, TCell &c, d
     c.up = vx * s.ix + vy * s.iy + vz * s.iz;
     c.vp = vx * s.jx + vy * s.jy + vz * s.jz;
     c.z = a.f.z[o];
-    c.rhoq[0] = rq[QR] * rgs;
-    c.rhoq[1] = rq[QS] * rgs;
-    c.rhoq[2] = rq[QG] * rgs;
-    c.qrgs[0] = qq[QR] * rgs;
-    c.qrgs[1] = qq[QS] * rgs;
-    c.qrgs[2] = qq[QG] * rgs;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        c.rhoq[t] = rq[sed_nq<NT>(t)] * rgs;
+        c.qrgs[t] = qq[sed_nq<NT>(t)] * rgs;
+        q_j[t] = qq[sed_nq<NT>(t)];
+    }
     rhog_j = rg;
-    q_j[0] = qq[QR];
-    q_j[1] = qq[QS];
-    q_j[2] = qq[QG];
     gsgam2_j = gs;
 }
 
 // Half-level values at wall j and the pre-transport wall quantities of FCell j.
+template <int NT>
 __device__ __forceinline__ void load_wall(const SweepCtx &s, const GridDev &g, int j, double rhog_j,
-                                          double rhog_jm1, const double q_j[3], const double q_jm1[3],
-                                          double gsgam2_j, double gsgam2_jm1, TCell &c, FCell &fc)
+                                          double rhog_jm1, const double *q_j, const double *q_jm1,
+                                          double gsgam2_j, double gsgam2_jm1, TCell<NT> &c, FCell<NT> &fc)
 {
     const StepArgs &a = *s.a;
     const int64_t o = s.p2 + (int64_t)j * s.ijdim;
@@ -435,7 +469,7 @@ __device__ __forceinline__ void load_wall(const SweepCtx &s, const GridDev &g, i
         fc.kinv = 0.5 * (rw * rw) / (fc.c2w0 * rhog_j + fc.c2w1 * rhog_jm1);
         const double rgsh = a.f.gam2h[o] / gsh;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             const double qh = 0.5 * (q_j[t] + q_jm1[t]) * rgsh;
             c.hv[t][0] = qh * fc.kinv;
             c.hv[t][1] = qh * rw;
@@ -443,31 +477,33 @@ __device__ __forceinline__ void load_wall(const SweepCtx &s, const GridDev &g, i
     } else {
         fc.kinv = 0.0;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             c.hv[t][0] = 0.0;
             c.hv[t][1] = 0.0;
         }
     }
 #pragma unroll
-    for (int t = 0; t < 3; t++) fc.drhoq[t] = 0.0;
+    for (int t = 0; t < NT; t++) fc.drhoq[t] = 0.0;
     fc.drhoe = fc.drhophi = fc.drhokinh = fc.drhogu = fc.drhogv = 0.0;
     fc.drhokinv = fc.drhogw = 0.0;
 }
 
 // Pre-transport data of an arbitrary cell/wall, from memory (general sweep only).
+template <int NT>
 __device__ double general_rhof(const SweepCtx &s, const GridDev &g, int j, int t, int i)
 {
-    TCell c;
-    double rg, qj[3], gs;
+    TCell<NT> c;
+    double rg, qj[NT], gs;
     load_tcell(s, j, c, rg, qj, gs);
     return rhof_of(c, t, i);
 }
+template <int NT>
 __device__ double general_hv(const SweepCtx &s, const GridDev &g, int j, int t, int i)
 {
     if (j < KMIN + 1 || j > KMAX) return 0.0;
-    TCell c, cm;
-    FCell fc;
-    double rg, qj[3], gs, rgm, qjm[3], gsm;
+    TCell<NT> c, cm;
+    FCell<NT> fc;
+    double rg, qj[NT], gs, rgm, qjm[NT], gsm;
     load_tcell(s, j, c, rg, qj, gs);
     load_tcell(s, j - 1, cm, rgm, qjm, gsm);
     load_wall(s, g, j, rg, rgm, qj, qjm, gs, gsm, c, fc);
@@ -476,15 +512,16 @@ __device__ double general_hv(const SweepCtx &s, const GridDev &g, int j, int t, 
 
 // Post-sedimentation writes of one interior cell of a general region: rhog, rhoge and
 // the sedimenting tracers (the rest of the state is finished by k_finish).
+template <int NT>
 __device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, double rg, const double rq[6], double e)
 {
     const Fields &f = s.a->f;
     const int64_t o = s.p2 + (int64_t)c * s.ijdim;
     f.rhog[o] = rg;
     f.rhoge[o] = e;
-    f.rhogq[s.p3 + (int64_t)(QR * KALL + c) * s.ijdim] = rq[QR];
-    f.rhogq[s.p3 + (int64_t)(QS * KALL + c) * s.ijdim] = rq[QS];
-    f.rhogq[s.p3 + (int64_t)(QG * KALL + c) * s.ijdim] = rq[QG];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+        f.rhogq[s.p3 + (int64_t)(sed_nq<NT>(t) * KALL + c) * s.ijdim] = rq[sed_nq<NT>(t)];
 }
 
 // ---------------------------------------------------------------------------------
@@ -494,6 +531,7 @@ __device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, doubl
 // k_finish (one thread per cell).  The arithmetic is in nsw6_step.h, the column
 // gathers and the per-wall flux (flux_wall) in nsw6_columns.h.
 // ---------------------------------------------------------------------------------
+template <int NT>
 __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -502,7 +540,7 @@ __global__ void __launch_bounds__(256) k_flux(StepArgs a)
     if (a.ws.region_flag[lb.l] != 0) return;
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
-    flux_wall(a, lb.l, lb.k, ij);
+    flux_wall<NT>(a, lb.l, lb.k, ij);
 }
 
 // ---------------------------------------------------------------------------------
@@ -510,6 +548,7 @@ __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 // single-cell regions and from the general sweep elsewhere; then the point-wise tail
 // (rediagnosis, saturation adjustment, final negative filter).
 // ---------------------------------------------------------------------------------
+template <int NT>
 __global__ void __launch_bounds__(256) k_finish(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -525,20 +564,19 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
     const bool interior = (k >= KMIN && k <= KMAX);
     const bool general = a.ws.region_flag[l] != 0;
 
-    // State after sedimentation.  QV, QC, QI and the halo levels only receive zero
-    // tendencies.
+    // State after sedimentation.  QV, QC, QI ("3WATER") and the halo levels only
+    // receive zero tendencies.
     TailIn<double> x;
     if (interior) {
         if (general) {
             x.rhog = f.rhog[p];
             x.rhoge = f.rhoge[p];
-            x.rhogq[QR] = f.rhogq[pq + QR * K2];
-            x.rhogq[QS] = f.rhogq[pq + QS * K2];
-            x.rhogq[QG] = f.rhogq[pq + QG * K2];
+#pragma unroll
+            for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = f.rhogq[pq + sed_nq<NT>(t) * K2];
         } else {
-            SedIn<double> si;
+            SedInN<double, NT> si;
             load_sed(a, l, ij, k, si);
-            SedOut<double> so;
+            SedOutN<double, NT> so;
             sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so, a.kin2kin != 0);
             f.rhogvx[p] = so.rvx;
             f.rhogvy[p] = so.rvy;
@@ -546,27 +584,26 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             f.rhogw[p] = so.rw;
             x.rhog = so.rg;
             x.rhoge = so.e;
-            x.rhogq[QR] = so.rq[0];
-            x.rhogq[QS] = so.rq[1];
-            x.rhogq[QG] = so.rq[2];
+#pragma unroll
+            for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = so.rq[t];
         }
     } else {
         x.rhog = a.ws.rhog[p] + 0.0;
         x.rhoge = a.ws.rhoge[p] + 0.0;
-        x.rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
-        x.rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
-        x.rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = a.ws.rhogq[pq + sed_nq<NT>(t) * K2] + 0.0;
         if (!general) {
             const int64_t pc = (int64_t)l * ijdim + ij;
             f.rhogvx[p] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
             f.rhogvy[p] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
             f.rhogvz[p] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
-            f.rhogw[p] = (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall(a, l, ij, x.rhog) : f.rhogw[p] + 0.0;
+            f.rhogw[p] =
+                (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog) : f.rhogw[p] + 0.0;
         }
     }
     x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
     x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
-    x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+    if (NT == 3) x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
     x.gsgam2 = f.gsgam2[p];
 
     TailOut<double> y;
@@ -596,7 +633,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
     f.GDCFRC[p] = 0.0 * 1.0;
 }
 
-template <bool GENERAL>
+template <bool GENERAL, int NT>
 __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -606,7 +643,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     if (ij >= ijdim) return;
     const GridDev &g = *a.grid;
     const Fields &f = a.f;
-    const int *bnd = a.ws.bounds + (size_t)l * BOUNDS_PER_REGION;
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
     const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
 
     SweepCtx s;
@@ -617,31 +654,31 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     s.K2 = (int64_t)KALL * ijdim;
     s.p2 = (int64_t)l * s.K2 + ij;
     s.p3 = (int64_t)l * NQ * s.K2 + ij;
-    s.pv = (int64_t)l * 3 * s.K2 + ij;
+    s.pv = (int64_t)l * NT * s.K2 + ij;
     const int64_t pc = (int64_t)l * ijdim + ij;
     s.ix = f.ix[pc]; s.iy = f.iy[pc]; s.iz = f.iz[pc];
     s.jx = f.jx[pc]; s.jy = f.jy[pc]; s.jz = f.jz[pc];
     const double *vt = a.ws.vt + s.pv;
     // deferred momentum: slots 0..3 of the region's flux block (k_flux skips general regions)
-    double *mom = GENERAL ? (a.ws.flux + (int64_t)l * FLUX_SLOTS * s.K2 + ij) : nullptr;
+    double *mom = GENERAL ? (a.ws.flux + (int64_t)l * FxSlots<NT>::N * s.K2 + ij) : nullptr;
 
     // window: cells w (cur) and w+1 (nxt); finalisation records for w-2 .. w+1
-    TCell cur, nxt;
-    FCell fm2, fm1, f0, fp1;
-    double rhog_cur, q_cur[3], gs_cur;
-    double rhog_nxt, q_nxt[3], gs_nxt;
-    WindWindow wind[3];
-    double fprev[3][6], fprevh[3][2];
+    TCell<NT> cur, nxt;
+    FCell<NT> fm2, fm1, f0, fp1;
+    double rhog_cur, q_cur[NT], gs_cur;
+    double rhog_nxt, q_nxt[NT], gs_nxt;
+    WindWindow wind[NT];
+    double fprev[NT][6], fprevh[NT][2];
     double rhog_new_prev = 0.0;       // rhog after sedimentation of the last finalised cell
     double rg_kmax = 0.0;             // rhog of cell kmax after sedimentation
     double precip0 = 0.0, precip1 = 0.0, lh = 0.0, prhoe = 0.0, prhophi = 0.0, prhokin = 0.0;
 
     // cell 0 (halo) and cell 1
     {
-        TCell c0;
-        double rg0, q0[3], gs0;
+        TCell<NT> c0;
+        double rg0, q0[NT], gs0;
         load_tcell(s, 0, c0, rg0, q0, gs0);
-        FCell fdummy;
+        FCell<NT> fdummy;
         load_wall(s, g, 0, rg0, rg0, q0, q0, gs0, gs0, c0, fdummy);
         fm1 = fdummy;     // cell 0 record (w-1 for w = 1)
         load_tcell(s, 1, cur, rhog_cur, q_cur, gs_cur);
@@ -649,7 +686,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         rhog_new_prev = rg0;          // halo cell 0 keeps its density
     }
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         wind_init(wind[t], vt[t * s.K2 + 1 * (int64_t)ijdim]);
 #pragma unroll
         for (int i = 0; i < 6; i++) fprev[t][i] = 0.0;    // f(0) = 0
@@ -657,13 +694,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
     }
 
     for (int w = KMIN; w <= KMAX + 1; w++) {
-        double fcur[3][6], fcurh[3][2];
+        double fcur[NT][6], fcurh[NT][2];
         if (w <= KMAX) {
             // ---- load cell w+1
             load_tcell(s, w + 1, nxt, rhog_nxt, q_nxt, gs_nxt);
             load_wall(s, g, w + 1, rhog_nxt, rhog_cur, q_nxt, q_cur, gs_nxt, gs_cur, nxt, fp1);
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
+            for (int t = 0; t < NT; t++) {
                 const double vt_kp1 = (w + 1 <= KMAX) ? vt[t * s.K2 + (int64_t)(w + 1) * ijdim] : 0.0;
                 double wh_kp1, vh_kp1, whh_kp1;
                 wind_next(wind[t], w, vt_kp1, wh_kp1, vh_kp1, whh_kp1);
@@ -683,7 +720,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                                 double rf;
                                 if (k2 == w) rf = rhof_of(cur, t, i);
                                 else if (k2 == w + 1) rf = rhof_of(nxt, t, i);
-                                else rf = GENERAL ? general_rhof(s, g, k2, t, i) : 0.0;
+                                else rf = GENERAL ? general_rhof<NT>(s, g, k2, t, i) : 0.0;
                                 acc = acc + rf * fa;
                                 zd = z0 - fa;
                             }
@@ -691,7 +728,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                         double rfc;
                         if (kc == w) rfc = rhof_of(cur, t, i);
                         else if (kc == w + 1) rfc = rhof_of(nxt, t, i);
-                        else rfc = GENERAL ? general_rhof(s, g, kc, t, i) : 0.0;
+                        else rfc = GENERAL ? general_rhof<NT>(s, g, kc, t, i) : 0.0;
                         acc = acc + rfc * zd;
                         fcur[t][i] = eps_filter(acc);
                     }
@@ -713,7 +750,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                                 double hvv;
                                 if (k2 == w) hvv = cur.hv[t][i];
                                 else if (k2 == w + 1) hvv = nxt.hv[t][i];
-                                else hvv = GENERAL ? general_hv(s, g, k2, t, i) : 0.0;
+                                else hvv = GENERAL ? general_hv<NT>(s, g, k2, t, i) : 0.0;
                                 acc = acc + hvv * fa;
                                 zd = z0 - fa;
                             }
@@ -721,7 +758,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                         double hvc;
                         if (kc == w) hvc = cur.hv[t][i];
                         else if (kc == w + 1) hvc = nxt.hv[t][i];
-                        else hvc = GENERAL ? general_hv(s, g, kc, t, i) : 0.0;
+                        else hvc = GENERAL ? general_hv<NT>(s, g, kc, t, i) : 0.0;
                         acc = acc + hvc * zd;
                         fcurh[t][i] = eps_filter(acc);
                     }
@@ -734,12 +771,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             f.GPREC[s.p2 + (int64_t)w * ijdim] = (0.0 + (-fcur[0][0] / dt)) * 1.0;
             if (w == KMIN) {
                 precip0 = precip0 - fcur[0][0] / dt;
-                precip1 = precip1 - fcur[1][0] / dt;
-                lh = lh + fcur[1][0] * C_LHF0 / dt;
-                precip1 = precip1 - fcur[2][0] / dt;
-                lh = lh + fcur[2][0] * C_LHF0 / dt;
 #pragma unroll
-                for (int t = 0; t < 3; t++) {
+                for (int t = 1; t < NT; t++) {   // QI ("4WATER"), QS, QG are solid
+                    precip1 = precip1 - fcur[t][0] / dt;
+                    lh = lh + fcur[t][0] * C_LHF0 / dt;
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
                     prhoe = prhoe - fcur[t][1] / dt;
                     prhophi = prhophi - fcur[t][2] / dt;
                     prhokin = prhokin - fcur[t][3] / dt - 0.0 / dt;   // f_h(kmin) = 0
@@ -747,7 +785,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
+            for (int t = 0; t < NT; t++) {
 #pragma unroll
                 for (int i = 0; i < 6; i++) fcur[t][i] = 0.0;
                 fcurh[t][0] = fcurh[t][1] = 0.0;
@@ -758,9 +796,9 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         if (w - 1 >= KMIN) {
             const double r = g.rdgz[w - 1];
 #pragma unroll
-            for (int t = 0; t < 3; t++) fm1.drhoq[t] = -(fcur[t][0] - fprev[t][0]) * r;
+            for (int t = 0; t < NT; t++) fm1.drhoq[t] = -(fcur[t][0] - fprev[t][0]) * r;
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
+            for (int t = 0; t < NT; t++) {
                 fm1.drhoe = fm1.drhoe - (fcur[t][1] - fprev[t][1]) * r;
                 fm1.drhophi = fm1.drhophi - (fcur[t][2] - fprev[t][2]) * r;
                 fm1.drhokinh = fm1.drhokinh - (fcur[t][3] - fprev[t][3]) * r;
@@ -772,13 +810,13 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         if (w >= KMIN + 1 && w <= KMAX) {
             const double r = g.rdgzh[w];
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
+            for (int t = 0; t < NT; t++) {
                 f0.drhokinv = f0.drhokinv - (fcurh[t][0] - fprevh[t][0]) * r;
                 f0.drhogw = f0.drhogw - (fcurh[t][1] - fprevh[t][1]) * r;
             }
         }
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
 #pragma unroll
             for (int i = 0; i < 6; i++) fprev[t][i] = fcur[t][i];
             fprevh[t][0] = fcurh[t][0];
@@ -809,22 +847,12 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
             double rw_n = fm2.rhogw + fm2.drhogw;
             double rg = fm2.rhog;
-            // nq = 1..5: QC(0), QR, QI(0), QS, QG
-            rq[QC] = rq[QC] + 0.0;
-            rq[QR] = rq[QR] + fm2.drhoq[0];
-            rg = rg + fm2.drhoq[0];
-            e = e - fm2.drhoq[0] * C_GRAV * z;
-            rq[QI] = rq[QI] + 0.0;
-            rq[QS] = rq[QS] + fm2.drhoq[1];
-            rg = rg + fm2.drhoq[1];
-            e = e - fm2.drhoq[1] * C_GRAV * z;
-            rq[QG] = rq[QG] + fm2.drhoq[2];
-            rg = rg + fm2.drhoq[2];
-            e = e - fm2.drhoq[2] * C_GRAV * z;
-            rq[QV] = rq[QV] + 0.0;
+            apply_sedimented(fm2, z, rq, rg, e);
             // KIN2EIN (:991-1010)
             const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
-            const double rg_cp = ((fm1.rhog + fm1.drhoq[0]) + fm1.drhoq[1]) + fm1.drhoq[2];
+            double rg_cp = fm1.rhog;
+#pragma unroll
+            for (int t = 0; t < NT; t++) rg_cp = rg_cp + fm1.drhoq[t];
             const double rw_cp = fm1.rhogw + fm1.drhogw;
             const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
             const double kinv_new_c =
@@ -850,7 +878,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
                 f.rhogvz[o] = rvz_n;
                 f.rhogw[o] = rw_n;
             }
-            store_sedimented(s, c, rg, rq, e);
+            store_sedimented<NT>(s, c, rg, rq, e);
         }
 
         // ---- shift the window
@@ -861,7 +889,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         rhog_cur = rhog_nxt;
         gs_cur = gs_nxt;
 #pragma unroll
-        for (int t = 0; t < 3; t++) q_cur[t] = q_nxt[t];
+        for (int t = 0; t < NT; t++) q_cur[t] = q_nxt[t];
     }
     // After the loop (w = 95 handled): cells up to 93 finalised. Cell 94 remains,
     // with cell 95 (halo) as its upper neighbour: fm2 = cell 94, fm1 = cell 95.
@@ -886,18 +914,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
         double rvz_n = rvz + fm2.drhogu * s.iz + fm2.drhogv * s.jz;
         double rw_n = fm2.rhogw + fm2.drhogw;
         double rg = fm2.rhog;
-        rq[QC] = rq[QC] + 0.0;
-        rq[QR] = rq[QR] + fm2.drhoq[0];
-        rg = rg + fm2.drhoq[0];
-        e = e - fm2.drhoq[0] * C_GRAV * z;
-        rq[QI] = rq[QI] + 0.0;
-        rq[QS] = rq[QS] + fm2.drhoq[1];
-        rg = rg + fm2.drhoq[1];
-        e = e - fm2.drhoq[1] * C_GRAV * z;
-        rq[QG] = rq[QG] + fm2.drhoq[2];
-        rg = rg + fm2.drhoq[2];
-        e = e - fm2.drhoq[2] * C_GRAV * z;
-        rq[QV] = rq[QV] + 0.0;
+        apply_sedimented(fm2, z, rq, rg, e);
         const double tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
         const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
         const double kinv_new_c = 0.5 * (rw_n * rw_n) / (fm2.c2w0 * rg + fm2.c2w1 * rhog_new_prev);
@@ -920,7 +937,7 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
             f.rhogvz[o] = rvz_n;
             f.rhogw[o] = rw_n;
         }
-        store_sedimented(s, c, rg, rq, e);
+        store_sedimented<NT>(s, c, rg, rq, e);
     }
     // halo cells 0 and 95: zero tendencies (their scalars are finished by k_finish)
 #pragma unroll
@@ -976,14 +993,28 @@ __global__ void __launch_bounds__(64) k_sweep(StepArgs a)
 // ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
+// The transport of NT sedimenting tracers: extremes and kcell bounds, the wall fluxes
+// and the general sweep, then the point-wise finish.
+template <int NT> static void launch_transport(const StepArgs &a, hipStream_t stream, hipEvent_t *ev)
+{
+    const dim3 grid_c((a.ijdim + 63) / 64, a.lall);
+    const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
+    const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
+    hipLaunchKernelGGL(k_extremes<NT>, grid_e, dim3(EXT_THREADS), 0, stream, a);
+    hipLaunchKernelGGL(k_bounds<NT>, dim3(2 * NT, a.lall), dim3(192), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[2], stream);
+    hipLaunchKernelGGL(k_flux<NT>, grid_l, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((k_sweep<true, NT>), grid_c, dim3(64), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL(k_finish<NT>, grid_l, dim3(256), 0, stream, a);
+    if (ev) (void)hipEventRecord(ev[4], stream);
+}
+
 extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev /* NSW6_STEP_EVENTS events or NULL */)
 {
     const StepArgs &a = *args;
     const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
-    const dim3 grid_c((a.ijdim + 63) / 64, a.lall);
-    const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
-    const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (precision == 1 && a.opt_indir)
         hipLaunchKernelGGL((k_rates<float, true, true>), grid_p, dim3(256), 0, stream, a);
@@ -996,14 +1027,10 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     if (a.cloudtop_intended)
         hipLaunchKernelGGL(k_cloudtop, dim3((a.ijdim + 255) / 256, a.lall), dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
-    hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
-    hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
-    if (ev) (void)hipEventRecord(ev[2], stream);
-    hipLaunchKernelGGL(k_flux, grid_l, dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_sweep<true>, grid_c, dim3(64), 0, stream, a);
-    if (ev) (void)hipEventRecord(ev[3], stream);
-    hipLaunchKernelGGL(k_finish, grid_l, dim3(256), 0, stream, a);
-    if (ev) (void)hipEventRecord(ev[4], stream);
+    if (a.four_water)
+        launch_transport<4>(a, stream, ev);
+    else
+        launch_transport<3>(a, stream, ev);
     return hipGetLastError();
 }
 
@@ -1078,8 +1105,8 @@ extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStrea
     const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
     hipLaunchKernelGGL((k_rates<double, false>), grid_p, dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_extremes, grid_e, dim3(EXT_THREADS), 0, stream, a);
-    hipLaunchKernelGGL(k_bounds, dim3(6, a.lall), dim3(192), 0, stream, a);
-    hipLaunchKernelGGL(k_flux, grid_l, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_extremes<3>, grid_e, dim3(EXT_THREADS), 0, stream, a);
+    hipLaunchKernelGGL(k_bounds<3>, dim3(6, a.lall), dim3(192), 0, stream, a);
+    hipLaunchKernelGGL(k_flux<3>, grid_l, dim3(256), 0, stream, a);
     return hipGetLastError();
 }

@@ -61,6 +61,7 @@ inline void to_dev_params(const nsw6_params &p, DevParams &d)
     d.cloudtop_intended = (p.cloudtop_intended != 0.0) ? 1.0 : 0.0;
     d.opt_indir = (p.opt_indir != 0.0) ? 1.0 : 0.0;
     d.kin2kin = (p.kin2kin != 0.0) ? 1.0 : 0.0;
+    d.four_water = (p.four_water != 0.0) ? 1.0 : 0.0;
 }
 
 // GRD_Setup (src/mod_debug.cpp:92-187) plus GRD_gz_shift of precip_transport_new

@@ -30,6 +30,7 @@ template <typename S> struct PointIn {
 };
 template <typename S> struct PointOut {
     S rhog, rhoge, rhogq[6], vt[3];   // pre-sedimentation state and fall speeds of QR, QS, QG
+    S vti;                            // fall speed of QI ("4WATER" only, else 0)
     S rho, q[6];                      // density and q after the filter (q(QV) stale)
     RateSwitches sw;
 };
@@ -126,6 +127,7 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
     y.vt[0] = S(o.Vtr);
     y.vt[1] = S(o.Vts);
     y.vt[2] = S(o.Vtg);
+    y.vti = S(o.Vti);
     y.rho = rho;
 #pragma unroll
     for (int n = 0; n < NQ; n++) y.q[n] = q[n];
@@ -227,21 +229,24 @@ template <typename S> struct CellIn {
     S rvx, rvy, rvz, rw, vx, vy, vz;
     double gs, gam2, gsh, gam2h, z;
 };
-template <typename S> struct Cell {
-    S rhoq[3], qrgs[3], q[3];   // sedimenting tracers QR, QS, QG
+// Transported data of one cell for NT sedimenting tracers t (nq = sed_nq<NT>(t)).
+template <typename S, int NT> struct CellN {
+    S rhoq[NT], qrgs[NT], q[NT];
     S tem, kinh, up, vp;
-    S hv[3][2];                 // half-level values at the cell's lower wall
+    S hv[NT][2];                // half-level values at the cell's lower wall
     S rhog, kinv;
     double z, gs;
 };
+template <typename S> using Cell = CellN<S, 3>;   // "3WATER": QR, QS, QG
 struct Unit {   // horizontal unit vectors of a column
     double ix, iy, iz, jx, jy, jz;
 };
 
-// Cell j from its raw data and the part of cell j-1 its lower wall needs.
-template <typename S>
-NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rhog, const S below_q[3],
-                         double below_gs, const Unit &u, Cell<S> &c)
+// Cell j from its raw data and the part of cell j-1 its lower wall needs
+// (below_q: q of the NT sedimenting tracers of cell j-1).
+template <typename S, int NT>
+NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rhog, const S *below_q,
+                         double below_gs, const Unit &u, CellN<S, NT> &c)
 {
     const bool interior = (j >= KMIN && j <= KMAX);
     const S rg = r.rhog;
@@ -262,14 +267,13 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
     c.up = r.vx * u.ix + r.vy * u.iy + r.vz * u.iz;
     c.vp = r.vx * u.jx + r.vy * u.jy + r.vz * u.jz;
     c.z = r.z;
-    c.q[0] = qq[1];
-    c.q[1] = qq[3];
-    c.q[2] = qq[4];
-    c.rhoq[0] = r.rq[1] * rgs;
-    c.rhoq[1] = r.rq[3] * rgs;
-    c.rhoq[2] = r.rq[4] * rgs;
 #pragma unroll
-    for (int t = 0; t < 3; t++) c.qrgs[t] = c.q[t] * rgs;
+    for (int t = 0; t < NT; t++) {
+        c.q[t] = qq[sed_nq<NT>(t) - 1];                 // rq holds QC..QG
+        c.rhoq[t] = r.rq[sed_nq<NT>(t) - 1] * rgs;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) c.qrgs[t] = c.q[t] * rgs;
     c.rhog = rg;
     c.gs = r.gs;
     double c2w0 = 0.0, c2w1 = 0.0;
@@ -281,7 +285,7 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
         c.kinv = 0.5 * (r.rw * r.rw) / (c2w0 * rg + c2w1 * below_rhog);
         const double rgsh = r.gam2h / r.gsh;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             const S qh = 0.5 * (c.q[t] + below_q[t]) * rgsh;
             c.hv[t][0] = qh * c.kinv;
             c.hv[t][1] = qh * r.rw;
@@ -289,11 +293,11 @@ NSW6_HD void derive_cell(const GridDev &g, int j, const CellIn<S> &r, S below_rh
     } else {
         c.kinv = S(0.0);
 #pragma unroll
-        for (int t = 0; t < 3; t++) c.hv[t][0] = c.hv[t][1] = S(0.0);
+        for (int t = 0; t < NT; t++) c.hv[t][0] = c.hv[t][1] = S(0.0);
     }
 }
 
-template <typename S> NSW6_HD S rhof_cell(const Cell<S> &c, int t, int i)
+template <typename S, int NT> NSW6_HD S rhof_cell(const CellN<S, NT> &c, int t, int i)
 {
     switch (i) {
     case 0: return c.rhoq[t];
@@ -311,24 +315,26 @@ template <typename S> NSW6_HD S rhof_cell(const Cell<S> &c, int t, int i)
 //     kc = w   : f = rhof(w) * zdis0
 //     kc = w+1 : f = -rhof(w) * dz(w) + rhof(w+1) * zdis0   (zdis0 kept, not refreshed)
 // with the half level on the full level's kcell_max (src/mod_precip_transport.cpp:
-// 890-892).  Non-mass fluxes are summed over the three tracers.  sflux[t] is the
-// tracer's surface flux -f(kmin)/dt and feeds precip (:900-958).
+// 890-892).  Non-mass fluxes are summed over the tracers in nq order.  sflux[t] is
+// the tracer's surface flux -f(kmin)/dt and feeds precip (:900-958).
 // ---------------------------------------------------------------------------------
-template <typename S> struct WallFlux {
-    S fq[3], fs[5], fh[2];
+template <typename S, int NT> struct WallFluxN {
+    S fq[NT], fs[5], fh[2];
 };
-struct WallGeom {          // departure decisions of one wall and column
-    bool up1[3];           // full level: the departure cell is w+1
-    bool half_next[3];     // half level: the cell taken is w+1
-    bool half_full[3];     // half level: the two-cell form -h(w) dz + h(w+1) zdis0
+template <typename S> using WallFlux = WallFluxN<S, 3>;
+template <int NT> struct WallGeomN {   // departure decisions of one wall and column
+    bool up1[NT];           // full level: the departure cell is w+1
+    bool half_next[NT];     // half level: the cell taken is w+1
+    bool half_full[NT];     // half level: the two-cell form -h(w) dz + h(w+1) zdis0
 };
+using WallGeom = WallGeomN<3>;
 
-template <typename S>
-NSW6_HD void wall_geometry(const GridDev &g, int w, double dt, double dt2, double dt3_6, const S V[3][5],
-                           const int kcmax[3], const int khmax[3], S z0[3], S zh0[3], WallGeom &geo)
+template <typename S, int NT>
+NSW6_HD void wall_geometry(const GridDev &g, int w, double dt, double dt2, double dt3_6, const S (*V)[5],
+                           const int *kcmax, const int *khmax, S *z0, S *zh0, WallGeomN<NT> &geo)
 {
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         z0[t] = zdis_full_at(g, dt, dt2, dt3_6, V[t], w);
         const int kc = departure_cell(g.gzh, w, w, kcmax[t], val(z0[t]), KMIN, KMAX);
         geo.up1[t] = (kcmax[t] != w) && (kc == w + 1);
@@ -344,16 +350,16 @@ NSW6_HD void wall_geometry(const GridDev &g, int w, double dt, double dt2, doubl
     }
 }
 
-template <typename S>
-NSW6_HD void wall_flux(const GridDev &g, int w, const Cell<S> &cur, const Cell<S> &nxt, const S z0[3],
-                       const S zh0[3], const WallGeom &geo, WallFlux<S> &F, S ft_surf[3][4])
+template <typename S, int NT>
+NSW6_HD void wall_flux(const GridDev &g, int w, const CellN<S, NT> &cur, const CellN<S, NT> &nxt, const S *z0,
+                       const S *zh0, const WallGeomN<NT> &geo, WallFluxN<S, NT> &F, S (*ft_surf)[4])
 {
 #pragma unroll
     for (int i = 0; i < 5; i++) F.fs[i] = S(0.0);
     F.fh[0] = F.fh[1] = S(0.0);
     const double dzw = g.dgz[w];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         S ft[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -383,15 +389,16 @@ NSW6_HD void wall_flux(const GridDev &g, int w, const Cell<S> &cur, const Cell<S
 }
 
 // Surface fluxes of wall kmin (src/mod_precip_transport.cpp:900-958) and mp_driver's
-// scaling by fraction_mp = 1 (src/mod_mp_driver.cpp:560-612).
+// scaling by fraction_mp = 1 (src/mod_mp_driver.cpp:560-612).  Tracer 0 (QR) is
+// liquid; QI, QS and QG are solid (precip row 1 and the fusion heat).
 template <typename S> struct Surface {
     S precip0, precip1, lh, prhoe, prhophi, prhokin;
 };
-template <typename S> NSW6_HD void surface_fluxes(const S ft[3][4], double dt, Surface<S> &s)
+template <int NT = 3, typename S> NSW6_HD void surface_fluxes(const S (*ft)[4], double dt, Surface<S> &s)
 {
     S precip0 = S(0.0), precip1 = S(0.0), lh = S(0.0), prhoe = S(0.0), prhophi = S(0.0), prhokin = S(0.0);
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         if (t == 0) {
             precip0 = precip0 - ft[t][0] / dt;
         } else {
@@ -414,18 +421,20 @@ template <typename S> NSW6_HD void surface_fluxes(const S ft[3][4], double dt, S
 // Sedimentation apply and KIN2EIN of interior cell c (src/mod_precip_transport.cpp:
 // 960-1011) from the fluxes of walls c-1 .. c+2.
 // ---------------------------------------------------------------------------------
-template <typename S> struct SedIn {
+template <typename S, int NT> struct SedInN {
     // fluxes: fq of walls c-1..c+2, fs of walls c..c+1, fh of walls c-1..c+1
-    S fq[4][3], fs[2][5], fh[3][2];
+    S fq[4][NT], fs[2][5], fh[3][2];
     S rw_c, rw_p;                     // pre-transport rhogw of cells c, c+1
     S rg_m, rg_c, rg_p;               // pre-transport rhog of cells c-1, c, c+1
-    S rq[3], re0;                     // QR, QS, QG and rhoge of cell c
+    S rq[NT], re0;                    // the sedimenting tracers and rhoge of cell c
     S rvx, rvy, rvz;                  // momentum of cell c
     double gs_m, gs_c, gs_p, gsh_c, gsh_p, z;
 };
-template <typename S> struct SedOut {
-    S rg, e, rq[3], rvx, rvy, rvz, rw;
+template <typename S, int NT> struct SedOutN {
+    S rg, e, rq[NT], rvx, rvy, rvz, rw;
 };
+template <typename S> using SedIn = SedInN<S, 3>;
+template <typename S> using SedOut = SedOutN<S, 3>;
 
 // KIN2KIN (src/mod_precip_transport.cpp:1012-1068), the alternative to KIN2EIN: the
 // velocities after sedimentation are rescaled so that each point keeps the kinetic
@@ -458,31 +467,33 @@ template <typename S> NSW6_HD S kin2kin_v(S rhogh, S kinv, S rw)
     return w * rhogh;
 }
 
-template <typename S>
-NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x, SedOut<S> &y,
+template <typename S, int NT>
+NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedInN<S, NT> &x, SedOutN<S, NT> &y,
                       bool kin2kin = false)
 {
     // tendencies of cell c (walls c and c+1; wall 95 carries zero fluxes)
     const double rc = g.rdgz[c];
-    S dq[3];
+    S dq[NT];
 #pragma unroll
-    for (int t = 0; t < 3; t++) dq[t] = -(x.fq[2][t] - x.fq[1][t]) * rc;
+    for (int t = 0; t < NT; t++) dq[t] = -(x.fq[2][t] - x.fq[1][t]) * rc;
     const S de = -(x.fs[1][0] - x.fs[0][0]) * rc;
     const S dphi = -(x.fs[1][1] - x.fs[0][1]) * rc;
     const S dkinh = -(x.fs[1][2] - x.fs[0][2]) * rc;
     const S du = -(x.fs[1][3] - x.fs[0][3]) * rc;
     const S dv = -(x.fs[1][4] - x.fs[0][4]) * rc;
     // mass tendencies of cells c+1 and c-1
-    S dqp[3] = {S(0.0), S(0.0), S(0.0)}, dqm[3] = {S(0.0), S(0.0), S(0.0)};
+    S dqp[NT], dqm[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) dqp[t] = dqm[t] = S(0.0);
     if (c + 1 <= KMAX) {
         const double r = g.rdgz[c + 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqp[t] = -(x.fq[3][t] - x.fq[2][t]) * r;
+        for (int t = 0; t < NT; t++) dqp[t] = -(x.fq[3][t] - x.fq[2][t]) * r;
     }
     if (c - 1 >= KMIN) {
         const double r = g.rdgz[c - 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqm[t] = -(x.fq[1][t] - x.fq[0][t]) * r;
+        for (int t = 0; t < NT; t++) dqm[t] = -(x.fq[1][t] - x.fq[0][t]) * r;
     }
     // half-level tendencies at walls c and c+1 (walls 2..94; the flux of wall 1 is 0)
     S hk_c = S(0.0), hw_c = S(0.0), hk_p = S(0.0), hw_p = S(0.0);
@@ -516,15 +527,19 @@ NSW6_HD void sediment(const GridDev &g, int c, const Unit &u, const SedIn<S> &x,
     y.rw = x.rw_c + hw_c;
     S rg = x.rg_c;
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         y.rq[t] = x.rq[t] + dq[t];
         rg = rg + dq[t];
         e = e - dq[t] * C_GRAV * x.z;
     }
     // KIN2EIN: kinetic energy before minus after, with the neighbours' new densities
     const S tmp2 = kinh_u + (w2c0 * kinv_cp + w2c1 * kinv_c);
-    const S rg_cp = ((x.rg_p + dqp[0]) + dqp[1]) + dqp[2];
-    const S rg_cm = ((x.rg_m + dqm[0]) + dqm[1]) + dqm[2];
+    S rg_cp = x.rg_p, rg_cm = x.rg_m;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        rg_cp = rg_cp + dqp[t];
+        rg_cm = rg_cm + dqm[t];
+    }
     const S rw_cp = x.rw_p + hw_p;
     const S kinh_new = 0.5 * (y.rvx * y.rvx + y.rvy * y.rvy + y.rvz * y.rvz) / rg;
     const S kinv_new_c = (c >= KMIN + 1) ? S(0.5 * (y.rw * y.rw) / (c2w0_c * rg + c2w1_c * rg_cm)) : S(0.0);

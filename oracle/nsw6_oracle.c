@@ -1559,7 +1559,8 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
             rgs[p] = R->gam2[p] / R->gsgam2[p];
             rgsh[p] = R->gam2h[p] / R->gsgam2h[p];
         }
-    int flags[NQMAX] = {0, 0, 1, 0, 1, 1};   /* 3WATER: QR, QS, QG */
+    /* preciptation_flag (src/mod_mp_nsw6.cpp:593-600): QR, QS, QG; QI too under "4WATER" */
+    int flags[NQMAX] = {0, 0, 1, P->four_water != 0.0, 1, 1};
     precip_transport_new(G, ijdim, R, Vt, flags, rgs, rgsh, dt, P->kin2kin != 0.0);
 
     /* src/mod_mp_nsw6.cpp:647-660: OPT_EXPLICIT_ICEGEN (sw_expice = 1) takes
