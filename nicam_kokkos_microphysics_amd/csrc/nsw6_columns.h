@@ -112,23 +112,23 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> 
     wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
 }
 
-// Flux workspace of wall w (FxSlots<NT>), frain = GPREC (src/mod_precip_transport.cpp:
-// 916-927) and, at wall kmin, the surface fluxes.  Walls 0 and 95 carry no flux.
-template <int NT = 3>
-NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
+// Flux slots of wall w (FxSlots<NT>) through put(slot, value), frain = GPREC
+// (src/mod_precip_transport.cpp:916-927) and, at wall kmin, the surface fluxes.  Walls 0
+// and 95 carry no flux.
+template <int NT, typename Put>
+NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
 {
     using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
     const Fields &f = a.f;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t p2 = (int64_t)l * K2 + ij;
-    double *fx = a.ws.flux + (int64_t)l * X::N * K2 + (int64_t)w * ijdim + ij;   // + slot*K2
     const int64_t ow = p2 + (int64_t)w * ijdim;
 
-    fx[X::RW * K2] = f.rhogw[ow];
+    put(X::RW, f.rhogw[ow]);
     if (w == 0 || w == KALL - 1) {
 #pragma unroll
-        for (int i = 0; i < X::RW; i++) fx[i * K2] = 0.0;
+        for (int i = 0; i < X::RW; i++) put(i, 0.0);
         f.GPREC[ow] = (0.0 + 0.0) * 1.0;
         return;
     }
@@ -139,11 +139,11 @@ NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
     wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);
 
 #pragma unroll
-    for (int t = 0; t < NT; t++) fx[(X::Q + t) * K2] = F.fq[t];
+    for (int t = 0; t < NT; t++) put(X::Q + t, F.fq[t]);
 #pragma unroll
-    for (int i = 0; i < 5; i++) fx[(X::E + i) * K2] = F.fs[i];
-    fx[X::KINV * K2] = F.fh[0];
-    fx[X::W * K2] = F.fh[1];
+    for (int i = 0; i < 5; i++) put(X::E + i, F.fs[i]);
+    put(X::KINV, F.fh[0]);
+    put(X::W, F.fh[1]);
     f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
     if (w == KMIN) {
         Surface<double> sf;
@@ -163,12 +163,32 @@ NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
     }
 }
 
+// The same into the flux workspace [l][FxSlots<NT>::N][k][ij].
+template <int NT = 3>
+NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
+{
+    const int64_t K2 = (int64_t)KALL * a.ijdim;
+    double *fx = a.ws.flux + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
+    flux_wall_to<NT>(a, l, w, ij, [&](int slot, double v) { fx[slot * K2] = v; });
+}
+
+// Reads of the flux workspace: slot of wall w of column (l, ij).
+template <int NT> struct FluxFromMemory {
+    const double *fx;
+    int64_t K2;
+    int ijdim;
+    NSW6_HD FluxFromMemory(const StepArgs &a, int l, int ij)
+        : fx(a.ws.flux + (int64_t)l * FxSlots<NT>::N * (int64_t)KALL * a.ijdim + ij),
+          K2((int64_t)KALL * a.ijdim), ijdim(a.ijdim) {}
+    NSW6_HD double operator()(int slot, int w) const { return fx[(int64_t)slot * K2 + (int64_t)w * ijdim]; }
+};
+
 // KIN2KIN at wall kmax+1 of a single-cell region, for the thread of halo cell
 // kmax+1: rhogkin_v is 0 at that wall, and the density of cell kmax after
 // sedimentation (its lower neighbour) is formed from the fluxes of walls kmax and
 // kmax+1 as sediment() forms it.  rhog_top is the halo cell's density.
-template <int NT>
-NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top)
+template <int NT, typename Fetch>
+NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top, const Fetch &FXV)
 {
     using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
@@ -176,27 +196,24 @@ NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_to
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t o = (int64_t)l * K2 + (int64_t)(KALL - 1) * ijdim + ij;
     const int64_t om = o - ijdim;
-    const double *fx = a.ws.flux + (int64_t)l * X::N * K2 + ij;
     const double rc = g.rdgz[KMAX];
     double rg = a.ws.rhog[om];
 #pragma unroll
-    for (int t = 0; t < NT; t++)
-        rg = rg + -(fx[(X::Q + t) * K2 + (int64_t)(KALL - 1) * ijdim] - fx[(X::Q + t) * K2 + (int64_t)KMAX * ijdim]) * rc;
+    for (int t = 0; t < NT; t++) rg = rg + -(FXV(X::Q + t, KALL - 1) - FXV(X::Q + t, KMAX)) * rc;
     const double gs = a.f.gsgam2[o], gsh = a.f.gsgam2h[o], gs_m = a.f.gsgam2[om];
     const double c2w0 = g.afact[KALL - 1] / gs * gsh, c2w1 = g.bfact[KALL - 1] / gs_m * gsh;
     return kin2kin_v(c2w0 * rhog_top + c2w1 * rg, 0.0 + 0.0, a.f.rhogw[o] + 0.0);
 }
 
-// Inputs of sediment() for interior cell c of column (l, ij) from the flux workspace.
-template <int NT>
-NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedInN<double, NT> &x)
+// Inputs of sediment() for interior cell c of column (l, ij); FXV(slot, wall) reads the
+// wall fluxes (from the flux workspace or from a block's LDS).
+template <int NT, typename Fetch>
+NSW6_HD void load_sed_from(const StepArgs &a, int l, int ij, int c, SedInN<double, NT> &x, const Fetch &FXV)
 {
     using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const Fields &f = a.f;
-    const double *fx = a.ws.flux + (int64_t)l * X::N * K2 + ij;
-    auto FXV = [&](int slot, int wall) { return fx[(int64_t)slot * K2 + (int64_t)wall * ijdim]; };
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         x.fq[0][t] = FXV(X::Q + t, c - 1);
@@ -233,6 +250,11 @@ NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedInN<double, NT
     x.gsh_c = f.gsgam2h[o];
     x.gsh_p = f.gsgam2h[o + ijdim];
     x.z = f.z[o];
+}
+template <int NT>
+NSW6_HD void load_sed(const StepArgs &a, int l, int ij, int c, SedInN<double, NT> &x)
+{
+    load_sed_from(a, l, ij, c, x, FluxFromMemory<NT>(a, l, ij));
 }
 
 }  // namespace nsw6

@@ -548,21 +548,17 @@ __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 // single-cell regions and from the general sweep elsewhere; then the point-wise tail
 // (rediagnosis, saturation adjustment, final negative filter).
 // ---------------------------------------------------------------------------------
-template <int NT>
-__global__ void __launch_bounds__(256) k_finish(StepArgs a)
+// One point (l, k, ij) of the finish: the sedimented state (sediment() on the wall
+// fluxes FXV gives, or the general sweep's result), then the point-wise tail.
+template <int NT, typename Fetch>
+__device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, int ij, bool general, const Fetch &FXV)
 {
     const int ijdim = a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
-    if (!lb.valid) return;
-    const int l = lb.l, k = lb.k;
-    const int ij = lb.bx * 256 + threadIdx.x;
-    if (ij >= ijdim) return;
     const Fields &f = a.f;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
     const bool interior = (k >= KMIN && k <= KMAX);
-    const bool general = a.ws.region_flag[l] != 0;
 
     // State after sedimentation.  QV, QC, QI ("3WATER") and the halo levels only
     // receive zero tendencies.
@@ -575,7 +571,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = f.rhogq[pq + sed_nq<NT>(t) * K2];
         } else {
             SedInN<double, NT> si;
-            load_sed(a, l, ij, k, si);
+            load_sed_from(a, l, ij, k, si, FXV);
             SedOutN<double, NT> so;
             sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so, a.kin2kin != 0);
             f.rhogvx[p] = so.rvx;
@@ -598,7 +594,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
             f.rhogvy[p] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
             f.rhogvz[p] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
             f.rhogw[p] =
-                (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog) : f.rhogw[p] + 0.0;
+                (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog, FXV) : f.rhogw[p] + 0.0;
         }
     }
     x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
@@ -631,6 +627,17 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
     f.re_solid[p] = 20.0E-6;
     f.GDCLW[p] = 0.0 * 1.0;
     f.GDCFRC[p] = 0.0 * 1.0;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_finish(StepArgs a)
+{
+    const int ijdim = a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
+    if (!lb.valid) return;
+    const int ij = lb.bx * 256 + threadIdx.x;
+    if (ij >= ijdim) return;
+    finish_point<NT>(a, lb.l, lb.k, ij, a.ws.region_flag[lb.l] != 0, FluxFromMemory<NT>(a, lb.l, ij));
 }
 
 template <bool GENERAL, int NT>
