@@ -88,28 +88,44 @@ template <int NT> struct WallStateN {
     WallGeomN<NT> geo;
 };
 using WallState = WallStateN<3>;
-template <int NT>
+// GEO_FIRST: the departure geometry first (fall speeds and kcell bounds only), then the
+// cells, whose loads go through an opaque copy of the column index so that they are not
+// hoisted above the geometry: both sets of inputs are then never live at once (the
+// forward flux pass fits four waves per SIMD this way).  The reverse wall pass keeps the
+// cells-first order, which suits its register allocation.
+template <int NT, bool GEO_FIRST = false>
 NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> &s)
 {
     const int ijdim = a.ijdim;
     const GridDev &g = *a.grid;
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij, pv = (int64_t)l * NT * K2 + ij;
-    s.u = load_unit(a.f, (int64_t)l * ijdim + ij);
+    const int64_t pv = (int64_t)l * NT * K2 + ij;
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
+    auto geometry = [&]() {
+        int kcmax[NT], khmax[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
+            kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+            khmax[t] = bnd[bounds_index(t, 1, w, 1)];
+        }
+        wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
+    };
+    int ijc = ij;
+    if (GEO_FIRST) {
+        geometry();
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(ijc));
+#endif
+    }
+    const int64_t p2 = (int64_t)l * K2 + ijc, p3 = (int64_t)l * NQ * K2 + ijc;
+    s.u = load_unit(a.f, (int64_t)l * ijdim + ijc);
     s.m = load_below<NT>(a, p2, p3, K2, ijdim, w);
     load_cell(a, p2, p3, K2, ijdim, w, s.r0);
     derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
     load_cell(a, p2, p3, K2, ijdim, w + 1, s.r1);
     derive_cell(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt);
-    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
-    int kcmax[NT], khmax[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
-        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
-        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
-    }
-    wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
+    if (!GEO_FIRST) geometry();
 }
 
 // Flux slots of wall w (FxSlots<NT>) through put(slot, value), frain = GPREC
@@ -133,7 +149,7 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
         return;
     }
     WallStateN<NT> s;
-    wall_state(a, l, w, ij, s);
+    wall_state<NT, true>(a, l, w, ij, s);
     WallFluxN<double, NT> F;
     double ft[NT][4];
     wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);

@@ -532,7 +532,7 @@ __device__ __forceinline__ void store_sedimented(const SweepCtx &s, int c, doubl
 // gathers and the per-wall flux (flux_wall) in nsw6_columns.h.
 // ---------------------------------------------------------------------------------
 template <int NT>
-__global__ void __launch_bounds__(256) k_flux(StepArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
