@@ -39,7 +39,7 @@ constexpr int EXT_PER_REGION_MAX = ext_per_region<4>();
 // one level; per-block partial extremes are stored as [l][k][4 NT][block], slot
 // t*4 + {0 full max, 1 full min, 2 half max, 3 half min}
 constexpr int EXT_THREADS = 256;
-constexpr int EXT_COLS_PER_THREAD = 4;
+constexpr int EXT_COLS_PER_THREAD = 16;
 constexpr int EXT_COLS_PER_BLOCK = EXT_THREADS * EXT_COLS_PER_THREAD;
 constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
 // Flux workspace [l][FLUX_SLOTS][k][ij].  For the single-cell regions, wall w holds:
