@@ -158,6 +158,8 @@ constexpr int NSW6_STEP_EVENTS = 5;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev);
+// k_rates<float> of the mixed-precision form (its own unit, nsw6_rates_mixed.hip)
+extern "C" hipError_t nsw6_launch_rates_mixed(const nsw6::StepArgs *args, hipStream_t stream);
 // MP_DIV_NUM sub-cycling: sums [l][KALL][ij] of GPREC followed by [l][DIV_COLUMN_SUMS][ij]
 // of precip (2 rows), precip_rhoe, precip_lh_heat, precip_rhophi, precip_rhokin.
 constexpr int DIV_COLUMN_SUMS = 6;
