@@ -771,9 +771,28 @@ NSW6_HD PointIn<double> opaque_point(const PointIn<double> &a)
     return r;
 }
 
+// Per-thread storage that the rate reverse parks its phase-1 results in across the
+// phase-2 recompute: element j at base[j * stride] (LDS on the device, a local array
+// on the host).  Kept in registers, the NRATE + 6 values would be live through the
+// whole recompute.
+constexpr int RATE_STASH = NRATE + 6;
+struct RateStash {
+    double *base;
+    int stride;
+    NSW6_HD double &operator[](int j) const { return base[(int64_t)j * stride]; }
+};
+// Compiler barrier: values written to the stash before it are read back after it
+// rather than kept in registers.
+NSW6_HD void stash_fence()
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" ::: "memory");
+#endif
+}
+
 // Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
 NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const double qt_b[6],
-                     const double vt_b[3], RateInB &b, double qt[6])
+                     const double vt_b[3], RateInB &b, double qt[6], const RateStash &st)
 {
     double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -969,11 +988,22 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
             if (lim[rPracs]) dqdt_b[QS] += v4; else raw_b[rPracs] += v4;
         }
     }
+    // the phase-1 cotangents wait in the stash while the forward values are recomputed
+#pragma unroll
+    for (int j = 0; j < NRATE; j++) st[j] = raw_b[j];
+#pragma unroll
+    for (int n = 0; n < 6; n++) st[NRATE + n] = dqdt_b[n];
+    stash_fence();
     // ---- phase 2: the forward values again, from inputs the compiler cannot see
     // through (so it recomputes them rather than keeping phase 1 values live), then
     // the reverse of the rate formulas
     const RateIn<double> in = opaque_copy(in0);
 #include "nsw6_rates_fwd.inc"
+    stash_fence();
+#pragma unroll
+    for (int j = 0; j < NRATE; j++) raw_b[j] = st[j];
+#pragma unroll
+    for (int n = 0; n < 6; n++) dqdt_b[n] = st[NRATE + n];
     // raw rates -> their pre-limiter forms
     double Pigen0_b = raw_b[rPigen], Pidep0_b = raw_b[rPidep], Psdep0_b = raw_b[rPsdep], Pgdep0_b = raw_b[rPgdep];
     double Praut0_b = raw_b[rPraut], Pihom0_b = raw_b[rPihom], Pihtr0_b = raw_b[rPihtr];
@@ -1420,7 +1450,7 @@ struct PointOutB {
 };
 
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
-                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
+                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st)
 {
     // ---- forward preprocessing, reverse of the state update, the rate kernel
     double drq_b[NQ], rq_b[NQ], rhoge0_b, rhog_b;
@@ -1448,7 +1478,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         for (int n = 1; n < NQ; n++) qt_b[n] -= qt_b[QV];
         // the rate kernel
         double qt[NQ];
-        rates_b(P, dt, in, qt_b, yb.vt, ib, qt);
+        rates_b(P, dt, in, qt_b, yb.vt, ib, qt, st);
         qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
 #pragma unroll
         for (int n = 0; n < NQ; n++) rhog_b += drq_b[n] * qt[n] * dt;
@@ -1527,6 +1557,15 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
             xb.rhogq[n] += rq_b[n] * m + a_b * (1.0 - m);
         }
     }
+}
+
+
+// The same with a stash of local (register or stack) storage.
+NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
+                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
+{
+    double buf[RATE_STASH];
+    point_rates_b(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1});
 }
 
 }  // namespace nsw6

@@ -45,16 +45,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     adj_wall(L.A, lb.l, lb.k, ij);
 }
 
+// LDS holds two things here (the pass runs one block per CU, so it has the room):
+// * the parameter block: the pass is too large for the compiler to prove that no store
+//   of the launch overwrites it in global memory, so its uniform reads would otherwise
+//   take the vector memory path (about 150 of them per point), each a round trip that
+//   one wave per SIMD cannot hide;
+// * the rate reverse's stash (RATE_STASH doubles per thread, column threadIdx.x), which
+//   takes those values out of the registers during the phase-2 recompute.
 __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
 {
     __shared__ double red[4][NCOEF];
-    const int ijdim = L.A.a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    __shared__ double stash[RATE_STASH][256];
+    __shared__ DevParams sP;
+    {
+        const int *src = reinterpret_cast<const int *>(L.A.a.P);
+        int *dst = reinterpret_cast<int *>(&sP);
+        for (int i = threadIdx.x; i < (int)(sizeof(DevParams) / sizeof(int)); i += 256) dst[i] = src[i];
+        __syncthreads();
+    }
+    AdjArgs A = L.A;
+    A.a.P = &sP;
+    const int ijdim = A.a.ijdim;
+    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     double tun_b[NCOEF];
 #pragma unroll
     for (int c = 0; c < NCOEF; c++) tun_b[c] = 0.0;
-    if (lb.valid && ij < ijdim) adj_point(L.A, lb.l, lb.k, ij, tun_b);
+    if (lb.valid && ij < ijdim) adj_point(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
     // per-block sums of the coefficient cotangents (every thread reaches this point)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll

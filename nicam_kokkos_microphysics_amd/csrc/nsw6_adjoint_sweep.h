@@ -285,7 +285,8 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
     }
 }
 
-NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
+// adj_point of a point of a single-cell region; st holds the rate reverse's stash.
+NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
     const int ijdim = a.ijdim;
@@ -293,12 +294,6 @@ NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOE
     const int64_t p2 = (int64_t)l * K2 + ij;
     const int64_t p = p2 + (int64_t)k * ijdim;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
-    if (a.ws.region_flag[l] != 0) {
-        // regions that need the general sweep get no gradient (the caller is told)
-        const double z[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
-        return;
-    }
     const Fields &f = a.f;
     auto CRV = [&](int slot, int c) {
         return (c >= KMIN && c <= KMAX) ? A.cr[slot * N2 + p2 + (int64_t)c * ijdim] : 0.0;
@@ -361,7 +356,7 @@ NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOE
         xb.rhog = xb.tem = xb.qv = 0.0;
 #pragma unroll
         for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
-        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b);
+        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b, st);
         g_rhog = xb.rhog;
         g_rhoge = 0.0;   // the interior rhoge is recomputed from tem
 #pragma unroll
@@ -376,6 +371,29 @@ NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOE
         for (int n = 0; n < NQ; n++) g_rhogq[n] = rq_b[n];
     }
     store_point_grad(A.grad, p, pq, K2, g_rhog, g_rhoge, g_rhogq, g_tem, g_qv, gvx, gvy, gvz, gw, gux, guy, guz);
+}
+
+// Regions that need the general sweep get no gradient (the caller is told).  The zero
+// gradients are stored on a path of their own, after the single-cell body rather than
+// ahead of it: a store ahead of the body's loads would send its uniform reads (the
+// parameters, the grid) through the vector memory path instead of the scalar cache.
+NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
+{
+    const StepArgs &a = A.a;
+    if (a.ws.region_flag[l] == 0) {
+        adj_point_single(A, l, k, ij, tun_b, st);
+        return;
+    }
+    const int64_t K2 = (int64_t)KALL * a.ijdim;
+    const int64_t p = (int64_t)l * K2 + (int64_t)k * a.ijdim + ij;
+    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * a.ijdim + ij;
+    const double z[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+}
+NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
+{
+    double buf[RATE_STASH];
+    adj_point(A, l, k, ij, tun_b, RateStash{buf, 1});
 }
 
 }  // namespace nsw6

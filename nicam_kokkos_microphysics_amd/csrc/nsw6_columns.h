@@ -141,13 +141,16 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
     const int64_t p2 = (int64_t)l * K2 + ij;
     const int64_t ow = p2 + (int64_t)w * ijdim;
 
-    put(X::RW, f.rhogw[ow]);
     if (w == 0 || w == KALL - 1) {
 #pragma unroll
         for (int i = 0; i < X::RW; i++) put(i, 0.0);
+        put(X::RW, f.rhogw[ow]);
         f.GPREC[ow] = (0.0 + 0.0) * 1.0;
         return;
     }
+    // Every store comes after every load: a store before them would make the compiler
+    // fetch the grid and kcell tables (uniform) through the vector memory path, one
+    // round trip per step of the departure-cell search, instead of the scalar cache.
     WallStateN<NT> s;
     wall_state<NT, true>(a, l, w, ij, s);
     WallFluxN<double, NT> F;
@@ -160,6 +163,7 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
     for (int i = 0; i < 5; i++) put(X::E + i, F.fs[i]);
     put(X::KINV, F.fh[0]);
     put(X::W, F.fh[1]);
+    put(X::RW, s.r0.rw);   // the pre-transport rhogw of cell w
     f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
     if (w == KMIN) {
         Surface<double> sf;

@@ -466,8 +466,11 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
     const bool interior = (k >= KMIN && k <= KMAX);
 
     // State after sedimentation.  QV, QC, QI ("3WATER") and the halo levels only
-    // receive zero tendencies.
+    // receive zero tendencies.  The momenta of the single-cell regions are stored with
+    // the other outputs at the end: every store comes after every load, so that the
+    // uniform reads of the grid and the parameters stay on the scalar cache.
     TailIn<double> x;
+    double mom[4];   // rhogvx, rhogvy, rhogvz, rhogw after sedimentation (single-cell regions)
     if (interior) {
         if (general) {
             x.rhog = f.rhog[p];
@@ -479,10 +482,10 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
             load_sed_from(a, l, ij, k, si, FXV);
             SedOutN<double, NT> so;
             sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so, a.kin2kin != 0);
-            f.rhogvx[p] = so.rvx;
-            f.rhogvy[p] = so.rvy;
-            f.rhogvz[p] = so.rvz;
-            f.rhogw[p] = so.rw;
+            mom[0] = so.rvx;
+            mom[1] = so.rvy;
+            mom[2] = so.rvz;
+            mom[3] = so.rw;
             x.rhog = so.rg;
             x.rhoge = so.e;
 #pragma unroll
@@ -495,11 +498,10 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
         for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = a.ws.rhogq[pq + sed_nq<NT>(t) * K2] + 0.0;
         if (!general) {
             const int64_t pc = (int64_t)l * ijdim + ij;
-            f.rhogvx[p] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
-            f.rhogvy[p] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
-            f.rhogvz[p] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
-            f.rhogw[p] =
-                (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog, FXV) : f.rhogw[p] + 0.0;
+            mom[0] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
+            mom[1] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
+            mom[2] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
+            mom[3] = (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog, FXV) : f.rhogw[p] + 0.0;
         }
     }
     x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
@@ -532,6 +534,12 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
     f.re_solid[p] = 20.0E-6;
     f.GDCLW[p] = 0.0 * 1.0;
     f.GDCFRC[p] = 0.0 * 1.0;
+    if (!general) {
+        f.rhogvx[p] = mom[0];
+        f.rhogvy[p] = mom[1];
+        f.rhogvz[p] = mom[2];
+        f.rhogw[p] = mom[3];
+    }
 }
 
 template <int NT>
