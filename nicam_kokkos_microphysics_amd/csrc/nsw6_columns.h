@@ -88,12 +88,9 @@ template <int NT> struct WallStateN {
     WallGeomN<NT> geo;
 };
 using WallState = WallStateN<3>;
-// GEO_FIRST: the departure geometry first (fall speeds and kcell bounds only), then the
-// cells, whose loads go through an opaque copy of the column index so that they are not
-// hoisted above the geometry: both sets of inputs are then never live at once (the
-// forward flux pass fits four waves per SIMD this way).  The reverse wall pass keeps the
-// cells-first order, which suits its register allocation.
-template <int NT, bool GEO_FIRST = false>
+// The cells first, then the fall speeds, the kcell bounds and the departure geometry:
+// every load of the wall is requested before any arithmetic waits on one.
+template <int NT>
 NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> &s)
 {
     const int ijdim = a.ijdim;
@@ -111,21 +108,14 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> 
         }
         wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
     };
-    int ijc = ij;
-    if (GEO_FIRST) {
-        geometry();
-#if defined(__HIP_DEVICE_COMPILE__)
-        asm volatile("" : "+v"(ijc));
-#endif
-    }
-    const int64_t p2 = (int64_t)l * K2 + ijc, p3 = (int64_t)l * NQ * K2 + ijc;
-    s.u = load_unit(a.f, (int64_t)l * ijdim + ijc);
+    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij;
+    s.u = load_unit(a.f, (int64_t)l * ijdim + ij);
     s.m = load_below<NT>(a, p2, p3, K2, ijdim, w);
     load_cell(a, p2, p3, K2, ijdim, w, s.r0);
     derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
     load_cell(a, p2, p3, K2, ijdim, w + 1, s.r1);
     derive_cell(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt);
-    if (!GEO_FIRST) geometry();
+    geometry();
 }
 
 // Flux slots of wall w (FxSlots<NT>) through put(slot, value), frain = GPREC
@@ -152,7 +142,7 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
     // fetch the grid and kcell tables (uniform) through the vector memory path, one
     // round trip per step of the departure-cell search, instead of the scalar cache.
     WallStateN<NT> s;
-    wall_state<NT, true>(a, l, w, ij, s);
+    wall_state<NT>(a, l, w, ij, s);
     WallFluxN<double, NT> F;
     double ft[NT][4];
     wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);
