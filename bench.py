@@ -220,6 +220,41 @@ def adjoint_cost(reps):
                       f"HIP events on the launch stream"}
 
 
+def adjoint_subcycled_cost(reps, div_num=2):
+    """Reverse mode of a sub-cycled call (MP_DIV_NUM = div_num) against the sub-cycled
+    forward call on the adjoint workload: CUDA events on the launch stream around
+    `reps` calls of each, after one warm-up of each."""
+    import torch
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    w = F.WORKLOADS["nsw6_16900x96_adjoint"]
+    gz, gzh, st = F.workload_state(w)
+    dev = F.to_device(st)
+    drv = NSW6(gz, gzh, max_columns=w.lall * w.ijdim)
+    drv.set_div_num(div_num)
+    stream = torch.cuda.current_stream()
+    fwd = F.copy_state(dev)
+    drv.step(fwd, w.dt, stream=stream)
+    cot = {"precip": 2.0 * fwd["precip"], "rhoge": torch.full_like(fwd["rhoge"], 1.0 / w.dt)}
+    grad, gcoef = drv.adjoint(dev, cot, w.dt, stream=stream)   # warm-up (allocates the scratch)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        drv.step(fwd, w.dt, stream=stream, check=False)
+    ev[1].record(stream)
+    for _ in range(reps):
+        drv.adjoint(dev, cot, w.dt, grad=grad, grad_coef=gcoef, stream=stream, check=False)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    rep = drv.check()
+    fwd_ms = ev[0].elapsed_time(ev[1]) / reps
+    adj_ms = ev[1].elapsed_time(ev[2]) / reps
+    drv.close()
+    return {"workload": w.name, "div_num": div_num, "forward_call_ms": fwd_ms, "adjoint_call_ms": adj_ms,
+            "ratio": adj_ms / fwd_ms, "calls_timed": reps, "general_regions": rep.n_general_regions}
+
+
 def subcycling_cost(steps, warmup, div_num=2):
     """MP_DIV_NUM sub-cycling on the FP64 workload (169,000 x 96): wall time of one
     mp_driver call of `div_num` sub-steps of dt / div_num, against div_num plain steps."""
@@ -355,6 +390,7 @@ def main():
         out["mixed"]["error_vs_fp64"] = mixed_error(F.WORKLOADS["nsw6_169000x96_mixed"], w.steps)
     if not args.only:
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
+        out["adjoint_subcycled"] = adjoint_subcycled_cost(args.adjoint_reps)
         out["subcycling"] = subcycling_cost(max(2, args.steps // 2), 1)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 8

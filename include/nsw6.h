@@ -180,8 +180,8 @@ int nsw6_set_precision(nsw6_ctx *ctx, int precision);
  * the ISO precipitations as the sub-step sums times 1/mp_div_num
  * (src/mod_mp_driver.cpp:420-612); the other diagnostics are the last sub-step's.
  * 1 <= mp_div_num <= NSW6_MAX_DIV_NUM.  The first call with mp_div_num > 1 allocates
- * the sums ((96 + 6) doubles per column).  nsw6_mp_driver_adj returns
- * NSW6_UNSUPPORTED while mp_div_num > 1. */
+ * the sums ((96 + 6) doubles per column).  nsw6_mp_driver_adj then differentiates
+ * the whole sub-cycled call (see there). */
 #define NSW6_MAX_DIV_NUM 1000
 int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num);
 
@@ -224,7 +224,16 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
  * the input state, and the saturation adjustment is differentiated at its converged
  * solution (implicit-function theorem).  Regions whose departures leave one cell
  * get gradient 0 and make nsw6_last_status() return NSW6_UNSUPPORTED.
- * A non-default scheme returns NSW6_UNSUPPORTED at once (nothing is launched). */
+ * A non-default scheme returns NSW6_UNSUPPORTED at once (nothing is launched).
+ * With mp_div_num = n > 1 (nsw6_set_div_num) the call is differentiated as the
+ * forward call runs it: n sub-steps of dt/n, precip, precip_rhoe/lh_heat/rhophi/
+ * rhokin, GPREC and the surface mass fluxes averaged over them (their cotangents reach
+ * every sub-step times 1/n), the state after the last sub-step.  The sub-step inputs
+ * are rebuilt by n-1 forward sub-steps into context scratch of 21*96 doubles per
+ * column for each of them, plus 2*21*96 + 6*96 + 9 doubles per column and the
+ * region flags (grown on demand).  A region that takes the general path in any
+ * sub-step is reported as above.  Timing ids 6/7 then cover the forward rebuild and
+ * the reversed sub-steps (each with its own forward part). */
 int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                        const nsw6_fields *state, const nsw6_fields *cot,
                        const double *cot_sflux, const nsw6_fields *grad,

@@ -104,7 +104,43 @@ __global__ void __launch_bounds__(256) k_adj_coef(AdjLaunch L, int nblocks)
     if (threadIdx.x == 0) L.grad_coef[c] = red[0];
 }
 
+// Elementwise helpers of the sub-cycled reverse mode (MP_DIV_NUM > 1, nsw6_api.cpp).
+__global__ void __launch_bounds__(256) k_scale(double *y, const double *x, double a, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+__global__ void __launch_bounds__(256) k_accumulate(double *y, const double *x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = y[i] + x[i];
+}
+__global__ void __launch_bounds__(64) k_merge_flags(int *acc, const int *flags, int n, int first)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < n) acc[i] = first ? flags[i] : (acc[i] | flags[i]);
+}
+
 }  // namespace
+
+extern "C" hipError_t nsw6_launch_scale(double *y, const double *x, double a, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, y, x, a, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t nsw6_launch_accumulate(double *y, const double *x, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, y, x, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t nsw6_launch_merge_flags(int *acc, const int *flags, int n, int first, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_merge_flags, dim3((n + 63) / 64), dim3(64), 0, stream, acc, flags, n, first);
+    return hipGetLastError();
+}
 
 // Number of blocks of the reverse passes (the coefficient partials hold NCOEF each).
 extern "C" int nsw6_adjoint_blocks(int lall, int ijdim) { return level_grid((ijdim + 255) / 256, KALL, lall); }

@@ -62,6 +62,12 @@ struct nsw6_ctx {
     int64_t t_adj_count = 0;
     int div_num = 1;                 // MP_DIV_NUM sub-steps per call
     double *div_sum = nullptr;       // their output sums (allocated at the first use)
+    // reverse mode of a sub-cycled call: sub-step inputs, cotangent carries and scaled
+    // output cotangents (grown on demand), and the general-region flags of all sub-steps
+    double *div_adj = nullptr;
+    size_t div_adj_doubles = 0;
+    int *div_adj_flags = nullptr;
+    int div_adj_regions = 0;
 };
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? NSW6_OK : NSW6_HIP_ERROR; }
@@ -316,32 +322,16 @@ static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)
     return NSW6_OK;
 }
 
-int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *state,
-                       const nsw6_fields *cot, const double *cot_sflux, const nsw6_fields *grad,
-                       double *grad_coef, double dt, double ct, void *stream)
+// One reverse step of dt (the whole of nsw6_mp_driver_adj at MP_DIV_NUM = 1).  The
+// diagnostics of the recomputed forward part go to context scratch.
+static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state, const nsw6_fields *cot,
+                    const double *cot_sflux, const nsw6_fields *grad, double *grad_coef, double dt,
+                    int keep_report, hipStream_t s, hipEvent_t *ev)
 {
-    (void)l_region;
-    (void)ct;
-    if (!ctx || !state || !cot || !grad || lall <= 0 || ijdim <= 0) return NSW6_BAD_ARG;
-    if ((int64_t)lall * ijdim > ctx->max_columns) return NSW6_BAD_ARG;
-    if (!(dt > 0.0)) return NSW6_BAD_ARG;
-    const nsw6_fields *fl = state;
-    const double *req[] = {fl->rhog, fl->rhogvx, fl->rhogvy, fl->rhogvz, fl->rhogw, fl->rhoge, fl->rhogq,
-                           fl->vx, fl->vy, fl->vz, fl->tem, fl->q, fl->gsgam2, fl->gsgam2h, fl->gam2,
-                           fl->gam2h, fl->ix, fl->iy, fl->iz, fl->jx, fl->jy, fl->jz, fl->z};
-    for (const double *p : req)
-        if (!p) return NSW6_BAD_ARG;
-    if (!adjoint_supported(ctx->P) || ctx->div_num != 1) return NSW6_UNSUPPORTED;
-    int st = use_device(ctx);
-    if (st == NSW6_OK) st = ensure_regions(ctx, lall);
-    if (st == NSW6_OK) st = ensure_adj_scratch(ctx, lall, ijdim);
-    if (st != NSW6_OK) return st;
-
     AdjArgs A;
     memset(&A, 0, sizeof A);
     StepArgs &a = A.a;
     a.f = to_fields(*state);
-    // diagnostics of the recomputed forward part go to scratch
     const size_t K2 = (size_t)KALL * (size_t)lall * (size_t)ijdim, C = (size_t)lall * (size_t)ijdim;
     double *d = ctx->adj.diag;
     a.f.re_liquid = d;
@@ -359,6 +349,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
     a.f.precip_rhophi = dc + 11 * C;
     a.f.precip_rhokin = dc + 12 * C;
     fill_step_args(ctx, lall, ijdim, dt, a);
+    a.keep_report = keep_report;
     A.cot = to_fields(*cot);
     A.cot_sflux = cot_sflux;
     A.grad = to_fields(*grad);
@@ -367,6 +358,230 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
     A.wr = A.cr + (size_t)NCR * K2;
     A.dws = A.wr + (size_t)NWR * K2;
     A.din = A.dws + (size_t)NDW * K2;
+    return hip_ok(nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev));
+}
+
+// The arrays one sub-step hands to the next: rhog, the momenta, rhoge, rhogq, tem and q
+// (QV is read; the halo levels consume the rest), plus rho, pre and qd, which a step
+// writes and the next overwrites.
+struct Carry {
+    double *rhog, *rhogvx, *rhogvy, *rhogvz, *rhogw, *rhoge, *rhogq, *tem, *q, *rho, *pre, *qd;
+};
+static constexpr int CARRY_K2 = 21;   // K2-sized rows of a Carry (rhogq and q hold 6 each)
+static Carry carry_at(double *base, size_t K2)
+{
+    Carry c;
+    double **one[] = {&c.rhog, &c.rhogvx, &c.rhogvy, &c.rhogvz, &c.rhogw, &c.rhoge, &c.tem, &c.rho, &c.pre};
+    size_t o = 0;
+    for (double **p : one) { *p = base + o; o += K2; }
+    c.rhogq = base + o; o += 6 * K2;
+    c.q = base + o; o += 6 * K2;
+    c.qd = nullptr;   // written by the forward sub-steps into the diagnostic scratch
+    return c;
+}
+static void put_carry(nsw6_fields &f, const Carry &c)
+{
+    f.rhog = c.rhog; f.rhogvx = c.rhogvx; f.rhogvy = c.rhogvy; f.rhogvz = c.rhogvz; f.rhogw = c.rhogw;
+    f.rhoge = c.rhoge; f.rhogq = c.rhogq; f.tem = c.tem; f.q = c.q;
+}
+
+// Scratch of the sub-cycled reverse mode, sized for the call (grown, never shrunk).
+static int ensure_div_adj(nsw6_ctx *ctx, size_t doubles, int lall)
+{
+    if (doubles <= ctx->div_adj_doubles && lall <= ctx->div_adj_regions) return NSW6_OK;
+    // a queued call may still use the old buffers
+    if (ctx->div_adj && hipDeviceSynchronize() != hipSuccess) return NSW6_HIP_ERROR;
+    (void)hipFree(ctx->div_adj);
+    (void)hipFree(ctx->div_adj_flags);
+    ctx->div_adj = nullptr;
+    ctx->div_adj_flags = nullptr;
+    ctx->div_adj_doubles = 0;
+    ctx->div_adj_regions = 0;
+    if (hipMalloc(&ctx->div_adj, sizeof(double) * doubles) != hipSuccess ||
+        hipMalloc(&ctx->div_adj_flags, sizeof(int) * (size_t)lall) != hipSuccess) {
+        (void)hipFree(ctx->div_adj);
+        ctx->div_adj = nullptr;
+        ctx->div_adj_flags = nullptr;
+        return NSW6_HIP_ERROR;
+    }
+    ctx->div_adj_doubles = doubles;
+    ctx->div_adj_regions = lall;
+    return NSW6_OK;
+}
+
+// Reverse mode of an mp_driver call of n = MP_DIV_NUM sub-steps of dt/n
+// (src/mod_mp_driver.cpp:420-612): the sub-step inputs are rebuilt forward (n-1 state
+// copies), then the sub-steps are reversed last to first.  The input cotangents of
+// sub-step m are the output cotangents of sub-step m-1 (rho, pre and qd, overwritten
+// by sub-step m, get none); the accumulated outputs (precip and its energy fluxes,
+// GPREC, the surface mass fluxes) are fraction_mp times the sum over the sub-steps, so
+// every sub-step receives fraction_mp times their cotangents; the cotangents of vx, vy,
+// vz and of the coefficients are summed over the sub-steps.  Events (NULL or 3): start,
+// after the forward rebuild, end.
+static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state, const nsw6_fields *cot,
+                         const double *cot_sflux, const nsw6_fields *grad, double *grad_coef, double dt,
+                         hipStream_t s, hipEvent_t *ev)
+{
+    const int n = ctx->div_num;
+    const double fraction_mp = 1.0 / double(n);
+    const double dt_mp = dt * fraction_mp;
+    const size_t C = (size_t)lall * (size_t)ijdim, K2 = (size_t)KALL * C;
+    // layout: n-1 carries | 2 cotangent carries | vx, vy, vz | write-only rows (qd and
+    // the constant diagnostics) | scaled GPREC | scaled column cotangents | coefficients
+    const size_t n_carry = (size_t)(n - 1) * CARRY_K2 * K2, n_cot = 2 * (size_t)CARRY_K2 * K2;
+    const size_t doubles = n_carry + n_cot + 3 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF;
+    int st = ensure_div_adj(ctx, doubles, lall);
+    if (st != NSW6_OK) return st;
+    double *base = ctx->div_adj;
+    double *cotbuf = base + n_carry;
+    double *vtmp = cotbuf + n_cot;
+    double *junk = vtmp + 3 * K2;   // qd (K2) and re_solid / GDCLW / GDCFRC (K2, same thread, same index)
+    double *gprec_s = junk + 2 * K2;
+    double *col_s = gprec_s + K2;
+    double *coef_tmp = col_s + 9 * C;
+    auto carry = [&](int m) { return carry_at(base + (size_t)(m - 1) * CARRY_K2 * K2, K2); };   // m = 1..n-1
+    auto cotc = [&](int i) { return carry_at(cotbuf + (size_t)i * CARRY_K2 * K2, K2); };
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
+    auto copy = [&](double *dst, const double *src, size_t count) {
+        return ok(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, s));
+    };
+
+    if (ev && !ok(hipEventRecord(ev[0], s))) return NSW6_HIP_ERROR;
+    // ---- forward: the inputs of sub-steps 1..n-1
+    {
+        StepArgs a;
+        memset(&a, 0, sizeof a);
+        a.f = to_fields(*state);
+        double *d = ctx->adj.diag;
+        a.f.re_liquid = d;
+        a.f.re_cld = d + K2;
+        a.f.GPREC = d + 2 * K2;
+        double *dc = d + 3 * K2;
+        a.f.rctop = dc;
+        a.f.tctop = dc + C;
+        a.f.rwtop = dc + 2 * C;
+        a.f.precip = dc + 3 * C;
+        a.f.ISO1_precip = dc + 5 * C;
+        a.f.ISO2_precip = dc + 7 * C;
+        a.f.precip_rhoe = dc + 9 * C;
+        a.f.precip_lh_heat = dc + 10 * C;
+        a.f.precip_rhophi = dc + 11 * C;
+        a.f.precip_rhokin = dc + 12 * C;
+        a.f.qd = junk;
+        a.f.re_solid = a.f.GDCLW = a.f.GDCFRC = junk + K2;
+        fill_step_args(ctx, lall, ijdim, dt_mp, a);
+        for (int m = 1; m < n; m++) {
+            const Carry dst = carry(m);
+            nsw6_fields src_f = *state;
+            if (m > 1) put_carry(src_f, carry(m - 1));
+            const double *rows[] = {src_f.rhog, src_f.rhogvx, src_f.rhogvy, src_f.rhogvz, src_f.rhogw,
+                                    src_f.rhoge, src_f.tem};
+            double *rows_d[] = {dst.rhog, dst.rhogvx, dst.rhogvy, dst.rhogvz, dst.rhogw, dst.rhoge, dst.tem};
+            for (int i = 0; i < 7; i++) copy(rows_d[i], rows[i], K2);
+            copy(dst.rhogq, src_f.rhogq, 6 * K2);
+            copy(dst.q, src_f.q, 6 * K2);
+            a.f.rhog = dst.rhog; a.f.rhogvx = dst.rhogvx; a.f.rhogvy = dst.rhogvy; a.f.rhogvz = dst.rhogvz;
+            a.f.rhogw = dst.rhogw; a.f.rhoge = dst.rhoge; a.f.rhogq = dst.rhogq; a.f.tem = dst.tem;
+            a.f.q = dst.q; a.f.rho = dst.rho; a.f.pre = dst.pre;
+            a.keep_report = (m > 1);
+            if (!ok(nsw6_launch_step(&a, NSW6_FP64, s, nullptr))) return NSW6_HIP_ERROR;
+        }
+    }
+    if (ev && !ok(hipEventRecord(ev[1], s))) return NSW6_HIP_ERROR;
+
+    // ---- the accumulated outputs' cotangents, times fraction_mp (NULL stays NULL)
+    nsw6_fields cdiag;
+    memset(&cdiag, 0, sizeof cdiag);
+    const double *csrc[] = {cot->precip, cot->precip_rhoe, cot->precip_lh_heat, cot->precip_rhophi,
+                            cot->precip_rhokin};
+    double **cdst[] = {&cdiag.precip, &cdiag.precip_rhoe, &cdiag.precip_lh_heat, &cdiag.precip_rhophi,
+                       &cdiag.precip_rhokin};
+    const size_t crows[] = {2 * C, C, C, C, C};
+    size_t off = 0;
+    for (int i = 0; i < 5; i++) {
+        if (csrc[i]) {
+            *cdst[i] = col_s + off;
+            if (!ok(nsw6_launch_scale(*cdst[i], csrc[i], fraction_mp, (int64_t)crows[i], s))) return NSW6_HIP_ERROR;
+        }
+        off += crows[i];
+    }
+    double *sflux_s = nullptr;
+    if (cot_sflux) {
+        sflux_s = col_s + off;   // 3 C
+        if (!ok(nsw6_launch_scale(sflux_s, cot_sflux, fraction_mp, (int64_t)(3 * C), s))) return NSW6_HIP_ERROR;
+    }
+    if (cot->GPREC) {
+        cdiag.GPREC = gprec_s;
+        if (!ok(nsw6_launch_scale(gprec_s, cot->GPREC, fraction_mp, (int64_t)K2, s))) return NSW6_HIP_ERROR;
+    }
+
+    // ---- reverse, last sub-step first
+    for (int m = n - 1; m >= 0; m--) {
+        nsw6_fields st_m = *state;
+        if (m > 0) put_carry(st_m, carry(m));
+        // output cotangents of sub-step m
+        nsw6_fields c_m = cdiag;
+        if (m == n - 1) {
+            c_m.rhog = cot->rhog; c_m.rhogvx = cot->rhogvx; c_m.rhogvy = cot->rhogvy; c_m.rhogvz = cot->rhogvz;
+            c_m.rhogw = cot->rhogw; c_m.rhoge = cot->rhoge; c_m.rhogq = cot->rhogq; c_m.tem = cot->tem;
+            c_m.q = cot->q; c_m.rho = cot->rho; c_m.pre = cot->pre; c_m.qd = cot->qd;
+        } else {
+            put_carry(c_m, cotc((m + 1) & 1));   // written by sub-step m+1; rho, pre, qd stay NULL
+        }
+        // input cotangents of sub-step m
+        nsw6_fields g_m;
+        memset(&g_m, 0, sizeof g_m);
+        if (m == 0) {
+            g_m = *grad;
+        } else {
+            put_carry(g_m, cotc(m & 1));
+        }
+        const bool first = (m == n - 1);
+        g_m.vx = grad->vx ? (first ? grad->vx : vtmp) : nullptr;
+        g_m.vy = grad->vy ? (first ? grad->vy : vtmp + K2) : nullptr;
+        g_m.vz = grad->vz ? (first ? grad->vz : vtmp + 2 * K2) : nullptr;
+        double *coef_m = grad_coef ? (first ? grad_coef : coef_tmp) : nullptr;
+        st = adj_step(ctx, lall, ijdim, &st_m, &c_m, sflux_s, &g_m, coef_m, dt_mp, 1, s, nullptr);
+        if (st != NSW6_OK) return st;
+        if (!first) {
+            if (grad->vx && !ok(nsw6_launch_accumulate(grad->vx, vtmp, (int64_t)K2, s))) return NSW6_HIP_ERROR;
+            if (grad->vy && !ok(nsw6_launch_accumulate(grad->vy, vtmp + K2, (int64_t)K2, s))) return NSW6_HIP_ERROR;
+            if (grad->vz && !ok(nsw6_launch_accumulate(grad->vz, vtmp + 2 * K2, (int64_t)K2, s)))
+                return NSW6_HIP_ERROR;
+            if (grad_coef && !ok(nsw6_launch_accumulate(grad_coef, coef_tmp, NSW6_NCOEF, s))) return NSW6_HIP_ERROR;
+        }
+        // a region that took the general path in any sub-step is reported
+        if (!ok(nsw6_launch_merge_flags(ctx->div_adj_flags, ctx->ws.region_flag, lall, first, s)))
+            return NSW6_HIP_ERROR;
+    }
+    if (!ok(hipMemcpyAsync(ctx->ws.region_flag, ctx->div_adj_flags, sizeof(int) * (size_t)lall,
+                           hipMemcpyDeviceToDevice, s)))
+        return NSW6_HIP_ERROR;
+    if (ev && !ok(hipEventRecord(ev[2], s))) return NSW6_HIP_ERROR;
+    return NSW6_OK;
+}
+
+int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *state,
+                       const nsw6_fields *cot, const double *cot_sflux, const nsw6_fields *grad,
+                       double *grad_coef, double dt, double ct, void *stream)
+{
+    (void)l_region;
+    (void)ct;
+    if (!ctx || !state || !cot || !grad || lall <= 0 || ijdim <= 0) return NSW6_BAD_ARG;
+    if ((int64_t)lall * ijdim > ctx->max_columns) return NSW6_BAD_ARG;
+    if (!(dt > 0.0)) return NSW6_BAD_ARG;
+    const nsw6_fields *fl = state;
+    const double *req[] = {fl->rhog, fl->rhogvx, fl->rhogvy, fl->rhogvz, fl->rhogw, fl->rhoge, fl->rhogq,
+                           fl->vx, fl->vy, fl->vz, fl->tem, fl->q, fl->gsgam2, fl->gsgam2h, fl->gam2,
+                           fl->gam2h, fl->ix, fl->iy, fl->iz, fl->jx, fl->jy, fl->jz, fl->z};
+    for (const double *p : req)
+        if (!p) return NSW6_BAD_ARG;
+    if (!adjoint_supported(ctx->P)) return NSW6_UNSUPPORTED;
+    int st = use_device(ctx);
+    if (st == NSW6_OK) st = ensure_regions(ctx, lall);
+    if (st == NSW6_OK) st = ensure_adj_scratch(ctx, lall, ijdim);
+    if (st != NSW6_OK) return st;
 
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t *ev = nullptr;
@@ -376,12 +591,15 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
             if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
         ev = pt.ev;
     }
-    hipError_t e = nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev);
+    if (ctx->div_num == 1)
+        st = adj_step(ctx, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt, 0, s, ev);
+    else
+        st = adj_subcycled(ctx, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt, s, ev);
     if (ctx->timing) ctx->pending_adj.push_back(pt);
     ctx->last_stream = s;
     ctx->last_lall = lall;
     ctx->last_adjoint = true;
-    return hip_ok(e);
+    return st;
 }
 
 int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region, double *rhog, double *rhogvx,
@@ -506,6 +724,8 @@ void nsw6_finalize(nsw6_ctx *ctx)
     (void)hipDeviceSynchronize();
     free_ws(ctx);
     if (ctx->div_sum) (void)hipFree(ctx->div_sum);
+    if (ctx->div_adj) (void)hipFree(ctx->div_adj);
+    if (ctx->div_adj_flags) (void)hipFree(ctx->div_adj_flags);
     if (ctx->grid_dev) (void)hipFree(ctx->grid_dev);
     if (ctx->P_dev) (void)hipFree(ctx->P_dev);
     delete ctx;

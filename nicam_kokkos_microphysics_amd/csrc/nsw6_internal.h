@@ -174,3 +174,7 @@ constexpr int NSW6_ADJ_EVENTS = 3;
 extern "C" int nsw6_adjoint_blocks(int lall, int ijdim);
 extern "C" hipError_t nsw6_launch_adjoint(const nsw6::AdjArgs *adj, double *coef_part, double *grad_coef,
                                           hipStream_t stream, hipEvent_t *ev);
+// Sub-cycled reverse mode: y = a x, y += x (n doubles), acc = flags (first) or acc | flags.
+extern "C" hipError_t nsw6_launch_scale(double *y, const double *x, double a, int64_t n, hipStream_t stream);
+extern "C" hipError_t nsw6_launch_accumulate(double *y, const double *x, int64_t n, hipStream_t stream);
+extern "C" hipError_t nsw6_launch_merge_flags(int *acc, const int *flags, int n, int first, hipStream_t stream);

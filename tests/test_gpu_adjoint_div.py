@@ -1,0 +1,161 @@
+"""Reverse mode of a sub-cycled mp_driver call: nsw6_mp_driver_adj after
+nsw6_set_div_num(n), n > 1 (src/mod_mp_driver.cpp:420-612 runs n NSW6 sub-steps of
+dt/n and averages precip, its energy fluxes and GPREC over them).
+
+* The call's gradient equals the chain written here from single-step reverse calls of
+  dt/n on a context with n = 1, bit for bit, for n = 2 and 3: the sub-step inputs are
+  rebuilt with single forward steps; every sub-step receives 1/n of the cotangents of
+  the averaged outputs (and of the surface mass fluxes); the input cotangents of
+  sub-step m are the state cotangents of sub-step m-1 (rho, pre, qd get none); the
+  cotangents of vx, vy, vz and the coefficients are summed.  A wrong scale, a lost
+  hand-over or a missed sum changes the result.
+* Central finite differences of the GPU's own sub-cycled forward call (h = 1e-8 and
+  2e-8 relative, directions whose two quotients differ by more than 1e-4 are crossing
+  a switch and are skipped; at least two of four must be checked) agree with the
+  gradient to 1e-5 relative along random directions through every input and
+  coefficient.  This check does not share the chain's construction.
+* A region that takes the general sedimentation path in a sub-step makes the call
+  report NSW6_UNSUPPORTED, as the single step does.
+"""
+import numpy as np
+import pytest
+
+from nicam_kokkos_microphysics_amd import _native as N
+from nicam_kokkos_microphysics_amd import fields as F
+
+from adjoint_helpers import DT, coefs, directional, random_cotangents, random_directions
+
+pytestmark = pytest.mark.gpu
+
+AVERAGED = ("precip", "precip_rhoe", "precip_lh_heat", "precip_rhophi", "precip_rhokin", "GPREC")
+CARRIED = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "tem", "q")
+
+
+def _state(lall=2, ijdim=256, seed=5):
+    return F.synth_state(lall, ijdim, seed=seed, frac_supersat=0.2, frac_towers=0.1)
+
+
+def _chain(one, dev, cot, cs, n):
+    """The sub-cycled reverse mode composed from single-step calls of dt/n."""
+    import torch
+    dt = DT * (1.0 / n)
+    frac = 1.0 / float(n)
+    inputs = [dev]
+    for m in range(1, n):
+        s = F.copy_state(inputs[-1])
+        one.step(s, dt)
+        inputs.append(s)
+    avg = {k: frac * cot[k] for k in AVERAGED if k in cot}
+    cs_n = None if cs is None else frac * cs
+    carry = {k: v for k, v in cot.items() if k not in AVERAGED}
+    gv, gcoef = None, None
+    for m in range(n - 1, -1, -1):
+        c = dict(avg)
+        c.update(carry)
+        g, gc = one.adjoint(inputs[m], c, dt, cot_sflux=cs_n)
+        v = {k: g[k] for k in ("vx", "vy", "vz")}
+        gv = v if gv is None else {k: gv[k] + v[k] for k in v}
+        gcoef = gc if gcoef is None else gcoef + gc
+        carry = {k: g[k] for k in CARRIED}
+    torch.cuda.synchronize()
+    out = dict(carry)
+    out.update(gv)
+    return out, gcoef
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_subcycled_reverse_is_the_chain_of_substeps(n):
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    gz, gzh, st = _state()
+    cot_h, cs_h = random_cotangents(st, gz, gzh, 13)
+    dev = F.to_device(st)
+    cot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot_h.items()}
+    cs = torch.from_numpy(np.ascontiguousarray(cs_h)).cuda()
+    cols = st["rhog"].shape[0] * st["rhog"].shape[-1]
+    sub = NSW6(gz, gzh, max_columns=cols)
+    sub.set_div_num(n)
+    one = NSW6(gz, gzh, max_columns=cols)
+    print(f"sub-cycled reverse n={n}", flush=True)
+    g, gc = sub.adjoint(dev, cot, DT, cot_sflux=cs)
+    torch.cuda.synchronize()
+    rep = sub.report()
+    assert rep.status == 0 and rep.n_general_regions == 0
+    want, want_c = _chain(one, dev, cot, cs, n)
+    for k in F.ADJ_INPUTS:
+        assert torch.equal(g[k], want[k]), (k, float((g[k] - want[k]).abs().max()))
+    assert torch.equal(gc, want_c)
+    # the sub-cycled gradient is not the single step's
+    g1, _ = one.adjoint(dev, cot, DT, cot_sflux=cs)
+    assert not torch.equal(g1["tem"], g["tem"])
+    sub.close()
+    one.close()
+
+
+def _loss(drv, st_dev):
+    """Sum of squared surface precipitation plus the rhoge tendency of one call."""
+    import torch
+    s = F.copy_state(st_dev)
+    rhoge_in = s["rhoge"].clone()
+    drv.step(s, DT)
+    torch.cuda.synchronize()
+    p = s["precip"]
+    return float((p * p).sum() + ((s["rhoge"] - rhoge_in) / DT).sum()), s
+
+
+def test_subcycled_reverse_matches_central_differences():
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state(lall=1, ijdim=256, seed=7)
+    drv = NSW6(gz, gzh, max_columns=256)
+    drv.set_div_num(2)
+    dev = F.to_device(st)
+    _, out = _loss(drv, dev)
+    cot = {"precip": 2.0 * out["precip"], "rhoge": torch.full_like(out["rhoge"], 1.0 / DT)}
+    grad, gc = drv.adjoint(dev, cot, DT)
+    grad["rhoge"] = grad["rhoge"] - 1.0 / DT   # explicit -rhoge_in/dt term of the loss
+    g = {k: v.cpu().numpy() for k, v in grad.items()}
+    gcoef = gc.cpu().numpy()
+    dirs, dcoef = random_directions(st, 31, 4)
+    c0 = coefs()
+
+    def fd(i, h):
+        vals = []
+        for sgn in (1.0, -1.0):
+            P = default_params()
+            for c, name in enumerate(N.TUNABLE_COEFFS):
+                setattr(P, name, c0[c] + sgn * h * dcoef[i][c])
+            drv.set_params(P)
+            sp = F.copy_state(st)
+            for k, d in dirs[i].items():
+                sp[k] = sp[k] + sgn * h * d
+            vals.append(_loss(drv, F.to_device(sp))[0])
+        return (vals[0] - vals[1]) / (2 * h)
+
+    checked = 0
+    for i in range(len(dirs)):
+        f1, f2 = fd(i, 1e-8), fd(i, 2e-8)
+        if abs(f1 - f2) > 1e-4 * abs(f1):
+            continue
+        s, _ = directional(g, gcoef, dirs[i], dcoef[i])
+        assert abs(s - f1) <= 1e-5 * abs(f1), (i, s, f1)
+        checked += 1
+    assert checked >= 2, checked
+    drv.close()
+
+
+def test_subcycled_reverse_reports_general_regions():
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6, NSW6Error
+    gz, gzh, st = _state(lall=2, ijdim=300, seed=11)
+    cot_h, _ = random_cotangents(st, gz, gzh, 5)
+    drv = NSW6(gz, gzh, max_columns=600)
+    drv.set_div_num(2)
+    dev = F.to_device(st)
+    cot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot_h.items()}
+    # sub-steps of 600 s carry rain beyond one cell near the surface: the general path
+    with pytest.raises(NSW6Error) as ei:
+        drv.adjoint(dev, cot, 1200.0)
+    assert ei.value.code == N.NSW6_UNSUPPORTED
+    assert drv.report().n_general_regions > 0
+    drv.close()
