@@ -775,7 +775,12 @@ NSW6_HD PointIn<double> opaque_point(const PointIn<double> &a)
 // phase-2 recompute: element j at base[j * stride] (LDS on the device, a local array
 // on the host).  Kept in registers, the NRATE + 6 values would be live through the
 // whole recompute.
-constexpr int RATE_STASH = NRATE + 6;
+// Slots: [0, NRATE) raw-rate cotangents, then dq/dt cotangents (6), the forward
+// tendencies qt[1..5] (5, from phase 1 to the return), and point_rates_b's own
+// state-update cotangents held across rates_b (POINT_HELD).
+constexpr int RATE_STASH = NRATE + 6 + 5;
+constexpr int POINT_HELD = 14;
+constexpr int POINT_STASH = RATE_STASH + POINT_HELD;
 struct RateStash {
     double *base;
     int stride;
@@ -993,6 +998,8 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     for (int j = 0; j < NRATE; j++) st[j] = raw_b[j];
 #pragma unroll
     for (int n = 0; n < 6; n++) st[NRATE + n] = dqdt_b[n];
+#pragma unroll
+    for (int n = 1; n <= 5; n++) st[NRATE + 6 + n - 1] = qt[n];
     stash_fence();
     // ---- phase 2: the forward values again, from inputs the compiler cannot see
     // through (so it recomputes them rather than keeping phase 1 values live), then
@@ -1437,6 +1444,9 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     b.temp += temp_b;
     b.temc += temc_b;
     b.sice_m1 += sice_b;
+    stash_fence();
+#pragma unroll
+    for (int n = 1; n <= 5; n++) qt[n] = st[NRATE + 6 + n - 1];
 }
 
 // ---------------------------------------------------------------------------------
@@ -1476,9 +1486,25 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         for (int n = 0; n < NQ; n++) qt_b[n] = drq_b[n] * rhog * dt;
 #pragma unroll
         for (int n = 1; n < NQ; n++) qt_b[n] -= qt_b[QV];
-        // the rate kernel
+        // the rate kernel; the state-update cotangents wait in the stash meanwhile
+#pragma unroll
+        for (int n = 0; n < NQ; n++) {
+            st[RATE_STASH + n] = drq_b[n];
+            st[RATE_STASH + NQ + n] = rq_b[n];
+        }
+        st[RATE_STASH + 2 * NQ] = rhoge0_b;
+        st[RATE_STASH + 2 * NQ + 1] = rhog_b;
+        stash_fence();
         double qt[NQ];
         rates_b(P, dt, in, qt_b, yb.vt, ib, qt, st);
+        stash_fence();
+#pragma unroll
+        for (int n = 0; n < NQ; n++) {
+            drq_b[n] = st[RATE_STASH + n];
+            rq_b[n] = st[RATE_STASH + NQ + n];
+        }
+        rhoge0_b = st[RATE_STASH + 2 * NQ];
+        rhog_b = st[RATE_STASH + 2 * NQ + 1];
         qt[QV] = -(qt[QC] + qt[QR] + qt[QI] + qt[QS] + qt[QG]);
 #pragma unroll
         for (int n = 0; n < NQ; n++) rhog_b += drq_b[n] * qt[n] * dt;
@@ -1564,7 +1590,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
 {
-    double buf[RATE_STASH];
+    double buf[POINT_STASH];
     point_rates_b(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1});
 }
 

@@ -50,12 +50,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 //   of the launch overwrites it in global memory, so its uniform reads would otherwise
 //   take the vector memory path (about 150 of them per point), each a round trip that
 //   one wave per SIMD cannot hide;
-// * the rate reverse's stash (RATE_STASH doubles per thread, column threadIdx.x), which
-//   takes those values out of the registers during the phase-2 recompute.
+// * the rate reverse's stash (POINT_STASH doubles per thread, column threadIdx.x), which
+//   takes the values that wait through the rate reverse and its phase-2 recompute out
+//   of the registers.
 __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
 {
     __shared__ double red[4][NCOEF];
-    __shared__ double stash[RATE_STASH][256];
+    __shared__ double stash[POINT_STASH][256];
     __shared__ DevParams sP;
     {
         const int *src = reinterpret_cast<const int *>(L.A.a.P);

@@ -392,7 +392,7 @@ NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOE
 }
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
 {
-    double buf[RATE_STASH];
+    double buf[POINT_STASH];
     adj_point(A, l, k, ij, tun_b, RateStash{buf, 1});
 }
 

@@ -16,11 +16,16 @@
 
 namespace nsw6 {
 
-// Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used).
+// Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used: they read
+// level 0 and take 0, so the loads need no branch and issue together).
 NSW6_HD void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
 {
 #pragma unroll
-    for (int d = -3; d <= 1; d++) V[d + 3] = (k + d >= 0) ? vt_col[(int64_t)(k + d) * ijdim] : 0.0;
+    for (int d = -3; d <= 1; d++) {
+        const int kk = (k + d >= 0) ? k + d : 0;
+        const double v = vt_col[(int64_t)kk * ijdim];
+        V[d + 3] = (k + d >= 0) ? v : 0.0;
+    }
 }
 
 // Pre-sedimentation cell j of the column at p2 = l*K2 + ij, p3 = l*NQ*K2 + ij.
@@ -88,8 +93,8 @@ template <int NT> struct WallStateN {
     WallGeomN<NT> geo;
 };
 using WallState = WallStateN<3>;
-// The cells first, then the fall speeds, the kcell bounds and the departure geometry:
-// every load of the wall is requested before any arithmetic waits on one.
+// Every load of the wall first (the cells, the fall speeds, the kcell bounds), then the
+// derived cells and the departure geometry: one memory round trip per wall.
 template <int NT>
 NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> &s)
 {
@@ -98,24 +103,21 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> 
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t pv = (int64_t)l * NT * K2 + ij;
     const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
-    auto geometry = [&]() {
-        int kcmax[NT], khmax[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
-            kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
-            khmax[t] = bnd[bounds_index(t, 1, w, 1)];
-        }
-        wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
-    };
     const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij;
     s.u = load_unit(a.f, (int64_t)l * ijdim + ij);
     s.m = load_below<NT>(a, p2, p3, K2, ijdim, w);
     load_cell(a, p2, p3, K2, ijdim, w, s.r0);
-    derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
     load_cell(a, p2, p3, K2, ijdim, w + 1, s.r1);
+    int kcmax[NT], khmax[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
+        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
+    }
+    derive_cell(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur);
     derive_cell(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt);
-    geometry();
+    wall_geometry(g, w, a.dt, a.dt2, a.dt3_6, s.V, kcmax, khmax, s.z0, s.zh0, s.geo);
 }
 
 // Flux slots of wall w (FxSlots<NT>) through put(slot, value), frain = GPREC

@@ -472,6 +472,11 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
     // the other outputs at the end: every store comes after every load, so that the
     // uniform reads of the grid and the parameters stay on the scalar cache.
     TailIn<double> x;
+    // read by every path: loaded before the branches, with the path's own loads
+    x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
+    x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
+    if (NT == 3) x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+    x.gsgam2 = f.gsgam2[p];
     double mom[4];   // rhogvx, rhogvy, rhogvz, rhogw after sedimentation (single-cell regions)
     if (interior) {
         if (general) {
@@ -506,10 +511,6 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
             mom[3] = (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog, FXV) : f.rhogw[p] + 0.0;
         }
     }
-    x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
-    x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
-    if (NT == 3) x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
-    x.gsgam2 = f.gsgam2[p];
 
     TailOut<double> y;
     const TailStatus st = tail(*a.P, interior, x, y);
