@@ -79,6 +79,10 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     const NcPoint nc = INDIR ? nc_indirect(P, f.unccn[p]) : nc_default(P);
     PointOut<double> y;
     point_rates<double, R>(P, a.dt, x, tun, c_a1_tab, c_a2_tab, y, nc);
+    // the effective radii before any store: after one, the compiler would read the
+    // parameters they use through the vector memory path instead of the scalar cache
+    double rceff = 0.0, rceff_cld = 0.0, rct = 0.0, tct = 0.0, csw = 0.0;
+    if (DIAG) effective_radius(P, y.sw, y.rho, x.tem, y.q, nc.Nc, rceff, rceff_cld, rct, tct, csw);
 
     a.ws.rhog[p] = y.rhog;
     a.ws.rhoge[p] = y.rhoge;
@@ -94,8 +98,6 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
         for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
     }
     if (!DIAG) return;
-    double rceff, rceff_cld, rct, tct, csw;
-    effective_radius(P, y.sw, y.rho, x.tem, y.q, nc.Nc, rceff, rceff_cld, rct, tct, csw);
     f.re_liquid[p] = rceff;
     f.re_cld[p] = rceff_cld;
     if (a.cloudtop_intended) {
