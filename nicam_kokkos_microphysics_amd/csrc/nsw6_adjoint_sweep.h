@@ -35,6 +35,17 @@ namespace nsw6 {
 
 NSW6_HD double cot_at(const double *p, int64_t i) { return p ? p[i] : 0.0; }
 
+// Record slot of level c of column p2 = l*K2 + ij, 0 outside the interior levels.  The
+// records exist for every level (0 at the halos), so the load is made at c clamped to
+// 0..KALL-1 without a branch and the result selected: the gathers of a pass then issue
+// together instead of one round trip each.
+NSW6_HD double record_at(const double *rec, int slot, int64_t N2, int64_t p2, int c, int ijdim)
+{
+    const int cc = (c < 0) ? 0 : ((c > KALL - 1) ? KALL - 1 : c);
+    const double v = rec[slot * N2 + p2 + (int64_t)cc * ijdim];
+    return (c >= KMIN && c <= KMAX) ? v : 0.0;
+}
+
 NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
 {
     const StepArgs &a = A.a;
@@ -185,9 +196,7 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
         // the wall's flux cotangent, gathered from the interior cells that read it:
         // cell w+1-m reads fq of wall w as its wall m (m = 0..3), cell w-m reads fs as
         // its wall m (m = 0..1), cell w+1-m reads fh as its wall m (m = 0..2)
-        auto CRV = [&](int slot, int c) {
-            return (c >= KMIN && c <= KMAX) ? A.cr[slot * N2 + p2 + (int64_t)c * ijdim] : 0.0;
-        };
+        auto CRV = [&](int slot, int c) { return record_at(A.cr, slot, N2, p2, c, ijdim); };
         WallFlux<double> Fb;
 #pragma unroll
         for (int t = 0; t < 3; t++) {
@@ -295,12 +304,8 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
     const int64_t p = p2 + (int64_t)k * ijdim;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
     const Fields &f = a.f;
-    auto CRV = [&](int slot, int c) {
-        return (c >= KMIN && c <= KMAX) ? A.cr[slot * N2 + p2 + (int64_t)c * ijdim] : 0.0;
-    };
-    auto WRV = [&](int slot, int w) {
-        return (w >= KMIN && w <= KMAX) ? A.wr[slot * N2 + p2 + (int64_t)w * ijdim] : 0.0;
-    };
+    auto CRV = [&](int slot, int c) { return record_at(A.cr, slot, N2, p2, c, ijdim); };
+    auto WRV = [&](int slot, int w) { return record_at(A.wr, slot, N2, p2, w, ijdim); };
     // cell k is cell c (own record), c-1 of cell k+1, c+1 of cell k-1; cell w-1 of
     // wall k+1, cell w of wall k, cell w+1 of wall k-1
     const double rg_b = A.dws[0 * N2 + p] + CRV(CR_RG + 1, k) + CRV(CR_RG + 0, k + 1) + CRV(CR_RG + 2, k - 1) +
