@@ -206,7 +206,8 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
     double *GPREC, double *CBMFX, void *stream);
 
 /* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
- * explicit ice nucleation; sw_expice = 1): given the step's INPUT state `state`
+ * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; with or without OPT_INDIR,
+ * which then reads `state->unccn`): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)
  * and cotangents of the step's outputs, write the cotangents of its inputs,
  *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.
@@ -218,7 +219,7 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
  * Input cotangents `grad` (NULL members are not written): rhog, rhogvx, rhogvy,
  *   rhogvz, rhogw, rhoge (nonzero on the two halo levels only: the interior internal
  *   energy is recomputed from tem), rhogq, vx, vy, vz, tem, q (QV only; other
- *   species get 0).  `grad_coef` (device, NSW6_NCOEF doubles, may be NULL): the
+ *   species get 0), unccn (through Nc^-1.79 of KK2000 under OPT_INDIR; 0 otherwise).  `grad_coef` (device, NSW6_NCOEF doubles, may be NULL): the
  *   cotangents of the tunable coefficients in NSW6_NCOEF order.
  * Discrete choices (switches, limiters, kcell bounds, departure cells) are taken at
  * the input state, and the saturation adjustment is differentiated at its converged
@@ -230,7 +231,7 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
  * rhokin, GPREC and the surface mass fluxes averaged over them (their cotangents reach
  * every sub-step times 1/n), the state after the last sub-step.  The sub-step inputs
  * are rebuilt by n-1 forward sub-steps into context scratch of 21*96 doubles per
- * column for each of them, plus 2*21*96 + 6*96 + 9 doubles per column and the
+ * column for each of them, plus 2*21*96 + 7*96 + 9 doubles per column and the
  * region flags (grown on demand).  A region that takes the general path in any
  * sub-step is reported as above.  Timing ids 6/7 then cover the forward rebuild and
  * the reversed sub-steps (each with its own forward part). */

@@ -706,6 +706,7 @@ namespace nsw6 {
 // ---------------------------------------------------------------------------------
 struct RateInB {
     double dens, temp, q[6], qsatl, qsati, pre, temc, sice_m1, sliq_1m, qv_qsati, tun[NCOEF];
+    double nc_pm179;   // Nc^-1.79 of the point (per point under OPT_INDIR)
 };
 
 // rates that enter the limiter blocks and the tendencies
@@ -796,7 +797,9 @@ NSW6_HD void stash_fence()
 }
 
 // Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
-NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const double qt_b[6],
+// pm179: Nc^-1.79 of the point (P.Nc_pow_m179 without OPT_INDIR); its cotangent is
+// added to b.nc_pm179.
+NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, double pm179, const double qt_b[6],
                      const double vt_b[3], RateInB &b, double qt[6], const RateStash &st)
 {
     double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
@@ -805,6 +808,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     // ---- phase 1: forward, limiter decisions, reverse of the limiter algebra -> raw_b
     {
         const RateIn<double> &in = in0;
+        const double nc_pm179 = pm179;
 #include "nsw6_rates_fwd.inc"
         // limiters: R = min(P0, dq_dt) * c; lim = true when dq_dt is taken
         double R[NRATE];
@@ -1005,6 +1009,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     // through (so it recomputes them rather than keeping phase 1 values live), then
     // the reverse of the rate formulas
     const RateIn<double> in = opaque_copy(in0);
+    const double nc_pm179 = opaque(pm179);
 #include "nsw6_rates_fwd.inc"
     stash_fence();
 #pragma unroll
@@ -1190,8 +1195,9 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
         if (biE) temc_b += Psaut0_b * (qi - qicrt) * P.beta_saut * ebi * P.gamma_saut;
     }
     // Praut0 = kk qc^2.47 Nc^-1.79
-    tun_b[0] += Praut0_b * Eqc * P.Nc_pow_m179;
-    if (qc > 0.0) q_b[QC] += Praut0_b * kk * P.Nc_pow_m179 * 2.47 * Eqc / qc;
+    tun_b[0] += Praut0_b * Eqc * nc_pm179;
+    b.nc_pm179 += Praut0_b * kk * Eqc;
+    if (qc > 0.0) q_b[QC] += Praut0_b * kk * nc_pm179 * 2.47 * Eqc / qc;
     // Pgacs0 = As kac Rdens Egs_mod N0g |Vtg - Vts| SGS
     {
         const double k0 = As * kac * N0g;
@@ -1454,13 +1460,15 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
 // ---------------------------------------------------------------------------------
 struct PointInB {
     double rhog, rhogq[6], tem, qv;
+    double nc_pm179;   // cotangent of the point's Nc^-1.79
 };
 struct PointOutB {
     double rhog, rhoge, rhogq[6], vt[3];
 };
 
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
-                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st)
+                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st,
+                           double pm179)
 {
     // ---- forward preprocessing, reverse of the state update, the rate kernel
     double drq_b[NQ], rq_b[NQ], rhoge0_b, rhog_b;
@@ -1496,7 +1504,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         st[RATE_STASH + 2 * NQ + 1] = rhog_b;
         stash_fence();
         double qt[NQ];
-        rates_b(P, dt, in, qt_b, yb.vt, ib, qt, st);
+        rates_b(P, dt, in, pm179, qt_b, yb.vt, ib, qt, st);
         stash_fence();
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
@@ -1510,6 +1518,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         for (int n = 0; n < NQ; n++) rhog_b += drq_b[n] * qt[n] * dt;
 #pragma unroll
         for (int i = 0; i < NCOEF; i++) tun_b[i] += ib.tun[i];
+        xb.nc_pm179 += ib.nc_pm179;
     }
     // ---- the preprocessing again, from inputs the compiler cannot see through
     const PointIn<double> xo = opaque_point(x);
@@ -1591,7 +1600,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
 {
     double buf[POINT_STASH];
-    point_rates_b(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1});
+    point_rates_b(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, P.Nc_pow_m179);
 }
 
 }  // namespace nsw6

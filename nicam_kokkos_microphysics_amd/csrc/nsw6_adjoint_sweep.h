@@ -275,9 +275,10 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 // added to tun_b.
 NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2, double g_rhog, double g_rhoge,
                               const double g_rhogq[NQ], double g_tem, double g_qv, double gvx, double gvy, double gvz,
-                              double gw, double gux, double guy, double guz)
+                              double gw, double gux, double guy, double guz, double g_unccn)
 {
     if (G.rhog) G.rhog[p] = g_rhog;
+    if (G.unccn) G.unccn[p] = g_unccn;
     if (G.rhoge) G.rhoge[p] = g_rhoge;
     if (G.tem) G.tem[p] = g_tem;
     if (G.rhogvx) G.rhogvx[p] = gvx;
@@ -295,6 +296,8 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
 }
 
 // adj_point of a point of a single-cell region; st holds the rate reverse's stash.
+// INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).
+template <bool INDIR>
 NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
@@ -338,7 +341,7 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
     const double guy = WRV(WR_C0 + 12, k) + WRV(WR_C1 + 12, k - 1);
     const double guz = WRV(WR_C0 + 13, k) + WRV(WR_C1 + 13, k - 1);
 
-    double g_rhog, g_rhoge, g_rhogq[NQ], g_tem = 0.0, g_qv = 0.0;
+    double g_rhog, g_rhoge, g_rhogq[NQ], g_tem = 0.0, g_qv = 0.0, g_unccn = 0.0;
     if (k >= KMIN && k <= KMAX) {
         const DevParams &P = *a.P;
         PointIn<double> x;
@@ -358,10 +361,19 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
 #pragma unroll
         for (int t = 0; t < 3; t++) yb.vt[t] = vt_b[t];
         PointInB xb;
-        xb.rhog = xb.tem = xb.qv = 0.0;
+        xb.rhog = xb.tem = xb.qv = xb.nc_pm179 = 0.0;
 #pragma unroll
         for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
-        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b, st);
+        // OPT_INDIR: Nc = max(unccn 1e-6, Nc_def) of the point (src/mod_mp_nsw6.cpp:475-484)
+        double pm179 = P.Nc_pow_m179, dpm179_dunccn = 0.0;
+        if (INDIR) {
+            const double un = f.unccn[p];
+            const NcPoint nc = nc_indirect(P, un);
+            pm179 = nc.pow_m179;
+            if (!(un * 1.0E-6 < P.Nc_def)) dpm179_dunccn = -1.79 * nc.pow_m179 / nc.Nc * 1.0E-6;
+        }
+        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b, st, pm179);
+        if (INDIR) g_unccn = xb.nc_pm179 * dpm179_dunccn;
         g_rhog = xb.rhog;
         g_rhoge = 0.0;   // the interior rhoge is recomputed from tem
 #pragma unroll
@@ -375,30 +387,35 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
 #pragma unroll
         for (int n = 0; n < NQ; n++) g_rhogq[n] = rq_b[n];
     }
-    store_point_grad(A.grad, p, pq, K2, g_rhog, g_rhoge, g_rhogq, g_tem, g_qv, gvx, gvy, gvz, gw, gux, guy, guz);
+    store_point_grad(A.grad, p, pq, K2, g_rhog, g_rhoge, g_rhogq, g_tem, g_qv, gvx, gvy, gvz, gw, gux, guy, guz,
+                     g_unccn);
 }
 
 // Regions that need the general sweep get no gradient (the caller is told).  The zero
 // gradients are stored on a path of their own, after the single-cell body rather than
 // ahead of it: a store ahead of the body's loads would send its uniform reads (the
 // parameters, the grid) through the vector memory path instead of the scalar cache.
+template <bool INDIR>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
     if (a.ws.region_flag[l] == 0) {
-        adj_point_single(A, l, k, ij, tun_b, st);
+        adj_point_single<INDIR>(A, l, k, ij, tun_b, st);
         return;
     }
     const int64_t K2 = (int64_t)KALL * a.ijdim;
     const int64_t p = (int64_t)l * K2 + (int64_t)k * a.ijdim + ij;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * a.ijdim + ij;
     const double z[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
 }
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
 {
     double buf[POINT_STASH];
-    adj_point(A, l, k, ij, tun_b, RateStash{buf, 1});
+    if (A.a.opt_indir)
+        adj_point<true>(A, l, k, ij, tun_b, RateStash{buf, 1});
+    else
+        adj_point<false>(A, l, k, ij, tun_b, RateStash{buf, 1});
 }
 
 }  // namespace nsw6

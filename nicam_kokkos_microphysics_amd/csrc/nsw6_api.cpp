@@ -287,11 +287,11 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
     return NSW6_OK;
 }
 
-// The reverse mode covers the default scheme only (see include/nsw6.h).
+// The reverse mode covers the default scheme, with or without OPT_INDIR (see include/nsw6.h).
 static bool adjoint_supported(const DevParams &P)
 {
-    return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0 && P.opt_indir == 0.0 &&
-           P.kin2kin == 0.0 && P.four_water == 0.0;
+    return P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0 && P.kin2kin == 0.0 &&
+           P.four_water == 0.0;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)
@@ -416,7 +416,7 @@ static int ensure_div_adj(nsw6_ctx *ctx, size_t doubles, int lall)
 // by sub-step m, get none); the accumulated outputs (precip and its energy fluxes,
 // GPREC, the surface mass fluxes) are fraction_mp times the sum over the sub-steps, so
 // every sub-step receives fraction_mp times their cotangents; the cotangents of vx, vy,
-// vz and of the coefficients are summed over the sub-steps.  Events (NULL or 3): start,
+// vz, unccn and of the coefficients are summed over the sub-steps.  Events (NULL or 3): start,
 // after the forward rebuild, end.
 static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state, const nsw6_fields *cot,
                          const double *cot_sflux, const nsw6_fields *grad, double *grad_coef, double dt,
@@ -426,16 +426,16 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     const double fraction_mp = 1.0 / double(n);
     const double dt_mp = dt * fraction_mp;
     const size_t C = (size_t)lall * (size_t)ijdim, K2 = (size_t)KALL * C;
-    // layout: n-1 carries | 2 cotangent carries | vx, vy, vz | write-only rows (qd and
-    // the constant diagnostics) | scaled GPREC | scaled column cotangents | coefficients
+    // layout: n-1 carries | 2 cotangent carries | vx, vy, vz, unccn | write-only rows (qd
+    // and the constant diagnostics) | scaled GPREC | scaled column cotangents | coefficients
     const size_t n_carry = (size_t)(n - 1) * CARRY_K2 * K2, n_cot = 2 * (size_t)CARRY_K2 * K2;
-    const size_t doubles = n_carry + n_cot + 3 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF;
+    const size_t doubles = n_carry + n_cot + 4 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF;
     int st = ensure_div_adj(ctx, doubles, lall);
     if (st != NSW6_OK) return st;
     double *base = ctx->div_adj;
     double *cotbuf = base + n_carry;
     double *vtmp = cotbuf + n_cot;
-    double *junk = vtmp + 3 * K2;   // qd (K2) and re_solid / GDCLW / GDCFRC (K2, same thread, same index)
+    double *junk = vtmp + 4 * K2;   // qd (K2) and re_solid / GDCLW / GDCFRC (K2, same thread, same index)
     double *gprec_s = junk + 2 * K2;
     double *col_s = gprec_s + K2;
     double *coef_tmp = col_s + 9 * C;
@@ -541,6 +541,7 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
         g_m.vx = grad->vx ? (first ? grad->vx : vtmp) : nullptr;
         g_m.vy = grad->vy ? (first ? grad->vy : vtmp + K2) : nullptr;
         g_m.vz = grad->vz ? (first ? grad->vz : vtmp + 2 * K2) : nullptr;
+        g_m.unccn = grad->unccn ? (first ? grad->unccn : vtmp + 3 * K2) : nullptr;
         double *coef_m = grad_coef ? (first ? grad_coef : coef_tmp) : nullptr;
         st = adj_step(ctx, lall, ijdim, &st_m, &c_m, sflux_s, &g_m, coef_m, dt_mp, 1, s, nullptr);
         if (st != NSW6_OK) return st;
@@ -548,6 +549,8 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
             if (grad->vx && !ok(nsw6_launch_accumulate(grad->vx, vtmp, (int64_t)K2, s))) return NSW6_HIP_ERROR;
             if (grad->vy && !ok(nsw6_launch_accumulate(grad->vy, vtmp + K2, (int64_t)K2, s))) return NSW6_HIP_ERROR;
             if (grad->vz && !ok(nsw6_launch_accumulate(grad->vz, vtmp + 2 * K2, (int64_t)K2, s)))
+                return NSW6_HIP_ERROR;
+            if (grad->unccn && !ok(nsw6_launch_accumulate(grad->unccn, vtmp + 3 * K2, (int64_t)K2, s)))
                 return NSW6_HIP_ERROR;
             if (grad_coef && !ok(nsw6_launch_accumulate(grad_coef, coef_tmp, NSW6_NCOEF, s))) return NSW6_HIP_ERROR;
         }
@@ -577,6 +580,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
                            fl->gam2h, fl->ix, fl->iy, fl->iz, fl->jx, fl->jy, fl->jz, fl->z};
     for (const double *p : req)
         if (!p) return NSW6_BAD_ARG;
+    if (ctx->P.opt_indir != 0.0 && !fl->unccn) return NSW6_BAD_ARG;   // read under OPT_INDIR
     if (!adjoint_supported(ctx->P)) return NSW6_UNSUPPORTED;
     int st = use_device(ctx);
     if (st == NSW6_OK) st = ensure_regions(ctx, lall);

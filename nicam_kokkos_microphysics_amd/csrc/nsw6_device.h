@@ -129,10 +129,14 @@ NSW6_HD NcPoint nc_default(const DevParams &P)
     n.Dc = P.Dc;
     return n;
 }
+// Where the maximum is Nc_def the host-formed constants are taken, so that such points
+// step (and reverse) exactly as without OPT_INDIR.
 NSW6_HD NcPoint nc_indirect(const DevParams &P, double unccn)
 {
+    const double nc = unccn * 1.0E-6;
+    if (nc < P.Nc_def) return nc_default(P);
     NcPoint n;
-    n.Nc = dmax(unccn * 1.0E-6, P.Nc_def);
+    n.Nc = nc;
     n.pow_m179 = pow(n.Nc, -1.79);
     n.Dc = 0.146 - 5.964E-2 * log(n.Nc / 2000.0);
     return n;

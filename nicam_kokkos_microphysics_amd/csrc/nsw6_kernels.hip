@@ -1027,7 +1027,10 @@ extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStrea
     const dim3 grid_p((a.ijdim + 255) / 256, KALL, a.lall);
     const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
-    hipLaunchKernelGGL((k_rates<double, false>), grid_p, dim3(256), 0, stream, a);
+    if (a.opt_indir)
+        hipLaunchKernelGGL((k_rates<double, false, true>), grid_p, dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL((k_rates<double, false>), grid_p, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(k_extremes<3>, grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds<3>, dim3(6, a.lall), dim3(192), 0, stream, a);
     hipLaunchKernelGGL(k_flux<3>, grid_l, dim3(256), 0, stream, a);

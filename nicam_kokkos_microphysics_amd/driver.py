@@ -127,7 +127,8 @@ class NSW6:
         input (device tensors, not modified), `cot` the cotangents of its outputs (keys
         of fields.ADJ_OUTPUTS; missing = 0), `cot_sflux` [l][3][ij] those of the
         surface mass fluxes of QR, QS, QG.  Returns (grad, grad_coef): the cotangents of
-        the inputs (keys of fields.ADJ_INPUTS) and of the 8 tunable coefficients."""
+        the inputs (keys of fields.ADJ_INPUTS, and unccn when the state holds it: nonzero
+        under OPT_INDIR only) and of the 8 tunable coefficients."""
         import torch
         rhog = state["rhog"]
         lall, ijdim = int(rhog.shape[0]), int(rhog.shape[-1])
@@ -139,6 +140,8 @@ class NSW6:
             raise ValueError(f"cot_sflux: shape {tuple(cot_sflux.shape)} != {(lall, 3, ijdim)}")
         if grad is None:
             grad = {k: torch.empty_like(state[k]) for k in ADJ_INPUTS}
+            if state.get("unccn") is not None:
+                grad["unccn"] = torch.empty_like(state["unccn"])
         if grad_coef is None:
             grad_coef = torch.empty(N.NCOEF, dtype=torch.float64, device=rhog.device)
         fs, fc, fg = field_ptrs(state), field_ptrs(cot), field_ptrs(grad)

@@ -7,7 +7,13 @@ It enters KK2000 (Nc^-1.79), Berry (Nc, Dc(Nc)) and the effective radius.
   Nc_def is that value (the init-time effective-radius constants kept); a missing
   unccn is an argument error.
 * GPU: a point-wise varying unccn matches the oracle within 1e-10 of max-abs, under
-  KK2000 and under Berry; the reverse mode reports the scheme as unsupported.
+  KK2000 and under Berry.
+* GPU reverse mode (KK2000): with unccn * 1e-6 below Nc_def everywhere the gradient is
+  the default scheme's, bit for bit, and the unccn cotangent is 0; with a point-wise
+  varying unccn the gradient (unccn included) matches central differences of the GPU
+  step (h = 1e-8 and 2e-8 relative; directions whose two quotients differ by more
+  than 1e-4 cross a switch and are skipped, at least two of four are checked) to 1e-5
+  relative; Berry stays unsupported.
 """
 import numpy as np
 import pytest
@@ -110,9 +116,92 @@ def test_gpu_aerosol_indirect_matches_oracle(kk):
     drv.step(base, DT)
     assert not torch.equal(base["re_liquid"], one["re_liquid"])
     drv.set_params(P_gpu)
-    # the reverse mode covers the default scheme only
+    # the reverse mode covers OPT_INDIR under KK2000 only
     from nicam_kokkos_microphysics_amd.driver import NSW6Error
-    with pytest.raises(NSW6Error) as e:
-        drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
-    assert e.value.code == N.NSW6_UNSUPPORTED
+    if kk == 0.0:
+        with pytest.raises(NSW6Error) as e:
+            drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
+        assert e.value.code == N.NSW6_UNSUPPORTED
+    drv.close()
+
+
+@pytest.mark.gpu
+def test_gpu_indirect_reverse_with_small_unccn_is_the_default_reverse():
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state(lall=2, ijdim=256)
+    st["unccn"][...] = 10.0e6          # 10 cm-3 < Nc_def = 50 cm-3
+    dev = F.to_device(st)
+    rng = np.random.default_rng(3)
+    cot = {"precip": torch.from_numpy(rng.uniform(-1, 1, st["precip"].shape)).cuda(),
+           "rhoge": torch.from_numpy(rng.uniform(-1, 1, st["rhoge"].shape)).cuda(),
+           "rhogq": torch.from_numpy(rng.uniform(-1, 1, st["rhogq"].shape) * 1e3).cuda()}
+    base = NSW6(gz, gzh, max_columns=512)
+    ind = NSW6(gz, gzh, params=_with(default_params(), opt_indir=1.0), max_columns=512)
+    g0, c0 = base.adjoint(dev, cot, DT)
+    g1, c1 = ind.adjoint(dev, cot, DT)
+    torch.cuda.synchronize()
+    for k in F.ADJ_INPUTS:
+        assert torch.equal(g0[k], g1[k]), k
+    assert torch.equal(c0, c1)
+    assert float(g1["unccn"].abs().max()) == 0.0
+    base.close()
+    ind.close()
+
+
+def _loss(drv, st_dev):
+    import torch
+    s = F.copy_state(st_dev)
+    rhoge_in = s["rhoge"].clone()
+    drv.step(s, DT)
+    torch.cuda.synchronize()
+    p = s["precip"]
+    return float((p * p).sum() + ((s["rhoge"] - rhoge_in) / DT).sum()), s
+
+
+@pytest.mark.gpu
+def test_gpu_indirect_reverse_matches_central_differences():
+    import torch
+    from adjoint_helpers import coefs, directional, random_directions
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state(lall=1, ijdim=256, seed=23)
+    rng = np.random.default_rng(6)
+    st["unccn"][...] = rng.uniform(0.0, 400.0e6, st["unccn"].shape)
+    P0 = _with(default_params(), opt_indir=1.0)
+    drv = NSW6(gz, gzh, params=P0, max_columns=256)
+    dev = F.to_device(st)
+    _, out = _loss(drv, dev)
+    cot = {"precip": 2.0 * out["precip"], "rhoge": torch.full_like(out["rhoge"], 1.0 / DT)}
+    grad, gc = drv.adjoint(dev, cot, DT)
+    grad["rhoge"] = grad["rhoge"] - 1.0 / DT   # explicit -rhoge_in/dt term of the loss
+    g = {k: v.cpu().numpy() for k, v in grad.items()}
+    assert np.abs(g["unccn"]).max() > 0.0
+    gcoef = gc.cpu().numpy()
+    dirs, dcoef = random_directions(st, 37, 4)
+    for i, d in enumerate(dirs):
+        d["unccn"] = st["unccn"] * rng.uniform(-1, 1, st["unccn"].shape)
+    c0 = coefs()
+
+    def fd(i, h):
+        vals = []
+        for sgn in (1.0, -1.0):
+            P = _with(default_params(), opt_indir=1.0)
+            for c, name in enumerate(N.TUNABLE_COEFFS):
+                setattr(P, name, c0[c] + sgn * h * dcoef[i][c])
+            drv.set_params(P)
+            sp = F.copy_state(st)
+            for k, d in dirs[i].items():
+                sp[k] = sp[k] + sgn * h * d
+            vals.append(_loss(drv, F.to_device(sp))[0])
+        return (vals[0] - vals[1]) / (2 * h)
+
+    checked = 0
+    for i in range(len(dirs)):
+        f1, f2 = fd(i, 1e-8), fd(i, 2e-8)
+        if abs(f1 - f2) > 1e-4 * abs(f1):
+            continue
+        s, _ = directional(g, gcoef, dirs[i], dcoef[i])
+        assert abs(s - f1) <= 1e-5 * abs(f1), (i, s, f1)
+        checked += 1
+    assert checked >= 2, checked
     drv.close()
