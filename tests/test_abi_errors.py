@@ -6,7 +6,7 @@ status has a message.  The reference stops the program instead (src/mod_debug.cp
   count limits, precision and sub-step ranges, snapshot arguments), and
   NSW6_NO_DEVICE where the machine has no GPU.
 * GPU: a step with too many columns, dt <= 0, a missing required array, or OPT_INDIR
-  without unccn is refused and leaves the state untouched.
+  without unccn (step or reverse mode) is refused and leaves the state untouched.
 """
 import ctypes
 
@@ -103,6 +103,9 @@ def test_gpu_refused_steps_leave_the_state_untouched():
     P.opt_indir = 1.0
     drv.set_params(P)
     refused(dict(one, unccn=None))                # OPT_INDIR reads unccn
+    with pytest.raises(NSW6Error) as e:           # so does its reverse mode
+        drv.adjoint(dict(one, unccn=None), {"precip": one["precip"]}, 15.0)
+    assert e.value.code == N.NSW6_BAD_ARG
     torch.cuda.synchronize()
     for k, v in before.items():
         assert torch.equal(dev[k], v), k
