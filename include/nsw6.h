@@ -207,7 +207,8 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
 
 /* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
  * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; with or without OPT_INDIR,
- * which then reads `state->unccn`): given the step's INPUT state `state`
+ * which then reads `state->unccn`; or Berry's autoconversion (sw_kk2000 = 0) without
+ * OPT_INDIR): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)
  * and cotangents of the step's outputs, write the cotangents of its inputs,
  *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.

@@ -798,7 +798,9 @@ NSW6_HD void stash_fence()
 
 // Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
 // pm179: Nc^-1.79 of the point (P.Nc_pow_m179 without OPT_INDIR); its cotangent is
-// added to b.nc_pm179.
+// added to b.nc_pm179.  KK_ONLY: sw_kk2000 = 1 known at compile time (otherwise read
+// from P at run time).
+template <bool KK_ONLY>
 NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, double pm179, const double qt_b[6],
                      const double vt_b[3], RateInB &b, double qt[6], const RateStash &st)
 {
@@ -1194,10 +1196,26 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
         tun_b[1] -= Psaut0_b * betai;
         if (biE) temc_b += Psaut0_b * (qi - qicrt) * P.beta_saut * ebi * P.gamma_saut;
     }
-    // Praut0 = kk qc^2.47 Nc^-1.79
-    tun_b[0] += Praut0_b * Eqc * nc_pm179;
-    b.nc_pm179 += Praut0_b * kk * Eqc;
-    if (qc > 0.0) q_b[QC] += Praut0_b * kk * nc_pm179 * 2.47 * Eqc / qc;
+    // Praut0 = swk Praut_kk + (1 - swk) Praut_be (Praut_kk alone under KK2000);
+    // Praut_kk = kk qc^2.47 Nc^-1.79
+    {
+        const double K_b = kk_only ? Praut0_b : Praut0_b * swk;
+        tun_b[0] += K_b * Eqc * nc_pm179;
+        b.nc_pm179 += K_b * kk * Eqc;
+        if (qc > 0.0) q_b[QC] += K_b * kk * nc_pm179 * 2.47 * Eqc / qc;
+    }
+    // Praut_be = Rdens 1.67e-5 rhoqc^2 / Bden, Bden = 5 + 3.66e-2 Nc_def / Dcq,
+    // Dcq = Dc rhoqc + EPS, rhoqc = 1000 dens qc
+    if (!kk_only) {
+        const double B_b = Praut0_b * (1.0 - swk);
+        const double num = Rdens * 1.67E-5 * rhoqc * rhoqc;
+        Rdens_b += B_b * 1.67E-5 * rhoqc * rhoqc / Bden;
+        double rhoqc_b = B_b * Rdens * 1.67E-5 * 2.0 * rhoqc / Bden;
+        const double Bden_b = -B_b * num / (Bden * Bden);
+        rhoqc_b -= Bden_b * 3.66E-2 * P.Nc_def / (Dcq * Dcq) * P.Dc;
+        dens_b += rhoqc_b * 1000.0 * qc;
+        q_b[QC] += rhoqc_b * 1000.0 * dens;
+    }
     // Pgacs0 = As kac Rdens Egs_mod N0g |Vtg - Vts| SGS
     {
         const double k0 = As * kac * N0g;
@@ -1310,11 +1328,24 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
         M2ds_b += Psacw0_b * qc * k0 * Cs * rho_fact;
         rf_b += Psacw0_b * qc * k0 * Cs * M2ds;
     }
-    // Pracw0 = 67 exp(1.15 log(qc qr + zcr)) (1 - zcr)
-    if (zcr == 0.0 && Xcr > 0.0) {
-        const double X_b = Pracw0_b * 67.0 * Ecr * 1.15 / Xcr;
-        q_b[QC] += X_b * qr;
-        q_b[QR] += X_b * qc;
+    // Pracw0 = swk Pracw_kk + (1 - swk) Pracw_or (Pracw_kk alone under KK2000);
+    // Pracw_kk = 67 exp(1.15 log(qc qr + zcr)) (1 - zcr)
+    {
+        const double K_b = kk_only ? Pracw0_b : Pracw0_b * swk;
+        if (zcr == 0.0 && Xcr > 0.0) {
+            const double X_b = K_b * 67.0 * Ecr * 1.15 / Xcr;
+            q_b[QC] += X_b * qr;
+            q_b[QR] += X_b * qc;
+        }
+    }
+    // Pracw_or = qc kac Erw N0r Cr GAM_3dr L3dr rho_fact
+    if (!kk_only) {
+        const double O_b = Pracw0_b * (1.0 - swk);
+        const double k0 = kac * P.Erw * N0r * P.GAM_3dr;
+        q_b[QC] += O_b * k0 * Cr * L3dr * rho_fact;
+        tun_b[4] += O_b * qc * k0 * L3dr * rho_fact;
+        L3dr_b += O_b * qc * k0 * Cr * rho_fact;
+        rf_b += O_b * qc * k0 * Cr * L3dr;
     }
     // Esi_mod = min(Esi, Esi exp(gamma_sacr temc)); Egs_mod likewise (Egs is not tunable)
     if (EsiE) {
@@ -1466,6 +1497,7 @@ struct PointOutB {
     double rhog, rhoge, rhogq[6], vt[3];
 };
 
+template <bool KK_ONLY>
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st,
                            double pm179)
@@ -1504,7 +1536,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         st[RATE_STASH + 2 * NQ + 1] = rhog_b;
         stash_fence();
         double qt[NQ];
-        rates_b(P, dt, in, pm179, qt_b, yb.vt, ib, qt, st);
+        rates_b<KK_ONLY>(P, dt, in, pm179, qt_b, yb.vt, ib, qt, st);
         stash_fence();
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
@@ -1600,7 +1632,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
 {
     double buf[POINT_STASH];
-    point_rates_b(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, P.Nc_pow_m179);
+    point_rates_b<false>(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, P.Nc_pow_m179);
 }
 
 }  // namespace nsw6

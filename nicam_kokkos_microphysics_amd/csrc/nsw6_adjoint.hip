@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // * the rate reverse's stash (POINT_STASH doubles per thread, column threadIdx.x), which
 //   takes the values that wait through the rate reverse and its phase-2 recompute out
 //   of the registers.
-template <bool INDIR>
+template <bool INDIR, bool KK_ONLY>
 __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
 {
     __shared__ double red[4][NCOEF];
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
     double tun_b[NCOEF];
 #pragma unroll
     for (int c = 0; c < NCOEF; c++) tun_b[c] = 0.0;
-    if (lb.valid && ij < ijdim) adj_point<INDIR>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
+    if (lb.valid && ij < ijdim) adj_point<INDIR, KK_ONLY>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
     // per-block sums of the coefficient cotangents (every thread reaches this point)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -162,10 +162,13 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
     hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
-    if (L.A.a.opt_indir)
-        hipLaunchKernelGGL(k_adj_point<true>, dim3(nb), dim3(256), 0, stream, L);
+    // the supported schemes: KK2000 with or without OPT_INDIR, Berry without it
+    if (!L.A.a.kk_only)
+        hipLaunchKernelGGL((k_adj_point<false, false>), dim3(nb), dim3(256), 0, stream, L);
+    else if (L.A.a.opt_indir)
+        hipLaunchKernelGGL((k_adj_point<true, true>), dim3(nb), dim3(256), 0, stream, L);
     else
-        hipLaunchKernelGGL(k_adj_point<false>, dim3(nb), dim3(256), 0, stream, L);
+        hipLaunchKernelGGL((k_adj_point<false, true>), dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_coef, dim3(NCOEF), dim3(256), 0, stream, L, nb);
     if (ev) (void)hipEventRecord(ev[2], stream);
     return hipGetLastError();

@@ -296,8 +296,9 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
 }
 
 // adj_point of a point of a single-cell region; st holds the rate reverse's stash.
-// INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).
-template <bool INDIR>
+// INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  KK_ONLY:
+// KK2000 alone (A.a.kk_only); otherwise sw_kk2000 is read at run time.
+template <bool INDIR, bool KK_ONLY>
 NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
@@ -372,7 +373,7 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
             pm179 = nc.pow_m179;
             if (!(un * 1.0E-6 < P.Nc_def)) dpm179_dunccn = -1.79 * nc.pow_m179 / nc.Nc * 1.0E-6;
         }
-        point_rates_b(P, a.dt, x, tun, yb, xb, tun_b, st, pm179);
+        point_rates_b<KK_ONLY>(P, a.dt, x, tun, yb, xb, tun_b, st, pm179);
         if (INDIR) g_unccn = xb.nc_pm179 * dpm179_dunccn;
         g_rhog = xb.rhog;
         g_rhoge = 0.0;   // the interior rhoge is recomputed from tem
@@ -395,12 +396,12 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
 // gradients are stored on a path of their own, after the single-cell body rather than
 // ahead of it: a store ahead of the body's loads would send its uniform reads (the
 // parameters, the grid) through the vector memory path instead of the scalar cache.
-template <bool INDIR>
+template <bool INDIR, bool KK_ONLY>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
     if (a.ws.region_flag[l] == 0) {
-        adj_point_single<INDIR>(A, l, k, ij, tun_b, st);
+        adj_point_single<INDIR, KK_ONLY>(A, l, k, ij, tun_b, st);
         return;
     }
     const int64_t K2 = (int64_t)KALL * a.ijdim;
@@ -413,9 +414,9 @@ NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOE
 {
     double buf[POINT_STASH];
     if (A.a.opt_indir)
-        adj_point<true>(A, l, k, ij, tun_b, RateStash{buf, 1});
+        adj_point<true, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
     else
-        adj_point<false>(A, l, k, ij, tun_b, RateStash{buf, 1});
+        adj_point<false, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
 }
 
 }  // namespace nsw6

@@ -127,6 +127,7 @@ struct StepArgs {
     int opt_indir;                // 1: aerosol indirect effect (k_rates<R, DIAG, true>)
     int kin2kin;                  // 1: KIN2KIN energy correction of the sedimentation
     int four_water;               // 1: "4WATER", QI sediments (transport kernels with NT = 4)
+    int kk_only;                  // 1: sw_kk2000 = 1, KK2000 alone (the reverse point pass's kernel)
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 

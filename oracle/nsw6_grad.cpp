@@ -1044,14 +1044,18 @@ extern "C" {
 
 // Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
 // NSW6_OK, NSW6_UNSUPPORTED if a region needs the general transport (its gradient is
-// 0) or for a non-default scheme.
+// 0) or for a scheme these passes do not cover: anything but Roh & Satoh snow,
+// sw_expice = 1, KIN2EIN, 3WATER with KK2000 or Berry's autoconversion, no OPT_INDIR
+// (the device reverse mode also covers OPT_INDIR).
 int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                       const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
                       const nsw6_fields *grad, double *grad_coef, double dt)
 {
     DevParams P;
     to_dev_params(*params, P);
-    if (!(P.sw_roh2014 == 1.0 && P.sw_kk2000 == 1.0 && P.sw_expice == 1.0)) return NSW6_UNSUPPORTED;
+    if (!(P.sw_roh2014 == 1.0 && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) && P.sw_expice == 1.0 &&
+          P.opt_indir == 0.0 && P.kin2kin == 0.0 && P.four_water == 0.0))
+        return NSW6_UNSUPPORTED;
     GridDev g;
     make_grid_dev(gz, gzh, g);
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)lall * K2, C = (int64_t)lall * ijdim;

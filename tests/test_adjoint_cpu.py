@@ -92,6 +92,35 @@ def test_adjoint_pieces_match_dual_numbers(case):
     assert np.all(err <= 1e-6), dict(zip(["point", "wall", "sediment", "tail"], err))
 
 
+@pytest.mark.parametrize("case", ["all_species", "evolved"])
+def test_berry_reverse_matches_dual_numbers(case):
+    """sw_kk2000 = 0 (Berry's autoconversion and the original collection of cloud water
+    by rain in place of KK2000, src/mod_mp_nsw6.cpp:1090-1100, 1140-1160): every reverse
+    piece against dual numbers at every point, and the whole step's reverse against
+    the dual-number directional derivative."""
+    P = O.params()
+    P.sw_kk2000 = 0.0
+    if case == "all_species":
+        gz, gzh, st = F.synth_state(2, 40, seed=21)
+        st = _everywhere(st, 1)
+    else:
+        gz, gzh, st = F.synth_state(2, 40, seed=22, frac_towers=0.5)
+        st = _everywhere(st, 2)
+        for _ in range(3):
+            assert O.mp_driver(st, gz, gzh, DT, P=P).status == 0
+    err, cnt = O.check_pieces(st, gz, gzh, DT, P=P)
+    assert np.all(cnt > 0)
+    assert np.all(err <= 1e-6), dict(zip(["point", "wall", "sediment", "tail"], err))
+    cot, cs = random_cotangents(st, gz, gzh, 7)
+    grad, gc, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=P)
+    assert status == 0
+    dirs, dcoef = random_directions(st, 13, 4)
+    _, dj = O.dual_vjp(st, gz, gzh, DT, cot, cs, dirs, dcoef, P=P)
+    for i in range(len(dirs)):
+        s_, a = directional(grad, gc, dirs[i], dcoef[i])
+        assert abs(s_ - dj[i]) <= 1e-13 * a, (i, s_, dj[i], a)
+
+
 def _state(case):
     if case == "default":
         return F.synth_state(2, 40, seed=11)

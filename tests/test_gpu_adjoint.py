@@ -2,7 +2,8 @@
 
 * Elementwise agreement with the host run of the same reverse passes
   (oracle.adjoint_host): every input cotangent within 1e-9 of its field's max-abs,
-  the coefficient cotangents within 1e-9 relative.
+  the coefficient cotangents within 1e-9 relative; also with Berry's autoconversion
+  (sw_kk2000 = 0).
 * Directional agreement with forward-mode dual numbers on the host (oracle.dual_vjp):
   |<grad, v> - dJ/dv| <= 1e-6 |dJ/dv| along random directions through every input
   field and the 8 coefficients.
@@ -67,6 +68,35 @@ def test_gpu_reverse_matches_host_passes(case):
         err = np.abs(g_gpu[k] - g_host[k]).max()
         assert err <= 1e-9 * scale + 1e-300, (k, err, scale)
     np.testing.assert_allclose(gc_gpu, gc_host, rtol=1e-9, atol=0)
+
+
+def test_gpu_berry_reverse_matches_host_passes():
+    """sw_kk2000 = 0 (Berry's autoconversion, the original rain accretion of cloud
+    water): the GPU reverse mode against the host passes (themselves checked against
+    dual numbers), elementwise within 1e-9 of max-abs."""
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    gz, gzh, st = _state("supersaturated")
+    Ph = O.params()
+    Ph.sw_kk2000 = 0.0
+    cot, cs = random_cotangents(st, gz, gzh, 7)
+    g_host, gc_host, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=Ph)
+    assert status == 0
+    Pg = default_params()
+    Pg.sw_kk2000 = 0.0
+    lall, ijdim = st["rhog"].shape[0], st["rhog"].shape[-1]
+    drv = NSW6(gz, gzh, params=Pg, max_columns=lall * ijdim)
+    dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+    grad, gc = drv.adjoint(F.to_device(st), dcot, DT, cot_sflux=torch.from_numpy(np.ascontiguousarray(cs)).cuda())
+    torch.cuda.synchronize()
+    for k in F.ADJ_INPUTS:
+        g = grad[k].cpu().numpy()
+        scale = np.abs(g_host[k]).max()
+        err = np.abs(g - g_host[k]).max()
+        assert err <= 1e-9 * scale + 1e-300, (k, err, scale)
+    np.testing.assert_allclose(gc.cpu().numpy(), gc_host, rtol=1e-9, atol=0)
+    drv.close()
 
 
 @pytest.mark.parametrize("case", ["default", "supersaturated"])
