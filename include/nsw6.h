@@ -206,9 +206,9 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
     double *GPREC, double *CBMFX, void *stream);
 
 /* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
- * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; with or without OPT_INDIR,
- * which then reads `state->unccn`; or Berry's autoconversion (sw_kk2000 = 0) without
- * OPT_INDIR): given the step's INPUT state `state`
+ * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; or the same with Berry's
+ * autoconversion, sw_kk2000 = 0; each with or without OPT_INDIR, which then reads
+ * `state->unccn`): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)
  * and cotangents of the step's outputs, write the cotangents of its inputs,
  *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.
@@ -220,7 +220,8 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
  * Input cotangents `grad` (NULL members are not written): rhog, rhogvx, rhogvy,
  *   rhogvz, rhogw, rhoge (nonzero on the two halo levels only: the interior internal
  *   energy is recomputed from tem), rhogq, vx, vy, vz, tem, q (QV only; other
- *   species get 0), unccn (through Nc^-1.79 of KK2000 under OPT_INDIR; 0 otherwise).  `grad_coef` (device, NSW6_NCOEF doubles, may be NULL): the
+ *   species get 0), unccn (under OPT_INDIR through the point's Nc: Nc^-1.79 of KK2000, Nc and
+ *   Dc(Nc) of Berry; 0 otherwise).  `grad_coef` (device, NSW6_NCOEF doubles, may be NULL): the
  *   cotangents of the tunable coefficients in NSW6_NCOEF order.
  * Discrete choices (switches, limiters, kcell bounds, departure cells) are taken at
  * the input state, and the saturation adjustment is differentiated at its converged

@@ -707,6 +707,7 @@ namespace nsw6 {
 struct RateInB {
     double dens, temp, q[6], qsatl, qsati, pre, temc, sice_m1, sliq_1m, qv_qsati, tun[NCOEF];
     double nc_pm179;   // Nc^-1.79 of the point (per point under OPT_INDIR)
+    double nc_n, nc_dc;   // Nc and Dc of the point (Berry's autoconversion)
 };
 
 // rates that enter the limiter blocks and the tendencies
@@ -797,11 +798,12 @@ NSW6_HD void stash_fence()
 }
 
 // Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
-// pm179: Nc^-1.79 of the point (P.Nc_pow_m179 without OPT_INDIR); its cotangent is
-// added to b.nc_pm179.  KK_ONLY: sw_kk2000 = 1 known at compile time (otherwise read
+// ncp: the point's droplet number Nc, Nc^-1.79 and Dc under OPT_INDIR; nullptr reads
+// them from P where they are used (so they are not held in registers through the
+// pass); their cotangents are added to b.nc_n, b.nc_pm179 and b.nc_dc.  KK_ONLY: sw_kk2000 = 1 known at compile time (otherwise read
 // from P at run time).
 template <bool KK_ONLY>
-NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, double pm179, const double qt_b[6],
+NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const NcPoint *ncp, const double qt_b[6],
                      const double vt_b[3], RateInB &b, double qt[6], const RateStash &st)
 {
     double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
@@ -810,7 +812,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
     // ---- phase 1: forward, limiter decisions, reverse of the limiter algebra -> raw_b
     {
         const RateIn<double> &in = in0;
-        const double nc_pm179 = pm179;
+        const double nc_pm179 = ncp ? ncp->pow_m179 : P.Nc_pow_m179;
 #include "nsw6_rates_fwd.inc"
         // limiters: R = min(P0, dq_dt) * c; lim = true when dq_dt is taken
         double R[NRATE];
@@ -1011,7 +1013,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
     // through (so it recomputes them rather than keeping phase 1 values live), then
     // the reverse of the rate formulas
     const RateIn<double> in = opaque_copy(in0);
-    const double nc_pm179 = opaque(pm179);
+    const double nc_pm179 = opaque(ncp ? ncp->pow_m179 : P.Nc_pow_m179);
 #include "nsw6_rates_fwd.inc"
     stash_fence();
 #pragma unroll
@@ -1204,15 +1206,19 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
         b.nc_pm179 += K_b * kk * Eqc;
         if (qc > 0.0) q_b[QC] += K_b * kk * nc_pm179 * 2.47 * Eqc / qc;
     }
-    // Praut_be = Rdens 1.67e-5 rhoqc^2 / Bden, Bden = 5 + 3.66e-2 Nc_def / Dcq,
+    // Praut_be = Rdens 1.67e-5 rhoqc^2 / Bden, Bden = 5 + 3.66e-2 Nc / Dcq,
     // Dcq = Dc rhoqc + EPS, rhoqc = 1000 dens qc
     if (!kk_only) {
+        const double nc_Nc = ncp ? ncp->Nc : P.Nc_def, nc_Dc = ncp ? ncp->Dc : P.Dc;
         const double B_b = Praut0_b * (1.0 - swk);
         const double num = Rdens * 1.67E-5 * rhoqc * rhoqc;
         Rdens_b += B_b * 1.67E-5 * rhoqc * rhoqc / Bden;
         double rhoqc_b = B_b * Rdens * 1.67E-5 * 2.0 * rhoqc / Bden;
         const double Bden_b = -B_b * num / (Bden * Bden);
-        rhoqc_b -= Bden_b * 3.66E-2 * P.Nc_def / (Dcq * Dcq) * P.Dc;
+        const double Dcq_b = -(Bden_b * 3.66E-2 * nc_Nc / (Dcq * Dcq));
+        rhoqc_b += Dcq_b * nc_Dc;
+        b.nc_n += Bden_b * 3.66E-2 / Dcq;
+        b.nc_dc += Dcq_b * rhoqc;
         dens_b += rhoqc_b * 1000.0 * qc;
         q_b[QC] += rhoqc_b * 1000.0 * dens;
     }
@@ -1491,7 +1497,8 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, d
 // ---------------------------------------------------------------------------------
 struct PointInB {
     double rhog, rhogq[6], tem, qv;
-    double nc_pm179;   // cotangent of the point's Nc^-1.79
+    double nc_pm179;      // cotangent of the point's Nc^-1.79
+    double nc_n, nc_dc;   // cotangents of the point's Nc and Dc
 };
 struct PointOutB {
     double rhog, rhoge, rhogq[6], vt[3];
@@ -1500,7 +1507,7 @@ struct PointOutB {
 template <bool KK_ONLY>
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st,
-                           double pm179)
+                           const NcPoint *ncp)
 {
     // ---- forward preprocessing, reverse of the state update, the rate kernel
     double drq_b[NQ], rq_b[NQ], rhoge0_b, rhog_b;
@@ -1536,7 +1543,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         st[RATE_STASH + 2 * NQ + 1] = rhog_b;
         stash_fence();
         double qt[NQ];
-        rates_b<KK_ONLY>(P, dt, in, pm179, qt_b, yb.vt, ib, qt, st);
+        rates_b<KK_ONLY>(P, dt, in, ncp, qt_b, yb.vt, ib, qt, st);
         stash_fence();
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
@@ -1551,6 +1558,8 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
 #pragma unroll
         for (int i = 0; i < NCOEF; i++) tun_b[i] += ib.tun[i];
         xb.nc_pm179 += ib.nc_pm179;
+        xb.nc_n += ib.nc_n;
+        xb.nc_dc += ib.nc_dc;
     }
     // ---- the preprocessing again, from inputs the compiler cannot see through
     const PointIn<double> xo = opaque_point(x);
@@ -1632,7 +1641,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
 {
     double buf[POINT_STASH];
-    point_rates_b<false>(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, P.Nc_pow_m179);
+    point_rates_b<false>(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, nullptr);
 }
 
 }  // namespace nsw6

@@ -162,8 +162,10 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
     hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
-    // the supported schemes: KK2000 with or without OPT_INDIR, Berry without it
-    if (!L.A.a.kk_only)
+    // the supported schemes: KK2000 or Berry, each with or without OPT_INDIR
+    if (!L.A.a.kk_only && L.A.a.opt_indir)
+        hipLaunchKernelGGL((k_adj_point<true, false>), dim3(nb), dim3(256), 0, stream, L);
+    else if (!L.A.a.kk_only)
         hipLaunchKernelGGL((k_adj_point<false, false>), dim3(nb), dim3(256), 0, stream, L);
     else if (L.A.a.opt_indir)
         hipLaunchKernelGGL((k_adj_point<true, true>), dim3(nb), dim3(256), 0, stream, L);

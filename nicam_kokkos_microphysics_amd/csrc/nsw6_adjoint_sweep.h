@@ -362,19 +362,24 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
 #pragma unroll
         for (int t = 0; t < 3; t++) yb.vt[t] = vt_b[t];
         PointInB xb;
-        xb.rhog = xb.tem = xb.qv = xb.nc_pm179 = 0.0;
+        xb.rhog = xb.tem = xb.qv = xb.nc_pm179 = xb.nc_n = xb.nc_dc = 0.0;
 #pragma unroll
         for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
-        // OPT_INDIR: Nc = max(unccn 1e-6, Nc_def) of the point (src/mod_mp_nsw6.cpp:475-484)
-        double pm179 = P.Nc_pow_m179, dpm179_dunccn = 0.0;
+        // OPT_INDIR: Nc = max(unccn 1e-6, Nc_def) of the point (src/mod_mp_nsw6.cpp:475-484),
+        // Nc^-1.79 and Dc = 0.146 - 5.964e-2 log(Nc / 2000) from it
+        NcPoint nc = nc_default(P);
+        double dnc_dunccn = 0.0;
         if (INDIR) {
             const double un = f.unccn[p];
-            const NcPoint nc = nc_indirect(P, un);
-            pm179 = nc.pow_m179;
-            if (!(un * 1.0E-6 < P.Nc_def)) dpm179_dunccn = -1.79 * nc.pow_m179 / nc.Nc * 1.0E-6;
+            nc = nc_indirect(P, un);
+            if (!(un * 1.0E-6 < P.Nc_def)) dnc_dunccn = 1.0E-6;
         }
-        point_rates_b<KK_ONLY>(P, a.dt, x, tun, yb, xb, tun_b, st, pm179);
-        if (INDIR) g_unccn = xb.nc_pm179 * dpm179_dunccn;
+        point_rates_b<KK_ONLY>(P, a.dt, x, tun, yb, xb, tun_b, st, INDIR ? &nc : nullptr);
+        if (INDIR) {
+            double nc_b = xb.nc_pm179 * (-1.79 * nc.pow_m179 / nc.Nc);
+            if (!KK_ONLY) nc_b += xb.nc_n + xb.nc_dc * (-5.964E-2 / nc.Nc);
+            g_unccn = nc_b * dnc_dunccn;
+        }
         g_rhog = xb.rhog;
         g_rhoge = 0.0;   // the interior rhoge is recomputed from tem
 #pragma unroll

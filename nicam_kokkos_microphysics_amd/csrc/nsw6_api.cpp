@@ -288,11 +288,11 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
     return NSW6_OK;
 }
 
-// The reverse mode covers the default scheme, with or without OPT_INDIR, and Berry's
-// autoconversion in place of KK2000 without OPT_INDIR (see include/nsw6.h).
+// The reverse mode covers the default scheme and Berry's autoconversion in place of
+// KK2000, each with or without OPT_INDIR (see include/nsw6.h).
 static bool adjoint_supported(const DevParams &P)
 {
-    const bool kk = (P.sw_kk2000 == 1.0), berry = (P.sw_kk2000 == 0.0 && P.opt_indir == 0.0);
+    const bool kk = (P.sw_kk2000 == 1.0), berry = (P.sw_kk2000 == 0.0);
     return P.sw_roh2014 == 1.0 && (kk || berry) && P.sw_expice == 1.0 && P.kin2kin == 0.0 &&
            P.four_water == 0.0;
 }

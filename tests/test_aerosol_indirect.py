@@ -8,12 +8,13 @@ It enters KK2000 (Nc^-1.79), Berry (Nc, Dc(Nc)) and the effective radius.
   unccn is an argument error.
 * GPU: a point-wise varying unccn matches the oracle within 1e-10 of max-abs, under
   KK2000 and under Berry.
-* GPU reverse mode (KK2000): with unccn * 1e-6 below Nc_def everywhere the gradient is
-  the default scheme's, bit for bit, and the unccn cotangent is 0; with a point-wise
-  varying unccn the gradient (unccn included) matches central differences of the GPU
-  step (h = 1e-8 and 2e-8 relative; directions whose two quotients differ by more
-  than 1e-4 cross a switch and are skipped, at least two of four are checked) to 1e-5
-  relative; Berry stays unsupported.
+* GPU reverse mode, under KK2000 and under Berry: with unccn * 1e-6 below Nc_def
+  everywhere the gradient is that of the same scheme without OPT_INDIR, bit for bit,
+  and the unccn cotangent is 0; with a point-wise varying unccn the gradient (unccn
+  included: through Nc^-1.79 under KK2000, through Nc and Dc(Nc) under Berry) matches
+  central differences of the GPU step (h = 1e-8 and 2e-8 relative; directions whose two
+  quotients differ by more than 1e-4 cross a switch and are skipped, at least two of
+  four are checked) to 1e-5 relative.
 """
 import numpy as np
 import pytest
@@ -115,18 +116,12 @@ def test_gpu_aerosol_indirect_matches_oracle(kk):
     drv.set_params(_with(default_params(), sw_kk2000=kk))
     drv.step(base, DT)
     assert not torch.equal(base["re_liquid"], one["re_liquid"])
-    drv.set_params(P_gpu)
-    # the reverse mode covers OPT_INDIR under KK2000 only
-    from nicam_kokkos_microphysics_amd.driver import NSW6Error
-    if kk == 0.0:
-        with pytest.raises(NSW6Error) as e:
-            drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
-        assert e.value.code == N.NSW6_UNSUPPORTED
     drv.close()
 
 
 @pytest.mark.gpu
-def test_gpu_indirect_reverse_with_small_unccn_is_the_default_reverse():
+@pytest.mark.parametrize("kk", [1.0, 0.0])
+def test_gpu_indirect_reverse_with_small_unccn_is_the_default_reverse(kk):
     import torch
     from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
     gz, gzh, st = _state(lall=2, ijdim=256)
@@ -136,8 +131,8 @@ def test_gpu_indirect_reverse_with_small_unccn_is_the_default_reverse():
     cot = {"precip": torch.from_numpy(rng.uniform(-1, 1, st["precip"].shape)).cuda(),
            "rhoge": torch.from_numpy(rng.uniform(-1, 1, st["rhoge"].shape)).cuda(),
            "rhogq": torch.from_numpy(rng.uniform(-1, 1, st["rhogq"].shape) * 1e3).cuda()}
-    base = NSW6(gz, gzh, max_columns=512)
-    ind = NSW6(gz, gzh, params=_with(default_params(), opt_indir=1.0), max_columns=512)
+    base = NSW6(gz, gzh, params=_with(default_params(), sw_kk2000=kk), max_columns=512)
+    ind = NSW6(gz, gzh, params=_with(default_params(), opt_indir=1.0, sw_kk2000=kk), max_columns=512)
     g0, c0 = base.adjoint(dev, cot, DT)
     g1, c1 = ind.adjoint(dev, cot, DT)
     torch.cuda.synchronize()
@@ -160,14 +155,15 @@ def _loss(drv, st_dev):
 
 
 @pytest.mark.gpu
-def test_gpu_indirect_reverse_matches_central_differences():
+@pytest.mark.parametrize("kk", [1.0, 0.0])
+def test_gpu_indirect_reverse_matches_central_differences(kk):
     import torch
     from adjoint_helpers import coefs, directional, random_directions
     from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
     gz, gzh, st = _state(lall=1, ijdim=256, seed=23)
     rng = np.random.default_rng(6)
     st["unccn"][...] = rng.uniform(0.0, 400.0e6, st["unccn"].shape)
-    P0 = _with(default_params(), opt_indir=1.0)
+    P0 = _with(default_params(), opt_indir=1.0, sw_kk2000=kk)
     drv = NSW6(gz, gzh, params=P0, max_columns=256)
     dev = F.to_device(st)
     _, out = _loss(drv, dev)
@@ -185,7 +181,7 @@ def test_gpu_indirect_reverse_matches_central_differences():
     def fd(i, h):
         vals = []
         for sgn in (1.0, -1.0):
-            P = _with(default_params(), opt_indir=1.0)
+            P = _with(default_params(), opt_indir=1.0, sw_kk2000=kk)
             for c, name in enumerate(N.TUNABLE_COEFFS):
                 setattr(P, name, c0[c] + sgn * h * dcoef[i][c])
             drv.set_params(P)
