@@ -34,9 +34,10 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjLaunch L)
     adj_cell(L.A, lb.l, lb.k, ij);
 }
 
-// Two waves per SIMD: ~32 doubles spill to scratch, which costs less than the
-// latency exposed at one wave per SIMD.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_adj_wall(AdjLaunch L)
+// One wave per SIMD: the 512-register budget keeps the pass out of scratch (the
+// overflow goes to 66 accumulation registers); at two waves 236 B per lane spilled
+// to scratch and the reverse passes took about 4 % longer.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_wall(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
