@@ -247,6 +247,16 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  * sub-steps the Newton counts cover all of them; n_general_regions counts the last. */
 int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *report);
 
+/* kcell_min / kcell_max of vadv1d_prep (src/mod_vadv1d.cpp:317-352), i.e. the
+ * per-region departure-cell bounds that the last nsw6_mp_driver* step (its last
+ * sub-step) or the forward part of the last nsw6_mp_driver_adj computed on the
+ * device, for regions 0..lall-1: out[l][NT][2 (full level, half level)][NSW6_KALL]
+ * [2 (kcell_min, kcell_max)], NT = 3 ("3WATER": QR, QS, QG) or 4 ("4WATER": QR, QI,
+ * QS, QG), 0-based levels.  The reference keeps them in host Views of
+ * precip_transport_new; here they are a read-out for checks (synchronises the last
+ * stream).  n_out = elements available at `out`. */
+int nsw6_kcell_bounds(nsw6_ctx *ctx, int lall, int32_t *out, int64_t n_out);
+
 /* Average duration (ms) of the named kernel over the launches recorded since the
  * last reset, measured with HIP events on the stream the kernel ran on.
  * which: 0 = whole step, 1 = process rates, 2 = fall-speed extremes + kcell bounds,

@@ -96,6 +96,7 @@ SIGNATURES = {
     "nsw6_mp_driver_adj": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), ctypes.POINTER(FieldPtrs), _P,
                                 ctypes.POINTER(FieldPtrs), _P, _D, _D, _P]),
     "nsw6_last_status": (_I, [_P, ctypes.POINTER(Report)]),
+    "nsw6_kcell_bounds": (_I, [_P, _I, _P, ctypes.c_int64]),
     "nsw6_timing_enable": (_I, [_P, _I]),
     "nsw6_timing_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_int64)]),
     "nsw6_timing_reset": (_I, [_P]),

@@ -52,6 +52,7 @@ struct nsw6_ctx {
     Workspace ws{};
     hipStream_t last_stream = nullptr;
     int last_lall = 0;
+    int last_nt = 3;                 // sedimenting tracers of the last step (kcell bounds layout)
     bool timing = false;
     std::vector<PendingTiming> pending;
     double t_sum[N_TIMERS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -262,6 +263,7 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     hipStream_t s = (hipStream_t)stream;
     ctx->last_stream = s;
     ctx->last_lall = lall;
+    ctx->last_nt = a.four_water ? 4 : 3;
     ctx->last_adjoint = false;
     for (int m = 0; m < ndiv; m++) {
         a.keep_report = (m > 0);
@@ -605,6 +607,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
     if (ctx->timing) ctx->pending_adj.push_back(pt);
     ctx->last_stream = s;
     ctx->last_lall = lall;
+    ctx->last_nt = 3;
     ctx->last_adjoint = true;
     return st;
 }
@@ -694,6 +697,15 @@ static int drain_timing(nsw6_ctx *ctx)
     }
     ctx->pending_adj.clear();
     return NSW6_OK;
+}
+
+int nsw6_kcell_bounds(nsw6_ctx *ctx, int lall, int32_t *out, int64_t n_out)
+{
+    if (!ctx || !out || lall <= 0 || lall > ctx->last_lall) return NSW6_BAD_ARG;
+    const int per = ctx->last_nt * 2 * KALL * 2;
+    if (n_out < (int64_t)lall * per) return NSW6_BAD_ARG;
+    if (hipStreamSynchronize(ctx->last_stream) != hipSuccess) return NSW6_HIP_ERROR;
+    return hip_ok(hipMemcpy(out, ctx->ws.bounds, sizeof(int32_t) * (size_t)lall * per, hipMemcpyDeviceToHost));
 }
 
 int nsw6_timing_enable(nsw6_ctx *ctx, int on)

@@ -162,6 +162,15 @@ class NSW6:
             raise NSW6Error(rc, rep)
         return rep
 
+    def kcell_bounds(self, lall: int, nt: int = 3) -> np.ndarray:
+        """kcell_min/kcell_max of the last step (nsw6_kcell_bounds) as int32
+        [l][nt][2 (full, half)][96][2 (kcell_min, kcell_max)]."""
+        out = np.empty((lall, nt, 2, N.KALL, 2), dtype=np.int32)
+        rc = self._L.nsw6_kcell_bounds(self._ctx, int(lall), out.ctypes.data, out.size)
+        if rc != 0:
+            raise NSW6Error(rc)
+        return out
+
     def report(self) -> N.Report:
         rep = N.Report()
         self._L.nsw6_last_status(self._ctx, ctypes.byref(rep))

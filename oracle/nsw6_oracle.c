@@ -79,6 +79,29 @@ static const double nu0 = 1.718E-5, dnu_dT = 5.28E-8;
 static const double TEM_MIN = 10.0;
 
 static double g_elapsed = 0.0;
+/* Test hook (ora_record_kcell_bounds): when set, precip_transport_new records the
+ * kcell_min/kcell_max of each of its vadv1d_prep calls into
+ * g_bounds_rec[l][NT][2 (full, half level)][KALL][2 (min, max)], NT = the number of
+ * sedimenting tracers, for region g_bounds_l. */
+static int *g_bounds_rec = NULL;
+static int g_bounds_l = 0;
+/* Test hook (ora_record_newton_margin): when set, saturation_adjustment records per
+ * point g_newton_rec[l][k][ij] the relative distance of the Newton's stopping test
+ * from its criterion over the last two iterations (see ora_record_newton_margin). */
+static double *g_newton_rec = NULL;
+
+/* Stopping margin of a Newton that ran `n` iterations, the last two steps being
+ * d_prev (only when n >= 2) and d_last: min over those of | |d| - crit | / crit. */
+static double newton_margin(int n, int converged, double d_prev, double d_last, double crit)
+{
+    if (n <= 0 || !converged) return 1e300;
+    double m = fabs(fabs(d_last) - crit) / crit;
+    if (n >= 2) {
+        double mp = fabs(fabs(d_prev) - crit) / crit;
+        if (mp < m) m = mp;
+    }
+    return m;
+}
 
 #define IDX(k, ij)      ((size_t)(k) * (size_t)ijdim + (size_t)(ij))
 #define QIDX(nq, k, ij) (((size_t)(nq) * KALL + (size_t)(k)) * (size_t)ijdim + (size_t)(ij))
@@ -931,6 +954,19 @@ typedef struct region_ptrs {
     double *ix, *iy, *iz, *jx, *jy, *jz;
 } region_ptrs;
 
+static void record_bounds(int t, int nt, int type, const int *kmin, const int *kmax)
+{
+    if (!g_bounds_rec) return;
+    int *b = g_bounds_rec + (((size_t)g_bounds_l * nt + t) * 2 + type) * KALL * 2;
+    for (int k = 0; k < KALL; k++) {
+        b[k * 2 + 0] = kmin[k];
+        b[k * 2 + 1] = kmax[k];
+    }
+}
+
+void ora_record_kcell_bounds(int *buf) { g_bounds_rec = buf; }
+void ora_record_newton_margin(double *buf) { g_newton_rec = buf; }
+
 static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, const double *Vterm,
                                  const int precipitating_flag[NQMAX], const double *rgs,
                                  const double *rgsh, double dt, int kin2kin)
@@ -990,11 +1026,16 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
         R->precip_rhokin[ij] = 0.0;
     }
 
+    int n_sed = 0;
+    for (int nq = 0; nq < NQMAX; nq++) n_sed += precipitating_flag[nq] != 0;
+    int nt_rec = -1;   /* sedimenting-tracer index of nq */
     for (int nq = 0; nq < NQMAX; nq++) {
         if (!precipitating_flag[nq]) continue;
+        nt_rec++;
         const double *Vt_nq = Vterm + (size_t)nq * K2;
         ora_vadv1d_prep(ijdim, KMIN, KMAX, G->dgz, G->gzh, Vt_nq, zdis0, kcell, kcell_max,
                         kcell_min, dt);
+        record_bounds(nt_rec, n_sed, 0, kcell_min, kcell_max);
 #pragma omp parallel for collapse(2)
         for (int k = 0; k < KALL; k++)
             for (int ij = 0; ij < ijdim; ij++) rhoq[IDX(k, ij)] = rhogq[QIDX(nq, k, ij)] * rgs[IDX(k, ij)];
@@ -1046,6 +1087,7 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
             }
             ora_vadv1d_prep(ijdim, KMIN + 1, KMAX, G->dgzh, GRD_gz_shift, Vtermh, zdis0h, kcellh,
                             kcellh_max, kcellh_min, dt);
+            record_bounds(nt_rec, n_sed, 1, kcellh_min, kcellh_max);
             for (int k = KMIN + 1; k <= KMAX; k++)
                 for (int ij = 0; ij < ijdim; ij++)
                     qh[IDX(k, ij)] = 0.5 * (q[QIDX(nq, k, ij)] + q[QIDX(nq, k - 1, ij)]) * rgsh[IDX(k, ij)];
@@ -1210,8 +1252,17 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
  * Returns 0 when the point is not supersaturated (no iteration), n > 0 when the
  * Newton converged after n iterations, -1 when the NaN guard fired and -2 when
  * itelim = 100 iterations did not converge (the reference aborts in both cases). */
+static int satadjust_liq_core(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                              double *tem, double *qv, double *qc, double *margin);
+
 int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_loc,
                             const double q6[6], double *tem, double *qv, double *qc)
+{
+    return satadjust_liq_core(rho, Emoist, qsum, qd_loc, q6, tem, qv, qc, NULL);
+}
+
+static int satadjust_liq_core(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                              double *tem, double *qv, double *qc, double *margin)
 {
     const double RTEM00 = 1.0 / CONST_TEM00;
     const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));  /* = 1e-8 */
@@ -1223,6 +1274,8 @@ int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_lo
     double psat = CONST_PSAT0 * (pow(t * RTEM00, CPovR_liq)) * exp(LovR_liq * (RTEM00 - rtem));
     double qsat = psat / (rho * CONST_Rvap * t);
     int code = 0;
+    double d_prev = 0.0, d_last = 0.0;
+    if (margin) *margin = 1e300;
     if (qsum - qsat > CONST_EPS) {
         int converged = 0, nan_break = 0, ite;
         for (ite = 0; ite < itelim; ite++) {
@@ -1239,6 +1292,8 @@ int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_lo
             double dEmoist_dT = t * dCVtot_dT + CVtot + dqsat_dT * LHV;
             double dtemp = (Emoist_new - Emoist) / dEmoist_dT;
             t = t - dtemp;
+            d_prev = d_last;
+            d_last = dtemp;
             if (fabs(dtemp) < dtemp_criteria) {
                 converged = 1;
                 break;
@@ -1246,6 +1301,7 @@ int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_lo
             if (t * 0.0 != 0.0) { nan_break = 1; break; }
         }
         code = converged ? ite + 1 : (nan_break ? -1 : -2);
+        if (margin) *margin = newton_margin(ite + 1, converged, d_prev, d_last, dtemp_criteria);
     }
     *tem = t;
     *qv = qq[I_QV];
@@ -1257,8 +1313,19 @@ int ora_satadjust_liq_point(double rho, double Emoist, double qsum, double qd_lo
  * with the liquid fraction alpha = clamp((T - LL) / (UL - LL), 0, 1).  Returns 0 (not
  * supersaturated), the iteration count, -1 (NaN) or -2 (itelim); writes the final
  * tem and the last iterate's q(QV), q(QC), q(QI). */
+static int satadjust_all_core(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                              double UL, double LL, double *tem, double *qv, double *qc, double *qi,
+                              double *margin);
+
 int ora_satadjust_all_point(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
                             double UL, double LL, double *tem, double *qv, double *qc, double *qi)
+{
+    return satadjust_all_core(rho, Emoist, qsum, qd_loc, q6, UL, LL, tem, qv, qc, qi, NULL);
+}
+
+static int satadjust_all_core(double rho, double Emoist, double qsum, double qd_loc, const double q6[6],
+                              double UL, double LL, double *tem, double *qv, double *qc, double *qi,
+                              double *margin)
 {
     const double RTEM00 = 1.0 / CONST_TEM00;
     const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));
@@ -1273,6 +1340,8 @@ int ora_satadjust_all_point(double rho, double Emoist, double qsum, double qd_lo
     double psat = psatl * alpha + psati * (1.0 - alpha);
     double qsat = psat / (rho * CONST_Rvap * t);
     int code = 0;
+    double d_prev = 0.0, d_last = 0.0;
+    if (margin) *margin = 1e300;
     if (qsum - qsat > CONST_EPS) {
         int converged = 0, nan_break = 0, ite;
         for (ite = 0; ite < 100; ite++) {
@@ -1304,10 +1373,13 @@ int ora_satadjust_all_point(double rho, double Emoist, double qsum, double qd_lo
             double dEmoist_dT = t * dCVtot_dT + CVt + dqsat_dT * LHV - dqi_dT * LHF;
             double dtemp = (Emoist_new - Emoist) / dEmoist_dT;
             t = t - dtemp;
+            d_prev = d_last;
+            d_last = dtemp;
             if (fabs(dtemp) < dtemp_criteria) { converged = 1; break; }
             if (t * 0.0 != 0.0) { nan_break = 1; break; }
         }
         code = converged ? ite + 1 : (nan_break ? -1 : -2);
+        if (margin) *margin = newton_margin(ite + 1, converged, d_prev, d_last, dtemp_criteria);
     }
     *tem = t;
     *qv = qq[I_QV];
@@ -1369,7 +1441,9 @@ static void saturation_adjustment(int ijdim, double *rhog, double *rhoge, double
                 for (int nq = 0; nq < 6; nq++) q6[nq] = q[QIDX(nq, k, ij)];
                 qv = q6[I_QV];
                 qc = q6[I_QC];
-                int ite = ora_satadjust_liq_point(rho[p], ein_moist[p], qsum[p], qd_loc[p], q6, &t, &qv, &qc);
+                double mg;
+                int ite = satadjust_liq_core(rho[p], ein_moist[p], qsum[p], qd_loc[p], q6, &t, &qv, &qc, &mg);
+                if (g_newton_rec) g_newton_rec[(size_t)l * K2 + p] = mg;
                 tem[p] = t;
                 q[QIDX(I_QV, k, ij)] = qv;
                 q[QIDX(I_QC, k, ij)] = qc;
@@ -1392,8 +1466,10 @@ static void saturation_adjustment(int ijdim, double *rhog, double *rhoge, double
                 qv = q6[I_QV];
                 qc = q6[I_QC];
                 qi = q6[I_QI];
-                int ite = ora_satadjust_all_point(rho[p], ein_moist[p], qsum[p], qd_loc[p], q6, UL, LL, &t, &qv,
-                                                  &qc, &qi);
+                double mg;
+                int ite = satadjust_all_core(rho[p], ein_moist[p], qsum[p], qd_loc[p], q6, UL, LL, &t, &qv,
+                                             &qc, &qi, &mg);
+                if (g_newton_rec) g_newton_rec[(size_t)l * K2 + p] = mg;
                 tem[p] = t;
                 q[QIDX(I_QV, k, ij)] = qv;
                 q[QIDX(I_QC, k, ij)] = qc;
@@ -1645,6 +1721,7 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
                 R.precip_rhophi[ij] = 0.0; R.precip_rhokin[ij] = 0.0;
             }
             clock_gettime(CLOCK_MONOTONIC, &t0);
+            g_bounds_l = l;
             mp_nsw6(P, G, ijdim, &R, f->re_liquid + o2, f->re_cld + o2, f->rctop + o1, f->rwtop + o1,
                     f->tctop + o1, dt_mp, ice_adjust, rep, l);
             clock_gettime(CLOCK_MONOTONIC, &t1);

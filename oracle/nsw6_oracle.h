@@ -73,6 +73,22 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
                       const nsw6_fields *f, double dt, double ct, int ice_adjust, int div_num,
                       nsw6_report *rep);
 
+/* Test hook: while `buf` is not NULL, every ora_mp_driver* call records the kcell
+ * bounds of its vadv1d_prep calls (src/mod_vadv1d.cpp:317-352) as
+ * buf[l][NT][2 (full level, half level)][NSW6_KALL][2 (kcell_min, kcell_max)], NT = 3
+ * ("3WATER") or 4 ("4WATER"); with sub-steps the last one's are kept. */
+void ora_record_kcell_bounds(int *buf);
+
+/* Test hook: while `buf` is not NULL, every ora_mp_driver* call records for each
+ * point of the saturation adjustment's Newton (src/mod_satadjust.cpp:828-894,
+ * 700-769) buf[l][k][ij] = the stopping margin min(| |dtemp_n| - c | / c,
+ * | |dtemp_n-1| - c | / c) over its last two steps, c = dtemp_criteria = 1e-8 K;
+ * 1e300 where no iteration ran or it did not converge.  A small margin marks a point
+ * where a last-bit change of the inputs can change the iteration count (and so
+ * q(QV) = qsat of the previous iterate by dqsat/dT * |dtemp|).  Interior levels
+ * only; the caller initialises the halo levels. */
+void ora_record_newton_margin(double *buf);
+
 /* Seconds of the last ora_mp_driver call (wall clock, like the reference's
  * "NSW6 Elapsed Time", src/mod_mp_driver.cpp:500-552). */
 double ora_last_elapsed(void);

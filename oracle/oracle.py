@@ -65,6 +65,10 @@ def lib():
         L.ora_mp_driver_div.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
                                         ctypes.POINTER(N.FieldPtrs), D, D, I, I, ctypes.POINTER(N.Report)]
         L.ora_mp_driver_div.restype = I
+        L.ora_record_kcell_bounds.argtypes = [P]
+        L.ora_record_kcell_bounds.restype = None
+        L.ora_record_newton_margin.argtypes = [P]
+        L.ora_record_newton_margin.restype = None
         L.ora_last_elapsed.restype = D
         L.ora_last_elapsed.argtypes = []
         L.ora_num_threads.restype = I
@@ -124,6 +128,34 @@ def bigloop_point(P, dt, dens, temp, q6, qsatl, qsati, pre, rhog, Nc, a1, a2, ma
     lib().ora_bigloop_point(ctypes.byref(P), dt, dens, temp, q.ctypes.data, qsatl, qsati, pre, rhog, Nc, a1, a2,
                             ma2, ctypes.byref(o))
     return o
+
+
+def mp_driver_with_bounds(state: dict, gz, gzh, dt: float, P=None, nt: int = 3):
+    """ora_mp_driver in place, also returning the kcell bounds of its vadv1d_prep calls
+    as an int array [l][nt][2 (full, half)][96][2 (kcell_min, kcell_max)]."""
+    rep, b, _ = mp_driver_traced(state, gz, gzh, dt, P=P, nt=nt, newton=False)
+    return rep, b
+
+
+def mp_driver_traced(state: dict, gz, gzh, dt: float, P=None, nt: int = 3, bounds: bool = True,
+                     newton: bool = True):
+    """ora_mp_driver in place with its test hooks on: returns (report, kcell bounds
+    [l][nt][2][96][2] or None, Newton stopping margins [l][96][ij] or None; see
+    ora_record_newton_margin)."""
+    lall, ijdim = state["rhog"].shape[0], state["rhog"].shape[-1]
+    b = np.full((lall, nt, 2, 96, 2), -99, dtype=np.int32) if bounds else None
+    m = np.full((lall, 96, ijdim), 1e300) if newton else None
+    L = lib()
+    if bounds:
+        L.ora_record_kcell_bounds(b.ctypes.data)
+    if newton:
+        L.ora_record_newton_margin(m.ctypes.data)
+    try:
+        rep = mp_driver(state, gz, gzh, dt, P=P)
+    finally:
+        L.ora_record_kcell_bounds(None)
+        L.ora_record_newton_margin(None)
+    return rep, b, m
 
 
 def last_elapsed() -> float:
