@@ -208,11 +208,16 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
 int nsw6_set_params(nsw6_ctx *ctx, const nsw6_params *params)
 {
     if (!ctx || !params) return NSW6_BAD_ARG;
-    to_dev_params(*params, ctx->P);
+    DevParams P;
+    to_dev_params(*params, P);
     // steps already queued read the old coefficients; let them finish first
     if (ctx->last_stream && hipStreamSynchronize(ctx->last_stream) != hipSuccess) return NSW6_HIP_ERROR;
     if (hipDeviceSynchronize() != hipSuccess) return NSW6_HIP_ERROR;
-    return hip_ok(hipMemcpy(ctx->P_dev, &ctx->P, sizeof(DevParams), hipMemcpyHostToDevice));
+    if (hipMemcpy(ctx->P_dev, &P, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess) return NSW6_HIP_ERROR;
+    // the host copy drives the choice of kernel variants: change it only once the
+    // device holds the same coefficients
+    ctx->P = P;
+    return NSW6_OK;
 }
 
 int nsw6_set_precision(nsw6_ctx *ctx, int precision)
