@@ -205,6 +205,25 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
     double *z, double *zh, double dt, double ct, double *GDCLW, double *GDCFRC,
     double *GPREC, double *CBMFX, void *stream);
 
+/* No counterpart in the reference: the same step as nsw6_mp_driver_fields for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or other
+ * allocating routine: the step's workspace (post-rate state, fall speeds, wall fluxes,
+ * partial extremes, kcell bounds, region flags, the device report and the MP_DIV_NUM
+ * sums) lives in one device array of the library per GPU, sized for
+ * NSW6_NOALLOC_MAX_COLUMNS columns in at most NSW6_NOALLOC_MAX_REGIONS regions (about
+ * 330 MB), which each GPU the library runs on holds from nsw6_init until the process
+ * ends.  Larger calls return NSW6_BAD_ARG.  `ctx` (any context of the device, from
+ * nsw6_init) supplies the coefficients, grid, precision and MP_DIV_NUM; kernel timing
+ * is not recorded.  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other contexts, other threads) race and must be ordered by
+ * the caller.  nsw6_last_status and nsw6_kcell_bounds read the report of the last
+ * call of `ctx`, which a later call on the same GPU overwrites. */
+#define NSW6_NOALLOC_MAX_COLUMNS 16900
+#define NSW6_NOALLOC_MAX_REGIONS 64
+int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
+                           const nsw6_fields *fields, double dt, double ct, void *stream);
+
 /* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
  * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; or the same with Berry's
  * autoconversion, sw_kk2000 = 0; each with or without OPT_INDIR, which then reads

@@ -21,6 +21,7 @@ NSW6_OK, NSW6_NOT_CONVERGED, NSW6_NAN_DETECTED = 0, 1, 2
 NSW6_BAD_ARG, NSW6_HIP_ERROR, NSW6_NO_DEVICE, NSW6_UNSUPPORTED = 3, 4, 5, 6
 NSW6_IO_ERROR = 7
 NSW6_MAX_DIV_NUM = 1000
+NSW6_NOALLOC_MAX_COLUMNS, NSW6_NOALLOC_MAX_REGIONS = 16900, 64
 NSW6_VERIFY_FIELDS = 12
 NSW6_FP64, NSW6_MIXED = 0, 1
 
@@ -93,6 +94,7 @@ SIGNATURES = {
     "nsw6_set_div_num": (_I, [_P, _I]),
     "nsw6_mp_driver_fields": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), _D, _D, _P]),
     "nsw6_mp_driver": (_I, [_P, _I, _I, _I] + [_P] * 46 + [_D, _D] + [_P] * 4 + [_P]),
+    "nsw6_mp_driver_noalloc": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), _D, _D, _P]),
     "nsw6_mp_driver_adj": (_I, [_P, _I, _I, _I, ctypes.POINTER(FieldPtrs), ctypes.POINTER(FieldPtrs), _P,
                                 ctypes.POINTER(FieldPtrs), _P, _D, _D, _P]),
     "nsw6_last_status": (_I, [_P, ctypes.POINTER(Report)]),

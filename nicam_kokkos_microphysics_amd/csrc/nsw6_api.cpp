@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -53,6 +54,9 @@ struct nsw6_ctx {
     hipStream_t last_stream = nullptr;
     int last_lall = 0;
     int last_nt = 3;                 // sedimenting tracers of the last step (kcell bounds layout)
+    // where the last call left its report, region flags and kcell bounds: the
+    // context's workspace, or the allocation-free call's scratch
+    const int *last_report = nullptr, *last_region_flag = nullptr, *last_bounds = nullptr;
     bool timing = false;
     std::vector<PendingTiming> pending;
     double t_sum[N_TIMERS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -72,6 +76,84 @@ struct nsw6_ctx {
 };
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? NSW6_OK : NSW6_HIP_ERROR; }
+
+// ---------------------------------------------------------------------------------
+// Workspace of the allocation-free entry point nsw6_mp_driver_noalloc: one file-scope
+// device array per GPU, sized for NSW6_NOALLOC_MAX_COLUMNS columns in at most
+// NSW6_NOALLOC_MAX_REGIONS regions and carved into the step's workspace by
+// noalloc_layout.  Its device address is looked up once per device at nsw6_init
+// (hipGetSymbolAddress, with that device current) and kept in g_scratch_ptr; host code
+// never passes the array's own name to a kernel or a copy.
+// ---------------------------------------------------------------------------------
+namespace {
+constexpr size_t round_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
+constexpr size_t ALIGN_D = 32;   // 256 bytes, in doubles
+struct NoallocLayout {           // offsets in doubles from the array's start
+    size_t rhog, rhoge, rhogq, vt, flux, part, div_sum, ints, total;
+};
+constexpr NoallocLayout noalloc_layout()
+{
+    const size_t cols = NSW6_NOALLOC_MAX_COLUMNS, regions = NSW6_NOALLOC_MAX_REGIONS;
+    const size_t K2 = (size_t)KALL * cols;
+    NoallocLayout L{};
+    size_t o = 0;
+    L.rhog = o;  o = round_up(o + K2, ALIGN_D);
+    L.rhoge = o; o = round_up(o + K2, ALIGN_D);
+    L.rhogq = o; o = round_up(o + K2 * NQ, ALIGN_D);
+    L.vt = o;    o = round_up(o + K2 * 4, ALIGN_D);
+    L.flux = o;  o = round_up(o + K2 * FLUX_SLOTS_MAX, ALIGN_D);
+    // per-block partial extremes: sum over regions of ceil(ijdim / block) blocks
+    L.part = o;  o = round_up(o + (size_t)KALL * 16 * (cols / EXT_COLS_PER_BLOCK + regions), ALIGN_D);
+    L.div_sum = o; o = round_up(o + cols * (KALL + DIV_COLUMN_SUMS), ALIGN_D);
+    // ints: kcell bounds, region flags, report, cloud top (two per double)
+    L.ints = o;
+    const size_t n_int = (size_t)EXT_PER_REGION_MAX * regions + regions + REPORT_INTS + cols;
+    o = round_up(o + (n_int + 1) / 2, ALIGN_D);
+    L.total = o;
+    return L;
+}
+constexpr NoallocLayout NOALLOC = noalloc_layout();
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC.total];
+
+static std::mutex g_scratch_mu;
+constexpr int MAX_DEVICES = 64;
+static double *g_scratch_ptr[MAX_DEVICES];   // device address of `scratch`, per device
+
+// Look up `scratch` on the current device (once per device).
+static int resolve_scratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return NSW6_BAD_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NSW6_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NSW6_HIP_ERROR;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return NSW6_OK;
+}
+
+// The step workspace inside the scratch of `device` (its address as resolved above).
+static Workspace noalloc_workspace(double *base, double **div_sum)
+{
+    Workspace w{};
+    w.rhog = base + NOALLOC.rhog;
+    w.rhoge = base + NOALLOC.rhoge;
+    w.rhogq = base + NOALLOC.rhogq;
+    w.vt = base + NOALLOC.vt;
+    w.flux = base + NOALLOC.flux;
+    w.part = base + NOALLOC.part;
+    *div_sum = base + NOALLOC.div_sum;
+    int *ip = reinterpret_cast<int *>(base + NOALLOC.ints);
+    w.bounds = ip;
+    ip += (size_t)EXT_PER_REGION_MAX * NSW6_NOALLOC_MAX_REGIONS;
+    w.region_flag = ip;
+    ip += NSW6_NOALLOC_MAX_REGIONS;
+    w.report = ip;
+    ip += REPORT_INTS;
+    w.cloud_top = ip;
+    return w;
+}
 
 static void free_ws(nsw6_ctx *c)
 {
@@ -128,9 +210,9 @@ static Fields to_fields(const nsw6_fields &s)
     return f;
 }
 
+// Everything of a step's arguments but the fields and the workspace.
 static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, StepArgs &a)
 {
-    a.ws = ctx->ws;
     a.P = ctx->P_dev;
     a.grid = ctx->grid_dev;
     a.lall = lall;
@@ -194,6 +276,7 @@ int nsw6_init(const nsw6_grid *grid, const nsw6_params *params, int64_t max_colu
         hipMemset(w.cloud_top, 0xff, sizeof(int) * (size_t)max_columns) != hipSuccess)
         st = NSW6_HIP_ERROR;
     if (st == NSW6_OK) st = ensure_regions(c, 64);
+    if (st == NSW6_OK) st = resolve_scratch(c->device);
     if (st != NSW6_OK) {
         free_ws(c);
         if (c->grid_dev) (void)hipFree(c->grid_dev);
@@ -227,13 +310,11 @@ int nsw6_set_precision(nsw6_ctx *ctx, int precision)
     return NSW6_OK;
 }
 
-int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *fl,
-                          double dt, double ct, void *stream)
+// Arguments every mp_driver step needs (the context's column limit is checked by the
+// caller).
+static int check_step_args(const nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *fl, double dt)
 {
-    (void)l_region;   // only stored by the reference
-    (void)ct;         // unused by the reference's NSW6 path
     if (!ctx || !fl || lall <= 0 || ijdim <= 0) return NSW6_BAD_ARG;
-    if ((int64_t)lall * ijdim > ctx->max_columns) return NSW6_BAD_ARG;
     if (!(dt > 0.0)) return NSW6_BAD_ARG;
     const double *req[] = {fl->rhog, fl->rhogvx, fl->rhogvy, fl->rhogvz, fl->rhogw, fl->rhoge, fl->rhogq,
                            fl->vx, fl->vy, fl->vz, fl->rho, fl->tem, fl->pre, fl->q, fl->qd, fl->precip,
@@ -245,47 +326,91 @@ int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, cons
     for (const double *p : req)
         if (!p) return NSW6_BAD_ARG;
     if (ctx->P.opt_indir != 0.0 && !fl->unccn) return NSW6_BAD_ARG;   // read under OPT_INDIR
-    int st = use_device(ctx);
+    return NSW6_OK;
+}
+
+// mp_driver's sub-step loop (src/mod_mp_driver.cpp:420-612) on the workspace of `a`:
+// MP_DIV_NUM steps of dt_mp, their output sums in `div_sum` and the scaling by
+// fraction_mp.  With `timed`, HIP events of each step go to the context's timers.
+static int run_steps(nsw6_ctx *ctx, StepArgs &a, int lall, double dt, double *div_sum, hipStream_t s, bool timed)
+{
+    const int ndiv = ctx->div_num;
+    // fraction_mp and dt_mp as src/mod_mp_driver.cpp:420-421
+    const double fraction_mp = 1.0 / double(ndiv);
+    fill_step_args(ctx, lall, a.ijdim, dt * fraction_mp, a);
+    ctx->last_stream = s;
+    ctx->last_lall = lall;
+    ctx->last_nt = a.four_water ? 4 : 3;
+    ctx->last_adjoint = false;
+    ctx->last_report = a.ws.report;
+    ctx->last_region_flag = a.ws.region_flag;
+    ctx->last_bounds = a.ws.bounds;
+    for (int m = 0; m < ndiv; m++) {
+        a.keep_report = (m > 0);
+        hipEvent_t *ev = nullptr;
+        PendingTiming pt;
+        if (timed) {
+            for (int i = 0; i < N_EVENTS; i++)
+                if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
+            ev = pt.ev;
+        }
+        hipError_t e = nsw6_launch_step(&a, ctx->precision, s, ev);
+        if (timed) ctx->pending.push_back(pt);
+        if (e == hipSuccess && ndiv > 1) e = nsw6_launch_div_sum(&a, div_sum, m == 0, s);
+        if (e != hipSuccess) return NSW6_HIP_ERROR;
+    }
+    if (ndiv > 1) return hip_ok(nsw6_launch_div_scale(&a, div_sum, fraction_mp, s));
+    return NSW6_OK;
+}
+
+int nsw6_mp_driver_fields(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *fl,
+                          double dt, double ct, void *stream)
+{
+    (void)l_region;   // only stored by the reference
+    (void)ct;         // unused by the reference's NSW6 path
+    int st = check_step_args(ctx, lall, ijdim, fl, dt);
+    if (st != NSW6_OK) return st;
+    if ((int64_t)lall * ijdim > ctx->max_columns) return NSW6_BAD_ARG;
+    st = use_device(ctx);
     if (st == NSW6_OK) st = ensure_regions(ctx, lall);
     if (st != NSW6_OK) return st;
 
-    const int ndiv = ctx->div_num;
-    if (ndiv > 1 && !ctx->div_sum) {
+    if (ctx->div_num > 1 && !ctx->div_sum) {
         const size_t n = sizeof(double) * (size_t)ctx->max_columns * (KALL + DIV_COLUMN_SUMS);
         if (hipMalloc(&ctx->div_sum, n) != hipSuccess) {
             ctx->div_sum = nullptr;
             return NSW6_HIP_ERROR;
         }
     }
-
-    // fraction_mp and dt_mp as src/mod_mp_driver.cpp:420-421
-    const double fraction_mp = 1.0 / double(ndiv);
     StepArgs a;
     memset(&a, 0, sizeof a);
     a.f = to_fields(*fl);
-    fill_step_args(ctx, lall, ijdim, dt * fraction_mp, a);
+    a.ws = ctx->ws;
+    a.ijdim = ijdim;
+    return run_steps(ctx, a, lall, dt, ctx->div_sum, (hipStream_t)stream, ctx->timing);
+}
 
-    hipStream_t s = (hipStream_t)stream;
-    ctx->last_stream = s;
-    ctx->last_lall = lall;
-    ctx->last_nt = a.four_water ? 4 : 3;
-    ctx->last_adjoint = false;
-    for (int m = 0; m < ndiv; m++) {
-        a.keep_report = (m > 0);
-        hipEvent_t *ev = nullptr;
-        PendingTiming pt;
-        if (ctx->timing) {
-            for (int i = 0; i < N_EVENTS; i++)
-                if (hipEventCreate(&pt.ev[i]) != hipSuccess) return NSW6_HIP_ERROR;
-            ev = pt.ev;
-        }
-        hipError_t e = nsw6_launch_step(&a, ctx->precision, s, ev);
-        if (ctx->timing) ctx->pending.push_back(pt);
-        if (e == hipSuccess && ndiv > 1) e = nsw6_launch_div_sum(&a, ctx->div_sum, m == 0, s);
-        if (e != hipSuccess) return NSW6_HIP_ERROR;
-    }
-    if (ndiv > 1) return hip_ok(nsw6_launch_div_scale(&a, ctx->div_sum, fraction_mp, s));
-    return NSW6_OK;
+int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const nsw6_fields *fl,
+                           double dt, double ct, void *stream)
+{
+    (void)l_region;
+    (void)ct;
+    int st = check_step_args(ctx, lall, ijdim, fl, dt);
+    if (st != NSW6_OK) return st;
+    if (lall > NSW6_NOALLOC_MAX_REGIONS || (int64_t)lall * ijdim > NSW6_NOALLOC_MAX_COLUMNS) return NSW6_BAD_ARG;
+    st = use_device(ctx);
+    if (st != NSW6_OK) return st;
+    double *base = g_scratch_ptr[ctx->device];   // resolved by nsw6_init on this device
+    if (!base) return NSW6_HIP_ERROR;
+    StepArgs a;
+    memset(&a, 0, sizeof a);
+    a.f = to_fields(*fl);
+    double *div_sum = nullptr;
+    a.ws = noalloc_workspace(base, &div_sum);
+    a.ijdim = ijdim;
+    // cloud_top needs no clearing: the array starts at 0, k_cloudtop puts back -1 after
+    // each use, and both read as "no cloudy level" (levels 1..94 only count)
+    return run_steps(ctx, a, lall, dt, div_sum, (hipStream_t)stream, false);
 }
 
 int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
@@ -359,6 +484,7 @@ static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state
     a.f.precip_rhophi = dc + 11 * C;
     a.f.precip_rhokin = dc + 12 * C;
     fill_step_args(ctx, lall, ijdim, dt, a);
+    a.ws = ctx->ws;
     a.keep_report = keep_report;
     A.cot = to_fields(*cot);
     A.cot_sflux = cot_sflux;
@@ -481,6 +607,7 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
         a.f.qd = junk;
         a.f.re_solid = a.f.GDCLW = a.f.GDCFRC = junk + K2;
         fill_step_args(ctx, lall, ijdim, dt_mp, a);
+        a.ws = ctx->ws;
         for (int m = 1; m < n; m++) {
             const Carry dst = carry(m);
             nsw6_fields src_f = *state;
@@ -614,6 +741,9 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
     ctx->last_lall = lall;
     ctx->last_nt = 3;
     ctx->last_adjoint = true;
+    ctx->last_report = ctx->ws.report;
+    ctx->last_region_flag = ctx->ws.region_flag;
+    ctx->last_bounds = ctx->ws.bounds;
     return st;
 }
 
@@ -649,8 +779,12 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *r)
         r->status = NSW6_HIP_ERROR;
         return NSW6_HIP_ERROR;
     }
+    if (!ctx->last_report) {   // no step yet
+        r->status = NSW6_OK;
+        return NSW6_OK;
+    }
     int rep[REPORT_INTS];
-    if (hipMemcpy(rep, ctx->ws.report, sizeof rep, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(rep, ctx->last_report, sizeof rep, hipMemcpyDeviceToHost) != hipSuccess) {
         r->status = NSW6_HIP_ERROR;
         return NSW6_HIP_ERROR;
     }
@@ -661,7 +795,7 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *r)
     r->first_ij = rep[5];
     if (ctx->last_lall > 0) {
         std::vector<int> flags((size_t)ctx->last_lall);
-        if (hipMemcpy(flags.data(), ctx->ws.region_flag, sizeof(int) * flags.size(), hipMemcpyDeviceToHost) !=
+        if (hipMemcpy(flags.data(), ctx->last_region_flag, sizeof(int) * flags.size(), hipMemcpyDeviceToHost) !=
             hipSuccess) {
             r->status = NSW6_HIP_ERROR;
             return NSW6_HIP_ERROR;
@@ -706,11 +840,11 @@ static int drain_timing(nsw6_ctx *ctx)
 
 int nsw6_kcell_bounds(nsw6_ctx *ctx, int lall, int32_t *out, int64_t n_out)
 {
-    if (!ctx || !out || lall <= 0 || lall > ctx->last_lall) return NSW6_BAD_ARG;
+    if (!ctx || !out || lall <= 0 || lall > ctx->last_lall || !ctx->last_bounds) return NSW6_BAD_ARG;
     const int per = ctx->last_nt * 2 * KALL * 2;
     if (n_out < (int64_t)lall * per) return NSW6_BAD_ARG;
     if (hipStreamSynchronize(ctx->last_stream) != hipSuccess) return NSW6_HIP_ERROR;
-    return hip_ok(hipMemcpy(out, ctx->ws.bounds, sizeof(int32_t) * (size_t)lall * per, hipMemcpyDeviceToHost));
+    return hip_ok(hipMemcpy(out, ctx->last_bounds, sizeof(int32_t) * (size_t)lall * per, hipMemcpyDeviceToHost));
 }
 
 int nsw6_timing_enable(nsw6_ctx *ctx, int on)
