@@ -41,6 +41,13 @@
 #define NQW_STR 1
 #define NQW_END 5
 
+/* ORA_PLANT = n (n >= 1) builds the oracle with mistake n planted, for
+ * tests/test_oracle_plants.py, which checks that the pin of each function catches its
+ * mistake.  The oracle proper is built with ORA_PLANT = 0. */
+#ifndef ORA_PLANT
+#define ORA_PLANT 0
+#endif
+
 static const double CONST_PI    = 3.141592653589793;
 static const double CONST_EPS   = 2.220446E-16;
 static const double CONST_UNDEF = -9.9999E+30;
@@ -253,7 +260,7 @@ static void thrmdyn_qd(int ijdim, const double *q, double *qd)
     for (int k = 0; k < KALL; k++)
         for (int ij = 0; ij < ijdim; ij++) {
             double v = 1.0;
-            for (int nq = NQW_STR; nq <= NQW_END; nq++) v = v - q[QIDX(nq, k, ij)];
+            for (int nq = (ORA_PLANT == 4 ? 0 : NQW_STR); nq <= NQW_END; nq++) v = v - q[QIDX(nq, k, ij)];
             qd[IDX(k, ij)] = v;
         }
 }
@@ -280,8 +287,13 @@ static void thrmdyn_tempre(int ijdim, const double *ein, const double *rho, cons
             }
             cv = cv + qd * CONST_CVdry;
             tem[IDX(k, ij)] = ein[IDX(k, ij)] / cv;
+#if ORA_PLANT == 5
+            pre[IDX(k, ij)] = rho[IDX(k, ij)] * tem[IDX(k, ij)] *
+                              (qd * CONST_Rvap + q[QIDX(I_QV, k, ij)] * CONST_Rdry);
+#else
             pre[IDX(k, ij)] = rho[IDX(k, ij)] * tem[IDX(k, ij)] *
                               (qd * CONST_Rdry + q[QIDX(I_QV, k, ij)] * CONST_Rvap);
+#endif
         }
 }
 
@@ -295,12 +307,15 @@ static void negative_filter(int ijdim, double *rhog, double *rhoge, double *rhog
     for (int k = KMIN; k <= KMAX; k++)
         for (int ij = 0; ij < ijdim; ij++) {
             double diffq = 0.0;
-            for (int nq = NQW_STR + 1; nq <= NQW_END; nq++) {
+            const int nq0 = (ORA_PLANT == 2) ? NQW_STR : NQW_STR + 1;
+            for (int nq = nq0; nq <= NQW_END; nq++) {
                 diffq += rhogq[QIDX(nq, k, ij)];
                 rhogq[QIDX(nq, k, ij)] = dmax(rhogq[QIDX(nq, k, ij)], 0.0);
             }
-            for (int nq = NQW_STR + 1; nq <= NQW_END; nq++) diffq -= rhogq[QIDX(nq, k, ij)];
+            for (int nq = nq0; nq <= NQW_END; nq++) diffq -= rhogq[QIDX(nq, k, ij)];
+#if ORA_PLANT != 1
             rhogq[QIDX(I_QV, k, ij)] += diffq;
+#endif
         }
 #pragma omp parallel for collapse(2)
     for (int k = KMIN; k <= KMAX; k++)
@@ -308,7 +323,7 @@ static void negative_filter(int ijdim, double *rhog, double *rhoge, double *rhog
             double diffq = rhogq[QIDX(I_QV, k, ij)];
             rhogq[QIDX(I_QV, k, ij)] = dmax(rhogq[QIDX(I_QV, k, ij)], 0.0);
             diffq -= rhogq[QIDX(I_QV, k, ij)];
-            rhog[IDX(k, ij)] = rhog[IDX(k, ij)] * (1.0 - diffq);
+            rhog[IDX(k, ij)] = rhog[IDX(k, ij)] * (1.0 - (ORA_PLANT == 3 ? -diffq : diffq));
             rho[IDX(k, ij)] = rhog[IDX(k, ij)] / gsgam2[IDX(k, ij)];
         }
     for (int nq = NQW_STR; nq <= NQW_END; nq++) {
@@ -447,7 +462,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     RLMDr_1br = pow(RLMDr, 4);
     RLMDr_2br = pow(RLMDr, 5);
     RLMDr_3br = pow(RLMDr, 6);
-    RLMDr_3dr = pow(RLMDr, 3) * RLMDr_dr;
+    RLMDr_3dr = pow(RLMDr, 3) * (ORA_PLANT == 14 ? 1.0 : RLMDr_dr);
     RLMDr_5dr = pow(RLMDr, 5) * RLMDr_dr;
     RLMDr_6dr = pow(RLMDr, 6) * RLMDr_dr;
 
@@ -506,7 +521,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     ROH_MOM(4.0)
     MOMs_2bs = sw_roh2014 * exp(ln10 * loga_) * exp(log(Xs2 + zerosw) * b_) * (1.0 - zerosw) +
                (1.0 - sw_roh2014) * MOMs_2bs;
-    ROH_MOM(2.25)
+    ROH_MOM(ORA_PLANT == 15 ? 2.0 : 2.25)
     MOMs_2ds = sw_roh2014 * exp(ln10 * loga_) * exp(log(Xs2 + zerosw) * b_) * (1.0 - zerosw) +
                (1.0 - sw_roh2014) * MOMs_2ds;
     ROH_MOM(1.625)
@@ -559,7 +574,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
                   (GAM_1br * RLMDr_1br * MOMs_2 + 2.0 * GAM_2br * RLMDr_2br * MOMs_1 +
                    GAM_3br * RLMDr_3br * MOMs_0);
     wk[I_Pgacr] = Ar * 0.25 * PI * Rdens * Egr * N0g * N0r * fabs(Vtg - Vtr) *
-                  (GAM_1br * RLMDr_1br * GAM_3 * RLMDg_3 + 2.0 * GAM_2br * RLMDr_2br * GAM_2 * RLMDg_2 +
+                  (GAM_1br * RLMDr_1br * GAM_3 * RLMDg_3 + (ORA_PLANT == 16 ? 1.0 : 2.0) * GAM_2br * RLMDr_2br * GAM_2 * RLMDg_2 +
                    GAM_3br * RLMDr_3br * GAM * RLMDg);
     wk[I_Pracs] = As * 0.25 * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
                   (MOMs_0bs * GAM_3 * RLMDr_3 + 2.0 * MOMs_1bs * GAM_2 * RLMDr_2 + MOMs_2bs * GAM * RLMDr);
@@ -584,9 +599,9 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     Nu = (nu0 + dnu_dT * temc) * Rdens;
     Glv = 1.0 / (LHV0 / (Ka * temp) * (LHV0 / (Rvap * temp) - 1.0) + 1.0 / (Kd * dens * qsatl));
     Giv = 1.0 / (LHS0 / (Ka * temp) * (LHS0 / (Rvap * temp) - 1.0) + 1.0 / (Kd * dens * qsati));
-    Gil = 1.0 / (LHF0 / (Ka * temc));   /* IEEE: temc == 0 -> 1/inf = 0 */
+    Gil = 1.0 / ((ORA_PLANT == 18 ? LHV0 : LHF0) / (Ka * temc));   /* IEEE: temc == 0 -> 1/inf = 0 */
 
-    ventr = f1r * GAM_2 * RLMDr_2 + f2r * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * GAM_5dr_h;
+    ventr = f1r * GAM_2 * (ORA_PLANT == 17 ? RLMDr_3 : RLMDr_2) + f2r * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * GAM_5dr_h;
     wk[I_Prevp] = 2.0 * PI * Rdens * N0r * (1.0 - dmin(Sliq, 1.0)) * Glv * ventr;
 
     rhoqi = dmax(dens * qi, EPS);
@@ -620,7 +635,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     wk[I_Pgmlt] = 2.0 * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * (wk[I_Pgacw] + wk[I_Pgacr]);
     wk[I_Pgmlt] = dmax(wk[I_Pgmlt], 0.0);
 
-    wk[I_Pgfrz] = 2.0 * PI * Rdens * N0r * 60.0 * B_frz * Ar * (exp(-A_frz * temc) - 1.0) * RLMDr_7;
+    wk[I_Pgfrz] = 2.0 * PI * Rdens * N0r * 60.0 * B_frz * Ar * (exp(-A_frz * temc) - 1.0) * (ORA_PLANT == 19 ? pow(RLMDr, 6) : RLMDr_7);
 
     dt1 = (exp(log(mi50) * ma2) - exp(log(mi40) * ma2)) / (a1 * ma2);
     Ni50 = qi * dt / (mi50 * dt1);
@@ -926,9 +941,15 @@ static void cnvvar_rhogkin_in(int ijdim, const double *rhog, const double *rhogv
 #pragma omp parallel for collapse(2)
     for (int k = KMIN + 1; k <= KMAX; k++)
         for (int g = 0; g < ijdim; g++)
+#if ORA_PLANT == 6
+            rhogkin_v[IDX(k, g)] = 0.5 * (rhogw[IDX(k, g)] * rhogw[IDX(k, g)]) /
+                                   (C2Wfact[K2 + IDX(k, g)] * rhog[IDX(k, g)] +
+                                    C2Wfact[IDX(k, g)] * rhog[IDX(k - 1, g)]);
+#else
             rhogkin_v[IDX(k, g)] = 0.5 * (rhogw[IDX(k, g)] * rhogw[IDX(k, g)]) /
                                    (C2Wfact[IDX(k, g)] * rhog[IDX(k, g)] +
                                     C2Wfact[K2 + IDX(k, g)] * rhog[IDX(k - 1, g)]);
+#endif
     for (int g = 0; g < ijdim; g++) {
         rhogkin_v[IDX(KMIN - 1, g)] = 0.0;
         rhogkin_v[IDX(KMIN, g)] = 0.0;
@@ -937,9 +958,15 @@ static void cnvvar_rhogkin_in(int ijdim, const double *rhog, const double *rhogv
 #pragma omp parallel for collapse(2)
     for (int k = KMIN; k <= KMAX; k++)
         for (int g = 0; g < ijdim; g++)
+#if ORA_PLANT == 7
+            rhogkin[IDX(k, g)] = rhogkin_h[IDX(k, g)] +
+                                 (W2Cfact[K2 + IDX(k, g)] * rhogkin_v[IDX(k + 1, g)] +
+                                  W2Cfact[IDX(k, g)] * rhogkin_v[IDX(k, g)]);
+#else
             rhogkin[IDX(k, g)] = rhogkin_h[IDX(k, g)] +
                                  (W2Cfact[IDX(k, g)] * rhogkin_v[IDX(k + 1, g)] +
                                   W2Cfact[K2 + IDX(k, g)] * rhogkin_v[IDX(k, g)]);
+#endif
     for (int g = 0; g < ijdim; g++) {
         rhogkin[IDX(KMIN - 1, g)] = 0.0;
         rhogkin[IDX(KMAX + 1, g)] = 0.0;
@@ -953,6 +980,34 @@ typedef struct region_ptrs {
     double *precip_rhophi, *precip_rhokin, *gprec, *gsgam2, *gsgam2h, *gam2, *gam2h;
     double *ix, *iy, *iz, *jx, *jy, *jz;
 } region_ptrs;
+
+/* C2Wfact [2][k][ij] and W2Cfact [2][k][ij] of precip_transport_new
+ * (src/mod_precip_transport.cpp:700-738): cell-to-wall and wall-to-cell weights. */
+static void c2w_w2c_fact(const nsw6_grid *G, int ijdim, const double *gsgam2, const double *gsgam2h,
+                         double *C2Wfact, double *W2Cfact)
+{
+    const size_t K2 = (size_t)KALL * ijdim;
+#pragma omp parallel for collapse(2)
+    for (int k = KMIN; k <= KMAX + 1; k++)
+        for (int ij = 0; ij < ijdim; ij++) {
+            C2Wfact[IDX(k, ij)] = G->afact[k] / gsgam2[IDX(k, ij)] * gsgam2h[IDX(k, ij)];
+            C2Wfact[K2 + IDX(k, ij)] = G->bfact[k] / gsgam2[IDX(k - 1, ij)] * gsgam2h[IDX(k, ij)];
+        }
+    for (int ij = 0; ij < ijdim; ij++) {
+        C2Wfact[IDX(KMIN - 1, ij)] = 0.0;
+        C2Wfact[K2 + IDX(KMIN - 1, ij)] = 0.0;
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = KMIN - 1; k <= KMAX; k++)
+        for (int ij = 0; ij < ijdim; ij++) {
+            W2Cfact[IDX(k, ij)] = G->cfact[k] * gsgam2[IDX(k, ij)] / gsgam2h[IDX(k + 1, ij)];
+            W2Cfact[K2 + IDX(k, ij)] = G->dfact[k] * gsgam2[IDX(k, ij)] / gsgam2h[IDX(k + 1, ij)];
+        }
+    for (int ij = 0; ij < ijdim; ij++) {
+        W2Cfact[IDX(KMAX + 1, ij)] = 0.0;
+        W2Cfact[K2 + IDX(KMAX + 1, ij)] = 0.0;
+    }
+}
 
 static void record_bounds(int t, int nt, int type, const int *kmin, const int *kmax)
 {
@@ -993,26 +1048,7 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
 
     for (int k = KMIN; k <= KMAX + 1; k++) GRD_gz_shift[k] = G->gz[k - 1];
 
-#pragma omp parallel for collapse(2)
-    for (int k = KMIN; k <= KMAX + 1; k++)
-        for (int ij = 0; ij < ijdim; ij++) {
-            C2Wfact[IDX(k, ij)] = G->afact[k] / R->gsgam2[IDX(k, ij)] * R->gsgam2h[IDX(k, ij)];
-            C2Wfact[K2 + IDX(k, ij)] = G->bfact[k] / R->gsgam2[IDX(k - 1, ij)] * R->gsgam2h[IDX(k, ij)];
-        }
-    for (int ij = 0; ij < ijdim; ij++) {
-        C2Wfact[IDX(KMIN - 1, ij)] = 0.0;
-        C2Wfact[K2 + IDX(KMIN - 1, ij)] = 0.0;
-    }
-#pragma omp parallel for collapse(2)
-    for (int k = KMIN - 1; k <= KMAX; k++)
-        for (int ij = 0; ij < ijdim; ij++) {
-            W2Cfact[IDX(k, ij)] = G->cfact[k] * R->gsgam2[IDX(k, ij)] / R->gsgam2h[IDX(k + 1, ij)];
-            W2Cfact[K2 + IDX(k, ij)] = G->dfact[k] * R->gsgam2[IDX(k, ij)] / R->gsgam2h[IDX(k + 1, ij)];
-        }
-    for (int ij = 0; ij < ijdim; ij++) {
-        W2Cfact[IDX(KMAX + 1, ij)] = 0.0;
-        W2Cfact[K2 + IDX(KMAX + 1, ij)] = 0.0;
-    }
+    c2w_w2c_fact(G, ijdim, R->gsgam2, R->gsgam2h, C2Wfact, W2Cfact);
 
     cnvvar_rhogkin_in(ijdim, rhog, R->rhogvx, R->rhogvy, R->rhogvz, R->rhogw, C2Wfact, W2Cfact,
                       rhogkin, rhogkin_h, rhogkin_v);
@@ -1044,7 +1080,7 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
         for (int k = KMIN; k <= KMAX; k++)
             for (int ij = 0; ij < ijdim; ij++)
                 drhoq[(size_t)nq * K2 + IDX(k, ij)] =
-                    -(fprec_q[IDX(k + 1, ij)] - fprec_q[IDX(k, ij)]) * G->rdgz[k];
+                    -(fprec_q[IDX(k + 1, ij)] - fprec_q[IDX(k, ij)]) * (ORA_PLANT == 11 ? G->rdgzh[k] : G->rdgz[k]);
 
         if (nq >= NQW_STR && nq <= NQW_END) {
 #pragma omp parallel for collapse(2)
@@ -1149,7 +1185,11 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
             rhoge[p] = rhoge[p] + drhoe[p] + drhophi[p];
             rhogkin_h[p] = rhogkin_h[p] + drhokin_h[p];
             rhogkin_v[p] = rhogkin_v[p] + drhokin_v[p];
+#if ORA_PLANT == 10
+            R->rhogvx[p] = R->rhogvx[p] + drhogu[p] * R->jx[ij] + drhogv[p] * R->ix[ij];
+#else
             R->rhogvx[p] = R->rhogvx[p] + drhogu[p] * R->ix[ij] + drhogv[p] * R->jx[ij];
+#endif
             R->rhogvy[p] = R->rhogvy[p] + drhogu[p] * R->iy[ij] + drhogv[p] * R->jy[ij];
             R->rhogvz[p] = R->rhogvz[p] + drhogu[p] * R->iz[ij] + drhogv[p] * R->jz[ij];
             R->rhogw[p] = R->rhogw[p] + drhogw[p];
@@ -1163,7 +1203,9 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
                     double d = drhoq[(size_t)nq * K2 + p];
                     rhogq[QIDX(nq, k, ij)] = rhogq[QIDX(nq, k, ij)] + d;
                     rhog[p] = rhog[p] + d;
+#if ORA_PLANT != 8
                     rhoge[p] = rhoge[p] - d * CONST_GRAV * R->z[p];
+#endif
                 }
         } else {
 #pragma omp parallel for collapse(2)
@@ -1189,7 +1231,8 @@ static void precip_transport_new(const nsw6_grid *G, int ijdim, region_ptrs *R, 
 #pragma omp parallel for collapse(2)
     for (int k = KMIN; k <= KMAX; k++)
         for (int ij = 0; ij < ijdim; ij++)
-            rhoge[IDX(k, ij)] = rhoge[IDX(k, ij)] + (tmp2[IDX(k, ij)] - tmp[IDX(k, ij)]);
+            rhoge[IDX(k, ij)] = rhoge[IDX(k, ij)] + (ORA_PLANT == 9 ? tmp[IDX(k, ij)] - tmp2[IDX(k, ij)]
+                                                                    : tmp2[IDX(k, ij)] - tmp[IDX(k, ij)]);
     } else {
     /* "KIN2KIN" (src/mod_precip_transport.cpp:1012-1068): velocities rescaled to the
      * transported kinetic energy, horizontal on cells [kmin,kmax], vertical on walls
@@ -1415,8 +1458,9 @@ static void saturation_adjustment(int ijdim, double *rhog, double *rhoge, double
         for (int k = 0; k < KALL; k++)
             for (int ij = 0; ij < ijdim; ij++) {
                 size_t p = IDX(k, ij);
-                ein_moist[p] = rhoge[p] / rhog[p] + q[QIDX(I_QV, k, ij)] * LHV;
+                ein_moist[p] = rhoge[p] / rhog[p] + (ORA_PLANT == 12 ? 0.0 : q[QIDX(I_QV, k, ij)] * LHV);
                 qsum[p] = q[QIDX(I_QV, k, ij)] + q[QIDX(I_QC, k, ij)];
+                if (ORA_PLANT == 13 && (k < KMIN || k > KMAX)) continue;
                 q[QIDX(I_QV, k, ij)] = qsum[p];
                 q[QIDX(I_QC, k, ij)] = 0.0;
             }
@@ -1655,6 +1699,27 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
     free(ma2); free(Vt); free(cva); free(rgs); free(rgsh); free(rctop_cld); free(tctop_cld);
 }
 
+/* Region l of the arrays of nsw6_fields. */
+static region_ptrs region_of(const nsw6_fields *f, int l, int ijdim)
+{
+    const size_t K2 = (size_t)KALL * ijdim;
+    region_ptrs R;
+    size_t o2 = (size_t)l * K2, o3 = (size_t)l * NQMAX * K2, o1 = (size_t)l * ijdim;
+    R.rhog = f->rhog + o2; R.rhogvx = f->rhogvx + o2; R.rhogvy = f->rhogvy + o2;
+    R.rhogvz = f->rhogvz + o2; R.rhogw = f->rhogw + o2; R.rhoge = f->rhoge + o2;
+    R.rhogq = f->rhogq + o3; R.rho = f->rho + o2; R.tem = f->tem + o2; R.pre = f->pre + o2;
+    R.vx = f->vx + o2; R.vy = f->vy + o2; R.vz = f->vz + o2; R.w = f->w ? f->w + o2 : NULL; R.q = f->q + o3;
+    R.unccn = f->unccn ? f->unccn + o2 : NULL;
+    R.qd = f->qd + o2; R.z = f->z + o2; R.precip = f->precip + (size_t)l * 2 * ijdim;
+    R.precip_rhoe = f->precip_rhoe + o1; R.precip_lh_heat = f->precip_lh_heat + o1;
+    R.precip_rhophi = f->precip_rhophi + o1; R.precip_rhokin = f->precip_rhokin + o1;
+    R.gprec = f->GPREC + o2; R.gsgam2 = f->gsgam2 + o2; R.gsgam2h = f->gsgam2h + o2;
+    R.gam2 = f->gam2 + o2; R.gam2h = f->gam2h + o2;
+    R.ix = f->ix + o1; R.iy = f->iy + o1; R.iz = f->iz + o1;
+    R.jx = f->jx + o1; R.jy = f->jy + o1; R.jz = f->jz + o1;
+    return R;
+}
+
 /* ---- mp_driver: src/mod_mp_driver.cpp:335-613 (MP_DIV_NUM = 1) ---------------- */
 int ora_mp_driver(const nsw6_params *P, const nsw6_grid *G, int lall, int ijdim,
                   const nsw6_fields *f, double dt, double ct, int ice_adjust, nsw6_report *rep)
@@ -1688,20 +1753,8 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
     double elapsed = 0.0;
 
     for (int l = 0; l < lall; l++) {
-        region_ptrs R;
-        size_t o2 = (size_t)l * K2, o3 = (size_t)l * NQMAX * K2, o1 = (size_t)l * ijdim;
-        R.rhog = f->rhog + o2; R.rhogvx = f->rhogvx + o2; R.rhogvy = f->rhogvy + o2;
-        R.rhogvz = f->rhogvz + o2; R.rhogw = f->rhogw + o2; R.rhoge = f->rhoge + o2;
-        R.rhogq = f->rhogq + o3; R.rho = f->rho + o2; R.tem = f->tem + o2; R.pre = f->pre + o2;
-        R.vx = f->vx + o2; R.vy = f->vy + o2; R.vz = f->vz + o2; R.w = f->w + o2; R.q = f->q + o3;
-        R.unccn = f->unccn ? f->unccn + o2 : NULL;
-        R.qd = f->qd + o2; R.z = f->z + o2; R.precip = f->precip + (size_t)l * 2 * ijdim;
-        R.precip_rhoe = f->precip_rhoe + o1; R.precip_lh_heat = f->precip_lh_heat + o1;
-        R.precip_rhophi = f->precip_rhophi + o1; R.precip_rhokin = f->precip_rhokin + o1;
-        R.gprec = f->GPREC + o2; R.gsgam2 = f->gsgam2 + o2; R.gsgam2h = f->gsgam2h + o2;
-        R.gam2 = f->gam2 + o2; R.gam2h = f->gam2h + o2;
-        R.ix = f->ix + o1; R.iy = f->iy + o1; R.iz = f->iz + o1;
-        R.jx = f->jx + o1; R.jy = f->jy + o1; R.jz = f->jz + o1;
+        region_ptrs R = region_of(f, l, ijdim);
+        size_t o2 = (size_t)l * K2, o1 = (size_t)l * ijdim;
 
         double *precip_sum = zalloc(2 * (size_t)ijdim);
         double *prh_sum = zalloc(ijdim), *plh_sum = zalloc(ijdim), *pphi_sum = zalloc(ijdim),
@@ -1763,6 +1816,70 @@ int ora_mp_driver_div(const nsw6_params *P, const nsw6_grid *G, int lall, int ij
     else if (rep->n_not_converged) rep->status = NSW6_NOT_CONVERGED;
     else rep->status = NSW6_OK;
     return rep->status;
+}
+
+/* ---- single phases of the step, for the pins of tests/ (one region, [k][ij]) ------- */
+void ora_thrmdyn(int ijdim, const double *q, const double *ein, const double *rho, double *qd, double *cv,
+                 double *tem, double *pre)
+{
+    thrmdyn_qd(ijdim, q, qd);
+    thrmdyn_cv(ijdim, qd, q, cv);
+    thrmdyn_tempre(ijdim, ein, rho, q, tem, pre);
+}
+
+void ora_negative_filter(int ijdim, double *rhog, double *rhoge, double *rhogq, double *rho, const double *tem,
+                         double *pre, double *q, const double *gsgam2)
+{
+    negative_filter(ijdim, rhog, rhoge, rhogq, rho, tem, pre, q, gsgam2);
+}
+
+void ora_rhogkin(const nsw6_grid *G, int ijdim, const double *rhog, const double *rhogvx, const double *rhogvy,
+                 const double *rhogvz, const double *rhogw, const double *gsgam2, const double *gsgam2h,
+                 double *rhogkin, double *rhogkin_h, double *rhogkin_v)
+{
+    const size_t K2 = (size_t)KALL * ijdim;
+    double *C2W = zalloc(2 * K2), *W2C = zalloc(2 * K2);
+    c2w_w2c_fact(G, ijdim, gsgam2, gsgam2h, C2W, W2C);
+    cnvvar_rhogkin_in(ijdim, rhog, rhogvx, rhogvy, rhogvz, rhogw, C2W, W2C, rhogkin, rhogkin_h, rhogkin_v);
+    free(C2W);
+    free(W2C);
+}
+
+/* precip_transport_new on region 0 of `f` (in place) with the fall speeds Vterm
+ * [6][k][ij], after mp_driver's zeroing of the surface outputs and GPREC, with rgs and
+ * rgsh as TagUpdateTEM forms them (interior levels). */
+void ora_precip_transport(const nsw6_params *P, const nsw6_grid *G, int ijdim, const nsw6_fields *f,
+                          const double *Vterm, double dt)
+{
+    const size_t K2 = (size_t)KALL * ijdim;
+    region_ptrs R = region_of(f, 0, ijdim);
+    for (size_t i = 0; i < 2 * (size_t)ijdim; i++) R.precip[i] = 0.0;
+    for (int ij = 0; ij < ijdim; ij++) {
+        R.precip_rhoe[ij] = 0.0; R.precip_lh_heat[ij] = 0.0;
+        R.precip_rhophi[ij] = 0.0; R.precip_rhokin[ij] = 0.0;
+    }
+    for (size_t i = 0; i < K2; i++) R.gprec[i] = 0.0;
+    double *rgs = zalloc(K2), *rgsh = zalloc(K2);
+    for (int k = KMIN; k <= KMAX; k++)
+        for (int ij = 0; ij < ijdim; ij++) {
+            rgs[IDX(k, ij)] = R.gam2[IDX(k, ij)] / R.gsgam2[IDX(k, ij)];
+            rgsh[IDX(k, ij)] = R.gam2h[IDX(k, ij)] / R.gsgam2h[IDX(k, ij)];
+        }
+    int flags[NQMAX] = {0, 0, 1, P->four_water != 0.0, 1, 1};
+    precip_transport_new(G, ijdim, &R, Vterm, flags, rgs, rgsh, dt, P->kin2kin != 0.0);
+    free(rgs);
+    free(rgsh);
+}
+
+/* SATURATION_adjustment on one region (in place) with the range mp_nsw6 sets for the
+ * scheme (src/mod_mp_nsw6.cpp:647-660). */
+void ora_saturation_adjustment(const nsw6_params *P, int ijdim, double *rhog, double *rhoge, double *rhogq,
+                               double *tem, double *q, double *qd, double *gsgam2)
+{
+    const int ice_adjust = (P->sw_expice != 1.0);
+    g_sat_ulimit = ice_adjust ? 273.16 : 100.0;
+    g_sat_llimit = ice_adjust ? 233.16 : 90.0;
+    saturation_adjustment(ijdim, rhog, rhoge, rhogq, tem, q, qd, gsgam2, ice_adjust, NULL, 0);
 }
 
 double ora_last_elapsed(void) { return g_elapsed; }

@@ -89,6 +89,25 @@ void ora_record_kcell_bounds(int *buf);
  * only; the caller initialises the halo levels. */
 void ora_record_newton_margin(double *buf);
 
+/* Single phases of the step on one region ([k][ij] arrays, tracers [6][k][ij]), for
+ * the pins of the tests: THRMDYN_qd/cv/tempre (src/mod_thrmdyn.cpp:98-180);
+ * negative_filter (src/mod_mp_nsw6.cpp:1750-1832, in place); cnvvar_rhogkin_in
+ * (src/mod_debug.cpp:269-330) with precip_transport_new's C2W/W2C factors;
+ * precip_transport_new (src/mod_precip_transport.cpp:573-1087) on region 0 of `f` in
+ * place, Vterm [6][k][ij]; SATURATION_adjustment (src/mod_satadjust.cpp:274-362) in
+ * place, with the scheme's range. */
+void ora_thrmdyn(int ijdim, const double *q, const double *ein, const double *rho, double *qd, double *cv,
+                 double *tem, double *pre);
+void ora_negative_filter(int ijdim, double *rhog, double *rhoge, double *rhogq, double *rho, const double *tem,
+                         double *pre, double *q, const double *gsgam2);
+void ora_rhogkin(const nsw6_grid *G, int ijdim, const double *rhog, const double *rhogvx, const double *rhogvy,
+                 const double *rhogvz, const double *rhogw, const double *gsgam2, const double *gsgam2h,
+                 double *rhogkin, double *rhogkin_h, double *rhogkin_v);
+void ora_precip_transport(const nsw6_params *P, const nsw6_grid *G, int ijdim, const nsw6_fields *f,
+                          const double *Vterm, double dt);
+void ora_saturation_adjustment(const nsw6_params *P, int ijdim, double *rhog, double *rhoge, double *rhogq,
+                               double *tem, double *q, double *qd, double *gsgam2);
+
 /* Seconds of the last ora_mp_driver call (wall clock, like the reference's
  * "NSW6 Elapsed Time", src/mod_mp_driver.cpp:500-552). */
 double ora_last_elapsed(void);

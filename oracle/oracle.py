@@ -29,51 +29,72 @@ def build(verbose: bool = False) -> None:
         print(res.stdout)
 
 
+def declare(L):
+    """ctypes signatures of liboracle_nsw6.so's entry points on a loaded library `L`
+    (the oracle, or a build of it with a planted mistake)."""
+    from nicam_kokkos_microphysics_amd import _native as N
+    P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    L.ora_gammafunc.restype = D
+    L.ora_gammafunc.argtypes = [D]
+    L.ora_params_init.argtypes = [ctypes.POINTER(N.Params)]
+    L.ora_params_init.restype = None
+    L.ora_grid_setup.argtypes = [P, P, ctypes.POINTER(N.Grid)]
+    L.ora_grid_setup.restype = None
+    L.ora_psat_liq.restype = D
+    L.ora_psat_liq.argtypes = [D]
+    L.ora_psat_ice.restype = D
+    L.ora_psat_ice.argtypes = [D]
+    L.ora_bergeron_point.argtypes = [D, P, P, P]
+    L.ora_bergeron_point.restype = None
+    L.ora_bigloop_point.argtypes = [ctypes.POINTER(N.Params), D, D, D, P, D, D, D, D, D, D, D, D, P]
+    L.ora_bigloop_point.restype = None
+    L.ora_vadv1d_prep.argtypes = [I, I, I, P, P, P, P, P, P, P, D]
+    L.ora_vadv1d_prep.restype = None
+    L.ora_vadv1d_getflux.argtypes = [I, I, I, P, P, P, P, P, P, P]
+    L.ora_vadv1d_getflux.restype = None
+    L.ora_satadjust_liq_point.argtypes = [D, D, D, D, P, P, P, P]
+    L.ora_satadjust_liq_point.restype = I
+    L.ora_satadjust_all_point.argtypes = [D, D, D, D, P, D, D, P, P, P, P]
+    L.ora_satadjust_all_point.restype = I
+    L.ora_mp_driver.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
+                                ctypes.POINTER(N.FieldPtrs), D, D, I, ctypes.POINTER(N.Report)]
+    L.ora_mp_driver.restype = I
+    L.ora_mp_driver_div.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
+                                    ctypes.POINTER(N.FieldPtrs), D, D, I, I, ctypes.POINTER(N.Report)]
+    L.ora_mp_driver_div.restype = I
+    L.ora_record_kcell_bounds.argtypes = [P]
+    L.ora_record_kcell_bounds.restype = None
+    L.ora_record_newton_margin.argtypes = [P]
+    L.ora_record_newton_margin.restype = None
+    L.ora_thrmdyn.argtypes = [I, P, P, P, P, P, P, P]
+    L.ora_thrmdyn.restype = None
+    L.ora_negative_filter.argtypes = [I, P, P, P, P, P, P, P, P]
+    L.ora_negative_filter.restype = None
+    L.ora_rhogkin.argtypes = [ctypes.POINTER(N.Grid), I] + [P] * 10
+    L.ora_rhogkin.restype = None
+    L.ora_precip_transport.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I,
+                                       ctypes.POINTER(N.FieldPtrs), P, D]
+    L.ora_precip_transport.restype = None
+    L.ora_saturation_adjustment.argtypes = [ctypes.POINTER(N.Params), I] + [P] * 7
+    L.ora_saturation_adjustment.restype = None
+    L.ora_last_elapsed.restype = D
+    L.ora_last_elapsed.argtypes = []
+    L.ora_num_threads.restype = I
+    L.ora_num_threads.argtypes = []
+    return L
+
+
+def load(path: str):
+    """A separately loaded, declared copy of an oracle build at `path`."""
+    return declare(ctypes.CDLL(path))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        from nicam_kokkos_microphysics_amd import _native as N
-        L = ctypes.CDLL(LIB)
-        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
-        L.ora_gammafunc.restype = D
-        L.ora_gammafunc.argtypes = [D]
-        L.ora_params_init.argtypes = [ctypes.POINTER(N.Params)]
-        L.ora_params_init.restype = None
-        L.ora_grid_setup.argtypes = [P, P, ctypes.POINTER(N.Grid)]
-        L.ora_grid_setup.restype = None
-        L.ora_psat_liq.restype = D
-        L.ora_psat_liq.argtypes = [D]
-        L.ora_psat_ice.restype = D
-        L.ora_psat_ice.argtypes = [D]
-        L.ora_bergeron_point.argtypes = [D, P, P, P]
-        L.ora_bergeron_point.restype = None
-        L.ora_bigloop_point.argtypes = [ctypes.POINTER(N.Params), D, D, D, P, D, D, D, D, D, D, D, D, P]
-        L.ora_bigloop_point.restype = None
-        L.ora_vadv1d_prep.argtypes = [I, I, I, P, P, P, P, P, P, P, D]
-        L.ora_vadv1d_prep.restype = None
-        L.ora_vadv1d_getflux.argtypes = [I, I, I, P, P, P, P, P, P, P]
-        L.ora_vadv1d_getflux.restype = None
-        L.ora_satadjust_liq_point.argtypes = [D, D, D, D, P, P, P, P]
-        L.ora_satadjust_liq_point.restype = I
-        L.ora_satadjust_all_point.argtypes = [D, D, D, D, P, D, D, P, P, P, P]
-        L.ora_satadjust_all_point.restype = I
-        L.ora_mp_driver.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
-                                    ctypes.POINTER(N.FieldPtrs), D, D, I, ctypes.POINTER(N.Report)]
-        L.ora_mp_driver.restype = I
-        L.ora_mp_driver_div.argtypes = [ctypes.POINTER(N.Params), ctypes.POINTER(N.Grid), I, I,
-                                        ctypes.POINTER(N.FieldPtrs), D, D, I, I, ctypes.POINTER(N.Report)]
-        L.ora_mp_driver_div.restype = I
-        L.ora_record_kcell_bounds.argtypes = [P]
-        L.ora_record_kcell_bounds.restype = None
-        L.ora_record_newton_margin.argtypes = [P]
-        L.ora_record_newton_margin.restype = None
-        L.ora_last_elapsed.restype = D
-        L.ora_last_elapsed.argtypes = []
-        L.ora_num_threads.restype = I
-        L.ora_num_threads.argtypes = []
-        _lib = L
+        _lib = declare(ctypes.CDLL(LIB))
     return _lib
 
 
@@ -122,10 +143,10 @@ def mp_driver(state: dict, gz, gzh, dt: float, ct: float = 0.0, P=None, ice_adju
     return rep
 
 
-def bigloop_point(P, dt, dens, temp, q6, qsatl, qsati, pre, rhog, Nc, a1, a2, ma2):
+def bigloop_point(P, dt, dens, temp, q6, qsatl, qsati, pre, rhog, Nc, a1, a2, ma2, L=None):
     o = BigLoopOut()
     q = np.ascontiguousarray(q6, dtype=np.float64)
-    lib().ora_bigloop_point(ctypes.byref(P), dt, dens, temp, q.ctypes.data, qsatl, qsati, pre, rhog, Nc, a1, a2,
+    (L or lib()).ora_bigloop_point(ctypes.byref(P), dt, dens, temp, q.ctypes.data, qsatl, qsati, pre, rhog, Nc, a1, a2,
                             ma2, ctypes.byref(o))
     return o
 
@@ -156,6 +177,56 @@ def mp_driver_traced(state: dict, gz, gzh, dt: float, P=None, nt: int = 3, bound
         L.ora_record_kcell_bounds(None)
         L.ora_record_newton_margin(None)
     return rep, b, m
+
+
+# ---------------------------------------------------------------------------------
+# single phases of the step on one region ([k][ij], tracers [6][k][ij]); L = a build
+# of the oracle (default: the oracle)
+# ---------------------------------------------------------------------------------
+def _c(a):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
+    return a.ctypes.data
+
+
+def thrmdyn(q, ein, rho, L=None):
+    """THRMDYN_qd/cv/tempre: returns (qd, cv, tem, pre)."""
+    ijdim = q.shape[-1]
+    qd, cv, tem, pre = (np.zeros_like(ein) for _ in range(4))
+    (L or lib()).ora_thrmdyn(ijdim, _c(q), _c(ein), _c(rho), _c(qd), _c(cv), _c(tem), _c(pre))
+    return qd, cv, tem, pre
+
+
+def negative_filter(rhog, rhoge, rhogq, rho, tem, pre, q, gsgam2, L=None):
+    """negative_filter in place on one region."""
+    (L or lib()).ora_negative_filter(rhog.shape[-1], _c(rhog), _c(rhoge), _c(rhogq), _c(rho), _c(tem), _c(pre),
+                                     _c(q), _c(gsgam2))
+
+
+def rhogkin(gz, gzh, rhog, rhogvx, rhogvy, rhogvz, rhogw, gsgam2, gsgam2h, L=None):
+    """cnvvar_rhogkin_in: returns (rhogkin, rhogkin_h, rhogkin_v)."""
+    G = grid(gz, gzh)
+    out = [np.zeros_like(rhog) for _ in range(3)]
+    (L or lib()).ora_rhogkin(ctypes.byref(G), rhog.shape[-1], _c(rhog), _c(rhogvx), _c(rhogvy), _c(rhogvz),
+                             _c(rhogw), _c(gsgam2), _c(gsgam2h), *[_c(o) for o in out])
+    return tuple(out)
+
+
+def precip_transport(state: dict, gz, gzh, Vterm, dt: float, P=None, L=None):
+    """precip_transport_new in place on region 0 of a host state; Vterm [6][96][ij]."""
+    from nicam_kokkos_microphysics_amd.fields import field_ptrs
+    P = params() if P is None else P
+    G = grid(gz, gzh)
+    fp = field_ptrs(state)
+    V = np.ascontiguousarray(Vterm, dtype=np.float64)
+    (L or lib()).ora_precip_transport(ctypes.byref(P), ctypes.byref(G), state["rhog"].shape[-1], ctypes.byref(fp),
+                                      V.ctypes.data, float(dt))
+
+
+def saturation_adjustment(rhog, rhoge, rhogq, tem, q, qd, gsgam2, P=None, L=None):
+    """SATURATION_adjustment in place on one region."""
+    P = params() if P is None else P
+    (L or lib()).ora_saturation_adjustment(ctypes.byref(P), rhog.shape[-1], _c(rhog), _c(rhoge), _c(rhogq),
+                                           _c(tem), _c(q), _c(qd), _c(gsgam2))
 
 
 def last_elapsed() -> float:
