@@ -108,6 +108,11 @@ typedef struct nsw6_params {
      * Vt(QI) = Vti of src/mod_mp_nsw6.cpp:1073-1075; sw_constVti = 1 with CONST_Vti > 0
      * forces the constant fall speed -CONST_Vti, as mp_nsw6_init sets it, :399-407). */
     double four_water;
+    /* dtemp_criteria of the saturation adjustment's Newton (src/mod_satadjust.cpp:788):
+     * it stops when |dtemp| < this [K]; the reference fixes 10^(-(RP_PREC+1)/2) = 1e-8
+     * (RP_PREC = 15, integer division).  A caller may tighten it; 0 makes every Newton
+     * point run to itelim = 100 and report NSW6_NOT_CONVERGED. */
+    double satadjust_dtemp_criteria;
 } nsw6_params;
 
 /* The 8 tunable NSW6 coefficients, in this order, for gradients and perturbations:

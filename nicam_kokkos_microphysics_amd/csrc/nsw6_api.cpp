@@ -52,7 +52,7 @@ struct nsw6_ctx {
     GridDev *grid_dev = nullptr;
     Workspace ws{};
     hipStream_t last_stream = nullptr;
-    int last_lall = 0;
+    int last_lall = 0, last_ijdim = 0;
     int last_nt = 3;                 // sedimenting tracers of the last step (kcell bounds layout)
     // where the last call left its report, region flags and kcell bounds: the
     // context's workspace, or the allocation-free call's scratch
@@ -340,6 +340,7 @@ static int run_steps(nsw6_ctx *ctx, StepArgs &a, int lall, double dt, double *di
     fill_step_args(ctx, lall, a.ijdim, dt * fraction_mp, a);
     ctx->last_stream = s;
     ctx->last_lall = lall;
+    ctx->last_ijdim = a.ijdim;
     ctx->last_nt = a.four_water ? 4 : 3;
     ctx->last_adjoint = false;
     ctx->last_report = a.ws.report;
@@ -739,6 +740,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
     if (ctx->timing) ctx->pending_adj.push_back(pt);
     ctx->last_stream = s;
     ctx->last_lall = lall;
+    ctx->last_ijdim = ijdim;
     ctx->last_nt = 3;
     ctx->last_adjoint = true;
     ctx->last_report = ctx->ws.report;
@@ -790,9 +792,14 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *r)
     }
     r->n_not_converged = rep[1];
     r->n_nan = rep[2];
-    r->first_l = rep[3];
-    r->first_k = rep[4];
-    r->first_ij = rep[5];
+    unsigned long long key;
+    memcpy(&key, rep + REPORT_KEY, sizeof key);
+    if (key != ~0ULL && ctx->last_ijdim > 0) {
+        const unsigned long long per_l = (unsigned long long)KALL * (unsigned long long)ctx->last_ijdim;
+        r->first_l = (int32_t)(key / per_l);
+        r->first_k = (int32_t)((key % per_l) / (unsigned long long)ctx->last_ijdim);
+        r->first_ij = (int32_t)(key % (unsigned long long)ctx->last_ijdim);
+    }
     if (ctx->last_lall > 0) {
         std::vector<int> flags((size_t)ctx->last_lall);
         if (hipMemcpy(flags.data(), ctx->last_region_flag, sizeof(int) * flags.size(), hipMemcpyDeviceToHost) !=

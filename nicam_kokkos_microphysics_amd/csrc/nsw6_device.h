@@ -92,7 +92,7 @@ struct DevParams {
     double Nc_pow_m179;        // pow(Nc_def, -1.79)
     double Dc;                 // 0.146 - 5.964e-2 * log(Nc_def / 2000)
     double LovR_liq, LovR_ice, RTEM00;
-    double dtemp_criteria;     // 10^(-(15+1)/2) = 1e-8
+    double dtemp_criteria;     // the Newton's stopping test; the reference's 10^(-(15+1)/2) = 1e-8
     // reciprocals and ratios of parameters, formed once on the host so that the
     // point-wise code multiplies instead of dividing
     double rK_r, rK_s, rK_g;    // 1/(Ar N0r GAM_1br), 1/(As N0s GAM_1bs), 1/(Ag N0g GAM_1bg)
@@ -163,7 +163,10 @@ NSW6_HD void bergeron(const double *a1_tab, const double *a2_tab, double tem,
                                          double &a1, double &a2, double &ma2)
 {
     double temc = dmin(dmax(tem - C_TEM00, -30.99), 0.0);
-    int itemc = (int)(-temc);
+    // the table index from a NaN-free copy: the same index for every finite temperature,
+    // and a NaN temperature (which the clamps above pass through) cannot index outside
+    // the tables; its NaN still reaches a1, a2 through `fact`
+    int itemc = (int)(-((temc == temc) ? temc : 0.0));
     double fact = -(temc + (double)(itemc - 1));
     double v1 = (1.0 - fact) * a1_tab[itemc] + (fact)*a1_tab[itemc + 1];
     double v2 = (1.0 - fact) * a2_tab[itemc] + (fact)*a2_tab[itemc + 1];

@@ -104,6 +104,8 @@ void nsw6_params_default(nsw6_params *p)
 
     p->kk_aut_rate = 1350.0;
     p->icenuc_coef = 4.92E-11;
+    /* dtemp_criteria = 10^(-(RP_PREC+1)/2), integer division (src/mod_satadjust.cpp:788) */
+    p->satadjust_dtemp_criteria = pow(10.0, (double)(-(15 + 1) / 2));
 
     {
         double Dc = 0.146 - 5.964E-2 * log(p->Nc_def / 2000.0);

@@ -70,8 +70,11 @@ __host__ __device__ constexpr int ext_index(int t, int type, int k, int mm)
     return ((t * 2 + type) * KALL + k) * 2 + mm;
 }
 __host__ __device__ constexpr int bounds_index(int t, int type, int k, int mm) { return ext_index(t, type, k, mm); }
-// report ints: [0] any failure, [1] not converged, [2] NaN, [3] l, [4] k, [5] ij
-constexpr int REPORT_INTS = 6;
+// report ints: [0] any failure, [1] not converged, [2] NaN, [3..5] unused, [6..7] the
+// first failing point as the 64-bit key (l * KALL + k) * ijdim + ij (atomicMin, so the
+// lexicographically first (l, k, ij) whatever the thread order; all ones = none)
+constexpr int REPORT_INTS = 8;
+constexpr int REPORT_KEY = 6;   // int offset of the key (8-byte aligned)
 
 // XCD-aware block order for the stencil kernels.  MI355X deals consecutive
 // workgroups round-robin to its 8 XCDs, each with its own L2.  A 1-D grid of

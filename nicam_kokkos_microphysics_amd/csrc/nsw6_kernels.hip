@@ -516,11 +516,9 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
     const TailStatus st = tail(*a.P, interior, x, y);
     if (st.newton < 0) {
         const int code = (st.newton == -1) ? 2 : 1;
-        if (atomicAdd(&a.ws.report[code], 1) == 0) {
-            a.ws.report[3] = l;
-            a.ws.report[4] = k;
-            a.ws.report[5] = ij;
-        }
+        atomicAdd(&a.ws.report[code], 1);
+        const unsigned long long key = ((unsigned long long)l * KALL + (unsigned long long)k) * ijdim + ij;
+        atomicMin(reinterpret_cast<unsigned long long *>(a.ws.report + REPORT_KEY), key);
         atomicOr(&a.ws.report[0], 1);
     }
     f.rhog[p] = y.rhog;

@@ -46,7 +46,7 @@ inline void to_dev_params(const nsw6_params &p, DevParams &d)
     d.LovR_liq = C_LHV0 / C_Rvap;
     d.LovR_ice = C_LHS0 / C_Rvap;
     d.RTEM00 = 1.0 / C_TEM00;
-    d.dtemp_criteria = pow(10.0, (double)(-(15 + 1) / 2));
+    d.dtemp_criteria = p.satadjust_dtemp_criteria;   // the reference: 10^(-(15+1)/2) = 1e-8
     d.rK_r = 1.0 / (p.Ar * p.N0r * p.GAM_1br);
     d.rK_s = 1.0 / (p.As * p.N0s * p.GAM_1bs);
     d.rK_g = 1.0 / (p.Ag * p.N0g * p.GAM_1bg);

@@ -47,6 +47,33 @@ def grid_setup(gz, gzh) -> N.Grid:
     return g
 
 
+def check_device_tensors(arrays: dict, lall: int, ijdim: int, device: int, what: str = "state"):
+    """Every non-None entry must be a contiguous float64 tensor on HIP device `device`
+    with its mp_driver shape: the C ABI hands these pointers to kernels, which would
+    fault on a host pointer and read past a short or narrower array."""
+    import torch
+    for k, v in arrays.items():
+        if v is None:
+            continue
+        if not isinstance(v, torch.Tensor) or not v.is_cuda:
+            raise TypeError(f"{what}[{k}]: must be a HIP device tensor, not {type(v).__name__}"
+                            + ("" if not isinstance(v, torch.Tensor) else " on the CPU"))
+        if v.device.index != device:
+            raise ValueError(f"{what}[{k}]: on cuda:{v.device.index}, the context is on cuda:{device}")
+        if v.dtype != torch.float64 or not v.is_contiguous():
+            raise ValueError(f"{what}[{k}]: must be contiguous float64")
+        if tuple(v.shape) != shape_of(k, lall, ijdim):
+            raise ValueError(f"{what}[{k}]: shape {tuple(v.shape)} != {shape_of(k, lall, ijdim)}")
+
+
+def _check_vector(v, shape, device, what):
+    import torch
+    if not isinstance(v, torch.Tensor) or not v.is_cuda or v.device.index != device:
+        raise TypeError(f"{what}: must be a HIP device tensor on cuda:{device}")
+    if v.dtype != torch.float64 or not v.is_contiguous() or tuple(v.shape) != tuple(shape):
+        raise ValueError(f"{what}: must be contiguous float64 of shape {tuple(shape)}")
+
+
 def _stream_handle(stream):
     if stream is None:
         import torch
@@ -71,6 +98,8 @@ class NSW6:
         if rc != 0:
             raise NSW6Error(rc)
         self._ctx = ctx
+        import torch
+        self.device = torch.cuda.current_device()   # nsw6_init took the current device
         self.set_precision(precision)
         self.set_div_num(div_num)
 
@@ -100,9 +129,7 @@ class NSW6:
         """Run mp_driver on a dict of device tensors (keys = the 50 argument names)."""
         rhog = state["rhog"]
         lall, ijdim = int(rhog.shape[0]), int(rhog.shape[-1])
-        for k, v in state.items():
-            if v is not None and tuple(v.shape) != shape_of(k, lall, ijdim):
-                raise ValueError(f"{k}: shape {tuple(v.shape)} != {shape_of(k, lall, ijdim)}")
+        check_device_tensors(state, lall, ijdim, self.device)
         fp = field_ptrs(state)
         rc = self._L.nsw6_mp_driver_fields(self._ctx, l_region, lall, ijdim, ctypes.byref(fp), float(dt),
                                            float(ct), _stream_handle(stream))
@@ -132,18 +159,18 @@ class NSW6:
         import torch
         rhog = state["rhog"]
         lall, ijdim = int(rhog.shape[0]), int(rhog.shape[-1])
-        for d in (state, cot):
-            for k, v in d.items():
-                if v is not None and tuple(v.shape) != shape_of(k, lall, ijdim):
-                    raise ValueError(f"{k}: shape {tuple(v.shape)} != {shape_of(k, lall, ijdim)}")
-        if cot_sflux is not None and tuple(cot_sflux.shape) != (lall, 3, ijdim):
-            raise ValueError(f"cot_sflux: shape {tuple(cot_sflux.shape)} != {(lall, 3, ijdim)}")
+        check_device_tensors(state, lall, ijdim, self.device, "state")
+        check_device_tensors(cot, lall, ijdim, self.device, "cot")
+        if cot_sflux is not None:
+            _check_vector(cot_sflux, (lall, 3, ijdim), self.device, "cot_sflux")
         if grad is None:
             grad = {k: torch.empty_like(state[k]) for k in ADJ_INPUTS}
             if state.get("unccn") is not None:
                 grad["unccn"] = torch.empty_like(state["unccn"])
+        check_device_tensors(grad, lall, ijdim, self.device, "grad")
         if grad_coef is None:
             grad_coef = torch.empty(N.NCOEF, dtype=torch.float64, device=rhog.device)
+        _check_vector(grad_coef, (N.NCOEF,), self.device, "grad_coef")
         fs, fc, fg = field_ptrs(state), field_ptrs(cot), field_ptrs(grad)
         cs = None if cot_sflux is None else ctypes.c_void_p(cot_sflux.data_ptr())
         rc = self._L.nsw6_mp_driver_adj(self._ctx, l_region, lall, ijdim, ctypes.byref(fs), ctypes.byref(fc), cs,

@@ -96,6 +96,9 @@ static int g_bounds_l = 0;
  * point g_newton_rec[l][k][ij] the relative distance of the Newton's stopping test
  * from its criterion over the last two iterations (see ora_record_newton_margin). */
 static double *g_newton_rec = NULL;
+/* the Newton's dtemp_criteria of the current step (nsw6_params.satadjust_dtemp_criteria;
+ * the reference's 10^(-(RP_PREC+1)/2) = 1e-8, src/mod_satadjust.cpp:788) */
+static double g_dtemp_criteria = 1.0e-8;
 
 /* Stopping margin of a Newton that ran `n` iterations, the last two steps being
  * d_prev (only when n >= 2) and d_last: min over those of | |d| - crit | / crit. */
@@ -204,6 +207,7 @@ void ora_params_init(nsw6_params *p)
     /* literals of the rate code exposed as tunables */
     p->kk_aut_rate = 1350.0;
     p->icenuc_coef = 4.92E-11;
+    p->satadjust_dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));   /* = 1e-8 */
     /* mp_nsw6_new::init effective-radius work (src/mod_mp_nsw6.cpp:511-516) */
     {
         double Dc = 0.146 - 5.964E-2 * log(p->Nc_def / 2000.0);
@@ -360,7 +364,9 @@ static const double a2_tab[32] = {
 void ora_bergeron_point(double tem, double *a1, double *a2, double *ma2)
 {
     double temc = dmin(dmax(tem - CONST_TEM00, -30.99), 0.0);
-    int itemc = (int)(-temc);
+    /* int(-temc) of a NaN is undefined (the reference indexes the tables with it); the
+     * index is taken from a NaN-free copy, the NaN still reaches a1, a2 through fact */
+    int itemc = (int)(-((temc == temc) ? temc : 0.0));
     double fact = -(temc + (double)(itemc - 1));   /* QUIRK: off-by-one kept (A-5) */
     double v1 = (1.0 - fact) * a1_tab[itemc] + (fact)*a1_tab[itemc + 1];
     double v2 = (1.0 - fact) * a2_tab[itemc] + (fact)*a2_tab[itemc + 1];
@@ -1308,7 +1314,7 @@ static int satadjust_liq_core(double rho, double Emoist, double qsum, double qd_
                               double *tem, double *qv, double *qc, double *margin)
 {
     const double RTEM00 = 1.0 / CONST_TEM00;
-    const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));  /* = 1e-8 */
+    const double dtemp_criteria = g_dtemp_criteria;   /* 10^(-(RP_PREC+1)/2) = 1e-8 by default */
     const int itelim = 100;
     double qq[6];
     for (int i = 0; i < 6; i++) qq[i] = q6[i];
@@ -1371,7 +1377,7 @@ static int satadjust_all_core(double rho, double Emoist, double qsum, double qd_
                               double *margin)
 {
     const double RTEM00 = 1.0 / CONST_TEM00;
-    const double dtemp_criteria = pow(10.0, (double)(-(RP_PREC + 1) / 2));
+    const double dtemp_criteria = g_dtemp_criteria;
     double qq[6];
     for (int i = 0; i < 6; i++) qq[i] = q6[i];
     double t = *tem;
@@ -1691,6 +1697,7 @@ static void mp_nsw6(const nsw6_params *P, const nsw6_grid *G, int ijdim, region_
     if (ice_adjust_override == 1) ice_adjust = 1;
     g_sat_ulimit = ice_adjust ? 273.16 : 100.0;
     g_sat_llimit = ice_adjust ? 233.16 : 90.0;
+    g_dtemp_criteria = P->satadjust_dtemp_criteria;
     saturation_adjustment(ijdim, R->rhog, R->rhoge, R->rhogq, R->tem, R->q, R->qd, R->gsgam2,
                           ice_adjust, rep, l);
     negative_filter(ijdim, R->rhog, R->rhoge, R->rhogq, R->rho, R->tem, R->pre, R->q, R->gsgam2);
@@ -1879,6 +1886,7 @@ void ora_saturation_adjustment(const nsw6_params *P, int ijdim, double *rhog, do
     const int ice_adjust = (P->sw_expice != 1.0);
     g_sat_ulimit = ice_adjust ? 273.16 : 100.0;
     g_sat_llimit = ice_adjust ? 233.16 : 90.0;
+    g_dtemp_criteria = P->satadjust_dtemp_criteria;
     saturation_adjustment(ijdim, rhog, rhoge, rhogq, tem, q, qd, gsgam2, ice_adjust, NULL, 0);
 }
 
