@@ -44,24 +44,87 @@ def parse():
     ap.add_argument("--workload", default=None)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=None,
-                    help="text file with a 'bytes_per_step: N' line: HBM bytes per step measured with "
-                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (default profiles/traffic_<workload>.txt)")
+    ap.add_argument("--counters", default=None,
+                    help="text file of per-launch counters of the workload, measured with rocprofv3 --pmc "
+                         "(default profiles/counters_<workload>.txt): HBM bytes per step (FETCH_SIZE / "
+                         "WRITE_SIZE) and the rate kernel's VALU issue cycles and FP64 FLOPs; used only when "
+                         "its kernel_sources_sha16 matches the sources of this tree")
+    ap.add_argument("--cpu-leg", type=int, default=0, metavar="COLUMNS", help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the measurement, run 3 plain device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run against known bytes)")
     return ap.parse_args()
 
 
-def read_traffic(path):
-    """HBM bytes per step from a profile summary line 'bytes_per_step: N' (None if absent)."""
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD for 4 cycles
+# (FP64 FMA/ADD/MUL at full rate), so the issue peak is 1024 x 2.4 GHz SIMD cycles/s and
+# the FP64 vector peak 256 x 128 FLOP/clk x 2.4 GHz = 78.6 TFLOP/s.
+SIMDS = 1024
+PEAK_CLOCK_HZ = 2.4e9
+FP64_PEAK_TFLOPS = 78.6
+
+# The sources whose code the per-launch counters describe: the step kernels, their
+# compile flags and the synthetic-input generator.  A counters file records their
+# digest; after any change to them the stored counters no longer describe the build.
+COUNTED_SOURCES = [os.path.join("nicam_kokkos_microphysics_amd", "csrc", n) for n in (
+    "nsw6_kernels.hip", "nsw6_rates_kernel.h", "nsw6_rates_mixed.hip", "nsw6_device.h", "nsw6_step.h",
+    "nsw6_columns.h", "nsw6_internal.h", "nsw6_setup.h", "nsw6_synth.c")] + \
+    [os.path.join("nicam_kokkos_microphysics_amd", "build_lib.py")]
+
+
+def kernel_sources_sha16():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in COUNTED_SOURCES:
+        h.update(rel.encode())
+        with open(os.path.join(REPO, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def read_counters(path):
+    """'key: value' lines of a counters file -> dict of floats (plus the digest as a
+    string); None when the file is missing or was measured on other sources."""
     if not os.path.exists(path):
         return None
+    out = {}
     with open(path) as fh:
         for line in fh:
-            if line.startswith("bytes_per_step:"):
-                return float(line.split(":", 1)[1])
-    return None
+            if line.startswith("#") or ":" not in line:
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            out[k] = v if k == "kernel_sources_sha16" else float(v)
+    return out
+
+
+def host_cores():
+    """Physical cores this process may run on: distinct (socket, core) pairs of
+    `lscpu -p` among the CPUs of its affinity mask, capped by the cgroup CPU quota.
+    Returns (cores, physical cores lscpu lists for the whole host)."""
+    import math
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    pairs, every = set(), set()
+    try:
+        txt = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        for line in txt.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) for x in line.split(","))
+            every.add((sock, core))
+            if cpu in allowed:
+                pairs.add((sock, core))
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pairs = every = {(0, c) for c in allowed}
+    cores = len(pairs)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            quota, period = fh.read().split()
+        if quota != "max":
+            cores = min(cores, max(1, math.floor(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return cores, len(every)
 
 
 def mixed_error(w, steps):
@@ -113,24 +176,47 @@ def mixed_error(w, steps):
                       f"of the FP64 field"}
 
 
-def cpu_baseline(reps: int):
-    """The CPU oracle (an OpenMP FP64 port of the reference path) on one region of
-    16,900 columns: 1 warm-up, median of `reps` timed steps of the mp_driver call."""
-    import numpy as np  # noqa: F401
+def cpu_leg(columns: int, reps: int):
+    """Child process of cpu_baseline: the CPU oracle (an OpenMP FP64 port of the
+    reference path) on the first `columns` columns of the FP64 workload (whole regions
+    of 16,900), 1 warm-up and the median of `reps` timed mp_driver calls."""
     from nicam_kokkos_microphysics_amd import fields as F
     from oracle import oracle as O
-    gz, gzh, st = F.synth_state(1, 16900, seed=20240101)
+    w = F.WORKLOADS["nsw6_169000x96_fp64"]
+    gz, gzh, st = F.workload_state(w, lall=columns // w.ijdim)
     P = O.params()
-    O.mp_driver(st, gz, gzh, 15.0, P=P)
+    O.mp_driver(st, gz, gzh, w.dt, P=P)
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        O.mp_driver(st, gz, gzh, 15.0, P=P)
+        O.mp_driver(st, gz, gzh, w.dt, P=P)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": 16900 / med, "unit": "column-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"1 region x 16,900 columns x 96 levels, dt=15 s, median of {reps} steps after 1 warm-up "
-                      f"({med * 1e3:.1f} ms/step)"}
+    print(json.dumps({"median_s": statistics.median(times), "threads": O.num_threads()}))
+
+
+def cpu_baseline(reps: int):
+    """The CPU oracle timed on this host's physical cores (one OpenMP thread per core,
+    OMP_PROC_BIND=close, OMP_PLACES=cores), in a child process so that the OpenMP
+    runtime starts with that binding: the whole 169,000-column workload, and one region
+    of 16,900 columns beside it."""
+    import subprocess
+    cores, host_total = host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    res = {}
+    for cols, n in ((169000, max(1, reps // 2)), (16900, reps)):
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", str(cols), "--cpu-reps", str(n)],
+                           env=env, capture_output=True, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU baseline at {cols} columns failed:\n{p.stderr[-2000:]}")
+        res[cols] = (json.loads(p.stdout.strip().splitlines()[-1]), n)
+    (big, nb), (small, ns) = res[169000], res[16900]
+    return {"value": 169000 / big["median_s"], "unit": "column-steps/s", "cores": big["threads"], "kind": "port",
+            "sample": f"10 regions x 16,900 columns x 96 levels (the workload), dt=15 s, median of {nb} steps after "
+                      f"1 warm-up ({big['median_s'] * 1e3:.0f} ms/step); OMP_NUM_THREADS={cores} (physical cores "
+                      f"available to the process; lscpu lists {host_total} on the host), OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores",
+            "region_16900": {"value": 16900 / small["median_s"], "unit": "column-steps/s",
+                             "ms_per_step": small["median_s"] * 1e3, "steps": ns, "cores": small["threads"]}}
 
 
 def time_steps(drv, dev, dt, steps, warmup, stream):
@@ -152,7 +238,7 @@ def time_steps(drv, dev, dt, steps, warmup, stream):
     rep = drv.check()
     step_ms, nev = drv.timing_read(0)
     kernel_ms = {name: drv.timing_read(i)[0] for i, name in
-                 ((1, "rates"), (2, "extremes+bounds"), (3, "flux"), (4, "finish"))}
+                 ((1, "rates"), (2, "extremes+bounds"), (3, "flux"), (8, "flux_general"), (4, "finish"))}
     drv.timing(False)
     return wall, step_ms, nev, kernel_ms, rep
 
@@ -177,6 +263,43 @@ def other_form(prec_other, steps, warmup):
     return {"workload": wname, "value": ncol * steps / wall, "unit": "column-steps/s",
             "ms_per_step": wall * 1e3 / steps, "event_ms": step_ms, "kernels_ms": kernel_ms,
             "general_regions": rep.n_general_regions}
+
+
+def reference_vgrid():
+    """The reference's 96-level vertical grid (data/vgrid/GRD_gz.dat, GRD_gzh.dat: 72 m
+    bottom layers), kept beside the tests as a fixture."""
+    import numpy as np
+    g = os.path.join(REPO, "tests", "golden")
+    return (np.fromfile(os.path.join(g, "GRD_gz.dat"), dtype="<f8"),
+            np.fromfile(os.path.join(g, "GRD_gzh.dat"), dtype="<f8"))
+
+
+def reference_vgrid_step():
+    """The FP64 step at 169,000 x 96 on the reference vgrid at dt = 30 s with deep
+    convection (10 % towers, 1 % rain shafts): 1 warm-up and 5 timed steps.  At
+    dt = 15 s no departure on that grid goes beyond the next cell, even under 40 g/kg
+    of rain; at 30 s rain falls through two layers and every region takes the general
+    wall fluxes.  Reports how many regions did, and the general flux kernel's share."""
+    import torch
+    from nicam_kokkos_microphysics_amd import _native as N
+    from nicam_kokkos_microphysics_amd import fields as F
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    w = F.WORKLOADS["nsw6_169000x96_refgrid_dt30"]
+    steps, warmup = w.steps - 1, 1
+    ncol = w.lall * w.ijdim
+    gz, gzh = reference_vgrid()
+    gz, gzh, st = F.workload_state(w, gz=gz, gzh=gzh)
+    dev = F.to_device(st)
+    del st
+    drv = NSW6(gz, gzh, max_columns=ncol, precision=N.NSW6_FP64)
+    wall, step_ms, _, kernel_ms, rep = time_steps(drv, dev, w.dt, steps, warmup, torch.cuda.current_stream())
+    drv.close()
+    del dev
+    torch.cuda.empty_cache()
+    return {"workload": w.name, "value": ncol * steps / wall, "unit": "column-steps/s",
+            "ms_per_step": wall * 1e3 / steps, "event_ms": step_ms, "kernels_ms": kernel_ms,
+            "general_regions": rep.n_general_regions, "regions": w.lall,
+            "general_flux_share": kernel_ms["flux_general"] / step_ms if step_ms > 0 else None}
 
 
 def adjoint_cost(reps):
@@ -331,6 +454,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the step shards by NICAM region with no "
                                    "communication, and multi-GPU runs are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.cpu_leg:
+        cpu_leg(args.cpu_leg, args.cpu_reps)
+        return
     if args.rollout:
         rollout_bench()
         return
@@ -352,7 +478,22 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     algo_bytes = ALGO_BYTES_PER_COLUMN * ncol
     achieved = algo_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
-    traffic = read_traffic(args.traffic or os.path.join(REPO, "profiles", f"traffic_{wname}.txt"))
+    cpath = args.counters or os.path.join(REPO, "profiles", f"counters_{wname}.txt")
+    counters = read_counters(cpath)
+    sha = kernel_sources_sha16()
+    stale = counters is not None and counters.get("kernel_sources_sha16") != sha
+    if stale:
+        counters = None
+    traffic = counters.get("bytes_per_step") if counters else None
+    valu = None
+    rates_ms = kernel_ms["rates"]
+    if counters and "k_rates_valu_cycles" in counters and rates_ms > 0:
+        cyc, flop = counters["k_rates_valu_cycles"], counters["k_rates_fp64_flop"]
+        tflops = flop / (rates_ms * 1e-3) / 1e12
+        valu = {"kernel": "k_rates", "bound": "valu", "launch_ms": rates_ms,
+                "issue_frac": cyc / (SIMDS * PEAK_CLOCK_HZ * rates_ms * 1e-3),
+                "fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS, "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+                "valu_cycles_per_launch": cyc, "fp64_flop_per_launch": flop}
 
     out = {
         "metric": "column-steps/s at 169k x 96, FP64 vs mixed at stated error; adjoint/forward ratio",
@@ -370,10 +511,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": wname, "regions": w.lall, "columns_per_region": w.ijdim, "levels": 96,
                    "columns": ncol, "dt_s": w.dt, "tracers": 6},
-        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_flux + k_sweep + k_finish)",
+        "roofline": {"bound": "hbm", "kernel": "nsw6 step (k_rates + k_extremes + k_bounds + k_flux + k_flux_general + k_finish)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": algo_bytes, "event_ms": step_ms, "launches_timed": nev},
+                     "algorithmic_bytes": algo_bytes, "event_ms": step_ms, "launches_timed": nev,
+                     "valu_frac": valu["issue_frac"] if valu else None, "valu": valu,
+                     "counters": os.path.relpath(cpath, REPO) if counters else None,
+                     "counters_stale": stale, "kernel_sources_sha16": sha},
         "kernels_ms": kernel_ms,
         "general_regions": rep.n_general_regions,
     }
@@ -389,6 +533,7 @@ def main():
     elif not args.only:
         out["mixed"]["error_vs_fp64"] = mixed_error(F.WORKLOADS["nsw6_169000x96_mixed"], w.steps)
     if not args.only:
+        out["reference_vgrid_dt30"] = reference_vgrid_step()
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
         out["adjoint_subcycled"] = adjoint_subcycled_cost(args.adjoint_reps)
         out["subcycling"] = subcycling_cost(max(2, args.steps // 2), 1)

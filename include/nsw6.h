@@ -284,8 +284,10 @@ int nsw6_kcell_bounds(nsw6_ctx *ctx, int lall, int32_t *out, int64_t n_out);
 /* Average duration (ms) of the named kernel over the launches recorded since the
  * last reset, measured with HIP events on the stream the kernel ran on.
  * which: 0 = whole step, 1 = process rates, 2 = fall-speed extremes + kcell bounds,
- * 3 = sedimentation fluxes, 4 = flux divergence + KIN2EIN + saturation adjustment tail;
- * for nsw6_mp_driver_adj: 5 = whole call, 6 = its forward part, 7 = the reverse passes.
+ * 3 = sedimentation fluxes of the single-cell regions, 4 = flux divergence + KIN2EIN +
+ * saturation adjustment tail, 8 = the wall fluxes of the general regions (an empty
+ * launch when there are none); for nsw6_mp_driver_adj: 5 = whole call, 6 = its forward
+ * part, 7 = the reverse passes.
  * Recording is off until nsw6_timing_enable(ctx, 1). */
 int nsw6_timing_enable(nsw6_ctx *ctx, int on);
 int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count);

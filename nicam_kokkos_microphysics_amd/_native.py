@@ -137,7 +137,7 @@ def lib() -> ctypes.CDLL:
 
 class SynthConfig(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("rh_lo", _D), ("rh_hi", _D), ("frac_supersat", _D),
-                ("frac_towers", _D), ("tsfc", _D)]
+                ("frac_towers", _D), ("tsfc", _D), ("frac_rainshafts", _D)]
 
 
 def synth_lib() -> ctypes.CDLL:

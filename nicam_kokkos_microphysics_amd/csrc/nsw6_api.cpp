@@ -19,8 +19,8 @@ namespace {
 
 constexpr int N_EVENTS = NSW6_STEP_EVENTS;
 // step, rates, extremes+bounds, fluxes, finish; adjoint call, its forward part, its
-// reverse passes
-constexpr int N_TIMERS = 8;
+// reverse passes; the wall fluxes of the general regions
+constexpr int N_TIMERS = 9;
 
 struct PendingTiming {
     hipEvent_t ev[N_EVENTS];
@@ -59,7 +59,7 @@ struct nsw6_ctx {
     const int *last_report = nullptr, *last_region_flag = nullptr, *last_bounds = nullptr;
     bool timing = false;
     std::vector<PendingTiming> pending;
-    double t_sum[N_TIMERS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double t_sum[N_TIMERS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t t_count = 0;
     AdjScratch adj;
     bool last_adjoint = false;
@@ -820,11 +820,12 @@ static int drain_timing(nsw6_ctx *ctx)
 {
     for (PendingTiming &pt : ctx->pending) {
         if (hipEventSynchronize(pt.ev[N_EVENTS - 1]) != hipSuccess) return NSW6_HIP_ERROR;
-        const int from[N_TIMERS] = {0, 0, 1, 2, 3}, to[N_TIMERS] = {4, 1, 2, 3, 4};
-        for (int i = 0; i < N_TIMERS; i++) {
+        // timer <- (from event, to event)
+        const int which[6] = {0, 1, 2, 3, 4, 8}, from[6] = {0, 0, 1, 2, 4, 3}, to[6] = {5, 1, 2, 3, 5, 4};
+        for (int i = 0; i < 6; i++) {
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, pt.ev[from[i]], pt.ev[to[i]]);
-            ctx->t_sum[i] += ms;
+            ctx->t_sum[which[i]] += ms;
         }
         ctx->t_count++;
         for (int i = 0; i < N_EVENTS; i++) (void)hipEventDestroy(pt.ev[i]);
@@ -866,7 +867,7 @@ int nsw6_timing_read(nsw6_ctx *ctx, int which, double *avg_ms, int64_t *count)
     if (!ctx || which < 0 || which >= N_TIMERS || !avg_ms) return NSW6_BAD_ARG;
     int st = drain_timing(ctx);
     if (st != NSW6_OK) return st;
-    const int64_t n = (which >= 5) ? ctx->t_adj_count : ctx->t_count;
+    const int64_t n = (which >= 5 && which <= 7) ? ctx->t_adj_count : ctx->t_count;
     *avg_ms = n ? ctx->t_sum[which] / (double)n : 0.0;
     if (count) *count = n;
     return NSW6_OK;

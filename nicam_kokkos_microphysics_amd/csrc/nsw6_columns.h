@@ -1,12 +1,13 @@
-// nsw6_columns.h -- column access of the single-cell transport, shared by the HIP
-// kernels (forward k_flux / k_finish and the reverse sweep) and the host check of the
-// reverse sweep: gathers of one cell, one wall window or one cell's divergence inputs
-// from the step workspace, and the per-wall flux computation itself.
+// nsw6_columns.h -- column access of the transport, shared by the HIP kernels
+// (forward k_flux / k_flux_general / k_finish and the reverse passes) and the host
+// check of the reverse passes: gathers of one cell, one wall window or one cell's
+// divergence inputs from the step workspace, and the per-wall flux computations.
 //
 // In a single-cell region (every departure cell in {k, k+1}) each flux of wall w
 // depends only on cells w-1 (rhog, q), w, w+1 and the fall speeds Vt(w-3..w+1)
 // (src/mod_vadv1d.cpp:393-450 with the kcell bounds of :317-352), so one thread per
-// wall computes it without a column sweep.
+// wall computes it without a column sweep.  In the other regions a wall's flux also
+// reads the cells of the region's kcell range at that wall, still one thread per wall.
 #pragma once
 
 #include <stdint.h>
@@ -175,6 +176,78 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
     }
 }
 
+// Wall w of a general region: the same slots as flux_wall_to, from the fluxes over the
+// region's kcell ranges (wall_flux_general).
+template <int NT, typename Put>
+NSW6_HD void flux_wall_general_to(const StepArgs &a, int l, int w, int ij, Put &&put)
+{
+    using X = FxSlots<NT>;
+    const int ijdim = a.ijdim;
+    const Fields &f = a.f;
+    const GridDev &g = *a.grid;
+    const int64_t K2 = (int64_t)KALL * ijdim;
+    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij;
+    const int64_t ow = p2 + (int64_t)w * ijdim;
+
+    if (w == 0 || w == KALL - 1) {
+#pragma unroll
+        for (int i = 0; i < X::RW; i++) put(i, 0.0);
+        put(X::RW, f.rhogw[ow]);
+        f.GPREC[ow] = (0.0 + 0.0) * 1.0;
+        return;
+    }
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
+    const int64_t pv = (int64_t)l * NT * K2 + ij;
+    double V[NT][5];
+    int kcmin[NT], kcmax[NT], khmin[NT], khmax[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
+        kcmin[t] = bnd[bounds_index(t, 0, w, 0)];
+        kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+        khmin[t] = bnd[bounds_index(t, 1, w, 0)];
+        khmax[t] = bnd[bounds_index(t, 1, w, 1)];
+    }
+    const double rw = f.rhogw[ow];
+    const Unit u = load_unit(f, (int64_t)l * ijdim + ij);
+    GeneralGeomN<double, NT> G;
+    general_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmin, kcmax, khmin, khmax, G);
+    auto cell_at = [&](int j, CellN<double, NT> &c) {
+        CellIn<double> r;
+        load_cell(a, p2, p3, K2, ijdim, j, r);
+        const BelowCellN<NT> b = load_below<NT>(a, p2, p3, K2, ijdim, j);
+        derive_cell(g, j, r, b.rhog, b.q, b.gs, u, c);
+    };
+    WallFluxN<double, NT> F;
+    double ft[NT][4];
+    wall_flux_general(g, w, G, cell_at, F, ft);
+
+#pragma unroll
+    for (int t = 0; t < NT; t++) put(X::Q + t, F.fq[t]);
+#pragma unroll
+    for (int i = 0; i < 5; i++) put(X::E + i, F.fs[i]);
+    put(X::KINV, F.fh[0]);
+    put(X::W, F.fh[1]);
+    put(X::RW, rw);
+    f.GPREC[ow] = (0.0 + (-F.fq[0] / a.dt)) * 1.0;
+    if (w == KMIN) {
+        Surface<double> sf;
+        surface_fluxes<NT>(ft, a.dt, sf);
+        const int64_t pc = (int64_t)l * ijdim + ij;
+        const int64_t pp = (int64_t)l * 2 * ijdim + ij;
+        f.precip[pp] = sf.precip0;
+        f.precip[pp + ijdim] = sf.precip1;
+        f.ISO1_precip[pp] = 0.0;
+        f.ISO1_precip[pp + ijdim] = 0.0;
+        f.ISO2_precip[pp] = 0.0;
+        f.ISO2_precip[pp + ijdim] = 0.0;
+        f.precip_rhoe[pc] = sf.prhoe;
+        f.precip_lh_heat[pc] = sf.lh;
+        f.precip_rhophi[pc] = sf.prhophi;
+        f.precip_rhokin[pc] = sf.prhokin;
+    }
+}
+
 // The same into the flux workspace [l][FxSlots<NT>::N][k][ij].
 template <int NT = 3>
 NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
@@ -182,6 +255,14 @@ NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
     const int64_t K2 = (int64_t)KALL * a.ijdim;
     double *fx = a.ws.flux + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
     flux_wall_to<NT>(a, l, w, ij, [&](int slot, double v) { fx[slot * K2] = v; });
+}
+
+template <int NT = 3>
+NSW6_HD void flux_wall_general(const StepArgs &a, int l, int w, int ij)
+{
+    const int64_t K2 = (int64_t)KALL * a.ijdim;
+    double *fx = a.ws.flux + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
+    flux_wall_general_to<NT>(a, l, w, ij, [&](int slot, double v) { fx[slot * K2] = v; });
 }
 
 // Reads of the flux workspace: slot of wall w of column (l, ij).

@@ -48,8 +48,7 @@ constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
 //                kinetic energy, and the two horizontal momentum components
 //   NT+5..NT+6   tracer-summed half-level fluxes of vertical kinetic energy and momentum
 //   NT+7         the pre-transport rhogw of cell w
-// The regions that take the general sweep use slots 0..3 of their own block for the
-// deferred momentum (rhogvx, rhogvy, rhogvz, rhogw).
+// (the same slots for the walls of the general regions, k_flux_general).
 template <int NT> struct FxSlots {
     static constexpr int Q = 0, E = NT, PHI = NT + 1, KINH = NT + 2, U = NT + 3, V = NT + 4, KINV = NT + 5,
                          W = NT + 6, RW = NT + 7, N = NT + 8;
@@ -113,7 +112,7 @@ struct Workspace {
     double *flux;                   // [l][FLUX_SLOTS][k][ij] (see FLUX_SLOTS)
     double *part;                   // [l][KALL][12][blocks per region] partial extremes
     int *bounds;                    // [l][BOUNDS_PER_REGION]
-    int *region_flag;               // [l] 1 = region needs the general sweep
+    int *region_flag;               // [l] 1 = region needs the general wall fluxes
     int *report;                    // [REPORT_INTS]
     int *cloud_top;                 // [l][ij] highest cloudy level of the step, -1 = none
 };
@@ -157,8 +156,8 @@ struct AdjArgs {
 }  // namespace nsw6
 
 // events recorded by nsw6_launch_step: before k_rates, after k_rates, after
-// k_extremes+k_bounds, after the flux kernels (k_flux, general sweep), after k_finish
-constexpr int NSW6_STEP_EVENTS = 5;
+// k_extremes+k_bounds, after k_flux, after k_flux_general, after k_finish
+constexpr int NSW6_STEP_EVENTS = 6;
 
 extern "C" hipError_t nsw6_launch_step(const nsw6::StepArgs *args, int precision, hipStream_t stream,
                                        hipEvent_t *ev);

@@ -217,6 +217,15 @@ NSW6_HD int departure_cell(const double *zh, int k, int kcmin, int kcmax, double
     if (kc == mkmax + 1) kc = mkmax;
     return kc;
 }
+// The swept-cell factor of cell k2 for wall k and departure cell kc
+// (src/mod_vadv1d.cpp:426-436): -dz for k <= k2 < kc, +dz for kc < k2 < k, else 0.
+NSW6_HD double fact_of(int k2, int kc, int k, double dz)
+{
+    const double f1 = (double)(k2 - (kc + 1)), f2 = (double)((k - 1) - k2);
+    const double f3 = (double)(k2 - k), f4 = (double)((kc - 1) - k2);
+    return dz * 0.25 * ((copysign(1.0, f1) + 1.0) * (copysign(1.0, f2) + 1.0) -
+                        (copysign(1.0, f3) + 1.0) * (copysign(1.0, f4) + 1.0));
+}
 // |f| < EPS -> 0 (src/mod_vadv1d.cpp:445-449)
 template <typename S> NSW6_HD S eps_filter(S f) { return f * (0.5 + copysign(0.5, fabs(val(f)) - C_EPS)); }
 NSW6_HD double eps_mask(double f) { return 0.5 + copysign(0.5, fabs(f) - C_EPS); }
@@ -384,6 +393,134 @@ NSW6_HD void wall_flux(const GridDev &g, int w, const CellN<S, NT> &cur, const C
                     v = (geo.half_next[t] ? hn : hc) * zh0[t];
                 F.fh[i] += eps_filter(v);
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Fluxes of wall w of a general region: departure cells anywhere in the region's kcell
+// range (src/mod_vadv1d.cpp:393-450), including its assignment of zdis inside the
+// k2 loop (the last k2 = kcell_max wins) and, for the half level, the range
+// kcellh_min .. kcell_max of the full level (src/mod_precip_transport.cpp:890-892).
+// The same outputs as wall_flux; the swept cells come from cell_at(j, c), which
+// derives cell j (with cell j-1 below it).
+// ---------------------------------------------------------------------------------
+template <typename S, int NT> struct GeneralGeomN {
+    int kcmin[NT], kcmax[NT], kc[NT];   // full level: the region's range, the column's cell
+    int khmin[NT], kch[NT];             // half level (walls 2..94)
+    S z0[NT], zh0[NT];
+    int lo, hi;                         // the cells every range covers together
+};
+
+template <typename S, int NT>
+NSW6_HD void general_geometry(const GridDev &g, int w, double dt, double dt2, double dt3_6, const S (*V)[5],
+                              const int *kcmin, const int *kcmax, const int *khmin, const int *khmax,
+                              GeneralGeomN<S, NT> &G)
+{
+    G.lo = w;
+    G.hi = w;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        G.kcmin[t] = kcmin[t];
+        G.kcmax[t] = kcmax[t];
+        G.z0[t] = zdis_full_at(g, dt, dt2, dt3_6, V[t], w);
+        G.kc[t] = departure_cell(g.gzh, w, kcmin[t], kcmax[t], val(G.z0[t]), KMIN, KMAX);
+        G.lo = (kcmin[t] < G.lo) ? kcmin[t] : G.lo;
+        G.hi = (kcmax[t] > G.hi) ? kcmax[t] : G.hi;
+        if (w >= KMIN + 1) {
+            G.khmin[t] = khmin[t];
+            G.zh0[t] = zdis_half_at(g, dt, dt2, dt3_6, V[t], w);
+            G.kch[t] = departure_cell(g.gz_shift, w, khmin[t], khmax[t], val(G.zh0[t]), KMIN + 1, KMAX);
+            G.lo = (khmin[t] < G.lo) ? khmin[t] : G.lo;
+        } else {
+            G.khmin[t] = w;
+            G.kch[t] = w;
+            G.zh0[t] = S(0.0);
+        }
+    }
+}
+
+// zdis after the k2 loop: zdis0 for the one-cell range, else zdis0 - fact(range end)
+NSW6_HD double general_zd_shift(const double *dz, int w, int kmin, int kmax, int kc)
+{
+    return (kmin == w && kmax == w) ? 0.0 : fact_of(kmax, kc, w, dz[kmax]);
+}
+
+template <typename S, int NT, typename CellAt>
+NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, CellAt &&cell_at,
+                               WallFluxN<S, NT> &F, S (*ft_surf)[4])
+{
+    S acc[NT][6], rfc[NT][6], acch[NT][2], hvc[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) acc[t][i] = rfc[t][i] = S(0.0);
+        acch[t][0] = acch[t][1] = hvc[t][0] = hvc[t][1] = S(0.0);
+    }
+    const bool half = (w >= KMIN + 1);
+    auto take = [&](int j, const CellN<S, NT> &c) {
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const bool one = (G.kcmin[t] == w && G.kcmax[t] == w);
+            const bool in = !one && j >= G.kcmin[t] && j <= G.kcmax[t];
+            const double fa = fact_of(j, G.kc[t], w, g.dgz[j]);
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const S rf = rhof_cell(c, t, i);
+                if (in) acc[t][i] = acc[t][i] + rf * fa;
+                if (j == G.kc[t]) rfc[t][i] = rf;
+            }
+            if (half) {
+                const bool hone = (G.khmin[t] == w && G.kcmax[t] == w);
+                const bool hin = !hone && j >= G.khmin[t] && j <= G.kcmax[t];
+                const double fh = fact_of(j, G.kch[t], w, g.dgzh[j]);
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    if (hin) acch[t][i] = acch[t][i] + c.hv[t][i] * fh;
+                    if (j == G.kch[t]) hvc[t][i] = c.hv[t][i];
+                }
+            }
+        }
+    };
+    for (int j = G.lo; j <= G.hi; j++) {
+        CellN<S, NT> c;
+        cell_at(j, c);
+        take(j, c);
+    }
+    // a departure cell the search left at the domain's end outside the ranges
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        if (G.kc[t] < G.lo || G.kc[t] > G.hi) {
+            CellN<S, NT> c;
+            cell_at(G.kc[t], c);
+#pragma unroll
+            for (int i = 0; i < 6; i++) rfc[t][i] = rhof_cell(c, t, i);
+        }
+        if (half && (G.kch[t] < G.lo || G.kch[t] > G.hi)) {
+            CellN<S, NT> c;
+            cell_at(G.kch[t], c);
+            hvc[t][0] = c.hv[t][0];
+            hvc[t][1] = c.hv[t][1];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) F.fs[i] = S(0.0);
+    F.fh[0] = F.fh[1] = S(0.0);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const S zd = G.z0[t] - general_zd_shift(g.dgz, w, G.kcmin[t], G.kcmax[t], G.kc[t]);
+        S ft[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) ft[i] = eps_filter(S(acc[t][i] + rfc[t][i] * zd));
+        F.fq[t] = ft[0];
+#pragma unroll
+        for (int i = 1; i < 6; i++) F.fs[i - 1] += ft[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) ft_surf[t][i] = ft[i];
+        if (half) {
+            const S zh = G.zh0[t] - general_zd_shift(g.dgzh, w, G.khmin[t], G.kcmax[t], G.kch[t]);
+#pragma unroll
+            for (int i = 0; i < 2; i++) F.fh[i] += eps_filter(S(acch[t][i] + hvc[t][i] * zh));
         }
     }
 }

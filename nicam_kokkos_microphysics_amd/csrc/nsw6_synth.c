@@ -24,6 +24,10 @@
  *   qs, qg   : the same above the 0 C level and below the tropopause.
  *   towers   : deep convective columns, RH ~ U[1.0,1.1] and qc or qi lognormal
  *              (median 1e-3, cap 3e-3) over 2-12 km.
+ *   rain shafts: columns of heavy precipitation: qr lognormal (median 2e-3, sigma 0.5,
+ *              cap 6e-3) below the 0 C level, qg (median 1e-3, cap 3e-3) above it up
+ *              to the tropopause.  Rain of 2-6 g/kg falls about 6-7 m/s, i.e. through
+ *              more than one 72-80 m bottom layer in dt = 15 s.
  *   winds    : u, v ~ U[-20,20] m/s rotated to Cartesian with random (lon, lat);
  *              w ~ U[-2,2] m/s at walls 2..94, 0 at walls 0, 1, 95.
  *   metrics  : gsgam2, gsgam2h, gam2, gam2h = 1 + U[-1e-3, 1e-3].
@@ -47,6 +51,7 @@ typedef struct nsw6_synth_config {
     double frac_supersat;     /* fraction of columns with RH ~ U[1.0,1.2]          */
     double frac_towers;       /* fraction of deep convective columns               */
     double tsfc;              /* surface temperature (300 K)                       */
+    double frac_rainshafts;   /* fraction of heavy-precipitation columns           */
 } nsw6_synth_config;
 
 static const double Rdry = 287.04, Rvap = 461.46, CVdry = 717.56, GRAV = 9.79764;
@@ -76,7 +81,7 @@ static inline double lognormal(double u1, double u2, double median, double sigma
 }
 
 enum { S_TCOL = 1, S_RH, S_QC1, S_QC2, S_QR1, S_QR2, S_QS1, S_QS2, S_QG1, S_QG2, S_U, S_V, S_W,
-       S_LON, S_LAT, S_G1, S_G2, S_G3, S_G4, S_SUPER, S_TOWER };
+       S_LON, S_LAT, S_G1, S_G2, S_G3, S_G4, S_SUPER, S_TOWER, S_SHAFT, S_QH1, S_QH2 };
 
 void nsw6_synth_default_config(nsw6_synth_config *c)
 {
@@ -86,6 +91,7 @@ void nsw6_synth_default_config(nsw6_synth_config *c)
     c->frac_supersat = 0.0;
     c->frac_towers = 0.0;
     c->tsfc = 300.0;
+    c->frac_rainshafts = 0.0;
 }
 
 /* Power-law grid: 94 layers over 0..40 km with dz from 80 m to 1 km. */
@@ -139,6 +145,7 @@ int nsw6_synth_fill(const nsw6_synth_config *cfg, const double *gz, const double
             double G1[KALL], G2[KALL], G3[KALL], G4[KALL];
             int supersat = urand(sd, S_SUPER, L, IJ, 0) < cfg->frac_supersat;
             int tower = urand(sd, S_TOWER, L, IJ, 0) < cfg->frac_towers;
+            int shaft = urand(sd, S_SHAFT, L, IJ, 0) < cfg->frac_rainshafts;
             double tsfc = cfg->tsfc + 4.0 * (urand(sd, S_TCOL, L, IJ, 0) - 0.5);
             const double gamma = 0.0065, ztp = 12000.0;
             const double ttp = tsfc - gamma * ztp;
@@ -172,6 +179,11 @@ int nsw6_synth_fill(const nsw6_synth_config *cfg, const double *gz, const double
                 } else if (z <= ztp) {
                     Q[4][k] = lognormal(urand(sd, S_QS1, L, IJ, k), urand(sd, S_QS2, L, IJ, k), 5.0e-5, 1.0, 5.0e-4);
                     Q[5][k] = lognormal(urand(sd, S_QG1, L, IJ, k), urand(sd, S_QG2, L, IJ, k), 5.0e-5, 1.0, 5.0e-4);
+                }
+                if (shaft) {
+                    const double h1 = urand(sd, S_QH1, L, IJ, k), h2 = urand(sd, S_QH2, L, IJ, k);
+                    if (warm) Q[2][k] = lognormal(h1, h2, 2.0e-3, 0.5, 6.0e-3);
+                    else if (z <= ztp) Q[5][k] = lognormal(h1, h2, 1.0e-3, 0.5, 3.0e-3);
                 }
                 /* rho from pre = rho T (qd Rdry + qv Rvap) with qv = RH psat / (rho Rv T) */
                 double qd = 1.0;

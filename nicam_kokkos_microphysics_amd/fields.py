@@ -98,6 +98,7 @@ class Workload:
     frac_supersat: float = 0.0
     frac_towers: float = 0.0
     seed: int = 20240101
+    frac_rainshafts: float = 0.0
 
 
 WORKLOADS = {
@@ -109,6 +110,12 @@ WORKLOADS = {
     "nsw6_169000x96_mixed": Workload("nsw6_169000x96_mixed", 10, 16900, 15.0, 10, frac_supersat=0.05),
     # adjoint workload
     "nsw6_16900x96_adjoint": Workload("nsw6_16900x96_adjoint", 1, 16900, 15.0, 1),
+    # the FP64 step on the reference's vgrid (72 m bottom layers) at dt = 30 s with deep
+    # convection (10 % towers, 1 % rain shafts): rain falls through two layers, so every
+    # region takes the general wall fluxes (6 steps: the reference's own saturation
+    # adjustment stops converging at a few points from the 9th step on)
+    "nsw6_169000x96_refgrid_dt30": Workload("nsw6_169000x96_refgrid_dt30", 10, 16900, 30.0, 6, frac_towers=0.10,
+                                            frac_rainshafts=0.01),
     # long rollout: 40 regions, 20 steps, 10 % deep convective towers
     "nsw6_676000x96_rollout": Workload("nsw6_676000x96_rollout", 40, 16900, 15.0, 20, frac_towers=0.10),
 }
@@ -124,7 +131,8 @@ def synth_grid():
 
 def synth_state(lall: int, ijdim: int, *, seed: int = 20240101, frac_supersat: float = 0.0,
                 frac_towers: float = 0.0, rh_lo: float = 0.5, rh_hi: float = 1.05, l_offset: int = 0,
-                ij_offset: int = 0, gz=None, gzh=None, tsfc: float | None = None) -> tuple:
+                ij_offset: int = 0, gz=None, gzh=None, tsfc: float | None = None,
+                frac_rainshafts: float = 0.0) -> tuple:
     """Seeded, self-consistent host state: (gz, gzh, state).  Columns are a pure
     function of (seed, l + l_offset, ij + ij_offset), so sub-blocks of a large run can
     be regenerated exactly."""
@@ -139,6 +147,7 @@ def synth_state(lall: int, ijdim: int, *, seed: int = 20240101, frac_supersat: f
     cfg.seed = seed
     cfg.frac_supersat = frac_supersat
     cfg.frac_towers = frac_towers
+    cfg.frac_rainshafts = frac_rainshafts
     cfg.rh_lo = rh_lo
     cfg.rh_hi = rh_hi
     if tsfc is not None:
@@ -153,7 +162,7 @@ def synth_state(lall: int, ijdim: int, *, seed: int = 20240101, frac_supersat: f
 
 def workload_state(w: Workload, lall: int | None = None, ijdim: int | None = None, **kw):
     return synth_state(lall or w.lall, ijdim or w.ijdim, seed=w.seed, frac_supersat=w.frac_supersat,
-                       frac_towers=w.frac_towers, **kw)
+                       frac_towers=w.frac_towers, frac_rainshafts=w.frac_rainshafts, **kw)
 
 
 def to_device(state: dict, device="cuda:0") -> dict:

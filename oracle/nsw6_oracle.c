@@ -16,6 +16,7 @@
  * Parity status of each function (how it is pinned) is listed in the tests that
  * exercise it (tests/test_oracle_*.py).
  */
+#define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c11 */
 #include "nsw6_oracle.h"
 
 #include <math.h>

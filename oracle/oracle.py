@@ -13,8 +13,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle_nsw6.so")
-GRAD_LIB = os.path.join(HERE, "libnsw6_grad.so")
+# NSW6_ORACLE_LIB / NSW6_GRAD_LIB select other builds of the two libraries (the
+# sanitizer builds of `make -C oracle sanitize`, under oracle/_san/)
+LIB = os.environ.get("NSW6_ORACLE_LIB") or os.path.join(HERE, "liboracle_nsw6.so")
+GRAD_LIB = os.environ.get("NSW6_GRAD_LIB") or os.path.join(HERE, "libnsw6_grad.so")
 GRAD_NDIR = 8
 
 _lib = None

@@ -148,3 +148,31 @@ def test_binding_refuses_host_arrays():
         check_device_tensors(st, 1, 16, 0)                                   # numpy arrays
     with pytest.raises(TypeError):
         check_device_tensors({k: torch.from_numpy(v) for k, v in st.items()}, 1, 16, 0)   # CPU tensors
+
+
+@pytest.mark.gpu
+def test_timing_slots_partition_the_step():
+    """The step timer (event before k_rates to event after k_finish) is the sum of the
+    per-kernel timers, which are consecutive event intervals on the launch stream; the
+    general sweep has its own slot."""
+    import torch
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    gz, gzh, st = F.synth_state(2, 2000, seed=25)
+    drv = NSW6(gz, gzh, max_columns=4000)
+    try:
+        dev = F.to_device(st)
+        drv.step(dev, DT)
+        torch.cuda.synchronize()
+        drv.timing(True)
+        drv.timing_reset()
+        for _ in range(3):
+            drv.step(dev, DT, check=False)
+        torch.cuda.synchronize()
+        total, n = drv.timing_read(0)
+        parts = [drv.timing_read(i)[0] for i in (1, 2, 3, 8, 4)]
+        assert n == 3 and drv.timing_read(8)[1] == 3
+        assert all(p >= 0.0 for p in parts) and total > 0.0
+        assert abs(sum(parts) - total) <= 1e-3 + 1e-3 * total, (parts, total)
+        drv.timing(False)
+    finally:
+        drv.close()

@@ -21,7 +21,12 @@ kcell search (:329-352).  These tests run that multi-block path:
   (include/problemsize.h:63);
 * a ragged 2 x 9,001 state in which one region takes the general (multi-cell)
   sedimentation and the other the single-cell path, over 3 steps;
-* the reverse mode at 2 x 9,001 against the host run of the same reverse passes.
+* the reverse mode at 2 x 9,001 against the host run of the same reverse passes;
+* 2 steps of a 2 x 9,001 state with deep convection on the reference's own vgrid
+  (72 m bottom layers) at dt = 60 s, where rain falls through up to 7 layers and
+  some departures point upward: both regions take the general wall fluxes (from the
+  third step on the reference's own saturation adjustment leaves points
+  unconverged in that state).
 
 Tolerance: 1e-10 of each field's max-abs (FP64 form); kcell bounds bit-exact.
 """
@@ -110,7 +115,7 @@ def _assert_step_parity(out, ref, margin):
     return int(border.sum())
 
 
-def _run(st, gz, gzh, steps, P_edit=None):
+def _run(st, gz, gzh, steps, P_edit=None, dt=DT):
     """`steps` consecutive steps along the oracle's trajectory: at each the GPU steps
     from the oracle's state and must agree with the oracle's step
     (_assert_step_parity), kcell bounds bit-exact.  Returns the GPU reports and the
@@ -131,11 +136,11 @@ def _run(st, gz, gzh, steps, P_edit=None):
         for s in range(steps):
             print(f"step {s + 1}/{steps}: lall={lall} ijdim={ijdim}", flush=True)
             one = F.to_device(ref)
-            drv.step(one, DT)
+            drv.step(one, dt)
             torch.cuda.synchronize()
             reps.append(drv.report())
             b_gpu = drv.kcell_bounds(lall)
-            r, b_ora, margin = O.mp_driver_traced(ref, gz, gzh, DT, P=P_ora)
+            r, b_ora, margin = O.mp_driver_traced(ref, gz, gzh, dt, P=P_ora)
             assert r.status == 0
             out = F.to_host(one)
             del one
@@ -243,3 +248,22 @@ def test_reverse_mode_multiblock_against_host_passes():
         err = np.abs(g - g_host[k]).max()
         assert err <= 1e-9 * scale + 1e-300, (k, err, scale)
     np.testing.assert_allclose(gc.cpu().numpy(), gc_host, rtol=1e-9, atol=0)
+
+
+def test_general_walls_reference_grid_dt60():
+    """Deep convection (10 % towers, 1 % rain shafts) on the reference vgrid at
+    dt = 60 s: kcell ranges of several cells and upward departures, so every region
+    takes k_flux_general; 2 steps along the oracle's trajectory, kcell bounds
+    bit-exact."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    gz = np.fromfile(os.path.join(here, "GRD_gz.dat"), dtype="<f8")
+    gzh = np.fromfile(os.path.join(here, "GRD_gzh.dat"), dtype="<f8")
+    gz, gzh, st = F.synth_state(2, 9001, seed=41, gz=gz, gzh=gzh, frac_towers=0.1, frac_rainshafts=0.01)
+    reps, bounds = _run(st, gz, gzh, 2, dt=60.0)
+    k = np.arange(96)
+    for r, b in zip(reps, bounds):
+        assert r.n_general_regions == 2
+        assert _general_regions(b).all()
+        assert (b[:, :, 0, 1:95, 1] - k[1:95]).max() >= 3      # departures 3+ cells up
+        assert (b[:, :, 0, 1:95, 0] - k[1:95]).min() <= -1     # and upward ones
