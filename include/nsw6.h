@@ -50,8 +50,8 @@ enum nsw6_status {
     NSW6_BAD_ARG        = 3,
     NSW6_HIP_ERROR      = 4,
     NSW6_NO_DEVICE      = 5,
-    NSW6_UNSUPPORTED    = 6,  /* the adjoint met a region that needs the general       */
-                              /* (multi-cell) sedimentation; its gradient is left 0   */
+    NSW6_UNSUPPORTED    = 6,  /* the reverse mode was asked for a scheme it does not  */
+                              /* cover (nsw6_mp_driver_adj)                           */
     NSW6_IO_ERROR       = 7   /* a snapshot file is missing, unreadable or of the     */
                               /* wrong size                                           */
 };
@@ -249,8 +249,9 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  *   cotangents of the tunable coefficients in NSW6_NCOEF order.
  * Discrete choices (switches, limiters, kcell bounds, departure cells) are taken at
  * the input state, and the saturation adjustment is differentiated at its converged
- * solution (implicit-function theorem).  Regions whose departures leave one cell
- * get gradient 0 and make nsw6_last_status() return NSW6_UNSUPPORTED.
+ * solution (implicit-function theorem).  Regions whose departures leave the next cell
+ * (the general wall fluxes) are differentiated like the others; nsw6_last_status()
+ * counts them in n_general_regions.
  * A non-default scheme returns NSW6_UNSUPPORTED at once (nothing is launched).
  * With mp_div_num = n > 1 (nsw6_set_div_num) the call is differentiated as the
  * forward call runs it: n sub-steps of dt/n, precip, precip_rhoe/lh_heat/rhophi/
@@ -259,7 +260,7 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  * are rebuilt by n-1 forward sub-steps into context scratch of 21*96 doubles per
  * column for each of them, plus 2*21*96 + 7*96 + 9 doubles per column and the
  * region flags (grown on demand).  A region that takes the general path in any
- * sub-step is reported as above.  Timing ids 6/7 then cover the forward rebuild and
+ * sub-step is counted as above.  Timing ids 6/7 then cover the forward rebuild and
  * the reversed sub-steps (each with its own forward part). */
 int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                        const nsw6_fields *state, const nsw6_fields *cot,

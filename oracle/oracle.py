@@ -275,8 +275,7 @@ def _shape(state):
 
 
 def step_fp64(state: dict, gz, gzh, dt: float, P=None) -> int:
-    """The templated step in FP64 on the host, in place (0 = ok, 1 = Newton failure,
-    -1 = a region needs the general transport)."""
+    """The templated step in FP64 on the host, in place (0 = ok, 1 = Newton failure)."""
     from nicam_kokkos_microphysics_amd.fields import field_ptrs
     P = params() if P is None else P
     gz = np.ascontiguousarray(gz, dtype=np.float64)

@@ -6,6 +6,8 @@
 * Forward-mode dual-number directional derivatives of the adjoint-check loss agree
   with central finite differences (h = 1e-6 relative) along random directions that
   perturb every input field and the 8 coefficients at once.
+* The general regions (departures beyond the next cell): the host step's general
+  wall fluxes against the oracle, and the whole reverse mode against dual numbers.
 * Every hand-written reverse piece (point rates, wall fluxes, cell divergence and
   KIN2EIN, point-wise tail with the implicit-function Newton adjoint) agrees with
   forward-mode dual numbers at every point, wall and cell of several states, including
@@ -178,14 +180,69 @@ def test_whole_step_reverse_per_field():
         assert abs(s - dj[i]) <= 1e-13 * a, (name, s, dj[i], a)
 
 
-def test_reverse_reports_general_regions():
-    """A region whose departures leave one cell gets gradient 0 and the unsupported
-    status, as the GPU entry point does."""
-    gz, gzh, st = F.synth_state(1, 16, seed=5)
-    st["tem"][:] += 0.0
-    cot, cs = random_cotangents(st, gz, gzh, 1)
-    grad, gc, status = O.adjoint_host(st, gz, gzh, 600.0, cot, cs)
-    if status == 0:
-        pytest.skip("dt = 600 s did not produce a multi-cell departure")
-    assert status == 6
-    assert all(np.all(g == 0) for g in grad.values())
+def _general_state(case):
+    """States whose every region takes the general wall fluxes: rain 25 times faster,
+    and deep convection on the reference vgrid at dt = 60 s (kcell ranges of several
+    cells, upward departures).  Returns (gz, gzh, state, dt, params)."""
+    import os
+    P = O.params()
+    if case == "fast_rain":
+        gz, gzh, st = F.synth_state(2, 24, seed=3)
+        P.Cr = 130.0 * 25
+        return gz, gzh, st, 15.0, P
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    gz = np.fromfile(os.path.join(g, "GRD_gz.dat"), dtype="<f8")
+    gzh = np.fromfile(os.path.join(g, "GRD_gzh.dat"), dtype="<f8")
+    gz, gzh, st = F.synth_state(2, 24, seed=41, gz=gz, gzh=gzh, frac_towers=0.1, frac_rainshafts=0.1)
+    return gz, gzh, st, 60.0, P
+
+
+def _general_cotangents(st, gz, gzh, dt, P, seed):
+    rng = np.random.default_rng(seed)
+    out = F.copy_state(st)
+    assert O.step_fp64(out, gz, gzh, dt, P=P) == 0
+    cot = {k: rng.uniform(-1, 1, out[k].shape) / (float(np.sqrt(np.mean(out[k] ** 2))) or 1.0)
+           for k in F.ADJ_OUTPUTS}
+    return cot, rng.uniform(-1, 1, (st["rhog"].shape[0], 3, st["rhog"].shape[-1])) / 1e-3
+
+
+@pytest.mark.parametrize("case", ["fast_rain", "reference_grid_dt60"])
+def test_general_regions_host_step_matches_oracle(case):
+    """The templated step's general wall fluxes (wall_flux_general, the statements the
+    GPU's k_flux_general runs) against the oracle, and every region is general."""
+    gz, gzh, st, dt, P = _general_state(case)
+    a, b = F.copy_state(st), F.copy_state(st)
+    rep, bounds, _ = O.mp_driver_traced(a, gz, gzh, dt, P=P, newton=False)
+    assert rep.status == 0
+    bnd = np.asarray(bounds).reshape(st["rhog"].shape[0], 3, 2, 96, 2)
+    k = np.arange(96)
+    assert (((bnd[:, :, 0, :, 0] != k) | (bnd[:, :, 0, :, 1] > k + 1))[:, :, 1:95]).any(axis=(1, 2)).all()
+    assert O.step_fp64(b, gz, gzh, dt, P=P) == 0
+    keys = ["rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rho", "tem", "pre", "qd", "precip",
+            "precip_rhoe", "precip_lh_heat", "precip_rhophi", "precip_rhokin", "GPREC"] + \
+        [f"rhogq:{t}" for t in F.TRACERS] + [f"q:{t}" for t in F.TRACERS]
+    worst = max(v[2] for v in field_errors(b, a, keys).values())
+    assert worst <= 1e-12, worst
+
+
+@pytest.mark.parametrize("case", ["fast_rain", "reference_grid_dt60"])
+def test_general_regions_reverse_matches_dual_numbers(case):
+    """The reverse of the general wall fluxes (adj_walls_general, one thread per
+    column adding into the wall records) with the other passes, on the host, against
+    dual numbers: random directions through every input and coefficient, and one
+    direction per input field so that a cotangent routed to the wrong field or level
+    cannot cancel."""
+    gz, gzh, st, dt, P = _general_state(case)
+    cot, cs = _general_cotangents(st, gz, gzh, dt, P, 7)
+    grad, gc, status = O.adjoint_host(st, gz, gzh, dt, cot, cs, P=P)
+    assert status == 0
+    dirs, dcoef = random_directions(st, 13, 4)
+    rng = np.random.default_rng(4)
+    for key in STATE_KEYS:
+        dirs.append({key: st[key] * rng.uniform(-1, 1, st[key].shape)})
+    dcoef = np.concatenate([dcoef, np.zeros((len(STATE_KEYS), 8))])
+    for lo in range(0, len(dirs), 8):
+        _, dj = O.dual_vjp(st, gz, gzh, dt, cot, cs, dirs[lo:lo + 8], dcoef[lo:lo + 8], P=P)
+        for i in range(len(dj)):
+            s, a = directional(grad, gc, dirs[lo + i], dcoef[lo + i])
+            assert abs(s - dj[i]) <= 1e-13 * a, (lo + i, s, dj[i], a)

@@ -13,7 +13,10 @@
 * Through two chained steps, the torch.autograd.Function gives the gradient of the
   chained direct calls to round-off and matches central differences of the
   two-step loss.
-* Determinism (two calls are bitwise equal) and the unsupported-region status.
+* Determinism (two calls are bitwise equal).
+* States whose every region takes the general wall fluxes (rain 25x faster; deep
+  convection on the reference vgrid at dt = 60 s): the GPU reverse mode against the
+  host passes and against dual numbers.
 """
 import numpy as np
 import pytest
@@ -160,9 +163,7 @@ def test_gpu_reverse_matches_central_differences():
     drv.close()
 
 
-def test_gpu_reverse_is_deterministic_and_reports_general_regions():
-    import torch
-    from nicam_kokkos_microphysics_amd.driver import NSW6, NSW6Error
+def test_gpu_reverse_is_deterministic():
     gz, gzh, st = _state("default", lall=2, ijdim=300)
     cot, cs = random_cotangents(st, gz, gzh, 5)
     a, ga, _ = _gpu_adjoint(st, gz, gzh, cot, cs)
@@ -170,16 +171,117 @@ def test_gpu_reverse_is_deterministic_and_reports_general_regions():
     for k in F.ADJ_INPUTS:
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(ga, gb)
-    # a 600 s step carries rain beyond one cell near the surface: the general path
-    drv = NSW6(gz, gzh, max_columns=600)
-    dev = F.to_device(st)
-    dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
-    with pytest.raises(NSW6Error) as ei:
-        drv.adjoint(dev, dcot, 600.0)
-    assert ei.value.code == N.NSW6_UNSUPPORTED
-    rep = drv.report()
-    assert rep.n_general_regions > 0
-    drv.close()
+
+
+def _general_case(case):
+    """States whose regions all take the general wall fluxes: rain 25 times faster
+    (the fast-rain parity case), and deep convection on the reference vgrid at
+    dt = 60 s (kcell ranges of several cells, upward departures)."""
+    import os
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import default_params
+    Po, Pg = O.params(), default_params()
+    if case == "fast_rain":
+        gz, gzh, st = F.synth_state(2, 400, seed=3)
+        Po.Cr = Pg.Cr = 130.0 * 25
+        return gz, gzh, st, 15.0, Po, Pg
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    gz = np.fromfile(os.path.join(g, "GRD_gz.dat"), dtype="<f8")
+    gzh = np.fromfile(os.path.join(g, "GRD_gzh.dat"), dtype="<f8")
+    gz, gzh, st = F.synth_state(2, 400, seed=41, gz=gz, gzh=gzh, frac_towers=0.1, frac_rainshafts=0.05)
+    return gz, gzh, st, 60.0, Po, Pg
+
+
+def _cotangents(st, gz, gzh, dt, P, seed):
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    out = F.copy_state(st)
+    assert O.step_fp64(out, gz, gzh, dt, P=P) == 0
+    cot = {k: rng.uniform(-1, 1, out[k].shape) / (float(np.sqrt(np.mean(out[k] ** 2))) or 1.0)
+           for k in F.ADJ_OUTPUTS}
+    return cot, rng.uniform(-1, 1, (st["rhog"].shape[0], 3, st["rhog"].shape[-1])) / 1e-3
+
+
+@pytest.mark.parametrize("case", ["fast_rain", "reference_grid_dt60"])
+def test_gpu_reverse_general_regions(case):
+    """Every region general: the GPU reverse mode (k_adj_wall_general) against the host
+    run of the same passes, elementwise within 1e-9 of each field's max-abs, and
+    against forward-mode dual numbers of the host step (directional, 1e-9 of the
+    absolute terms), along random directions and per input field.
+
+    Two kinds of points have no derivative, and there the two builds may legitimately
+    differ: where a tracer's input rhogq is exactly 0 (the rain fall speed goes as
+    (rho q)^(1/8) behind the zero switch; the value after the rates is round-off), and
+    at the kinks of the rates' limiters, where the last bits of each build's arithmetic
+    decide the side.  The first are left out; at each point of the second kind (at
+    most 32) the GPU's value must be one of the host step's one-sided difference
+    quotients and the host's the other, and the directions vanish there."""
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    gz, gzh, st, dt, Po, Pg = _general_case(case)
+    cot, cs = _cotangents(st, gz, gzh, dt, Po, 7)
+    g_host, gc_host, status = O.adjoint_host(st, gz, gzh, dt, cot, cs, P=Po)
+    assert status == 0
+    lall, ijdim = st["rhog"].shape[0], st["rhog"].shape[-1]
+    drv = NSW6(gz, gzh, params=Pg, max_columns=lall * ijdim)
+    try:
+        dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+        print(f"adjoint {case}", flush=True)
+        grad, gc = drv.adjoint(F.to_device(st), dcot, dt,
+                               cot_sflux=torch.from_numpy(np.ascontiguousarray(cs)).cuda())
+        torch.cuda.synchronize()
+        rep = drv.report()
+    finally:
+        drv.close()
+    assert rep.status == 0 and rep.n_general_regions == lall
+    g_gpu = {k: v.cpu().numpy() for k, v in grad.items()}
+    gc_gpu = gc.cpu().numpy()
+
+    def J(s):
+        j, _ = O.dual_vjp(s, gz, gzh, dt, cot, cs, [{}], np.zeros((1, 8)), P=Po)
+        return j
+
+    kinks = []          # (field, index) where the two builds took different sides
+    for k in F.ADJ_INPUTS:
+        keep = (st[k] != 0) if k == "rhogq" else np.ones(g_host[k].shape, dtype=bool)
+        scale = np.abs(g_host[k][keep]).max()
+        off = (np.abs(g_gpu[k] - g_host[k]) > 1e-9 * scale) & keep
+        kinks += [(k, tuple(int(v) for v in i)) for i in np.argwhere(off)]
+    assert len(kinks) <= 32, len(kinks)
+    j0 = J(st) if kinks else None
+    for k, i in kinks:
+        x0 = st[k][i]
+        h = 1e-7 * abs(x0) or 1e-12
+        sp, sm = F.copy_state(st), F.copy_state(st)
+        sp[k][i] = x0 + h
+        sm[k][i] = x0 - h
+        right, left = (J(sp) - j0) / h, (j0 - J(sm)) / h
+        near = lambda a, b: abs(a - b) <= 1e-3 * max(abs(a), abs(b))
+        assert not near(right, left), (k, i, right, left)       # a kink indeed
+        assert (near(g_gpu[k][i], right) and near(g_host[k][i], left)) or \
+               (near(g_gpu[k][i], left) and near(g_host[k][i], right)), (k, i, g_gpu[k][i], g_host[k][i], left, right)
+    print(f"{len(kinks)} kink entries", flush=True)
+    np.testing.assert_allclose(gc_gpu, gc_host, rtol=1e-9, atol=0)
+
+    dirs, dcoef = random_directions(st, 17, 4)
+    rng = np.random.default_rng(4)
+    for key in STATE_KEYS[:4]:
+        dirs.append({key: st[key] * rng.uniform(-1, 1, st[key].shape)})
+    dcoef = np.concatenate([dcoef, np.zeros((4, 8))])
+    # no direction moves an input at a kink point
+    pts = {(i[0],) + i[-2:] for _, i in kinks}
+    for d in dirs:
+        for key, v in d.items():
+            for p in pts:
+                if v.ndim == 3:
+                    v[p] = 0.0
+                else:
+                    v[p[0], :, p[1], p[2]] = 0.0
+    _, dj = O.dual_vjp(st, gz, gzh, dt, cot, cs, dirs, dcoef, P=Po)
+    for i in range(len(dirs)):
+        s, a = directional(g_gpu, gc_gpu, dirs[i], dcoef[i])
+        assert abs(s - dj[i]) <= 1e-9 * a, (i, s, dj[i], a)
 
 
 def test_autograd_function_and_two_step_chain():

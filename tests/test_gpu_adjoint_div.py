@@ -14,8 +14,8 @@ dt/n and averages precip, its energy fluxes and GPREC over them).
   a switch and are skipped; at least two of four must be checked) agree with the
   gradient to 1e-5 relative along random directions through every input and
   coefficient.  This check does not share the chain's construction.
-* A region that takes the general sedimentation path in a sub-step makes the call
-  report NSW6_UNSUPPORTED, as the single step does.
+* With rain 25x faster every sub-step takes the general wall fluxes, and the call is
+  still the chain of its sub-steps' reverse calls.
 """
 import numpy as np
 import pytest
@@ -144,18 +144,31 @@ def test_subcycled_reverse_matches_central_differences():
     drv.close()
 
 
-def test_subcycled_reverse_reports_general_regions():
+def test_subcycled_reverse_with_general_regions_is_the_chain_of_substeps():
+    """Rain 25x faster: the sub-steps take the general wall fluxes; the sub-cycled
+    call's gradient is still the chain of single-step reverse calls, bit for bit."""
     import torch
-    from nicam_kokkos_microphysics_amd.driver import NSW6, NSW6Error
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
     gz, gzh, st = _state(lall=2, ijdim=300, seed=11)
-    cot_h, _ = random_cotangents(st, gz, gzh, 5)
-    drv = NSW6(gz, gzh, max_columns=600)
-    drv.set_div_num(2)
+    cot_h, cs_h = random_cotangents(st, gz, gzh, 5)
+    P = default_params()
+    P.Cr = 130.0 * 25
     dev = F.to_device(st)
     cot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot_h.items()}
-    # sub-steps of 600 s carry rain beyond one cell near the surface: the general path
-    with pytest.raises(NSW6Error) as ei:
-        drv.adjoint(dev, cot, 1200.0)
-    assert ei.value.code == N.NSW6_UNSUPPORTED
-    assert drv.report().n_general_regions > 0
-    drv.close()
+    cs = torch.from_numpy(np.ascontiguousarray(cs_h)).cuda()
+    sub = NSW6(gz, gzh, params=P, max_columns=600)
+    sub.set_div_num(2)
+    one = NSW6(gz, gzh, params=P, max_columns=600)
+    try:
+        print("sub-cycled reverse with general regions", flush=True)
+        g, gc = sub.adjoint(dev, cot, DT, cot_sflux=cs)
+        torch.cuda.synchronize()
+        rep = sub.report()
+        assert rep.status == 0 and rep.n_general_regions == 2
+        want, want_c = _chain(one, dev, cot, cs, 2)
+        for k in F.ADJ_INPUTS:
+            assert torch.equal(g[k], want[k]), (k, float((g[k] - want[k]).abs().max()))
+        assert torch.equal(gc, want_c)
+    finally:
+        sub.close()
+        one.close()
