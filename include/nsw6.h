@@ -230,7 +230,7 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                            const nsw6_fields *fields, double dt, double ct, void *stream);
 
 /* Reverse mode of one FP64 step (default scheme: Roh & Satoh snow moments, KK2000,
- * explicit ice nucleation; sw_expice = 1; KIN2EIN, 3WATER; or the same with Berry's
+ * explicit ice nucleation; sw_expice = 1; KIN2EIN or KIN2KIN, 3WATER; or the same with Berry's
  * autoconversion, sw_kk2000 = 0; each with or without OPT_INDIR, which then reads
  * `state->unccn`): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)

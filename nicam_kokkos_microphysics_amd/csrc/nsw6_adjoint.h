@@ -285,8 +285,82 @@ struct SedInB {
 };
 NSW6_HD void zero(SedInB &s) { zero_words(s); }
 
+// Reverse of kin2kin_v (nsw6_step.h): out = w f rhogh with w = rw / rhogh,
+// f = sqrt(|kin_v0 / kin_v|), kin_v0 = kinv / rhogh, kin_v = w^2 / 2 (f = 1 when kin_v
+// is at most 1e-20).  Adds the cotangents of rhogh, kinv and rw.
+NSW6_HD void kin2kin_v_b(double rhogh, double kinv, double rw, double out_b, double &rhogh_b, double &kinv_b,
+                         double &rw_b)
+{
+    const double k0 = kinv / rhogh;
+    const double w = rw / rhogh;
+    const double kv = 0.5 * (w * w);
+    double f = 1.0, w_b;
+    // out = w' rhogh, w' = w f
+    if (kv > 1.0E-20) {
+        const double r = k0 / kv;
+        f = sqrt(fabs(r));
+        rhogh_b += out_b * (w * f);
+        const double wp_b = out_b * rhogh;
+        w_b = wp_b * f;
+        const double f_b = wp_b * w;
+        const double r_b = (f > 0.0) ? f_b * copysign(1.0, r) / (2.0 * f) : 0.0;
+        const double k0_b = r_b / kv;
+        const double kv_b = -r_b * r / kv;
+        w_b += kv_b * w;
+        kinv_b += k0_b / rhogh;
+        rhogh_b -= k0_b * k0 / rhogh;
+    } else {
+        rhogh_b += out_b * w;
+        w_b = out_b * rhogh;
+    }
+    rw_b += w_b / rhogh;
+    rhogh_b -= w_b * w / rhogh;
+}
+
+// Reverse of kin2kin_h (nsw6_step.h): out_i = v_i f rg with v_i = rv_i / rg,
+// f = sqrt(|kin_h0 / kin_h|), kin_h0 = kinh / rg, kin_h = |v|^2 / 2 (f = 1 when kin_h is
+// at most 1e-20).  rv holds the momenta before the scaling; adds the cotangents of
+// rg, kinh and rv.
+NSW6_HD void kin2kin_h_b(double rg, double kinh, const double rv[3], const double out_b[3], double &rg_b,
+                         double &kinh_b, double rv_b[3])
+{
+    const double k0 = kinh / rg;
+    double v[3], v_b[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) v[i] = rv[i] / rg;
+    const double kh = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double f = 1.0;
+    const bool scaled = kh > 1.0E-20;
+    if (scaled) f = sqrt(fabs(k0 / kh));
+    double f_b = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        rg_b += out_b[i] * (v[i] * f);
+        const double vp_b = out_b[i] * rg;
+        v_b[i] = vp_b * f;
+        f_b += vp_b * v[i];
+    }
+    if (scaled) {
+        const double r = k0 / kh;
+        const double r_b = (f > 0.0) ? f_b * copysign(1.0, r) / (2.0 * f) : 0.0;
+        const double k0_b = r_b / kh;
+        const double kh_b = -r_b * r / kh;
+#pragma unroll
+        for (int i = 0; i < 3; i++) v_b[i] += kh_b * v[i];
+        kinh_b += k0_b / rg;
+        rg_b -= k0_b * k0 / rg;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        rv_b[i] += v_b[i] / rg;
+        rg_b -= v_b[i] * v[i] / rg;
+    }
+}
+
+// Reverse of sediment (NT = 3), with the KIN2KIN energy correction (kin2kin) or the
+// default KIN2EIN one.
 NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<double> &x, const SedOut<double> &yb,
-                        SedInB &xb)
+                        SedInB &xb, bool kin2kin = false)
 {
     // forward recomputation
     const double rc = g.rdgz[c];
@@ -330,11 +404,32 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
 
     // reverse
     const double e_b = yb.e;
-    const double tmp2_b = e_b, tmpk_b = -e_b, e1_b = e_b;
+    // KIN2EIN: e += tmp2 - tmpk; KIN2KIN: the momenta are rescaled instead
+    const double tmp2_b = kin2kin ? 0.0 : e_b, tmpk_b = kin2kin ? 0.0 : -e_b, e1_b = e_b;
     const double kinh_new_b = tmpk_b;
     const double kinv_new_cp_b = tmpk_b * w2c0, kinv_new_c_b = tmpk_b * w2c1;
     double rg_b = yb.rg, rwn_b = yb.rw, rg_cm_b = 0.0, rg_cp_b = 0.0, rw_cp_b = 0.0;
     double rvxn_b = yb.rvx, rvyn_b = yb.rvy, rvzn_b = yb.rvz;
+    double k2k_kinh_u_b = 0.0, k2k_kinv_c_b = 0.0;
+    if (kin2kin) {
+        // y.rw = kin2kin_v(c2w0_c rg + c2w1_c rg_cm, kinv_c, rw_n)
+        const double kinh = 0.5 * (x.rvx * x.rvx + x.rvy * x.rvy + x.rvz * x.rvz) / x.rg_c;
+        const double dkinh = -(x.fs[1][2] - x.fs[0][2]) * rc;
+        const double hk_c = has_hc ? -(x.fh[1][0] - x.fh[0][0]) * g.rdgzh[c] : 0.0;
+        const double kinv_c = kinv0_c + hk_c;
+        double rhogh_b = 0.0, rw_b = 0.0;
+        kin2kin_v_b(c2w0_c * rg + c2w1_c * rg_cm, kinv_c, rw_n, yb.rw, rhogh_b, k2k_kinv_c_b, rw_b);
+        rwn_b = rw_b;
+        rg_b += rhogh_b * c2w0_c;
+        rg_cm_b += rhogh_b * c2w1_c;
+        // (y.rvx, y.rvy, y.rvz) = kin2kin_h(rg, kinh_u, rv_n)
+        const double rvn[3] = {rvx_n, rvy_n, rvz_n}, ob[3] = {yb.rvx, yb.rvy, yb.rvz};
+        double rvn_b[3] = {0.0, 0.0, 0.0};
+        kin2kin_h_b(rg, kinh + dkinh, rvn, ob, rg_b, k2k_kinh_u_b, rvn_b);
+        rvxn_b = rvn_b[0];
+        rvyn_b = rvn_b[1];
+        rvzn_b = rvn_b[2];
+    }
     if (has_hc) {
         rwn_b += kinv_new_c_b * rw_n / den_nc;
         const double d_b = -kinv_new_c_b * kinv_new_c / den_nc;
@@ -362,7 +457,7 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
         dqm_b[t] = rg_cm_b;
         dqp_b[t] = rg_cp_b;
     }
-    const double kinh_u_b = tmp2_b, kinv_cp_b = tmp2_b * w2c0, kinv_c_b = tmp2_b * w2c1;
+    const double kinh_u_b = tmp2_b + k2k_kinh_u_b, kinv_cp_b = tmp2_b * w2c0, kinv_c_b = tmp2_b * w2c1 + k2k_kinv_c_b;
     xb.rg_c += rg_b;
 #pragma unroll
     for (int t = 0; t < 3; t++) {

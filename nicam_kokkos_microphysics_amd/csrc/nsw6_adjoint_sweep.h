@@ -93,7 +93,7 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
             load_sed(a, l, ij, k, si);
             const Unit u = load_unit(a.f, (int64_t)l * ijdim + ij);
             SedOut<double> so;
-            sediment(*a.grid, k, u, si, so);
+            sediment(*a.grid, k, u, si, so, a.kin2kin != 0);
             x.rhog = so.rg;
             x.rhoge = so.e;
             x.rhogq[QR] = so.rq[0];
@@ -112,7 +112,7 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
             sb.rw = cot_at(C.rhogw, p);
             SedInB s;
             zero(s);
-            sediment_b(*a.grid, k, u, si, sb, s);
+            sediment_b(*a.grid, k, u, si, sb, s, a.kin2kin != 0);
 #pragma unroll
             for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -151,11 +151,18 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
             dws[1] = xb.rhoge;
 #pragma unroll
             for (int n = 0; n < NQ; n++) dws[2 + n] = xb.rhogq[n];
-            // halo momentum passes through unchanged
+            // halo momentum passes through unchanged, except the top rhogw under KIN2KIN:
+            // kin2kin_v with rhogkin_v = 0 there (0 when w^2/2 > 1e-20, else rhogw)
             din[0] = cot_at(C.rhogvx, p);
             din[1] = cot_at(C.rhogvy, p);
             din[2] = cot_at(C.rhogvz, p);
             din[3] = cot_at(C.rhogw, p);
+            if (a.kin2kin && k == KALL - 1) {
+                const double rhogh = kin2kin_top_rhogh<3>(a, l, ij, x.rhog, FluxFromMemory<3>(a, l, ij));
+                double rhogh_b = 0.0, kinv_b = 0.0, rw_b = 0.0;
+                kin2kin_v_b(rhogh, 0.0, a.f.rhogw[p] + 0.0, din[3], rhogh_b, kinv_b, rw_b);
+                din[3] = rw_b;   // rhogh's cotangent is 0 on either side of the threshold
+            }
         }
     }
 #pragma unroll

@@ -426,8 +426,7 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
 static bool adjoint_supported(const DevParams &P)
 {
     const bool kk = (P.sw_kk2000 == 1.0), berry = (P.sw_kk2000 == 0.0);
-    return P.sw_roh2014 == 1.0 && (kk || berry) && P.sw_expice == 1.0 && P.kin2kin == 0.0 &&
-           P.four_water == 0.0;
+    return P.sw_roh2014 == 1.0 && (kk || berry) && P.sw_expice == 1.0 && P.four_water == 0.0;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)

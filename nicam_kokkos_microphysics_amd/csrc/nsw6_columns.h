@@ -281,7 +281,7 @@ template <int NT> struct FluxFromMemory {
 // sedimentation (its lower neighbour) is formed from the fluxes of walls kmax and
 // kmax+1 as sediment() forms it.  rhog_top is the halo cell's density.
 template <int NT, typename Fetch>
-NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top, const Fetch &FXV)
+NSW6_HD double kin2kin_top_rhogh(const StepArgs &a, int l, int ij, double rhog_top, const Fetch &FXV)
 {
     using X = FxSlots<NT>;
     const int ijdim = a.ijdim;
@@ -295,7 +295,13 @@ NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_to
     for (int t = 0; t < NT; t++) rg = rg + -(FXV(X::Q + t, KALL - 1) - FXV(X::Q + t, KMAX)) * rc;
     const double gs = a.f.gsgam2[o], gsh = a.f.gsgam2h[o], gs_m = a.f.gsgam2[om];
     const double c2w0 = g.afact[KALL - 1] / gs * gsh, c2w1 = g.bfact[KALL - 1] / gs_m * gsh;
-    return kin2kin_v(c2w0 * rhog_top + c2w1 * rg, 0.0 + 0.0, a.f.rhogw[o] + 0.0);
+    return c2w0 * rhog_top + c2w1 * rg;
+}
+template <int NT, typename Fetch>
+NSW6_HD double kin2kin_top_wall(const StepArgs &a, int l, int ij, double rhog_top, const Fetch &FXV)
+{
+    const int64_t o = (int64_t)l * KALL * a.ijdim + (int64_t)(KALL - 1) * a.ijdim + ij;
+    return kin2kin_v(kin2kin_top_rhogh<NT>(a, l, ij, rhog_top, FXV), 0.0 + 0.0, a.f.rhogw[o] + 0.0);
 }
 
 // Inputs of sediment() for interior cell c of column (l, ij); FXV(slot, wall) reads the

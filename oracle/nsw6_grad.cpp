@@ -293,7 +293,7 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                     si.z = h.z[p];
                     SedOut<S> so;
                     const Unit u = {h.ix[pc], h.iy[pc], h.iz[pc], h.jx[pc], h.jy[pc], h.jz[pc]};
-                    sediment(g, k, u, si, so);
+                    sediment(g, k, u, si, so, P.kin2kin != 0.0);
                     o_rvx[p] = so.rvx;
                     o_rvy[p] = so.rvy;
                     o_rvz[p] = so.rvz;
@@ -313,6 +313,17 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                     o_rvy[p] = h.rhogvy[p] + 0.0 * h.iy[pc] + 0.0 * h.jy[pc];
                     o_rvz[p] = h.rhogvz[p] + 0.0 * h.iz[pc] + 0.0 * h.jz[pc];
                     o_rw[p] = h.rhogw[p] + 0.0;
+                    if (P.kin2kin != 0.0 && k == KALL - 1) {
+                        // KIN2KIN at wall kmax+1: rhogkin_v = 0, the lower cell after sedimentation
+                        const S *fx = &flux[(int64_t)l * 10 * K2 + ij];
+                        S rg = wrhog[p - ijdim];
+                        for (int t = 0; t < 3; t++)
+                            rg = rg + -(fx[t * K2 + (int64_t)(KALL - 1) * ijdim] - fx[t * K2 + (int64_t)KMAX * ijdim]) *
+                                          g.rdgz[KMAX];
+                        const double c2w0 = g.afact[KALL - 1] / h.gsgam2[p] * h.gsgam2h[p];
+                        const double c2w1 = g.bfact[KALL - 1] / h.gsgam2[p - ijdim] * h.gsgam2h[p];
+                        o_rw[p] = kin2kin_v(S(c2w0 * (wrhog[p] + 0.0) + c2w1 * rg), S(0.0 + 0.0), S(h.rhogw[p] + 0.0));
+                    }
                     h.GPREC[p] = S(0.0);
                 }
                 x.rhogq[QV] = wrhogq[pq + QV * K2] + 0.0;
@@ -1064,7 +1075,7 @@ extern "C" {
 
 // Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
 // NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but Roh & Satoh snow,
-// sw_expice = 1, KIN2EIN, 3WATER with KK2000 or Berry's autoconversion, no OPT_INDIR
+// sw_expice = 1, 3WATER (KIN2EIN or KIN2KIN) with KK2000 or Berry's autoconversion, no OPT_INDIR
 // (the device reverse mode also covers OPT_INDIR).
 int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                       const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
@@ -1073,7 +1084,7 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
     DevParams P;
     to_dev_params(*params, P);
     if (!(P.sw_roh2014 == 1.0 && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) && P.sw_expice == 1.0 &&
-          P.opt_indir == 0.0 && P.kin2kin == 0.0 && P.four_water == 0.0))
+          P.opt_indir == 0.0 && P.four_water == 0.0))
         return NSW6_UNSUPPORTED;
     GridDev g;
     make_grid_dev(gz, gzh, g);
@@ -1115,6 +1126,7 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
     a.dt3_6 = a.dt3 / 6.0;
+    a.kin2kin = (P.kin2kin != 0.0);
     A.cot = host_fields(*cot);
     A.cot_sflux = cot_sflux;
     A.grad = host_fields(*grad);

@@ -48,3 +48,70 @@ def directional(grad, gcoef, d, dc):
             s += float(np.sum(g * d[k]))
             a += float(np.sum(np.abs(g * d[k])))
     return s, a
+
+
+def output_cotangents(st, gz, gzh, dt, P, seed):
+    """random_cotangents for any dt and parameters."""
+    rng = np.random.default_rng(seed)
+    out = F.copy_state(st)
+    assert O.step_fp64(out, gz, gzh, dt, P=P) == 0
+    cot = {k: rng.uniform(-1, 1, out[k].shape) / (float(np.sqrt(np.mean(out[k] ** 2))) or 1.0)
+           for k in F.ADJ_OUTPUTS}
+    return cot, rng.uniform(-1, 1, (st["rhog"].shape[0], 3, st["rhog"].shape[-1])) / 1e-3
+
+
+def compare_with_host(st, gz, gzh, dt, P, cot, cs, g_gpu, gc_gpu, g_host, gc_host, max_kinks=32):
+    """The GPU reverse mode against the host run of the same passes, elementwise
+    within 1e-9 of each field's max-abs, and against the host step's dual numbers,
+    directionally within 1e-9 of the absolute terms.
+
+    Two kinds of points have no derivative, and there the two builds may differ:
+    where a tracer's input rhogq is exactly 0 (the rain fall speed goes as
+    (rho q)^(1/8) behind the zero switch and the value after the rates is round-off),
+    left out; and the kinks of the rates' limiters, where the last bits of each
+    build's arithmetic decide the side: at each (at most max_kinks entries) the GPU's
+    value must be one of the host step's one-sided difference quotients and the
+    host's the other, and the directions vanish there.  Returns the kink count."""
+    def J(s):
+        j, _ = O.dual_vjp(s, gz, gzh, dt, cot, cs, [{}], np.zeros((1, 8)), P=P)
+        return j
+
+    kinks = []
+    for k in F.ADJ_INPUTS:
+        keep = (st[k] != 0) if k == "rhogq" else np.ones(g_host[k].shape, dtype=bool)
+        scale = np.abs(g_host[k][keep]).max()
+        off = (np.abs(g_gpu[k] - g_host[k]) > 1e-9 * scale) & keep
+        kinks += [(k, tuple(int(v) for v in i)) for i in np.argwhere(off)]
+    assert len(kinks) <= max_kinks, len(kinks)
+    j0 = J(st) if kinks else None
+    near = lambda a, b: abs(a - b) <= 1e-3 * max(abs(a), abs(b))
+    for k, i in kinks:
+        x0 = st[k][i]
+        h = 1e-7 * abs(x0) or 1e-12
+        sp, sm = F.copy_state(st), F.copy_state(st)
+        sp[k][i] = x0 + h
+        sm[k][i] = x0 - h
+        right, left = (J(sp) - j0) / h, (j0 - J(sm)) / h
+        assert not near(right, left), (k, i, right, left)
+        assert (near(g_gpu[k][i], right) and near(g_host[k][i], left)) or \
+               (near(g_gpu[k][i], left) and near(g_host[k][i], right)), \
+            (k, i, g_gpu[k][i], g_host[k][i], left, right)
+    np.testing.assert_allclose(gc_gpu, gc_host, rtol=1e-9, atol=0)
+    dirs, dcoef = random_directions(st, 17, 4)
+    rng = np.random.default_rng(4)
+    for key in STATE_KEYS[:4]:
+        dirs.append({key: st[key] * rng.uniform(-1, 1, st[key].shape)})
+    dcoef = np.concatenate([dcoef, np.zeros((4, 8))])
+    pts = {(i[0],) + i[-2:] for _, i in kinks}
+    for d in dirs:
+        for v in d.values():
+            for p in pts:
+                if v.ndim == 3:
+                    v[p] = 0.0
+                else:
+                    v[p[0], :, p[1], p[2]] = 0.0
+    _, dj = O.dual_vjp(st, gz, gzh, dt, cot, cs, dirs, dcoef, P=P)
+    for i in range(len(dirs)):
+        s, a = directional(g_gpu, gc_gpu, dirs[i], dcoef[i])
+        assert abs(s - dj[i]) <= 1e-9 * a, (i, s, dj[i], a)
+    return len(kinks)

@@ -24,7 +24,8 @@ import pytest
 from nicam_kokkos_microphysics_amd import _native as N
 from nicam_kokkos_microphysics_amd import fields as F
 
-from adjoint_helpers import DT, STATE_KEYS, coefs, directional, random_cotangents, random_directions
+from adjoint_helpers import (DT, STATE_KEYS, coefs, compare_with_host, directional, output_cotangents,
+                             random_cotangents, random_directions)
 
 pytestmark = pytest.mark.gpu
 
@@ -192,35 +193,17 @@ def _general_case(case):
     return gz, gzh, st, 60.0, Po, Pg
 
 
-def _cotangents(st, gz, gzh, dt, P, seed):
-    from oracle import oracle as O
-    rng = np.random.default_rng(seed)
-    out = F.copy_state(st)
-    assert O.step_fp64(out, gz, gzh, dt, P=P) == 0
-    cot = {k: rng.uniform(-1, 1, out[k].shape) / (float(np.sqrt(np.mean(out[k] ** 2))) or 1.0)
-           for k in F.ADJ_OUTPUTS}
-    return cot, rng.uniform(-1, 1, (st["rhog"].shape[0], 3, st["rhog"].shape[-1])) / 1e-3
-
-
 @pytest.mark.parametrize("case", ["fast_rain", "reference_grid_dt60"])
 def test_gpu_reverse_general_regions(case):
     """Every region general: the GPU reverse mode (k_adj_wall_general) against the host
-    run of the same passes, elementwise within 1e-9 of each field's max-abs, and
-    against forward-mode dual numbers of the host step (directional, 1e-9 of the
-    absolute terms), along random directions and per input field.
-
-    Two kinds of points have no derivative, and there the two builds may legitimately
-    differ: where a tracer's input rhogq is exactly 0 (the rain fall speed goes as
-    (rho q)^(1/8) behind the zero switch; the value after the rates is round-off), and
-    at the kinks of the rates' limiters, where the last bits of each build's arithmetic
-    decide the side.  The first are left out; at each point of the second kind (at
-    most 32) the GPU's value must be one of the host step's one-sided difference
-    quotients and the host's the other, and the directions vanish there."""
+    run of the same passes and against dual numbers (compare_with_host: elementwise
+    1e-9 of max-abs and directional 1e-9, points without a derivative handled
+    apart)."""
     import torch
     from oracle import oracle as O
     from nicam_kokkos_microphysics_amd.driver import NSW6
     gz, gzh, st, dt, Po, Pg = _general_case(case)
-    cot, cs = _cotangents(st, gz, gzh, dt, Po, 7)
+    cot, cs = output_cotangents(st, gz, gzh, dt, Po, 7)
     g_host, gc_host, status = O.adjoint_host(st, gz, gzh, dt, cot, cs, P=Po)
     assert status == 0
     lall, ijdim = st["rhog"].shape[0], st["rhog"].shape[-1]
@@ -236,52 +219,8 @@ def test_gpu_reverse_general_regions(case):
         drv.close()
     assert rep.status == 0 and rep.n_general_regions == lall
     g_gpu = {k: v.cpu().numpy() for k, v in grad.items()}
-    gc_gpu = gc.cpu().numpy()
-
-    def J(s):
-        j, _ = O.dual_vjp(s, gz, gzh, dt, cot, cs, [{}], np.zeros((1, 8)), P=Po)
-        return j
-
-    kinks = []          # (field, index) where the two builds took different sides
-    for k in F.ADJ_INPUTS:
-        keep = (st[k] != 0) if k == "rhogq" else np.ones(g_host[k].shape, dtype=bool)
-        scale = np.abs(g_host[k][keep]).max()
-        off = (np.abs(g_gpu[k] - g_host[k]) > 1e-9 * scale) & keep
-        kinks += [(k, tuple(int(v) for v in i)) for i in np.argwhere(off)]
-    assert len(kinks) <= 32, len(kinks)
-    j0 = J(st) if kinks else None
-    for k, i in kinks:
-        x0 = st[k][i]
-        h = 1e-7 * abs(x0) or 1e-12
-        sp, sm = F.copy_state(st), F.copy_state(st)
-        sp[k][i] = x0 + h
-        sm[k][i] = x0 - h
-        right, left = (J(sp) - j0) / h, (j0 - J(sm)) / h
-        near = lambda a, b: abs(a - b) <= 1e-3 * max(abs(a), abs(b))
-        assert not near(right, left), (k, i, right, left)       # a kink indeed
-        assert (near(g_gpu[k][i], right) and near(g_host[k][i], left)) or \
-               (near(g_gpu[k][i], left) and near(g_host[k][i], right)), (k, i, g_gpu[k][i], g_host[k][i], left, right)
-    print(f"{len(kinks)} kink entries", flush=True)
-    np.testing.assert_allclose(gc_gpu, gc_host, rtol=1e-9, atol=0)
-
-    dirs, dcoef = random_directions(st, 17, 4)
-    rng = np.random.default_rng(4)
-    for key in STATE_KEYS[:4]:
-        dirs.append({key: st[key] * rng.uniform(-1, 1, st[key].shape)})
-    dcoef = np.concatenate([dcoef, np.zeros((4, 8))])
-    # no direction moves an input at a kink point
-    pts = {(i[0],) + i[-2:] for _, i in kinks}
-    for d in dirs:
-        for key, v in d.items():
-            for p in pts:
-                if v.ndim == 3:
-                    v[p] = 0.0
-                else:
-                    v[p[0], :, p[1], p[2]] = 0.0
-    _, dj = O.dual_vjp(st, gz, gzh, dt, cot, cs, dirs, dcoef, P=Po)
-    for i in range(len(dirs)):
-        s, a = directional(g_gpu, gc_gpu, dirs[i], dcoef[i])
-        assert abs(s - dj[i]) <= 1e-9 * a, (i, s, dj[i], a)
+    n = compare_with_host(st, gz, gzh, dt, Po, cot, cs, g_gpu, gc.cpu().numpy(), g_host, gc_host)
+    print(f"{n} kink entries", flush=True)
 
 
 def test_autograd_function_and_two_step_chain():

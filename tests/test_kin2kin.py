@@ -14,7 +14,12 @@ internal energy.
     wherever w is not negligible;
   - the internal energy is not corrected: rhoge differs from the KIN2EIN step.
 * GPU: parity with the oracle within 1e-10 of max-abs, single-cell and general
-  regions; the reverse mode reports the scheme as unsupported.
+  regions.
+* Reverse mode (the momentum rescaling of kin2kin_h/kin2kin_v and the top wall's
+  reset differentiated in sediment_b and adj_cell): the host passes against
+  forward-mode dual numbers (1e-13 of the absolute terms, random directions and one
+  per input field), and the GPU against the host passes and the dual numbers, on
+  single-cell and general regions.
 """
 import numpy as np
 import pytest
@@ -109,7 +114,62 @@ def test_gpu_kin2kin_matches_oracle(vscale):
     assert not bad, bad
     if vscale > 1:
         assert reps[0].n_general_regions > 0
-    with pytest.raises(NSW6Error) as e:
-        drv.adjoint(F.to_device(st), {"precip": dev["precip"]}, DT)
-    assert e.value.code == N.NSW6_UNSUPPORTED
     drv.close()
+
+
+def _reverse_case(vscale):
+    from oracle import oracle as O
+    gz, gzh, st = _state(lall=2, ijdim=24 if vscale == 1.0 else 32, seed=31)
+    P = O.params()
+    P.kin2kin = 1.0
+    P.Cr = P.Cr * vscale
+    return gz, gzh, st, P
+
+
+@pytest.mark.parametrize("vscale", [1.0, 25.0])
+def test_kin2kin_reverse_matches_dual_numbers(vscale):
+    from oracle import oracle as O
+    from adjoint_helpers import STATE_KEYS, directional, output_cotangents, random_directions
+    gz, gzh, st, P = _reverse_case(vscale)
+    cot, cs = output_cotangents(st, gz, gzh, DT, P, 7)
+    grad, gc, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=P)
+    assert status == 0
+    dirs, dcoef = random_directions(st, 13, 4)
+    rng = np.random.default_rng(4)
+    for key in STATE_KEYS:
+        dirs.append({key: st[key] * rng.uniform(-1, 1, st[key].shape)})
+    dcoef = np.concatenate([dcoef, np.zeros((len(STATE_KEYS), 8))])
+    _, dj = O.dual_vjp(st, gz, gzh, DT, cot, cs, dirs, dcoef, P=P)
+    for i in range(len(dirs)):
+        s, a = directional(grad, gc, dirs[i], dcoef[i])
+        assert abs(s - dj[i]) <= 1e-13 * a, (i, s, dj[i], a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vscale", [1.0, 25.0])
+def test_gpu_kin2kin_reverse(vscale):
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    from adjoint_helpers import compare_with_host, output_cotangents
+    gz, gzh, _, P_ora = _reverse_case(vscale)
+    _, _, st = _state(lall=2, ijdim=300, seed=32)
+    cot, cs = output_cotangents(st, gz, gzh, DT, P_ora, 7)
+    g_host, gc_host, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=P_ora)
+    assert status == 0
+    P_gpu = default_params()
+    P_gpu.kin2kin = 1.0
+    P_gpu.Cr = P_gpu.Cr * vscale
+    drv = NSW6(gz, gzh, params=P_gpu, max_columns=600)
+    try:
+        dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+        print(f"KIN2KIN adjoint vscale={vscale}", flush=True)
+        grad, gc = drv.adjoint(F.to_device(st), dcot, DT, cot_sflux=torch.from_numpy(np.ascontiguousarray(cs)).cuda())
+        torch.cuda.synchronize()
+        rep = drv.report()
+    finally:
+        drv.close()
+    assert rep.status == 0 and (rep.n_general_regions > 0) == (vscale > 1.0)
+    n = compare_with_host(st, gz, gzh, DT, P_ora, cot, cs, {k: v.cpu().numpy() for k, v in grad.items()},
+                          gc.cpu().numpy(), g_host, gc_host)
+    print(f"{n} kink entries", flush=True)
