@@ -258,10 +258,12 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  * rhokin, GPREC and the surface mass fluxes averaged over them (their cotangents reach
  * every sub-step times 1/n), the state after the last sub-step.  The sub-step inputs
  * are rebuilt by n-1 forward sub-steps into context scratch of 21*96 doubles per
- * column for each of them, plus 2*21*96 + 7*96 + 9 doubles per column and the
- * region flags (grown on demand).  A region that takes the general path in any
- * sub-step is counted as above.  Timing ids 6/7 then cover the forward rebuild and
- * the reversed sub-steps (each with its own forward part). */
+ * column for each of them, plus 2*21*96 + 7*96 + 9 doubles per column, a second step
+ * workspace of about 22*96 doubles per column, and the region flags (grown on demand).
+ * The last sub-step's forward part goes to the second workspace, so that sub-step
+ * n-2 reuses the forward part its rebuild left in the first.  A region that takes the
+ * general path in any sub-step is counted as above.  Timing ids 6/7 then cover the
+ * forward rebuild and the reversed sub-steps. */
 int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                        const nsw6_fields *state, const nsw6_fields *cot,
                        const double *cot_sflux, const nsw6_fields *grad,

@@ -159,16 +159,19 @@ extern "C" hipError_t nsw6_launch_merge_flags(int *acc, const int *flags, int n,
 extern "C" int nsw6_adjoint_blocks(int lall, int ijdim) { return level_grid((ijdim + 255) / 256, KALL, lall); }
 
 // Events (NULL or 3): before the forward part, after it, after the reverse passes.
+// forward = 0 skips the forward part (the workspace already holds it).
 extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part, double *grad_coef,
-                                          hipStream_t stream, hipEvent_t *ev)
+                                          hipStream_t stream, hipEvent_t *ev, int forward)
 {
     AdjLaunch L;
     L.A = *adj;
     L.coef_part = coef_part;
     L.grad_coef = grad_coef;
     if (ev) (void)hipEventRecord(ev[0], stream);
-    hipError_t e = nsw6_launch_forward_to_flux(&L.A.a, stream);
-    if (e != hipSuccess) return e;
+    if (forward) {
+        hipError_t e = nsw6_launch_forward_to_flux(&L.A.a, stream);
+        if (e != hipSuccess) return e;
+    }
     if (ev) (void)hipEventRecord(ev[1], stream);
     const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
     hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);

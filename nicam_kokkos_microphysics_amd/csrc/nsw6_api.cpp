@@ -459,9 +459,11 @@ static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)
 
 // One reverse step of dt (the whole of nsw6_mp_driver_adj at MP_DIV_NUM = 1).  The
 // diagnostics of the recomputed forward part go to context scratch.
+// One FP64 reverse-mode step on workspace `ws`; forward = 0 when `ws` already holds the
+// forward part of `state` (nsw6_launch_adjoint).
 static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state, const nsw6_fields *cot,
                     const double *cot_sflux, const nsw6_fields *grad, double *grad_coef, double dt,
-                    int keep_report, hipStream_t s, hipEvent_t *ev)
+                    int keep_report, hipStream_t s, hipEvent_t *ev, const Workspace &ws, int forward)
 {
     AdjArgs A;
     memset(&A, 0, sizeof A);
@@ -484,7 +486,7 @@ static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state
     a.f.precip_rhophi = dc + 11 * C;
     a.f.precip_rhokin = dc + 12 * C;
     fill_step_args(ctx, lall, ijdim, dt, a);
-    a.ws = ctx->ws;
+    a.ws = ws;
     a.keep_report = keep_report;
     A.cot = to_fields(*cot);
     A.cot_sflux = cot_sflux;
@@ -494,7 +496,7 @@ static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state
     A.wr = A.cr + (size_t)NCR * K2;
     A.dws = A.wr + (size_t)NWR * K2;
     A.din = A.dws + (size_t)NDW * K2;
-    return hip_ok(nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev));
+    return hip_ok(nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev, forward));
 }
 
 // The arrays one sub-step hands to the next: rhog, the momenta, rhoge, rhogq, tem and q
@@ -564,8 +566,13 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     const size_t C = (size_t)lall * (size_t)ijdim, K2 = (size_t)KALL * C;
     // layout: n-1 carries | 2 cotangent carries | vx, vy, vz, unccn | write-only rows (qd
     // and the constant diagnostics) | scaled GPREC | scaled column cotangents | coefficients
+    // | a second step workspace (3WATER) for the forward part of the last sub-step, so
+    // that the step workspace keeps the forward part of sub-step n-2 from the rebuild
     const size_t n_carry = (size_t)(n - 1) * CARRY_K2 * K2, n_cot = 2 * (size_t)CARRY_K2 * K2;
-    const size_t doubles = n_carry + n_cot + 4 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF;
+    const size_t alt_part = (size_t)KALL * 16 * (C / EXT_COLS_PER_BLOCK + (size_t)lall);
+    const size_t alt_ints = (size_t)EXT_PER_REGION_MAX * (size_t)lall + (size_t)lall + REPORT_INTS + C;
+    const size_t n_alt = (size_t)(1 + 1 + NQ + 3 + FLUX_SLOTS) * K2 + alt_part + (alt_ints + 1) / 2;
+    const size_t doubles = n_carry + n_cot + 4 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF + n_alt;
     int st = ensure_div_adj(ctx, doubles, lall);
     if (st != NSW6_OK) return st;
     double *base = ctx->div_adj;
@@ -575,6 +582,21 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     double *gprec_s = junk + 2 * K2;
     double *col_s = gprec_s + K2;
     double *coef_tmp = col_s + 9 * C;
+    Workspace alt{};
+    {
+        double *a = coef_tmp + NSW6_NCOEF;
+        alt.rhog = a; a += K2;
+        alt.rhoge = a; a += K2;
+        alt.rhogq = a; a += NQ * K2;
+        alt.vt = a; a += 3 * K2;
+        alt.flux = a; a += (size_t)FLUX_SLOTS * K2;
+        alt.part = a; a += alt_part;
+        int *ip = reinterpret_cast<int *>(a);
+        alt.bounds = ip; ip += (size_t)EXT_PER_REGION_MAX * (size_t)lall;
+        alt.region_flag = ip; ip += lall;
+        alt.report = ip; ip += REPORT_INTS;
+        alt.cloud_top = ip;
+    }
     auto carry = [&](int m) { return carry_at(base + (size_t)(m - 1) * CARRY_K2 * K2, K2); };   // m = 1..n-1
     auto cotc = [&](int i) { return carry_at(cotbuf + (size_t)i * CARRY_K2 * K2, K2); };
     hipError_t e = hipSuccess;
@@ -680,7 +702,11 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
         g_m.vz = grad->vz ? (first ? grad->vz : vtmp + 2 * K2) : nullptr;
         g_m.unccn = grad->unccn ? (first ? grad->unccn : vtmp + 3 * K2) : nullptr;
         double *coef_m = grad_coef ? (first ? grad_coef : coef_tmp) : nullptr;
-        st = adj_step(ctx, lall, ijdim, &st_m, &c_m, sflux_s, &g_m, coef_m, dt_mp, 1, s, nullptr);
+        // the rebuild left the forward part of sub-step n-2 in the step workspace: the last
+        // sub-step recomputes its own in the second workspace, and sub-step n-2 reuses it
+        const Workspace &use = (m == n - 1) ? alt : ctx->ws;
+        st = adj_step(ctx, lall, ijdim, &st_m, &c_m, sflux_s, &g_m, coef_m, dt_mp, 1, s, nullptr, use,
+                      m == n - 2 ? 0 : 1);
         if (st != NSW6_OK) return st;
         if (!first) {
             if (grad->vx && !ok(nsw6_launch_accumulate(grad->vx, vtmp, (int64_t)K2, s))) return NSW6_HIP_ERROR;
@@ -692,7 +718,7 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
             if (grad_coef && !ok(nsw6_launch_accumulate(grad_coef, coef_tmp, NSW6_NCOEF, s))) return NSW6_HIP_ERROR;
         }
         // a region that took the general path in any sub-step is counted
-        if (!ok(nsw6_launch_merge_flags(ctx->div_adj_flags, ctx->ws.region_flag, lall, first, s)))
+        if (!ok(nsw6_launch_merge_flags(ctx->div_adj_flags, use.region_flag, lall, first, s)))
             return NSW6_HIP_ERROR;
     }
     if (!ok(hipMemcpyAsync(ctx->ws.region_flag, ctx->div_adj_flags, sizeof(int) * (size_t)lall,
@@ -733,7 +759,7 @@ int nsw6_mp_driver_adj(nsw6_ctx *ctx, int l_region, int lall, int ijdim, const n
         ev = pt.ev;
     }
     if (ctx->div_num == 1)
-        st = adj_step(ctx, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt, 0, s, ev);
+        st = adj_step(ctx, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt, 0, s, ev, ctx->ws, 1);
     else
         st = adj_subcycled(ctx, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt, s, ev);
     if (ctx->timing) ctx->pending_adj.push_back(pt);

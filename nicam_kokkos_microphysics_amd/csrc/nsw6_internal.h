@@ -172,11 +172,13 @@ extern "C" hipError_t nsw6_launch_div_scale(const nsw6::StepArgs *args, const do
                                             hipStream_t stream);
 
 // events recorded by nsw6_launch_adjoint: before the forward part, after it, after the
-// reverse passes
+// reverse passes.  forward = 0: adj->a.ws already holds the forward part (the rates'
+// state and fall speeds, kcell bounds and region flags, wall fluxes) of this input
+// state, so it is not recomputed.
 constexpr int NSW6_ADJ_EVENTS = 3;
 extern "C" int nsw6_adjoint_blocks(int lall, int ijdim);
 extern "C" hipError_t nsw6_launch_adjoint(const nsw6::AdjArgs *adj, double *coef_part, double *grad_coef,
-                                          hipStream_t stream, hipEvent_t *ev);
+                                          hipStream_t stream, hipEvent_t *ev, int forward);
 // Sub-cycled reverse mode: y = a x, y += x (n doubles), acc = flags (first) or acc | flags.
 extern "C" hipError_t nsw6_launch_scale(double *y, const double *x, double a, int64_t n, hipStream_t stream);
 extern "C" hipError_t nsw6_launch_accumulate(double *y, const double *x, int64_t n, hipStream_t stream);
