@@ -18,16 +18,23 @@
 namespace nsw6 {
 
 // Vt(k-3 .. k+1) of one tracer of one column (levels below 0 are never used: they read
-// level 0 and take 0, so the loads need no branch and issue together).
-NSW6_HD void load_vt_window(const double *vt_col, int64_t ijdim, int k, double V[5])
+// level 0 and take 0, so the loads need no branch and issue together).  The fall speeds
+// are stored as WS: double, or float in the mixed-precision form, whose FP32 rate
+// kernel computes them in float (the stored values are its results exactly).
+template <typename WS = double>
+NSW6_HD void load_vt_window(const WS *vt_col, int64_t ijdim, int k, double V[5])
 {
 #pragma unroll
     for (int d = -3; d <= 1; d++) {
         const int kk = (k + d >= 0) ? k + d : 0;
-        const double v = vt_col[(int64_t)kk * ijdim];
+        const double v = (double)vt_col[(int64_t)kk * ijdim];
         V[d + 3] = (k + d >= 0) ? v : 0.0;
     }
 }
+// The fall-speed and flux workspace as WS (the arrays are allocated as doubles; a float
+// form uses the first half of each).
+template <typename WS> NSW6_HD WS *ws_vt(const StepArgs &a) { return reinterpret_cast<WS *>(a.ws.vt); }
+template <typename WS> NSW6_HD WS *ws_flux(const StepArgs &a) { return reinterpret_cast<WS *>(a.ws.flux); }
 
 // Pre-sedimentation cell j of the column at p2 = l*K2 + ij, p3 = l*NQ*K2 + ij.
 NSW6_HD void load_cell(const StepArgs &a, int64_t p2, int64_t p3, int64_t K2, int ijdim, int j, CellIn<double> &r)
@@ -96,7 +103,7 @@ template <int NT> struct WallStateN {
 using WallState = WallStateN<3>;
 // Every load of the wall first (the cells, the fall speeds, the kcell bounds), then the
 // derived cells and the departure geometry: one memory round trip per wall.
-template <int NT>
+template <int NT, typename WS = double>
 NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> &s)
 {
     const int ijdim = a.ijdim;
@@ -112,7 +119,7 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> 
     int kcmax[NT], khmax[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, s.V[t]);
+        load_vt_window(ws_vt<WS>(a) + pv + t * K2, ijdim, w, s.V[t]);
         kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
         khmax[t] = bnd[bounds_index(t, 1, w, 1)];
     }
@@ -124,7 +131,7 @@ NSW6_HD void wall_state(const StepArgs &a, int l, int w, int ij, WallStateN<NT> 
 // Flux slots of wall w (FxSlots<NT>) through put(slot, value), frain = GPREC
 // (src/mod_precip_transport.cpp:916-927) and, at wall kmin, the surface fluxes.  Walls 0
 // and 95 carry no flux.
-template <int NT, typename Put>
+template <int NT, typename Put, typename WS = double>
 NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
 {
     using X = FxSlots<NT>;
@@ -145,7 +152,7 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
     // fetch the grid and kcell tables (uniform) through the vector memory path, one
     // round trip per step of the departure-cell search, instead of the scalar cache.
     WallStateN<NT> s;
-    wall_state<NT>(a, l, w, ij, s);
+    wall_state<NT, WS>(a, l, w, ij, s);
     WallFluxN<double, NT> F;
     double ft[NT][4];
     wall_flux(*a.grid, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, F, ft);
@@ -178,7 +185,7 @@ NSW6_HD void flux_wall_to(const StepArgs &a, int l, int w, int ij, Put &&put)
 
 // Wall w of a general region: the same slots as flux_wall_to, from the fluxes over the
 // region's kcell ranges (wall_flux_general).
-template <int NT, typename Put>
+template <int NT, typename Put, typename WS = double>
 NSW6_HD void flux_wall_general_to(const StepArgs &a, int l, int w, int ij, Put &&put)
 {
     using X = FxSlots<NT>;
@@ -202,7 +209,7 @@ NSW6_HD void flux_wall_general_to(const StepArgs &a, int l, int w, int ij, Put &
     int kcmin[NT], kcmax[NT], khmin[NT], khmax[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
+        load_vt_window(ws_vt<WS>(a) + pv + t * K2, ijdim, w, V[t]);
         kcmin[t] = bnd[bounds_index(t, 0, w, 0)];
         kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
         khmin[t] = bnd[bounds_index(t, 1, w, 0)];
@@ -248,32 +255,35 @@ NSW6_HD void flux_wall_general_to(const StepArgs &a, int l, int w, int ij, Put &
     }
 }
 
-// The same into the flux workspace [l][FxSlots<NT>::N][k][ij].
-template <int NT = 3>
+// The same into the flux workspace [l][FxSlots<NT>::N][k][ij] of WS (fall speeds and
+// fluxes as float in the mixed-precision form).
+template <int NT = 3, typename WS = double>
 NSW6_HD void flux_wall(const StepArgs &a, int l, int w, int ij)
 {
     const int64_t K2 = (int64_t)KALL * a.ijdim;
-    double *fx = a.ws.flux + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
-    flux_wall_to<NT>(a, l, w, ij, [&](int slot, double v) { fx[slot * K2] = v; });
+    WS *fx = ws_flux<WS>(a) + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
+    auto put = [&](int slot, double v) { fx[slot * K2] = (WS)v; };
+    flux_wall_to<NT, decltype(put) &, WS>(a, l, w, ij, put);
 }
 
-template <int NT = 3>
+template <int NT = 3, typename WS = double>
 NSW6_HD void flux_wall_general(const StepArgs &a, int l, int w, int ij)
 {
     const int64_t K2 = (int64_t)KALL * a.ijdim;
-    double *fx = a.ws.flux + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
-    flux_wall_general_to<NT>(a, l, w, ij, [&](int slot, double v) { fx[slot * K2] = v; });
+    WS *fx = ws_flux<WS>(a) + (int64_t)l * FxSlots<NT>::N * K2 + (int64_t)w * a.ijdim + ij;
+    auto put = [&](int slot, double v) { fx[slot * K2] = (WS)v; };
+    flux_wall_general_to<NT, decltype(put) &, WS>(a, l, w, ij, put);
 }
 
 // Reads of the flux workspace: slot of wall w of column (l, ij).
-template <int NT> struct FluxFromMemory {
-    const double *fx;
+template <int NT, typename WS = double> struct FluxFromMemory {
+    const WS *fx;
     int64_t K2;
     int ijdim;
     NSW6_HD FluxFromMemory(const StepArgs &a, int l, int ij)
-        : fx(a.ws.flux + (int64_t)l * FxSlots<NT>::N * (int64_t)KALL * a.ijdim + ij),
+        : fx(ws_flux<WS>(a) + (int64_t)l * FxSlots<NT>::N * (int64_t)KALL * a.ijdim + ij),
           K2((int64_t)KALL * a.ijdim), ijdim(a.ijdim) {}
-    NSW6_HD double operator()(int slot, int w) const { return fx[(int64_t)slot * K2 + (int64_t)w * ijdim]; }
+    NSW6_HD double operator()(int slot, int w) const { return (double)fx[(int64_t)slot * K2 + (int64_t)w * ijdim]; }
 };
 
 // KIN2KIN at wall kmax+1 of a single-cell region, for the thread of halo cell

@@ -79,7 +79,7 @@ __device__ __forceinline__ double wave_min(double v)
 // size.  The block's 12 extremes (3 tracers x {full, half} x {max, min}) go to the
 // partials array [l][k][4 NT][block] that k_bounds reduces (NT = 3 or 4 tracers).
 // ---------------------------------------------------------------------------------
-template <int NT>
+template <int NT, typename WS = double>
 __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(EXT_THREADS) k_extremes(StepArgs a)
     const GridDev &g = *a.grid;
     const double dt = a.dt, dt2 = a.dt2, dt3 = a.dt3_6;
     const int64_t K2 = (int64_t)KALL * ijdim;
-    const double *vt = a.ws.vt + (int64_t)l * NT * K2;
+    const WS *vt = ws_vt<WS>(a) + (int64_t)l * NT * K2;
 
     double e[4 * NT];
 #pragma unroll
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(192) k_bounds(StepArgs a)
 // ---------------------------------------------------------------------------------
 // Three waves per SIMD: the cells, the fall speeds and the kcell bounds of the wall are
 // all requested up front (one memory round trip per wave) at 150 VGPRs without spills.
-template <int NT>
+template <int NT, typename WS = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     if (a.ws.region_flag[lb.l] != 0) return;
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
-    flux_wall<NT>(a, lb.l, lb.k, ij);
+    flux_wall<NT, WS>(a, lb.l, lb.k, ij);
 }
 
 // The walls of the general regions (departures beyond the next cell, upward, or out
@@ -222,7 +222,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 // (level, 256-column block) tiles, which keeps the launch cheap when no region needs
 // it (a tenth of the blocks of k_flux's grid).  The blocks that one XCD receives
 // (every NUM_XCD-th) take consecutive tiles of one chunk, i.e. neighbouring levels of
-// the same columns, whose cells they share through that XCD's L2.
+// the same columns, whose cells they share through that XCD's L2.  Two waves per SIMD:
+// unforced the kernel takes 276 registers and runs at one (2.82 ms against 2.35 ms
+// for every region general at 169,000 x 96, with 20 registers spilled at two).
 constexpr int GENERAL_TILES = 4;
 static inline int general_blocks(int ijdim)
 {
@@ -230,8 +232,8 @@ static inline int general_blocks(int ijdim)
     const int per_xcd = (total + NUM_XCD * GENERAL_TILES - 1) / (NUM_XCD * GENERAL_TILES);
     return per_xcd * NUM_XCD;
 }
-template <int NT>
-__global__ void __launch_bounds__(256) k_flux_general(StepArgs a)
+template <int NT, typename WS = double>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_flux_general(StepArgs a)
 {
     const int l = blockIdx.y;
     if (a.ws.region_flag[l] == 0) return;
@@ -243,7 +245,7 @@ __global__ void __launch_bounds__(256) k_flux_general(StepArgs a)
         const int L = xcd * per + j;
         if (L >= total) break;
         const int ij = (L / KALL) * 256 + threadIdx.x;
-        if (ij < ijdim) flux_wall_general<NT>(a, l, L % KALL, ij);
+        if (ij < ijdim) flux_wall_general<NT, WS>(a, l, L % KALL, ij);
     }
 }
 
@@ -329,7 +331,7 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
 
 // Three waves per SIMD: with the registers four waves leave (126 of 128 used) the
 // kernel is about 1 % slower.
-template <int NT>
+template <int NT, typename WS = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_finish(StepArgs a)
 {
     const int ijdim = a.ijdim;
@@ -337,26 +339,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     if (!lb.valid) return;
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
-    finish_point<NT>(a, lb.l, lb.k, ij, FluxFromMemory<NT>(a, lb.l, ij));
+    finish_point<NT>(a, lb.l, lb.k, ij, FluxFromMemory<NT, WS>(a, lb.l, ij));
 }
 
 // ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
 // The transport of NT sedimenting tracers: extremes and kcell bounds, the wall fluxes
-// of the single-cell and of the general regions, then the point-wise finish.
-template <int NT> static void launch_transport(const StepArgs &a, hipStream_t stream, hipEvent_t *ev)
+// of the single-cell and of the general regions, then the point-wise finish.  WS is the
+// storage of the fall speeds and the wall fluxes: double, or float for the mixed form.
+template <int NT, typename WS> static void launch_transport(const StepArgs &a, hipStream_t stream, hipEvent_t *ev)
 {
     const dim3 grid_e(level_grid((a.ijdim + EXT_COLS_PER_BLOCK - 1) / EXT_COLS_PER_BLOCK, KMAX - KMIN + 1, a.lall));
     const dim3 grid_l(level_grid((a.ijdim + 255) / 256, KALL, a.lall));
-    hipLaunchKernelGGL(k_extremes<NT>, grid_e, dim3(EXT_THREADS), 0, stream, a);
+    hipLaunchKernelGGL((k_extremes<NT, WS>), grid_e, dim3(EXT_THREADS), 0, stream, a);
     hipLaunchKernelGGL(k_bounds<NT>, dim3(2 * NT, a.lall), dim3(192), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[2], stream);
-    hipLaunchKernelGGL(k_flux<NT>, grid_l, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((k_flux<NT, WS>), grid_l, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    hipLaunchKernelGGL(k_flux_general<NT>, dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((k_flux_general<NT, WS>), dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
-    hipLaunchKernelGGL(k_finish<NT>, grid_l, dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((k_finish<NT, WS>), grid_l, dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[5], stream);
 }
 
@@ -375,10 +378,17 @@ extern "C" hipError_t nsw6_launch_step(const StepArgs *args, int precision, hipS
     if (a.cloudtop_intended)
         hipLaunchKernelGGL(k_cloudtop, dim3((a.ijdim + 255) / 256, a.lall), dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[1], stream);
-    if (a.four_water)
-        launch_transport<4>(a, stream, ev);
-    else
-        launch_transport<3>(a, stream, ev);
+    if (precision == 1) {
+        if (a.four_water)
+            launch_transport<4, float>(a, stream, ev);
+        else
+            launch_transport<3, float>(a, stream, ev);
+    } else {
+        if (a.four_water)
+            launch_transport<4, double>(a, stream, ev);
+        else
+            launch_transport<3, double>(a, stream, ev);
+    }
     return hipGetLastError();
 }
 

@@ -52,13 +52,15 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
     const int nt = a.four_water ? 4 : 3;
     const int64_t pv = (int64_t)l * nt * K2 + (int64_t)k * ijdim + ij;   // + t*K2
     const Fields &f = a.f;
+    // the fall speeds in the rate type: FP64, or FP32 for the mixed form's transport
+    R *vt = reinterpret_cast<R *>(a.ws.vt);
 
     if (k == 0 || k == KALL - 1) {
         a.ws.rhog[p] = f.rhog[p];
         a.ws.rhoge[p] = f.rhoge[p];
 #pragma unroll
         for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
-        for (int t = 0; t < nt; t++) a.ws.vt[pv + t * K2] = 0.0;
+        for (int t = 0; t < nt; t++) vt[pv + t * K2] = R(0);
         if (DIAG) {
             f.re_liquid[p] = 0.0;
             f.re_cld[p] = 0.0;
@@ -89,13 +91,13 @@ __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 #pragma unroll
     for (int n = 0; n < NQ; n++) a.ws.rhogq[pq + n * K2] = y.rhogq[n];
     if (a.four_water) {
-        a.ws.vt[pv] = y.vt[0];
-        a.ws.vt[pv + K2] = y.vti;
-        a.ws.vt[pv + 2 * K2] = y.vt[1];
-        a.ws.vt[pv + 3 * K2] = y.vt[2];
+        vt[pv] = R(y.vt[0]);
+        vt[pv + K2] = R(y.vti);
+        vt[pv + 2 * K2] = R(y.vt[1]);
+        vt[pv + 3 * K2] = R(y.vt[2]);
     } else {
 #pragma unroll
-        for (int t = 0; t < 3; t++) a.ws.vt[pv + t * K2] = y.vt[t];
+        for (int t = 0; t < 3; t++) vt[pv + t * K2] = R(y.vt[t]);
     }
     if (!DIAG) return;
     f.re_liquid[p] = rceff;
