@@ -565,13 +565,16 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
     cv = cv + qd2 * C_CVdry;
     const double rho0 = rhog0 / gs;
     const double tem0 = (rhoge0 / rhog0) / cv;
-    const double ein_moist = rhoge0 / rhog0 + q0[QV] * C_LHV0;
-    const double qsum = q0[QV] + q0[QC];
+    // satadjust_all (ice = true: mixed phase, QI adjusted too) or satadjust_liq
+    const bool ice = P.ice_adjust != 0.0;
+    const double ein_moist = ice ? rhoge0 / rhog0 + q0[QV] * C_LHV0 - q0[QI] * C_LHF0 : rhoge0 / rhog0 + q0[QV] * C_LHV0;
+    const double qsum = ice ? q0[QV] + q0[QC] + q0[QI] : q0[QV] + q0[QC];
     double q[6];
 #pragma unroll
     for (int n = 0; n < 6; n++) q[n] = q0[n];
     q[QV] = qsum;
     q[QC] = 0.0;
+    if (ice) q[QI] = 0.0;
     double CVtot = qd * C_CVdry;
 #pragma unroll
     for (int n = 1; n <= 5; n++) CVtot = CVtot + q[n] * C_CVW;
@@ -583,8 +586,12 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
     if (interior) {
 #pragma unroll
         for (int n = 1; n <= 5; n++) qdl = qdl - q[n];
-        newton = satadjust_liq_newton(P, tem, q, qsum, qdl, rho_sa, ein_moist);
-        if (newton != 0) q[QC] = qsum - q[QV];
+        if (ice) {
+            newton = satadjust_all_newton(P, tem, q, qsum, qdl, rho_sa, ein_moist);
+        } else {
+            newton = satadjust_liq_newton(P, tem, q, qsum, qdl, rho_sa, ein_moist);
+            if (newton != 0) q[QC] = qsum - q[QV];
+        }
     }
     const bool solved = interior && newton != 0;
     double rq[6];
@@ -592,6 +599,7 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
     for (int n = 0; n < 6; n++) rq[n] = x.rhogq[n];
     rq[QV] = rhog0 * q[QV];
     rq[QC] = rhog0 * q[QC];
+    if (ice) rq[QI] = rhog0 * q[QI];
     double CVtot2 = qd * C_CVdry;
 #pragma unroll
     for (int n = 1; n <= 5; n++) CVtot2 = CVtot2 + q[n] * C_CVW;
@@ -704,13 +712,18 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
         tem_b += yb.rhoge * rhog0 * CVtot2;
         CVtot2_b += yb.rhoge * rhog0 * tem;
     }
-    // rhogq[QV] = rhog0 q[QV], rhogq[QC] = rhog0 q[QC] (others pass through)
+    // rhogq[QV] = rhog0 q[QV], rhogq[QC] = rhog0 q[QC], under satadjust_all also
+    // rhogq[QI] = rhog0 q[QI] (others pass through)
     rhog0_b += rq_b[QV] * q[QV] + rq_b[QC] * q[QC];
     q_b[QV] += rq_b[QV] * rhog0;
     q_b[QC] += rq_b[QC] * rhog0;
+    if (ice) {
+        rhog0_b += rq_b[QI] * q[QI];
+        q_b[QI] += rq_b[QI] * rhog0;
+    }
     double rhogq_in_b[6];
 #pragma unroll
-    for (int n = 0; n < 6; n++) rhogq_in_b[n] = (n == QV || n == QC) ? 0.0 : rq_b[n];
+    for (int n = 0; n < 6; n++) rhogq_in_b[n] = (n == QV || n == QC || (ice && n == QI)) ? 0.0 : rq_b[n];
     // CVtot2 = qd CVdry + sum_{1..5} q[n] CVW  (q[QC] is the adjusted one)
     qd_b += CVtot2_b * C_CVdry;
     q_b[QC] += CVtot2_b * C_CVW;
@@ -719,7 +732,29 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
     // saturation adjustment
     double qsum_b = 0.0, ein_moist_b = 0.0, rho_sa_b = 0.0, qdl_b = 0.0, CVtot_b = 0.0;
     double qrest_b = 0.0;
-    if (solved) {
+    if (solved && ice) {
+        // q[QV] = qsat(T*), q[QC] = (qsum - qsat) alpha(T*), q[QI] = (qsum - qsat)(1 - alpha);
+        // T* root of R_all(T; qsum, qdl, qrest, rho_sa, ein), qrest = qr + qs + qg
+        const SatAllSlopes sl = satadjust_all_slopes(P, tem, rho_sa);
+        const double d = qsum - sl.qsat;
+        const double d_b = q_b[QC] * sl.alpha + q_b[QI] * (1.0 - sl.alpha);
+        const double alpha_b = (q_b[QC] - q_b[QI]) * d;
+        qsum_b += d_b;
+        const double qsat_b = q_b[QV] - d_b;
+        q_b[QV] = q_b[QC] = q_b[QI] = 0.0;
+        tem_b += alpha_b * sl.dalpha + qsat_b * sl.dqsat;
+        rho_sa_b -= qsat_b * sl.qsat / rho_sa;
+        const double qrest = (q[QR] + q[QS]) + q[QG];
+        const double dRdT = satadjust_all_dRdT(P, tem, qsum, qdl, qrest, rho_sa);
+        const double lam = -tem_b / dRdT;
+        tem_b = 0.0;
+        // R = T (qdl CVdry + (qsum - qsat + qrest) CVW) + qsat LHV0 - (qsum - qsat)(1 - alpha) LHF0 - ein
+        qdl_b += lam * tem * C_CVdry;
+        qsum_b += lam * (tem * C_CVW - (1.0 - sl.alpha) * C_LHF0);
+        qrest_b += lam * tem * C_CVW;
+        rho_sa_b -= lam * (sl.qsat / rho_sa) * (C_LHV0 - tem * C_CVW + (1.0 - sl.alpha) * C_LHF0);
+        ein_moist_b -= lam;
+    } else if (solved) {
         // q[QC] = qsum - q[QV]; q[QV] = qsat(T*, rho_sa); T* root of R(T; qsum, qdl, qrest, rho_sa, ein)
         qsum_b += q_b[QC];
         q_b[QV] -= q_b[QC];
@@ -743,7 +778,7 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
         rho_sa_b += lam * dqsat_drho * (C_LHV0 - tem * C_CVW);
         ein_moist_b -= lam;
     } else {
-        // no iteration: q[QV] = qsum, q[QC] = 0, tem = tem1
+        // no iteration: q[QV] = qsum, q[QC] = 0 (and q[QI] = 0 under satadjust_all), tem = tem1
         qsum_b += q_b[QV];
         q_b[QV] = 0.0;
         q_b[QC] = 0.0;
@@ -758,15 +793,19 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
     qd_b += CVtot_b * C_CVdry;
 #pragma unroll
     for (int n = 2; n <= 5; n++) q_b[n] += CVtot_b * C_CVW;
+    // under satadjust_all, q[QI] was set to 0 before the solve
+    if (ice) q_b[QI] = 0.0;
     // rho_sa = rhog0 / gs
     rhog0_b += rho_sa_b / gs;
-    // qsum = q0[QV] + q0[QC]
+    // qsum = q0[QV] + q0[QC] (+ q0[QI])
     q0_b[QV] += qsum_b;
     q0_b[QC] += qsum_b;
-    // ein_moist = rhoge0/rhog0 + q0[QV] LHV0
+    if (ice) q0_b[QI] += qsum_b;
+    // ein_moist = rhoge0/rhog0 + q0[QV] LHV0 (- q0[QI] LHF0)
     rhoge0_b += ein_moist_b / rhog0;
     rhog0_b -= ein_moist_b * (rhoge0 / rhog0) / rhog0;
     q0_b[QV] += ein_moist_b * C_LHV0;
+    if (ice) q0_b[QI] -= ein_moist_b * C_LHF0;
     // halo pre / rho terms of the first rediagnosis
     q0_b[QV] += q0v_pre_b;
     // tem0 = (rhoge0 / rhog0) / cv (cv independent of q)
@@ -837,6 +876,23 @@ constexpr int tend_blk[6] = {-1, 0, 2, 1, 4, 5};
 
 NSW6_HD double sel(bool c, double a, double b) { return c ? a : b; }
 
+// Reverse of bergeron(): the temperature cotangent from those of a1, a2 and ma2 = 1 -
+// a2 (a1 = v1 1e-3^ma2; v1, v2 linear in the clamped temperature within a table cell;
+// zero where the clamp holds the temperature).
+NSW6_HD double bergeron_b(const double *a1_tab, const double *a2_tab, double tem, double a1, double ma2, double a1_b,
+                          double a2_b, double ma2_b)
+{
+    const double x = tem - C_TEM00;
+    if (x < -30.99 || 0.0 < x || !(x == x)) return 0.0;
+    const int itemc = (int)(-x);
+    const double v1_b = a1_b * pow(1.0E-3, ma2);
+    const double v2_b = a2_b - (ma2_b + a1_b * a1 * log(1.0E-3));
+    // v = (1 - fact) t[i] + fact t[i+1], fact = -(temc + i - 1)
+    const double fact_b =
+        v1_b * (a1_tab[itemc + 1] - a1_tab[itemc]) + v2_b * (a2_tab[itemc + 1] - a2_tab[itemc]);
+    return -fact_b;
+}
+
 // A copy of the rate inputs that the device compiler treats as unknown values, so
 // that expressions of them evaluated before and after are computed twice rather
 // than kept in registers in between.
@@ -895,11 +951,13 @@ NSW6_HD void stash_fence()
 // Also returns the forward tendencies qt[1..5] (the values rates<double> gives).
 // ncp: the point's droplet number Nc, Nc^-1.79 and Dc under OPT_INDIR; nullptr reads
 // them from P where they are used (so they are not held in registers through the
-// pass); their cotangents are added to b.nc_n, b.nc_pm179 and b.nc_dc.  KK_ONLY: sw_kk2000 = 1 known at compile time (otherwise read
-// from P at run time).
-template <bool KK_ONLY>
+// pass); their cotangents are added to b.nc_n, b.nc_pm179 and b.nc_dc.  DEFAULT_SCHEME:
+// sw_kk2000 = 1 and sw_expice = 1 known at compile time (otherwise read from P at run
+// time).  a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
+template <bool DEFAULT_SCHEME>
 NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const NcPoint *ncp, const double qt_b[6],
-                     const double vt_b[3], RateInB &b, double qt[6], const RateStash &st)
+                     const double vt_b[3], RateInB &b, double qt[6], const RateStash &st, const double *a1_tab,
+                     const double *a2_tab)
 {
     double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -929,17 +987,17 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
             lim[j] = (dqdt[sp] < p0);
             R[j] = (lim[j] ? dqdt[sp] : p0) * c;
         };
-        setlim(rPigen, Pigen0, QV, ie);
-        setlim(rPidep, Pidep0, QV, ie);
+        setlim(rPigen, Pigen0, QV, iex);
+        setlim(rPidep, Pidep0, QV, iex);
         setlim(rPsdep, Psdep0, QV, ie);
         setlim(rPgdep, Pgdep0, QV, ie);
         const double Pracw1 = Pracw0 + Psacw0 * we + Pgacw0 * we;
         setlim(rPraut, Praut0, QC, 1.0);
         setlim(rPracw, Pracw1, QC, 1.0);
-        setlim(rPihom, Pihom0, QC, ie);
-        setlim(rPihtr, Pihtr0, QC, ie);
+        setlim(rPihom, Pihom0, QC, iex);
+        setlim(rPihtr, Pihtr0, QC, iex);
         setlim(rPsacw, Psacw0, QC, ie);
-        setlim(rPsfw, 0.0, QC, 0.0);
+        setlim(rPsfw, Psfw0, QC, ieb);
         setlim(rPgacw, Pgacw0, QC, ie);
         setlim(rPrevp, Prevp0, QR, 1.0);
         double Piacr1, Psacr1;
@@ -957,8 +1015,8 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
         }
         setlim(rPgacr, Pgacr0, QR, ie);
         setlim(rPgfrz, Pgfrz0, QR, ie);
-        setlim(rPisub, Pisub0, QI, ie);
-        setlim(rPimlt, Pimlt0, QI, we);
+        setlim(rPisub, Pisub0, QI, iex);
+        setlim(rPimlt, Pimlt0, QI, wex);
         setlim(rPsaut, Psaut0, QI, ie);
         double Praci1;
         {
@@ -969,7 +1027,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
             limq[rPraci_s] = limq[rPraci_g] = QI;
         }
         setlim(rPsaci, Psaci0, QI, ie);
-        setlim(rPsfi, 0.0, QI, 0.0);
+        setlim(rPsfi, Psfi0, QI, ieb);
         setlim(rPgaci, Pgaci0, QI, ie);
         setlim(rPssub, Pssub0, QS, ie);
         setlim(rPsmlt, Psmlt0, QS, we);
@@ -1148,6 +1206,22 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     Rdens_b += Pgfrz0_b * kfrz * frz * L7;
     frz_b += Pgfrz0_b * kfrz * Rdens * L7;
     L7_b += Pgfrz0_b * kfrz * Rdens * frz;
+    // Bergeron growth: Psfw0 = Ni50 W, W = a1 mi50^a2 + kW dens qc, Psfi0 = qi / dt1,
+    // Ni50 = qi dt / (mi50 dt1), dt1 = (mi50^ma2 - mi40^ma2) / (a1 ma2)
+    if (berg) {
+        const double Psfw0_b = raw_b[rPsfw], Psfi0_b = raw_b[rPsfi];
+        const double Ni50_b = Psfw0_b * bW, W_b = Psfw0_b * bNi50;
+        double a1_b = W_b * bE2;
+        const double a2_b = W_b * in.a1 * bE2 * P.ln_mi50;
+        const double kW = PI * P.Eiw * P.Ri50 * P.Ri50 * P.vti50;
+        dens_b += W_b * kW * qc;
+        q_b[QC] += W_b * kW * dens;
+        q_b[QI] += Ni50_b * dt / (P.mi50 * bdt1) + Psfi0_b / bdt1;
+        const double dt1_b = -(Ni50_b * bNi50 + Psfi0_b * Psfi0) / bdt1;
+        a1_b -= dt1_b * bdt1 / in.a1;
+        const double ma2_b = -dt1_b * bdt1 / in.ma2 + dt1_b / (in.a1 * in.ma2) * (P.ln_mi50 * bE50 - P.ln_mi40 * bE40);
+        temp_b += bergeron_b(a1_tab, a2_tab, temp, in.a1, in.ma2, a1_b, a2_b, ma2_b);
+    }
     // Pgmlt00 = kev Rdens N0g Gil ventg + cmlt (Pgacw0 + Pgacr0)
     double ventg_b = 0.0, cmlt_b = 0.0;
     if (gmlt_pos) {
@@ -1599,10 +1673,11 @@ struct PointOutB {
     double rhog, rhoge, rhogq[6], vt[3];
 };
 
-template <bool KK_ONLY>
+// a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
+template <bool DEFAULT_SCHEME>
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
                            const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const RateStash &st,
-                           const NcPoint *ncp)
+                           const NcPoint *ncp, const double *a1_tab, const double *a2_tab)
 {
     // ---- forward preprocessing, reverse of the state update, the rate kernel
     double drq_b[NQ], rq_b[NQ], rhoge0_b, rhog_b;
@@ -1638,7 +1713,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         st[RATE_STASH + 2 * NQ + 1] = rhog_b;
         stash_fence();
         double qt[NQ];
-        rates_b<KK_ONLY>(P, dt, in, ncp, qt_b, yb.vt, ib, qt, st);
+        rates_b<DEFAULT_SCHEME>(P, dt, in, ncp, qt_b, yb.vt, ib, qt, st, a1_tab, a2_tab);
         stash_fence();
 #pragma unroll
         for (int n = 0; n < NQ; n++) {
@@ -1733,10 +1808,11 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
 
 // The same with a stash of local (register or stack) storage.
 NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> &x, const double tun[NCOEF],
-                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF])
+                           const PointOutB &yb, PointInB &xb, double tun_b[NCOEF], const double *a1_tab,
+                           const double *a2_tab)
 {
     double buf[POINT_STASH];
-    point_rates_b<false>(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, nullptr);
+    point_rates_b<false>(P, dt, x, tun, yb, xb, tun_b, RateStash{buf, 1}, nullptr, a1_tab, a2_tab);
 }
 
 }  // namespace nsw6

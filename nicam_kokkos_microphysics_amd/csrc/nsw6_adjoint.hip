@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "nsw6_adjoint_sweep.h"
+#include "nsw6_tables.h"
 
 using namespace nsw6;
 
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(64) k_adj_wall_general(AdjLaunch L)
 // * the rate reverse's stash (POINT_STASH doubles per thread, column threadIdx.x), which
 //   takes the values that wait through the rate reverse and its phase-2 recompute out
 //   of the registers.
-template <bool INDIR, bool KK_ONLY>
+template <bool INDIR, bool DEFAULT_SCHEME>
 __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
 {
     __shared__ double red[4][NCOEF];
@@ -78,13 +79,15 @@ __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
     }
     AdjArgs A = L.A;
     A.a.P = &sP;
+    A.a1_tab = c_a1_tab;
+    A.a2_tab = c_a2_tab;
     const int ijdim = A.a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     double tun_b[NCOEF];
 #pragma unroll
     for (int c = 0; c < NCOEF; c++) tun_b[c] = 0.0;
-    if (lb.valid && ij < ijdim) adj_point<INDIR, KK_ONLY>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
+    if (lb.valid && ij < ijdim) adj_point<INDIR, DEFAULT_SCHEME>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
     // per-block sums of the coefficient cotangents (every thread reaches this point)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -178,9 +181,9 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
     hipLaunchKernelGGL(k_adj_wall_general, dim3((L.A.a.ijdim + 63) / 64, L.A.a.lall), dim3(64), 0, stream, L);
     // the supported schemes: KK2000 or Berry, each with or without OPT_INDIR
-    if (!L.A.a.kk_only && L.A.a.opt_indir)
+    if (!L.A.a.default_scheme && L.A.a.opt_indir)
         hipLaunchKernelGGL((k_adj_point<true, false>), dim3(nb), dim3(256), 0, stream, L);
-    else if (!L.A.a.kk_only)
+    else if (!L.A.a.default_scheme)
         hipLaunchKernelGGL((k_adj_point<false, false>), dim3(nb), dim3(256), 0, stream, L);
     else if (L.A.a.opt_indir)
         hipLaunchKernelGGL((k_adj_point<true, true>), dim3(nb), dim3(256), 0, stream, L);

@@ -483,9 +483,10 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
 }
 
 // adj_point of a point; st holds the rate reverse's stash.
-// INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  KK_ONLY:
-// KK2000 alone (A.a.kk_only); otherwise sw_kk2000 is read at run time.
-template <bool INDIR, bool KK_ONLY>
+// INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  DEFAULT_SCHEME:
+// KK2000 alone with explicit ice generation (A.a.default_scheme); otherwise sw_kk2000 and
+// sw_expice are read at run time.
+template <bool INDIR, bool DEFAULT_SCHEME>
 NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
     const StepArgs &a = A.a;
@@ -561,10 +562,10 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
             nc = nc_indirect(P, un);
             if (!(un * 1.0E-6 < P.Nc_def)) dnc_dunccn = 1.0E-6;
         }
-        point_rates_b<KK_ONLY>(P, a.dt, x, tun, yb, xb, tun_b, st, INDIR ? &nc : nullptr);
+        point_rates_b<DEFAULT_SCHEME>(P, a.dt, x, tun, yb, xb, tun_b, st, INDIR ? &nc : nullptr, A.a1_tab, A.a2_tab);
         if (INDIR) {
             double nc_b = xb.nc_pm179 * (-1.79 * nc.pow_m179 / nc.Nc);
-            if (!KK_ONLY) nc_b += xb.nc_n + xb.nc_dc * (-5.964E-2 / nc.Nc);
+            if (!DEFAULT_SCHEME) nc_b += xb.nc_n + xb.nc_dc * (-5.964E-2 / nc.Nc);
             g_unccn = nc_b * dnc_dunccn;
         }
         g_rhog = xb.rhog;
@@ -584,10 +585,10 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
                      g_unccn);
 }
 
-template <bool INDIR, bool KK_ONLY>
+template <bool INDIR, bool DEFAULT_SCHEME>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
-    adj_point_single<INDIR, KK_ONLY>(A, l, k, ij, tun_b, st);
+    adj_point_single<INDIR, DEFAULT_SCHEME>(A, l, k, ij, tun_b, st);
 }
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
 {

@@ -113,6 +113,10 @@ NSW6_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 NSW6_HD double dmin(double a, double b) { return (b < a) ? b : a; }
 NSW6_HD float dmax(float a, float b) { return (a < b) ? b : a; }
 NSW6_HD float dmin(float a, float b) { return (b < a) ? b : a; }
+// the value of a scalar (the identity for the floating-point types; a dual-number
+// scalar overloads it)
+NSW6_HD double val(double x) { return x; }
+NSW6_HD float val(float x) { return x; }
 
 // Cloud droplet number of a point and the two functions of it the rates use:
 // Nc_def everywhere (the constants formed on the host), or under the aerosol
@@ -158,18 +162,22 @@ NSW6_HD double psat_ice(const DevParams &P, double tem)
     return C_PSAT0 * 1.0 * exp(P.LovR_ice * (P.RTEM00 - rtem));
 }
 
-// Bergeron_param at one point (keeps the reference's off-by-one interpolation).
-NSW6_HD void bergeron(const double *a1_tab, const double *a2_tab, double tem,
-                                         double &a1, double &a2, double &ma2)
+// Bergeron_param at one point (keeps the reference's off-by-one interpolation).  The
+// scalar type S is double in the kernels; a dual-number scalar carries the
+// temperature tangent through the interpolation (the table cell is piecewise
+// constant) and the power (its pow(double, S) overload).
+template <typename S>
+NSW6_HD void bergeron(const double *a1_tab, const double *a2_tab, S tem, S &a1, S &a2, S &ma2)
 {
-    double temc = dmin(dmax(tem - C_TEM00, -30.99), 0.0);
+    S temc = dmin(dmax(tem - C_TEM00, S(-30.99)), S(0.0));
     // the table index from a NaN-free copy: the same index for every finite temperature,
     // and a NaN temperature (which the clamps above pass through) cannot index outside
     // the tables; its NaN still reaches a1, a2 through `fact`
-    int itemc = (int)(-((temc == temc) ? temc : 0.0));
-    double fact = -(temc + (double)(itemc - 1));
-    double v1 = (1.0 - fact) * a1_tab[itemc] + (fact)*a1_tab[itemc + 1];
-    double v2 = (1.0 - fact) * a2_tab[itemc] + (fact)*a2_tab[itemc + 1];
+    const double tc = val(temc);
+    int itemc = (int)(-((tc == tc) ? tc : 0.0));
+    S fact = -(temc + (double)(itemc - 1));
+    S v1 = (1.0 - fact) * a1_tab[itemc] + (fact)*a1_tab[itemc + 1];
+    S v2 = (1.0 - fact) * a2_tab[itemc] + (fact)*a2_tab[itemc + 1];
     ma2 = 1.0 - v2;
     a2 = v2;
     a1 = v1 * pow(1.0E-3, ma2);

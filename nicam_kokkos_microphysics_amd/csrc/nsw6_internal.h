@@ -129,7 +129,8 @@ struct StepArgs {
     int opt_indir;                // 1: aerosol indirect effect (k_rates<R, DIAG, true>)
     int kin2kin;                  // 1: KIN2KIN energy correction of the sedimentation
     int four_water;               // 1: "4WATER", QI sediments (transport kernels with NT = 4)
-    int kk_only;                  // 1: sw_kk2000 = 1, KK2000 alone (the reverse point pass's kernel)
+    int default_scheme;           // 1: KK2000 alone (sw_kk2000 = 1) with explicit ice generation
+                                  // (sw_expice = 1): the reverse point pass's kernel for them
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 
@@ -151,6 +152,7 @@ struct AdjArgs {
     const double *cot_sflux;    // [l][3][ij] or NULL
     Fields grad;                // input cotangents written (NULL pointer = not wanted)
     double *cr, *wr, *dws, *din;   // records [slot][l][k][ij]
+    const double *a1_tab, *a2_tab;  // Bergeron tables a1(T), a2(T) (32 entries each)
 };
 
 }  // namespace nsw6

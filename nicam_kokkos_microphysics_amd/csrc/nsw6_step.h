@@ -7,16 +7,13 @@
 // reproduces it to round-off.
 //
 // A scalar type S provides the arithmetic operators, sqrt/exp/log/expm1/fabs/
-// copysign, dmax/dmin (select by value) and val(S) -> double.
+// copysign, pow(double, S), dmax/dmin (select by value) and val(S) -> double.
 #pragma once
 
 #include "nsw6_device.h"
 #include "nsw6_internal.h"
 
 namespace nsw6 {
-
-NSW6_HD double val(double x) { return x; }
-NSW6_HD float val(float x) { return x; }
 
 // ---------------------------------------------------------------------------------
 // Point-wise rate update (k_rates): first negative_filter (src/mod_mp_nsw6.cpp:
@@ -42,7 +39,7 @@ template <typename S> NSW6_HD S psat_of(double LovR, double RTEM00, S tem)
 }
 
 // a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation,
-// sw_expice != 1; the Bergeron parameters then enter as values, not differentiated).
+// sw_expice != 1).
 template <typename S, typename R>
 NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, const S tun[NCOEF],
                          const double *a1_tab, const double *a2_tab, PointOut<S> &y, const NcPoint &nc)
@@ -91,8 +88,8 @@ NSW6_HD void point_rates(const DevParams &P, double dt, const PointIn<S> &x, con
     in.qsati = R(qsati);
     in.pre = R(pre);
     in.rhog = R(rhog);
-    double a1 = 1.0, a2 = 0.0, ma2 = 1.0;
-    if (P.sw_expice != 1.0) bergeron(a1_tab, a2_tab, val(tem), a1, a2, ma2);
+    S a1 = S(1.0), a2 = S(0.0), ma2 = S(1.0);
+    if (P.sw_expice != 1.0) bergeron(a1_tab, a2_tab, tem, a1, a2, ma2);
     in.a1 = R(a1);
     in.a2 = R(a2);
     in.ma2 = R(ma2);
@@ -820,6 +817,63 @@ NSW6_HD double satadjust_dRdT(const DevParams &P, double tem, double qsum, doubl
     return CVt - tem * dqsat * C_CVW + dqsat * C_LHV0;
 }
 
+// The mixed-phase saturation of the satadjust_all solve at temperature T: liquid
+// fraction alpha(T) = clamp((T - LL) / (UL - LL), 0, 1) over SATURATION_Setrange's
+// limits, and qsat = (alpha psat_liq + (1 - alpha) psat_ice) / (rho Rvap T).
+template <typename S> NSW6_HD S satadjust_all_alpha(const DevParams &P, S tem)
+{
+    return dmin(dmax((tem - P.sat_llimit) / (P.sat_ulimit - P.sat_llimit), S(0.0)), S(1.0));
+}
+template <typename S> NSW6_HD S satadjust_all_qsat(const DevParams &P, S tem, S rho_sa)
+{
+    const S alpha = satadjust_all_alpha(P, tem);
+    const S psat = psat_of(P.LovR_liq, P.RTEM00, tem) * alpha + psat_of(P.LovR_ice, P.RTEM00, tem) * (1.0 - alpha);
+    return psat / (rho_sa * C_Rvap * tem);
+}
+// Residual E_all(T) - Emoist of the satadjust_all solve: with q(QV) = qsat(T),
+// q(QC) = (qsum - qsat) alpha, q(QI) = (qsum - qsat)(1 - alpha) and the reference's
+// heat capacity sum, E_all(T) = T (qdl CVdry + (qsum - qsat + qrest) CVW) + qsat LHV0
+// - q(QI) LHF0 (qrest = qr + qs + qg).
+template <typename S>
+NSW6_HD S satadjust_all_residual(const DevParams &P, S tem, S qsum, S qdl, S qrest, S rho_sa, S ein_moist)
+{
+    const S alpha = satadjust_all_alpha(P, tem);
+    const S qsat = satadjust_all_qsat(P, tem, rho_sa);
+    const S d = qsum - qsat;
+    const S CVt = qdl * C_CVdry + (d + qrest) * C_CVW;
+    return tem * CVt + qsat * C_LHV0 - d * (1.0 - alpha) * C_LHF0 - ein_moist;
+}
+// alpha, qsat and their temperature derivatives at fixed rho (the clamp's derivative
+// is taken on the side its selections take)
+struct SatAllSlopes {
+    double alpha, dalpha, qsat, dqsat;
+};
+NSW6_HD SatAllSlopes satadjust_all_slopes(const DevParams &P, double tem, double rho_sa)
+{
+    SatAllSlopes s;
+    const double a = (tem - P.sat_llimit) / (P.sat_ulimit - P.sat_llimit);
+    s.alpha = dmin(dmax(a, 0.0), 1.0);
+    s.dalpha = (a < 0.0 || 1.0 < a) ? 0.0 : 1.0 / (P.sat_ulimit - P.sat_llimit);
+    const double psl = psat_of(P.LovR_liq, P.RTEM00, tem), psi = psat_of(P.LovR_ice, P.RTEM00, tem);
+    const double dpsl = (tem > TEM_MIN) ? psl * P.LovR_liq / (tem * tem) : 0.0;
+    const double dpsi = (tem > TEM_MIN) ? psi * P.LovR_ice / (tem * tem) : 0.0;
+    const double psat = psl * s.alpha + psi * (1.0 - s.alpha);
+    const double dpsat = dpsl * s.alpha + dpsi * (1.0 - s.alpha) + (psl - psi) * s.dalpha;
+    const double den = rho_sa * C_Rvap * tem;
+    s.qsat = psat / den;
+    s.dqsat = dpsat / den - s.qsat / tem;
+    return s;
+}
+// dE_all/dT of that residual at fixed inputs
+NSW6_HD double satadjust_all_dRdT(const DevParams &P, double tem, double qsum, double qdl, double qrest,
+                                  double rho_sa)
+{
+    const SatAllSlopes s = satadjust_all_slopes(P, tem, rho_sa);
+    const double d = qsum - s.qsat;
+    const double CVt = qdl * C_CVdry + (d + qrest) * C_CVW;
+    return CVt - tem * s.dqsat * C_CVW + s.dqsat * C_LHV0 + (s.dqsat * (1.0 - s.alpha) + d * s.dalpha) * C_LHF0;
+}
+
 // Status of one point's tail (for the device report).
 struct TailStatus {
     int newton;   // satadjust_liq_newton's return value
@@ -834,6 +888,18 @@ NSW6_HD double newton_tem(const DevParams &, double tem_final, double, double, d
     return tem_final;
 }
 NSW6_HD double newton_qsat(const DevParams &, double qv_final, double, double) { return qv_final; }
+// The same for the satadjust_all solve: its temperature, and q(QV), q(QC), q(QI).
+NSW6_HD double newton_tem_all(const DevParams &, double tem_final, double, double, double, double, double)
+{
+    return tem_final;
+}
+NSW6_HD void newton_q_all(const DevParams &, const double q_final[6], double, double, double, double &qv,
+                          double &qc, double &qi)
+{
+    qv = q_final[QV];
+    qc = q_final[QC];
+    qi = q_final[QI];
+}
 
 template <typename S>
 NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, TailOut<S> &y)
@@ -884,13 +950,11 @@ NSW6_HD TailStatus tail(const DevParams &P, bool interior, const TailIn<S> &x, T
 #pragma unroll
         for (int n = 0; n < 6; n++) qv6[n] = val(q[n]);
         if (ice) {
-            // (the dual-number check covers the default scheme: values only here)
             st.newton = satadjust_all_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));
             if (st.newton != 0) {
-                tem = S(tv);
-                q[QV] = S(qv6[QV]);
-                q[QC] = S(qv6[QC]);
-                q[QI] = S(qv6[QI]);
+                const S qrest = (q[QR] + q[QS]) + q[QG];
+                tem = newton_tem_all(P, tv, qsum, qdl, qrest, rho_sa, ein_moist);
+                newton_q_all(P, qv6, tem, rho_sa, qsum, q[QV], q[QC], q[QI]);
             }
         } else {
             st.newton = satadjust_liq_newton(P, tv, qv6, val(qsum), val(qdl), val(rho_sa), val(ein_moist));

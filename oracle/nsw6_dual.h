@@ -2,8 +2,9 @@
 // directions (value + N partial derivatives, all FP64), the scalar type of the
 // host gradient check of the NSW6 adjoint.  It provides what the templated step
 // (nicam_kokkos_microphysics_amd/csrc/nsw6_step.h) asks of a scalar: arithmetic,
-// sqrt/exp/log/expm1/fabs/copysign, dmax/dmin selecting by value, val(), and the
-// implicit-function derivative of the saturation-adjustment solution.
+// sqrt/exp/log/expm1/fabs/copysign, pow of a constant base, dmax/dmin selecting by
+// value, val(), and the implicit-function derivatives of the saturation-adjustment
+// solutions (liquid and mixed-phase).
 #pragma once
 
 #include <math.h>
@@ -141,6 +142,12 @@ template <int N> NSW6_HD Dual<N> log(const Dual<N> &a)
     for (int i = 0; i < N; i++) r.d[i] = (a.d[i] == 0.0) ? 0.0 : a.d[i] / a.v;
     return r;
 }
+// a^x for a constant base a > 0
+template <int N> NSW6_HD Dual<N> pow(double a, const Dual<N> &x)
+{
+    const double p = ::pow(a, x.v);
+    return chain(x, p, p * ::log(a));
+}
 template <int N> NSW6_HD Dual<N> fabs(const Dual<N> &a) { return chain(a, ::fabs(a.v), (a.v < 0.0) ? -1.0 : 1.0); }
 // copysign(a, b) = |a| sign(b): the tangent follows a only
 template <int N> NSW6_HD Dual<N> copysign(const Dual<N> &a, const Dual<N> &b)
@@ -170,6 +177,35 @@ NSW6_HD Dual<N> newton_qsat(const nsw6::DevParams &P, double qv_final, Dual<N> t
     Dual<N> qs = nsw6::psat_of(P.LovR_liq, P.RTEM00, tem) / (rho_sa * nsw6::C_Rvap * tem);
     qs.v = qv_final;
     return qs;
+}
+
+// The converged satadjust_all temperature: the root of E_all(T; x) - Emoist(x) = 0,
+// tangent T' = -(dR/dx x') / (dR/dT).
+template <int N>
+NSW6_HD Dual<N> newton_tem_all(const nsw6::DevParams &P, double tem_final, Dual<N> qsum, Dual<N> qdl,
+                               Dual<N> qrest, Dual<N> rho_sa, Dual<N> ein_moist)
+{
+    const Dual<N> R = nsw6::satadjust_all_residual(P, Dual<N>(tem_final), qsum, qdl, qrest, rho_sa, ein_moist);
+    const double dRdT = nsw6::satadjust_all_dRdT(P, tem_final, qsum.v, qdl.v, qrest.v, rho_sa.v);
+    Dual<N> T(tem_final);
+    for (int i = 0; i < N; i++) T.d[i] = -R.d[i] / dRdT;
+    return T;
+}
+// q(QV) = qsat, q(QC) = (qsum - qsat) alpha, q(QI) = (qsum - qsat)(1 - alpha) at the
+// solution: the Newton's values with the tangents of these functions of (T, rho, qsum)
+template <int N>
+NSW6_HD void newton_q_all(const nsw6::DevParams &P, const double q_final[6], Dual<N> tem, Dual<N> rho_sa,
+                          Dual<N> qsum, Dual<N> &qv, Dual<N> &qc, Dual<N> &qi)
+{
+    const Dual<N> alpha = nsw6::satadjust_all_alpha(P, tem);
+    const Dual<N> qs = nsw6::satadjust_all_qsat(P, tem, rho_sa);
+    const Dual<N> d = qsum - qs;
+    qv = qs;
+    qc = d * alpha;
+    qi = d * (1.0 - alpha);
+    qv.v = q_final[nsw6::QV];
+    qc.v = q_final[nsw6::QC];
+    qi.v = q_final[nsw6::QI];
 }
 
 }  // namespace nsw6dual

@@ -694,7 +694,7 @@ extern "C" int grad_check_pieces(const double *gz, const double *gzh, const nsw6
                 PointInB xb;
                 memset(&xb, 0, sizeof xb);
                 double tb[NCOEF] = {0, 0, 0, 0, 0, 0, 0, 0};
-                point_rates_b(P, dt, x, tun, yb, xb, tb);
+                point_rates_b(P, dt, x, tun, yb, xb, tb, A1_TAB, A2_TAB);
                 double vx = xb.rhog * v[0] + xb.tem * v[7] + xb.qv * v[8];
                 for (int n = 0; n < 6; n++) vx += xb.rhogq[n] * v[1 + n];
                 for (int c = 0; c < NCOEF; c++) vx += tb[c] * v[9 + c];
@@ -1075,16 +1075,16 @@ extern "C" {
 
 // Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
 // NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but Roh & Satoh snow,
-// sw_expice = 1, 3WATER (KIN2EIN or KIN2KIN) with KK2000 or Berry's autoconversion, no OPT_INDIR
-// (the device reverse mode also covers OPT_INDIR).
+// 3WATER (KIN2EIN or KIN2KIN) with KK2000 or Berry's autoconversion, with or without explicit ice
+// generation, no OPT_INDIR (the device reverse mode also covers OPT_INDIR).
 int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                       const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
                       const nsw6_fields *grad, double *grad_coef, double dt)
 {
     DevParams P;
     to_dev_params(*params, P);
-    if (!(P.sw_roh2014 == 1.0 && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) && P.sw_expice == 1.0 &&
-          P.opt_indir == 0.0 && P.four_water == 0.0))
+    if (!(P.sw_roh2014 == 1.0 && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) && P.opt_indir == 0.0 &&
+          P.four_water == 0.0))
         return NSW6_UNSUPPORTED;
     GridDev g;
     make_grid_dev(gz, gzh, g);
@@ -1127,6 +1127,8 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
     a.dt3 = pow(dt, 3);
     a.dt3_6 = a.dt3 / 6.0;
     a.kin2kin = (P.kin2kin != 0.0);
+    A.a1_tab = A1_TAB;
+    A.a2_tab = A2_TAB;
     A.cot = host_fields(*cot);
     A.cot_sflux = cot_sflux;
     A.grad = host_fields(*grad);

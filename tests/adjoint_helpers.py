@@ -10,6 +10,16 @@ DT = 15.0
 STATE_KEYS = ["rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "tem", "q", "vx", "vy", "vz"]
 
 
+def all_species(st, seed):
+    """All five hydrometeors at every level (lognormal, median 3e-5), consistent rhogq."""
+    rng = np.random.default_rng(seed)
+    s = F.copy_state(st)
+    for n in (1, 2, 3, 4, 5):
+        s["q"][:, n] += np.exp(rng.normal(np.log(3e-5), 1.5, s["q"][:, n].shape)).clip(0, 2e-3)
+    s["rhogq"][:] = s["q"] * s["rhog"][:, None]
+    return s
+
+
 def coefs():
     P = O.params()
     return np.array([P.kk_aut_rate, P.qicrt_saut, P.Esi, P.Esr, P.Cr, P.Cs, P.Cg, P.icenuc_coef])

@@ -8,6 +8,9 @@
   perturb every input field and the 8 coefficients at once.
 * The general regions (departures beyond the next cell): the host step's general
   wall fluxes against the oracle, and the whole reverse mode against dual numbers.
+* Without explicit ice generation (sw_expice = 0: Bergeron growth and the mixed-phase
+  saturation adjustment): every reverse piece and the whole reverse mode against dual
+  numbers, and those dual numbers against central differences.
 * Every hand-written reverse piece (point rates, wall fluxes, cell divergence and
   KIN2EIN, point-wise tail with the implicit-function Newton adjoint) agrees with
   forward-mode dual numbers at every point, wall and cell of several states, including
@@ -20,20 +23,13 @@ from nicam_kokkos_microphysics_amd import fields as F
 from nicam_kokkos_microphysics_amd.check import field_errors
 from oracle import oracle as O
 
-from adjoint_helpers import directional, random_cotangents, random_directions
+from adjoint_helpers import all_species, directional, random_cotangents, random_directions
 
 DT = 15.0
 STATE_KEYS = ["rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "tem", "q", "vx", "vy", "vz"]
 
 
-def _everywhere(st, seed):
-    """All five hydrometeors at every level (lognormal, median 3e-5), consistent rhogq."""
-    rng = np.random.default_rng(seed)
-    s = F.copy_state(st)
-    for n in (1, 2, 3, 4, 5):
-        s["q"][:, n] += np.exp(rng.normal(np.log(3e-5), 1.5, s["q"][:, n].shape)).clip(0, 2e-3)
-    s["rhogq"][:] = s["q"] * s["rhog"][:, None]
-    return s
+_everywhere = all_species
 
 
 def _coefs():
@@ -121,6 +117,80 @@ def test_berry_reverse_matches_dual_numbers(case):
     for i in range(len(dirs)):
         s_, a = directional(grad, gc, dirs[i], dcoef[i])
         assert abs(s_ - dj[i]) <= 1e-13 * a, (i, s_, dj[i], a)
+
+
+@pytest.mark.parametrize("case", ["all_species", "supersaturated", "evolved", "berry"])
+def test_no_explicit_ice_reverse_matches_dual_numbers(case):
+    """sw_expice = 0: the Bergeron growth Psfw, Psfi between -30 and 0 degC (with a1,
+    a2 of the table interpolation differentiated in temperature) in place of explicit
+    ice generation, and the mixed-phase saturation adjustment (satadjust_all, its
+    converged solution differentiated through the implicit-function theorem).  Every
+    reverse piece against dual numbers at every point, and the whole step's reverse
+    against the dual-number directional derivative (the dual numbers of this scheme are
+    themselves checked against central differences below)."""
+    P = O.params()
+    P.sw_expice = 0.0
+    if case == "all_species":
+        gz, gzh, st = F.synth_state(2, 40, seed=21)
+        st = _everywhere(st, 1)
+    elif case == "supersaturated":
+        gz, gzh, st = F.synth_state(2, 40, seed=5, frac_supersat=0.3)
+    else:
+        if case == "berry":
+            P.sw_kk2000 = 0.0
+        gz, gzh, st = F.synth_state(2, 40, seed=22, frac_towers=0.5)
+        st = _everywhere(st, 2)
+        for _ in range(3):
+            assert O.mp_driver(st, gz, gzh, DT, P=P).status == 0
+    # points where the Bergeron growth acts: -30 < T - 273.15 < 0 with cloud water and ice
+    # (the supersaturated state exercises the mixed-phase adjustment instead)
+    berg = (st["tem"] > 243.15) & (st["tem"] < 273.15) & (st["q"][:, 1] > 0) & (st["q"][:, 3] > 0)
+    assert case == "supersaturated" or berg[:, 1:-1].sum() >= 100, berg.sum()
+    err, cnt = O.check_pieces(st, gz, gzh, DT, P=P)
+    assert np.all(cnt > 0)
+    assert np.all(err <= 1e-6), dict(zip(["point", "wall", "sediment", "tail"], err))
+    cot, cs = random_cotangents(st, gz, gzh, 7)
+    grad, gc, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=P)
+    assert status == 0
+    dirs, dcoef = random_directions(st, 13, 4)
+    _, dj = O.dual_vjp(st, gz, gzh, DT, cot, cs, dirs, dcoef, P=P)
+    for i in range(len(dirs)):
+        s_, a = directional(grad, gc, dirs[i], dcoef[i])
+        assert abs(s_ - dj[i]) <= 1e-13 * a, (i, s_, dj[i], a)
+
+
+@pytest.mark.parametrize("case", ["supersaturated", "all_species"])
+def test_no_explicit_ice_dual_matches_central_differences(case):
+    """The dual numbers of the sw_expice = 0 step (Bergeron a1, a2 tangents, the
+    satadjust_all implicit-function tangent) against central differences of the host
+    step (h = 1e-6 relative) along random directions through every input and the 8
+    coefficients."""
+    P = O.params()
+    P.sw_expice = 0.0
+    if case == "supersaturated":
+        gz, gzh, st = F.synth_state(1, 32, seed=5, frac_supersat=0.3)
+    else:
+        gz, gzh, st = F.synth_state(1, 32, seed=21)
+        st = _everywhere(st, 1)
+    a, b = F.copy_state(st), F.copy_state(st)
+    assert O.mp_driver(a, gz, gzh, DT, P=P).status == 0
+    assert O.step_fp64(b, gz, gzh, DT, P=P) == 0
+    assert np.abs(a["tem"] - b["tem"]).max() <= 1e-9
+    rng = np.random.default_rng(3)
+    c0 = _coefs()
+    dirs, dcoef = [], []
+    for _ in range(3):
+        dirs.append({k: st[k] * rng.uniform(-1, 1, st[k].shape) for k in STATE_KEYS})
+        dcoef.append(c0 * rng.uniform(-1, 1, 8))
+    dcoef = np.array(dcoef)
+    _, dl = O.dual_loss(st, gz, gzh, DT, dirs, dcoef, P=P)
+    h = 1e-6
+    for i in range(3):
+        sp = {k: (st[k] + h * dirs[i][k] if k in dirs[i] else st[k]) for k in st}
+        sm = {k: (st[k] - h * dirs[i][k] if k in dirs[i] else st[k]) for k in st}
+        fd = (O.loss_fp64(sp, gz, gzh, DT, coef=c0 + h * dcoef[i], P=P) -
+              O.loss_fp64(sm, gz, gzh, DT, coef=c0 - h * dcoef[i], P=P)) / (2 * h)
+        assert abs(dl[i] - fd) <= 1e-7 * abs(fd), (i, dl[i], fd)
 
 
 def _state(case):

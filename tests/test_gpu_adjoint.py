@@ -3,7 +3,8 @@
 * Elementwise agreement with the host run of the same reverse passes
   (oracle.adjoint_host): every input cotangent within 1e-9 of its field's max-abs,
   the coefficient cotangents within 1e-9 relative; also with Berry's autoconversion
-  (sw_kk2000 = 0).
+  (sw_kk2000 = 0), and without explicit ice generation (sw_expice = 0: Bergeron growth
+  and the mixed-phase saturation adjustment).
 * Directional agreement with forward-mode dual numbers on the host (oracle.dual_vjp):
   |<grad, v> - dJ/dv| <= 1e-6 |dJ/dv| along random directions through every input
   field and the 8 coefficients.
@@ -24,8 +25,8 @@ import pytest
 from nicam_kokkos_microphysics_amd import _native as N
 from nicam_kokkos_microphysics_amd import fields as F
 
-from adjoint_helpers import (DT, STATE_KEYS, coefs, compare_with_host, directional, output_cotangents,
-                             random_cotangents, random_directions)
+from adjoint_helpers import (DT, STATE_KEYS, all_species, coefs, compare_with_host, directional,
+                             output_cotangents, random_cotangents, random_directions)
 
 pytestmark = pytest.mark.gpu
 
@@ -101,6 +102,48 @@ def test_gpu_berry_reverse_matches_host_passes():
         assert err <= 1e-9 * scale + 1e-300, (k, err, scale)
     np.testing.assert_allclose(gc.cpu().numpy(), gc_host, rtol=1e-9, atol=0)
     drv.close()
+
+
+@pytest.mark.parametrize("case", ["all_species", "supersaturated", "evolved_berry"])
+def test_gpu_no_explicit_ice_reverse(case):
+    """sw_expice = 0 (Bergeron growth, mixed-phase saturation adjustment): the GPU
+    reverse mode against the host run of the same passes and against dual numbers
+    (compare_with_host: elementwise 1e-9 of max-abs, directional 1e-9, points without a
+    derivative handled apart); with KK2000 and with Berry's autoconversion."""
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    Po, Pg = O.params(), default_params()
+    Po.sw_expice = Pg.sw_expice = 0.0
+    if case == "all_species":
+        gz, gzh, st = F.synth_state(2, 300, seed=21)
+        st = all_species(st, 1)
+    elif case == "supersaturated":
+        gz, gzh, st = F.synth_state(2, 300, seed=5, frac_supersat=0.3)
+    else:
+        Po.sw_kk2000 = Pg.sw_kk2000 = 0.0
+        gz, gzh, st = F.synth_state(2, 300, seed=22, frac_towers=0.5)
+        st = all_species(st, 2)
+        for _ in range(3):
+            assert O.mp_driver(st, gz, gzh, DT, P=Po).status == 0
+    cot, cs = output_cotangents(st, gz, gzh, DT, Po, 7)
+    g_host, gc_host, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=Po)
+    assert status == 0
+    lall, ijdim = st["rhog"].shape[0], st["rhog"].shape[-1]
+    drv = NSW6(gz, gzh, params=Pg, max_columns=lall * ijdim)
+    try:
+        dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+        print(f"adjoint sw_expice=0 {case}", flush=True)
+        grad, gc = drv.adjoint(F.to_device(st), dcot, DT,
+                               cot_sflux=torch.from_numpy(np.ascontiguousarray(cs)).cuda())
+        torch.cuda.synchronize()
+        rep = drv.report()
+    finally:
+        drv.close()
+    assert rep.status == 0
+    g_gpu = {k: v.cpu().numpy() for k, v in grad.items()}
+    n = compare_with_host(st, gz, gzh, DT, Po, cot, cs, g_gpu, gc.cpu().numpy(), g_host, gc_host)
+    print(f"{n} kink entries", flush=True)
 
 
 @pytest.mark.parametrize("case", ["default", "supersaturated"])
