@@ -229,11 +229,11 @@ int nsw6_mp_driver(nsw6_ctx *ctx, int lall, int ijdim, int l_region,
 int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
                            const nsw6_fields *fields, double dt, double ct, void *stream);
 
-/* Reverse mode of one FP64 step (Roh & Satoh snow moments; KK2000, or Berry's
- * autoconversion with sw_kk2000 = 0; explicit ice nucleation, sw_expice = 1, or without
- * it, sw_expice = 0: the Bergeron growth and the mixed-phase saturation adjustment;
- * KIN2EIN or KIN2KIN, 3WATER; each with or without OPT_INDIR, which then reads
- * `state->unccn`): given the step's INPUT state `state`
+/* Reverse mode of one FP64 step (Roh & Satoh snow moments, or the exponential ones with
+ * sw_roh2014 = 0; KK2000, or Berry's autoconversion with sw_kk2000 = 0; explicit ice
+ * nucleation, sw_expice = 1, or without it, sw_expice = 0: the Bergeron growth and the
+ * mixed-phase saturation adjustment; KIN2EIN or KIN2KIN, 3WATER; each with or without
+ * OPT_INDIR, which then reads `state->unccn`): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)
  * and cotangents of the step's outputs, write the cotangents of its inputs,
  *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.
@@ -253,8 +253,8 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  * solution (implicit-function theorem).  Regions whose departures leave the next cell
  * (the general wall fluxes) are differentiated like the others; nsw6_last_status()
  * counts them in n_general_regions.
- * Any other scheme (non-Roh snow moments, 4WATER, a fractional sw_kk2000) returns
- * NSW6_UNSUPPORTED at once (nothing is launched).
+ * Any other scheme (4WATER, a switch other than 0 or 1) returns NSW6_UNSUPPORTED at
+ * once (nothing is launched).
  * With mp_div_num = n > 1 (nsw6_set_div_num) the call is differentiated as the
  * forward call runs it: n sub-steps of dt/n, precip, precip_rhoe/lh_heat/rhophi/
  * rhokin, GPREC and the surface mass fluxes averaged over them (their cotangents reach

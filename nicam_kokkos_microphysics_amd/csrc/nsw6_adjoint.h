@@ -8,9 +8,11 @@
 // of the piecewise-smooth step.  The saturation adjustment is differentiated through
 // the implicit-function theorem at the Newton solution.
 //
-// Coverage: the default scheme (Roh & Satoh 2014 snow, KK2000 autoconversion,
-// explicit ice generation sw_expice = 1) and the single-cell transport; the host
-// rejects other configurations before launching the adjoint.
+// Coverage: Roh & Satoh 2014 or exponential snow moments, KK2000 or Berry's
+// autoconversion, with or without explicit ice generation (Bergeron growth and the
+// mixed-phase saturation adjustment), OPT_INDIR, KIN2EIN or KIN2KIN, "3WATER"
+// transport with single-cell and general regions; the host rejects other
+// configurations before launching the adjoint.
 #pragma once
 
 #include "nsw6_step.h"
@@ -835,8 +837,8 @@ NSW6_HD void tail_b(const DevParams &P, bool interior, const TailIn<double> &x, 
 namespace nsw6 {
 
 // ---------------------------------------------------------------------------------
-// Process rates (reverse of rates<double> for the default scheme: Roh & Satoh 2014
-// snow moments, KK2000 autoconversion, explicit ice generation)
+// Process rates (reverse of rates<double>: Roh & Satoh 2014 or exponential snow
+// moments, KK2000 or Berry's autoconversion, with or without explicit ice generation)
 // ---------------------------------------------------------------------------------
 struct RateInB {
     double dens, temp, q[6], qsatl, qsati, pre, temc, sice_m1, sliq_1m, qv_qsati, tun[NCOEF];
@@ -875,6 +877,28 @@ constexpr double blk_s[NBLK][BLKMAX] = {
 constexpr int tend_blk[6] = {-1, 0, 2, 1, 4, 5};
 
 NSW6_HD double sel(bool c, double a, double b) { return c ? a : b; }
+
+// Slope of the exponential snow size distribution and the powers of it that the
+// moments take (rates<R> without Roh & Satoh): x = dens qs / (As N0s GAM_1bs) + zs,
+// s = x^(1/4) (1 - zs), sds = s^(1/4).
+struct SnowSlope {
+    double x, s, sds, s2, s3, s1bs, s2bs, s3bs, s3ds, s5dsh;
+};
+NSW6_HD SnowSlope snow_slope(const DevParams &P, double dens, double qs, double zs)
+{
+    SnowSlope L;
+    L.x = dens * qs * P.rK_s + zs;
+    L.s = sqrt(sqrt(L.x)) * (1.0 - zs);
+    L.sds = sqrt(sqrt(L.s));
+    L.s2 = L.s * L.s;
+    L.s3 = L.s2 * L.s;
+    L.s1bs = L.s2 * L.s2;
+    L.s2bs = L.s1bs * L.s;
+    L.s3bs = L.s3 * L.s3;
+    L.s3ds = L.s3 * L.sds;
+    L.s5dsh = sqrt(L.s2bs * L.sds);
+    return L;
+}
 
 // Reverse of bergeron(): the temperature cotangent from those of a1, a2 and ma2 = 1 -
 // a2 (a1 = v1 1e-3^ma2; v1, v2 linear in the clamped temperature within a table cell;
@@ -952,8 +976,8 @@ NSW6_HD void stash_fence()
 // ncp: the point's droplet number Nc, Nc^-1.79 and Dc under OPT_INDIR; nullptr reads
 // them from P where they are used (so they are not held in registers through the
 // pass); their cotangents are added to b.nc_n, b.nc_pm179 and b.nc_dc.  DEFAULT_SCHEME:
-// sw_kk2000 = 1 and sw_expice = 1 known at compile time (otherwise read from P at run
-// time).  a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
+// sw_kk2000 = 1, sw_expice = 1 and sw_roh2014 = 1 known at compile time (otherwise read
+// from P at run time).  a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
 template <bool DEFAULT_SCHEME>
 NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const NcPoint *ncp, const double qt_b[6],
                      const double vt_b[3], RateInB &b, double qt[6], const RateStash &st, const double *a1_tab,
@@ -1196,6 +1220,7 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     double L3dr_b = 0.0, L5dr_b = 0.0, L6dr_b = 0.0;
     double G1_b = 0.0, Gdg_b = 0.0, G2_b = 0.0, G3_b = 0.0, G3dg_b = 0.0, G5dg_b = 0.0;
     double M0_b = 0.0, M1_b = 0.0, M2_b = 0.0, M0bs_b = 0.0, M1bs_b = 0.0, M2bs_b = 0.0, M2ds_b = 0.0, M5dsh_b = 0.0;
+    double Lsds_b = 0.0;
     double Vtr_b = vt_b[0], Vts_b = vt_b[1], Vtg_b = vt_b[2];
     double Esi_mod_b = 0.0, Egs_mod_b = 0.0;
     double Giv_b = 0.0, Glv_b = 0.0, Gil_b = 0.0, Nu_b = 0.0, Ka_b = 0.0, Kd_b = 0.0;
@@ -1557,8 +1582,12 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
         tun_b[5] -= Vts_b * rho_fact * RMV;
         rf_b -= Vts_b * Cs * RMV;
         const double RMV_b = -Vts_b * Cs * rho_fact;
-        M2ds_b += RMV_b / Vden;
-        M0bs_b -= RMV_b * RMV / Vden;
+        if (roh) {
+            M2ds_b += RMV_b / Vden;
+            M0bs_b -= RMV_b * RMV / Vden;
+        } else {
+            Lsds_b += RMV_b * P.GAM_1bsds / P.GAM_1bs;
+        }
         // Vtr = -Cr rho_fact kVr Ldr
         tun_b[4] -= Vtr_b * rho_fact * kVr * Ldr;
         rf_b -= Vtr_b * Cr * kVr * Ldr;
@@ -1579,6 +1608,29 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
         const double Xg_b = G1_b * 0.25 * G1 / Xg;
         dens_b += Xg_b * qg / Kg;
         q_b[QG] += Xg_b * dens / Kg;
+    }
+    // snow moments without Roh & Satoh: M = N0s GAM_. Ls^n (n = 1..6), M2ds = c Ls^3 Lsds,
+    // M5dsh = c sqrt(Ls^5 Lsds), Lsds = Ls^(1/4), Ls = Xs^(1/4) (1 - zs)
+    if (!roh) {
+        const double dn = opaque(dens), qn = opaque(qs), N0s = P.N0s;
+        const SnowSlope L = snow_slope(P, dn, qn, sw.zs);
+        double Ls_b = M0_b * N0s * GAM + M1_b * N0s * GAM_2 * 2.0 * L.s + M2_b * N0s * GAM_3 * 3.0 * L.s2 +
+                      M0bs_b * N0s * P.GAM_1bs * 4.0 * L.s3 + M1bs_b * N0s * P.GAM_2bs * 5.0 * L.s1bs +
+                      M2bs_b * N0s * P.GAM_3bs * 6.0 * L.s2bs;
+        Ls_b += M2ds_b * N0s * P.GAM_3ds * 3.0 * L.s2 * L.sds;
+        Lsds_b += M2ds_b * N0s * P.GAM_3ds * L.s3;
+        if (L.s5dsh > 0.0) {
+            const double u_b = M5dsh_b * N0s * P.GAM_5ds_h * 0.5 / L.s5dsh;
+            Ls_b += u_b * 5.0 * L.s1bs * L.sds;
+            Lsds_b += u_b * L.s2bs;
+        }
+        if (L.s > 0.0) Ls_b += Lsds_b * 0.25 * L.sds / L.s;
+        if (sw.zs == 0.0) {
+            const double Xs_b = Ls_b * 0.25 * L.s / L.x;
+            dens_b += Xs_b * qn * P.rK_s;
+            q_b[QS] += Xs_b * dn * P.rK_s;
+        }
+        M0_b = M1_b = M2_b = M0bs_b = M1bs_b = M2bs_b = M2ds_b = M5dsh_b = 0.0;
     }
     // snow moments: M = m omz (m = exp(ln10 loga + lx b)), M2 = Xs2
     double Xs2_b = M2_b;

@@ -484,8 +484,8 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
 
 // adj_point of a point; st holds the rate reverse's stash.
 // INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  DEFAULT_SCHEME:
-// KK2000 alone with explicit ice generation (A.a.default_scheme); otherwise sw_kk2000 and
-// sw_expice are read at run time.
+// KK2000 alone, explicit ice generation and Roh & Satoh snow (A.a.default_scheme);
+// otherwise sw_kk2000, sw_expice and sw_roh2014 are read at run time.
 template <bool INDIR, bool DEFAULT_SCHEME>
 NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {

@@ -221,7 +221,7 @@ static void fill_step_args(const nsw6_ctx *ctx, int lall, int ijdim, double dt, 
     a.opt_indir = ctx->P.opt_indir != 0.0;
     a.kin2kin = ctx->P.kin2kin != 0.0;
     a.four_water = ctx->P.four_water != 0.0;
-    a.default_scheme = ctx->P.sw_kk2000 == 1.0 && ctx->P.sw_expice == 1.0;
+    a.default_scheme = ctx->P.sw_kk2000 == 1.0 && ctx->P.sw_expice == 1.0 && ctx->P.sw_roh2014 == 1.0;
     a.dt = dt;
     a.dt2 = pow(dt, 2);
     a.dt3 = pow(dt, 3);
@@ -421,14 +421,16 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
     return NSW6_OK;
 }
 
-// The reverse mode covers the Roh & Satoh snow moments with KK2000 or Berry's
-// autoconversion, with or without explicit ice generation (without it: the Bergeron
-// growth and the mixed-phase saturation adjustment), each with or without OPT_INDIR,
-// under "3WATER" (see include/nsw6.h).
+// The reverse mode covers the Roh & Satoh or the exponential snow moments, KK2000 or
+// Berry's autoconversion, with or without explicit ice generation (without it: the
+// Bergeron growth and the mixed-phase saturation adjustment), each with or without
+// OPT_INDIR, under "3WATER" (see include/nsw6.h).
 static bool adjoint_supported(const DevParams &P)
 {
     const bool kk = (P.sw_kk2000 == 1.0), berry = (P.sw_kk2000 == 0.0);
-    return P.sw_roh2014 == 1.0 && (kk || berry) && P.four_water == 0.0;
+    const bool roh = (P.sw_roh2014 == 1.0), expo = (P.sw_roh2014 == 0.0);
+    const bool expice = (P.sw_expice == 1.0 || P.sw_expice == 0.0);
+    return (roh || expo) && (kk || berry) && expice && P.four_water == 0.0;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)

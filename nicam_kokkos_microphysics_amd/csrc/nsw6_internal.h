@@ -129,8 +129,9 @@ struct StepArgs {
     int opt_indir;                // 1: aerosol indirect effect (k_rates<R, DIAG, true>)
     int kin2kin;                  // 1: KIN2KIN energy correction of the sedimentation
     int four_water;               // 1: "4WATER", QI sediments (transport kernels with NT = 4)
-    int default_scheme;           // 1: KK2000 alone (sw_kk2000 = 1) with explicit ice generation
-                                  // (sw_expice = 1): the reverse point pass's kernel for them
+    int default_scheme;           // 1: KK2000 alone (sw_kk2000 = 1), explicit ice generation
+                                  // (sw_expice = 1) and Roh & Satoh snow (sw_roh2014 = 1): the
+                                  // reverse point pass's kernel for them
     double dt, dt2, dt3, dt3_6;   // dt, dt^2, dt^3, dt^3/6
 };
 

@@ -1074,17 +1074,17 @@ void seed_state(HostState<Dual<N>> &h, const nsw6_fields *f, int ndir, const nsw
 extern "C" {
 
 // Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
-// NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but Roh & Satoh snow,
-// 3WATER (KIN2EIN or KIN2KIN) with KK2000 or Berry's autoconversion, with or without explicit ice
-// generation, no OPT_INDIR (the device reverse mode also covers OPT_INDIR).
+// NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but 3WATER (KIN2EIN or
+// KIN2KIN) with the Roh & Satoh or the exponential snow moments, KK2000 or Berry's autoconversion, with or
+// without explicit ice generation, no OPT_INDIR (the device reverse mode also covers OPT_INDIR).
 int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                       const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
                       const nsw6_fields *grad, double *grad_coef, double dt)
 {
     DevParams P;
     to_dev_params(*params, P);
-    if (!(P.sw_roh2014 == 1.0 && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) && P.opt_indir == 0.0 &&
-          P.four_water == 0.0))
+    if (!((P.sw_roh2014 == 1.0 || P.sw_roh2014 == 0.0) && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) &&
+          (P.sw_expice == 1.0 || P.sw_expice == 0.0) && P.opt_indir == 0.0 && P.four_water == 0.0))
         return NSW6_UNSUPPORTED;
     GridDev g;
     make_grid_dev(gz, gzh, g);

@@ -20,6 +20,26 @@ def all_species(st, seed):
     return s
 
 
+def non_roh(P):
+    """The parameters as the reference sets them without Roh & Satoh
+    (src/mod_mp_nsw6.cpp:276-356 with Roh_flag false): sw_roh2014 = 0, N0g = 4e6,
+    As = pi rho_s / 6, Bs = 3 (and the Gamma functions of it), Esi = 1, Egi = 0.1,
+    Egs = 1."""
+    import math
+    P.sw_roh2014 = 0.0
+    P.N0g = 4.0e6
+    P.As = math.pi * P.rho_s / 6.0
+    P.Bs = 3.0
+    P.GAM_1bs = math.gamma(1.0 + P.Bs)
+    P.GAM_2bs = math.gamma(2.0 + P.Bs)
+    P.GAM_3bs = math.gamma(3.0 + P.Bs)
+    P.GAM_1bsds = math.gamma(1.0 + P.Bs + P.Ds)
+    P.Esi = 1.0
+    P.Egi = 0.1
+    P.Egs = 1.0
+    return P
+
+
 def coefs():
     P = O.params()
     return np.array([P.kk_aut_rate, P.qicrt_saut, P.Esi, P.Esr, P.Cr, P.Cs, P.Cg, P.icenuc_coef])

@@ -11,6 +11,8 @@
 * Without explicit ice generation (sw_expice = 0: Bergeron growth and the mixed-phase
   saturation adjustment): every reverse piece and the whole reverse mode against dual
   numbers, and those dual numbers against central differences.
+* Without the Roh & Satoh snow moments (sw_roh2014 = 0), alone and with every other
+  switch off: the same, the dual numbers checked against differences at single points.
 * Every hand-written reverse piece (point rates, wall fluxes, cell divergence and
   KIN2EIN, point-wise tail with the implicit-function Newton adjoint) agrees with
   forward-mode dual numbers at every point, wall and cell of several states, including
@@ -23,7 +25,7 @@ from nicam_kokkos_microphysics_amd import fields as F
 from nicam_kokkos_microphysics_amd.check import field_errors
 from oracle import oracle as O
 
-from adjoint_helpers import all_species, directional, random_cotangents, random_directions
+from adjoint_helpers import all_species, directional, non_roh, random_cotangents, random_directions
 
 DT = 15.0
 STATE_KEYS = ["rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge", "rhogq", "tem", "q", "vx", "vy", "vz"]
@@ -191,6 +193,76 @@ def test_no_explicit_ice_dual_matches_central_differences(case):
         fd = (O.loss_fp64(sp, gz, gzh, DT, coef=c0 + h * dcoef[i], P=P) -
               O.loss_fp64(sm, gz, gzh, DT, coef=c0 - h * dcoef[i], P=P)) / (2 * h)
         assert abs(dl[i] - fd) <= 1e-7 * abs(fd), (i, dl[i], fd)
+
+
+@pytest.mark.parametrize("case", ["all_species", "evolved", "every_switch_off"])
+def test_non_roh_reverse_matches_dual_numbers(case):
+    """sw_roh2014 = 0 with the reference's parameters for it (non_roh): the snow moments
+    of the exponential size distribution from the slope (dens qs / (As N0s
+    GAM_1bs))^(1/4) in place of Roh & Satoh's fit; alone, and with explicit ice
+    generation and KK2000 off too (sw_expice = sw_kk2000 = 0).  Every reverse piece
+    against dual numbers at every point, and the whole step's reverse against the
+    dual-number directional derivative."""
+    P = non_roh(O.params())
+    if case == "all_species":
+        gz, gzh, st = F.synth_state(2, 40, seed=21)
+        st = _everywhere(st, 1)
+    else:
+        if case == "every_switch_off":
+            P.sw_expice = 0.0
+            P.sw_kk2000 = 0.0
+        gz, gzh, st = F.synth_state(2, 40, seed=22, frac_towers=0.5)
+        st = _everywhere(st, 2)
+        for _ in range(3):
+            assert O.mp_driver(st, gz, gzh, DT, P=P).status == 0
+    assert (st["q"][:, 4, 1:-1] > 1e-8).sum() >= 1000
+    err, cnt = O.check_pieces(st, gz, gzh, DT, P=P)
+    assert np.all(cnt > 0)
+    assert np.all(err <= 1e-6), dict(zip(["point", "wall", "sediment", "tail"], err))
+    cot, cs = random_cotangents(st, gz, gzh, 7)
+    grad, gc, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=P)
+    assert status == 0
+    dirs, dcoef = random_directions(st, 13, 4)
+    _, dj = O.dual_vjp(st, gz, gzh, DT, cot, cs, dirs, dcoef, P=P)
+    for i in range(len(dirs)):
+        s_, a = directional(grad, gc, dirs[i], dcoef[i])
+        assert abs(s_ - dj[i]) <= 1e-13 * a, (i, s_, dj[i], a)
+
+
+@pytest.mark.parametrize("case", ["supersaturated", "all_species"])
+def test_non_roh_dual_matches_differences_at_points(case):
+    """The dual numbers of the sw_roh2014 = 0 step against differences of the host step
+    at single points with snow (rhog and tem, 40 points each, h = 1e-6 relative).  The
+    limiters put some points on a kink (the one-sided differences differ by more than
+    the curvature): there the dual number must equal one of the one-sided differences,
+    elsewhere the central one; within 1e-6 of max(|derivative|, 1)."""
+    P = non_roh(O.params())
+    if case == "supersaturated":
+        gz, gzh, st = F.synth_state(1, 32, seed=5, frac_supersat=0.3)
+    else:
+        gz, gzh, st = F.synth_state(1, 32, seed=21)
+        st = _everywhere(st, 1)
+    c0 = _coefs()
+    rng = np.random.default_rng(1)
+    cand = np.argwhere(st["q"][0, 4, 1:-1] > 1e-8) + [1, 0]
+    dirs, keys = [], []
+    for k, j in cand[rng.choice(len(cand), 40, replace=False)]:
+        for key in ("rhog", "tem"):
+            d = np.zeros_like(st[key])
+            d[0, k, j] = st[key][0, k, j]
+            dirs.append({key: d})
+            keys.append(key)
+    _, dl = O.dual_loss(st, gz, gzh, DT, dirs, np.zeros((len(dirs), 8)), P=P)
+    L0 = O.loss_fp64(st, gz, gzh, DT, coef=c0, P=P)
+    h = 1e-6
+    for i, key in enumerate(keys):
+        sp, sm = dict(st), dict(st)
+        sp[key] = st[key] + h * dirs[i][key]
+        sm[key] = st[key] - h * dirs[i][key]
+        r = (O.loss_fp64(sp, gz, gzh, DT, coef=c0, P=P) - L0) / h
+        l = (L0 - O.loss_fp64(sm, gz, gzh, DT, coef=c0, P=P)) / h
+        err = min(abs(dl[i] - r), abs(dl[i] - l), abs(dl[i] - 0.5 * (r + l)))
+        assert err <= 1e-6 * max(abs(dl[i]), 1.0), (i, key, dl[i], r, l)
 
 
 def _state(case):

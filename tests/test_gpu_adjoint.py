@@ -3,8 +3,9 @@
 * Elementwise agreement with the host run of the same reverse passes
   (oracle.adjoint_host): every input cotangent within 1e-9 of its field's max-abs,
   the coefficient cotangents within 1e-9 relative; also with Berry's autoconversion
-  (sw_kk2000 = 0), and without explicit ice generation (sw_expice = 0: Bergeron growth
-  and the mixed-phase saturation adjustment).
+  (sw_kk2000 = 0), without explicit ice generation (sw_expice = 0: Bergeron growth and
+  the mixed-phase saturation adjustment), without Roh & Satoh snow (sw_roh2014 = 0) and
+  with all three switched.
 * Directional agreement with forward-mode dual numbers on the host (oracle.dual_vjp):
   |<grad, v> - dJ/dv| <= 1e-6 |dJ/dv| along random directions through every input
   field and the 8 coefficients.
@@ -25,7 +26,7 @@ import pytest
 from nicam_kokkos_microphysics_amd import _native as N
 from nicam_kokkos_microphysics_amd import fields as F
 
-from adjoint_helpers import (DT, STATE_KEYS, all_species, coefs, compare_with_host, directional,
+from adjoint_helpers import (DT, STATE_KEYS, all_species, coefs, compare_with_host, directional, non_roh,
                              output_cotangents, random_cotangents, random_directions)
 
 pytestmark = pytest.mark.gpu
@@ -104,28 +105,49 @@ def test_gpu_berry_reverse_matches_host_passes():
     drv.close()
 
 
-@pytest.mark.parametrize("case", ["all_species", "supersaturated", "evolved_berry"])
-def test_gpu_no_explicit_ice_reverse(case):
-    """sw_expice = 0 (Bergeron growth, mixed-phase saturation adjustment): the GPU
-    reverse mode against the host run of the same passes and against dual numbers
-    (compare_with_host: elementwise 1e-9 of max-abs, directional 1e-9, points without a
-    derivative handled apart); with KK2000 and with Berry's autoconversion."""
-    import torch
+def _scheme_case(case):
+    """States and parameters of the non-default schemes: without explicit ice generation
+    (sw_expice = 0), without Roh & Satoh snow (sw_roh2014 = 0, with the reference's
+    parameters for it), with Berry's autoconversion (sw_kk2000 = 0) and with every
+    switch off."""
     from oracle import oracle as O
-    from nicam_kokkos_microphysics_amd.driver import NSW6, default_params
+    from nicam_kokkos_microphysics_amd.driver import default_params
     Po, Pg = O.params(), default_params()
-    Po.sw_expice = Pg.sw_expice = 0.0
-    if case == "all_species":
+    scheme, state = case.split(":")
+    edits = {"no_expice": {"sw_expice": 0.0}, "no_expice_berry": {"sw_expice": 0.0, "sw_kk2000": 0.0},
+             "non_roh": {"sw_roh2014": 0.0},
+             "all_off": {"sw_roh2014": 0.0, "sw_expice": 0.0, "sw_kk2000": 0.0}}[scheme]
+    for k, v in edits.items():
+        setattr(Po, k, v)
+        setattr(Pg, k, v)
+    if Po.sw_roh2014 == 0.0:
+        non_roh(Po)
+        non_roh(Pg)
+    if state == "all_species":
         gz, gzh, st = F.synth_state(2, 300, seed=21)
         st = all_species(st, 1)
-    elif case == "supersaturated":
+    elif state == "supersaturated":
         gz, gzh, st = F.synth_state(2, 300, seed=5, frac_supersat=0.3)
     else:
-        Po.sw_kk2000 = Pg.sw_kk2000 = 0.0
         gz, gzh, st = F.synth_state(2, 300, seed=22, frac_towers=0.5)
         st = all_species(st, 2)
         for _ in range(3):
             assert O.mp_driver(st, gz, gzh, DT, P=Po).status == 0
+    return gz, gzh, st, Po, Pg
+
+
+@pytest.mark.parametrize("case", ["no_expice:all_species", "no_expice:supersaturated", "no_expice_berry:evolved",
+                                  "non_roh:all_species", "all_off:evolved"])
+def test_gpu_scheme_variants_reverse(case):
+    """sw_expice = 0 (Bergeron growth, mixed-phase saturation adjustment), sw_roh2014 = 0
+    (exponential snow moments), with KK2000 or Berry's autoconversion: the GPU reverse
+    mode against the host run of the same passes and against dual numbers
+    (compare_with_host: elementwise 1e-9 of max-abs, directional 1e-9, points without a
+    derivative handled apart)."""
+    import torch
+    from oracle import oracle as O
+    from nicam_kokkos_microphysics_amd.driver import NSW6
+    gz, gzh, st, Po, Pg = _scheme_case(case)
     cot, cs = output_cotangents(st, gz, gzh, DT, Po, 7)
     g_host, gc_host, status = O.adjoint_host(st, gz, gzh, DT, cot, cs, P=Po)
     assert status == 0
@@ -133,7 +155,7 @@ def test_gpu_no_explicit_ice_reverse(case):
     drv = NSW6(gz, gzh, params=Pg, max_columns=lall * ijdim)
     try:
         dcot = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
-        print(f"adjoint sw_expice=0 {case}", flush=True)
+        print(f"adjoint {case}", flush=True)
         grad, gc = drv.adjoint(F.to_device(st), dcot, DT,
                                cot_sflux=torch.from_numpy(np.ascontiguousarray(cs)).cuda())
         torch.cuda.synchronize()
