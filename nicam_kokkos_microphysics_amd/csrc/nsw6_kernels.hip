@@ -23,6 +23,8 @@
 #include <float.h>
 #include <stdint.h>
 
+// double exp and log from the LDS tables of nsw6_math.h (k_rates, k_finish)
+#define NSW6_TABLE_MATH 1
 #include "nsw6_device.h"
 #include "nsw6_columns.h"
 #include "nsw6_internal.h"
@@ -334,6 +336,7 @@ __device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, in
 template <int NT, typename WS = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_finish(StepArgs a)
 {
+    load_math_tables();   // the saturation adjustment's exp
     const int ijdim = a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
     if (!lb.valid) return;

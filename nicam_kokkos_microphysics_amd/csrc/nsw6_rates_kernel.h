@@ -23,6 +23,7 @@ namespace nsw6 {
 template <typename R, bool DIAG, bool INDIR = false>
 __global__ void __launch_bounds__(256) k_rates(StepArgs a)
 {
+    load_math_tables();
     const int ijdim = a.ijdim;
     const int l = blockIdx.z, k = blockIdx.y;
     const int ij = blockIdx.x * blockDim.x + threadIdx.x;

@@ -4,6 +4,8 @@
 // approximations; the mixed form's limiters amplify those differences over a rollout.
 #include <hip/hip_runtime.h>
 
+// the FP64 parts' exp and log from the LDS tables of nsw6_math.h
+#define NSW6_TABLE_MATH 1
 #include "nsw6_internal.h"
 #include "nsw6_rates_kernel.h"
 

@@ -1289,4 +1289,22 @@ int grad_dual_vjp(const double *gz, const double *gzh, const nsw6_params *params
     return fails ? 1 : 0;
 }
 
+// The device's table-driven exp (which = 0) and log (which = 1) of nsw6_math.h, run on
+// the host with host copies of its tables, for the accuracy test.
+int grad_math_tang(int which, const double *x, double *y, int64_t n)
+{
+    static const double exp2_tab[64] = {NSW6_EXP2_TAB};
+    static const double log_tab[130] = {NSW6_LOG_TAB};
+    for (int64_t i = 0; i < n; i++) y[i] = which == 0 ? exp_tang(x[i], exp2_tab) : log_tang(x[i], log_tab);
+    return (which == 0 || which == 1) ? 0 : 1;
+}
+// The tables themselves (64 and 130 doubles), for the test that recomputes them.
+void grad_math_tables(double *exp2_tab, double *log_tab)
+{
+    static const double e[64] = {NSW6_EXP2_TAB};
+    static const double l[130] = {NSW6_LOG_TAB};
+    memcpy(exp2_tab, e, sizeof(e));
+    memcpy(log_tab, l, sizeof(l));
+}
+
 }  // extern "C"

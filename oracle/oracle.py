@@ -266,6 +266,10 @@ def grad_lib():
         L.grad_adjoint_host.restype = I
         L.grad_dual_vjp.argtypes = [P, P, ctypes.POINTER(N.Params), I, I, FP, D, FP, P, I, FP, P, P, P]
         L.grad_dual_vjp.restype = I
+        L.grad_math_tang.argtypes = [I, P, P, ctypes.c_int64]
+        L.grad_math_tang.restype = I
+        L.grad_math_tables.argtypes = [P, P]
+        L.grad_math_tables.restype = None
         _glib = L
     return _glib
 
