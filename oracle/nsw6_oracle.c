@@ -13,8 +13,12 @@
  *
  * Build: gcc -O2 -fno-fast-math -ffp-contract=off -fopenmp (oracle/Makefile).
  *
- * Parity status of each function (how it is pinned) is listed in the tests that
- * exercise it (tests/test_oracle_*.py).
+ * Every function is pinned by checks independent of this text (tests/pin_checks.py,
+ * tests/test_oracle_*.py): closed forms, quadratures of the physical integrals,
+ * budgets, the reference's tables, vgrid files and parameters; each pin catches a
+ * mistake planted under ORA_PLANT.  Parity unpinned: the last bits of exp, log and pow
+ * (the reference's Kokkos math is absent; the parity bar absorbs them) and the reading
+ * of the racy cloud-top diagnostics in the cloudtop_intended mode.
  */
 #define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c11 */
 #include "nsw6_oracle.h"
@@ -446,7 +450,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     rho_fact = sqrt(dens00 * Rdens);
     temc = temp - TEM00;
 
-    wk[I_delta1] = (0.5 + copysign(0.5, qr - 1.0E-4));
+    wk[I_delta1] = (0.5 + copysign(0.5, qr - (ORA_PLANT == 36 ? 1.0E-5 : 1.0E-4)));
     wk[I_delta2] = (0.5 + copysign(0.5, 1.0E-4 - qr)) * (0.5 + copysign(0.5, 1.0E-4 - qs));
     wk[I_spsati] = 0.5 + copysign(0.5, Sice - 1.0);
     wk[I_iceflg] = 0.5 - copysign(0.5, temc);
@@ -541,7 +545,7 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
 #undef ROH_MOM
 
     zerosw = 0.5 - copysign(0.5, qg - 1.0E-12);
-    RLMDg = sqrt(sqrt(dens * qg / (Ag * N0g * GAM_1bg) + zerosw)) * (1.0 - zerosw);
+    RLMDg = sqrt(sqrt(dens * qg / (Ag * (ORA_PLANT == 37 ? N0r : N0g) * GAM_1bg) + zerosw)) * (1.0 - zerosw);
     RLMDg_dg = sqrt(RLMDg);
     RLMDg_2 = pow(RLMDg, 2);
     RLMDg_3 = pow(RLMDg, 3);
@@ -557,35 +561,36 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
           (sw_constVti) * (-CONST_Vti);
     Vtr = -Cr * rho_fact * GAM_1brdr / GAM_1br * RLMDr_dr;
     Vts = -Cs * rho_fact * RMOMs_Vt;
-    Vtg = -Cg * rho_fact * GAM_1bgdg / GAM_1bg * RLMDg_dg;
+    Vtg = -Cg * rho_fact * GAM_1bgdg / GAM_1bg * (ORA_PLANT == 41 ? RLMDg : RLMDg_dg);
 
     Ni0 = dmax(exp(-0.1 * temc), 1.0) * 1000.0;
-    Qi0 = P->icenuc_coef * exp(log(Ni0) * 1.33) * Rdens;
+    Qi0 = P->icenuc_coef * exp(log(Ni0) * (ORA_PLANT == 30 ? 1.3 : 1.33)) * Rdens;
     wk[I_Pigen] = dmax(dmin(Qi0 - qi, qv - qsati), 0.0) / dt;
 
     Esi_mod = dmin(Esi, Esi * exp(gamma_sacr * temc));
     Egs_mod = dmin(Egs, Egs * exp(gamma_gacs * temc));
 
-    Pracw_orig = qc * 0.25 * PI * Erw * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
+    Pracw_orig = qc * 0.25 * PI * Erw * N0r * Cr * (ORA_PLANT == 38 ? GAM_3 : GAM_3dr) * RLMDr_3dr * rho_fact;
     zerosw = 0.5 - copysign(0.5, qc * qr - 1.0E-12);
-    Pracw_kk = 67.0 * exp(log(qc * qr + zerosw) * 1.15) * (1.0 - zerosw);
+    Pracw_kk = 67.0 * exp(log(qc * qr + zerosw) * (ORA_PLANT == 40 ? 1.5 : 1.15)) * (1.0 - zerosw);
     wk[I_Pracw] = (1.0 - sw_kk2000) * Pracw_orig + (sw_kk2000)*Pracw_kk;
 
     wk[I_Psacw] = qc * 0.25 * PI * Esw * Cs * MOMs_2ds * rho_fact;
-    wk[I_Pgacw] = qc * 0.25 * PI * Egw * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
+    wk[I_Pgacw] = qc * 0.25 * PI * Egw * N0g * Cg * (ORA_PLANT == 20 ? GAM_3 : GAM_3dg) * RLMDg_3dg * rho_fact;
     wk[I_Praci] = qi * 0.25 * PI * Eri * N0r * Cr * GAM_3dr * RLMDr_3dr * rho_fact;
-    wk[I_Psaci] = qi * 0.25 * PI * Esi_mod * Cs * MOMs_2ds * rho_fact;
-    wk[I_Pgaci] = qi * 0.25 * PI * Egi * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
-    wk[I_Piacr] = qi * Ar / mi * 0.25 * PI * Eri * N0r * Cr * GAM_6dr * RLMDr_6dr * rho_fact;
+    wk[I_Psaci] = qi * 0.25 * PI * (ORA_PLANT == 22 ? Esi : Esi_mod) * Cs * MOMs_2ds * rho_fact;
+    wk[I_Pgaci] = qi * 0.25 * PI * (ORA_PLANT == 21 ? Egw : Egi) * N0g * Cg * GAM_3dg * RLMDg_3dg * rho_fact;
+    wk[I_Piacr] = qi * Ar / mi * 0.25 * PI * Eri * N0r * Cr * (ORA_PLANT == 23 ? GAM_3dr : GAM_6dr) * RLMDr_6dr * rho_fact;
     wk[I_Psacr] = Ar * 0.25 * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
-                  (GAM_1br * RLMDr_1br * MOMs_2 + 2.0 * GAM_2br * RLMDr_2br * MOMs_1 +
+                  (GAM_1br * RLMDr_1br * MOMs_2 + (ORA_PLANT == 24 ? 1.0 : 2.0) * GAM_2br * RLMDr_2br * MOMs_1 +
                    GAM_3br * RLMDr_3br * MOMs_0);
     wk[I_Pgacr] = Ar * 0.25 * PI * Rdens * Egr * N0g * N0r * fabs(Vtg - Vtr) *
                   (GAM_1br * RLMDr_1br * GAM_3 * RLMDg_3 + (ORA_PLANT == 16 ? 1.0 : 2.0) * GAM_2br * RLMDr_2br * GAM_2 * RLMDg_2 +
                    GAM_3br * RLMDr_3br * GAM * RLMDg);
     wk[I_Pracs] = As * 0.25 * PI * Rdens * Esr * N0r * fabs(Vtr - Vts) *
-                  (MOMs_0bs * GAM_3 * RLMDr_3 + 2.0 * MOMs_1bs * GAM_2 * RLMDr_2 + MOMs_2bs * GAM * RLMDr);
-    wk[I_Pgacs] = As * 0.25 * PI * Rdens * Egs_mod * N0g * fabs(Vtg - Vts) *
+                  (MOMs_0bs * GAM_3 * RLMDr_3 + 2.0 * MOMs_1bs * GAM_2 * RLMDr_2 +
+                   (ORA_PLANT == 25 ? MOMs_1bs : MOMs_2bs) * GAM * RLMDr);
+    wk[I_Pgacs] = As * 0.25 * PI * Rdens * Egs_mod * N0g * fabs(Vtg - (ORA_PLANT == 26 ? Vtr : Vts)) *
                   (MOMs_0bs * GAM_3 * RLMDg_3 + 2.0 * MOMs_1bs * GAM_2 * RLMDg_2 + MOMs_2bs * GAM * RLMDg);
 
     rhoqc = dens * qc * 1000.0;
@@ -599,32 +604,32 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     betai = dmin(beta_saut, beta_saut * exp(gamma_saut * temc));
     wk[I_Psaut] = dmax(betai * (qi - qicrt_saut), 0.0);
     betas = dmin(beta_gaut, beta_gaut * exp(gamma_gaut * temc));
-    wk[I_Pgaut] = dmax(betas * (qs - qscrt_gaut), 0.0);
+    wk[I_Pgaut] = dmax(betas * (qs - (ORA_PLANT == 34 ? qicrt_saut : qscrt_gaut)), 0.0);
 
     Ka = (Ka0 + dKa_dT * temc);
     Kd = (Kd0 + dKd_dT * temc) * PRE00 / pre;
     Nu = (nu0 + dnu_dT * temc) * Rdens;
     Glv = 1.0 / (LHV0 / (Ka * temp) * (LHV0 / (Rvap * temp) - 1.0) + 1.0 / (Kd * dens * qsatl));
-    Giv = 1.0 / (LHS0 / (Ka * temp) * (LHS0 / (Rvap * temp) - 1.0) + 1.0 / (Kd * dens * qsati));
+    Giv = 1.0 / ((ORA_PLANT == 27 ? LHV0 : LHS0) / (Ka * temp) * (LHS0 / (Rvap * temp) - 1.0) + 1.0 / (Kd * dens * qsati));
     Gil = 1.0 / ((ORA_PLANT == 18 ? LHV0 : LHF0) / (Ka * temc));   /* IEEE: temc == 0 -> 1/inf = 0 */
 
     ventr = f1r * GAM_2 * (ORA_PLANT == 17 ? RLMDr_3 : RLMDr_2) + f2r * sqrt(Cr * rho_fact / Nu * RLMDr_5dr) * GAM_5dr_h;
     wk[I_Prevp] = 2.0 * PI * Rdens * N0r * (1.0 - dmin(Sliq, 1.0)) * Glv * ventr;
 
     rhoqi = dmax(dens * qi, EPS);
-    XNi = dmin(dmax(5.38E+7 * exp(log(rhoqi) * 0.75), 1.0E+3), 1.0E+6);
+    XNi = dmin(dmax(5.38E+7 * exp(log(rhoqi) * (ORA_PLANT == 29 ? 0.7 : 0.75)), 1.0E+3), 1.0E+6);
     XMi = rhoqi / XNi;
     Di = dmin(Di_a * sqrt(XMi), Di_max);
     tmp = 4.0 * Di * XNi * Rdens * (Sice - 1.0) * Giv;
     wk[I_Pidep] = (wk[I_spsati]) * (tmp);
     wk[I_Pisub] = (1.0 - wk[I_spsati]) * (-tmp);
 
-    sw = (0.5 - copysign(0.5, temc + 40.0));
+    sw = (0.5 - copysign(0.5, temc + (ORA_PLANT == 32 ? 38.0 : 40.0)));
     wk[I_Pihom] = sw * qc / dt;
     sw = (0.5 + copysign(0.5, temc + 40.0)) * (0.5 - copysign(0.5, temc));
-    wk[I_Pihtr] = sw * (dens / rho_w * (qc * qc) / (Nc_ihtr * 1.0E+6)) * B_frz * (exp(-A_frz * temc) - 1.0);
+    wk[I_Pihtr] = sw * (dens / rho_w * (ORA_PLANT == 31 ? qc : qc * qc) / (Nc_ihtr * 1.0E+6)) * B_frz * (exp(-A_frz * temc) - 1.0);
     sw = (0.5 + copysign(0.5, temc));
-    wk[I_Pimlt] = sw * qi / dt;
+    wk[I_Pimlt] = sw * (ORA_PLANT == 39 ? qs : qi) / dt;
 
     vents = f1s * MOMs_1 + f2s * sqrt(Cs * rho_fact / Nu) * MOMs_5ds_h;
     tmp = 2.0 * PI * Rdens * (Sice - 1.0) * Giv * vents;
@@ -634,19 +639,19 @@ void ora_bigloop_point(const nsw6_params *P, double dt, double dens, double temp
     wk[I_Psmlt] = 2.0 * PI * Rdens * Gil * vents + CL * temc / LHF0 * (wk[I_Psacw] + wk[I_Psacr]);
     wk[I_Psmlt] = dmax(wk[I_Psmlt], 0.0);
 
-    ventg = f1g * GAM_2 * RLMDg_2 + f2g * sqrt(Cg * rho_fact / Nu * RLMDg_5dg) * GAM_5dg_h;
+    ventg = f1g * GAM_2 * RLMDg_2 + f2g * sqrt(Cg * rho_fact / Nu * (ORA_PLANT == 28 ? RLMDg_3dg : RLMDg_5dg)) * GAM_5dg_h;
     tmp = 2.0 * PI * Rdens * N0g * (Sice - 1.0) * Giv * ventg;
     wk[I_Pgdep] = (wk[I_spsati]) * (tmp);
     wk[I_Pgsub] = (1.0 - wk[I_spsati]) * (-tmp);
 
-    wk[I_Pgmlt] = 2.0 * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * (wk[I_Pgacw] + wk[I_Pgacr]);
+    wk[I_Pgmlt] = 2.0 * PI * Rdens * N0g * Gil * ventg + CL * temc / LHF0 * ((ORA_PLANT == 33 ? 0.0 : wk[I_Pgacw]) + wk[I_Pgacr]);
     wk[I_Pgmlt] = dmax(wk[I_Pgmlt], 0.0);
 
     wk[I_Pgfrz] = 2.0 * PI * Rdens * N0r * 60.0 * B_frz * Ar * (exp(-A_frz * temc) - 1.0) * (ORA_PLANT == 19 ? pow(RLMDr, 6) : RLMDr_7);
 
     dt1 = (exp(log(mi50) * ma2) - exp(log(mi40) * ma2)) / (a1 * ma2);
     Ni50 = qi * dt / (mi50 * dt1);
-    wk[I_Psfw] = Ni50 * (a1 * pow(mi50, a2) + PI * Eiw * dens * qc * Ri50 * Ri50 * vti50);
+    wk[I_Psfw] = Ni50 * (a1 * pow(mi50, a2) + PI * Eiw * dens * qc * Ri50 * (ORA_PLANT == 35 ? 1.0 : Ri50) * vti50);
     wk[I_Psfi] = qi / dt1;
 
     /* limiter */
@@ -1339,7 +1344,7 @@ static int satadjust_liq_core(double rho, double Emoist, double qsum, double qd_
             double Emoist_new = t * CVtot + qq[I_QV] * LHV;
             double dqsat_dT = (LovR_liq / (pow(t, 2)) + CVovR_liq / t) * qsat;
             double dCVtot_dT = dqsat_dT * (CVW[I_QV] - CVW[I_QC]);
-            double dEmoist_dT = t * dCVtot_dT + CVtot + dqsat_dT * LHV;
+            double dEmoist_dT = t * dCVtot_dT + (ORA_PLANT == 42 ? 0.0 : CVtot) + dqsat_dT * LHV;
             double dtemp = (Emoist_new - Emoist) / dEmoist_dT;
             t = t - dtemp;
             d_prev = d_last;
@@ -1420,7 +1425,7 @@ static int satadjust_all_core(double rho, double Emoist, double qsum, double qd_
             double dqc_dT = (qsum - qsat) * dalpha_dT - dqsat_dT * alpha;
             double dqi_dT = -(qsum - qsat) * dalpha_dT - dqsat_dT * (1.0 - alpha);
             double dCVtot_dT = dqsat_dT * CVW[I_QV] + dqc_dT * CVW[I_QC] + dqi_dT * CVW[I_QI];
-            double dEmoist_dT = t * dCVtot_dT + CVt + dqsat_dT * LHV - dqi_dT * LHF;
+            double dEmoist_dT = t * dCVtot_dT + CVt + dqsat_dT * LHV - (ORA_PLANT == 43 ? 0.0 : dqi_dT * LHF);
             double dtemp = (Emoist_new - Emoist) / dEmoist_dT;
             t = t - dtemp;
             d_prev = d_last;

@@ -27,12 +27,28 @@ oracle's own text:
   e + qv LHV is conserved; unsaturated interior points are unchanged; on the halo
   levels (no Newton) the condensate is evaporated; the internal energy is rebuilt as
   rhog T cv.
-* Process rates (src/mod_mp_nsw6.cpp:930-1247), at single-species points with dt small
-  enough that no limiter binds: Praci, Pgacr and Pgfrz, and the size integral of Prevp,
-  as numerical quadratures of the physical integrals over the exponential size
-  distributions (the slope from the mass content); Psacw and Psmlt from the snow
-  moments of Field et al. (2005), evaluated from the published polynomial with the
-  reference's coefficient arrays (tests/golden/reference_constants.json).
+* Process rates (src/mod_mp_nsw6.cpp:930-1247), at points with one or two species and
+  dt small enough that no limiter binds:
+  - collection (Praci, Piacr, Pgacr, Pgacw, Pgaci, Psacw, Psaci, Psacr, Pracs, Pgacs, the
+    original Pracw) as numerical quadratures of the swept volume and collision kernels
+    over the exponential size distributions (the slope from the mass content) and the
+    snow moments of Field et al. (2005), evaluated from the published polynomial with
+    the reference's coefficient arrays (tests/golden/reference_constants.json);
+  - vapour growth and sublimation (Prevp, Psdep/Pssub, Pgdep/Pgsub) as ventilated
+    diffusional growth per particle integrated over the distributions, with the
+    thermodynamic factor of Pruppacher and Klett; cloud ice (Pidep/Pisub) and
+    nucleation (Pigen) after Hong, Dudhia and Chen (2004);
+  - freezing and melting: Bigg's freezing of rain (Pgfrz) and of monodisperse cloud
+    droplets (Pihtr), homogeneous freezing below -40 C and melting above 0 C (Pihom,
+    Pimlt), melting of snow and graupel by conduction plus the heat of the collected
+    liquid (Psmlt, Pgmlt);
+  - conversions: Lin et al. (1983) autoconversion to snow and graupel, the Bergeron
+    growth time integrated from the growth law (Psfi, Psfw), KK2000 autoconversion and
+    accretion, Berry's autoconversion;
+  - the partitions of the collected mass between snow and graupel, the slope
+    parameters and the mass-weighted fall speeds.
+* The saturation Newton slopes: restated from the pinned saturation pressures with
+  numerical derivatives; the oracle's iteration counts and roots must agree.
 """
 import json
 import math
@@ -50,6 +66,7 @@ with open(os.path.join(GOLD, "reference_constants.json")) as _fh:
 C = _REF["constants"]
 CVdry, Rdry, Rvap = C["CONST_CVdry"], C["CONST_Rdry"], C["CONST_Rvap"]
 LHV0, LHF0, GRAV, TEM00 = C["CONST_LHV0"], C["CONST_LHF0"], C["CONST_GRAV"], C["CONST_TEM00"]
+LHS0 = C["CONST_LHS0"]
 PI = math.pi
 DENS00 = 1.28                       # src/mod_mp_nsw6.cpp:8
 KA0, DKA_DT = 2.428e-2, 7.47e-5     # src/mod_mp_nsw6.cpp:120-121
@@ -267,15 +284,19 @@ def check_saturation_wrapper(L):
 
 
 # --------------------------------------------------------------------- process rates
-def _point(L, P, T, rho, q6, dt=1e-3, sliq=None):
+def _point(L, P, T, rho, q6, dt=1e-3, sliq=None, sice=None, berg=(1.0, 0.5, 0.5)):
+    """One TagBigLoop evaluation; sliq / sice set the vapour from the saturation ratio
+    over liquid / ice; berg = (a1, a2, 1 - a2) of the Bergeron growth law."""
     qsl = L.ora_psat_liq(T) / (rho * Rvap * T)
     qsi = L.ora_psat_ice(T) / (rho * Rvap * T)
     q6 = list(q6)
     if sliq is not None:
         q6[0] = sliq * qsl
+    if sice is not None:
+        q6[0] = sice * qsi
     qd = 1 - sum(q6[1:])
     pre = rho * (qd * Rdry + q6[0] * Rvap) * T
-    o = O.bigloop_point(P, dt, rho, T, np.array(q6), qsl, qsi, pre, rho, 50.0, 1.0, 0.5, 0.5, L=L)
+    o = O.bigloop_point(P, dt, rho, T, np.array(q6), qsl, qsi, pre, rho, 50.0, berg[0], berg[1], berg[2], L=L)
     return o, qsl, pre
 
 
@@ -385,6 +406,420 @@ def check_psmlt(L):
     assert _rel(o.wk[I_Psmlt], want) < 1e-10, ("Psmlt", o.wk[I_Psmlt], want)
 
 
+# ---- further rates: collection, vapour growth, phase changes, conversions, partitions
+def _exp_moment(n, N0, R):
+    """int_0^inf D^n N0 exp(-D/R) dD by quadrature."""
+    return _quad(lambda D: D ** n * N0 * math.exp(-D / R))
+
+
+def _sweep(C_, D_, N0, R, rf, extra=lambda D: 1.0):
+    """int (pi/4) D^2 V(D) extra(D) N(D) dD: the volume swept per unit time by the
+    particles of an exponential distribution (V = C D^D rf), weighted by extra(D)."""
+    return _quad(lambda D: 0.25 * PI * D * D * C_ * D ** D_ * rf * extra(D) * N0 * math.exp(-D / R))
+
+
+def _mass_mean_speed(C_, D_, A_, B_, R_, rf):
+    num = _quad(lambda D: A_ * D ** B_ * C_ * D ** D_ * rf * math.exp(-D / R_))
+    return num / _quad(lambda D: A_ * D ** B_ * math.exp(-D / R_))
+
+
+def _snow_speed(P, Tc, M2, rf):
+    """Mass-weighted mean fall speed of snow with m = As D^2, V = Cs D^Ds rf over the
+    Field et al. (2005) moments."""
+    assert P.Bs == 2.0
+    return P.Cs * rf * _field_moment(2.0 + P.Ds, Tc, M2) / _field_moment(2.0, Tc, M2)
+
+
+def _diffusion(T, rho, pre, qsat, L_):
+    """Thermodynamic factor of vapour diffusion to a particle (Pruppacher and Klett):
+    1 / (L/(Ka T) (L/(Rv T) - 1) + 1/(Kd rho qsat))."""
+    Tc = T - TEM00
+    Ka, Kd = KA0 + DKA_DT * Tc, (KD0 + DKD_DT * Tc) * PRE00 / pre
+    return 1.0 / (L_ / (Ka * T) * (L_ / (Rvap * T) - 1.0) + 1.0 / (Kd * rho * qsat))
+
+
+def check_collection_by_graupel(L):
+    """Pgacw, Pgaci: graupel sweeping cloud water / cloud ice (Egi set to 0.1, its
+    default 0 makes Pgaci vanish)."""
+    P = O.params()
+    P.Egi = 0.1
+    T, rho, qc, qi, qg = 263.0, 0.9, 2e-4, 1e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, qc, 0, qi, 0, qg], sliq=1.0)
+    Rg = _slope(rho, qg, P.Ag, P.N0g, P.Bg)
+    sw = _sweep(P.Cg, P.Dg, P.N0g, Rg, math.sqrt(DENS00 / rho))
+    assert _rel(o.wk[I_Pgacw], qc * P.Egw * sw) < 1e-10, ("Pgacw", o.wk[I_Pgacw], qc * P.Egw * sw)
+    assert _rel(o.wk[I_Pgaci], qi * P.Egi * sw) < 1e-10, ("Pgaci", o.wk[I_Pgaci], qi * P.Egi * sw)
+
+
+def check_psaci(L):
+    """Snow sweeping cloud ice with the temperature-dependent efficiency
+    min(Esi, Esi exp(gamma_sacr Tc)), over the Field et al. snow moments."""
+    P = O.params()
+    T, rho, qi, qs = 263.0, 0.9, 1e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, 0, 0, qi, qs, 0], sice=1.0)
+    Tc = T - TEM00
+    E = min(P.Esi, P.Esi * math.exp(P.gamma_sacr * Tc))
+    want = qi * E * 0.25 * PI * P.Cs * math.sqrt(DENS00 / rho) * _field_moment(2.0 + P.Ds, Tc, rho * qs / P.As)
+    assert _rel(o.wk[I_Psaci], want) < 1e-10, ("Psaci", o.wk[I_Psaci], want)
+
+
+def check_piacr(L):
+    """Cloud-ice crystals (qi / mi of them per kg) each sweeping the rain mass."""
+    P = O.params()
+    T, rho, qi, qr = 263.0, 0.9, 1e-4, 2e-4
+    o, _, _ = _point(L, P, T, rho, [0, 0, qr, qi, 0, 0], dt=1e-6, sice=1.0)   # a fast rate: no limiter
+    R = _slope(rho, qr, P.Ar, P.N0r, P.Br)
+    want = qi / P.mi * P.Eri * _sweep(P.Cr, P.Dr, P.N0r, R, math.sqrt(DENS00 / rho),
+                                     lambda D: P.Ar * D ** P.Br)
+    assert _rel(o.wk[I_Piacr], want) < 1e-10, ("Piacr", o.wk[I_Piacr], want)
+
+
+def check_rain_snow_collection(L):
+    """Psacr (rain mass collected by snow) and Pracs (snow mass collected by rain): the
+    collision kernel (pi/4)(Dr + Ds)^2 |Vr - Vs| with mass-weighted mean speeds, over
+    the rain distribution and the snow moments.  The snow's second moment is its
+    prognostic one, M2 = rho qs / As (Bs = 2), where Psacr's Ds^2 term needs it; the
+    other moments (and M2 in Pracs and the speed, as the reference takes them) follow
+    the Field et al. relation, whose value at n = 2 differs from M2 by up to 10 %."""
+    P = O.params()
+    T, rho, qr, qs = 268.0, 0.9, 2e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, 0, qr, 0, qs, 0], sliq=1.0)
+    Tc, rf = T - TEM00, math.sqrt(DENS00 / rho)
+    Rr = _slope(rho, qr, P.Ar, P.N0r, P.Br)
+    M2 = rho * qs / P.As
+    Ms = [_field_moment(float(n), Tc, M2) for n in range(5)]
+    dV = abs(_mass_mean_speed(P.Cr, P.Dr, P.Ar, P.Br, Rr, rf) - _snow_speed(P, Tc, M2, rf))
+    mr = [_quad(lambda D, j=j: D ** j * P.Ar * D ** P.Br * P.N0r * math.exp(-D / Rr)) for j in range(3)]
+    nr = [_exp_moment(j, P.N0r, Rr) for j in range(3)]
+    want_sacr = (1 / rho) * 0.25 * PI * P.Esr * dV * (mr[0] * M2 + 2 * mr[1] * Ms[1] + mr[2] * Ms[0])
+    want_racs = (1 / rho) * 0.25 * PI * P.Esr * dV * P.As * (nr[2] * Ms[2] + 2 * nr[1] * Ms[3] + nr[0] * Ms[4])
+    assert _rel(o.wk[I_Psacr], want_sacr) < 1e-10, ("Psacr", o.wk[I_Psacr], want_sacr)
+    assert _rel(o.wk[I_Pracs], want_racs) < 1e-10, ("Pracs", o.wk[I_Pracs], want_racs)
+
+
+def check_pgacs(L):
+    """Graupel collecting snow (Egs set to 1, its default 0 makes Pgacs vanish), with the
+    efficiency min(Egs, Egs exp(gamma_gacs Tc))."""
+    P = O.params()
+    P.Egs = 1.0
+    T, rho, qs, qg = 263.0, 0.9, 3e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, 0, 0, 0, qs, qg], sice=1.0)
+    Tc, rf = T - TEM00, math.sqrt(DENS00 / rho)
+    Rg = _slope(rho, qg, P.Ag, P.N0g, P.Bg)
+    M2 = rho * qs / P.As
+    Ms = [_field_moment(float(n), Tc, M2) for n in range(5)]
+    dV = abs(_mass_mean_speed(P.Cg, P.Dg, P.Ag, P.Bg, Rg, rf) - _snow_speed(P, Tc, M2, rf))
+    ng = [_exp_moment(j, P.N0g, Rg) for j in range(3)]
+    E = min(P.Egs, P.Egs * math.exp(P.gamma_gacs * Tc))
+    want = (1 / rho) * 0.25 * PI * E * dV * P.As * (ng[2] * Ms[2] + 2 * ng[1] * Ms[3] + ng[0] * Ms[4])
+    assert _rel(o.wk[I_Pgacs], want) < 1e-10, ("Pgacs", o.wk[I_Pgacs], want)
+
+
+def _vent(f1, f2, C_, D_, N0, R, rf, nu):
+    """int D (f1 + f2 sqrt(V D / nu)) N(D) dD: the ventilated capacitance sum of an
+    exponential distribution."""
+    return _quad(lambda D: D * (f1 + f2 * math.sqrt(C_ * D ** D_ * rf * D / nu)) * N0 * math.exp(-D / R))
+
+
+def check_vapour_growth_of_snow_and_graupel(L):
+    """Psdep/Pssub and Pgdep/Pgsub: ventilated diffusional growth 2 pi D (Si - 1) G_ice
+    (f1 + f2 sqrt(V D / nu)) per particle, over the Field et al. snow moments and the
+    graupel distribution, on either side of ice saturation."""
+    P = O.params()
+    T, rho, qs, qg = 258.0, 0.8, 3e-4, 3e-4
+    Tc, rf = T - TEM00, math.sqrt(DENS00 / rho)
+    nu = (NU0 + DNU_DT * Tc) / rho
+    M2 = rho * qs / P.As
+    vs = P.f1s * _field_moment(1.0, Tc, M2) + P.f2s * math.sqrt(P.Cs * rf / nu) * _field_moment((3 + P.Ds) / 2, Tc, M2)
+    Rg = _slope(rho, qg, P.Ag, P.N0g, P.Bg)
+    vg = _vent(P.f1g, P.f2g, P.Cg, P.Dg, P.N0g, Rg, rf, nu)
+    for Si in (1.1, 0.8):
+        o, _, pre = _point(L, P, T, rho, [0, 0, 0, 0, qs, qg], sice=Si)
+        qsi = L.ora_psat_ice(T) / (rho * Rvap * T)
+        G = _diffusion(T, rho, pre, qsi, LHS0)
+        s_rate, g_rate = 2 * PI / rho * (Si - 1) * G * vs, 2 * PI / rho * (Si - 1) * G * vg
+        if Si > 1:
+            got = (o.wk[I_Psdep], o.wk[I_Pgdep], o.wk[I_Pssub], o.wk[I_Pgsub])
+            want = (s_rate, g_rate, 0.0, 0.0)
+        else:
+            got = (o.wk[I_Pssub], o.wk[I_Pgsub], o.wk[I_Psdep], o.wk[I_Pgdep])
+            want = (-s_rate, -g_rate, 0.0, 0.0)
+        assert _rel(got[0], want[0]) < 1e-10 and _rel(got[1], want[1]) < 1e-10, (Si, got, want)
+        assert got[2] == 0.0 and got[3] == 0.0, (Si, got)
+
+
+def check_cloud_ice_vapour_growth(L):
+    """Pidep/Pisub after Hong, Dudhia and Chen (2004): crystal number
+    Ni = 5.38e7 (rho qi)^0.75 m^-3 kept within [1e3, 1e6], mass m = rho qi / Ni, diameter
+    D = 11.9 m^0.5 (at most Di_max), disk capacitance D / pi: growth 4 D (Si - 1) G_ice
+    per crystal.  Points inside and at both ends of the number clip."""
+    P = O.params()
+    assert P.Di_a == 11.9
+    T, rho = 258.0, 0.8
+    for qi in (5e-5, 1e-9, 5e-2):
+        for Si in (1.1, 0.8):
+            o, _, pre = _point(L, P, T, rho, [0, 0, 0, qi, 0, 0], sice=Si)
+            qsi = L.ora_psat_ice(T) / (rho * Rvap * T)
+            Ni = min(max(5.38e7 * (rho * qi) ** 0.75, 1e3), 1e6)
+            D = min(11.9 * math.sqrt(rho * qi / Ni), P.Di_max)
+            rate = Ni * 4 * D * (Si - 1) * _diffusion(T, rho, pre, qsi, LHS0) / rho
+            got, want = (o.wk[I_Pidep], -o.wk[I_Pisub]) if Si > 1 else (-o.wk[I_Pisub], o.wk[I_Pidep])
+            assert _rel(got, rate) < 1e-10 and want == 0.0, (qi, Si, got, rate)
+
+
+def check_pigen(L):
+    """Ice nucleation after Hong et al. (2004): Ni0 = 1e3 exp(0.1 (T0 - T)) m^-3 (at least
+    1e3), mass 4.92e-11 Ni0^1.33 kg m^-3; the crystals made in one step are bounded by
+    the vapour above ice saturation."""
+    P = O.params()
+    assert P.icenuc_coef == 4.92e-11
+    T, rho, dt = 253.0, 0.8, 1e-3
+    qsi = L.ora_psat_ice(T) / (rho * Rvap * T)
+    Qi0 = 4.92e-11 * (1e3 * math.exp(0.1 * (TEM00 - T))) ** 1.33 / rho
+    for Si, qi in ((1.2, 0.0), (1.2, 0.5 * Qi0), (1.0 + 0.2 * Qi0 / qsi, 0.0)):
+        o, _, _ = _point(L, P, T, rho, [0, 0, 0, qi, 0, 0], dt=dt, sice=Si)
+        want = max(min(Qi0 - qi, (Si - 1) * qsi), 0.0) / dt
+        assert _rel(o.wk[I_Pigen], want) < 1e-9, (Si, qi, o.wk[I_Pigen], want)
+
+
+def check_cloud_phase_changes(L):
+    """Pihtr: Bigg (1953) freezing of monodisperse droplets, probability B (exp(-A Tc) - 1)
+    per volume and time, droplet volume rho qc / (rho_w Nc), Nc = Nc_ihtr cm^-3, between
+    -40 and 0 C.  Pihom: all cloud water freezes below -40 C, Pimlt: all cloud ice melts
+    above 0 C (within the step)."""
+    P = O.params()
+    rho, qc, qi, dt = 0.9, 2e-4, 1e-4, 1e-3
+    for T in (263.0, 236.0):
+        o, _, _ = _point(L, P, T, rho, [0, qc, 0, 0, 0, 0], dt=dt, sliq=1.0)
+        Tc = T - TEM00
+        J = P.B_frz * (math.exp(-P.A_frz * Tc) - 1.0)
+        want = qc * (rho * qc / (1000.0 * P.Nc_ihtr * 1e6)) * J
+        assert _rel(o.wk[I_Pihtr], want) < 1e-10, (T, o.wk[I_Pihtr], want)
+        assert o.wk[I_Pihom] == 0.0, (T, o.wk[I_Pihom])
+    for T in (233.1, 230.0):   # below -40 C
+        o, _, _ = _point(L, P, T, rho, [0, qc, 0, 0, 0, 0], dt=dt, sliq=1.0)
+        # the cloud water's other sinks share the limited qc / dt
+        assert _rel(o.wk[I_Pihom], qc / dt) < 1e-7 and o.wk[I_Pihtr] == 0.0, (T, o.wk[I_Pihom], o.wk[I_Pihtr])
+    o, _, _ = _point(L, P, 234.0, rho, [0, qc, 0, 0, 0, 0], dt=dt, sliq=1.0)   # -39.15 C
+    assert o.wk[I_Pihom] == 0.0
+    o, _, _ = _point(L, P, 275.0, rho, [0, 0, 0, qi, 0, 0], dt=dt, sliq=0.9)
+    assert _rel(o.wk[I_Pimlt], qi / dt) < 1e-12, o.wk[I_Pimlt]
+    o, _, _ = _point(L, P, 272.0, rho, [0, 0, 0, qi, 0, 0], dt=dt, sice=1.0)
+    assert o.wk[I_Pimlt] == 0.0
+
+
+def check_pgmlt(L):
+    """Graupel melting: conduction 2 pi D Ka Tc / Lf (f1 + f2 sqrt(V D / nu)) per particle
+    plus the heat CL Tc / Lf carried by the cloud water and rain it collects (the
+    unlimited Pgacw and Pgacr of the same point)."""
+    P = O.params()
+    T, rho, qc, qg = 278.0, 1.0, 1e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, qc, 0, 0, 0, qg], sliq=0.9)
+    Tc, rf = T - TEM00, math.sqrt(DENS00 / rho)
+    nu = (NU0 + DNU_DT * Tc) / rho
+    Rg = _slope(rho, qg, P.Ag, P.N0g, P.Bg)
+    cond = 2 * PI / rho * (KA0 + DKA_DT * Tc) * Tc / LHF0 * _vent(P.f1g, P.f2g, P.Cg, P.Dg, P.N0g, Rg, rf, nu)
+    gacw = qc * P.Egw * _sweep(P.Cg, P.Dg, P.N0g, Rg, rf)
+    want = cond + C["CONST_CL"] * Tc / LHF0 * gacw
+    assert _rel(o.wk[I_Pgmlt], want) < 1e-10, ("Pgmlt", o.wk[I_Pgmlt], want)
+
+
+def check_autoconversion_to_snow_and_graupel(L):
+    """Psaut (ice to snow) and Pgaut (snow to graupel) after Lin et al. (1983): zero up to
+    the threshold, then linear in the excess with the rate beta min(1, exp(gamma Tc))
+    (beta_gaut set to 1e-3, its default 0 makes Pgaut vanish)."""
+    P = O.params()
+    P.beta_gaut = 1e-3
+    T, rho = 258.0, 0.8
+    Tc = T - TEM00
+    bi = P.beta_saut * min(1.0, math.exp(P.gamma_saut * Tc))
+    bs = P.beta_gaut * min(1.0, math.exp(P.gamma_gaut * Tc))
+    for qi, qs in ((2e-4, P.qscrt_gaut * 1.5), (0.0, P.qscrt_gaut * 0.5)):
+        o, _, _ = _point(L, P, T, rho, [0, 0, 0, qi, qs, 0], sice=1.0)
+        wi, ws = max(bi * (qi - P.qicrt_saut), 0.0), max(bs * (qs - P.qscrt_gaut), 0.0)
+        assert _rel(o.wk[I_Psaut], wi) < 1e-12 or (wi == 0.0 and o.wk[I_Psaut] == 0.0), (qi, o.wk[I_Psaut], wi)
+        assert _rel(o.wk[I_Pgaut], ws) < 1e-12 or (ws == 0.0 and o.wk[I_Pgaut] == 0.0), (qs, o.wk[I_Pgaut], ws)
+
+
+def check_bergeron_rates(L):
+    """Psfi, Psfw without explicit ice generation (sw_expice = 0), between -30 and 0 C:
+    dt1 is the time a crystal growing as dm/dt = a1 m^a2 takes from mi40 to mi50
+    (integrated numerically); Psfi = qi / dt1 turns the ice into snow, and the Ni50 =
+    qi dt / (mi50 dt1) crystals that reach 50 um in the step grow by diffusion and by
+    riming pi Ri50^2 vti50 Eiw rho qc."""
+    P = O.params()
+    P.sw_expice = 0.0
+    T, rho, qc, qi, dt = 258.0, 0.8, 2e-4, 1e-4, 1e-3
+    a1, a2 = np.zeros(1), np.zeros(1)
+    ma2 = np.zeros(1)
+    L.ora_bergeron_point(T - TEM00, a1.ctypes.data, a2.ctypes.data, ma2.ctypes.data)
+    o, _, _ = _point(L, P, T, rho, [0, qc, 0, qi, 0, 0], dt=dt, sliq=1.0, berg=(a1[0], a2[0], ma2[0]))
+    dt1, _ = integrate.quad(lambda m: 1.0 / (a1[0] * m ** a2[0]), P.mi40, P.mi50, epsabs=0.0, epsrel=1e-13)
+    Ni50 = qi * dt / (P.mi50 * dt1)
+    want_w = Ni50 * (a1[0] * P.mi50 ** a2[0] + PI * P.Ri50 ** 2 * P.vti50 * P.Eiw * rho * qc)
+    assert _rel(o.wk[I_Psfi], qi / dt1) < 1e-10, ("Psfi", o.wk[I_Psfi], qi / dt1)
+    assert _rel(o.wk[I_Psfw], want_w) < 1e-10, ("Psfw", o.wk[I_Psfw], want_w)
+
+
+def check_partitions(L):
+    """Where the collected mass goes: rain-ice collisions (Piacr, Praci) feed graupel when
+    qr >= 1e-4 and snow below; snow-rain collisions (Psacr) feed snow when both qr and qs
+    are below 1e-4 and graupel otherwise, and Pracs acts only then (src/mod_mp_nsw6.cpp
+    delta1, delta2)."""
+    P = O.params()
+    T, rho, qi = 263.0, 0.9, 1e-4
+    for qr, qs in ((2e-4, 5e-5), (5e-5, 5e-5), (5e-5, 2e-4)):
+        o, _, _ = _point(L, P, T, rho, [0, 0, qr, qi, qs, 0], dt=1e-6, sice=1.0)
+        w = o.wk
+        big_r, both_small = qr >= 1e-4, qr < 1e-4 and qs < 1e-4
+        assert w[I_Piacr] > 0 and w[I_Praci] > 0 and w[I_Psacr] > 0
+        assert (w[I_Piacr_g] > 0) == big_r and (w[I_Piacr_s] > 0) == (not big_r), (qr, qs)
+        assert (w[I_Praci_g] > 0) == big_r and (w[I_Praci_s] > 0) == (not big_r), (qr, qs)
+        assert (w[I_Psacr_s] > 0) == both_small and (w[I_Psacr_g] > 0) == (not both_small), (qr, qs)
+        assert (w[I_Pracs] > 0) == (not both_small), (qr, qs)
+        assert _rel(w[I_Piacr_g] + w[I_Piacr_s], w[I_Piacr]) < 1e-12
+        assert _rel(w[I_Praci_g] + w[I_Praci_s], w[I_Praci]) < 1e-12
+        assert _rel(w[I_Psacr_g] + w[I_Psacr_s], w[I_Psacr]) < 1e-12
+
+
+def check_slopes(L):
+    """RLMDr, RLMDg (and RLMDs of the exponential snow, with the parameters the reference
+    sets without Roh and Satoh: sw_roh2014 = 0, As = pi rho_s / 6, Bs = 3): 1/lambda of
+    the exponential distribution whose mass integral is rho q; 0 below q = 1e-12."""
+    P = O.params()
+    T, rho = 263.0, 0.9
+    for q in (2e-4, 3e-6):
+        o, _, _ = _point(L, P, T, rho, [0, 0, q, 0, 0, q], sice=1.0)
+        assert _rel(o.wk[I_RLMDr], _slope(rho, q, P.Ar, P.N0r, P.Br)) < 1e-13, o.wk[I_RLMDr]
+        assert _rel(o.wk[I_RLMDg], _slope(rho, q, P.Ag, P.N0g, P.Bg)) < 1e-13, o.wk[I_RLMDg]
+    o, _, _ = _point(L, P, T, rho, [0, 0, 1e-13, 0, 0, 1e-13], sice=1.0)
+    assert o.wk[I_RLMDr] == 0.0 and o.wk[I_RLMDg] == 0.0
+    P.sw_roh2014, P.As, P.Bs = 0.0, PI * P.rho_s / 6.0, 3.0
+    P.GAM_1bs = math.gamma(1.0 + P.Bs)
+    o, _, _ = _point(L, P, T, rho, [0, 0, 0, 0, 2e-4, 0], sice=1.0)
+    assert _rel(o.wk[I_RLMDs], _slope(rho, 2e-4, P.As, P.N0s, P.Bs)) < 1e-13, o.wk[I_RLMDs]
+
+
+def check_pracw_original(L):
+    """The original rain collection of cloud water (sw_kk2000 = 0): qc Erw times the
+    volume the rain sweeps."""
+    P = O.params()
+    P.sw_kk2000 = 0.0
+    T, rho, qc, qr = 285.0, 1.0, 2e-4, 2e-4
+    o, _, _ = _point(L, P, T, rho, [0, qc, qr, 0, 0, 0], sliq=1.0)
+    R = _slope(rho, qr, P.Ar, P.N0r, P.Br)
+    want = qc * P.Erw * _sweep(P.Cr, P.Dr, P.N0r, R, math.sqrt(DENS00 / rho))
+    assert _rel(o.wk[I_Pracw], want) < 1e-10, ("Pracw", o.wk[I_Pracw], want)
+
+
+def check_fall_speeds_and_kk_accretion(L):
+    """The fall speeds handed to the sedimentation are the mass-weighted mean speeds of
+    the distributions (rain and graupel exponential, snow over the Field et al. moments),
+    negative downward; KK2000's accretion of cloud water by rain is 67 (qc qr)^1.15
+    (Khairoutdinov and Kogan 2000, eq. 33)."""
+    P = O.params()
+    T, rho, qc, qr, qs, qg = 285.0, 1.0, 2e-4, 2e-4, 3e-4, 3e-4
+    o, _, _ = _point(L, P, T, rho, [0, qc, qr, 0, 0, 0], sliq=1.0)
+    want = 67.0 * (qc * qr) ** 1.15
+    assert _rel(o.wk[I_Pracw], want) < 1e-12, ("Pracw", o.wk[I_Pracw], want)
+    T = 263.0
+    o, _, _ = _point(L, P, T, rho, [0, 0, qr, 0, qs, qg], sice=1.0)
+    rf = math.sqrt(DENS00 / rho)
+    vr = _mass_mean_speed(P.Cr, P.Dr, P.Ar, P.Br, _slope(rho, qr, P.Ar, P.N0r, P.Br), rf)
+    vg = _mass_mean_speed(P.Cg, P.Dg, P.Ag, P.Bg, _slope(rho, qg, P.Ag, P.N0g, P.Bg), rf)
+    vs = _snow_speed(P, T - TEM00, rho * qs / P.As, rf)
+    for got, want in ((o.Vt[2], -vr), (o.Vt[4], -vs), (o.Vt[5], -vg)):
+        assert _rel(got, want) < 1e-10, (got, want)
+
+
+# --------------------------------------------------------------------- Newton slopes
+def _deriv(f, T):
+    h = 1e-6 * T
+    return (f(T + h) - f(T - h)) / (2 * h)
+
+
+def _newton_numpy(R, S, T0, crit=1e-8, itelim=100):
+    """T <- T - R(T) / S(T) with the stopping rule of satadjust (|dT| < crit).  Returns
+    (iterations, T)."""
+    T = T0
+    for ite in range(1, itelim + 1):
+        d = R(T) / S(T)
+        T = T - d
+        if abs(d) < crit:
+            return ite, T
+    return -2, T
+
+
+def check_newton_slopes(L):
+    """The slopes of the saturation Newton solves.  Their residual is the moist internal
+    energy of the saturated state minus the target, E(T) = T CV(T) + qsat(T) LHV
+    [- qi(T) LHF], where CV = CVdry (qd + condensate) counts only the condensate
+    (qsum - qsat) besides the dry air.  The slope the reference forms is CV(T) +
+    LHV qsat' [- LHF qi'] with two simplifications: the change of CV with qsat drops out
+    (under SIMPLE every water species has the heat capacity CVdry, and its formula counts
+    the vapour too), and qsat' is the saturation pressure's derivative over rho Rvap T,
+    without the 1/T of the ideal-gas conversion (its coefficient CVovR, CPovR - 1 in
+    general, is 0 under SIMPLE); qi' = -(qsum - qsat) alpha' - qsat' (1 - alpha).  The
+    iteration therefore converges linearly.  It is restated here with the pressure
+    derivatives taken numerically from the pinned saturation pressures; the oracle must
+    take the same number of iterations (one more or less where the stopping test is
+    borderline) to the same temperature.
+    satadjust_liq at warm points; satadjust_all inside its liquid-ice range, where
+    alpha(T) and the fusion heat enter."""
+    import ctypes
+    cv, Lv, Lf = CVdry, LHV0, LHF0
+    for rho, T0, ss in ((1.0, 290.0, 1.1), (0.8, 280.0, 1.3), (1.1, 300.0, 1.02)):
+        qsum = ss * L.ora_psat_liq(T0) / (rho * Rvap * T0)
+        Em = T0 * cv + qsum * Lv
+        q6 = np.zeros(6)
+        q6[0] = qsum
+        t, qv, qc = ctypes.c_double(T0), ctypes.c_double(qsum), ctypes.c_double(0.0)
+        ite = L.ora_satadjust_liq_point(rho, Em, qsum, 1.0, q6.ctypes.data, ctypes.byref(t), ctypes.byref(qv),
+                                        ctypes.byref(qc))
+
+        def qsat(T):
+            return L.ora_psat_liq(T) / (rho * Rvap * T)
+
+        def R(T):
+            return T * cv * (1.0 + qsum - qsat(T)) + qsat(T) * Lv - Em
+
+        def S(T):
+            return cv * (1.0 + qsum - qsat(T)) + Lv * _deriv(L.ora_psat_liq, T) / (rho * Rvap * T)
+        n, T = _newton_numpy(R, S, T0)
+        assert ite > 0 and abs(ite - n) <= 1 and abs(t.value - T) < 1e-9, ("liq", T0, ite, n, t.value, T)
+    UL, LL = 273.16, 233.16
+    for rho, T0, ss in ((0.6, 250.0, 1.3), (0.7, 260.0, 1.1), (0.5, 240.0, 1.5)):
+        def alpha(T):
+            return min(max((T - LL) / (UL - LL), 0.0), 1.0)
+
+        def psat(T):
+            a = alpha(T)
+            return a * L.ora_psat_liq(T) + (1 - a) * L.ora_psat_ice(T)
+
+        def qsat(T):
+            return psat(T) / (rho * Rvap * T)
+        qsum = ss * qsat(T0)
+        Em = T0 * cv + qsum * Lv
+        q6 = np.zeros(6)
+        q6[0] = qsum
+        t, qv, qc, qi = ctypes.c_double(T0), ctypes.c_double(qsum), ctypes.c_double(0.0), ctypes.c_double(0.0)
+        ite = L.ora_satadjust_all_point(rho, Em, qsum, 1.0, q6.ctypes.data, UL, LL, ctypes.byref(t), ctypes.byref(qv),
+                                        ctypes.byref(qc), ctypes.byref(qi))
+
+        def qice(T):
+            return (qsum - qsat(T)) * (1 - alpha(T))
+
+        def R(T):
+            return T * cv * (1.0 + qsum - qsat(T)) + qsat(T) * Lv - qice(T) * Lf - Em
+
+        def S(T):
+            dq = _deriv(psat, T) / (rho * Rvap * T)
+            dqi = -(qsum - qsat(T)) / (UL - LL) - dq * (1 - alpha(T))
+            return cv * (1.0 + qsum - qsat(T)) + Lv * dq - Lf * dqi
+        n, T = _newton_numpy(R, S, T0)
+        assert LL < T < UL and ite > 0 and abs(ite - n) <= 1 and abs(t.value - T) < 1e-9, \
+            ("all", T0, ite, n, t.value, T)
+
 def _wk_index(name):
     """Position of a rate in the oracle's wk[] enum (parsed from its source)."""
     import re
@@ -394,8 +829,11 @@ def _wk_index(name):
     return names.index(name)
 
 
-I_Psmlt, I_Pgfrz, I_Pgacr, I_Praci, I_Psacw, I_Prevp = (_wk_index(n) for n in ("I_Psmlt", "I_Pgfrz", "I_Pgacr",
-                                                                               "I_Praci", "I_Psacw", "I_Prevp"))
+for _n in ("Psmlt", "Pgfrz", "Pgacr", "Praci", "Psacw", "Prevp", "Pgacw", "Pgaci", "Psaci", "Piacr", "Psacr",
+           "Pracs", "Pgacs", "Psdep", "Pssub", "Pgdep", "Pgsub", "Pidep", "Pisub", "Pigen", "Pihtr", "Pihom",
+           "Pimlt", "Pgmlt", "Psaut", "Pgaut", "Psfi", "Psfw", "Piacr_s", "Piacr_g", "Praci_s", "Praci_g",
+           "Psacr_s", "Psacr_g", "RLMDr", "RLMDs", "RLMDg", "Pracw"):
+    globals()["I_" + _n] = _wk_index("I_" + _n)
 
 # pin name -> (check, the plants it must catch)
 PINS = {
@@ -410,4 +848,21 @@ PINS = {
     "prevp": (check_prevp, (17,)),
     "psmlt": (check_psmlt, (18,)),
     "pgfrz": (check_pgfrz, (19,)),
+    "collection_by_graupel": (check_collection_by_graupel, (20, 21)),
+    "psaci": (check_psaci, (22,)),
+    "piacr": (check_piacr, (23,)),
+    "rain_snow_collection": (check_rain_snow_collection, (24, 25)),
+    "pgacs": (check_pgacs, (26,)),
+    "vapour_growth_of_snow_and_graupel": (check_vapour_growth_of_snow_and_graupel, (27, 28)),
+    "cloud_ice_vapour_growth": (check_cloud_ice_vapour_growth, (29,)),
+    "pigen": (check_pigen, (30,)),
+    "cloud_phase_changes": (check_cloud_phase_changes, (31, 32, 39)),
+    "pgmlt": (check_pgmlt, (33,)),
+    "autoconversion_to_snow_and_graupel": (check_autoconversion_to_snow_and_graupel, (34,)),
+    "bergeron_rates": (check_bergeron_rates, (35,)),
+    "partitions": (check_partitions, (36,)),
+    "slopes": (check_slopes, (37,)),
+    "pracw_original": (check_pracw_original, (38,)),
+    "fall_speeds_and_kk_accretion": (check_fall_speeds_and_kk_accretion, (40, 41)),
+    "newton_slopes": (check_newton_slopes, (42, 43)),
 }

@@ -25,6 +25,30 @@ function to fail on it, while the same pin passes on the oracle
  17  Prevp's ventilation with lambda^3 instead of lambda^2
  18  Psmlt's conduction term over LHV0 instead of LHF0
  19  Pgfrz with lambda^6 instead of lambda^7
+ 20  Pgacw with Gamma(3) instead of Gamma(3 + Dg)
+ 21  Pgaci with the cloud-water efficiency Egw instead of Egi
+ 22  Psaci without the temperature factor of its efficiency
+ 23  Piacr with Gamma(3 + Dr) instead of Gamma(6 + Dr)
+ 24  Psacr without the factor 2 of the cross term
+ 25  Pracs with the snow moment 3 instead of 4
+ 26  Pgacs with the rain speed instead of the snow speed
+ 27  the ice diffusion factor with LHV0 instead of LHS0
+ 28  the graupel ventilation with lambda^(3+dg) instead of lambda^(5+dg)
+ 29  the cloud-ice number with (rho qi)^0.7 instead of ^0.75
+ 30  the nucleated ice mass with Ni0^1.3 instead of ^1.33
+ 31  Pihtr linear in qc
+ 32  Pihom from -38 C instead of -40 C
+ 33  Pgmlt without the heat of the collected cloud water
+ 34  Pgaut with the ice threshold instead of the snow threshold
+ 35  Psfw's riming with Ri50 instead of Ri50^2
+ 36  the rain partition at qr = 1e-5 instead of 1e-4
+ 37  the graupel slope with the rain intercept N0r
+ 38  the original Pracw with Gamma(3) instead of Gamma(3 + Dr)
+ 39  Pimlt melting the snow mass instead of the ice mass
+ 40  KK2000's accretion with exponent 1.5 instead of 1.15
+ 41  the graupel fall speed with lambda instead of lambda^(dg)
+ 42  satadjust_liq's slope without the heat capacity
+ 43  satadjust_all's slope without the fusion term
 """
 import os
 import subprocess
