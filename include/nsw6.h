@@ -232,16 +232,18 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
 /* Reverse mode of one FP64 step (Roh & Satoh snow moments, or the exponential ones with
  * sw_roh2014 = 0; KK2000, or Berry's autoconversion with sw_kk2000 = 0; explicit ice
  * nucleation, sw_expice = 1, or without it, sw_expice = 0: the Bergeron growth and the
- * mixed-phase saturation adjustment; KIN2EIN or KIN2KIN, 3WATER; each with or without
- * OPT_INDIR, which then reads `state->unccn`): given the step's INPUT state `state`
+ * mixed-phase saturation adjustment; KIN2EIN or KIN2KIN; 3WATER, or 4WATER with QI
+ * sedimenting at Vti (or CONST_Vti under sw_constVti); each with or without OPT_INDIR,
+ * which then reads `state->unccn`): given the step's INPUT state `state`
  * (not modified; the diagnostics a step would write are kept in context scratch)
  * and cotangents of the step's outputs, write the cotangents of its inputs,
  *     grad = (d outputs / d inputs)^T cot,   grad_coef = (d outputs / d coef)^T cot.
  * Output cotangents `cot` (NULL members = 0): rhog, rhogvx, rhogvy, rhogvz, rhogw,
  *   rhoge, rhogq, rho, tem, pre, q, qd (after the step), precip, precip_rhoe,
  *   precip_lh_heat, precip_rhophi, precip_rhokin, GPREC; plus `cot_sflux`
- *   [l][3][ij] (NULL = 0) for the surface mass fluxes sflux_t = -f_t(kmin)/dt of
- *   QR, QS, QG.  The other diagnostics carry no cotangent.
+ *   [l][NT][ij] (NULL = 0) for the surface mass fluxes sflux_t = -f_t(kmin)/dt of the
+ *   NT sedimenting tracers: QR, QS, QG (3WATER) or QR, QI, QS, QG (4WATER).  The other
+ *   diagnostics carry no cotangent.
  * Input cotangents `grad` (NULL members are not written): rhog, rhogvx, rhogvy,
  *   rhogvz, rhogw, rhoge (nonzero on the two halo levels only: the interior internal
  *   energy is recomputed from tem), rhogq, vx, vy, vz, tem, q (QV only; other
@@ -253,15 +255,15 @@ int nsw6_mp_driver_noalloc(nsw6_ctx *ctx, int l_region, int lall, int ijdim,
  * solution (implicit-function theorem).  Regions whose departures leave the next cell
  * (the general wall fluxes) are differentiated like the others; nsw6_last_status()
  * counts them in n_general_regions.
- * Any other scheme (4WATER, a switch other than 0 or 1) returns NSW6_UNSUPPORTED at
- * once (nothing is launched).
+ * Any other scheme (a switch other than 0 or 1) returns NSW6_UNSUPPORTED at once
+ * (nothing is launched).
  * With mp_div_num = n > 1 (nsw6_set_div_num) the call is differentiated as the
  * forward call runs it: n sub-steps of dt/n, precip, precip_rhoe/lh_heat/rhophi/
  * rhokin, GPREC and the surface mass fluxes averaged over them (their cotangents reach
  * every sub-step times 1/n), the state after the last sub-step.  The sub-step inputs
  * are rebuilt by n-1 forward sub-steps into context scratch of 21*96 doubles per
- * column for each of them, plus 2*21*96 + 7*96 + 9 doubles per column, a second step
- * workspace of about 22*96 doubles per column, and the region flags (grown on demand).
+ * column for each of them, plus 2*21*96 + 7*96 + 10 doubles per column, a second step
+ * workspace of about 24*96 doubles per column, and the region flags (grown on demand).
  * The last sub-step's forward part goes to the second workspace, so that sub-step
  * n-2 reuses the forward part its rebuild left in the first.  A region that takes the
  * general path in any sub-step is counted as above.  Timing ids 6/7 then cover the

@@ -118,9 +118,10 @@ NSW6_HD void zdis_half_b(const GridDev &g, double dt, double dt2, double dt3_6, 
 // ---------------------------------------------------------------------------------
 // Transport cells and wall fluxes
 // ---------------------------------------------------------------------------------
-struct CellB {   // cotangent of a Cell
-    double rhoq[3], qrgs[3], q[3], tem, kinh, up, vp, hv[3][2], rhog, kinv;
+template <int NT> struct CellBN {   // cotangent of a CellN
+    double rhoq[NT], qrgs[NT], q[NT], tem, kinh, up, vp, hv[NT][2], rhog, kinv;
 };
+using CellB = CellBN<3>;
 struct CellInB {   // cotangent of a CellIn (differentiable members)
     double rhog, rq[5], rhoge, rvx, rvy, rvz, rw, vx, vy, vz;
 };
@@ -132,10 +133,10 @@ template <typename T> NSW6_HD void zero_words(T &v)
 #pragma unroll
     for (int i = 0; i < (int)(sizeof(T) / sizeof(double)); i++) w[i] = 0.0;
 }
-NSW6_HD void zero(CellB &c) { zero_words(c); }
+template <int NT> NSW6_HD void zero(CellBN<NT> &c) { zero_words(c); }
 NSW6_HD void zero(CellInB &c) { zero_words(c); }
 
-NSW6_HD void rhof_b(const Cell<double> &c, int t, int i, double b, CellB &cb)
+template <int NT> NSW6_HD void rhof_b(const CellN<double, NT> &c, int t, int i, double b, CellBN<NT> &cb)
 {
     switch (i) {
     case 0: cb.rhoq[t] += b; break;
@@ -161,11 +162,11 @@ NSW6_HD void rhof_b(const Cell<double> &c, int t, int i, double b, CellB &cb)
 
 // Reverse of wall_flux for tracer T (a compile-time index, so that the Cell
 // cotangents stay in registers).
-template <int T>
-NSW6_HD void wall_flux_b_tracer(const GridDev &g, int w, const Cell<double> &cur, const Cell<double> &nxt,
-                                const double z0[3], const double zh0[3], const WallGeom &geo,
-                                const WallFlux<double> &Fb, const double ft_surf_b[3][4], CellB &cur_b,
-                                CellB &nxt_b, double z0_b[3], double zh0_b[3])
+template <int T, int NT>
+NSW6_HD void wall_flux_b_tracer(const GridDev &g, int w, const CellN<double, NT> &cur, const CellN<double, NT> &nxt,
+                                const double z0[NT], const double zh0[NT], const WallGeomN<NT> &geo,
+                                const WallFluxN<double, NT> &Fb, const double ft_surf_b[NT][4], CellBN<NT> &cur_b,
+                                CellBN<NT> &nxt_b, double z0_b[NT], double zh0_b[NT])
 {
     const double dzw = g.dgz[w];
 #pragma unroll
@@ -203,25 +204,32 @@ NSW6_HD void wall_flux_b_tracer(const GridDev &g, int w, const Cell<double> &cur
 }
 
 // Reverse of wall_flux (the departure geometry is given).
-NSW6_HD void wall_flux_b(const GridDev &g, int w, const Cell<double> &cur, const Cell<double> &nxt,
-                         const double z0[3], const double zh0[3], const WallGeom &geo, const WallFlux<double> &Fb,
-                         const double ft_surf_b[3][4], CellB &cur_b, CellB &nxt_b, double z0_b[3], double zh0_b[3])
+template <int NT>
+NSW6_HD void wall_flux_b(const GridDev &g, int w, const CellN<double, NT> &cur, const CellN<double, NT> &nxt,
+                         const double z0[NT], const double zh0[NT], const WallGeomN<NT> &geo,
+                         const WallFluxN<double, NT> &Fb, const double ft_surf_b[NT][4], CellBN<NT> &cur_b,
+                         CellBN<NT> &nxt_b, double z0_b[NT], double zh0_b[NT])
 {
-    wall_flux_b_tracer<0>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
-    wall_flux_b_tracer<1>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
-    wall_flux_b_tracer<2>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    wall_flux_b_tracer<0, NT>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    wall_flux_b_tracer<1, NT>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    wall_flux_b_tracer<2, NT>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
+    if constexpr (NT == 4)
+        wall_flux_b_tracer<3, NT>(g, w, cur, nxt, z0, zh0, geo, Fb, ft_surf_b, cur_b, nxt_b, z0_b, zh0_b);
 }
 
 // Reverse of derive_cell: Cell cotangent -> raw cell cotangent and the below-cell
 // cotangents (rhog and q of cell j-1).
+template <int NT>
 NSW6_HD void derive_cell_b(const GridDev &g, int j, const CellIn<double> &r, double below_rhog,
-                           const double below_q[3], double below_gs, const Unit &u, const Cell<double> &c,
-                           const CellB &cb, CellInB &rb, double &below_rhog_b, double below_q_b[3])
+                           const double below_q[NT], double below_gs, const Unit &u, const CellN<double, NT> &c,
+                           const CellBN<NT> &cb, CellInB &rb, double &below_rhog_b, double below_q_b[NT])
 {
     const bool interior = (j >= KMIN && j <= KMAX);
     const double rg = r.rhog;
     const double rgs = interior ? r.gam2 / r.gs : 0.0;
-    double q_b[3] = {cb.q[0], cb.q[1], cb.q[2]};
+    double q_b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) q_b[t] = cb.q[t];
     double rg_b = cb.rhog;
     if (j >= KMIN + 1 && j <= KMAX) {
         const double c2w0 = g.afact[j] / r.gs * r.gsh;
@@ -229,7 +237,7 @@ NSW6_HD void derive_cell_b(const GridDev &g, int j, const CellIn<double> &r, dou
         const double rgsh = r.gam2h / r.gsh;
         double kinv_b = cb.kinv;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             const double qh = 0.5 * (c.q[t] + below_q[t]) * rgsh;
             const double qh_b = cb.hv[t][0] * c.kinv + cb.hv[t][1] * r.rw;
             kinv_b += cb.hv[t][0] * qh;
@@ -245,13 +253,13 @@ NSW6_HD void derive_cell_b(const GridDev &g, int j, const CellIn<double> &r, dou
     }
     double qq_b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < 3; t++) q_b[t] += cb.qrgs[t] * rgs;
-    rb.rq[1] += cb.rhoq[0] * rgs;
-    rb.rq[3] += cb.rhoq[1] * rgs;
-    rb.rq[4] += cb.rhoq[2] * rgs;
-    qq_b[1] += q_b[0];
-    qq_b[3] += q_b[1];
-    qq_b[4] += q_b[2];
+    for (int t = 0; t < NT; t++) q_b[t] += cb.qrgs[t] * rgs;
+    // tracer t is rq[sed_nq(t) - 1] (rq holds QC..QG)
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        rb.rq[sed_nq<NT>(t) - 1] += cb.rhoq[t] * rgs;
+        qq_b[sed_nq<NT>(t) - 1] += q_b[t];
+    }
     rb.vx += cb.up * u.ix + cb.vp * u.jx;
     rb.vy += cb.up * u.iy + cb.vp * u.jy;
     rb.vz += cb.up * u.iz + cb.vp * u.jz;
@@ -281,11 +289,12 @@ NSW6_HD void derive_cell_b(const GridDev &g, int j, const CellIn<double> &r, dou
 // ---------------------------------------------------------------------------------
 // Divergence and KIN2EIN of one cell (reverse of sediment)
 // ---------------------------------------------------------------------------------
-struct SedInB {
-    double fq[4][3], fs[2][5], fh[3][2];
-    double rw_c, rw_p, rg_m, rg_c, rg_p, rq[3], re0, rvx, rvy, rvz;
+template <int NT> struct SedInBN {
+    double fq[4][NT], fs[2][5], fh[3][2];
+    double rw_c, rw_p, rg_m, rg_c, rg_p, rq[NT], re0, rvx, rvy, rvz;
 };
-NSW6_HD void zero(SedInB &s) { zero_words(s); }
+using SedInB = SedInBN<3>;
+template <int NT> NSW6_HD void zero(SedInBN<NT> &s) { zero_words(s); }
 
 // Reverse of kin2kin_v (nsw6_step.h): out = w f rhogh with w = rw / rhogh,
 // f = sqrt(|kin_v0 / kin_v|), kin_v0 = kinv / rhogh, kin_v = w^2 / 2 (f = 1 when kin_v
@@ -359,25 +368,29 @@ NSW6_HD void kin2kin_h_b(double rg, double kinh, const double rv[3], const doubl
     }
 }
 
-// Reverse of sediment (NT = 3), with the KIN2KIN energy correction (kin2kin) or the
-// default KIN2EIN one.
-NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<double> &x, const SedOut<double> &yb,
-                        SedInB &xb, bool kin2kin = false)
+// Reverse of sediment for NT sedimenting tracers, with the KIN2KIN energy correction
+// (kin2kin) or the default KIN2EIN one.
+template <int NT>
+NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedInN<double, NT> &x,
+                        const SedOutN<double, NT> &yb, SedInBN<NT> &xb, bool kin2kin = false)
 {
     // forward recomputation
     const double rc = g.rdgz[c];
-    double dq[3], dqp[3] = {0.0, 0.0, 0.0}, dqm[3] = {0.0, 0.0, 0.0};
+    double dq[NT], dqp[NT], dqm[NT];
 #pragma unroll
-    for (int t = 0; t < 3; t++) dq[t] = -(x.fq[2][t] - x.fq[1][t]) * rc;
+    for (int t = 0; t < NT; t++) {
+        dq[t] = -(x.fq[2][t] - x.fq[1][t]) * rc;
+        dqp[t] = dqm[t] = 0.0;
+    }
     const double du = -(x.fs[1][3] - x.fs[0][3]) * rc;
     const double dv = -(x.fs[1][4] - x.fs[0][4]) * rc;
     const bool has_p = (c + 1 <= KMAX), has_m = (c - 1 >= KMIN), has_hc = (c >= KMIN + 1);
     if (has_p)
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqp[t] = -(x.fq[3][t] - x.fq[2][t]) * g.rdgz[c + 1];
+        for (int t = 0; t < NT; t++) dqp[t] = -(x.fq[3][t] - x.fq[2][t]) * g.rdgz[c + 1];
     if (has_m)
 #pragma unroll
-        for (int t = 0; t < 3; t++) dqm[t] = -(x.fq[1][t] - x.fq[0][t]) * g.rdgz[c - 1];
+        for (int t = 0; t < NT; t++) dqm[t] = -(x.fq[1][t] - x.fq[0][t]) * g.rdgz[c - 1];
     const double hw_c = has_hc ? -(x.fh[1][1] - x.fh[0][1]) * g.rdgzh[c] : 0.0;
     const double hw_p = has_p ? -(x.fh[2][1] - x.fh[1][1]) * g.rdgzh[c + 1] : 0.0;
     const double c2w0_c = g.afact[c] / x.gs_c * x.gsh_c, c2w1_c = g.bfact[c] / x.gs_m * x.gsh_c;
@@ -392,11 +405,13 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
     const double rvy_n = x.rvy + du * u.iy + dv * u.jy;
     const double rvz_n = x.rvz + du * u.iz + dv * u.jz;
     const double rw_n = x.rw_c + hw_c;
-    double rg = x.rg_c;
+    double rg = x.rg_c, rg_cp = x.rg_p, rg_cm = x.rg_m;
 #pragma unroll
-    for (int t = 0; t < 3; t++) rg = rg + dq[t];
-    const double rg_cp = ((x.rg_p + dqp[0]) + dqp[1]) + dqp[2];
-    const double rg_cm = ((x.rg_m + dqm[0]) + dqm[1]) + dqm[2];
+    for (int t = 0; t < NT; t++) {
+        rg = rg + dq[t];
+        rg_cp = rg_cp + dqp[t];
+        rg_cm = rg_cm + dqm[t];
+    }
     const double rw_cp = x.rw_p + hw_p;
     const double kinh_new = 0.5 * (rvx_n * rvx_n + rvy_n * rvy_n + rvz_n * rvz_n) / rg;
     const double den_nc = c2w0_c * rg + c2w1_c * rg_cm;
@@ -451,18 +466,18 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
     double hw_p_b = 0.0, hw_c_b = 0.0, hk_p_b = 0.0, hk_c_b = 0.0;
     xb.rw_p += rw_cp_b;
     hw_p_b += rw_cp_b;
-    double dqm_b[3], dqp_b[3], dq_b[3];
+    double dqm_b[NT], dqp_b[NT], dq_b[NT];
     xb.rg_m += rg_cm_b;
     xb.rg_p += rg_cp_b;
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         dqm_b[t] = rg_cm_b;
         dqp_b[t] = rg_cp_b;
     }
     const double kinh_u_b = tmp2_b + k2k_kinh_u_b, kinv_cp_b = tmp2_b * w2c0, kinv_c_b = tmp2_b * w2c1 + k2k_kinv_c_b;
     xb.rg_c += rg_b;
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         dq_b[t] = rg_b + yb.rq[t] - e1_b * C_GRAV * x.z;
         xb.rq[t] += yb.rq[t];
     }
@@ -512,7 +527,7 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
         xb.fh[1][1] += hw_p_b * r;
         const double rq = g.rdgz[c + 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             xb.fq[3][t] -= dqp_b[t] * rq;
             xb.fq[2][t] += dqp_b[t] * rq;
         }
@@ -520,13 +535,13 @@ NSW6_HD void sediment_b(const GridDev &g, int c, const Unit &u, const SedIn<doub
     if (has_m) {
         const double rq = g.rdgz[c - 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             xb.fq[1][t] -= dqm_b[t] * rq;
             xb.fq[0][t] += dqm_b[t] * rq;
         }
     }
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         xb.fq[2][t] -= dq_b[t] * rc;
         xb.fq[1][t] += dq_b[t] * rc;
     }
@@ -980,7 +995,7 @@ NSW6_HD void stash_fence()
 // from P at run time).  a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
 template <bool DEFAULT_SCHEME>
 NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, const NcPoint *ncp, const double qt_b[6],
-                     const double vt_b[3], RateInB &b, double qt[6], const RateStash &st, const double *a1_tab,
+                     const double vt_b[3], double vti_b, RateInB &b, double qt[6], const RateStash &st, const double *a1_tab,
                      const double *a2_tab)
 {
     double raw_b[NRATE], dqdt_b[6] = {0, 0, 0, 0, 0, 0};
@@ -1574,6 +1589,13 @@ NSW6_HD void rates_b(const DevParams &P, double dt, const RateIn<double> &in0, c
     }
     // fall speeds
     {
+        // Vti = (1 - sw_constVti) (-3.29) (dens qi + zi)^0.16 (1 - zi) + sw_constVti (-CONST_Vti)
+        // ("4WATER" only), zi = [qi < 1e-8]
+        if (vti_b != 0.0 && !(qi - 1.0E-8 < 0.0)) {
+            const double dV = (1.0 - P.sw_constVti) * (-3.29) * 0.16 * exp(log(dens * qi) * (0.16 - 1.0));
+            dens_b += vti_b * dV * qi;
+            q_b[QI] += vti_b * dV * dens;
+        }
         // Vtg = -Cg rho_fact kVg Gdg (evaluated as ((-Cg rho_fact) GAM_1bgdg / GAM_1bg) Gdg)
         tun_b[6] -= Vtg_b * rho_fact * kVg * Gdg;
         rf_b -= Vtg_b * Cg * kVg * Gdg;
@@ -1722,7 +1744,7 @@ struct PointInB {
     double nc_n, nc_dc;   // cotangents of the point's Nc and Dc
 };
 struct PointOutB {
-    double rhog, rhoge, rhogq[6], vt[3];
+    double rhog, rhoge, rhogq[6], vt[3], vti;   // vt: QR, QS, QG; vti: QI ("4WATER")
 };
 
 // a1_tab/a2_tab: the Bergeron tables (read only without explicit ice generation).
@@ -1765,7 +1787,7 @@ NSW6_HD void point_rates_b(const DevParams &P, double dt, const PointIn<double> 
         st[RATE_STASH + 2 * NQ + 1] = rhog_b;
         stash_fence();
         double qt[NQ];
-        rates_b<DEFAULT_SCHEME>(P, dt, in, ncp, qt_b, yb.vt, ib, qt, st, a1_tab, a2_tab);
+        rates_b<DEFAULT_SCHEME>(P, dt, in, ncp, qt_b, yb.vt, yb.vti, ib, qt, st, a1_tab, a2_tab);
         stash_fence();
 #pragma unroll
         for (int n = 0; n < NQ; n++) {

@@ -27,34 +27,37 @@ struct AdjLaunch {
     double *grad_coef;   // [NCOEF]
 };
 
+template <int NT>
 __global__ void __launch_bounds__(256) k_adj_cell(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     if (!lb.valid || ij >= ijdim) return;
-    adj_cell(L.A, lb.l, lb.k, ij);
+    adj_cell<NT>(L.A, lb.l, lb.k, ij);
 }
 
 // One wave per SIMD: the 512-register budget keeps the pass out of scratch (the
 // overflow goes to 66 accumulation registers); at two waves 236 B per lane spilled
 // to scratch and the reverse passes took about 4 % longer.
+template <int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_wall(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
     const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     if (!lb.valid || ij >= ijdim) return;
-    adj_wall(L.A, lb.l, lb.k, ij);
+    adj_wall<NT>(L.A, lb.l, lb.k, ij);
 }
 
 // The general regions' walls: one thread per column walks them (adj_walls_general);
 // the single-cell regions' blocks return at once.
+template <int NT>
 __global__ void __launch_bounds__(64) k_adj_wall_general(AdjLaunch L)
 {
     const int ij = blockIdx.x * 64 + threadIdx.x;
     if (ij >= L.A.a.ijdim) return;
-    adj_walls_general(L.A, blockIdx.y, ij);
+    adj_walls_general<NT>(L.A, blockIdx.y, ij);
 }
 
 // LDS holds two things here (the pass runs one block per CU, so it has the room):
@@ -65,7 +68,7 @@ __global__ void __launch_bounds__(64) k_adj_wall_general(AdjLaunch L)
 // * the rate reverse's stash (POINT_STASH doubles per thread, column threadIdx.x), which
 //   takes the values that wait through the rate reverse and its phase-2 recompute out
 //   of the registers.
-template <bool INDIR, bool DEFAULT_SCHEME>
+template <int NT, bool INDIR, bool DEFAULT_SCHEME>
 __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
 {
     __shared__ double red[4][NCOEF];
@@ -87,7 +90,8 @@ __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
     double tun_b[NCOEF];
 #pragma unroll
     for (int c = 0; c < NCOEF; c++) tun_b[c] = 0.0;
-    if (lb.valid && ij < ijdim) adj_point<INDIR, DEFAULT_SCHEME>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
+    if (lb.valid && ij < ijdim)
+        adj_point<NT, INDIR, DEFAULT_SCHEME>(A, lb.l, lb.k, ij, tun_b, RateStash{&stash[0][threadIdx.x], 256});
     // per-block sums of the coefficient cotangents (every thread reaches this point)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -158,6 +162,25 @@ extern "C" hipError_t nsw6_launch_merge_flags(int *acc, const int *flags, int n,
     return hipGetLastError();
 }
 
+// The reverse passes for NT sedimenting tracers.  The point pass of the default scheme
+// has a kernel of its own (without the Berry, Bergeron and exponential-moment terms);
+// "4WATER" takes the general one.
+template <int NT> static void launch_reverse_passes(const AdjLaunch &L, int nb, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_adj_cell<NT>, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_wall<NT>, dim3(nb), dim3(256), 0, stream, L);
+    hipLaunchKernelGGL(k_adj_wall_general<NT>, dim3((L.A.a.ijdim + 63) / 64, L.A.a.lall), dim3(64), 0, stream, L);
+    const bool dflt = (NT == 3) && L.A.a.default_scheme;
+    if (!dflt && L.A.a.opt_indir)
+        hipLaunchKernelGGL((k_adj_point<NT, true, false>), dim3(nb), dim3(256), 0, stream, L);
+    else if (!dflt)
+        hipLaunchKernelGGL((k_adj_point<NT, false, false>), dim3(nb), dim3(256), 0, stream, L);
+    else if (L.A.a.opt_indir)
+        hipLaunchKernelGGL((k_adj_point<3, true, true>), dim3(nb), dim3(256), 0, stream, L);
+    else
+        hipLaunchKernelGGL((k_adj_point<3, false, true>), dim3(nb), dim3(256), 0, stream, L);
+}
+
 // Number of blocks of the reverse passes (the coefficient partials hold NCOEF each).
 extern "C" int nsw6_adjoint_blocks(int lall, int ijdim) { return level_grid((ijdim + 255) / 256, KALL, lall); }
 
@@ -177,18 +200,10 @@ extern "C" hipError_t nsw6_launch_adjoint(const AdjArgs *adj, double *coef_part,
     }
     if (ev) (void)hipEventRecord(ev[1], stream);
     const int nb = nsw6_adjoint_blocks(L.A.a.lall, L.A.a.ijdim);
-    hipLaunchKernelGGL(k_adj_cell, dim3(nb), dim3(256), 0, stream, L);
-    hipLaunchKernelGGL(k_adj_wall, dim3(nb), dim3(256), 0, stream, L);
-    hipLaunchKernelGGL(k_adj_wall_general, dim3((L.A.a.ijdim + 63) / 64, L.A.a.lall), dim3(64), 0, stream, L);
-    // the supported schemes: KK2000 or Berry, each with or without OPT_INDIR
-    if (!L.A.a.default_scheme && L.A.a.opt_indir)
-        hipLaunchKernelGGL((k_adj_point<true, false>), dim3(nb), dim3(256), 0, stream, L);
-    else if (!L.A.a.default_scheme)
-        hipLaunchKernelGGL((k_adj_point<false, false>), dim3(nb), dim3(256), 0, stream, L);
-    else if (L.A.a.opt_indir)
-        hipLaunchKernelGGL((k_adj_point<true, true>), dim3(nb), dim3(256), 0, stream, L);
+    if (L.A.a.four_water)
+        launch_reverse_passes<4>(L, nb, stream);
     else
-        hipLaunchKernelGGL((k_adj_point<false, true>), dim3(nb), dim3(256), 0, stream, L);
+        launch_reverse_passes<3>(L, nb, stream);
     hipLaunchKernelGGL(k_adj_coef, dim3(NCOEF), dim3(256), 0, stream, L, nb);
     if (ev) (void)hipEventRecord(ev[2], stream);
     return hipGetLastError();

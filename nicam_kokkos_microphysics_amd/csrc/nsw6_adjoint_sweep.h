@@ -49,17 +49,19 @@ NSW6_HD double record_at(const double *rec, int slot, int64_t N2, int64_t p2, in
     return (c >= KMIN && c <= KMAX) ? v : 0.0;
 }
 
+template <int NT>
 NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
 {
+    using CR = CellRec<NT>;
     const StepArgs &a = A.a;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
     const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
     const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
     const bool interior = (k >= KMIN && k <= KMAX);
-    double cr[NCR], dws[NDW], din[NDI];
+    double cr[CR::N], dws[NDW], din[NDI];
 #pragma unroll
-    for (int i = 0; i < NCR; i++) cr[i] = 0.0;
+    for (int i = 0; i < CR::N; i++) cr[i] = 0.0;
 #pragma unroll
     for (int i = 0; i < NDW; i++) dws[i] = 0.0;
 #pragma unroll
@@ -83,69 +85,65 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
         x.gsgam2 = a.f.gsgam2[p];
         x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
         x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
-        x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
+        if (NT == 3) x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
         TailInB xb;
         xb.rhog = xb.rhoge = 0.0;
 #pragma unroll
         for (int n = 0; n < NQ; n++) xb.rhogq[n] = 0.0;
         if (interior) {
-            SedIn<double> si;
-            load_sed(a, l, ij, k, si);
+            SedInN<double, NT> si;
+            load_sed<NT>(a, l, ij, k, si);
             const Unit u = load_unit(a.f, (int64_t)l * ijdim + ij);
-            SedOut<double> so;
+            SedOutN<double, NT> so;
             sediment(*a.grid, k, u, si, so, a.kin2kin != 0);
             x.rhog = so.rg;
             x.rhoge = so.e;
-            x.rhogq[QR] = so.rq[0];
-            x.rhogq[QS] = so.rq[1];
-            x.rhogq[QG] = so.rq[2];
+#pragma unroll
+            for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = so.rq[t];
             tail_b(*a.P, true, x, yb, xb);
-            SedOut<double> sb;
+            SedOutN<double, NT> sb;
             sb.rg = xb.rhog;
             sb.e = xb.rhoge;
-            sb.rq[0] = xb.rhogq[QR];
-            sb.rq[1] = xb.rhogq[QS];
-            sb.rq[2] = xb.rhogq[QG];
+#pragma unroll
+            for (int t = 0; t < NT; t++) sb.rq[t] = xb.rhogq[sed_nq<NT>(t)];
             sb.rvx = cot_at(C.rhogvx, p);
             sb.rvy = cot_at(C.rhogvy, p);
             sb.rvz = cot_at(C.rhogvz, p);
             sb.rw = cot_at(C.rhogw, p);
-            SedInB s;
+            SedInBN<NT> s;
             zero(s);
-            sediment_b(*a.grid, k, u, si, sb, s, a.kin2kin != 0);
+            sediment_b<NT>(*a.grid, k, u, si, sb, s, a.kin2kin != 0);
 #pragma unroll
             for (int m = 0; m < 4; m++)
 #pragma unroll
-                for (int t = 0; t < 3; t++) cr[CR_FQ + m * 3 + t] = s.fq[m][t];
+                for (int t = 0; t < NT; t++) cr[CR::FQ + m * NT + t] = s.fq[m][t];
 #pragma unroll
             for (int m = 0; m < 2; m++)
 #pragma unroll
-                for (int i = 0; i < 5; i++) cr[CR_FS + m * 5 + i] = s.fs[m][i];
+                for (int i = 0; i < 5; i++) cr[CR::FS + m * 5 + i] = s.fs[m][i];
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll
-                for (int i = 0; i < 2; i++) cr[CR_FH + m * 2 + i] = s.fh[m][i];
-            cr[CR_RG + 0] = s.rg_m;
-            cr[CR_RG + 1] = s.rg_c;
-            cr[CR_RG + 2] = s.rg_p;
-            cr[CR_RW + 0] = s.rw_c;
-            cr[CR_RW + 1] = s.rw_p;
+                for (int i = 0; i < 2; i++) cr[CR::FH + m * 2 + i] = s.fh[m][i];
+            cr[CR::RG + 0] = s.rg_m;
+            cr[CR::RG + 1] = s.rg_c;
+            cr[CR::RG + 2] = s.rg_p;
+            cr[CR::RW + 0] = s.rw_c;
+            cr[CR::RW + 1] = s.rw_p;
             dws[1] = s.re0;
             dws[2 + QV] = xb.rhogq[QV];
             dws[2 + QC] = xb.rhogq[QC];
-            dws[2 + QR] = s.rq[0];
-            dws[2 + QI] = xb.rhogq[QI];
-            dws[2 + QS] = s.rq[1];
-            dws[2 + QG] = s.rq[2];
+            if (NT == 3) dws[2 + QI] = xb.rhogq[QI];
+#pragma unroll
+            for (int t = 0; t < NT; t++) dws[2 + sed_nq<NT>(t)] = s.rq[t];
             din[0] = s.rvx;
             din[1] = s.rvy;
             din[2] = s.rvz;
         } else {
             x.rhog = a.ws.rhog[p] + 0.0;
             x.rhoge = a.ws.rhoge[p] + 0.0;
-            x.rhogq[QR] = a.ws.rhogq[pq + QR * K2] + 0.0;
-            x.rhogq[QS] = a.ws.rhogq[pq + QS * K2] + 0.0;
-            x.rhogq[QG] = a.ws.rhogq[pq + QG * K2] + 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = a.ws.rhogq[pq + sed_nq<NT>(t) * K2] + 0.0;
             tail_b(*a.P, false, x, yb, xb);
             dws[0] = xb.rhog;
             dws[1] = xb.rhoge;
@@ -158,7 +156,7 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
             din[2] = cot_at(C.rhogvz, p);
             din[3] = cot_at(C.rhogw, p);
             if (a.kin2kin && k == KALL - 1) {
-                const double rhogh = kin2kin_top_rhogh<3>(a, l, ij, x.rhog, FluxFromMemory<3>(a, l, ij));
+                const double rhogh = kin2kin_top_rhogh<NT>(a, l, ij, x.rhog, FluxFromMemory<NT>(a, l, ij));
                 double rhogh_b = 0.0, kinv_b = 0.0, rw_b = 0.0;
                 kin2kin_v_b(rhogh, 0.0, a.f.rhogw[p] + 0.0, din[3], rhogh_b, kinv_b, rw_b);
                 din[3] = rw_b;   // rhogh's cotangent is 0 on either side of the threshold
@@ -166,7 +164,7 @@ NSW6_HD void adj_cell(const AdjArgs &A, int l, int k, int ij)
         }
     }
 #pragma unroll
-    for (int i = 0; i < NCR; i++) A.cr[i * N2 + p] = cr[i];
+    for (int i = 0; i < CR::N; i++) A.cr[i * N2 + p] = cr[i];
 #pragma unroll
     for (int i = 0; i < NDW; i++) A.dws[i * N2 + p] = dws[i];
 #pragma unroll
@@ -194,8 +192,10 @@ NSW6_HD void put_cell_b(const CellInB &b, double *o)
 // (m = 0..3), cell w-m reads fs as its wall m (m = 0..1), cell w+1-m reads fh as its
 // wall m (m = 0..2); plus frain = GPREC and, at wall kmin, the surface fluxes (ftb,
 // the per-tracer fluxes before their sums).
-NSW6_HD void wall_flux_cot(const AdjArgs &A, int l, int w, int ij, WallFlux<double> &Fb, double ftb[3][4])
+template <int NT>
+NSW6_HD void wall_flux_cot(const AdjArgs &A, int l, int w, int ij, WallFluxN<double, NT> &Fb, double ftb[NT][4])
 {
+    using CR = CellRec<NT>;
     const StepArgs &a = A.a;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
@@ -204,25 +204,25 @@ NSW6_HD void wall_flux_cot(const AdjArgs &A, int l, int w, int ij, WallFlux<doub
     const double dt = a.dt;
     auto CRV = [&](int slot, int c) { return record_at(A.cr, slot, N2, p2, c, ijdim); };
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         double s = 0.0;
 #pragma unroll
-        for (int m = 0; m < 4; m++) s += CRV(CR_FQ + m * 3 + t, w + 1 - m);
+        for (int m = 0; m < 4; m++) s += CRV(CR::FQ + m * NT + t, w + 1 - m);
         Fb.fq[t] = s;
     }
 #pragma unroll
-    for (int i = 0; i < 5; i++) Fb.fs[i] = CRV(CR_FS + i, w) + CRV(CR_FS + 5 + i, w - 1);
+    for (int i = 0; i < 5; i++) Fb.fs[i] = CRV(CR::FS + i, w) + CRV(CR::FS + 5 + i, w - 1);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
         double s = 0.0;
 #pragma unroll
-        for (int m = 0; m < 3; m++) s += CRV(CR_FH + m * 2 + i, w + 1 - m);
+        for (int m = 0; m < 3; m++) s += CRV(CR::FH + m * 2 + i, w + 1 - m);
         Fb.fh[i] = s;
     }
     // GPREC(w) = frain = -fq_QR(w) / dt
     Fb.fq[0] += -cot_at(A.cot.GPREC, pw) / dt;
 #pragma unroll
-    for (int t = 0; t < 3; t++)
+    for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int i = 0; i < 4; i++) ftb[t][i] = 0.0;
     if (w == KMIN) {
@@ -230,67 +230,74 @@ NSW6_HD void wall_flux_cot(const AdjArgs &A, int l, int w, int ij, WallFlux<doub
         const double c_p0 = cot_at(A.cot.precip, pp), c_p1 = cot_at(A.cot.precip, pp + ijdim);
         const double c_lh = cot_at(A.cot.precip_lh_heat, pc), c_e = cot_at(A.cot.precip_rhoe, pc);
         const double c_phi = cot_at(A.cot.precip_rhophi, pc), c_kin = cot_at(A.cot.precip_rhokin, pc);
+        // tracer 0 (QR) is liquid, the others (QI under "4WATER", QS, QG) solid
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             ftb[t][0] = (t == 0) ? -c_p0 / dt : (-c_p1 / dt + c_lh * C_LHF0 / dt);
             ftb[t][1] = -c_e / dt;
             ftb[t][2] = -c_phi / dt;
             ftb[t][3] = -c_kin / dt;
-            if (A.cot_sflux) Fb.fq[t] += -A.cot_sflux[((int64_t)l * 3 + t) * ijdim + ij] / dt;
+            if (A.cot_sflux) Fb.fq[t] += -A.cot_sflux[((int64_t)l * NT + t) * ijdim + ij] / dt;
         }
     }
 }
 
+template <int NT>
 NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 {
+    using WRr = WallRec<NT>;
     const StepArgs &a = A.a;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
     const int64_t p2 = (int64_t)l * K2 + ij;
     const int64_t pw = p2 + (int64_t)w * ijdim;
-    double wr[NWR];
+    double wr[WRr::N];
 #pragma unroll
-    for (int i = 0; i < NWR; i++) wr[i] = 0.0;
+    for (int i = 0; i < WRr::N; i++) wr[i] = 0.0;
 
     if (a.ws.region_flag[l] == 0 && w >= KMIN && w <= KMAX) {
         const GridDev &g = *a.grid;
-        WallFlux<double> Fb;
-        double ftb[3][4];
-        wall_flux_cot(A, l, w, ij, Fb, ftb);
-        WallState s;
-        wall_state(a, l, w, ij, s);
-        CellB cb0, cb1;
+        WallFluxN<double, NT> Fb;
+        double ftb[NT][4];
+        wall_flux_cot<NT>(A, l, w, ij, Fb, ftb);
+        WallStateN<NT> s;
+        wall_state<NT>(a, l, w, ij, s);
+        CellBN<NT> cb0, cb1;
         zero(cb0);
         zero(cb1);
-        double z0b[3] = {0.0, 0.0, 0.0}, zh0b[3] = {0.0, 0.0, 0.0};
-        wall_flux_b(g, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, Fb, ftb, cb0, cb1, z0b, zh0b);
+        double z0b[NT], zh0b[NT];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) z0b[t] = zh0b[t] = 0.0;
+        wall_flux_b<NT>(g, w, s.cur, s.nxt, s.z0, s.zh0, s.geo, Fb, ftb, cb0, cb1, z0b, zh0b);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
             double Vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             zdis_full_b(g, a.dt, a.dt2, a.dt3_6, s.V[t], w, z0b[t], Vb);
             if (w >= KMIN + 1) zdis_half_b(g, a.dt, a.dt2, a.dt3_6, s.V[t], w, zh0b[t], Vb);
 #pragma unroll
-            for (int d = 0; d < 5; d++) wr[WR_V + t * 5 + d] = Vb[d];
+            for (int d = 0; d < 5; d++) wr[WRr::V + t * 5 + d] = Vb[d];
         }
         CellInB rb0, rb1;
         zero(rb0);
         zero(rb1);
-        double brg_b = 0.0, bq_b[3] = {0.0, 0.0, 0.0};
-        derive_cell_b(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt, cb1, rb1, cb0.rhog, cb0.q);
-        derive_cell_b(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur, cb0, rb0, brg_b, bq_b);
+        double brg_b = 0.0, bq_b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) bq_b[t] = 0.0;
+        derive_cell_b<NT>(g, w + 1, s.r1, s.cur.rhog, s.cur.q, s.cur.gs, s.u, s.nxt, cb1, rb1, cb0.rhog, cb0.q);
+        derive_cell_b<NT>(g, w, s.r0, s.m.rhog, s.m.q, s.m.gs, s.u, s.cur, cb0, rb0, brg_b, bq_b);
         // q_m[t] = rhogq_t(w-1) / rhog(w-1)
         double rgm_b = brg_b;
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            wr[WR_M + 1 + t] = bq_b[t] / s.m.rhog;
+        for (int t = 0; t < NT; t++) {
+            wr[WRr::M + 1 + t] = bq_b[t] / s.m.rhog;
             rgm_b -= bq_b[t] * s.m.q[t] / s.m.rhog;
         }
-        wr[WR_M + 0] = rgm_b;
-        put_cell_b(rb0, wr + WR_C0);
-        put_cell_b(rb1, wr + WR_C1);
+        wr[WRr::M + 0] = rgm_b;
+        put_cell_b(rb0, wr + WRr::C0);
+        put_cell_b(rb1, wr + WRr::C1);
     }
 #pragma unroll
-    for (int i = 0; i < NWR; i++) A.wr[i * N2 + pw] = wr[i];
+    for (int i = 0; i < WRr::N; i++) A.wr[i * N2 + pw] = wr[i];
 }
 
 // Reverse of the wall fluxes of a general region (wall_flux_general), for one column:
@@ -302,77 +309,82 @@ NSW6_HD void adj_wall(const AdjArgs &A, int l, int w, int ij)
 // of cell j-1 that cell j's lower wall reads (rhog, q) to those of wall j-1 (to the
 // cell-(w-1) slots of wall 1 for the halo cell 0), and the cotangent of Vt(j) to the
 // d = 3 slot of wall j.  adj_point gathers them like any others.
+template <int NT>
 NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
 {
+    using WRr = WallRec<NT>;
     const StepArgs &a = A.a;
     if (a.ws.region_flag[l] == 0) return;
     const int ijdim = a.ijdim;
     const GridDev &g = *a.grid;
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
     const int64_t p2 = (int64_t)l * K2 + ij, p3 = (int64_t)l * NQ * K2 + ij;
-    const int64_t pv = (int64_t)l * 3 * K2 + ij;
-    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<3>();
+    const int64_t pv = (int64_t)l * NT * K2 + ij;
+    const int *bnd = a.ws.bounds + (size_t)l * ext_per_region<NT>();
     const Unit u = load_unit(a.f, (int64_t)l * ijdim + ij);
     auto WR = [&](int slot, int w) -> double & { return A.wr[slot * N2 + p2 + (int64_t)w * ijdim]; };
-    auto derive = [&](int j, CellIn<double> &r, BelowCell &bl, Cell<double> &c) {
+    auto derive = [&](int j, CellIn<double> &r, BelowCellN<NT> &bl, CellN<double, NT> &c) {
         load_cell(a, p2, p3, K2, ijdim, j, r);
-        bl = load_below<3>(a, p2, p3, K2, ijdim, j);
+        bl = load_below<NT>(a, p2, p3, K2, ijdim, j);
         derive_cell(g, j, r, bl.rhog, bl.q, bl.gs, u, c);
     };
     // cell j's cotangent cb through derive_cell_b into the column's wall records
-    auto put_cell = [&](int j, const CellIn<double> &r, const BelowCell &bl, const Cell<double> &c, const CellB &cb) {
+    auto put_cell = [&](int j, const CellIn<double> &r, const BelowCellN<NT> &bl, const CellN<double, NT> &c,
+                        const CellBN<NT> &cb) {
         CellInB rb;
         zero(rb);
-        double brg_b = 0.0, bq_b[3] = {0.0, 0.0, 0.0};
-        derive_cell_b(g, j, r, bl.rhog, bl.q, bl.gs, u, c, cb, rb, brg_b, bq_b);
+        double brg_b = 0.0, bq_b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) bq_b[t] = 0.0;
+        derive_cell_b<NT>(g, j, r, bl.rhog, bl.q, bl.gs, u, c, cb, rb, brg_b, bq_b);
         double o[14];
         put_cell_b(rb, o);
 #pragma unroll
-        for (int i = 0; i < 14; i++) WR(WR_C0 + i, j) += o[i];
+        for (int i = 0; i < 14; i++) WR(WRr::C0 + i, j) += o[i];
         // q_m[t] = rhogq_t(j-1) / rhog(j-1)
-        double rgm_b = brg_b, rqm_b[3];
+        double rgm_b = brg_b, rqm_b[NT];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             rqm_b[t] = bq_b[t] / bl.rhog;
             rgm_b -= bq_b[t] * bl.q[t] / bl.rhog;
         }
         if (j - 1 >= KMIN) {
-            WR(WR_C0 + 0, j - 1) += rgm_b;
+            WR(WRr::C0 + 0, j - 1) += rgm_b;
 #pragma unroll
-            for (int t = 0; t < 3; t++) WR(WR_C0 + sed_nq<3>(t), j - 1) += rqm_b[t];
+            for (int t = 0; t < NT; t++) WR(WRr::C0 + sed_nq<NT>(t), j - 1) += rqm_b[t];
         } else {
-            WR(WR_M + 0, KMIN) += rgm_b;
+            WR(WRr::M + 0, KMIN) += rgm_b;
 #pragma unroll
-            for (int t = 0; t < 3; t++) WR(WR_M + 1 + t, KMIN) += rqm_b[t];
+            for (int t = 0; t < NT; t++) WR(WRr::M + 1 + t, KMIN) += rqm_b[t];
         }
     };
     for (int w = KMIN; w <= KMAX; w++) {
-        WallFlux<double> Fb;
-        double ftb[3][4];
-        wall_flux_cot(A, l, w, ij, Fb, ftb);
-        double V[3][5];
-        int kcmin[3], kcmax[3], khmin[3], khmax[3];
+        WallFluxN<double, NT> Fb;
+        double ftb[NT][4];
+        wall_flux_cot<NT>(A, l, w, ij, Fb, ftb);
+        double V[NT][5];
+        int kcmin[NT], kcmax[NT], khmin[NT], khmax[NT];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             load_vt_window(a.ws.vt + pv + t * K2, ijdim, w, V[t]);
             kcmin[t] = bnd[bounds_index(t, 0, w, 0)];
             kcmax[t] = bnd[bounds_index(t, 0, w, 1)];
             khmin[t] = bnd[bounds_index(t, 1, w, 0)];
             khmax[t] = bnd[bounds_index(t, 1, w, 1)];
         }
-        GeneralGeomN<double, 3> G;
+        GeneralGeomN<double, NT> G;
         general_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmin, kcmax, khmin, khmax, G);
-        double acc[3][6], rfc[3][6], acch[3][2], hvc[3][2];
-        general_sums(g, w, G, [&](int j, Cell<double> &c) {
+        double acc[NT][6], rfc[NT][6], acch[NT][2], hvc[NT][2];
+        general_sums(g, w, G, [&](int j, CellN<double, NT> &c) {
             CellIn<double> r;
-            BelowCell bl;
+            BelowCellN<NT> bl;
             derive(j, r, bl, c);
         }, acc, rfc, acch, hvc);
         const bool half = (w >= KMIN + 1);
         // cotangents of the sums before the |f| < EPS filter, and of zdis0
-        double vb[3][6], vhb[3][2], zd[3], zh[3], z0b[3], zh0b[3];
+        double vb[NT][6], vhb[NT][2], zd[NT], zh[NT], z0b[NT], zh0b[NT];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             zd[t] = G.z0[t] - general_zd_shift(g.dgz, w, G.kcmin[t], G.kcmax[t], G.kc[t]);
             z0b[t] = 0.0;
 #pragma unroll
@@ -393,13 +405,13 @@ NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
         // the swept cells of the ranges
         for (int j = G.lo; j <= G.hi; j++) {
             CellIn<double> r;
-            BelowCell bl;
-            Cell<double> c;
+            BelowCellN<NT> bl;
+            CellN<double, NT> c;
             derive(j, r, bl, c);
-            CellB cb;
+            CellBN<NT> cb;
             zero(cb);
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
+            for (int t = 0; t < NT; t++) {
                 const bool one = (G.kcmin[t] == w && G.kcmax[t] == w);
                 const bool in = !one && j >= G.kcmin[t] && j <= G.kcmax[t];
                 const double fa = in ? fact_of(j, G.kc[t], w, g.dgz[j]) : 0.0;
@@ -419,13 +431,13 @@ NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
         }
         // departure cells the search left at the domain's end outside the ranges
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             if (G.kc[t] < G.lo || G.kc[t] > G.hi) {
                 CellIn<double> r;
-                BelowCell bl;
-                Cell<double> c;
+                BelowCellN<NT> bl;
+                CellN<double, NT> c;
                 derive(G.kc[t], r, bl, c);
-                CellB cb;
+                CellBN<NT> cb;
                 zero(cb);
 #pragma unroll
                 for (int i = 0; i < 6; i++) rhof_b(c, t, i, vb[t][i] * zd[t], cb);
@@ -433,10 +445,10 @@ NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
             }
             if (half && (G.kch[t] < G.lo || G.kch[t] > G.hi)) {
                 CellIn<double> r;
-                BelowCell bl;
-                Cell<double> c;
+                BelowCellN<NT> bl;
+                CellN<double, NT> c;
                 derive(G.kch[t], r, bl, c);
-                CellB cb;
+                CellBN<NT> cb;
                 zero(cb);
                 cb.hv[t][0] = vhb[t][0] * zh[t];
                 cb.hv[t][1] = vhb[t][1] * zh[t];
@@ -445,14 +457,14 @@ NSW6_HD void adj_walls_general(const AdjArgs &A, int l, int ij)
         }
         // the fall speeds Vt(w-3..w+1) through the departure distances
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
+        for (int t = 0; t < NT; t++) {
             double Vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             zdis_full_b(g, a.dt, a.dt2, a.dt3_6, V[t], w, z0b[t], Vb);
             if (half) zdis_half_b(g, a.dt, a.dt2, a.dt3_6, V[t], w, zh0b[t], Vb);
 #pragma unroll
             for (int d = 0; d < 5; d++) {
                 const int j = w + d - 3;
-                if (j >= KMIN && j <= KMAX) WR(WR_V + t * 5 + 3, j) += Vb[d];
+                if (j >= KMIN && j <= KMAX) WR(WRr::V + t * 5 + 3, j) += Vb[d];
             }
         }
     }
@@ -486,9 +498,11 @@ NSW6_HD void store_point_grad(const Fields &G, int64_t p, int64_t pq, int64_t K2
 // INDIR: OPT_INDIR (the point's droplet number from unccn; A.a.opt_indir).  DEFAULT_SCHEME:
 // KK2000 alone, explicit ice generation and Roh & Satoh snow (A.a.default_scheme);
 // otherwise sw_kk2000, sw_expice and sw_roh2014 are read at run time.
-template <bool INDIR, bool DEFAULT_SCHEME>
+template <int NT, bool INDIR, bool DEFAULT_SCHEME>
 NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
+    using CR = CellRec<NT>;
+    using WRr = WallRec<NT>;
     const StepArgs &a = A.a;
     const int ijdim = a.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)a.lall * K2;
@@ -500,35 +514,34 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
     auto WRV = [&](int slot, int w) { return record_at(A.wr, slot, N2, p2, w, ijdim); };
     // cell k is cell c (own record), c-1 of cell k+1, c+1 of cell k-1; cell w-1 of
     // wall k+1, cell w of wall k, cell w+1 of wall k-1
-    const double rg_b = A.dws[0 * N2 + p] + CRV(CR_RG + 1, k) + CRV(CR_RG + 0, k + 1) + CRV(CR_RG + 2, k - 1) +
-                        WRV(WR_M + 0, k + 1) + WRV(WR_C0 + 0, k) + WRV(WR_C1 + 0, k - 1);
-    const double re_b = A.dws[1 * N2 + p] + WRV(WR_C0 + 6, k) + WRV(WR_C1 + 6, k - 1);
+    const double rg_b = A.dws[0 * N2 + p] + CRV(CR::RG + 1, k) + CRV(CR::RG + 0, k + 1) + CRV(CR::RG + 2, k - 1) +
+                        WRV(WRr::M + 0, k + 1) + WRV(WRr::C0 + 0, k) + WRV(WRr::C1 + 0, k - 1);
+    const double re_b = A.dws[1 * N2 + p] + WRV(WRr::C0 + 6, k) + WRV(WRr::C1 + 6, k - 1);
     double rq_b[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; n++) {
         rq_b[n] = A.dws[(2 + n) * N2 + p];
-        if (n >= 1) rq_b[n] += WRV(WR_C0 + n, k) + WRV(WR_C1 + n, k - 1);
+        if (n >= 1) rq_b[n] += WRV(WRr::C0 + n, k) + WRV(WRr::C1 + n, k - 1);
     }
-    rq_b[QR] += WRV(WR_M + 1, k + 1);
-    rq_b[QS] += WRV(WR_M + 2, k + 1);
-    rq_b[QG] += WRV(WR_M + 3, k + 1);
-    // Vt(k) is Vt(w + d - 3) of wall w = k + 3 - d, d = 0..4
-    double vt_b[3];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) rq_b[sed_nq<NT>(t)] += WRV(WRr::M + 1 + t, k + 1);
+    // Vt(k) is Vt(w + d - 3) of wall w = k + 3 - d, d = 0..4
+    double vt_b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
         double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < 5; d++) s += WRV(WR_V + t * 5 + d, k + 3 - d);
+        for (int d = 0; d < 5; d++) s += WRV(WRr::V + t * 5 + d, k + 3 - d);
         vt_b[t] = s;
     }
-    const double gvx = A.din[0 * N2 + p] + WRV(WR_C0 + 7, k) + WRV(WR_C1 + 7, k - 1);
-    const double gvy = A.din[1 * N2 + p] + WRV(WR_C0 + 8, k) + WRV(WR_C1 + 8, k - 1);
-    const double gvz = A.din[2 * N2 + p] + WRV(WR_C0 + 9, k) + WRV(WR_C1 + 9, k - 1);
-    const double gw = A.din[3 * N2 + p] + CRV(CR_RW + 0, k) + CRV(CR_RW + 1, k - 1) + WRV(WR_C0 + 10, k) +
-                      WRV(WR_C1 + 10, k - 1);
-    const double gux = WRV(WR_C0 + 11, k) + WRV(WR_C1 + 11, k - 1);
-    const double guy = WRV(WR_C0 + 12, k) + WRV(WR_C1 + 12, k - 1);
-    const double guz = WRV(WR_C0 + 13, k) + WRV(WR_C1 + 13, k - 1);
+    const double gvx = A.din[0 * N2 + p] + WRV(WRr::C0 + 7, k) + WRV(WRr::C1 + 7, k - 1);
+    const double gvy = A.din[1 * N2 + p] + WRV(WRr::C0 + 8, k) + WRV(WRr::C1 + 8, k - 1);
+    const double gvz = A.din[2 * N2 + p] + WRV(WRr::C0 + 9, k) + WRV(WRr::C1 + 9, k - 1);
+    const double gw = A.din[3 * N2 + p] + CRV(CR::RW + 0, k) + CRV(CR::RW + 1, k - 1) + WRV(WRr::C0 + 10, k) +
+                      WRV(WRr::C1 + 10, k - 1);
+    const double gux = WRV(WRr::C0 + 11, k) + WRV(WRr::C1 + 11, k - 1);
+    const double guy = WRV(WRr::C0 + 12, k) + WRV(WRr::C1 + 12, k - 1);
+    const double guz = WRV(WRr::C0 + 13, k) + WRV(WRr::C1 + 13, k - 1);
 
     double g_rhog, g_rhoge, g_rhogq[NQ], g_tem = 0.0, g_qv = 0.0, g_unccn = 0.0;
     if (k >= KMIN && k <= KMAX) {
@@ -547,8 +560,11 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
         yb.rhoge = re_b;
 #pragma unroll
         for (int n = 0; n < NQ; n++) yb.rhogq[n] = rq_b[n];
-#pragma unroll
-        for (int t = 0; t < 3; t++) yb.vt[t] = vt_b[t];
+        // the rate kernel's fall speeds: QR, QS, QG and, under "4WATER", QI (tracer 1)
+        yb.vt[0] = vt_b[0];
+        yb.vt[1] = vt_b[NT - 2];
+        yb.vt[2] = vt_b[NT - 1];
+        yb.vti = (NT == 4) ? vt_b[1] : 0.0;
         PointInB xb;
         xb.rhog = xb.tem = xb.qv = xb.nc_pm179 = xb.nc_n = xb.nc_dc = 0.0;
 #pragma unroll
@@ -585,18 +601,19 @@ NSW6_HD void adj_point_single(const AdjArgs &A, int l, int k, int ij, double tun
                      g_unccn);
 }
 
-template <bool INDIR, bool DEFAULT_SCHEME>
+template <int NT, bool INDIR, bool DEFAULT_SCHEME>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF], const RateStash &st)
 {
-    adj_point_single<INDIR, DEFAULT_SCHEME>(A, l, k, ij, tun_b, st);
+    adj_point_single<NT, INDIR, DEFAULT_SCHEME>(A, l, k, ij, tun_b, st);
 }
+template <int NT>
 NSW6_HD void adj_point(const AdjArgs &A, int l, int k, int ij, double tun_b[NCOEF])
 {
     double buf[POINT_STASH];
     if (A.a.opt_indir)
-        adj_point<true, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
+        adj_point<NT, true, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
     else
-        adj_point<false, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
+        adj_point<NT, false, false>(A, l, k, ij, tun_b, RateStash{buf, 1});
 }
 
 }  // namespace nsw6

@@ -33,7 +33,8 @@ struct PendingAdjTiming {
 // coefficient partials and the diagnostics the recomputed forward part writes (kept
 // away from the caller's arrays).  Allocated on the first nsw6_mp_driver_adj call.
 struct AdjScratch {
-    double *rec = nullptr;    // [ADJ_RECORD_SLOTS][max_columns * KALL]
+    double *rec = nullptr;    // [rec_slots][max_columns * KALL]
+    int rec_slots = 0;        // adj_record_slots<NT>() of the scheme it was sized for
     double *diag = nullptr;   // re_liquid, re_cld, GPREC [K2]; 13 rows of [columns]
     double *part = nullptr;   // [blocks][NSW6_NCOEF]
     int part_blocks = 0;
@@ -424,29 +425,39 @@ int nsw6_set_div_num(nsw6_ctx *ctx, int mp_div_num)
 // The reverse mode covers the Roh & Satoh or the exponential snow moments, KK2000 or
 // Berry's autoconversion, with or without explicit ice generation (without it: the
 // Bergeron growth and the mixed-phase saturation adjustment), each with or without
-// OPT_INDIR, under "3WATER" (see include/nsw6.h).
+// OPT_INDIR, under "3WATER" or "4WATER" (see include/nsw6.h).
 static bool adjoint_supported(const DevParams &P)
 {
     const bool kk = (P.sw_kk2000 == 1.0), berry = (P.sw_kk2000 == 0.0);
     const bool roh = (P.sw_roh2014 == 1.0), expo = (P.sw_roh2014 == 0.0);
     const bool expice = (P.sw_expice == 1.0 || P.sw_expice == 0.0);
-    return (roh || expo) && (kk || berry) && expice && P.four_water == 0.0;
+    return (roh || expo) && (kk || berry) && expice;
 }
 
 static int ensure_adj_scratch(nsw6_ctx *ctx, int lall, int ijdim)
 {
     AdjScratch &s = ctx->adj;
     const size_t K2 = (size_t)KALL * (size_t)ctx->max_columns;
-    if (!s.rec) {
-        if (hipMalloc(&s.rec, sizeof(double) * K2 * ADJ_RECORD_SLOTS) != hipSuccess ||
-            hipMalloc(&s.diag, sizeof(double) * (3 * K2 + 13 * (size_t)ctx->max_columns)) != hipSuccess ||
+    if (!s.diag) {
+        if (hipMalloc(&s.diag, sizeof(double) * (3 * K2 + 13 * (size_t)ctx->max_columns)) != hipSuccess ||
             hipMalloc(&s.coef, sizeof(double) * NSW6_NCOEF) != hipSuccess) {
-            (void)hipFree(s.rec);
             (void)hipFree(s.diag);
             (void)hipFree(s.coef);
-            s = AdjScratch();
+            s.diag = s.coef = nullptr;
             return NSW6_HIP_ERROR;
         }
+    }
+    // the records of the current scheme ("4WATER" needs more slots): grown when a
+    // change of scheme by nsw6_set_params asks for more
+    const int slots = ctx->P.four_water != 0.0 ? adj_record_slots<4>() : adj_record_slots<3>();
+    if (slots > s.rec_slots) {
+        // a queued adjoint may still read the old records
+        if (s.rec && hipDeviceSynchronize() != hipSuccess) return NSW6_HIP_ERROR;
+        (void)hipFree(s.rec);
+        s.rec = nullptr;
+        s.rec_slots = 0;
+        if (hipMalloc(&s.rec, sizeof(double) * K2 * (size_t)slots) != hipSuccess) return NSW6_HIP_ERROR;
+        s.rec_slots = slots;
     }
     const int nb = nsw6_adjoint_blocks(lall, ijdim);
     if (nb > s.part_blocks) {
@@ -495,10 +506,11 @@ static int adj_step(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *state
     A.cot = to_fields(*cot);
     A.cot_sflux = cot_sflux;
     A.grad = to_fields(*grad);
-    // records [slot][l][k][ij] of the current batch shape
+    // records [slot][l][k][ij] of the current batch shape and tracer count
+    const bool four = a.four_water != 0;
     A.cr = ctx->adj.rec;
-    A.wr = A.cr + (size_t)NCR * K2;
-    A.dws = A.wr + (size_t)NWR * K2;
+    A.wr = A.cr + (size_t)(four ? CellRec<4>::N : CellRec<3>::N) * K2;
+    A.dws = A.wr + (size_t)(four ? WallRec<4>::N : WallRec<3>::N) * K2;
     A.din = A.dws + (size_t)NDW * K2;
     return hip_ok(nsw6_launch_adjoint(&A, ctx->adj.part, grad_coef ? grad_coef : ctx->adj.coef, s, ev, forward));
 }
@@ -570,13 +582,14 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     const size_t C = (size_t)lall * (size_t)ijdim, K2 = (size_t)KALL * C;
     // layout: n-1 carries | 2 cotangent carries | vx, vy, vz, unccn | write-only rows (qd
     // and the constant diagnostics) | scaled GPREC | scaled column cotangents | coefficients
-    // | a second step workspace (3WATER) for the forward part of the last sub-step, so
-    // that the step workspace keeps the forward part of sub-step n-2 from the rebuild
+    // | a second step workspace (sized for "4WATER") for the forward part of the last
+    // sub-step, so that the step workspace keeps the forward part of sub-step n-2 from the
+    // rebuild
     const size_t n_carry = (size_t)(n - 1) * CARRY_K2 * K2, n_cot = 2 * (size_t)CARRY_K2 * K2;
     const size_t alt_part = (size_t)KALL * 16 * (C / EXT_COLS_PER_BLOCK + (size_t)lall);
     const size_t alt_ints = (size_t)EXT_PER_REGION_MAX * (size_t)lall + (size_t)lall + REPORT_INTS + C;
-    const size_t n_alt = (size_t)(1 + 1 + NQ + 3 + FLUX_SLOTS) * K2 + alt_part + (alt_ints + 1) / 2;
-    const size_t doubles = n_carry + n_cot + 4 * K2 + 2 * K2 + K2 + 9 * C + NSW6_NCOEF + n_alt;
+    const size_t n_alt = (size_t)(1 + 1 + NQ + 4 + FLUX_SLOTS_MAX) * K2 + alt_part + (alt_ints + 1) / 2;
+    const size_t doubles = n_carry + n_cot + 4 * K2 + 2 * K2 + K2 + 10 * C + NSW6_NCOEF + n_alt;
     int st = ensure_div_adj(ctx, doubles, lall);
     if (st != NSW6_OK) return st;
     double *base = ctx->div_adj;
@@ -585,15 +598,15 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     double *junk = vtmp + 4 * K2;   // qd (K2) and re_solid / GDCLW / GDCFRC (K2, same thread, same index)
     double *gprec_s = junk + 2 * K2;
     double *col_s = gprec_s + K2;
-    double *coef_tmp = col_s + 9 * C;
+    double *coef_tmp = col_s + 10 * C;   // 6 C of column cotangents, NT C of surface fluxes
     Workspace alt{};
     {
         double *a = coef_tmp + NSW6_NCOEF;
         alt.rhog = a; a += K2;
         alt.rhoge = a; a += K2;
         alt.rhogq = a; a += NQ * K2;
-        alt.vt = a; a += 3 * K2;
-        alt.flux = a; a += (size_t)FLUX_SLOTS * K2;
+        alt.vt = a; a += 4 * K2;
+        alt.flux = a; a += (size_t)FLUX_SLOTS_MAX * K2;
         alt.part = a; a += alt_part;
         int *ip = reinterpret_cast<int *>(a);
         alt.bounds = ip; ip += (size_t)EXT_PER_REGION_MAX * (size_t)lall;
@@ -671,8 +684,9 @@ static int adj_subcycled(nsw6_ctx *ctx, int lall, int ijdim, const nsw6_fields *
     }
     double *sflux_s = nullptr;
     if (cot_sflux) {
-        sflux_s = col_s + off;   // 3 C
-        if (!ok(nsw6_launch_scale(sflux_s, cot_sflux, fraction_mp, (int64_t)(3 * C), s))) return NSW6_HIP_ERROR;
+        const size_t nt = ctx->P.four_water != 0.0 ? 4 : 3;
+        sflux_s = col_s + off;   // nt C
+        if (!ok(nsw6_launch_scale(sflux_s, cot_sflux, fraction_mp, (int64_t)(nt * C), s))) return NSW6_HIP_ERROR;
     }
     if (cot->GPREC) {
         cdiag.GPREC = gprec_s;

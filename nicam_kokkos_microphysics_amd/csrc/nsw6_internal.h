@@ -136,16 +136,28 @@ struct StepArgs {
 };
 
 // Reverse sweep (nsw6_adjoint_sweep.h): record arrays [slot][l][k][ij] written by one
-// pass and gathered by the next.
-// cell record slots
-constexpr int CR_FQ = 0, CR_FS = 12, CR_FH = 22, CR_RG = 28, CR_RW = 31, NCR = 33;
-// wall record slots: cell w-1 (rhog, QR, QS, QG); cells w and w+1 in CellInB order
-// (rhog, rq[5], rhoge, rvx, rvy, rvz, rw, vx, vy, vz); Vt(w-3..w+1) of QR, QS, QG
-constexpr int WR_M = 0, WR_C0 = 4, WR_C1 = 18, WR_V = 32, NWR = 47;
+// pass and gathered by the next, for NT sedimenting tracers.
+// cell record slots: the cotangents of fq of walls c-1..c+2 (4 NT), of fs of walls c, c+1
+// (10), of fh of walls c-1..c+1 (6), of rhog of cells c-1..c+1 (3) and of the
+// pre-transport rhogw of cells c, c+1 (2)
+template <int NT> struct CellRec {
+    static constexpr int FQ = 0, FS = 4 * NT, FH = FS + 10, RG = FH + 6, RW = RG + 3, N = RW + 2;
+};
+// wall record slots: cell w-1 (rhog and the NT tracers); cells w and w+1 in CellInB
+// order (rhog, rq[5], rhoge, rvx, rvy, rvz, rw, vx, vy, vz); Vt(w-3..w+1) of the NT tracers
+template <int NT> struct WallRec {
+    static constexpr int M = 0, C0 = 1 + NT, C1 = C0 + 14, V = C1 + 14, N = V + 5 * NT;
+};
+constexpr int CR_FQ = CellRec<3>::FQ, CR_FS = CellRec<3>::FS, CR_FH = CellRec<3>::FH, CR_RG = CellRec<3>::RG,
+              CR_RW = CellRec<3>::RW, NCR = CellRec<3>::N;
+constexpr int WR_M = WallRec<3>::M, WR_C0 = WallRec<3>::C0, WR_C1 = WallRec<3>::C1, WR_V = WallRec<3>::V,
+              NWR = WallRec<3>::N;
 // own-cell cotangents: pre-sedimentation state (rhog, rhoge, rhogq[6]) and the input
 // momentum (rhogvx, rhogvy, rhogvz, rhogw)
 constexpr int NDW = 8, NDI = 4;
-constexpr int ADJ_RECORD_SLOTS = NCR + NWR + NDW + NDI;
+// 92 slots for NT = 3, 102 for NT = 4
+template <int NT> constexpr int adj_record_slots() { return CellRec<NT>::N + WallRec<NT>::N + NDW + NDI; }
+static_assert(adj_record_slots<3>() == 92 && adj_record_slots<4>() == 102, "record layout");
 
 struct AdjArgs {
     StepArgs a;                 // forward step state (workspace filled up to the fluxes)

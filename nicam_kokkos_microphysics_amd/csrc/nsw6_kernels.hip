@@ -469,9 +469,16 @@ extern "C" hipError_t nsw6_launch_forward_to_flux(const StepArgs *args, hipStrea
         hipLaunchKernelGGL((k_rates<double, false, true>), grid_p, dim3(256), 0, stream, a);
     else
         hipLaunchKernelGGL((k_rates<double, false>), grid_p, dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_extremes<3>, grid_e, dim3(EXT_THREADS), 0, stream, a);
-    hipLaunchKernelGGL(k_bounds<3>, dim3(6, a.lall), dim3(192), 0, stream, a);
-    hipLaunchKernelGGL(k_flux<3>, grid_l, dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_flux_general<3>, dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
+    if (a.four_water) {
+        hipLaunchKernelGGL(k_extremes<4>, grid_e, dim3(EXT_THREADS), 0, stream, a);
+        hipLaunchKernelGGL(k_bounds<4>, dim3(8, a.lall), dim3(192), 0, stream, a);
+        hipLaunchKernelGGL(k_flux<4>, grid_l, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_flux_general<4>, dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL(k_extremes<3>, grid_e, dim3(EXT_THREADS), 0, stream, a);
+        hipLaunchKernelGGL(k_bounds<3>, dim3(6, a.lall), dim3(192), 0, stream, a);
+        hipLaunchKernelGGL(k_flux<3>, grid_l, dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_flux_general<3>, dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
+    }
     return hipGetLastError();
 }

@@ -152,8 +152,9 @@ class NSW6:
                 grad_coef=None, ct: float = 0.0, l_region: int = 0, stream=None, check: bool = True):
         """Reverse mode of one FP64 step (nsw6_mp_driver_adj): `state` holds the step's
         input (device tensors, not modified), `cot` the cotangents of its outputs (keys
-        of fields.ADJ_OUTPUTS; missing = 0), `cot_sflux` [l][3][ij] those of the
-        surface mass fluxes of QR, QS, QG.  Returns (grad, grad_coef): the cotangents of
+        of fields.ADJ_OUTPUTS; missing = 0), `cot_sflux` [l][NT][ij] those of the
+        surface mass fluxes of the falling tracers (NT = 3: QR, QS, QG; under
+        four_water NT = 4: QR, QI, QS, QG).  Returns (grad, grad_coef): the cotangents of
         the inputs (keys of fields.ADJ_INPUTS, and unccn when the state holds it: nonzero
         under OPT_INDIR only) and of the 8 tunable coefficients."""
         import torch
@@ -162,7 +163,8 @@ class NSW6:
         check_device_tensors(state, lall, ijdim, self.device, "state")
         check_device_tensors(cot, lall, ijdim, self.device, "cot")
         if cot_sflux is not None:
-            _check_vector(cot_sflux, (lall, 3, ijdim), self.device, "cot_sflux")
+            nt = 4 if self.params.four_water != 0.0 else 3
+            _check_vector(cot_sflux, (lall, nt, ijdim), self.device, "cot_sflux")
         if grad is None:
             grad = {k: torch.empty_like(state[k]) for k in ADJ_INPUTS}
             if state.get("unccn") is not None:

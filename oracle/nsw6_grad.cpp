@@ -50,18 +50,18 @@ template <typename S> struct HostState {
 };
 
 struct Bounds {
-    std::vector<int> kcmin, kcmax, khmin, khmax;   // [l][3][KALL]
+    std::vector<int> kcmin, kcmax, khmin, khmax;   // [l][NT][KALL]
     std::vector<int> general;                      // [l]
 };
 
 // kcell bounds of one region from the fall-speed values (k_extremes + k_bounds,
 // src/mod_vadv1d.cpp:317-352)
-template <typename S>
+template <int NT = 3, typename S>
 void region_bounds(const GridDev &g, double dt, double dt2, double dt3, int l, int ijdim, const std::vector<S> &vt,
                    Bounds &b)
 {
     const int64_t K2 = (int64_t)KALL * ijdim;
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < NT; t++) {
         for (int type = 0; type < 2; type++) {
             const double *zh = (type == 0) ? g.gzh : g.gz_shift;
             const int mkmin = (type == 0) ? KMIN : KMIN + 1, mkmax = KMAX;
@@ -72,7 +72,7 @@ void region_bounds(const GridDev &g, double dt, double dt2, double dt3, int l, i
                     for (int ij = 0; ij < ijdim; ij++) {
                         double V[5];
                         for (int d = -3; d <= 1; d++)
-                            V[d + 3] = (k + d >= 0) ? val(vt[((int64_t)l * 3 + t) * K2 + (int64_t)(k + d) * ijdim + ij]) : 0.0;
+                            V[d + 3] = (k + d >= 0) ? val(vt[((int64_t)l * NT + t) * K2 + (int64_t)(k + d) * ijdim + ij]) : 0.0;
                         const double z = (type == 0) ? zdis_full_at(g, dt, dt2, dt3, V, k)
                                                      : zdis_half_at(g, dt, dt2, dt3, V, k);
                         if (z == z) {
@@ -99,22 +99,30 @@ void region_bounds(const GridDev &g, double dt, double dt2, double dt3, int l, i
                     if (amax > 0.0 && zh[k] - zzmax < zh[k]) b.general[l] = 1;
                     if (zh[k] - zzmin >= zh[ktop]) b.general[l] = 1;
                 }
-                (type == 0 ? b.kcmin : b.khmin)[((size_t)l * 3 + t) * KALL + k] = kcmin;
-                (type == 0 ? b.kcmax : b.khmax)[((size_t)l * 3 + t) * KALL + k] = kcmax;
+                (type == 0 ? b.kcmin : b.khmin)[((size_t)l * NT + t) * KALL + k] = kcmin;
+                (type == 0 ? b.kcmax : b.khmax)[((size_t)l * NT + t) * KALL + k] = kcmax;
             }
         }
     }
 }
 
 // One step over all regions (0; the general regions through wall_flux_general).
-template <typename S>
-int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF], HostState<S> &h, int *n_newton_fail)
+// The fall speed of sedimenting tracer t of NT from the rate kernel's outputs (QR, QS,
+// QG in vt; QI in vti under "4WATER").
+template <int NT, typename S> S fall_speed(const PointOut<S> &y, int t)
+{
+    if (NT == 3) return y.vt[t];
+    return t == 0 ? y.vt[0] : (t == 1 ? y.vti : y.vt[t - 1]);
+}
+
+template <int NT, typename S>
+int host_step_n(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF], HostState<S> &h, int *n_newton_fail)
 {
     const int lall = h.lall, ijdim = h.ijdim;
     const int64_t K2 = (int64_t)KALL * ijdim;
     const int64_t N2 = (int64_t)lall * K2;
     const double dt2 = pow(dt, 2), dt3_6 = pow(dt, 3) / 6.0;
-    std::vector<S> wrhog(N2), wrhoge(N2), wrhogq(N2 * NQ), vt(N2 * 3);
+    std::vector<S> wrhog(N2), wrhoge(N2), wrhogq(N2 * NQ), vt(N2 * NT);
 
     // ---- rates
 #pragma omp parallel for collapse(2) schedule(static)
@@ -123,12 +131,12 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
             for (int ij = 0; ij < ijdim; ij++) {
                 const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
                 const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
-                const int64_t pv = (int64_t)l * 3 * K2 + (int64_t)k * ijdim + ij;
+                const int64_t pv = (int64_t)l * NT * K2 + (int64_t)k * ijdim + ij;
                 if (k == 0 || k == KALL - 1) {
                     wrhog[p] = h.rhog[p];
                     wrhoge[p] = h.rhoge[p];
                     for (int n = 0; n < NQ; n++) wrhogq[pq + n * K2] = h.rhogq[pq + n * K2];
-                    for (int t = 0; t < 3; t++) vt[pv + t * K2] = S(0.0);
+                    for (int t = 0; t < NT; t++) vt[pv + t * K2] = S(0.0);
                     continue;
                 }
                 PointIn<S> x;
@@ -142,20 +150,21 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                 wrhog[p] = y.rhog;
                 wrhoge[p] = y.rhoge;
                 for (int n = 0; n < NQ; n++) wrhogq[pq + n * K2] = y.rhogq[n];
-                for (int t = 0; t < 3; t++) vt[pv + t * K2] = y.vt[t];
+                for (int t = 0; t < NT; t++) vt[pv + t * K2] = fall_speed<NT>(y, t);
             }
 
     // ---- kcell bounds
     Bounds b;
-    b.kcmin.assign((size_t)lall * 3 * KALL, 0);
-    b.kcmax.assign((size_t)lall * 3 * KALL, 0);
-    b.khmin.assign((size_t)lall * 3 * KALL, 0);
-    b.khmax.assign((size_t)lall * 3 * KALL, 0);
+    b.kcmin.assign((size_t)lall * NT * KALL, 0);
+    b.kcmax.assign((size_t)lall * NT * KALL, 0);
+    b.khmin.assign((size_t)lall * NT * KALL, 0);
+    b.khmax.assign((size_t)lall * NT * KALL, 0);
     b.general.assign(lall, 0);
-    for (int l = 0; l < lall; l++) region_bounds(g, dt, dt2, dt3_6, l, ijdim, vt, b);
+    for (int l = 0; l < lall; l++) region_bounds<NT>(g, dt, dt2, dt3_6, l, ijdim, vt, b);
 
-    // ---- fluxes of walls 1..94 (slots as the flux workspace: fq 3, fs 5, fh 2)
-    std::vector<S> flux(N2 * 10, S(0.0));
+    // ---- fluxes of walls 1..94 (slots as the flux workspace: fq NT, fs 5, fh 2)
+    constexpr int NS = NT + 7;
+    std::vector<S> flux(N2 * NS, S(0.0));
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
         for (int w = KMIN; w <= KMAX; w++)
@@ -184,62 +193,62 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                 const int64_t om = (int64_t)l * K2 + (int64_t)(w - 1) * ijdim + ij;
                 const int64_t oqm = (int64_t)l * NQ * K2 + (int64_t)(w - 1) * ijdim + ij;
                 const S rg_m = wrhog[om];
-                const S q_m[3] = {wrhogq[oqm + QR * K2] / rg_m, wrhogq[oqm + QS * K2] / rg_m,
-                                  wrhogq[oqm + QG * K2] / rg_m};
-                Cell<S> cur, nxt;
+                S q_m[NT];
+                for (int t = 0; t < NT; t++) q_m[t] = wrhogq[oqm + sed_nq<NT>(t) * K2] / rg_m;
+                CellN<S, NT> cur, nxt;
                 CellIn<S> r;
                 raw(w, r);
                 derive_cell(g, w, r, rg_m, q_m, h.gsgam2[om], u, cur);
                 raw(w + 1, r);
                 derive_cell(g, w + 1, r, cur.rhog, cur.q, cur.gs, u, nxt);
-                S V[3][5];
-                int kcmin[3], kcmax[3], khmin[3], khmax[3];
-                for (int t = 0; t < 3; t++) {
+                S V[NT][5];
+                int kcmin[NT], kcmax[NT], khmin[NT], khmax[NT];
+                for (int t = 0; t < NT; t++) {
                     for (int d = -3; d <= 1; d++)
-                        V[t][d + 3] = (w + d >= 0) ? vt[((int64_t)l * 3 + t) * K2 + (int64_t)(w + d) * ijdim + ij] : S(0.0);
-                    kcmin[t] = b.kcmin[((size_t)l * 3 + t) * KALL + w];
-                    kcmax[t] = b.kcmax[((size_t)l * 3 + t) * KALL + w];
-                    khmin[t] = b.khmin[((size_t)l * 3 + t) * KALL + w];
-                    khmax[t] = b.khmax[((size_t)l * 3 + t) * KALL + w];
+                        V[t][d + 3] = (w + d >= 0) ? vt[((int64_t)l * NT + t) * K2 + (int64_t)(w + d) * ijdim + ij] : S(0.0);
+                    kcmin[t] = b.kcmin[((size_t)l * NT + t) * KALL + w];
+                    kcmax[t] = b.kcmax[((size_t)l * NT + t) * KALL + w];
+                    khmin[t] = b.khmin[((size_t)l * NT + t) * KALL + w];
+                    khmax[t] = b.khmax[((size_t)l * NT + t) * KALL + w];
                 }
-                WallFlux<S> F;
-                S ft[3][4];
+                WallFluxN<S, NT> F;
+                S ft[NT][4];
                 if (b.general[l]) {
-                    GeneralGeomN<S, 3> G;
+                    GeneralGeomN<S, NT> G;
                     general_geometry(g, w, dt, dt2, dt3_6, V, kcmin, kcmax, khmin, khmax, G);
-                    auto cell_at = [&](int j, Cell<S> &c) {
+                    auto cell_at = [&](int j, CellN<S, NT> &c) {
                         CellIn<S> rj;
                         raw(j, rj);
                         const int64_t ob = (int64_t)l * K2 + (int64_t)(j - 1) * ijdim + ij;
                         const int64_t oqb = (int64_t)l * NQ * K2 + (int64_t)(j - 1) * ijdim + ij;
                         const S rg_b = wrhog[ob];
-                        const S q_b[3] = {wrhogq[oqb + QR * K2] / rg_b, wrhogq[oqb + QS * K2] / rg_b,
-                                          wrhogq[oqb + QG * K2] / rg_b};
+                        S q_b[NT];
+                        for (int t = 0; t < NT; t++) q_b[t] = wrhogq[oqb + sed_nq<NT>(t) * K2] / rg_b;
                         derive_cell(g, j, rj, rg_b, q_b, h.gsgam2[ob], u, c);
                     };
                     wall_flux_general(g, w, G, cell_at, F, ft);
                 } else {
-                    S z0[3], zh0[3];
-                    WallGeom geo;
+                    S z0[NT], zh0[NT];
+                    WallGeomN<NT> geo;
                     wall_geometry(g, w, dt, dt2, dt3_6, V, kcmax, khmax, z0, zh0, geo);
                     wall_flux(g, w, cur, nxt, z0, zh0, geo, F, ft);
                 }
-                S *fx = &flux[(int64_t)l * 10 * K2 + (int64_t)w * ijdim + ij];
-                for (int t = 0; t < 3; t++) fx[t * K2] = F.fq[t];
-                for (int i = 0; i < 5; i++) fx[(3 + i) * K2] = F.fs[i];
-                fx[8 * K2] = F.fh[0];
-                fx[9 * K2] = F.fh[1];
+                S *fx = &flux[(int64_t)l * NS * K2 + (int64_t)w * ijdim + ij];
+                for (int t = 0; t < NT; t++) fx[t * K2] = F.fq[t];
+                for (int i = 0; i < 5; i++) fx[(NT + i) * K2] = F.fs[i];
+                fx[(NT + 5) * K2] = F.fh[0];
+                fx[(NT + 6) * K2] = F.fh[1];
                 h.GPREC[om + ijdim] = (0.0 + (-F.fq[0] / dt)) * 1.0;
                 if (w == KMIN) {
                     Surface<S> sf;
-                    surface_fluxes(ft, dt, sf);
+                    surface_fluxes<NT>(ft, dt, sf);
                     h.precip[(int64_t)l * 2 * ijdim + ij] = sf.precip0;
                     h.precip[(int64_t)l * 2 * ijdim + ijdim + ij] = sf.precip1;
                     h.prhoe[pc] = sf.prhoe;
                     h.lh[pc] = sf.lh;
                     h.prhophi[pc] = sf.prhophi;
                     h.prhokin[pc] = sf.prhokin;
-                    for (int t = 0; t < 3; t++) h.sflux[((int64_t)l * 3 + t) * ijdim + ij] = -F.fq[t] / dt;
+                    for (int t = 0; t < NT; t++) h.sflux[((int64_t)l * NT + t) * ijdim + ij] = -F.fq[t] / dt;
                 }
             }
 
@@ -256,31 +265,29 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                 const bool interior = (k >= KMIN && k <= KMAX);
                 TailIn<S> x;
                 if (interior) {
-                    SedIn<S> si;
-                    const S *fx = &flux[(int64_t)l * 10 * K2 + ij];
+                    SedInN<S, NT> si;
+                    const S *fx = &flux[(int64_t)l * NS * K2 + ij];
                     auto FXV = [&](int slot, int wall) { return fx[(int64_t)slot * K2 + (int64_t)wall * ijdim]; };
-                    for (int t = 0; t < 3; t++) {
+                    for (int t = 0; t < NT; t++) {
                         si.fq[0][t] = FXV(t, k - 1);
                         si.fq[1][t] = FXV(t, k);
                         si.fq[2][t] = FXV(t, k + 1);
                         si.fq[3][t] = (k + 1 <= KMAX) ? FXV(t, k + 2) : S(0.0);
                     }
                     for (int i = 0; i < 5; i++) {
-                        si.fs[0][i] = FXV(3 + i, k);
-                        si.fs[1][i] = FXV(3 + i, k + 1);
+                        si.fs[0][i] = FXV(NT + i, k);
+                        si.fs[1][i] = FXV(NT + i, k + 1);
                     }
                     for (int m = 0; m < 3; m++) {
-                        si.fh[m][0] = FXV(8, k - 1 + m);
-                        si.fh[m][1] = FXV(9, k - 1 + m);
+                        si.fh[m][0] = FXV(NT + 5, k - 1 + m);
+                        si.fh[m][1] = FXV(NT + 6, k - 1 + m);
                     }
                     si.rw_c = h.rhogw[p];
                     si.rw_p = h.rhogw[p + ijdim];
                     si.rg_m = wrhog[p - ijdim];
                     si.rg_c = wrhog[p];
                     si.rg_p = wrhog[p + ijdim];
-                    si.rq[0] = wrhogq[pq + QR * K2];
-                    si.rq[1] = wrhogq[pq + QS * K2];
-                    si.rq[2] = wrhogq[pq + QG * K2];
+                    for (int t = 0; t < NT; t++) si.rq[t] = wrhogq[pq + sed_nq<NT>(t) * K2];
                     si.re0 = wrhoge[p];
                     si.rvx = h.rhogvx[p];
                     si.rvy = h.rhogvy[p];
@@ -291,7 +298,7 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                     si.gsh_c = h.gsgam2h[p];
                     si.gsh_p = h.gsgam2h[p + ijdim];
                     si.z = h.z[p];
-                    SedOut<S> so;
+                    SedOutN<S, NT> so;
                     const Unit u = {h.ix[pc], h.iy[pc], h.iz[pc], h.jx[pc], h.jy[pc], h.jz[pc]};
                     sediment(g, k, u, si, so, P.kin2kin != 0.0);
                     o_rvx[p] = so.rvx;
@@ -300,24 +307,20 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                     o_rw[p] = so.rw;
                     x.rhog = so.rg;
                     x.rhoge = so.e;
-                    x.rhogq[QR] = so.rq[0];
-                    x.rhogq[QS] = so.rq[1];
-                    x.rhogq[QG] = so.rq[2];
+                    for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = so.rq[t];
                 } else {
                     x.rhog = wrhog[p] + 0.0;
                     x.rhoge = wrhoge[p] + 0.0;
-                    x.rhogq[QR] = wrhogq[pq + QR * K2] + 0.0;
-                    x.rhogq[QS] = wrhogq[pq + QS * K2] + 0.0;
-                    x.rhogq[QG] = wrhogq[pq + QG * K2] + 0.0;
+                    for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = wrhogq[pq + sed_nq<NT>(t) * K2] + 0.0;
                     o_rvx[p] = h.rhogvx[p] + 0.0 * h.ix[pc] + 0.0 * h.jx[pc];
                     o_rvy[p] = h.rhogvy[p] + 0.0 * h.iy[pc] + 0.0 * h.jy[pc];
                     o_rvz[p] = h.rhogvz[p] + 0.0 * h.iz[pc] + 0.0 * h.jz[pc];
                     o_rw[p] = h.rhogw[p] + 0.0;
                     if (P.kin2kin != 0.0 && k == KALL - 1) {
                         // KIN2KIN at wall kmax+1: rhogkin_v = 0, the lower cell after sedimentation
-                        const S *fx = &flux[(int64_t)l * 10 * K2 + ij];
+                        const S *fx = &flux[(int64_t)l * NS * K2 + ij];
                         S rg = wrhog[p - ijdim];
-                        for (int t = 0; t < 3; t++)
+                        for (int t = 0; t < NT; t++)
                             rg = rg + -(fx[t * K2 + (int64_t)(KALL - 1) * ijdim] - fx[t * K2 + (int64_t)KMAX * ijdim]) *
                                           g.rdgz[KMAX];
                         const double c2w0 = g.afact[KALL - 1] / h.gsgam2[p] * h.gsgam2h[p];
@@ -328,7 +331,7 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
                 }
                 x.rhogq[QV] = wrhogq[pq + QV * K2] + 0.0;
                 x.rhogq[QC] = wrhogq[pq + QC * K2] + 0.0;
-                x.rhogq[QI] = wrhogq[pq + QI * K2] + 0.0;
+                if (NT == 3) x.rhogq[QI] = wrhogq[pq + QI * K2] + 0.0;
                 x.gsgam2 = h.gsgam2[p];
                 TailOut<S> y;
                 const TailStatus st = tail(P, interior, x, y);
@@ -352,7 +355,14 @@ int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF
     return 0;
 }
 
-template <typename S> void alloc_outputs(HostState<S> &h)
+template <typename S>
+int host_step(const DevParams &P, const GridDev &g, double dt, const S tun[NCOEF], HostState<S> &h, int *n_newton_fail)
+{
+    return P.four_water != 0.0 ? host_step_n<4>(P, g, dt, tun, h, n_newton_fail)
+                               : host_step_n<3>(P, g, dt, tun, h, n_newton_fail);
+}
+
+template <typename S> void alloc_outputs(HostState<S> &h, int nt = 3)
 {
     const int64_t N2 = (int64_t)h.lall * KALL * h.ijdim, NC = (int64_t)h.lall * h.ijdim;
     h.rho.assign(N2, S(0.0));
@@ -364,7 +374,7 @@ template <typename S> void alloc_outputs(HostState<S> &h)
     h.lh.assign(NC, S(0.0));
     h.prhophi.assign(NC, S(0.0));
     h.prhokin.assign(NC, S(0.0));
-    h.sflux.assign(3 * NC, S(0.0));
+    h.sflux.assign((size_t)nt * NC, S(0.0));
 }
 
 template <typename S> void bind_metrics(HostState<S> &h, const nsw6_fields *f)
@@ -414,7 +424,7 @@ int grad_step_fp64(const double *gz, const double *gzh, const nsw6_params *param
     cp(h.tem, f->tem, N2);
     cp(h.q, f->q, N2 * NQ);
     bind_metrics(h, f);
-    alloc_outputs(h);
+    alloc_outputs(h, P.four_water != 0.0 ? 4 : 3);
     double tun[NCOEF];
     tunables(P, tun);
     int fails = 0;
@@ -443,8 +453,9 @@ int grad_step_fp64(const double *gz, const double *gzh, const nsw6_params *param
     return fails ? 1 : 0;
 }
 
-// Surface mass fluxes sflux_t = -f_t(kmin)/dt of QR, QS, QG ([l][3][ij]) of one FP64
-// host step from the state `f` (not modified).
+// Surface mass fluxes sflux_t = -f_t(kmin)/dt of the falling tracers ([l][NT][ij]: QR,
+// QS, QG, or QR, QI, QS, QG under four_water) of one FP64 host step from the state `f`
+// (not modified).
 int grad_sflux_fp64(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
                     const nsw6_fields *f, double dt, double *sflux)
 {
@@ -470,7 +481,7 @@ int grad_sflux_fp64(const double *gz, const double *gzh, const nsw6_params *para
     cp(h.tem, f->tem, N2);
     cp(h.q, f->q, N2 * NQ);
     bind_metrics(h, f);
-    alloc_outputs(h);
+    alloc_outputs(h, P.four_water != 0.0 ? 4 : 3);
     double tun[NCOEF];
     tunables(P, tun);
     int fails = 0;
@@ -507,7 +518,7 @@ int grad_loss_fp64(const double *gz, const double *gzh, const nsw6_params *param
     cp(h.q, f->q, N2 * NQ);
     const std::vector<double> rhoge_in(h.rhoge);
     bind_metrics(h, f);
-    alloc_outputs(h);
+    alloc_outputs(h, P.four_water != 0.0 ? 4 : 3);
     double tun[NCOEF];
     tunables(P, tun);
     if (coef)
@@ -569,7 +580,7 @@ int grad_dual_loss(const double *gz, const double *gzh, const nsw6_params *param
 #undef SEED
     const std::vector<D> rhoge_in(h.rhoge);
     bind_metrics(h, f);
-    alloc_outputs(h);
+    alloc_outputs(h, P.four_water != 0.0 ? 4 : 3);
     double t0[NCOEF];
     tunables(P, t0);
     D tun[NCOEF];
@@ -1073,25 +1084,20 @@ void seed_state(HostState<Dual<N>> &h, const nsw6_fields *f, int ndir, const nsw
 
 extern "C" {
 
-// Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
-// NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but 3WATER (KIN2EIN or
-// KIN2KIN) with the Roh & Satoh or the exponential snow moments, KK2000 or Berry's autoconversion, with or
-// without explicit ice generation, no OPT_INDIR (the device reverse mode also covers OPT_INDIR).
-int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
-                      const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
-                      const nsw6_fields *grad, double *grad_coef, double dt)
+}  // extern "C"
+
+namespace {
+// The host run of the reverse passes for NT sedimenting tracers (grad_adjoint_host).
+template <int NT>
+int adjoint_host_n(const DevParams &P, const GridDev &g, int lall, int ijdim, const nsw6_fields *state,
+                   const nsw6_fields *cot, const double *cot_sflux, const nsw6_fields *grad, double *grad_coef,
+                   double dt)
 {
-    DevParams P;
-    to_dev_params(*params, P);
-    if (!((P.sw_roh2014 == 1.0 || P.sw_roh2014 == 0.0) && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) &&
-          (P.sw_expice == 1.0 || P.sw_expice == 0.0) && P.opt_indir == 0.0 && P.four_water == 0.0))
-        return NSW6_UNSUPPORTED;
-    GridDev g;
-    make_grid_dev(gz, gzh, g);
     const int64_t K2 = (int64_t)KALL * ijdim, N2 = (int64_t)lall * K2, C = (int64_t)lall * ijdim;
-    std::vector<double> w_rhog(N2), w_rhoge(N2), w_rhogq(N2 * NQ), w_vt(N2 * 3), w_flux(N2 * FLUX_SLOTS, 0.0);
-    std::vector<int> w_bounds((size_t)lall * BOUNDS_PER_REGION, 0), w_flag(lall, 0);
-    std::vector<double> diag(3 * N2 + 13 * C, 0.0), rec((size_t)ADJ_RECORD_SLOTS * N2, 0.0);
+    std::vector<double> w_rhog(N2), w_rhoge(N2), w_rhogq(N2 * NQ), w_vt(N2 * NT),
+        w_flux(N2 * FxSlots<NT>::N, 0.0);
+    std::vector<int> w_bounds((size_t)lall * ext_per_region<NT>(), 0), w_flag(lall, 0);
+    std::vector<double> diag(3 * N2 + 13 * C, 0.0), rec((size_t)adj_record_slots<NT>() * N2, 0.0);
 
     AdjArgs A;
     memset(&A, 0, sizeof A);
@@ -1127,14 +1133,15 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
     a.dt3 = pow(dt, 3);
     a.dt3_6 = a.dt3 / 6.0;
     a.kin2kin = (P.kin2kin != 0.0);
+    a.four_water = (NT == 4);
     A.a1_tab = A1_TAB;
     A.a2_tab = A2_TAB;
     A.cot = host_fields(*cot);
     A.cot_sflux = cot_sflux;
     A.grad = host_fields(*grad);
     A.cr = rec.data();
-    A.wr = A.cr + (size_t)NCR * N2;
-    A.dws = A.wr + (size_t)NWR * N2;
+    A.wr = A.cr + (size_t)CellRec<NT>::N * N2;
+    A.dws = A.wr + (size_t)WallRec<NT>::N * N2;
     A.din = A.dws + (size_t)NDW * N2;
 
     double tun[NCOEF];
@@ -1147,12 +1154,12 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
             for (int ij = 0; ij < ijdim; ij++) {
                 const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
                 const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
-                const int64_t pv = (int64_t)l * 3 * K2 + (int64_t)k * ijdim + ij;
+                const int64_t pv = (int64_t)l * NT * K2 + (int64_t)k * ijdim + ij;
                 if (k == 0 || k == KALL - 1) {
                     w_rhog[p] = f.rhog[p];
                     w_rhoge[p] = f.rhoge[p];
                     for (int n = 0; n < NQ; n++) w_rhogq[pq + n * K2] = f.rhogq[pq + n * K2];
-                    for (int t = 0; t < 3; t++) w_vt[pv + t * K2] = 0.0;
+                    for (int t = 0; t < NT; t++) w_vt[pv + t * K2] = 0.0;
                     continue;
                 }
                 PointIn<double> x;
@@ -1166,25 +1173,25 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
                 w_rhog[p] = y.rhog;
                 w_rhoge[p] = y.rhoge;
                 for (int n = 0; n < NQ; n++) w_rhogq[pq + n * K2] = y.rhogq[n];
-                for (int t = 0; t < 3; t++) w_vt[pv + t * K2] = y.vt[t];
+                for (int t = 0; t < NT; t++) w_vt[pv + t * K2] = fall_speed<NT>(y, t);
             }
     // ---- kcell bounds in the device layout (kcell_max is what the fluxes read)
     Bounds b;
-    b.kcmin.assign((size_t)lall * 3 * KALL, 0);
-    b.kcmax.assign((size_t)lall * 3 * KALL, 0);
-    b.khmin.assign((size_t)lall * 3 * KALL, 0);
-    b.khmax.assign((size_t)lall * 3 * KALL, 0);
+    b.kcmin.assign((size_t)lall * NT * KALL, 0);
+    b.kcmax.assign((size_t)lall * NT * KALL, 0);
+    b.khmin.assign((size_t)lall * NT * KALL, 0);
+    b.khmax.assign((size_t)lall * NT * KALL, 0);
     b.general.assign(lall, 0);
-    for (int l = 0; l < lall; l++) region_bounds(g, dt, a.dt2, a.dt3_6, l, ijdim, w_vt, b);
+    for (int l = 0; l < lall; l++) region_bounds<NT>(g, dt, a.dt2, a.dt3_6, l, ijdim, w_vt, b);
     for (int l = 0; l < lall; l++) {
         w_flag[l] = b.general[l];
-        int *bl = &w_bounds[(size_t)l * BOUNDS_PER_REGION];
-        for (int t = 0; t < 3; t++)
+        int *bl = &w_bounds[(size_t)l * ext_per_region<NT>()];
+        for (int t = 0; t < NT; t++)
             for (int k = 0; k < KALL; k++) {
-                bl[bounds_index(t, 0, k, 0)] = b.kcmin[((size_t)l * 3 + t) * KALL + k];
-                bl[bounds_index(t, 1, k, 0)] = b.khmin[((size_t)l * 3 + t) * KALL + k];
-                bl[bounds_index(t, 0, k, 1)] = b.kcmax[((size_t)l * 3 + t) * KALL + k];
-                bl[bounds_index(t, 1, k, 1)] = b.khmax[((size_t)l * 3 + t) * KALL + k];
+                bl[bounds_index(t, 0, k, 0)] = b.kcmin[((size_t)l * NT + t) * KALL + k];
+                bl[bounds_index(t, 1, k, 0)] = b.khmin[((size_t)l * NT + t) * KALL + k];
+                bl[bounds_index(t, 0, k, 1)] = b.kcmax[((size_t)l * NT + t) * KALL + k];
+                bl[bounds_index(t, 1, k, 1)] = b.khmax[((size_t)l * NT + t) * KALL + k];
             }
     }
     // ---- wall fluxes
@@ -1193,28 +1200,28 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
         for (int w = 0; w < KALL; w++)
             for (int ij = 0; ij < ijdim; ij++) {
                 if (w_flag[l])
-                    flux_wall_general(a, l, w, ij);
+                    flux_wall_general<NT>(a, l, w, ij);
                 else
-                    flux_wall(a, l, w, ij);
+                    flux_wall<NT>(a, l, w, ij);
             }
     // ---- reverse passes
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
         for (int k = 0; k < KALL; k++)
-            for (int ij = 0; ij < ijdim; ij++) adj_cell(A, l, k, ij);
+            for (int ij = 0; ij < ijdim; ij++) adj_cell<NT>(A, l, k, ij);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
         for (int w = 0; w < KALL; w++)
-            for (int ij = 0; ij < ijdim; ij++) adj_wall(A, l, w, ij);
+            for (int ij = 0; ij < ijdim; ij++) adj_wall<NT>(A, l, w, ij);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
-        for (int ij = 0; ij < ijdim; ij++) adj_walls_general(A, l, ij);
+        for (int ij = 0; ij < ijdim; ij++) adj_walls_general<NT>(A, l, ij);
     std::vector<double> tb((size_t)lall * KALL * NCOEF, 0.0);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 0; l < lall; l++)
         for (int k = 0; k < KALL; k++) {
             double *t = &tb[((size_t)l * KALL + k) * NCOEF];
-            for (int ij = 0; ij < ijdim; ij++) adj_point(A, l, k, ij, t);
+            for (int ij = 0; ij < ijdim; ij++) adj_point<NT>(A, l, k, ij, t);
         }
     if (grad_coef)
         for (int c = 0; c < NCOEF; c++) {
@@ -1223,6 +1230,30 @@ int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *pa
             grad_coef[c] = s;
         }
     return NSW6_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// Host reverse mode with the arguments of nsw6_mp_driver_adj (host arrays).  Returns
+// NSW6_OK, or NSW6_UNSUPPORTED for a scheme these passes do not cover: anything but
+// the Roh & Satoh or the exponential snow moments, KK2000 or Berry's autoconversion,
+// with or without explicit ice generation, KIN2EIN or KIN2KIN, 3WATER or 4WATER, no
+// OPT_INDIR (the device reverse mode also covers OPT_INDIR).
+int grad_adjoint_host(const double *gz, const double *gzh, const nsw6_params *params, int lall, int ijdim,
+                      const nsw6_fields *state, const nsw6_fields *cot, const double *cot_sflux,
+                      const nsw6_fields *grad, double *grad_coef, double dt)
+{
+    DevParams P;
+    to_dev_params(*params, P);
+    if (!((P.sw_roh2014 == 1.0 || P.sw_roh2014 == 0.0) && (P.sw_kk2000 == 1.0 || P.sw_kk2000 == 0.0) &&
+          (P.sw_expice == 1.0 || P.sw_expice == 0.0) && P.opt_indir == 0.0))
+        return NSW6_UNSUPPORTED;
+    GridDev g;
+    make_grid_dev(gz, gzh, g);
+    return P.four_water != 0.0
+               ? adjoint_host_n<4>(P, g, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt)
+               : adjoint_host_n<3>(P, g, lall, ijdim, state, cot, cot_sflux, grad, grad_coef, dt);
 }
 
 // J = sum <cot, y(x)> + sum cot_sflux sflux over the step's outputs (the output list
@@ -1245,7 +1276,7 @@ int grad_dual_vjp(const double *gz, const double *gzh, const nsw6_params *params
     h.ijdim = ijdim;
     seed_state(h, f, ndir, dirs);
     bind_metrics(h, f);
-    alloc_outputs(h);
+    alloc_outputs(h, P.four_water != 0.0 ? 4 : 3);
     double t0[NCOEF];
     tunables(P, t0);
     D tun[NCOEF];
