@@ -468,7 +468,11 @@ def main():
     wname = args.workload or ("nsw6_169000x96_mixed" if args.precision == "mixed" else "nsw6_169000x96_fp64")
     w = F.WORKLOADS[wname]
     ncol = w.lall * w.ijdim
-    gz, gzh, st = F.workload_state(w)
+    if wname == "nsw6_169000x96_refgrid_dt30":   # its state is built on the reference vgrid
+        gz, gzh = reference_vgrid()
+        gz, gzh, st = F.workload_state(w, gz=gz, gzh=gzh)
+    else:
+        gz, gzh, st = F.workload_state(w)
     dev = F.to_device(st)
     del st
     prec = N.NSW6_MIXED if args.precision == "mixed" else N.NSW6_FP64

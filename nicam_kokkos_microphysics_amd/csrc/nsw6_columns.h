@@ -219,15 +219,19 @@ NSW6_HD void flux_wall_general_to(const StepArgs &a, int l, int w, int ij, Put &
     const Unit u = load_unit(f, (int64_t)l * ijdim + ij);
     GeneralGeomN<double, NT> G;
     general_geometry(g, w, a.dt, a.dt2, a.dt3_6, V, kcmin, kcmax, khmin, khmax, G);
-    auto cell_at = [&](int j, CellN<double, NT> &c) {
-        CellIn<double> r;
-        load_cell(a, p2, p3, K2, ijdim, j, r);
-        const BelowCellN<NT> b = load_below<NT>(a, p2, p3, K2, ijdim, j);
-        derive_cell(g, j, r, b.rhog, b.q, b.gs, u, c);
+    auto raw_at = [&](int j, CellIn<double> &r) { load_cell(a, p2, p3, K2, ijdim, j, r); };
+    auto below_at = [&](int j) {
+        const BelowCellN<NT> m = load_below<NT>(a, p2, p3, K2, ijdim, j);
+        BelowOf<double, NT> b;
+        b.rhog = m.rhog;
+#pragma unroll
+        for (int t = 0; t < NT; t++) b.q[t] = m.q[t];
+        b.gs = m.gs;
+        return b;
     };
     WallFluxN<double, NT> F;
     double ft[NT][4];
-    wall_flux_general(g, w, G, cell_at, F, ft);
+    wall_flux_general(g, w, G, u, raw_at, below_at, F, ft);
 
 #pragma unroll
     for (int t = 0; t < NT; t++) put(X::Q + t, F.fq[t]);

@@ -224,9 +224,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 // (level, 256-column block) tiles, which keeps the launch cheap when no region needs
 // it (a tenth of the blocks of k_flux's grid).  The blocks that one XCD receives
 // (every NUM_XCD-th) take consecutive tiles of one chunk, i.e. neighbouring levels of
-// the same columns, whose cells they share through that XCD's L2.  Two waves per SIMD:
-// unforced the kernel takes 276 registers and runs at one (2.82 ms against 2.35 ms
-// for every region general at 169,000 x 96, with 20 registers spilled at two).
+// the same columns, whose cells they share through that XCD's L2.  Two waves per SIMD
+// (223 registers, no spills); forced to three it spills 56 registers and takes 1.8x
+// as long.
 constexpr int GENERAL_TILES = 4;
 static inline int general_blocks(int ijdim)
 {

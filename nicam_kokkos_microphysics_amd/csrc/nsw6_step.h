@@ -504,31 +504,112 @@ NSW6_HD void general_sums(const GridDev &g, int w, const GeneralGeomN<S, NT> &G,
     }
 }
 
-template <typename S, int NT, typename CellAt>
-NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, CellAt &&cell_at,
-                               WallFluxN<S, NT> &F, S (*ft_surf)[4])
+// Cell j-1 as derive_cell takes it for cell j's lower wall.
+template <typename S, int NT> struct BelowOf {
+    S rhog, q[NT];
+    double gs;
+};
+
+// The forward form.  zdis after the k2 loop is known before it, so the departure
+// cell's term rhof(kcell)·zdis joins the sum when the sweep reaches kcell instead of
+// being kept aside (general_sums keeps it aside for the reverse mode): with NT tracers
+// that frees 8·NT live values across the sweep, and the sum's order differs from the
+// reference's by the position of that one term (round-off only).  The sweep derives
+// each cell from its raw data (raw_at(j, r)) and from the cell below it, which is the
+// cell derived just before (below_at(j) only for the first and for cells outside the
+// ranges), and requests cell j+1's raw data before deriving cell j.
+template <typename S, int NT, typename RawAt, typename BelowAt>
+NSW6_HD void wall_flux_general(const GridDev &g, int w, const GeneralGeomN<S, NT> &G, const Unit &u,
+                               RawAt &&raw_at, BelowAt &&below_at, WallFluxN<S, NT> &F, S (*ft_surf)[4])
 {
-    S acc[NT][6], rfc[NT][6], acch[NT][2], hvc[NT][2];
-    general_sums(g, w, G, cell_at, acc, rfc, acch, hvc);
     const bool half = (w >= KMIN + 1);
+    S acc[NT][6], acch[NT][2], zd[NT], zh[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        zd[t] = G.z0[t] - general_zd_shift(g.dgz, w, G.kcmin[t], G.kcmax[t], G.kc[t]);
+        zh[t] = half ? S(G.zh0[t] - general_zd_shift(g.dgzh, w, G.khmin[t], G.kcmax[t], G.kch[t])) : S(0.0);
+#pragma unroll
+        for (int i = 0; i < 6; i++) acc[t][i] = S(0.0);
+        acch[t][0] = acch[t][1] = S(0.0);
+    }
+    auto take = [&](int j, const CellN<S, NT> &c) {
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const bool one = (G.kcmin[t] == w && G.kcmax[t] == w);
+            const bool in = !one && j >= G.kcmin[t] && j <= G.kcmax[t];
+            const double fa = fact_of(j, G.kc[t], w, g.dgz[j]);
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const S rf = rhof_cell(c, t, i);
+                if (in) acc[t][i] = acc[t][i] + rf * fa;
+                if (j == G.kc[t]) acc[t][i] = acc[t][i] + rf * zd[t];
+            }
+            if (half) {
+                const bool hone = (G.khmin[t] == w && G.kcmax[t] == w);
+                const bool hin = !hone && j >= G.khmin[t] && j <= G.kcmax[t];
+                const double fh = fact_of(j, G.kch[t], w, g.dgzh[j]);
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    if (hin) acch[t][i] = acch[t][i] + c.hv[t][i] * fh;
+                    if (j == G.kch[t]) acch[t][i] = acch[t][i] + c.hv[t][i] * zh[t];
+                }
+            }
+        }
+    };
+    auto cell_at = [&](int j, CellN<S, NT> &c) {
+        CellIn<S> r;
+        raw_at(j, r);
+        const BelowOf<S, NT> b = below_at(j);
+        derive_cell(g, j, r, b.rhog, b.q, b.gs, u, c);
+    };
+    {
+        BelowOf<S, NT> b = below_at(G.lo);
+        CellIn<S> rn;
+        raw_at(G.lo, rn);
+        for (int j = G.lo; j <= G.hi; j++) {
+            const CellIn<S> r = rn;
+            if (j < G.hi) raw_at(j + 1, rn);
+            CellN<S, NT> c;
+            derive_cell(g, j, r, b.rhog, b.q, b.gs, u, c);
+            take(j, c);
+            b.rhog = c.rhog;
+#pragma unroll
+            for (int t = 0; t < NT; t++) b.q[t] = c.q[t];
+            b.gs = c.gs;
+        }
+    }
+    // a departure cell the search left at the domain's end outside the ranges
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        if (G.kc[t] < G.lo || G.kc[t] > G.hi) {
+            CellN<S, NT> c;
+            cell_at(G.kc[t], c);
+#pragma unroll
+            for (int i = 0; i < 6; i++) acc[t][i] = acc[t][i] + rhof_cell(c, t, i) * zd[t];
+        }
+        if (half && (G.kch[t] < G.lo || G.kch[t] > G.hi)) {
+            CellN<S, NT> c;
+            cell_at(G.kch[t], c);
+            acch[t][0] = acch[t][0] + c.hv[t][0] * zh[t];
+            acch[t][1] = acch[t][1] + c.hv[t][1] * zh[t];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 5; i++) F.fs[i] = S(0.0);
     F.fh[0] = F.fh[1] = S(0.0);
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        const S zd = G.z0[t] - general_zd_shift(g.dgz, w, G.kcmin[t], G.kcmax[t], G.kc[t]);
         S ft[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ft[i] = eps_filter(S(acc[t][i] + rfc[t][i] * zd));
+        for (int i = 0; i < 6; i++) ft[i] = eps_filter(acc[t][i]);
         F.fq[t] = ft[0];
 #pragma unroll
         for (int i = 1; i < 6; i++) F.fs[i - 1] += ft[i];
 #pragma unroll
         for (int i = 0; i < 4; i++) ft_surf[t][i] = ft[i];
         if (half) {
-            const S zh = G.zh0[t] - general_zd_shift(g.dgzh, w, G.khmin[t], G.kcmax[t], G.kch[t]);
 #pragma unroll
-            for (int i = 0; i < 2; i++) F.fh[i] += eps_filter(S(acch[t][i] + hvc[t][i] * zh));
+            for (int i = 0; i < 2; i++) F.fh[i] += eps_filter(acch[t][i]);
         }
     }
 }

@@ -216,17 +216,16 @@ int host_step_n(const DevParams &P, const GridDev &g, double dt, const S tun[NCO
                 if (b.general[l]) {
                     GeneralGeomN<S, NT> G;
                     general_geometry(g, w, dt, dt2, dt3_6, V, kcmin, kcmax, khmin, khmax, G);
-                    auto cell_at = [&](int j, CellN<S, NT> &c) {
-                        CellIn<S> rj;
-                        raw(j, rj);
+                    auto below_at = [&](int j) {
                         const int64_t ob = (int64_t)l * K2 + (int64_t)(j - 1) * ijdim + ij;
                         const int64_t oqb = (int64_t)l * NQ * K2 + (int64_t)(j - 1) * ijdim + ij;
-                        const S rg_b = wrhog[ob];
-                        S q_b[NT];
-                        for (int t = 0; t < NT; t++) q_b[t] = wrhogq[oqb + sed_nq<NT>(t) * K2] / rg_b;
-                        derive_cell(g, j, rj, rg_b, q_b, h.gsgam2[ob], u, c);
+                        BelowOf<S, NT> bl;
+                        bl.rhog = wrhog[ob];
+                        for (int t = 0; t < NT; t++) bl.q[t] = wrhogq[oqb + sed_nq<NT>(t) * K2] / bl.rhog;
+                        bl.gs = h.gsgam2[ob];
+                        return bl;
                     };
-                    wall_flux_general(g, w, G, cell_at, F, ft);
+                    wall_flux_general(g, w, G, u, raw, below_at, F, ft);
                 } else {
                     S z0[NT], zh0[NT];
                     WallGeomN<NT> geo;
