@@ -209,7 +209,7 @@ template <int NT, typename WS = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_flux(StepArgs a)
 {
     const int ijdim = a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
+    const LevelBlock lb = level_block<FLUX_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
     if (!lb.valid) return;
     if (a.ws.region_flag[lb.l] != 0) return;
     const int ij = lb.bx * 256 + threadIdx.x;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 {
     load_math_tables();   // the saturation adjustment's exp
     const int ijdim = a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
+    const LevelBlock lb = level_block<FINISH_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
     if (!lb.valid) return;
     const int ij = lb.bx * 256 + threadIdx.x;
     if (ij >= ijdim) return;
