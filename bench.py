@@ -67,8 +67,8 @@ FP64_PEAK_TFLOPS = 78.6
 # compile flags and the synthetic-input generator.  A counters file records their
 # digest; after any change to them the stored counters no longer describe the build.
 COUNTED_SOURCES = [os.path.join("nicam_kokkos_microphysics_amd", "csrc", n) for n in (
-    "nsw6_kernels.hip", "nsw6_rates_kernel.h", "nsw6_rates_mixed.hip", "nsw6_device.h", "nsw6_step.h",
-    "nsw6_columns.h", "nsw6_internal.h", "nsw6_setup.h", "nsw6_tables.h", "nsw6_math.h", "nsw6_synth.c")] + \
+    "nsw6_kernels.hip", "nsw6_rates_kernel.h", "nsw6_rates_mixed.hip", "nsw6_finish.hip", "nsw6_device.h",
+    "nsw6_step.h", "nsw6_columns.h", "nsw6_internal.h", "nsw6_setup.h", "nsw6_tables.h", "nsw6_math.h", "nsw6_synth.c")] + \
     [os.path.join("nicam_kokkos_microphysics_amd", "build_lib.py")]
 
 

@@ -79,10 +79,12 @@ def build(verbose: bool = True) -> None:
     obj_h = os.path.join(BUILD, "nsw6_host.o")
     obj_s = os.path.join(BUILD, "nsw6_snapshot.o")
     obj_m = os.path.join(BUILD, "nsw6_rates_mixed.o")
+    obj_f = os.path.join(BUILD, "nsw6_finish.o")
     steps = [
         [HIPCC, *HIP_FLAGS, *KERNEL_FLAGS, "-c", os.path.join(CSRC, "nsw6_kernels.hip"), "-o", obj_k],
         [HIPCC, *HIP_FLAGS, *ADJ_FLAGS, "-c", os.path.join(CSRC, "nsw6_adjoint.hip"), "-o", obj_r],
         [HIPCC, *HIP_FLAGS, *MIXED_FLAGS, "-c", os.path.join(CSRC, "nsw6_rates_mixed.hip"), "-o", obj_m],
+        [HIPCC, *HIP_FLAGS, *KERNEL_FLAGS, "-c", os.path.join(CSRC, "nsw6_finish.hip"), "-o", obj_f],
         [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-x", "hip", "-c", os.path.join(CSRC, "nsw6_api.cpp"), "-o", obj_a],
         ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_host.c"), "-o", obj_h],
         ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "nsw6_snapshot.c"), "-o", obj_s],
@@ -91,7 +93,7 @@ def build(verbose: bool = True) -> None:
         for fut in [pool.submit(_run, cmd, verbose) for cmd in steps]:
             fut.result()
     tmp = LIB_NSW6 + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_m, obj_r, obj_a, obj_h, obj_s],
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj_k, obj_m, obj_f, obj_r, obj_a, obj_h, obj_s],
          verbose)
     os.replace(tmp, LIB_NSW6)
     tmp = LIB_SYNTH + ".tmp"

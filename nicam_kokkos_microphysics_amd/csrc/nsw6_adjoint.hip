@@ -31,7 +31,7 @@ template <int NT>
 __global__ void __launch_bounds__(256) k_adj_cell(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    const LevelBlock lb = level_block<ADJ_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     if (!lb.valid || ij >= ijdim) return;
     adj_cell<NT>(L.A, lb.l, lb.k, ij);
@@ -44,7 +44,7 @@ template <int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_wall(AdjLaunch L)
 {
     const int ijdim = L.A.a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
+    const LevelBlock lb = level_block<ADJ_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, L.A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     if (!lb.valid || ij >= ijdim) return;
     adj_wall<NT>(L.A, lb.l, lb.k, ij);
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_adj_point(AdjLaunch L)
     A.a1_tab = c_a1_tab;
     A.a2_tab = c_a2_tab;
     const int ijdim = A.a.ijdim;
-    const LevelBlock lb = level_block(blockIdx.x, (ijdim + 255) / 256, KALL, 0, A.a.lall);
+    const LevelBlock lb = level_block<ADJ_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, A.a.lall);
     const int ij = lb.bx * 256 + threadIdx.x;
     double tun_b[NCOEF];
 #pragma unroll

@@ -16,14 +16,15 @@
 //              next cell, upward, or out of the domain) over the regions' kcell ranges.
 //   k_finish   point-wise: flux divergences and KIN2EIN (:960-1011), rediagnosis
 //              (:1074-1086), saturation adjustment
-//              (src/mod_satadjust.cpp:274-362,771-896) and the final negative filter.
+//              (src/mod_satadjust.cpp:274-362,771-896) and the final negative filter;
+//              in nsw6_finish.hip, a unit of its own.
 // Every access of a wavefront is one contiguous 512-byte segment of a level (ij
 // fastest).  No MFMA: nothing here is a dense contraction.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
 
-// double exp and log from the LDS tables of nsw6_math.h (k_rates, k_finish)
+// double exp and log from the LDS tables of nsw6_math.h (k_rates)
 #define NSW6_TABLE_MATH 1
 #include "nsw6_device.h"
 #include "nsw6_columns.h"
@@ -252,100 +253,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 
 // ---------------------------------------------------------------------------------
-// k_finish: one thread per point.  The sedimented state from sediment() on the wall
-// fluxes, then the point-wise tail (rediagnosis, saturation adjustment, final
-// negative filter).
-// ---------------------------------------------------------------------------------
-template <int NT, typename Fetch>
-__device__ __forceinline__ void finish_point(const StepArgs &a, int l, int k, int ij, const Fetch &FXV)
-{
-    const int ijdim = a.ijdim;
-    const Fields &f = a.f;
-    const int64_t K2 = (int64_t)KALL * ijdim;
-    const int64_t p = ((int64_t)l * KALL + k) * ijdim + ij;
-    const int64_t pq = (int64_t)l * NQ * K2 + (int64_t)k * ijdim + ij;
-    const bool interior = (k >= KMIN && k <= KMAX);
-
-    // State after sedimentation.  QV, QC, QI ("3WATER") and the halo levels only
-    // receive zero tendencies.  The momenta are stored with the other outputs at the
-    // end: every store comes after every load, so that the uniform reads of the grid
-    // and the parameters stay on the scalar cache.
-    TailIn<double> x;
-    // read by every path: loaded before the branches, with the path's own loads
-    x.rhogq[QV] = a.ws.rhogq[pq + QV * K2] + 0.0;
-    x.rhogq[QC] = a.ws.rhogq[pq + QC * K2] + 0.0;
-    if (NT == 3) x.rhogq[QI] = a.ws.rhogq[pq + QI * K2] + 0.0;
-    x.gsgam2 = f.gsgam2[p];
-    double mom[4];   // rhogvx, rhogvy, rhogvz, rhogw after sedimentation
-    if (interior) {
-        SedInN<double, NT> si;
-        load_sed_from(a, l, ij, k, si, FXV);
-        SedOutN<double, NT> so;
-        sediment(*a.grid, k, load_unit(f, (int64_t)l * ijdim + ij), si, so, a.kin2kin != 0);
-        mom[0] = so.rvx;
-        mom[1] = so.rvy;
-        mom[2] = so.rvz;
-        mom[3] = so.rw;
-        x.rhog = so.rg;
-        x.rhoge = so.e;
-#pragma unroll
-        for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = so.rq[t];
-    } else {
-        x.rhog = a.ws.rhog[p] + 0.0;
-        x.rhoge = a.ws.rhoge[p] + 0.0;
-#pragma unroll
-        for (int t = 0; t < NT; t++) x.rhogq[sed_nq<NT>(t)] = a.ws.rhogq[pq + sed_nq<NT>(t) * K2] + 0.0;
-        const int64_t pc = (int64_t)l * ijdim + ij;
-        mom[0] = f.rhogvx[p] + 0.0 * f.ix[pc] + 0.0 * f.jx[pc];
-        mom[1] = f.rhogvy[p] + 0.0 * f.iy[pc] + 0.0 * f.jy[pc];
-        mom[2] = f.rhogvz[p] + 0.0 * f.iz[pc] + 0.0 * f.jz[pc];
-        mom[3] = (a.kin2kin && k == KALL - 1) ? kin2kin_top_wall<NT>(a, l, ij, x.rhog, FXV) : f.rhogw[p] + 0.0;
-    }
-
-    TailOut<double> y;
-    const TailStatus st = tail(*a.P, interior, x, y);
-    if (st.newton < 0) {
-        const int code = (st.newton == -1) ? 2 : 1;
-        atomicAdd(&a.ws.report[code], 1);
-        const unsigned long long key = ((unsigned long long)l * KALL + (unsigned long long)k) * ijdim + ij;
-        atomicMin(reinterpret_cast<unsigned long long *>(a.ws.report + REPORT_KEY), key);
-        atomicOr(&a.ws.report[0], 1);
-    }
-    f.rhog[p] = y.rhog;
-    f.rhoge[p] = y.rhoge;
-    f.rho[p] = y.rho;
-    f.tem[p] = y.tem;
-    f.pre[p] = y.pre;
-    f.qd[p] = y.qd;
-#pragma unroll
-    for (int n = 0; n < 6; n++) {
-        f.rhogq[pq + n * K2] = y.rhogq[n];
-        f.q[pq + n * K2] = y.q[n];
-    }
-    f.re_solid[p] = 20.0E-6;
-    f.GDCLW[p] = 0.0 * 1.0;
-    f.GDCFRC[p] = 0.0 * 1.0;
-    f.rhogvx[p] = mom[0];
-    f.rhogvy[p] = mom[1];
-    f.rhogvz[p] = mom[2];
-    f.rhogw[p] = mom[3];
-}
-
-// Three waves per SIMD: with the registers four waves leave (126 of 128 used) the
-// kernel is about 1 % slower.
-template <int NT, typename WS = double>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_finish(StepArgs a)
-{
-    load_math_tables();   // the saturation adjustment's exp
-    const int ijdim = a.ijdim;
-    const LevelBlock lb = level_block<FINISH_BAND>(blockIdx.x, (ijdim + 255) / 256, KALL, 0, a.lall);
-    if (!lb.valid) return;
-    const int ij = lb.bx * 256 + threadIdx.x;
-    if (ij >= ijdim) return;
-    finish_point<NT>(a, lb.l, lb.k, ij, FluxFromMemory<NT, WS>(a, lb.l, ij));
-}
-
-// ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
 // The transport of NT sedimenting tracers: extremes and kcell bounds, the wall fluxes
@@ -362,7 +269,7 @@ template <int NT, typename WS> static void launch_transport(const StepArgs &a, h
     if (ev) (void)hipEventRecord(ev[3], stream);
     hipLaunchKernelGGL((k_flux_general<NT, WS>), dim3(general_blocks(a.ijdim), a.lall), dim3(256), 0, stream, a);
     if (ev) (void)hipEventRecord(ev[4], stream);
-    hipLaunchKernelGGL((k_finish<NT, WS>), grid_l, dim3(256), 0, stream, a);
+    (void)nsw6_launch_finish(&a, NT, sizeof(WS) == sizeof(float), stream);   // errors surface at hipGetLastError
     if (ev) (void)hipEventRecord(ev[5], stream);
 }
 
