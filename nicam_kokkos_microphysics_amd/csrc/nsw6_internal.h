@@ -39,7 +39,7 @@ constexpr int EXT_PER_REGION_MAX = ext_per_region<4>();
 // one level; per-block partial extremes are stored as [l][k][4 NT][block], slot
 // t*4 + {0 full max, 1 full min, 2 half max, 3 half min}
 constexpr int EXT_THREADS = 256;
-constexpr int EXT_COLS_PER_THREAD = 16;
+constexpr int EXT_COLS_PER_THREAD = 8;   // 0.204 ms per step at 169,000 x 96; 16: 0.213, 4: 0.214, 32: 0.244
 constexpr int EXT_COLS_PER_BLOCK = EXT_THREADS * EXT_COLS_PER_THREAD;
 constexpr int BOUNDS_PER_REGION = EXT_PER_REGION;
 // Flux workspace [l][FLUX_SLOTS][k][ij].  For the single-cell regions, wall w holds:
